@@ -1,0 +1,125 @@
+"""ctypes binding of ``libfacet_hip.so`` (the C-ABI in ``include/facet_hip.h``).
+
+The library is built in-tree by ``python -m streamsim_amd.build``.  Loading
+fails loudly when it is missing: there is no CPU fallback for the engine.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfacet_hip.so"
+
+FH_OK = 0
+FH_INVALID_ARGUMENT = 1
+FH_LOGIC_ERROR = 2
+FH_RUNTIME_ERROR = 3
+FH_OVERFLOW_ERROR = 4
+
+FH_PRECISION_FP32 = 0
+FH_PRECISION_BF16_DC = 1
+
+
+class FhConfig(C.Structure):
+    _fields_ = [
+        ("raysPerSide", C.c_uint32),
+        ("sunRaysPerSide", C.c_uint32),
+        ("seed", C.c_uint32),
+        ("gpuIndex", C.c_uint32),
+        ("rayEpsilon", C.c_float),
+        ("reorder", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
+    ]
+
+
+class FhDirection(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class FhGraphDiagnostics(C.Structure):
+    _fields_ = [
+        ("facetCount", C.c_uint32),
+        ("surfaceCount", C.c_uint32),
+        ("directedEdgeCount", C.c_uint32),
+        ("fragmentOverflow", C.c_uint32),
+        ("hashOverflow", C.c_uint32),
+        ("maxClosureError", C.c_float),
+    ]
+
+
+class FhSolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_uint32),
+        ("maxDelta", C.c_float),
+        ("maxRelDelta", C.c_float),
+        ("refreshes", C.c_uint32),
+    ]
+
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_F = C.c_float
+_I = C.c_int
+
+# name -> (restype, argtypes).  Every symbol include/facet_hip.h declares.
+SIGNATURES = {
+    "fh_status_string": (C.c_char_p, [_I]),
+    "fh_last_error": (C.c_char_p, [_P]),
+    "fh_create": (_I, [C.POINTER(FhConfig), C.POINTER(_P)]),
+    "fh_destroy": (None, [_P]),
+    "fh_set_geometry": (_I, [_P, _P, _U32]),
+    "fh_build_form_factors": (_I, [_P, C.POINTER(FhGraphDiagnostics)]),
+    "fh_set_graph": (_I, [_P, _U32, _P, _P, _P, _P, _P, C.POINTER(FhGraphDiagnostics)]),
+    "fh_directed_edge_count": (_U32, [_P]),
+    "fh_get_graph": (_I, [_P, _P, _P, _P, _P, _P]),
+    "fh_solve": (_I, [_P, _P, _F, _U32, _F, _P, C.POINTER(FhSolveStats)]),
+    "fh_solve_accelerated": (_I, [_P, _P, _F, _P, _U32, _U32, _F, _F, _P, C.POINTER(FhSolveStats)]),
+    "fh_incident_irradiance": (_I, [_P, _P, _P, _F, _P]),
+    "fh_compute_sunlit_fraction": (_I, [_P, FhDirection, _P]),
+    "fh_facet_count": (_U32, [_P]),
+    "fh_surface_count": (_U32, [_P]),
+    "fh_solve_bands": (_I, [_P, _U32, _P, _P, _P, _P, _P, _P, _U32, _F, _F, _U32, _P, C.POINTER(FhSolveStats)]),
+    "fh_solve_bands_device": (_I, [_P, _U32, _P, _P, _P, _P, _P, _P, _U32, _F, _F, _U32, _P,
+                                   C.POINTER(FhSolveStats)]),
+    "fh_incident_bands": (_I, [_P, _U32, _P, _P, _P, _P]),
+    "fh_sweep_device": (_I, [_P, _U32, _P, _P, _P, _P, _P, _U32]),
+    "fh_band_stride": (_U32, [_U32]),
+}
+
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Load the engine library (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"HIP engine library missing: {LIB_PATH} (build it with `python -m streamsim_amd.build`)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class LogicError(RuntimeError):
+    """Call-order error (std::logic_error in the reference)."""
+
+
+def raise_for(status: int, ctx) -> None:
+    if status == FH_OK:
+        return
+    lib = load()
+    msg = lib.fh_last_error(ctx)
+    text = msg.decode() if msg else lib.fh_status_string(status).decode()
+    if status == FH_INVALID_ARGUMENT:
+        raise ValueError(text)
+    if status == FH_LOGIC_ERROR:
+        raise LogicError(text)
+    if status == FH_OVERFLOW_ERROR:
+        raise OverflowError(text)
+    raise RuntimeError(text)

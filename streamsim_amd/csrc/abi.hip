@@ -1,0 +1,537 @@
+// abi.hip -- extern "C" entry points of include/facet_hip.h.
+//
+// Each entry point mirrors one facetvk::FacetrtVulkan operation
+// (models/histream/src/facetrt/facetrt_vulkan.{h,cpp}): same argument
+// meaning, same validation and the same error classes, returned as
+// fh_status codes instead of exceptions.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <memory>
+
+#include "solver.hpp"
+
+using fh::fail;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <class F>
+int guarded(fh_context* c, F&& body) {
+    try {
+        if (c) FH_HIP(hipSetDevice(c->device));
+        body();
+        if (c) c->err.clear();
+        return FH_OK;
+    } catch (const fh::Error& e) {
+        if (c) c->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (c) c->err = "host allocation failed";
+        return FH_RUNTIME_ERROR;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        return FH_RUNTIME_ERROR;
+    }
+}
+
+void require_geometry(const fh_context* c, const char* op) {
+    if (!c->hasGeometry) fail(FH_LOGIC_ERROR, std::string("setGeometry() must be called before ") + op);
+}
+
+void require_graph(const fh_context* c, const char* op) {
+    if (!c->hasGraph) fail(FH_LOGIC_ERROR, std::string("buildVisibilityGraph() must be called before ") + op);
+}
+
+// facetrt_vulkan.cpp:1160-1175
+void validate_optics(const fh_context* c, const fh_surface_optics* o) {
+    if (!o) fail(FH_INVALID_ARGUMENT, "Optics array must contain exactly two sides per facet");
+    for (uint32_t s = 0; s < c->S; ++s) {
+        const fh_surface_optics& v = o[s];
+        if (!std::isfinite(v.reflectance) || !std::isfinite(v.transmittance) || !std::isfinite(v.emission) ||
+            !std::isfinite(v.directIrradiance) || v.reflectance < 0.0f || v.transmittance < 0.0f ||
+            v.emission < 0.0f || v.directIrradiance < 0.0f || v.reflectance + v.transmittance > 1.0001f)
+            fail(FH_INVALID_ARGUMENT,
+                 "Optical values must be finite, non-negative, and reflectance+transmittance<=1");
+    }
+}
+
+void validate_sky(float sky) {
+    if (!std::isfinite(sky) || sky < 0.0f) fail(FH_INVALID_ARGUMENT, "skyRadiosity must be finite and non-negative");
+}
+
+void validate_relax(float relaxation) {
+    if (!(relaxation > 0.0f && relaxation <= 1.0f)) fail(FH_INVALID_ARGUMENT, "GPU Jacobi relaxation must be in (0,1]");
+}
+
+void validate_bands(const fh_context* c, uint32_t nb, const float* r, const float* t, const float* e, const float* D,
+                    const float* sky) {
+    if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
+    if (!r || !t || !e || !D || !sky) fail(FH_INVALID_ARGUMENT, "band arrays must not be null");
+    const size_t n = size_t(c->S) * nb;
+    for (size_t i = 0; i < n; ++i) {
+        if (!std::isfinite(r[i]) || !std::isfinite(t[i]) || !std::isfinite(e[i]) || !std::isfinite(D[i]) ||
+            r[i] < 0.0f || t[i] < 0.0f || e[i] < 0.0f || D[i] < 0.0f || r[i] + t[i] > 1.0001f)
+            fail(FH_INVALID_ARGUMENT,
+                 "Optical values must be finite, non-negative, and reflectance+transmittance<=1");
+    }
+    for (uint32_t b = 0; b < nb; ++b) validate_sky(sky[b]);
+}
+
+template <class T>
+void upload(fh_context* c, T* dst, const T* src, size_t n) {
+    if (n) FH_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+}
+
+template <class T>
+void download(fh_context* c, T* dst, const T* src, size_t n) {
+    if (n) FH_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipStreamSynchronize(c->stream));
+}
+
+// Shared tail of fh_solve / fh_solve_accelerated: optics AoS -> r,t,e,D;
+// fold; Jacobi; download.
+void solve_single(fh_context* c, const fh_surface_optics* optics, float sky, const float* init,
+                  fh::JacobiOptions opt, float* radiosity, fh_solve_stats* stats) {
+    const uint32_t S = c->S;
+    c->fbuf.ensure(size_t(S) * 4u);
+    upload(c, c->fbuf.p, reinterpret_cast<const float*>(optics), size_t(S) * 4u);
+    c->rr.ensure(S);
+    c->tt.ensure(S);
+    c->cc.ensure(S);
+    c->DD.ensure(S);
+    c->HH.ensure(S);  // emission staging
+    fh::launch_split_optics(c, reinterpret_cast<const float4*>(c->fbuf.p), c->rr.p, c->tt.p, c->HH.p, c->DD.p);
+    fh::set_sky(c, &sky, 1u, 1u);
+    fh::launch_fold(c, c->HH.p, c->rr.p, c->tt.p, c->DD.p, 1u, c->cc.p);
+    c->B0.ensure(S);
+    c->B1.ensure(S);
+    if (init) upload(c, c->B0.p, init, S);
+    else fh::launch_fill_f32(c, c->B0.p, S, 0.0f);
+    float* bufs[2] = {c->B0.p, c->B1.p};
+    opt.ldb = 1u;
+    const int fin = fh::run_jacobi(c, opt, c->cc.p, c->rr.p, c->tt.p, bufs, stats);
+    if (radiosity) download(c, radiosity, bufs[fin], S);
+}
+
+void solve_bands_impl(fh_context* c, uint32_t nb, const float* dr, const float* dt, const float* de, const float* dD,
+                      const float* sky, const float* dinit, uint32_t maxIter, float tolRel, float relax,
+                      uint32_t precision, float* dout, fh_solve_stats* stats) {
+    const uint32_t S = c->S;
+    const uint32_t ldb = fh::band_stride(nb);
+    const size_t n = size_t(S) * ldb;
+    c->rr.ensure(n);
+    c->tt.ensure(n);
+    c->cc.ensure(n);
+    c->DD.ensure(n);
+    c->HH.ensure(n);
+    c->B0.ensure(n);
+    c->B1.ensure(n);
+    fh::launch_pad(c, dr, nb, ldb, c->rr.p);
+    fh::launch_pad(c, dt, nb, ldb, c->tt.p);
+    fh::launch_pad(c, de, nb, ldb, c->HH.p);
+    fh::launch_pad(c, dD, nb, ldb, c->DD.p);
+    fh::set_sky(c, sky, nb, ldb);
+    fh::launch_fold(c, c->HH.p, c->rr.p, c->tt.p, c->DD.p, ldb, c->cc.p);
+    if (dinit) fh::launch_pad(c, dinit, nb, ldb, c->B0.p);
+    else fh::launch_fill_f32(c, c->B0.p, n, 0.0f);
+    float* bufs[2] = {c->B0.p, c->B1.p};
+    fh::JacobiOptions opt;
+    opt.ldb = ldb;
+    opt.precision = precision;
+    opt.relax = relax;
+    opt.maxIter = maxIter;
+    opt.tolRel = tolRel;
+    const int fin = fh::run_jacobi(c, opt, c->cc.p, c->rr.p, c->tt.p, bufs, stats);
+    fh::launch_unpad(c, bufs[fin], nb, ldb, dout);
+    FH_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fh_status_string(int status) {
+    switch (status) {
+        case FH_OK: return "ok";
+        case FH_INVALID_ARGUMENT: return "invalid argument";
+        case FH_LOGIC_ERROR: return "logic error";
+        case FH_RUNTIME_ERROR: return "runtime error";
+        case FH_OVERFLOW_ERROR: return "overflow error";
+        default: return "unknown status";
+    }
+}
+
+const char* fh_last_error(const fh_context* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int fh_create(const fh_config* config, fh_context** out) {
+    g_create_error.clear();
+    if (!out) return FH_INVALID_ARGUMENT;
+    *out = nullptr;
+    try {
+        if (!config) fail(FH_INVALID_ARGUMENT, "config must not be null");
+        if (config->raysPerSide == 0u || config->raysPerSide > 2048u)
+            fail(FH_INVALID_ARGUMENT, "raysPerSide must be in [1, 2048]");
+        if (config->sunRaysPerSide == 0u || config->sunRaysPerSide > 65536u)
+            fail(FH_INVALID_ARGUMENT, "sunRaysPerSide must be in [1, 65536]");
+        if (!std::isfinite(config->rayEpsilon) || config->rayEpsilon < 0.0f)
+            fail(FH_INVALID_ARGUMENT, "rayEpsilon must be finite and non-negative");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) fail(FH_RUNTIME_ERROR, "no HIP device available");
+        if (int(config->gpuIndex) >= count) fail(FH_RUNTIME_ERROR, "gpuIndex out of range");
+        auto c = std::make_unique<fh_context>();
+        c->cfg = *config;
+        c->device = int(config->gpuIndex);
+        FH_HIP(hipSetDevice(c->device));
+        FH_HIP(hipStreamCreate(&c->stream));
+        FH_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->hctrl), sizeof(fh::SolveCtrl), hipHostMallocDefault));
+        c->ctrl.ensure(1);
+        *out = c.release();
+        return FH_OK;
+    } catch (const fh::Error& e) {
+        g_create_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        return FH_RUNTIME_ERROR;
+    }
+}
+
+void fh_destroy(fh_context* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->hctrl) (void)hipHostFree(c->hctrl);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+uint32_t fh_facet_count(const fh_context* c) { return c ? c->F : 0u; }
+uint32_t fh_surface_count(const fh_context* c) { return c ? c->S : 0u; }
+uint32_t fh_band_stride(uint32_t nb) { return fh::band_stride(nb); }
+uint32_t fh_directed_edge_count(const fh_context* c) { return c && c->hasGraph ? c->nnz : 0u; }
+
+// setGeometry / validateGeometry (facetrt_vulkan.cpp:425-471)
+int fh_set_geometry(fh_context* c, const fh_vertex* v, uint32_t vertexCount) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        if (!v || vertexCount == 0u || vertexCount % 3u != 0u)
+            fail(FH_INVALID_ARGUMENT, "Geometry must be a non-empty triangle list");
+        if (vertexCount / 3u > 0x7fffffffu / 2u) fail(FH_OVERFLOW_ERROR, "too many facets");
+        for (uint32_t i = 0; i < vertexCount; ++i) {
+            for (int k = 0; k < 3; ++k)
+                if (!std::isfinite(v[i].position[k])) fail(FH_INVALID_ARGUMENT, "Geometry contains a non-finite position");
+            const float* n = v[i].normal;
+            const float len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            if (!(len > 0.99f && len < 1.01f)) fail(FH_INVALID_ARGUMENT, "Facet normals must be normalized");
+        }
+        c->hasGeometry = false;
+        c->hasGraph = false;
+        const uint32_t F = vertexCount / 3u;
+        std::vector<float> tri9(size_t(F) * 9u);
+        std::vector<float4> nrm(F);
+        for (uint32_t f = 0; f < F; ++f) {
+            for (int k = 0; k < 3; ++k)
+                for (int q = 0; q < 3; ++q) tri9[size_t(f) * 9u + 3u * k + q] = v[3u * f + k].position[q];
+            nrm[f] = make_float4(v[3u * f].normal[0], v[3u * f].normal[1], v[3u * f].normal[2], 0.0f);
+        }
+        fh::BvhHost bvh;
+        fh::build_bvh(tri9.data(), F, bvh);
+        c->F = F;
+        c->S = 2u * F;
+        c->facetVerts.ensure(tri9.size());
+        c->facetNormal.ensure(F);
+        c->nodes.ensure(bvh.nodes.size());
+        c->tris.ensure(bvh.tris.size());
+        upload(c, c->facetVerts.p, tri9.data(), tri9.size());
+        upload(c, c->facetNormal.p, nrm.data(), nrm.size());
+        upload(c, c->nodes.p, bvh.nodes.data(), bvh.nodes.size());
+        upload(c, c->tris.p, bvh.tris.data(), bvh.tris.size());
+        FH_HIP(hipStreamSynchronize(c->stream));
+        c->bvhDepth = bvh.depth;
+        c->bvhNodes = uint32_t(bvh.nodes.size());
+        c->hasGeometry = true;
+    });
+}
+
+// buildVisibilityGraph (vkh:80) as ray-cast form factors
+int fh_build_form_factors(fh_context* c, fh_graph_diagnostics* out) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_geometry(c, "buildVisibilityGraph()");
+        c->hasGraph = false;
+        const uint32_t S = c->S, R = c->cfg.raysPerSide;
+        c->hits.ensure(size_t(S) * R);
+        c->cnt16.ensure(size_t(S) * R);
+        c->rowlen.ensure(S);
+        c->sky.ensure(S);
+        c->den.ensure(S);
+        c->rowptr.ensure(size_t(S) + 1u);
+        fh::launch_trace_graph(c, R, c->cfg.seed, c->cfg.rayEpsilon);
+        fh::launch_rows(c, R);
+        fh::launch_scan(c, c->rowlen.p, S, c->rowptr.p);
+        uint32_t nnz = 0;
+        download(c, &nnz, c->rowptr.p + S, 1);
+        c->nnz = nnz;
+        c->col.ensure(nnz);
+        c->cnt.ensure(nnz);
+        c->val32.ensure(nnz);
+        c->val16.ensure(nnz);
+        c->fsky.ensure(S);
+        fh::launch_compact(c, R);
+        c->misc.ensure(4);
+        fh::launch_fill_u32(c, c->misc.p, 4, 0u);
+        fh::launch_weights(c, c->misc.p);
+        uint32_t bits = 0;
+        download(c, &bits, c->misc.p, 1);
+        std::memcpy(&c->closure, &bits, sizeof(float));
+        c->hits.release();
+        c->cnt16.release();
+        c->hasGraph = true;
+        if (out) {
+            out->facetCount = c->F;
+            out->surfaceCount = S;
+            out->directedEdgeCount = nnz;
+            out->fragmentOverflow = 0u;
+            out->hashOverflow = 0u;
+            out->maxClosureError = c->closure;
+        }
+    });
+}
+
+int fh_set_graph(fh_context* c, uint32_t nnz, const uint32_t* rowOffsets, const uint32_t* neighbors,
+                 const uint32_t* counts, const uint32_t* denominators, const uint32_t* skyCounts,
+                 fh_graph_diagnostics* out) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_geometry(c, "setGraph()");
+        const uint32_t S = c->S;
+        if (!rowOffsets || !denominators || !skyCounts || (nnz && (!neighbors || !counts)))
+            fail(FH_INVALID_ARGUMENT, "graph arrays must not be null");
+        if (rowOffsets[0] != 0u || rowOffsets[S] != nnz) fail(FH_INVALID_ARGUMENT, "rowOffsets must run from 0 to nnz");
+        for (uint32_t s = 0; s < S; ++s)
+            if (rowOffsets[s + 1u] < rowOffsets[s]) fail(FH_INVALID_ARGUMENT, "rowOffsets must be non-decreasing");
+        for (uint32_t e = 0; e < nnz; ++e)
+            if (neighbors[e] >= S) fail(FH_RUNTIME_ERROR, "visibility graph contains an invalid surface index");
+        bool any = false;
+        for (uint32_t s = 0; s < S; ++s) any = any || denominators[s] > 0u;
+        if (!any) fail(FH_RUNTIME_ERROR, "Visibility raster has no surface samples; radiation results are invalid");
+        c->hasGraph = false;
+        c->nnz = nnz;
+        c->rowptr.ensure(size_t(S) + 1u);
+        c->col.ensure(nnz);
+        c->cnt.ensure(nnz);
+        c->den.ensure(S);
+        c->sky.ensure(S);
+        c->val32.ensure(nnz);
+        c->val16.ensure(nnz);
+        c->fsky.ensure(S);
+        upload(c, c->rowptr.p, rowOffsets, size_t(S) + 1u);
+        upload(c, c->col.p, neighbors, nnz);
+        upload(c, c->cnt.p, counts, nnz);
+        upload(c, c->den.p, denominators, S);
+        upload(c, c->sky.p, skyCounts, S);
+        c->misc.ensure(4);
+        fh::launch_fill_u32(c, c->misc.p, 4, 0u);
+        fh::launch_weights(c, c->misc.p);
+        uint32_t bits = 0;
+        download(c, &bits, c->misc.p, 1);
+        std::memcpy(&c->closure, &bits, sizeof(float));
+        c->hasGraph = true;
+        if (out) {
+            out->facetCount = c->F;
+            out->surfaceCount = S;
+            out->directedEdgeCount = nnz;
+            out->fragmentOverflow = 0u;
+            out->hashOverflow = 0u;
+            out->maxClosureError = c->closure;
+        }
+    });
+}
+
+int fh_get_graph(fh_context* c, uint32_t* rowOffsets, uint32_t* neighbors, uint32_t* counts, uint32_t* denominators,
+                 uint32_t* skyCounts) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "getGraph()");
+        if (rowOffsets) download(c, rowOffsets, c->rowptr.p, size_t(c->S) + 1u);
+        if (neighbors) download(c, neighbors, c->col.p, c->nnz);
+        if (counts) download(c, counts, c->cnt.p, c->nnz);
+        if (denominators) download(c, denominators, c->den.p, c->S);
+        if (skyCounts) download(c, skyCounts, c->sky.p, c->S);
+    });
+}
+
+// solve (facetrt_vulkan.cpp:1177-1247): zero start, fixed sweeps
+int fh_solve(fh_context* c, const fh_surface_optics* optics, float sky, uint32_t iterations, float relaxation,
+             float* radiosity, fh_solve_stats* stats) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "solve()");
+        validate_optics(c, optics);
+        validate_sky(sky);
+        validate_relax(relaxation);
+        fh::JacobiOptions opt;
+        opt.relax = relaxation;
+        opt.maxIter = iterations ? iterations : 1u;
+        solve_single(c, optics, sky, nullptr, opt, radiosity, stats);
+    });
+}
+
+// solveAccelerated (facetrt_vulkan.cpp:1249-1358)
+int fh_solve_accelerated(fh_context* c, const fh_surface_optics* optics, float sky, const float* init,
+                         uint32_t maximumIterations, uint32_t batchIterations, float tolerance, float relaxation,
+                         float* radiosity, fh_solve_stats* stats) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "solveAccelerated()");
+        validate_optics(c, optics);
+        validate_sky(sky);
+        validate_relax(relaxation);
+        if (maximumIterations == 0u || batchIterations == 0u || !std::isfinite(tolerance) || tolerance < 0.0f)
+            fail(FH_INVALID_ARGUMENT, "Accelerated solve iterations must be positive and tolerance non-negative");
+        if (init)
+            for (uint32_t s = 0; s < c->S; ++s)
+                if (!std::isfinite(init[s]) || init[s] < 0.0f)
+                    fail(FH_INVALID_ARGUMENT, "Initial radiosity must contain one finite non-negative value per surface");
+        fh::JacobiOptions opt;
+        opt.relax = relaxation;
+        opt.maxIter = maximumIterations;
+        opt.tolAbs = tolerance;
+        opt.checkEvery = batchIterations;
+        solve_single(c, optics, sky, init, opt, radiosity, stats);
+    });
+}
+
+// incidentIrradiance (facetrt_vulkan.cpp:1360-1399), computed on the device
+int fh_incident_irradiance(fh_context* c, const float* radiosity, const fh_surface_optics* optics, float sky,
+                           float* irradiance) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "incidentIrradiance()");
+        if (!radiosity || !optics || !irradiance)
+            fail(FH_INVALID_ARGUMENT, "Incident irradiance requires a completed solve and visibility graph");
+        validate_sky(sky);
+        const uint32_t S = c->S;
+        c->fbuf.ensure(size_t(S) * 4u);
+        c->rr.ensure(S);
+        c->tt.ensure(S);
+        c->DD.ensure(S);
+        c->HH.ensure(S);
+        c->B0.ensure(S);
+        upload(c, c->fbuf.p, reinterpret_cast<const float*>(optics), size_t(S) * 4u);
+        fh::launch_split_optics(c, reinterpret_cast<const float4*>(c->fbuf.p), c->rr.p, c->tt.p, c->HH.p, c->DD.p);
+        upload(c, c->B0.p, radiosity, S);
+        fh::set_sky(c, &sky, 1u, 1u);
+        fh::run_incident(c, 1u, c->B0.p, c->DD.p, c->HH.p);
+        download(c, irradiance, c->HH.p, S);
+    });
+}
+
+// computeSunlitFraction (facetrt_vulkan.cpp:1064-1158) by sun rays
+int fh_compute_sunlit_fraction(fh_context* c, fh_direction sun, float* fraction) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_geometry(c, "computeSunlitFraction()");
+        if (!fraction) fail(FH_INVALID_ARGUMENT, "fraction must not be null");
+        if (!std::isfinite(sun.x) || !std::isfinite(sun.y) || !std::isfinite(sun.z) ||
+            (sun.x == 0.0f && sun.y == 0.0f && sun.z == 0.0f))
+            fail(FH_INVALID_ARGUMENT, "sun direction must be finite and non-zero");
+        const uint32_t S = c->S;
+        c->misc.ensure(S);
+        c->HH.ensure(S);
+        fh::launch_fill_u32(c, c->misc.p, S, 0u);
+        fh::launch_sun(c, sun.x, sun.y, sun.z, c->misc.p, c->HH.p);
+        download(c, fraction, c->HH.p, S);
+    });
+}
+
+int fh_solve_bands(fh_context* c, uint32_t nb, const float* r, const float* t, const float* e, const float* D,
+                   const float* sky, const float* init, uint32_t maxIter, float tolRel, float relaxation,
+                   uint32_t precision, float* radiosity, fh_solve_stats* stats) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "solveBands()");
+        validate_bands(c, nb, r, t, e, D, sky);
+        validate_relax(relaxation);
+        if (precision > FH_PRECISION_BF16_DC) fail(FH_INVALID_ARGUMENT, "unknown precision");
+        if (!radiosity) fail(FH_INVALID_ARGUMENT, "radiosity must not be null");
+        const size_t n = size_t(c->S) * nb;
+        fh::DBuf<float> in;
+        in.ensure(n * (init ? 6u : 5u));
+        float* dr = in.p;
+        float* dt = dr + n;
+        float* de = dt + n;
+        float* dD = de + n;
+        float* dout = dD + n;
+        float* dinit = init ? dout + n : nullptr;
+        upload(c, dr, r, n);
+        upload(c, dt, t, n);
+        upload(c, de, e, n);
+        upload(c, dD, D, n);
+        if (init) upload(c, dinit, init, n);
+        solve_bands_impl(c, nb, dr, dt, de, dD, sky, dinit, maxIter ? maxIter : 1u, tolRel, relaxation, precision,
+                         dout, stats);
+        download(c, radiosity, dout, n);
+    });
+}
+
+int fh_solve_bands_device(fh_context* c, uint32_t nb, const float* dr, const float* dt, const float* de,
+                          const float* dD, const float* sky, const float* dinit, uint32_t maxIter, float tolRel,
+                          float relaxation, uint32_t precision, float* dout, fh_solve_stats* stats) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "solveBands()");
+        if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
+        if (!dr || !dt || !de || !dD || !sky || !dout) fail(FH_INVALID_ARGUMENT, "band arrays must not be null");
+        for (uint32_t b = 0; b < nb; ++b) validate_sky(sky[b]);
+        validate_relax(relaxation);
+        if (precision > FH_PRECISION_BF16_DC) fail(FH_INVALID_ARGUMENT, "unknown precision");
+        solve_bands_impl(c, nb, dr, dt, de, dD, sky, dinit, maxIter ? maxIter : 1u, tolRel, relaxation, precision,
+                         dout, stats);
+    });
+}
+
+int fh_incident_bands(fh_context* c, uint32_t nb, const float* radiosity, const float* direct, const float* sky,
+                      float* irradiance) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "incidentIrradiance()");
+        if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
+        if (!radiosity || !direct || !sky || !irradiance) fail(FH_INVALID_ARGUMENT, "arrays must not be null");
+        for (uint32_t b = 0; b < nb; ++b) validate_sky(sky[b]);
+        const uint32_t ldb = fh::band_stride(nb);
+        const size_t n = size_t(c->S) * nb, np = size_t(c->S) * ldb;
+        fh::DBuf<float> tmp;
+        tmp.ensure(2 * n + 3 * np);
+        float* dB = tmp.p;
+        float* dD = dB + n;
+        float* pB = dD + n;
+        float* pD = pB + np;
+        float* pH = pD + np;
+        upload(c, dB, radiosity, n);
+        upload(c, dD, direct, n);
+        fh::launch_pad(c, dB, nb, ldb, pB);
+        fh::launch_pad(c, dD, nb, ldb, pD);
+        fh::set_sky(c, sky, nb, ldb);
+        fh::run_incident(c, ldb, pB, pD, pH);
+        fh::launch_unpad(c, pH, nb, ldb, dB);
+        download(c, irradiance, dB, n);
+    });
+}
+
+int fh_sweep_device(fh_context* c, uint32_t nb, const float* dc, const float* dr, const float* dt, const float* dBin,
+                    float* dBout, uint32_t precision) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "sweep()");
+        if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
+        if (!dc || !dr || !dt || !dBin || !dBout) fail(FH_INVALID_ARGUMENT, "arrays must not be null");
+        fh::launch_ctrl_reset(c, 0xffffffffu, -1.0f, -1.0f, 1u);
+        fh::launch_single_sweep(c, fh::band_stride(nb), dc, dr, dt, dBin, dBout, precision == FH_PRECISION_BF16_DC);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+// engine.hpp -- internal types of the MI355X facet engine (not part of the C-ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/facet_hip.h"
+
+namespace fh {
+
+// Error carried across the C-ABI as an fh_status code + message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+#define FH_HIP(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            ::fh::fail(FH_RUNTIME_ERROR, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device buffer that only grows.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    size_t cap = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void ensure(size_t count) {
+        if (count > cap) {
+            release();
+            const size_t bytes = (count ? count : 1) * sizeof(T);
+            FH_HIP(hipMalloc(reinterpret_cast<void**>(&p), bytes));
+            cap = count ? count : 1;
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = cap = 0;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Child-pair BVH node (64 B): the boxes of both children, so one node fetch
+// (4 x 16 B) tests two boxes.
+//   x = (lo0.x, hi0.x, lo1.x, hi1.x), y = (...y...), z = (...z...)
+//   meta = (child0, child1, count0, count1): count > 0 -> leaf with
+//   primitives [child, child + count); count == 0 && child >= 0 -> internal
+//   node index; child < 0 -> empty slot (its box never hits).
+struct alignas(16) BvhNode {
+    float4 x, y, z;
+    int4 meta;
+};
+
+struct BvhHost {
+    std::vector<BvhNode> nodes;
+    std::vector<float4> tris;        // 3 per primitive slot; v0.w = facet id bits
+    std::vector<uint32_t> primFacet;  // slot -> facet
+    int depth = 0;
+};
+
+// Binned-SAH build over facet triangles tri9 (9 floats per facet).
+void build_bvh(const float* tri9, uint32_t F, BvhHost& out);
+
+constexpr int kBvhMaxDepth = 48;  // stack depth bound of the traversal kernels
+
+// Device-side solve control block (one per context).
+struct SolveCtrl {
+    uint32_t done;
+    uint32_t iter;
+    uint32_t maxIter;
+    uint32_t checkEvery;  // absolute tolerance checked when iter % checkEvery == 0
+    float tolRel;         // < 0: off
+    float tolAbs;         // < 0: off
+    uint32_t absBits;     // running max |dB| (float bits) of the sweep in flight
+    uint32_t relBits;     // running max |dB|/|B|
+    uint32_t arrived;     // blocks finished in the sweep in flight
+    float lastAbs;
+    float lastRel;
+    uint32_t refreshes;
+    uint32_t pad[4];
+};
+
+uint32_t band_stride(uint32_t nb);
+
+}  // namespace fh
+
+struct fh_context {
+    fh_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    uint32_t F = 0, S = 0;
+    bool hasGeometry = false;
+    bool hasGraph = false;
+
+    // geometry
+    fh::DBuf<float> facetVerts;      // F x 9
+    fh::DBuf<float4> facetNormal;    // F
+    fh::DBuf<fh::BvhNode> nodes;
+    fh::DBuf<float4> tris;
+    int bvhDepth = 0;
+    uint32_t bvhNodes = 0;
+
+    // graph (CSR over sides)
+    uint32_t nnz = 0;
+    float closure = 0.0f;
+    fh::DBuf<uint32_t> rowptr, col, cnt, den, sky;
+    fh::DBuf<float> val32, fsky;
+    fh::DBuf<uint16_t> val16;
+
+    // scratch for the ray graph build
+    fh::DBuf<uint32_t> hits, rowlen, scanTmp;
+    fh::DBuf<uint16_t> cnt16;
+
+    // solve workspace
+    fh::DBuf<float> B0, B1, cc, rr, tt, DD, c2, HH, skyv;
+    fh::DBuf<fh::SolveCtrl> ctrl;
+    fh::SolveCtrl* hctrl = nullptr;  // pinned mirror
+    fh::DBuf<uint32_t> misc;         // small counters
+    fh::DBuf<float> fbuf;            // small float scratch
+};

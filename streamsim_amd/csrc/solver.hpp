@@ -1,0 +1,54 @@
+// solver.hpp -- host-side orchestration shared by the C-ABI and the FacetEB
+// driver: kernel launchers and the device-converged Jacobi loop.
+#pragma once
+
+#include "engine.hpp"
+
+namespace fh {
+
+struct SweepArgs;
+
+// raytrace.hip
+void launch_trace_graph(fh_context* c, uint32_t R, uint32_t seed, float tmin);
+void launch_rows(fh_context* c, uint32_t R);
+void launch_scan(fh_context* c, const uint32_t* in, uint32_t n, uint32_t* out);
+void launch_compact(fh_context* c, uint32_t R);
+void launch_weights(fh_context* c, uint32_t* closureBits);
+void launch_sun(fh_context* c, float sx, float sy, float sz, uint32_t* lit, float* frac);
+
+// spmm.hip
+void launch_fold(fh_context* c, const float* e, const float* r, const float* t, const float* D, uint32_t ldb,
+                 float* out);
+void launch_pad(fh_context* c, const float* src, uint32_t nb, uint32_t ldb, float* dst);
+void launch_unpad(fh_context* c, const float* src, uint32_t nb, uint32_t ldb, float* dst);
+void launch_split_optics(fh_context* c, const float4* o, float* r, float* t, float* e, float* D);
+void launch_ctrl_reset(fh_context* c, uint32_t maxIter, float tolRel, float tolAbs, uint32_t checkEvery);
+void launch_fill_u32(fh_context* c, uint32_t* p, size_t n, uint32_t v);
+void launch_fill_f32(fh_context* c, float* p, size_t n, float v);
+
+// One solve of (I - M) B = c on the current graph: bufs[0] holds the start
+// value; c/r/t are padded [S][ldb] device arrays.  Returns the index (0/1)
+// of the buffer holding the result and fills stats.
+struct JacobiOptions {
+    uint32_t ldb = 1;
+    uint32_t precision = FH_PRECISION_FP32;
+    float relax = 1.0f;
+    uint32_t maxIter = 64;
+    float tolRel = -1.0f;
+    float tolAbs = -1.0f;
+    uint32_t checkEvery = 1;
+    uint32_t innerBf16 = 6;  // BF16 sweeps between FP32 refreshes
+};
+int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const float* rr, const float* tt,
+               float* bufs[2], fh_solve_stats* stats);
+
+// H = max(0, fsky S + G(B) + D) for all bands ([S][ldb])
+void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, float* H);
+
+void set_sky(fh_context* c, const float* sky, uint32_t nb, uint32_t ldb);
+
+// One sweep, no host synchronisation (timing entry).
+void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
+                         const float* Bin, float* Bout, bool w16);
+
+}  // namespace fh

@@ -1,0 +1,228 @@
+"""Host-side mirror of the reference's device backend class.
+
+``FacetEngine`` has the operations of ``facetvk::FacetrtVulkan``
+(models/histream/src/facetrt/facetrt_vulkan.h:70-104) with the same argument
+meaning and error classes (ValueError = std::invalid_argument,
+LogicError = std::logic_error, RuntimeError, OverflowError); every call goes
+through the C-ABI of ``include/facet_hip.h`` into HIP kernels on one GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import FhConfig, FhDirection, FhGraphDiagnostics, FhSolveStats, raise_for
+
+
+@dataclass
+class GraphDiagnostics:
+    """facetvk::GraphDiagnostics (facetrt_vulkan.h:49-56)."""
+    facetCount: int
+    surfaceCount: int
+    directedEdgeCount: int
+    fragmentOverflow: int
+    hashOverflow: int
+    maxClosureError: float
+
+
+@dataclass
+class SolveResult:
+    """facetvk::SolveResult (facetrt_vulkan.h:58-62) plus the relative delta."""
+    radiosity: np.ndarray
+    iterations: int
+    maxDelta: float
+    maxRelDelta: float = 0.0
+    refreshes: int = 0
+
+
+@dataclass
+class Graph:
+    rowOffsets: np.ndarray
+    neighbors: np.ndarray
+    counts: np.ndarray
+    denominators: np.ndarray
+    skyCounts: np.ndarray
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    x = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and x.shape != shape:
+        x = x.reshape(shape)
+    return x
+
+
+def _diag(d: FhGraphDiagnostics) -> GraphDiagnostics:
+    return GraphDiagnostics(d.facetCount, d.surfaceCount, d.directedEdgeCount, d.fragmentOverflow,
+                            d.hashOverflow, d.maxClosureError)
+
+
+class FacetEngine:
+    """One GPU context: geometry, form factors, radiosity solves, sunlit fraction."""
+
+    def __init__(self, rays_per_side: int = 256, sun_rays_per_side: int = 64, seed: int = 1,
+                 gpu_index: int = 0, ray_epsilon: float = 1.0e-5):
+        self._lib = _lib.load()
+        cfg = FhConfig()
+        cfg.raysPerSide = rays_per_side
+        cfg.sunRaysPerSide = sun_rays_per_side
+        cfg.seed = seed
+        cfg.gpuIndex = gpu_index
+        cfg.rayEpsilon = ray_epsilon
+        cfg.reorder = 0
+        self.config = cfg
+        h = C.c_void_p()
+        st = self._lib.fh_create(C.byref(cfg), C.byref(h))
+        if st != _lib.FH_OK:
+            raise_for(st, None)
+        self._h = h
+
+    # -- lifecycle --------------------------------------------------------
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.fh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+    def _check(self, st: int) -> None:
+        raise_for(st, self._h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def facetCount(self) -> int:
+        return int(self._lib.fh_facet_count(self._h))
+
+    def surfaceCount(self) -> int:
+        return int(self._lib.fh_surface_count(self._h))
+
+    # -- geometry and form factors -----------------------------------------
+    def setGeometry(self, vertices) -> None:
+        v = _f32(vertices)
+        if v.ndim != 2 or v.shape[1] != 6:
+            raise ValueError("vertices must be an (N, 6) array of position + normal")
+        self._check(self._lib.fh_set_geometry(self._h, _ptr(v), v.shape[0]))
+
+    def buildVisibilityGraph(self) -> GraphDiagnostics:
+        d = FhGraphDiagnostics()
+        self._check(self._lib.fh_build_form_factors(self._h, C.byref(d)))
+        return _diag(d)
+
+    def setGraph(self, rowOffsets, neighbors, counts, denominators, skyCounts) -> GraphDiagnostics:
+        ro = np.ascontiguousarray(rowOffsets, dtype=np.uint32)
+        nb = np.ascontiguousarray(neighbors, dtype=np.uint32)
+        ct = np.ascontiguousarray(counts, dtype=np.uint32)
+        de = np.ascontiguousarray(denominators, dtype=np.uint32)
+        sk = np.ascontiguousarray(skyCounts, dtype=np.uint32)
+        S = self.surfaceCount()
+        if ro.size != S + 1 or de.size != S or sk.size != S or nb.size != ct.size:
+            raise ValueError("graph array sizes do not match the geometry")
+        d = FhGraphDiagnostics()
+        self._check(self._lib.fh_set_graph(self._h, nb.size, _ptr(ro), _ptr(nb), _ptr(ct), _ptr(de), _ptr(sk),
+                                           C.byref(d)))
+        return _diag(d)
+
+    def getGraph(self) -> Graph:
+        S = self.surfaceCount()
+        nnz = int(self._lib.fh_directed_edge_count(self._h))
+        g = Graph(np.zeros(S + 1, np.uint32), np.zeros(nnz, np.uint32), np.zeros(nnz, np.uint32),
+                  np.zeros(S, np.uint32), np.zeros(S, np.uint32))
+        self._check(self._lib.fh_get_graph(self._h, _ptr(g.rowOffsets), _ptr(g.neighbors), _ptr(g.counts),
+                                           _ptr(g.denominators), _ptr(g.skyCounts)))
+        return g
+
+    def computeSunlitFraction(self, direction) -> np.ndarray:
+        S = self.surfaceCount()
+        out = np.zeros(S, np.float32)
+        d = FhDirection(float(direction[0]), float(direction[1]), float(direction[2]))
+        self._check(self._lib.fh_compute_sunlit_fraction(self._h, d, _ptr(out)))
+        return out
+
+    # -- radiosity -----------------------------------------------------------
+    def _optics(self, optics) -> np.ndarray:
+        o = _f32(optics)
+        S = self.surfaceCount()
+        if o.shape != (S, 4):
+            raise ValueError("Optics array must contain exactly two sides per facet")
+        return o
+
+    def solve(self, optics, skyRadiosity: float, iterations: int = 64, relaxation: float = 1.0) -> SolveResult:
+        o = self._optics(optics)
+        out = np.zeros(self.surfaceCount(), np.float32)
+        st = FhSolveStats()
+        self._check(self._lib.fh_solve(self._h, _ptr(o), skyRadiosity, iterations, relaxation, _ptr(out),
+                                       C.byref(st)))
+        return SolveResult(out, st.iterations, st.maxDelta, st.maxRelDelta, st.refreshes)
+
+    def solveAccelerated(self, optics, skyRadiosity: float, initialRadiosity=None, maximumIterations: int = 64,
+                         batchIterations: int = 8, tolerance: float = 1.0e-4,
+                         relaxation: float = 1.0) -> SolveResult:
+        o = self._optics(optics)
+        init = None if initialRadiosity is None else _f32(initialRadiosity)
+        if init is not None and init.size != self.surfaceCount():
+            raise ValueError("Initial radiosity must contain one finite non-negative value per surface")
+        out = np.zeros(self.surfaceCount(), np.float32)
+        st = FhSolveStats()
+        self._check(self._lib.fh_solve_accelerated(self._h, _ptr(o), skyRadiosity, _ptr(init), maximumIterations,
+                                                   batchIterations, tolerance, relaxation, _ptr(out), C.byref(st)))
+        return SolveResult(out, st.iterations, st.maxDelta, st.maxRelDelta, st.refreshes)
+
+    def incidentIrradiance(self, solution, optics, skyRadiosity: float) -> np.ndarray:
+        B = solution.radiosity if isinstance(solution, SolveResult) else solution
+        B = _f32(B)
+        o = self._optics(optics)
+        if B.size != self.surfaceCount():
+            raise ValueError("Incident irradiance requires a completed solve and visibility graph")
+        out = np.zeros(self.surfaceCount(), np.float32)
+        self._check(self._lib.fh_incident_irradiance(self._h, _ptr(B), _ptr(o), skyRadiosity, _ptr(out)))
+        return out
+
+    def solveBands(self, reflectance, transmittance, emission, direct, sky, init=None, maxIter: int = 1000,
+                   tolRel: float = 1.0e-6, relaxation: float = 1.0,
+                   precision: int = _lib.FH_PRECISION_FP32) -> SolveResult:
+        S = self.surfaceCount()
+        r = _f32(reflectance)
+        nb = r.size // S
+        shape = (S, nb)
+        r = r.reshape(shape)
+        t = _f32(transmittance, shape)
+        e = _f32(emission, shape)
+        D = _f32(direct, shape)
+        sk = _f32(sky).reshape(-1)
+        if sk.size != nb:
+            raise ValueError("sky must hold one value per band")
+        i0 = None if init is None else _f32(init, shape)
+        out = np.zeros(shape, np.float32)
+        st = FhSolveStats()
+        self._check(self._lib.fh_solve_bands(self._h, nb, _ptr(r), _ptr(t), _ptr(e), _ptr(D), _ptr(sk), _ptr(i0),
+                                             maxIter, tolRel, relaxation, precision, _ptr(out), C.byref(st)))
+        return SolveResult(out, st.iterations, st.maxDelta, st.maxRelDelta, st.refreshes)
+
+    def incidentBands(self, radiosity, direct, sky) -> np.ndarray:
+        B = _f32(radiosity)
+        S = self.surfaceCount()
+        nb = B.size // S
+        B = B.reshape(S, nb)
+        D = _f32(direct, (S, nb))
+        sk = _f32(sky).reshape(-1)
+        out = np.zeros((S, nb), np.float32)
+        self._check(self._lib.fh_incident_bands(self._h, nb, _ptr(B), _ptr(D), _ptr(sk), _ptr(out)))
+        return out
