@@ -175,6 +175,124 @@ int fh_sweep_device(fh_context *ctx, uint32_t nb, const float *d_c, const float 
                     const float *d_t, const float *d_Bin, float *d_Bout, uint32_t precision);
 uint32_t fh_band_stride(uint32_t nb);
 
+/* The context's HIP stream (hipStream_t), for event timing by the caller. */
+void *fh_stream(const fh_context *ctx);
+
+
+/* ---- FacetEB: coupled shortwave / longwave radiosity + energy balance ---- */
+
+/* Energy-balance fields of SurfaceParameters (core:113-143). */
+typedef struct {
+    float tirReflectance;
+    float tirTransmittance;
+    float surfaceResistance;   /* s m-1 */
+    float specificHeat;        /* soil c, J kg-1 K-1 */
+    float density;             /* soil rho, kg m-3 */
+    float thermalConductivity; /* soil k, W m-1 K-1 */
+    float soilMoisture;
+    float heatCapacityPerArea; /* leaf C_A, J m-2 K-1 */
+    float convectiveScale;
+    int32_t soilTemperatureMethod; /* 0: G = 0.35 Rn, 1: k dT, 2: 1.5 k dT */
+    int32_t vegetation;            /* 1: leaf transpires (LE on) */
+    int32_t reserved;
+} fh_eb_material;
+
+/* Scene description for runFacetEBCore (core:3164-4154): facets
+ * [0, leafFacetCount) are leaves (one temperature for both sides), the rest
+ * ground (one temperature per side).  swReflectance / swTransmittance are
+ * [materialCount][swBands]: the window-centre optics of each shortwave band
+ * (makeEnergyShortwaveOptics, core:952-1003). */
+int fh_eb_set_scene(fh_context *ctx, uint32_t leafFacetCount, const uint32_t *facetMaterial,
+                    uint32_t materialCount, const fh_eb_material *materials, uint32_t swBands,
+                    const float *swReflectance, const float *swTransmittance);
+
+typedef enum {
+    FH_WARM_PREVIOUS = 0, /* start each node from the previous node's state */
+    FH_WARM_SURROGATE = 1 /* start the coupled fixed point from the MLP guess */
+} fh_warm_start;
+
+/* One meteorological node (core:3462-3846). */
+typedef struct {
+    fh_direction sun;          /* unit vector toward the sun              */
+    float airTemperatureK;     /* clamp(Tair + 273.15, 220, 340)           */
+    float vaporPressure;       /* hPa                                      */
+    float pressure;            /* hPa                                      */
+    float windSpeed;           /* m s-1                                    */
+    float skyLongwave;         /* W m-2                                    */
+    float dTime;               /* s                                        */
+    float temperatureTolerance;/* K (also the residual-equivalent bound)  */
+    float temperatureRelaxation;
+    uint32_t maxCoupling;      /* coupling iteration budget                */
+    uint32_t vegetationMethod; /* 0 | 1                                    */
+    float radiosityTolerance;  /* relative change, SW and LW solves        */
+    uint32_t maxSweeps;        /* per radiosity solve                      */
+    uint32_t precision;        /* fh_precision of the SW and LW solves     */
+    uint32_t warmStart;        /* fh_warm_start                            */
+    uint32_t firstNode;        /* 1: T = airTemperatureK (core:3470-3479)  */
+    uint32_t zeroStart;        /* 1: every solve starts from B = 0         */
+    uint32_t recordFeatures;   /* 1: keep the surrogate features (training) */
+} fh_eb_node_params;
+
+typedef struct {
+    uint32_t couplingIterations;
+    uint32_t converged;
+    uint32_t swSweeps;
+    uint32_t lwSweeps;
+    float maxTemperatureDelta;
+    float maxResidualEquivalent;
+    uint32_t swRefreshes;
+    uint32_t reserved;
+    double swInput;      /* sum A (D + fsky S) over sides and bands, W     */
+    double swAbsorbed;   /* sum A alpha H                                  */
+    double swEscaped;    /* sum A fsky B                                   */
+    double netRadiation; /* sum A Rn over energy-balance sides             */
+} fh_eb_node_stats;
+
+/* Runs one node fully on the device: sunlit rays, SW optics, multi-band SW
+ * solve, absorbed SW, then the coupling loop (LW source from T in FP64, LW
+ * solve, incident LW, FP64 relaxed Newton per facet) until
+ * max dT <= tol and max |residual|/max(1, response) <= tol, and the final
+ * LW solve + residual (core:3834-3846).  swDirectNormal / swDiffuse hold
+ * one value per SW band (W m-2). */
+int fh_eb_run_node(fh_context *ctx, const fh_eb_node_params *params, const float *swDirectNormal,
+                   const float *swDiffuse, fh_eb_node_stats *stats);
+
+/* Downloads the node state; any pointer may be NULL.  Per side unless noted:
+ * T, absorbed SW, LW radiosity, sunlit fraction, net radiation, sensible,
+ * latent and storage heat; swRadiosity is [surfaceCount][swBands]. */
+int fh_eb_get_state(fh_context *ctx, float *temperature, float *absorbedShortwave,
+                    float *longwaveRadiosity, float *sunlit, float *netRadiation, float *sensibleHeat,
+                    float *latentHeat, float *storageHeat, float *swRadiosity);
+/* Replaces the temperature state (surfaceCount values). */
+int fh_eb_set_temperature(fh_context *ctx, const float *temperature);
+
+
+/* ---- learned warm start (no counterpart in the reference) -------------- */
+/* MLP 9 -> hidden -> hidden -> 1 with tanh-form GELU.  Parameter layout:
+ * W1[hidden][9], b1[hidden], W2[hidden][hidden], b2[hidden], W3[hidden], b3.
+ * Inputs per energy-balance unit (leaf facet, or ground side): absorbed SW,
+ * sky-view fraction, ground-view fraction, LW emissivity, material index,
+ * air temperature, wind speed, previous temperature, sky LW; normalised as
+ * (x - featureMean) / featureStd.  Output y: T = Tair + targetMean +
+ * targetStd * y. */
+uint32_t fh_mlp_param_count(uint32_t hidden);
+int fh_mlp_set_params(fh_context *ctx, uint32_t hidden, const float *params, const float *featureMean,
+                      const float *featureStd, float targetMean, float targetStd);
+int fh_mlp_get_params(fh_context *ctx, float *params);
+/* Raw network output y for n normalised input rows (host arrays). */
+int fh_mlp_forward(fh_context *ctx, const float *x, uint32_t n, float *y);
+/* Mean-squared error over n samples and its gradient (host arrays). */
+int fh_mlp_loss_grad(fh_context *ctx, const float *x, const float *target, uint32_t n, float *grad,
+                     float *loss);
+/* Adam training on device arrays d_x [n][9] (normalised) and d_target [n]. */
+int fh_mlp_train(fh_context *ctx, const float *d_x, const float *d_target, uint32_t n, uint32_t batch,
+                 uint32_t epochs, float learningRate, uint32_t seed, float *lastLoss);
+/* Energy-balance units of the scene and, after a node run with
+ * recordFeatures or the surrogate, their raw features [units][9] and current
+ * temperatures [units]. */
+uint32_t fh_eb_unit_count(const fh_context *ctx);
+int fh_eb_get_features(fh_context *ctx, float *features, float *temperature);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,35 @@ class FhSolveStats(C.Structure):
     ]
 
 
+class FhEbMaterial(C.Structure):
+    _fields_ = [
+        ("tirReflectance", C.c_float), ("tirTransmittance", C.c_float), ("surfaceResistance", C.c_float),
+        ("specificHeat", C.c_float), ("density", C.c_float), ("thermalConductivity", C.c_float),
+        ("soilMoisture", C.c_float), ("heatCapacityPerArea", C.c_float), ("convectiveScale", C.c_float),
+        ("soilTemperatureMethod", C.c_int32), ("vegetation", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class FhEbNodeParams(C.Structure):
+    _fields_ = [
+        ("sun", FhDirection), ("airTemperatureK", C.c_float), ("vaporPressure", C.c_float),
+        ("pressure", C.c_float), ("windSpeed", C.c_float), ("skyLongwave", C.c_float), ("dTime", C.c_float),
+        ("temperatureTolerance", C.c_float), ("temperatureRelaxation", C.c_float), ("maxCoupling", C.c_uint32),
+        ("vegetationMethod", C.c_uint32), ("radiosityTolerance", C.c_float), ("maxSweeps", C.c_uint32),
+        ("precision", C.c_uint32), ("warmStart", C.c_uint32), ("firstNode", C.c_uint32),
+        ("zeroStart", C.c_uint32), ("recordFeatures", C.c_uint32),
+    ]
+
+
+class FhEbNodeStats(C.Structure):
+    _fields_ = [
+        ("couplingIterations", C.c_uint32), ("converged", C.c_uint32), ("swSweeps", C.c_uint32),
+        ("lwSweeps", C.c_uint32), ("maxTemperatureDelta", C.c_float), ("maxResidualEquivalent", C.c_float),
+        ("swRefreshes", C.c_uint32), ("reserved", C.c_uint32), ("swInput", C.c_double),
+        ("swAbsorbed", C.c_double), ("swEscaped", C.c_double), ("netRadiation", C.c_double),
+    ]
+
+
 _P = C.c_void_p
 _U32 = C.c_uint32
 _F = C.c_float
@@ -84,6 +113,19 @@ SIGNATURES = {
     "fh_incident_bands": (_I, [_P, _U32, _P, _P, _P, _P]),
     "fh_sweep_device": (_I, [_P, _U32, _P, _P, _P, _P, _P, _U32]),
     "fh_band_stride": (_U32, [_U32]),
+    "fh_stream": (_P, [_P]),
+    "fh_eb_set_scene": (_I, [_P, _U32, _P, _U32, _P, _U32, _P, _P]),
+    "fh_eb_run_node": (_I, [_P, C.POINTER(FhEbNodeParams), _P, _P, C.POINTER(FhEbNodeStats)]),
+    "fh_eb_get_state": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fh_eb_set_temperature": (_I, [_P, _P]),
+    "fh_mlp_param_count": (_U32, [_U32]),
+    "fh_mlp_set_params": (_I, [_P, _U32, _P, _P, _P, _F, _F]),
+    "fh_mlp_get_params": (_I, [_P, _P]),
+    "fh_mlp_forward": (_I, [_P, _P, _U32, _P]),
+    "fh_mlp_loss_grad": (_I, [_P, _P, _P, _U32, _P, _P]),
+    "fh_mlp_train": (_I, [_P, _P, _P, _U32, _U32, _U32, _F, _U32, _P]),
+    "fh_eb_unit_count": (_U32, [_P]),
+    "fh_eb_get_features": (_I, [_P, _P, _P]),
 }
 
 _lib: C.CDLL | None = None

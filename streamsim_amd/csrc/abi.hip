@@ -211,6 +211,7 @@ void fh_destroy(fh_context* c) {
 uint32_t fh_facet_count(const fh_context* c) { return c ? c->F : 0u; }
 uint32_t fh_surface_count(const fh_context* c) { return c ? c->S : 0u; }
 uint32_t fh_band_stride(uint32_t nb) { return fh::band_stride(nb); }
+void* fh_stream(const fh_context* c) { return c ? static_cast<void*>(c->stream) : nullptr; }
 uint32_t fh_directed_edge_count(const fh_context* c) { return c && c->hasGraph ? c->nnz : 0u; }
 
 // setGeometry / validateGeometry (facetrt_vulkan.cpp:425-471)

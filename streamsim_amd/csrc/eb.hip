@@ -1,0 +1,637 @@
+// eb.hip -- FacetEB on the device: one meteorological node of the coupled
+// shortwave / longwave radiosity and surface energy balance.
+//
+// Follows runFacetEBCore (models/histream/src/facetrt/facetrtcore.cpp,
+// "core"):
+//   shortwave windows -> optics, solve, incident, absorbed   core:3495-3565
+//   energy-balance constants (h_a, rho_a, r_a)              core:3567-3578
+//   fluxes per side (leaf facets share side 0's T)          core:3592-3720
+//   residual / thermal response                             core:3726-3743
+//   relaxed Newton update, clamp [220, 360] K               core:3758-3827
+//   convergence: max dT <= tol and max res/resp <= tol      core:3829-3831
+//   final LW solve + residual, converged flag               core:3834-3846
+// with the band loop fused into the multi-band SpMM, every optics array
+// built on the device from material tables, and the Newton step and the
+// scene-level energy sums in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "solver.hpp"
+
+namespace fh {
+
+constexpr double kSigma = 5.670374419e-8;
+constexpr double kPiD = 3.14159265358979323846;
+
+struct EbConst {
+    double Ta, ea, P, ha, rho, ra, dTime, relax;
+    uint32_t vegMethod;
+    uint32_t leafCount;
+};
+
+__device__ __forceinline__ double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// core:854-867
+__device__ __forceinline__ double sat_vp(double T) {
+    const double c = clampd(T - 273.15, -80.0, 80.0);
+    return 6.107 * pow(10.0, 7.5 * c / (237.3 + c));
+}
+__device__ __forceinline__ double sat_vp_slope(double T) {
+    const double c = clampd(T - 273.15, -80.0, 80.0);
+    return sat_vp(T) * 2.3026 * 7.5 * 237.3 / ((237.3 + c) * (237.3 + c));
+}
+// core:869-905
+__device__ __forceinline__ void latent(double T, const EbConst& k, double rs, double& le, double& dle) {
+    const double c = T - 273.15;
+    const double lam = (2.501 - 0.002361 * c) * 1.0e6;
+    const double Pm = fmax(100.0, k.P);
+    const double deficit = 0.622 * fmax(0.0, sat_vp(T) - k.ea) / Pm;
+    const double inv = 1.0 / fmax(1.0, k.ra + rs);
+    le = k.rho * lam * deficit * inv;
+    dle = fmax(0.0, k.rho * inv * (-2361.0 * deficit + lam * 0.622 * sat_vp_slope(T) / Pm));
+}
+// core:907-915
+__device__ __forceinline__ double soil_storage(const fh_eb_material& m, double dT) {
+    const double inertia = sqrt(fmax(0.0, double(m.specificHeat) * m.density * m.thermalConductivity));
+    return 2.0 * inertia / sqrt(kPiD * fmax(1.0, dT));
+}
+
+// facet areas (FP64), for the scene-level energy sums
+__global__ void k_areas(const float* __restrict__ fv, uint32_t F, double* __restrict__ area) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const float* t = fv + 9 * size_t(f);
+    const double ux = double(t[3]) - t[0], uy = double(t[4]) - t[1], uz = double(t[5]) - t[2];
+    const double vx = double(t[6]) - t[0], vy = double(t[7]) - t[1], vz = double(t[8]) - t[2];
+    const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+    area[f] = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+}
+
+// per-side LW optics constants (makeEnergyLongwaveOptics, core:1005-1036,
+// angular strength 0)
+__global__ void k_lw_setup(uint32_t S, const uint32_t* __restrict__ mat, const fh_eb_material* __restrict__ mats,
+                           float* __restrict__ r, float* __restrict__ t, float* __restrict__ eps) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const fh_eb_material m = mats[mat[s >> 1]];
+    const float rr = fminf(fmaxf(m.tirReflectance, 0.0f), 1.0f);
+    const float tt = fminf(fmaxf(m.tirTransmittance, 0.0f), 1.0f - rr);
+    const float e = fmaxf(0.0f, 1.0f - rr - tt);
+    r[s] = fminf(fmaxf(1.0f - tt - e, 0.0f), 1.0f);
+    t[s] = tt;
+    eps[s] = fminf(fmaxf(1.0f - r[s] - tt, 0.0f), 1.0f);
+}
+
+// LW source c = e + r fsky_s L + t fsky_o L with e = eps sigma T^4 (FP64)
+__global__ void k_lw_source(uint32_t S, const float* __restrict__ T, const float* __restrict__ eps,
+                            const float* __restrict__ r, const float* __restrict__ t, const float* __restrict__ fsky,
+                            float L, float* __restrict__ e, float* __restrict__ c) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const double Tc = clampd(double(T[s]), 1.0, 1000.0);
+    const float em = float(double(eps[s]) * kSigma * Tc * Tc * Tc * Tc);
+    e[s] = em;
+    c[s] = em + r[s] * (fsky[s] * L) + t[s] * (fsky[s ^ 1u] * L);
+}
+
+// SW window optics per side and band (makeEnergyShortwaveOptics,
+// core:952-1003, angular strength 0, no reflectance texture)
+__global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_t* __restrict__ mat,
+                            const float* __restrict__ refl, const float* __restrict__ trans,
+                            const float4* __restrict__ nrm, const float* __restrict__ sunlit, float sx, float sy,
+                            float sz, const float* __restrict__ dni, float* __restrict__ r, float* __restrict__ t,
+                            float* __restrict__ D) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(S) * ldb) return;
+    const uint32_t s = uint32_t(i / ldb), b = uint32_t(i - size_t(s) * ldb);
+    if (b >= nb) {
+        r[i] = 0.0f;
+        t[i] = 0.0f;
+        D[i] = 0.0f;
+        return;
+    }
+    const uint32_t m = mat[s >> 1];
+    const float rr = fminf(fmaxf(refl[size_t(m) * nb + b], 0.0f), 1.0f);
+    const float tt = fminf(fmaxf(trans[size_t(m) * nb + b], 0.0f), 1.0f - rr);
+    r[i] = rr;
+    t[i] = fminf(tt, 1.0f - rr);
+    const float4 n = nrm[s >> 1];
+    const float sg = (s & 1u) ? -1.0f : 1.0f;
+    const float cosv = fmaxf(0.0f, (sg * n.x) * sx + (sg * n.y) * sy + (sg * n.z) * sz);
+    D[i] = fminf(fmaxf(sunlit[s], 0.0f), 1.0f) * cosv * dni[b];
+}
+
+// absorbed SW = sum_b alpha H, incident = sum_b H; FP64 block partials of
+// sum A (D + fsky S), sum A alpha H, sum A fsky B for the energy closure.
+__global__ __launch_bounds__(256) void k_absorb(uint32_t S, uint32_t nb, uint32_t ldb, const float* __restrict__ r,
+                                                const float* __restrict__ t, const float* __restrict__ H,
+                                                const float* __restrict__ D, const float* __restrict__ B,
+                                                const float* __restrict__ fsky, const float* __restrict__ skyv,
+                                                const double* __restrict__ area, float* __restrict__ absorbed,
+                                                float* __restrict__ incident, double* __restrict__ part) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    double pin = 0.0, pabs = 0.0, pesc = 0.0;
+    if (s < S) {
+        float a = 0.0f, h = 0.0f;
+        const double A = area[s >> 1];
+        for (uint32_t b = 0; b < nb; ++b) {
+            const size_t i = size_t(s) * ldb + b;
+            const float alpha = fminf(fmaxf(1.0f - r[i] - t[i], 0.0f), 1.0f);
+            a += alpha * H[i];
+            h += H[i];
+            pin += A * (double(D[i]) + double(fsky[s]) * skyv[b]);
+            pabs += A * double(alpha) * H[i];
+            pesc += A * double(fsky[s]) * B[i];
+        }
+        absorbed[s] = a;
+        incident[s] = h;
+    }
+    __shared__ double sh[3][256];
+    sh[0][threadIdx.x] = pin;
+    sh[1][threadIdx.x] = pabs;
+    sh[2][threadIdx.x] = pesc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (int(threadIdx.x) < off)
+            for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) part[size_t(blockIdx.x) * 4 + k] = sh[k][0];
+}
+
+// Fluxes, residual and relaxed Newton update (FP64), one lane per facet.
+// Writes per-block (max dT, max residual-equivalent) and the FP64 partial of
+// sum A Rn over energy-balance sides.
+__global__ __launch_bounds__(256) void k_newton(uint32_t F, EbConst k, const uint32_t* __restrict__ mat,
+                                                const fh_eb_material* __restrict__ mats,
+                                                const float* __restrict__ absorbed, const float* __restrict__ H,
+                                                const float* __restrict__ e, const float* __restrict__ eps,
+                                                float* __restrict__ T, const float* __restrict__ Tprev, int update,
+                                                float* __restrict__ Rn_out, float* __restrict__ H_out,
+                                                float* __restrict__ LE_out, float* __restrict__ G_out,
+                                                const double* __restrict__ area, float2* __restrict__ part,
+                                                double* __restrict__ dpart) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    float myDT = 0.0f, myEq = 0.0f;
+    double rnSum = 0.0;
+    if (f < F) {
+        const fh_eb_material m = mats[mat[f]];
+        const uint32_t s0 = 2u * f;
+        double rn[2];
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t s = s0 + q;
+            rn[q] = double(absorbed[s]) + double(eps[s]) * H[s] - double(e[s]);
+        }
+        if (f < k.leafCount) {
+            const double Tl = T[s0];
+            const double net = rn[0] + rn[1];
+            const double hs = 2.0 * k.ha * m.convectiveScale * (Tl - k.Ta);
+            double rs = m.surfaceResistance;
+            if (k.vegMethod == 1u) {
+                const double par = fmax(0.0, double(absorbed[s0]) + absorbed[s0 + 1]);
+                const double open = 0.25 + 0.75 * par / (par + 180.0);
+                const double tc = (Tl - 273.15 - 25.0) / 18.0;
+                const double resp = exp(-(tc * tc));
+                rs = clampd(m.surfaceResistance / fmax(0.08, open * resp), 40.0, 100000.0);
+            }
+            double le = 0.0, dle = 0.0;
+            if (m.vegetation) {
+                latent(Tl, k, rs, le, dle);
+                le *= 2.0;
+                dle *= 2.0;
+            }
+            const double gd = fmax(0.0, double(m.heatCapacityPerArea)) / k.dTime;
+            const double g = gd * (Tl - double(Tprev[s0]));
+            const double res = net - hs - le - g;
+            const double der = 8.0 * double(eps[s0]) * kSigma * Tl * Tl * Tl + 2.0 * k.ha * m.convectiveScale + dle + gd;
+            myEq = float(fabs(res) / fmax(1.0, der));
+            rnSum += area[f] * net;
+            for (int q = 0; q < 2; ++q) {
+                Rn_out[s0 + q] = float(net);
+                H_out[s0 + q] = float(hs);
+                LE_out[s0 + q] = float(le);
+                G_out[s0 + q] = float(g);
+            }
+            if (update) {
+                const double cand = clampd(Tl + res / fmax(1.0, der), 220.0, 360.0);
+                const double upd = Tl + k.relax * (cand - Tl);
+                myDT = float(fabs(upd - Tl));
+                T[s0] = float(upd);
+                T[s0 + 1] = float(upd);
+            }
+        } else {
+            const bool dry = m.soilMoisture < 0.03f;
+            for (int q = 0; q < 2; ++q) {
+                const uint32_t s = s0 + q;
+                const double Ts = T[s];
+                const double hs = k.ha * (Ts - k.Ta);
+                double le = 0.0, dle = 0.0;
+                if (!dry) latent(Ts, k, m.surfaceResistance, le, dle);
+                double g, gd;
+                if (m.soilTemperatureMethod == 0) {
+                    gd = 0.0;
+                    g = 0.35 * rn[q];
+                } else {
+                    gd = soil_storage(m, k.dTime) * (m.soilTemperatureMethod == 2 ? 1.5 : 1.0);
+                    g = gd * (Ts - double(Tprev[s]));
+                }
+                const double res = rn[q] - hs - le - g;
+                const double der = 4.0 * double(eps[s]) * kSigma * Ts * Ts * Ts + k.ha + dle + gd;
+                myEq = fmaxf(myEq, float(fabs(res) / fmax(1.0, der)));
+                rnSum += area[f] * rn[q];
+                Rn_out[s] = float(rn[q]);
+                H_out[s] = float(hs);
+                LE_out[s] = float(le);
+                G_out[s] = float(g);
+                if (update) {
+                    const double cand = clampd(Ts + res / fmax(1.0, der), 220.0, 360.0);
+                    const double upd = Ts + k.relax * (cand - Ts);
+                    myDT = fmaxf(myDT, float(fabs(upd - Ts)));
+                    T[s] = float(upd);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int msk = 32; msk > 0; msk >>= 1) {
+        myDT = fmaxf(myDT, __shfl_xor(myDT, msk, 64));
+        myEq = fmaxf(myEq, __shfl_xor(myEq, msk, 64));
+        rnSum += __shfl_xor(rnSum, msk, 64);
+    }
+    __shared__ float sdt[4], seq[4];
+    __shared__ double srn[4];
+    if ((threadIdx.x & 63) == 0) {
+        sdt[threadIdx.x >> 6] = myDT;
+        seq[threadIdx.x >> 6] = myEq;
+        srn[threadIdx.x >> 6] = rnSum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = make_float2(fmaxf(fmaxf(sdt[0], sdt[1]), fmaxf(sdt[2], sdt[3])),
+                                       fmaxf(fmaxf(seq[0], seq[1]), fmaxf(seq[2], seq[3])));
+        dpart[size_t(blockIdx.x) * 4 + 3] = (srn[0] + srn[1]) + (srn[2] + srn[3]);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_reduce_max2(const float2* __restrict__ part, uint32_t n,
+                                                      float2* __restrict__ out) {
+    float a = 0.0f, b = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
+        a = fmaxf(a, part[i].x);
+        b = fmaxf(b, part[i].y);
+    }
+    __shared__ float sa[1024], sb[1024];
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (int(threadIdx.x) < off) {
+            sa[threadIdx.x] = fmaxf(sa[threadIdx.x], sa[threadIdx.x + off]);
+            sb[threadIdx.x] = fmaxf(sb[threadIdx.x], sb[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = make_float2(sa[0], sb[0]);
+}
+
+// sums column k (k < 4) of n rows of 4 doubles, fixed order (deterministic)
+__global__ __launch_bounds__(256) void k_reduce_sum4(const double* __restrict__ part, uint32_t n, uint32_t k0,
+                                                     uint32_t k1, double* __restrict__ out) {
+    __shared__ double sh[256];
+    for (uint32_t k = k0; k < k1; ++k) {
+        double acc = 0.0;
+        for (uint32_t i = threadIdx.x; i < n; i += 256u) acc += part[size_t(i) * 4 + k];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (int(threadIdx.x) < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = sh[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_copy_f32(const float* __restrict__ a, float* __restrict__ b, size_t n) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+static inline dim3 blocks_for(size_t n, uint32_t bs = 256) { return dim3(uint32_t((n + bs - 1) / bs)); }
+
+static void copy_f32(fh_context* c, const float* a, float* b, size_t n) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_copy_f32, blocks_for(n), dim3(256), 0, c->stream, a, b, n);
+    FH_HIP(hipGetLastError());
+}
+
+// surrogate warm start and feature capture (mlp.hip)
+void surrogate_guess(fh_context* c, const EbConst& k, float windSpeed, float skyLongwave, float Ta);
+void eb_features(fh_context* c, float Ta, float wind, float sky);
+
+void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMaterial, uint32_t nMat,
+                  const fh_eb_material* mats, uint32_t swNb, const float* refl, const float* trans) {
+    EbState& e = c->eb;
+    const uint32_t F = c->F, S = c->S;
+    e.ready = false;
+    e.leafCount = leafCount;
+    e.nMat = nMat;
+    e.swNb = swNb;
+    e.swLdb = band_stride(swNb);
+    e.material.ensure(F);
+    e.mats.ensure(nMat);
+    e.swRefl.ensure(size_t(nMat) * swNb);
+    e.swTrans.ensure(size_t(nMat) * swNb);
+    e.area.ensure(F);
+    FH_HIP(hipMemcpyAsync(e.material.p, facetMaterial, sizeof(uint32_t) * F, hipMemcpyHostToDevice, c->stream));
+    FH_HIP(hipMemcpyAsync(e.mats.p, mats, sizeof(fh_eb_material) * nMat, hipMemcpyHostToDevice, c->stream));
+    if (swNb) {
+        FH_HIP(hipMemcpyAsync(e.swRefl.p, refl, sizeof(float) * nMat * swNb, hipMemcpyHostToDevice, c->stream));
+        FH_HIP(hipMemcpyAsync(e.swTrans.p, trans, sizeof(float) * nMat * swNb, hipMemcpyHostToDevice, c->stream));
+    }
+    for (DBuf<float>* b : {&e.T, &e.Tprev, &e.absorbed, &e.incident, &e.sunlit, &e.rlw, &e.tlw, &e.eps, &e.elw,
+                           &e.clw, &e.Hlw, &e.zero, &e.Blw0, &e.Blw1, &e.Rn, &e.Hs, &e.LE, &e.G})
+        b->ensure(S);
+    const size_t nsw = size_t(S) * (e.swLdb ? e.swLdb : 1u);
+    for (DBuf<float>* b : {&e.Bsw0, &e.Bsw1, &e.rsw, &e.tsw, &e.Dsw, &e.csw, &e.Hsw}) b->ensure(nsw);
+    e.dni.ensure(e.swLdb ? e.swLdb : 1u);
+    e.lit.ensure(S);
+    e.dsum.ensure(4);
+    e.fout.ensure(1);
+    hipLaunchKernelGGL(k_areas, blocks_for(F), dim3(256), 0, c->stream, c->facetVerts.p, F, e.area.p);
+    hipLaunchKernelGGL(k_lw_setup, blocks_for(S), dim3(256), 0, c->stream, S, e.material.p, e.mats.p, e.rlw.p,
+                       e.tlw.p, e.eps.p);
+    launch_fill_f32(c, e.zero.p, S, 0.0f);
+    launch_fill_f32(c, e.absorbed.p, S, 0.0f);
+    launch_fill_f32(c, e.incident.p, S, 0.0f);
+    launch_fill_f32(c, e.Bsw0.p, nsw, 0.0f);
+    launch_fill_f32(c, e.Blw0.p, S, 0.0f);
+    for (DBuf<float>* b : {&e.Rn, &e.Hs, &e.LE, &e.G, &e.sunlit}) launch_fill_f32(c, b->p, S, 0.0f);
+    FH_HIP(hipGetLastError());
+    FH_HIP(hipStreamSynchronize(c->stream));
+    e.swValid = e.lwValid = e.tValid = false;
+    e.ready = true;
+}
+
+static void newton(fh_context* c, const EbConst& k, int update, float& maxDT, float& maxEq, double* rnSum) {
+    EbState& e = c->eb;
+    const uint32_t F = c->F;
+    const uint32_t nbk = (F + 255u) / 256u;
+    c->partial.ensure(nbk);
+    e.dpart.ensure(size_t(nbk) * 4u);
+    hipLaunchKernelGGL(k_newton, dim3(nbk), dim3(256), 0, c->stream, F, k, e.material.p, e.mats.p, e.absorbed.p,
+                       e.Hlw.p, e.elw.p, e.eps.p, e.T.p, e.Tprev.p, update, e.Rn.p, e.Hs.p, e.LE.p, e.G.p, e.area.p,
+                       c->partial.p, e.dpart.p);
+    hipLaunchKernelGGL(k_reduce_max2, dim3(1), dim3(1024), 0, c->stream, c->partial.p, nbk, e.fout.p);
+    if (rnSum) hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(256), 0, c->stream, e.dpart.p, nbk, 3u, 4u, e.dsum.p);
+    FH_HIP(hipGetLastError());
+    float2 h;
+    FH_HIP(hipMemcpyAsync(&h, e.fout.p, sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+    double d4[4] = {0, 0, 0, 0};
+    if (rnSum) FH_HIP(hipMemcpyAsync(d4, e.dsum.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipStreamSynchronize(c->stream));
+    maxDT = h.x;
+    maxEq = h.y;
+    if (rnSum) *rnSum = d4[3];
+}
+
+// LW: source from T, warm or zero start, solve, incident (into e.Hlw)
+static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p) {
+    EbState& e = c->eb;
+    const uint32_t S = c->S;
+    hipLaunchKernelGGL(k_lw_source, blocks_for(S), dim3(256), 0, c->stream, S, e.T.p, e.eps.p, e.rlw.p, e.tlw.p,
+                       c->fsky.p, p.skyLongwave, e.elw.p, e.clw.p);
+    FH_HIP(hipGetLastError());
+    if (!e.lwValid || p.zeroStart) launch_fill_f32(c, e.Blw0.p, S, 0.0f);
+    const float L = p.skyLongwave;
+    set_sky(c, &L, 1u, 1u);
+    JacobiOptions o;
+    o.ldb = 1u;
+    o.precision = p.precision;
+    o.maxIter = p.maxSweeps;
+    o.tolRel = p.radiosityTolerance;
+    float* bufs[2] = {e.Blw0.p, e.Blw1.p};
+    fh_solve_stats st{};
+    const int fin = run_jacobi(c, o, e.clw.p, e.rlw.p, e.tlw.p, bufs, &st);
+    if (fin == 1) e.Blw0.swap(e.Blw1);
+    e.lwValid = true;
+    run_incident(c, 1u, e.Blw0.p, e.zero.p, e.Hlw.p);
+    return st.iterations;
+}
+
+void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost, const float* diffHost,
+                 fh_eb_node_stats* out) {
+    EbState& e = c->eb;
+    const uint32_t S = c->S;
+    fh_eb_node_stats st{};
+    // --- shortwave ---------------------------------------------------------
+    const float sunLen = std::sqrt(p.sun.x * p.sun.x + p.sun.y * p.sun.y + p.sun.z * p.sun.z);
+    double swTotal = 0.0;
+    for (uint32_t b = 0; b < e.swNb; ++b) swTotal += double(dniHost[b]) + double(diffHost[b]);
+    if (p.sun.y > 0.0f && sunLen > 0.0f) {
+        launch_fill_u32(c, e.lit.p, S, 0u);
+        launch_sun(c, p.sun.x, p.sun.y, p.sun.z, e.lit.p, e.sunlit.p);
+    } else {
+        launch_fill_f32(c, e.sunlit.p, S, 0.0f);
+    }
+    if (e.swNb && swTotal > 0.0) {
+        const uint32_t nb = e.swNb, ldb = e.swLdb;
+        std::vector<float> dpad(ldb, 0.0f);
+        for (uint32_t b = 0; b < nb; ++b) dpad[b] = p.sun.y > 0.0f ? dniHost[b] : 0.0f;
+        FH_HIP(hipMemcpyAsync(e.dni.p, dpad.data(), sizeof(float) * ldb, hipMemcpyHostToDevice, c->stream));
+        const size_t n = size_t(S) * ldb;
+        hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p, e.swRefl.p,
+                           e.swTrans.p, c->facetNormal.p, e.sunlit.p, p.sun.x, p.sun.y, p.sun.z, e.dni.p, e.rsw.p,
+                           e.tsw.p, e.Dsw.p);
+        FH_HIP(hipGetLastError());
+        set_sky(c, diffHost, nb, ldb);
+        launch_fill_f32(c, e.Hsw.p, n, 0.0f);  // emission is zero in the SW
+        launch_fold(c, e.Hsw.p, e.rsw.p, e.tsw.p, e.Dsw.p, ldb, e.csw.p);
+        if (!e.swValid || p.zeroStart) launch_fill_f32(c, e.Bsw0.p, n, 0.0f);
+        JacobiOptions o;
+        o.ldb = ldb;
+        o.precision = p.precision;
+        o.maxIter = p.maxSweeps;
+        o.tolRel = p.radiosityTolerance;
+        float* bufs[2] = {e.Bsw0.p, e.Bsw1.p};
+        fh_solve_stats ss{};
+        const int fin = run_jacobi(c, o, e.csw.p, e.rsw.p, e.tsw.p, bufs, &ss);
+        if (fin == 1) e.Bsw0.swap(e.Bsw1);
+        st.swSweeps = ss.iterations;
+        st.swRefreshes = ss.refreshes;
+        run_incident(c, ldb, e.Bsw0.p, e.Dsw.p, e.Hsw.p);
+        const uint32_t nbk = (S + 255u) / 256u;
+        e.dpart.ensure(size_t(nbk) * 4u);
+        hipLaunchKernelGGL(k_absorb, dim3(nbk), dim3(256), 0, c->stream, S, nb, ldb, e.rsw.p, e.tsw.p, e.Hsw.p,
+                           e.Dsw.p, e.Bsw0.p, c->fsky.p, c->skyv.p, e.area.p, e.absorbed.p, e.incident.p, e.dpart.p);
+        hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(256), 0, c->stream, e.dpart.p, nbk, 0u, 3u, e.dsum.p);
+        FH_HIP(hipGetLastError());
+        double d4[4];
+        FH_HIP(hipMemcpyAsync(d4, e.dsum.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
+        FH_HIP(hipStreamSynchronize(c->stream));
+        st.swInput = d4[0];
+        st.swAbsorbed = d4[1];
+        st.swEscaped = d4[2];
+        e.swValid = true;
+    } else {
+        launch_fill_f32(c, e.absorbed.p, S, 0.0f);
+        launch_fill_f32(c, e.incident.p, S, 0.0f);
+        if (e.swNb) launch_fill_f32(c, e.Bsw0.p, size_t(S) * e.swLdb, 0.0f);
+        e.swValid = false;
+    }
+    // --- temperature state ----------------------------------------------------
+    EbConst k{};
+    k.Ta = p.airTemperatureK;
+    k.ea = p.vaporPressure;
+    k.P = std::max(100.0, double(p.pressure));
+    k.ha = 5.8 + 4.1 * std::sqrt(std::max(0.1, double(p.windSpeed)));
+    k.rho = k.P * 100.0 / (287.05 * k.Ta);
+    k.ra = k.rho * 1005.0 / std::max(0.1, k.ha);
+    k.dTime = std::max(1.0, double(p.dTime));
+    k.relax = p.temperatureRelaxation;
+    k.vegMethod = p.vegetationMethod;
+    k.leafCount = e.leafCount;
+    if (p.firstNode || !e.tValid) launch_fill_f32(c, e.T.p, S, p.airTemperatureK);
+    copy_f32(c, e.T.p, e.Tprev.p, S);
+    e.tValid = true;
+    if (p.warmStart == FH_WARM_SURROGATE)
+        surrogate_guess(c, k, p.windSpeed, p.skyLongwave, p.airTemperatureK);
+    else if (p.recordFeatures)
+        eb_features(c, p.airTemperatureK, p.windSpeed, p.skyLongwave);
+    // --- coupling loop -------------------------------------------------------
+    float maxDT = 0.0f, maxEq = 0.0f;
+    uint32_t iters = 0;
+    for (uint32_t it = 0; it < std::max(1u, p.maxCoupling); ++it) {
+        st.lwSweeps += lw_solve(c, p);
+        newton(c, k, 1, maxDT, maxEq, nullptr);
+        iters = it + 1u;
+        if (maxDT <= p.temperatureTolerance && maxEq <= p.temperatureTolerance) break;
+    }
+    st.lwSweeps += lw_solve(c, p);
+    float dtFinal = 0.0f;
+    double rn = 0.0;
+    newton(c, k, 0, dtFinal, maxEq, &rn);
+    st.couplingIterations = iters;
+    st.maxTemperatureDelta = maxDT;
+    st.maxResidualEquivalent = maxEq;
+    st.converged = (maxDT <= p.temperatureTolerance && maxEq <= p.temperatureTolerance) ? 1u : 0u;
+    st.netRadiation = rn;
+    if (out) *out = st;
+}
+
+}  // namespace fh
+
+using fh::fail;
+
+namespace {
+template <class Fn>
+int guarded_eb(fh_context* c, Fn&& body) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    try {
+        FH_HIP(hipSetDevice(c->device));
+        body();
+        c->err.clear();
+        return FH_OK;
+    } catch (const fh::Error& e) {
+        c->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return FH_RUNTIME_ERROR;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int fh_eb_set_scene(fh_context* c, uint32_t leafFacetCount, const uint32_t* facetMaterial, uint32_t materialCount,
+                    const fh_eb_material* materials, uint32_t swBands, const float* swReflectance,
+                    const float* swTransmittance) {
+    return guarded_eb(c, [&] {
+        if (!c->hasGraph) fail(FH_LOGIC_ERROR, "buildVisibilityGraph() must be called before the energy balance");
+        if (leafFacetCount > c->F) fail(FH_INVALID_ARGUMENT, "leafFacetCount exceeds the facet count");
+        if (!facetMaterial || !materials || materialCount == 0u) fail(FH_INVALID_ARGUMENT, "materials must be given");
+        if (swBands > 4096u || (swBands && (!swReflectance || !swTransmittance)))
+            fail(FH_INVALID_ARGUMENT, "shortwave band tables must be given");
+        for (uint32_t f = 0; f < c->F; ++f)
+            if (facetMaterial[f] >= materialCount) fail(FH_INVALID_ARGUMENT, "facet material index out of range");
+        for (uint32_t m = 0; m < materialCount; ++m) {
+            const fh_eb_material& x = materials[m];
+            if (!std::isfinite(x.tirReflectance) || !std::isfinite(x.tirTransmittance) ||
+                !std::isfinite(x.surfaceResistance) || !std::isfinite(x.heatCapacityPerArea) ||
+                !std::isfinite(x.convectiveScale) || x.soilTemperatureMethod < 0 || x.soilTemperatureMethod > 2)
+                fail(FH_INVALID_ARGUMENT, "invalid energy-balance material");
+        }
+        for (size_t i = 0; i < size_t(materialCount) * swBands; ++i)
+            if (!std::isfinite(swReflectance[i]) || !std::isfinite(swTransmittance[i]))
+                fail(FH_INVALID_ARGUMENT, "shortwave optics must be finite");
+        fh::eb_set_scene(c, leafFacetCount, facetMaterial, materialCount, materials, swBands, swReflectance,
+                         swTransmittance);
+    });
+}
+
+int fh_eb_run_node(fh_context* c, const fh_eb_node_params* p, const float* swDirectNormal, const float* swDiffuse,
+                   fh_eb_node_stats* stats) {
+    return guarded_eb(c, [&] {
+        if (!c->eb.ready) fail(FH_LOGIC_ERROR, "fh_eb_set_scene() must be called before fh_eb_run_node()");
+        if (!p) fail(FH_INVALID_ARGUMENT, "params must not be null");
+        if (c->eb.swNb && (!swDirectNormal || !swDiffuse)) fail(FH_INVALID_ARGUMENT, "shortwave inputs must be given");
+        for (uint32_t b = 0; b < c->eb.swNb; ++b)
+            if (!std::isfinite(swDirectNormal[b]) || !std::isfinite(swDiffuse[b]) || swDirectNormal[b] < 0.0f ||
+                swDiffuse[b] < 0.0f)
+                fail(FH_INVALID_ARGUMENT, "shortwave inputs must be finite and non-negative");
+        if (!std::isfinite(p->airTemperatureK) || p->airTemperatureK < 150.0f || p->airTemperatureK > 400.0f)
+            fail(FH_INVALID_ARGUMENT, "air temperature out of range");
+        if (!std::isfinite(p->skyLongwave) || p->skyLongwave < 0.0f) fail(FH_INVALID_ARGUMENT, "sky longwave invalid");
+        if (!(p->temperatureRelaxation > 0.0f && p->temperatureRelaxation <= 1.0f))
+            fail(FH_INVALID_ARGUMENT, "temperature relaxation must be in (0,1]");
+        if (!(p->temperatureTolerance > 0.0f)) fail(FH_INVALID_ARGUMENT, "temperature tolerance must be positive");
+        if (p->precision > FH_PRECISION_BF16_DC || p->warmStart > FH_WARM_SURROGATE)
+            fail(FH_INVALID_ARGUMENT, "unknown precision or warm-start mode");
+        fh::eb_run_node(c, *p, swDirectNormal, swDiffuse, stats);
+    });
+}
+
+int fh_eb_get_state(fh_context* c, float* T, float* absorbed, float* Blw, float* sunlit, float* Rn, float* Hs,
+                    float* LE, float* G, float* swRadiosity) {
+    return guarded_eb(c, [&] {
+        fh::EbState& e = c->eb;
+        if (!e.ready) fail(FH_LOGIC_ERROR, "fh_eb_set_scene() must be called before fh_eb_get_state()");
+        const size_t S = c->S;
+        auto get = [&](float* dst, const float* src, size_t n) {
+            if (dst) FH_HIP(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        };
+        get(T, e.T.p, S);
+        get(absorbed, e.absorbed.p, S);
+        get(Blw, e.Blw0.p, S);
+        get(sunlit, e.sunlit.p, S);
+        get(Rn, e.Rn.p, S);
+        get(Hs, e.Hs.p, S);
+        get(LE, e.LE.p, S);
+        get(G, e.G.p, S);
+        if (swRadiosity && e.swNb) {
+            fh::DBuf<float> tmp;
+            tmp.ensure(S * e.swNb);
+            fh::launch_unpad(c, e.Bsw0.p, e.swNb, e.swLdb, tmp.p);
+            get(swRadiosity, tmp.p, S * e.swNb);
+            FH_HIP(hipStreamSynchronize(c->stream));
+        }
+        FH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int fh_eb_set_temperature(fh_context* c, const float* T) {
+    return guarded_eb(c, [&] {
+        if (!c->eb.ready) fail(FH_LOGIC_ERROR, "fh_eb_set_scene() must be called before fh_eb_set_temperature()");
+        if (!T) fail(FH_INVALID_ARGUMENT, "temperature must not be null");
+        for (uint32_t s = 0; s < c->S; ++s)
+            if (!std::isfinite(T[s]) || T[s] < 1.0f || T[s] > 1000.0f) fail(FH_INVALID_ARGUMENT, "temperature out of range");
+        FH_HIP(hipMemcpyAsync(c->eb.T.p, T, sizeof(float) * c->S, hipMemcpyHostToDevice, c->stream));
+        FH_HIP(hipStreamSynchronize(c->stream));
+        c->eb.tValid = true;
+    });
+}
+
+}  // extern "C"

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/facet_hip.h"
@@ -52,6 +53,11 @@ struct DBuf {
         p = nullptr;
         n = cap = 0;
     }
+    void swap(DBuf& o) noexcept {
+        std::swap(p, o.p);
+        std::swap(n, o.n);
+        std::swap(cap, o.cap);
+    }
     size_t bytes() const { return n * sizeof(T); }
 };
 
@@ -86,16 +92,57 @@ struct SolveCtrl {
     uint32_t checkEvery;  // absolute tolerance checked when iter % checkEvery == 0
     float tolRel;         // < 0: off
     float tolAbs;         // < 0: off
-    uint32_t absBits;     // running max |dB| (float bits) of the sweep in flight
-    uint32_t relBits;     // running max |dB|/|B|
-    uint32_t arrived;     // blocks finished in the sweep in flight
     float lastAbs;
     float lastRel;
     uint32_t refreshes;
-    uint32_t pad[4];
+    uint32_t pad[7];
 };
 
 uint32_t band_stride(uint32_t nb);
+
+// FacetEB device state (one per context).
+struct EbState {
+    bool ready = false;
+    uint32_t leafCount = 0;
+    uint32_t nMat = 0;
+    uint32_t swNb = 0, swLdb = 0;
+    bool swValid = false;  // Bsw holds the previous node's SW solution
+    bool lwValid = false;  // Blw holds a LW solution
+    bool tValid = false;   // T holds a state
+    DBuf<uint32_t> material;       // F
+    DBuf<fh_eb_material> mats;     // nMat
+    DBuf<float> swRefl, swTrans;   // nMat x swNb
+    DBuf<double> area;             // F
+    // per side
+    DBuf<float> T, Tprev, absorbed, incident, sunlit, rlw, tlw, eps, elw, clw, Hlw, zero;
+    DBuf<float> Blw0, Blw1;
+    DBuf<float> Rn, Hs, LE, G;
+    // SW [S][swLdb]
+    DBuf<float> Bsw0, Bsw1, rsw, tsw, Dsw, csw, Hsw, dni;
+    DBuf<double> dpart;            // per-block FP64 partial sums
+    DBuf<double> dsum;             // reduced FP64 sums
+    DBuf<float2> fout;             // reduced (max dT, max residual-equivalent)
+    DBuf<uint32_t> lit;
+    // surrogate inputs / outputs per energy-balance unit (leaf facet or
+    // ground side): unit u < leafCount is facet u, else ground side
+    // 2 leafCount + (u - leafCount)
+    DBuf<float> gview;             // per side: view fraction of ground sides
+    DBuf<float> feat;              // units x 9 raw features
+    DBuf<float> featN;             // units x 9 normalised
+    DBuf<float> guess;             // units
+    bool featValid = false;
+};
+
+// Surrogate MLP (9 -> H -> H -> 1, GELU) parameters and normalisation.
+struct MlpState {
+    uint32_t hidden = 0;
+    bool ready = false;
+    DBuf<float> params;            // W1[H][9], b1[H], W2[H][H], b2[H], W3[H], b3
+    float inMean[9] = {}, inStd[9] = {};
+    float outMean = 0.0f, outStd = 1.0f;
+    // training workspace
+    DBuf<float> A1, H1, A2, H2, y, dA2, dA1, grad, m, v, red;
+};
 
 }  // namespace fh
 
@@ -131,7 +178,11 @@ struct fh_context {
     // solve workspace
     fh::DBuf<float> B0, B1, cc, rr, tt, DD, c2, HH, skyv;
     fh::DBuf<fh::SolveCtrl> ctrl;
+    fh::DBuf<float2> partial;        // per-block residual partials
     fh::SolveCtrl* hctrl = nullptr;  // pinned mirror
     fh::DBuf<uint32_t> misc;         // small counters
     fh::DBuf<float> fbuf;            // small float scratch
+
+    fh::EbState eb;
+    fh::MlpState mlp;
 };

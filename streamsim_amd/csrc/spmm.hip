@@ -27,6 +27,8 @@
 // iteration count and raises `done`, after which queued sweeps exit at once.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "solver.hpp"
 
 namespace fh {
@@ -56,6 +58,7 @@ struct SweepArgs {
     float relax;
     uint32_t judge;  // 1: this sweep may end the solve on tolRel
     SolveCtrl* ctrl;
+    float2* partial;  // per-block (max |dB|, max |dB|/|B|)
 };
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
@@ -164,14 +167,14 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
         }
     }
     if (MODE == MODE_INCIDENT) return;
-    // block max-reduction of the residuals
+    // block max-reduction of the residuals into this block's partial slot;
+    // k_finalize reduces the partials (no atomics or fences on the hot path)
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) {
         myAbs = fmaxf(myAbs, __shfl_xor(myAbs, m, 64));
         myRel = fmaxf(myRel, __shfl_xor(myRel, m, 64));
     }
     __shared__ float sAbs[4], sRel[4];
-    __shared__ bool last;
     const int wave = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         sAbs[wave] = myAbs;
@@ -181,28 +184,47 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     if (threadIdx.x == 0) {
         const float bA = fmaxf(fmaxf(sAbs[0], sAbs[1]), fmaxf(sAbs[2], sAbs[3]));
         const float bR = fmaxf(fmaxf(sRel[0], sRel[1]), fmaxf(sRel[2], sRel[3]));
-        atomicMax(&ctrl->absBits, __float_as_uint(bA));
-        atomicMax(&ctrl->relBits, __float_as_uint(bR));
-        __threadfence();
-        const uint32_t prev = atomicAdd(&ctrl->arrived, 1u);
-        last = prev == gridDim.x - 1;
+        a.partial[blockIdx.x] = make_float2(bA, bR);
+    }
+}
+
+// Reduces the per-block residuals of the sweep just run and advances the
+// control block: iteration count, last residuals, done flag.
+__global__ __launch_bounds__(1024) void k_finalize(SolveCtrl* ctrl, const float2* __restrict__ partial, uint32_t n,
+                                                   uint32_t judge, uint32_t refresh) {
+    if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
+    float mA = 0.0f, mR = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
+        const float2 p = partial[i];
+        mA = fmaxf(mA, p.x);
+        mR = fmaxf(mR, p.y);
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        mA = fmaxf(mA, __shfl_xor(mA, m, 64));
+        mR = fmaxf(mR, __shfl_xor(mR, m, 64));
+    }
+    __shared__ float sA[16], sR[16];
+    if ((threadIdx.x & 63) == 0) {
+        sA[threadIdx.x >> 6] = mA;
+        sR[threadIdx.x >> 6] = mR;
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        const float absd = __uint_as_float(atomicExch(&ctrl->absBits, 0u));
-        const float reld = __uint_as_float(atomicExch(&ctrl->relBits, 0u));
-        atomicExch(&ctrl->arrived, 0u);
+    if (threadIdx.x == 0) {
+        float absd = 0.0f, reld = 0.0f;
+        for (int w = 0; w < 16; ++w) {
+            absd = fmaxf(absd, sA[w]);
+            reld = fmaxf(reld, sR[w]);
+        }
         const uint32_t it = ctrl->iter + 1u;
         ctrl->iter = it;
         ctrl->lastAbs = absd;
         ctrl->lastRel = reld;
-        if (MODE == MODE_REFRESH) ctrl->refreshes += 1u;
+        if (refresh) ctrl->refreshes += 1u;
         bool stop = it >= ctrl->maxIter;
-        if (a.judge && ctrl->tolRel >= 0.0f && reld <= ctrl->tolRel) stop = true;
+        if (judge && ctrl->tolRel >= 0.0f && reld <= ctrl->tolRel) stop = true;
         if (ctrl->tolAbs >= 0.0f && (it % ctrl->checkEvery) == 0u && absd <= ctrl->tolAbs) stop = true;
         if (stop) ctrl->done = 1u;
-        __threadfence();
     }
 }
 
@@ -260,10 +282,17 @@ static int pick_nbc(uint32_t ldb) {
 }
 
 static int pick_team(const fh_context* c) {
+    // FH_SPMM_TEAM=4|16|64 overrides the choice (tuning knob)
+    static const int forced = [] {
+        const char* e = std::getenv("FH_SPMM_TEAM");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (forced == 4 || forced == 8 || forced == 16 || forced == 32 || forced == 64) return forced;
     const double avg = c->S ? double(c->nnz) / double(c->S) : 0.0;
     if (avg < 12.0) return 4;
-    if (avg < 48.0) return 16;
-    return 64;
+    if (avg < 24.0) return 8;
+    if (avg < 96.0) return 16;
+    return 32;
 }
 
 template <int NBC, int TEAM>
@@ -282,9 +311,12 @@ template <int NBC>
 static void launch_team(int team, int mode, bool w16, const SweepArgs& a, hipStream_t st) {
     const uint64_t threads = uint64_t(a.F) * team;
     const uint32_t grid = uint32_t((threads + 255u) / 256u);
-    if (team == 4) launch_mode<NBC, 4>(mode, w16, a, grid ? grid : 1u, st);
-    else if (team == 16) launch_mode<NBC, 16>(mode, w16, a, grid ? grid : 1u, st);
-    else launch_mode<NBC, 64>(mode, w16, a, grid ? grid : 1u, st);
+    const uint32_t g = grid ? grid : 1u;
+    if (team == 4) launch_mode<NBC, 4>(mode, w16, a, g, st);
+    else if (team == 8) launch_mode<NBC, 8>(mode, w16, a, g, st);
+    else if (team == 16) launch_mode<NBC, 16>(mode, w16, a, g, st);
+    else if (team == 32) launch_mode<NBC, 32>(mode, w16, a, g, st);
+    else launch_mode<NBC, 64>(mode, w16, a, g, st);
 }
 
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
@@ -298,12 +330,19 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.w16 = c->val16.p;
     a.fsky = c->fsky.p;
     a.ctrl = c->ctrl.p;
+    const uint64_t threads = uint64_t(c->F) * uint64_t(team);
+    const uint32_t grid = uint32_t((threads + 255u) / 256u);
+    c->partial.ensure(grid ? grid : 1u);
+    a.partial = c->partial.p;
     switch (nbc) {
         case 1: launch_team<1>(team, mode, w16, a, c->stream); break;
         case 4: launch_team<4>(team, mode, w16, a, c->stream); break;
         case 8: launch_team<8>(team, mode, w16, a, c->stream); break;
         default: launch_team<16>(team, mode, w16, a, c->stream); break;
     }
+    if (mode != MODE_INCIDENT)
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, grid ? grid : 1u,
+                           a.judge, uint32_t(mode == MODE_REFRESH));
     FH_HIP(hipGetLastError());
 }
 

@@ -14,7 +14,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib
-from ._lib import FhConfig, FhDirection, FhGraphDiagnostics, FhSolveStats, raise_for
+from ._lib import (FhConfig, FhDirection, FhEbMaterial, FhEbNodeParams, FhEbNodeStats, FhGraphDiagnostics,
+                   FhSolveStats, raise_for)
 
 
 @dataclass
@@ -226,3 +227,97 @@ class FacetEngine:
         out = np.zeros((S, nb), np.float32)
         self._check(self._lib.fh_incident_bands(self._h, nb, _ptr(B), _ptr(D), _ptr(sk), _ptr(out)))
         return out
+
+    # -- FacetEB ---------------------------------------------------------------
+    def ebSetScene(self, leaf_count: int, facet_material, materials, sw_reflectance=None, sw_transmittance=None):
+        """materials: sequence of dicts with the FhEbMaterial fields; sw tables
+        [materialCount][swBands] (window-centre optics)."""
+        mat = np.ascontiguousarray(facet_material, dtype=np.uint32)
+        arr = (FhEbMaterial * len(materials))()
+        for i, m in enumerate(materials):
+            for k, _ in FhEbMaterial._fields_:
+                if k in m:
+                    setattr(arr[i], k, m[k])
+        nb = 0
+        r = t = None
+        if sw_reflectance is not None:
+            r = _f32(sw_reflectance)
+            t = _f32(sw_transmittance)
+            nb = r.shape[1]
+        self._check(self._lib.fh_eb_set_scene(self._h, leaf_count, _ptr(mat), len(materials),
+                                              C.cast(arr, C.c_void_p), nb, _ptr(r), _ptr(t)))
+        self._eb_bands = nb
+
+    def ebRunNode(self, params: FhEbNodeParams, sw_direct_normal=None, sw_diffuse=None) -> FhEbNodeStats:
+        d = None if sw_direct_normal is None else _f32(sw_direct_normal)
+        f = None if sw_diffuse is None else _f32(sw_diffuse)
+        st = FhEbNodeStats()
+        self._check(self._lib.fh_eb_run_node(self._h, C.byref(params), _ptr(d), _ptr(f), C.byref(st)))
+        return st
+
+    def ebGetState(self) -> dict:
+        S = self.surfaceCount()
+        names = ("temperature", "absorbedShortwave", "longwaveRadiosity", "sunlit", "netRadiation",
+                 "sensibleHeat", "latentHeat", "storageHeat")
+        out = {k: np.zeros(S, np.float32) for k in names}
+        nb = getattr(self, "_eb_bands", 0)
+        sw = np.zeros((S, max(nb, 1)), np.float32) if nb else None
+        self._check(self._lib.fh_eb_get_state(self._h, *[_ptr(out[k]) for k in names], _ptr(sw)))
+        if sw is not None:
+            out["shortwaveRadiosity"] = sw
+        return out
+
+    def ebSetTemperature(self, T) -> None:
+        T = _f32(T)
+        self._check(self._lib.fh_eb_set_temperature(self._h, _ptr(T)))
+
+    def ebUnitCount(self) -> int:
+        return int(self._lib.fh_eb_unit_count(self._h))
+
+    def ebGetFeatures(self):
+        n = self.ebUnitCount()
+        feat = np.zeros((n, 9), np.float32)
+        T = np.zeros(n, np.float32)
+        self._check(self._lib.fh_eb_get_features(self._h, _ptr(feat), _ptr(T)))
+        return feat, T
+
+    # -- surrogate MLP -----------------------------------------------------------
+    @staticmethod
+    def mlpParamCount(hidden: int) -> int:
+        return int(_lib.load().fh_mlp_param_count(hidden))
+
+    def mlpSetParams(self, hidden: int, params, feature_mean=None, feature_std=None, target_mean: float = 0.0,
+                     target_std: float = 1.0) -> None:
+        p = _f32(params)
+        if p.size != self.mlpParamCount(hidden):
+            raise ValueError("parameter count does not match the hidden width")
+        fm = None if feature_mean is None else _f32(feature_mean)
+        fs = None if feature_std is None else _f32(feature_std)
+        self._check(self._lib.fh_mlp_set_params(self._h, hidden, _ptr(p), _ptr(fm), _ptr(fs), target_mean,
+                                                target_std))
+        self._mlp_hidden = hidden
+
+    def mlpGetParams(self) -> np.ndarray:
+        out = np.zeros(self.mlpParamCount(self._mlp_hidden), np.float32)
+        self._check(self._lib.fh_mlp_get_params(self._h, _ptr(out)))
+        return out
+
+    def mlpForward(self, x) -> np.ndarray:
+        x = _f32(x)
+        y = np.zeros(x.shape[0], np.float32)
+        self._check(self._lib.fh_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def mlpLossGrad(self, x, target):
+        x = _f32(x)
+        t = _f32(target)
+        g = np.zeros(self.mlpParamCount(self._mlp_hidden), np.float32)
+        loss = C.c_float(0.0)
+        self._check(self._lib.fh_mlp_loss_grad(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
+        return float(loss.value), g
+
+    def mlpTrainDevice(self, d_x: int, d_t: int, n: int, batch: int, epochs: int, lr: float, seed: int = 0) -> float:
+        loss = C.c_float(0.0)
+        self._check(self._lib.fh_mlp_train(self._h, C.c_void_p(d_x), C.c_void_p(d_t), n, batch, epochs, lr, seed,
+                                           C.byref(loss)))
+        return float(loss.value)

@@ -276,4 +276,48 @@ def tiny_pair(gap: float = 1.0, size: float = 1.0) -> Scene:
     return _finish("pair", top, bottom)
 
 
+def load_tree_with_soil(path, soil_resolution: int = 48) -> Scene:
+    """OBJ scene as loadTreeWithSoil (core:362-473): faces fan-triangulated,
+    y shifted so the lowest vertex sits at 0, every face an object facet,
+    plus a soil grid of soil_resolution^2 x 2 triangles at y = -0.01 over the
+    object's footprint widened by half its larger horizontal extent."""
+    positions = []
+    faces = []
+    for line in open(path):
+        tok = line.split()
+        if not tok:
+            continue
+        if tok[0] == "v":
+            positions.append([float(x) for x in tok[1:4]])
+        elif tok[0] == "f":
+            idx = []
+            for t in tok[1:]:
+                raw = int(t.split("/")[0])
+                idx.append(raw - 1 if raw > 0 else len(positions) + raw)
+            for i in range(1, len(idx) - 1):
+                faces.append((idx[0], idx[i], idx[i + 1]))
+    P = np.asarray(positions, np.float32)
+    lo = P.min(axis=0)
+    hi = P.max(axis=0)
+    P[:, 1] -= lo[1]
+    tri = P[np.asarray(faces)]
+    x0, x1, z0, z1 = float(lo[0]), float(hi[0]), float(lo[2]), float(hi[2])
+    margin = np.float32(0.5) * max(np.float32(hi[0] - lo[0]), np.float32(hi[2] - lo[2]))
+    x0, x1, z0, z1 = [np.float32(v) for v in (x0 - margin, x1 + margin, z0 - margin, z1 + margin)]
+    n = soil_resolution
+    soil = []
+    for row in range(n):
+        za = z0 + (z1 - z0) * np.float32(row) / np.float32(n)
+        zb = z0 + (z1 - z0) * np.float32(row + 1) / np.float32(n)
+        for col in range(n):
+            xa = x0 + (x1 - x0) * np.float32(col) / np.float32(n)
+            xb = x0 + (x1 - x0) * np.float32(col + 1) / np.float32(n)
+            soil.append([[xa, -0.01, za], [xb, -0.01, zb], [xb, -0.01, za]])
+            soil.append([[xa, -0.01, za], [xa, -0.01, zb], [xb, -0.01, zb]])
+    sc = _finish("obj", tri, np.asarray(soil, np.float32))
+    sc.bmin = np.array([x0, -0.01, z0], np.float32)
+    sc.bmax = np.array([x1, hi[1] - lo[1], z1], np.float32)
+    return sc
+
+
 BUILDERS = {"row_crop": row_crop, "forest": forest, "urban": urban, "terrain": terrain}

@@ -1,0 +1,291 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+Integer / index work (form-factor hit counts, CSR indices, sunlit counts) is
+bit-exact against the oracle's host replay of the same seeded rays.
+Radiosities agree within 1e-5 relative and temperatures within 0.01 K.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from streamsim_amd import faceteb, optics, scenes
+from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5  # radiosity parity bound
+
+
+def _engine(R=64, K=16, seed=7):
+    from streamsim_amd import FacetEngine
+    return FacetEngine(rays_per_side=R, sun_rays_per_side=K, seed=seed)
+
+
+def _as_oracle_graph(oracle, g):
+    return oracle.Graph(g.rowOffsets, g.neighbors, g.counts, g.denominators, g.skyCounts)
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
+
+
+@pytest.mark.parametrize("leaves,cells,R,seed", [(20, 6, 64, 7), (400, 40, 256, 1)])
+def test_form_factor_graph_bitexact(oracle, leaves, cells, R, seed):
+    print(f"graph case leaves={leaves} cells={cells} R={R}", flush=True)
+    sc = scenes.row_crop(leaves_per_row=leaves, cells=cells)
+    eng = _engine(R=R, seed=seed)
+    eng.setGeometry(sc.vertices)
+    d = eng.buildVisibilityGraph()
+    g = eng.getGraph()
+    og = oracle.ray_graph(sc.vertices, seed, R)
+    assert d.directedEdgeCount == og.nnz
+    assert d.maxClosureError == 0.0
+    for k in ("rowOffsets", "neighbors", "counts", "denominators", "skyCounts"):
+        assert np.array_equal(getattr(g, k), getattr(og, k)), k
+
+
+def test_form_factor_graph_forest_crown(oracle):
+    # one forest crown over a ground patch: full-sphere leaf normals
+    print("graph case crown", flush=True)
+    rng = np.random.default_rng(5)
+    crown = scenes.crown_leaves((2.0, 4.0, 2.0), (2.0, 3.0, 2.0), 400, 0.1, rng)
+    sc = scenes._finish("crown", crown, scenes.grid_ground(0.0, 4.0, 0.0, 4.0, 10, 10))
+    eng = _engine(R=96, seed=3)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    g = eng.getGraph()
+    og = oracle.ray_graph(sc.vertices, 3, 96)
+    assert np.array_equal(g.rowOffsets, og.rowOffsets)
+    assert np.array_equal(g.neighbors, og.neighbors)
+    assert np.array_equal(g.counts, og.counts)
+    assert np.array_equal(g.skyCounts, og.skyCounts)
+
+
+def test_sunlit_bitexact(oracle):
+    print("sunlit case", flush=True)
+    sc = scenes.row_crop(leaves_per_row=60, cells=10)
+    eng = _engine(R=32, K=32, seed=9)
+    eng.setGeometry(sc.vertices)
+    for zen, az in ((30.0, 135.0), (60.0, 250.0), (0.0, 0.0)):
+        sun = oracle.sun_direction(zen, az)
+        f = eng.computeSunlitFraction(sun)
+        of = oracle.ray_sunlit(sc.vertices, sun, 9, 32)
+        assert np.array_equal(f, of), (zen, az)
+
+
+def test_reference_solve_on_raster_graph(oracle):
+    # the reference's own graph (A-buffer restatement) uploaded through
+    # fh_set_graph; fixed 64 sweeps from zero vs solveCpu (FP64 gather)
+    print("reference solve case", flush=True)
+    sc = scenes.row_crop()
+    g = oracle.raster_graph(sc.vertices, sc.bmin, sc.bmax, raster=512, layers=64)
+    eng = _engine()
+    eng.setGeometry(sc.vertices)
+    d = eng.setGraph(g.rowOffsets, g.neighbors, g.counts, g.denominators, g.skyCounts)
+    assert d.directedEdgeCount == g.nnz and d.maxClosureError == 0.0
+    sun = oracle.sun_direction(30.0, 135.0)
+    sl = oracle.raster_sunlit(sc.vertices, sc.bmin, sc.bmax, sun, raster=512, layers=64)
+    o = optics.make_optics(sc, sl, sun)
+    res = eng.solve(o, 100.0, 64, 1.0)
+    ref, md = oracle.solve(g, o, 100.0, 64, 1.0)
+    assert res.iterations == 64
+    assert _rel(res.radiosity, ref) < RTOL
+    H = eng.incidentIrradiance(res, o, 100.0)
+    oH = oracle.incident(g, ref, o, 100.0)
+    assert _rel(H, oH) < RTOL
+    # relaxation path
+    res2 = eng.solve(o, 100.0, 40, 0.7)
+    ref2, _ = oracle.solve(g, o, 100.0, 40, 0.7)
+    assert _rel(res2.radiosity, ref2) < RTOL
+
+
+def test_solve_accelerated_batches(oracle):
+    print("accelerated case", flush=True)
+    sc = scenes.row_crop(leaves_per_row=100, cells=12)
+    eng = _engine(R=64)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    o = optics.make_optics(sc, np.ones(sc.surface_count, np.float32), oracle.sun_direction(30.0, 135.0))
+    res = eng.solveAccelerated(o, 100.0, None, 64, 8, 1.0e-4, 1.0)
+    assert res.iterations % 8 == 0 and res.iterations <= 64
+    assert res.maxDelta <= 1.0e-4 or res.iterations == 64
+    ref, _ = oracle.solve(og, o, 100.0, res.iterations, 1.0)
+    assert _rel(res.radiosity, ref) < RTOL
+    # warm start from the converged answer stops after one batch
+    res2 = eng.solveAccelerated(o, 100.0, res.radiosity, 64, 8, 1.0e-4, 1.0)
+    assert res2.iterations == 8
+
+
+def _c1_bands(oracle, sc, sl, sun):
+    tir = dict(leaf_r=0.02, leaf_t=0.0, soil_r=0.05, leaf_sunlit_T=300.0, leaf_shaded_T=296.0,
+               soil_sunlit_T=305.0, soil_shaded_T=295.0, sky_T=250.0)
+    return optics.spectral_optics(sc, sl, sun, [650.0, 850.0, 10000.0], [0.05, 0.45, 0.02], [0.05, 0.45, 0.0],
+                                  [0.15, 0.25, 0.05], tir)
+
+
+@pytest.mark.parametrize("precision", [FH_PRECISION_FP32, FH_PRECISION_BF16_DC])
+def test_row_crop_three_bands_converged(oracle, precision):
+    # configuration 1: 4800 facets, 3 bands, 256 rays, relative change < 1e-6
+    print(f"row crop bands precision={precision}", flush=True)
+    sc = scenes.row_crop()
+    eng = _engine(R=256, K=64, seed=1)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    sun = oracle.sun_direction(30.0, 135.0)
+    sl = eng.computeSunlitFraction(sun)
+    r, t, e, D, sky = _c1_bands(oracle, sc, sl, sun)
+    res = eng.solveBands(r, t, e, D, sky, tolRel=1e-6, precision=precision)
+    assert res.maxRelDelta <= 1e-6
+    ref, it, last = oracle.solve_bands(og, r, t, e, D, sky, tol_rel=1e-9)
+    assert _rel(res.radiosity, ref) < RTOL
+    H = eng.incidentBands(res.radiosity, D, sky)
+    oH = oracle.incident_bands(og, ref, D, sky)
+    assert _rel(H, oH) < RTOL
+    if precision == FH_PRECISION_BF16_DC:
+        assert res.refreshes >= 1
+
+
+def test_direct_lu_small_graph(oracle):
+    print("direct LU case", flush=True)
+    sc = scenes.row_crop(leaves_per_row=40, cells=6)
+    eng = _engine(R=128)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    rng = np.random.default_rng(2)
+    S = sc.surface_count
+    o = np.zeros((S, 4), np.float32)
+    o[:, 0] = rng.uniform(0.0, 0.6, S)
+    o[:, 1] = rng.uniform(0.0, 0.35, S)
+    o[:, 2] = rng.uniform(0.0, 20.0, S)
+    o[:, 3] = rng.uniform(0.0, 400.0, S)
+    Bd = oracle.solve_direct(og, o, 60.0)
+    for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC):
+        res = eng.solveBands(o[:, :1], o[:, 1:2], o[:, 2:3], o[:, 3:4], [60.0], tolRel=1e-7, precision=prec)
+        assert _rel(res.radiosity[:, 0], Bd) < RTOL
+
+
+def test_error_paths():
+    print("error paths", flush=True)
+    from streamsim_amd._lib import LogicError
+    eng = _engine()
+    with pytest.raises(LogicError):
+        eng.buildVisibilityGraph()
+    sc = scenes.row_crop(leaves_per_row=5, cells=2)
+    bad = sc.vertices.copy()
+    bad[0, 3:6] = [0.0, 2.0, 0.0]
+    with pytest.raises(ValueError):
+        eng.setGeometry(bad)
+    eng.setGeometry(sc.vertices)
+    o = np.zeros((sc.surface_count, 4), np.float32)
+    with pytest.raises(LogicError):
+        eng.solve(o, 100.0)
+    eng.buildVisibilityGraph()
+    o[0, 0] = 0.8
+    o[0, 1] = 0.5
+    with pytest.raises(ValueError):
+        eng.solve(o, 100.0)
+    o[0, :2] = 0.1
+    with pytest.raises(ValueError):
+        eng.solve(o, 100.0, 64, 1.5)
+    with pytest.raises(ValueError):
+        eng.solve(o, -1.0)
+    res = eng.solve(o, 100.0, 8)
+    assert res.iterations == 8
+
+
+def _eb_case(oracle, leaves=40, cells=6, R=128, K=32):
+    sc = scenes.row_crop(leaves_per_row=leaves, cells=cells)
+    eng = _engine(R=R, K=K, seed=4)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    win = faceteb.ShortwaveWindows.for_bands(4)
+    faceteb.setup_scene(eng, sc, win)
+    return sc, eng, win
+
+
+def test_eb_node_matches_oracle(oracle):
+    print("eb node case", flush=True)
+    sc, eng, win = _eb_case(oracle)
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    cfg = faceteb.EbConfig(tolerance=1e-3, radiosity_tolerance=1e-7, max_coupling=200)
+    met = faceteb.diurnal_meteo(24)[12]
+    dni, dif = win.irradiance(met.shortwave, math.cos(math.radians(met.zenith)))
+    p = faceteb.node_params(met, cfg, first=True)
+    st = eng.ebRunNode(p, dni, dif)
+    assert st.converged
+    s = eng.ebGetState()
+    # shortwave: oracle multi-band solve on the same graph, sunlit and optics
+    r_tab, t_tab = faceteb.material_tables(["leaf", "soil", "building"], win)
+    mat = np.repeat(sc.material, 2)
+    r = r_tab[mat]
+    t = np.minimum(np.clip(t_tab[mat], 0, 1 - r), 1 - r)
+    sun = faceteb.sun_vector(met.zenith, met.azimuth)
+    cos = np.maximum(0.0, optics.side_normals(sc) @ sun).astype(np.float32)
+    D = (np.clip(s["sunlit"], 0, 1) * cos)[:, None] * dni[None, :]
+    sw, _, _ = oracle.solve_bands(og, r, t, np.zeros_like(r), D, dif, tol_rel=1e-9)
+    H = oracle.incident_bands(og, sw, D, dif)
+    absorbed = np.sum(np.clip(1 - r - t, 0, 1) * H, axis=1)
+    assert _rel(s["absorbedShortwave"][absorbed > 1], absorbed[absorbed > 1]) < 1e-4
+    # coupled energy balance in FP64 with the oracle from the same start
+    mats = []
+    for m in (faceteb.LEAF, faceteb.SOIL, faceteb.BUILDING):
+        om = oracle.OrEbMaterial()
+        for k, v in m.items():
+            setattr(om, k, v)
+        mats.append(om)
+    op = oracle.OrEbParams()
+    op.airTemperatureK = met.air_temperature_k
+    op.vaporPressure = met.vapor_pressure
+    op.pressure = met.pressure
+    op.windSpeed = met.wind
+    op.skyLongwave = met.longwave
+    op.dTime = cfg.dtime
+    op.tolerance = 1e-3
+    op.relaxation = cfg.relaxation
+    op.maxCoupling = 200
+    op.lwMaxSweeps = 200
+    op.lwTolRel = 1e-9
+    T0 = np.full(sc.surface_count, met.air_temperature_k, np.float32)
+    T, Blw, it, conv, _, _ = oracle.eb_node(og, sc.facet_count, sc.leaf_count, sc.material, mats,
+                                            s["absorbedShortwave"], op, T0, T0)
+    assert conv
+    assert np.max(np.abs(s["temperature"] - T)) < 0.01
+    assert _rel(s["longwaveRadiosity"], Blw) < 1e-4
+
+
+def test_eb_diurnal_plain_vs_bf16(oracle):
+    # the BF16 + defect-correction path reaches the same converged state
+    print("eb diurnal case", flush=True)
+    sc, eng, win = _eb_case(oracle, leaves=30, cells=5, R=64, K=16)
+    met = faceteb.diurnal_meteo(6)
+    cfg = faceteb.EbConfig(tolerance=0.01)
+    s1 = faceteb.run_diurnal(eng, met, win, cfg, precision=FH_PRECISION_FP32)
+    T1 = eng.ebGetState()["temperature"]
+    s2 = faceteb.run_diurnal(eng, met, win, cfg, precision=FH_PRECISION_BF16_DC)
+    T2 = eng.ebGetState()["temperature"]
+    assert all(s1.converged) and all(s2.converged)
+    assert np.max(np.abs(T1 - T2)) < 0.02
+
+
+def test_mlp_gradient_matches_oracle(oracle):
+    print("mlp gradient case", flush=True)
+    eng = _engine()
+    rng = np.random.default_rng(8)
+    H = 32
+    n_par = eng.mlpParamCount(H)
+    P = rng.normal(0, 0.3, n_par).astype(np.float32)
+    eng.mlpSetParams(H, P)
+    x = rng.normal(0, 1, (64, 9)).astype(np.float32)
+    t = rng.normal(0, 1, 64).astype(np.float32)
+    loss, g = eng.mlpLossGrad(x, t)
+    oloss, og = oracle.mlp_loss_grad(P, x, t, H)
+    assert loss == pytest.approx(oloss, rel=1e-5)
+    np.testing.assert_allclose(g, og, rtol=1e-3, atol=1e-5 * np.max(np.abs(og)))
+    y = eng.mlpForward(x)
+    np.testing.assert_allclose(y, oracle.mlp_forward(P, x, H), rtol=1e-4, atol=1e-5)
