@@ -1,0 +1,279 @@
+"""Benchmark: converged facet-band-steps/s of the coupled radiosity + energy
+balance diurnal run on one MI355X, surrogate+BF16 vs plain FP32, plus the
+achieved HBM GB/s of the SpMM sweep against the 8 TB/s peak.
+
+A step is one meteorological node: sunlit rays, the multi-band shortwave
+solve, absorbed shortwave, and the longwave / energy-balance coupling run
+until max dT <= 0.01 K and every radiosity solve's relative change <= 1e-5.
+facet-band-steps = facets x (shortwave bands + 1 longwave band) x steps.
+
+  python bench.py [--gpus 1] [--steps 24] [--warmup 2] [--workload forest]
+
+Prints one JSON line.  The CPU baseline is the oracle's restatement of the
+reference CPU path on one node of the same workload, timed on 1 host core.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # forest scene with coupled energy balance: 60k facets, 1024 rays, 8 SW
+    # bands + LW, 24 hourly steps
+    "forest": dict(builder="forest", rays=1024, sw_bands=8, sun_rays=32, steps=24),
+    # large terrain forest: ~1.0 M facets, 512 rays, 47 SW bands + LW
+    "terrain": dict(builder="terrain", rays=512, sw_bands=47, sun_rays=16, steps=24),
+    # urban block + trees: ~262k facets, 512 rays, 31 SW bands + LW
+    "urban": dict(builder="urban", rays=512, sw_bands=31, sun_rays=16, steps=24),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="forest", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--surrogate", default=str(ROOT / "streamsim_amd" / "data" / "surrogate.npz"))
+    ap.add_argument("--roofline-reps", type=int, default=50)
+    return ap.parse_args()
+
+
+def sweep_bytes(S: int, nnz: int, nb: int, w: int) -> int:
+    """Algorithmic HBM bytes of one sweep: row offsets, (col + value) per
+    entry, fsky, and per side-band c, r, t read + B read + B write."""
+    return 4 * (S + 1) + (4 + w) * nnz + 4 * S + 20 * S * nb
+
+
+def time_sweep(eng, nb: int, precision: int, reps: int):
+    """Average duration of the SpMM sweep kernel, HIP events on the engine's
+    stream, inputs resident in HBM."""
+    import torch
+    from streamsim_amd import _lib
+    lib = _lib.load()
+    h = eng.handle
+    S = eng.surfaceCount()
+    ldb = lib.fh_band_stride(nb)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    c = torch.rand(S, ldb, device="cuda", generator=g) * 50
+    r = torch.rand(S, ldb, device="cuda", generator=g) * 0.4
+    t = r * 0.5
+    B0 = torch.rand(S, ldb, device="cuda", generator=g) * 100
+    B1 = torch.zeros_like(B0)
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(lib.fh_stream(h))
+    args = [c.data_ptr(), r.data_ptr(), t.data_ptr()]
+    for i in range(3):
+        assert lib.fh_sweep_device(h, nb, *args, B0.data_ptr(), B1.data_ptr(), precision) == 0
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(reps):
+        src, dst = (B0, B1) if i % 2 == 0 else (B1, B0)
+        st = lib.fh_sweep_device(h, nb, *args, src.data_ptr(), dst.data_ptr(), precision)
+        if st:
+            raise RuntimeError(lib.fh_last_error(h).decode())
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps * 1.0e-3  # seconds per sweep
+
+
+def cpu_baseline(eng, scene, windows, meteo, cfg, nb_total: int, budget_s: float = 25.0):
+    """The oracle (1 core) on one daytime node of the same graph: multi-band
+    SW solve, incident, absorbed, then the coupled FP64 energy balance, both
+    to the benchmark's tolerances.  Returns facet-band-steps/s."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_ref as O
+    from streamsim_amd import faceteb, optics
+
+    g = eng.getGraph()
+    og = O.Graph(g.rowOffsets, g.neighbors, g.counts, g.denominators, g.skyCounts)
+    k = max(range(len(meteo)), key=lambda i: meteo[i].shortwave)
+    met = meteo[k]
+    cz = math.cos(math.radians(met.zenith))
+    dni, dif = windows.irradiance(met.shortwave, cz)
+    sun = faceteb.sun_vector(met.zenith, met.azimuth)
+    sl = eng.computeSunlitFraction(sun)
+    r_tab, t_tab = faceteb.material_tables(["leaf", "soil", "building"], windows)
+    mat = np.repeat(scene.material, 2)
+    r = r_tab[mat]
+    t = np.minimum(np.clip(t_tab[mat], 0, 1 - r), 1 - r)
+    cos = np.maximum(0.0, optics.side_normals(scene) @ sun).astype(np.float32)
+    D = (np.clip(sl, 0, 1) * cos)[:, None] * dni[None, :]
+    t0 = time.perf_counter()
+    B, it, _ = O.solve_bands(og, r, t, np.zeros_like(r), D, dif, tol_rel=cfg.radiosity_tolerance)
+    H = O.incident_bands(og, B, D, dif)
+    absorbed = np.sum(np.clip(1 - r - t, 0, 1) * H, axis=1).astype(np.float32)
+    mats = []
+    for m in (faceteb.LEAF, faceteb.SOIL, faceteb.BUILDING):
+        om = O.OrEbMaterial()
+        for kk, v in m.items():
+            setattr(om, kk, v)
+        mats.append(om)
+    p = O.OrEbParams()
+    p.airTemperatureK = met.air_temperature_k
+    p.vaporPressure = met.vapor_pressure
+    p.pressure = met.pressure
+    p.windSpeed = met.wind
+    p.skyLongwave = met.longwave
+    p.dTime = cfg.dtime
+    p.tolerance = cfg.tolerance
+    p.relaxation = cfg.relaxation
+    p.maxCoupling = cfg.max_coupling
+    p.lwMaxSweeps = cfg.max_sweeps
+    p.lwTolRel = cfg.radiosity_tolerance
+    T0 = np.full(scene.surface_count, met.air_temperature_k, np.float32)
+    O.eb_node(og, scene.facet_count, scene.leaf_count, scene.material, mats, absorbed, p, T0, T0)
+    dt = time.perf_counter() - t0
+    return scene.facet_count * nb_total / dt, dt, f"1 converged daytime node (hour {met.hour:.1f}, {it} SW sweeps)"
+
+
+def load_surrogate(eng, path: str) -> bool:
+    if not os.path.exists(path):
+        return False
+    z = np.load(path, allow_pickle=False)
+    eng.mlpSetParams(int(z["hidden"]), z["params"], z["feature_mean"], z["feature_std"], float(z["target_mean"]),
+                     float(z["target_std"]))
+    return True
+
+
+def main() -> None:
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path is built)"}))
+        sys.exit(2)
+    wl = WORKLOADS[a.workload]
+    steps = a.steps if a.steps is not None else wl["steps"]
+    from streamsim_amd import FacetEngine, faceteb, scenes
+    from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32, FH_WARM_PREVIOUS
+    import torch
+    torch.cuda.init()
+
+    log(f"[bench] workload={a.workload} steps={steps} warmup={a.warmup}")
+    t = time.perf_counter()
+    scene = scenes.BUILDERS[wl["builder"]]()
+    log(f"[bench] scene {scene.facet_count} facets ({scene.leaf_count} objects) in {time.perf_counter() - t:.1f}s")
+    eng = FacetEngine(rays_per_side=wl["rays"], sun_rays_per_side=wl["sun_rays"], seed=1)
+    t = time.perf_counter()
+    eng.setGeometry(scene.vertices)
+    t_bvh = time.perf_counter() - t
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    diag = eng.buildVisibilityGraph()
+    torch.cuda.synchronize()
+    t_graph = time.perf_counter() - t
+    S = scene.surface_count
+    rays_per_s = S * wl["rays"] / t_graph
+    log(f"[bench] BVH {t_bvh:.2f}s, form factors {t_graph:.3f}s ({rays_per_s:.3g} rays/s), "
+        f"nnz {diag.directedEdgeCount} ({diag.directedEdgeCount / S:.1f}/side)")
+
+    windows = faceteb.ShortwaveWindows.for_bands(wl["sw_bands"])
+    nb_total = windows.count + 1
+    faceteb.setup_scene(eng, scene, windows)
+    cfg = faceteb.EbConfig()
+    meteo = faceteb.diurnal_meteo(24)
+    day = [meteo[k % 24] for k in range(steps)]
+    have_sur = load_surrogate(eng, a.surrogate)
+    fast_warm = 1 if have_sur else FH_WARM_PREVIOUS
+
+    def run(precision, warm):
+        if a.warmup:
+            faceteb.run_diurnal(eng, meteo[:a.warmup], windows, cfg, precision=precision, warm=warm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = faceteb.run_diurnal(eng, day, windows, cfg, precision=precision, warm=warm)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, st
+
+    t_plain, st_plain = run(FH_PRECISION_FP32, FH_WARM_PREVIOUS)
+    log(f"[bench] plain FP32: {t_plain:.3f}s {st_plain.summary()}")
+    t_fast, st_fast = run(FH_PRECISION_BF16_DC, fast_warm)
+    log(f"[bench] BF16{'+surrogate' if have_sur else ''}: {t_fast:.3f}s {st_fast.summary()}")
+    work = scene.facet_count * nb_total * steps
+    value = work / t_fast
+    plain_value = work / t_plain
+
+    # roofline of the dominant kernel: the shortwave SpMM sweep
+    nb = windows.count
+    nnz = diag.directedEdgeCount
+    sec = time_sweep(eng, nb, FH_PRECISION_BF16_DC, a.roofline_reps)
+    sec32 = time_sweep(eng, nb, FH_PRECISION_FP32, a.roofline_reps)
+    alg = sweep_bytes(S, nnz, nb, 2)
+    alg32 = sweep_bytes(S, nnz, nb, 4)
+    achieved = alg / sec / 1e9
+    log(f"[bench] SW sweep nb={nb}: bf16 {sec * 1e6:.1f} us ({achieved:.0f} GB/s), "
+        f"fp32 {sec32 * 1e6:.1f} us ({alg32 / sec32 / 1e9:.0f} GB/s)")
+    traffic = None
+    tf = ROOT / "profiles" / f"sweep_traffic_{a.workload}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        v, dt, sample = cpu_baseline(eng, scene, windows, meteo, cfg, nb_total)
+        cpu = {"value": v, "unit": "facet-band-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{sample}; {dt:.1f}s on 1 core"}
+        log(f"[bench] cpu baseline {v:.4g} facet-band-steps/s ({dt:.1f}s)")
+
+    out = {
+        "metric": "converged facet-band-steps/s at 0.01 K: surrogate+BF16 vs plain FP32; HBM GB/s",
+        "value": value,
+        "unit": "facet-band-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": t_fast / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": a.workload,
+            "facets": scene.facet_count,
+            "sides": S,
+            "rays_per_side": wl["rays"],
+            "bands": nb_total,
+            "shortwave_bands": nb,
+            "time_steps": steps,
+            "tolerance_K": cfg.tolerance,
+            "radiosity_rel_tol": cfg.radiosity_tolerance,
+            "fast_path": "bf16-F + fp32 defect correction" + (" + surrogate warm start" if have_sur else ""),
+            "parallelism": "single GPU",
+        },
+        "plain_fp32": {"value": plain_value, "seconds": t_plain, **st_plain.summary()},
+        "fast": {"value": value, "seconds": t_fast, **st_fast.summary()},
+        "speedup_vs_plain": t_plain / t_fast,
+        "form_factors": {"seconds": t_graph, "rays_per_s": rays_per_s, "directed_edges": nnz,
+                         "bvh_host_seconds": t_bvh},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": f"k_sweep shortwave nb={nb} bf16", "bytes_per_launch": alg,
+                     "us_per_launch": sec * 1e6, "fp32_us_per_launch": sec32 * 1e6,
+                     "fp32_achieved": alg32 / sec32 / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.destroy()
+
+
+if __name__ == "__main__":
+    main()

@@ -167,7 +167,8 @@ int fh_solve_bands_device(fh_context *ctx, uint32_t nb, const float *d_reflectan
 int fh_incident_bands(fh_context *ctx, uint32_t nb, const float *radiosity, const float *direct,
                       const float *sky, float *irradiance);
 
-/* One FP32 Jacobi sweep on device pointers (benchmark / roofline entry).
+/* One Jacobi sweep on device pointers, launched as a single kernel with no
+ * convergence bookkeeping and no synchronisation (benchmark / roofline entry).
  * d_c, d_r, d_t, d_Bin, d_Bout are [surfaceCount][ldb]; ldb = nb rounded
  * up to a multiple of 4 (1 when nb == 1).  precision selects the F values
  * read (FP32 or BF16). */

@@ -530,7 +530,6 @@ int fh_sweep_device(fh_context* c, uint32_t nb, const float* dc, const float* dr
         require_graph(c, "sweep()");
         if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
         if (!dc || !dr || !dt || !dBin || !dBout) fail(FH_INVALID_ARGUMENT, "arrays must not be null");
-        fh::launch_ctrl_reset(c, 0xffffffffu, -1.0f, -1.0f, 1u);
         fh::launch_single_sweep(c, fh::band_stride(nb), dc, dr, dt, dBin, dBout, precision == FH_PRECISION_BF16_DC);
     });
 }

@@ -57,6 +57,7 @@ struct SweepArgs {
     uint32_t nchunks;
     float relax;
     uint32_t judge;  // 1: this sweep may end the solve on tolRel
+    uint32_t standalone;  // 1: timing launch, ignores the control block
     SolveCtrl* ctrl;
     float2* partial;  // per-block (max |dB|, max |dB|/|B|)
 };
@@ -91,7 +92,7 @@ __device__ __forceinline__ void team_reduce(float (&acc)[NBC]) {
 template <int NBC, int TEAM, int MODE, bool W16>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     SolveCtrl* ctrl = a.ctrl;
-    if (MODE != MODE_INCIDENT) {
+    if (MODE != MODE_INCIDENT && !a.standalone) {
         if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
     }
     const int lane = threadIdx.x % TEAM;
@@ -340,7 +341,7 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         case 8: launch_team<8>(team, mode, w16, a, c->stream); break;
         default: launch_team<16>(team, mode, w16, a, c->stream); break;
     }
-    if (mode != MODE_INCIDENT)
+    if (mode != MODE_INCIDENT && !a.standalone)
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, grid ? grid : 1u,
                            a.judge, uint32_t(mode == MODE_REFRESH));
     FH_HIP(hipGetLastError());
@@ -493,6 +494,7 @@ void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const flo
     a.ldb = ldb;
     a.relax = 1.0f;
     a.judge = 0u;
+    a.standalone = 1u;
     launch_sweep(c, MODE_SWEEP, w16, a);
 }
 }  // namespace fh
