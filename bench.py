@@ -63,37 +63,10 @@ def sweep_bytes(S: int, nnz: int, nb: int, w: int) -> int:
     return 4 * (S + 1) + (4 + w) * nnz + 4 * S + 20 * S * nb
 
 
-def time_sweep(eng, nb: int, precision: int, reps: int):
-    """Average duration of the SpMM sweep kernel, HIP events on the engine's
-    stream, inputs resident in HBM."""
-    import torch
-    from streamsim_amd import _lib
-    lib = _lib.load()
-    h = eng.handle
-    S = eng.surfaceCount()
-    ldb = lib.fh_band_stride(nb)
-    g = torch.Generator(device="cuda").manual_seed(0)
-    c = torch.rand(S, ldb, device="cuda", generator=g) * 50
-    r = torch.rand(S, ldb, device="cuda", generator=g) * 0.4
-    t = r * 0.5
-    B0 = torch.rand(S, ldb, device="cuda", generator=g) * 100
-    B1 = torch.zeros_like(B0)
-    torch.cuda.synchronize()
-    stream = torch.cuda.ExternalStream(lib.fh_stream(h))
-    args = [c.data_ptr(), r.data_ptr(), t.data_ptr()]
-    for i in range(3):
-        assert lib.fh_sweep_device(h, nb, *args, B0.data_ptr(), B1.data_ptr(), precision) == 0
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for i in range(reps):
-        src, dst = (B0, B1) if i % 2 == 0 else (B1, B0)
-        st = lib.fh_sweep_device(h, nb, *args, src.data_ptr(), dst.data_ptr(), precision)
-        if st:
-            raise RuntimeError(lib.fh_last_error(h).decode())
-    e1.record(stream)
-    e1.synchronize()
-    return e0.elapsed_time(e1) / reps * 1.0e-3  # seconds per sweep
+def time_sweep(eng, nb: int, precision: int, reps: int) -> float:
+    """Average duration (s) of the SpMM sweep kernel: HIP events recorded on
+    the engine's stream around `reps` back-to-back launches, inputs in HBM."""
+    return eng.benchmarkSweep(nb, precision, reps)
 
 
 def cpu_baseline(eng, scene, windows, meteo, cfg, nb_total: int, budget_s: float = 25.0):
@@ -164,8 +137,6 @@ def main() -> None:
     steps = a.steps if a.steps is not None else wl["steps"]
     from streamsim_amd import FacetEngine, faceteb, scenes
     from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32, FH_WARM_PREVIOUS
-    import torch
-    torch.cuda.init()
 
     log(f"[bench] workload={a.workload} steps={steps} warmup={a.warmup}")
     t = time.perf_counter()
@@ -175,10 +146,10 @@ def main() -> None:
     t = time.perf_counter()
     eng.setGeometry(scene.vertices)
     t_bvh = time.perf_counter() - t
-    torch.cuda.synchronize()
+    eng.synchronize()
     t = time.perf_counter()
     diag = eng.buildVisibilityGraph()
-    torch.cuda.synchronize()
+    eng.synchronize()
     t_graph = time.perf_counter() - t
     S = scene.surface_count
     rays_per_s = S * wl["rays"] / t_graph
@@ -197,10 +168,10 @@ def main() -> None:
     def run(precision, warm):
         if a.warmup:
             faceteb.run_diurnal(eng, meteo[:a.warmup], windows, cfg, precision=precision, warm=warm)
-        torch.cuda.synchronize()
+        eng.synchronize()
         t0 = time.perf_counter()
         st = faceteb.run_diurnal(eng, day, windows, cfg, precision=precision, warm=warm)
-        torch.cuda.synchronize()
+        eng.synchronize()
         return time.perf_counter() - t0, st
 
     t_plain, st_plain = run(FH_PRECISION_FP32, FH_WARM_PREVIOUS)

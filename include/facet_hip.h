@@ -179,6 +179,14 @@ uint32_t fh_band_stride(uint32_t nb);
 /* The context's HIP stream (hipStream_t), for event timing by the caller. */
 void *fh_stream(const fh_context *ctx);
 
+/* Average duration (microseconds, HIP events on the context's stream) of
+ * `reps` back-to-back single-kernel SpMM sweeps (fh_sweep_device) over the
+ * current graph with nb bands, on engine-owned HBM inputs. */
+int fh_benchmark_sweep(fh_context *ctx, uint32_t nb, uint32_t precision, uint32_t reps,
+                       float *microsecondsPerSweep);
+/* Waits for all work queued on the context's stream. */
+int fh_synchronize(fh_context *ctx);
+
 
 /* ---- FacetEB: coupled shortwave / longwave radiosity + energy balance ---- */
 
@@ -285,9 +293,11 @@ int fh_mlp_forward(fh_context *ctx, const float *x, uint32_t n, float *y);
 /* Mean-squared error over n samples and its gradient (host arrays). */
 int fh_mlp_loss_grad(fh_context *ctx, const float *x, const float *target, uint32_t n, float *grad,
                      float *loss);
-/* Adam training on device arrays d_x [n][9] (normalised) and d_target [n]. */
-int fh_mlp_train(fh_context *ctx, const float *d_x, const float *d_target, uint32_t n, uint32_t batch,
-                 uint32_t epochs, float learningRate, uint32_t seed, float *lastLoss);
+/* Adam training on n normalised samples x [n][9], target [n] (host arrays,
+ * uploaded once): `epochs` shuffled passes of `batch`-sample steps, learning
+ * rate learningRate * lrDecay^epoch. */
+int fh_mlp_train(fh_context *ctx, const float *x, const float *target, uint32_t n, uint32_t batch,
+                 uint32_t epochs, float learningRate, float lrDecay, uint32_t seed, float *lastLoss);
 /* Energy-balance units of the scene and, after a node run with
  * recordFeatures or the surrogate, their raw features [units][9] and current
  * temperatures [units]. */

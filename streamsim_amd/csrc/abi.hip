@@ -534,4 +534,46 @@ int fh_sweep_device(fh_context* c, uint32_t nb, const float* dc, const float* dr
     });
 }
 
+int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t reps, float* us) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "sweep()");
+        if (nb == 0u || nb > 4096u || reps == 0u || !us) fail(FH_INVALID_ARGUMENT, "invalid benchmark arguments");
+        const uint32_t ldb = fh::band_stride(nb);
+        const size_t n = size_t(c->S) * ldb;
+        fh::DBuf<float> buf;
+        buf.ensure(5 * n);
+        float* cc = buf.p;
+        float* rr = cc + n;
+        float* tt = rr + n;
+        float* B0 = tt + n;
+        float* B1 = B0 + n;
+        fh::launch_fill_f32(c, cc, n, 25.0f);
+        fh::launch_fill_f32(c, rr, n, 0.2f);
+        fh::launch_fill_f32(c, tt, n, 0.1f);
+        fh::launch_fill_f32(c, B0, n, 50.0f);
+        fh::launch_fill_f32(c, B1, n, 0.0f);
+        const bool w16 = precision == FH_PRECISION_BF16_DC;
+        for (int i = 0; i < 3; ++i) fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16);
+        hipEvent_t e0, e1;
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        FH_HIP(hipEventRecord(e0, c->stream));
+        for (uint32_t i = 0; i < reps; ++i)
+            fh::launch_single_sweep(c, ldb, cc, rr, tt, (i & 1u) ? B1 : B0, (i & 1u) ? B0 : B1, w16);
+        FH_HIP(hipEventRecord(e1, c->stream));
+        FH_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *us = ms * 1000.0f / float(reps);
+    });
+}
+
+int fh_synchronize(fh_context* c) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] { FH_HIP(hipStreamSynchronize(c->stream)); });
+}
+
 }  // extern "C"

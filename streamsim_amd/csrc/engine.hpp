@@ -140,6 +140,7 @@ struct MlpState {
     DBuf<float> params;            // W1[H][9], b1[H], W2[H][H], b2[H], W3[H], b3
     float inMean[9] = {}, inStd[9] = {};
     float outMean = 0.0f, outStd = 1.0f;
+    uint64_t adamStep = 0;         // Adam steps taken since the parameters were set
     // training workspace
     DBuf<float> A1, H1, A2, H2, y, dA2, dA1, grad, m, v, red;
 };

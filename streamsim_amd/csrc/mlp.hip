@@ -348,6 +348,7 @@ void mlp_set(fh_context* c, uint32_t H, const float* params, const float* inMean
     }
     m.outMean = outMean;
     m.outStd = outStd;
+    m.adamStep = 0;
     m.m.ensure(np);
     m.v.ensure(np);
     launch_fill_f32(c, m.m.p, np, 0.0f);
@@ -444,43 +445,45 @@ int fh_mlp_loss_grad(fh_context* c, const float* x, const float* target, uint32_
     });
 }
 
-int fh_mlp_train(fh_context* c, const float* d_x, const float* d_t, uint32_t n, uint32_t batch, uint32_t epochs,
-                 float lr, uint32_t seed, float* lastLoss) {
+int fh_mlp_train(fh_context* c, const float* x, const float* target, uint32_t n, uint32_t batch, uint32_t epochs,
+                 float lr, float lrDecay, uint32_t seed, float* lastLoss) {
     return guarded_mlp(c, [&] {
         if (!c->mlp.ready) fail(FH_LOGIC_ERROR, "no surrogate parameters are set");
-        if (!d_x || !d_t || n == 0u || batch == 0u) fail(FH_INVALID_ARGUMENT, "training data must be given");
+        if (!x || !target || n == 0u || batch == 0u) fail(FH_INVALID_ARGUMENT, "training data must be given");
+        if (!(lr > 0.0f) || !(lrDecay > 0.0f)) fail(FH_INVALID_ARGUMENT, "learning rate must be positive");
         batch = std::min(batch, n);
+        fh::DBuf<float> dx, dt;
+        dx.ensure(size_t(n) * fh::kIn);
+        dt.ensure(n);
+        FH_HIP(hipMemcpyAsync(dx.p, x, sizeof(float) * n * fh::kIn, hipMemcpyHostToDevice, c->stream));
+        FH_HIP(hipMemcpyAsync(dt.p, target, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
         fh::ensure_train(c, batch);
         const uint32_t np = fh::mlp_param_count(c->mlp.hidden);
         const uint32_t steps = n / batch;
         std::mt19937 rng(seed);
         std::vector<uint32_t> order(steps);
         for (uint32_t i = 0; i < steps; ++i) order[i] = i;
-        float b1t = 1.0f, b2t = 1.0f;
         const float b1 = 0.9f, b2 = 0.999f;
-        for (uint32_t ep = 0; ep < epochs; ++ep) {
+        float elr = lr;
+        for (uint32_t ep = 0; ep < epochs; ++ep, elr *= lrDecay) {
             std::shuffle(order.begin(), order.end(), rng);
             for (uint32_t k = 0; k < steps; ++k) {
                 const size_t off = size_t(order[k]) * batch;
-                fh::batch_grad(c, d_x + off * fh::kIn, d_t + off, batch);
-                b1t *= b1;
-                b2t *= b2;
+                fh::batch_grad(c, dx.p + off * fh::kIn, dt.p + off, batch);
+                const double t = double(++c->mlp.adamStep);
+                const float c1 = float(1.0 - std::pow(double(b1), t));
+                const float c2 = float(1.0 - std::pow(double(b2), t));
                 hipLaunchKernelGGL(fh::k_adam, dim3((np + 255u) / 256u), dim3(256), 0, c->stream, c->mlp.params.p,
-                                   c->mlp.grad.p, c->mlp.m.p, c->mlp.v.p, np, lr, b1, b2, 1.0f - b1t, 1.0f - b2t,
-                                   1.0e-8f);
+                                   c->mlp.grad.p, c->mlp.m.p, c->mlp.v.p, np, elr, b1, b2, c1, c2, 1.0e-8f);
                 FH_HIP(hipGetLastError());
             }
         }
-        if (lastLoss) {
-            std::vector<float> parts(batch);
-            FH_HIP(hipMemcpyAsync(parts.data(), c->mlp.red.p, sizeof(float) * batch, hipMemcpyDeviceToHost,
-                                  c->stream));
-            FH_HIP(hipStreamSynchronize(c->stream));
-            double acc = 0.0;
-            for (float p : parts) acc += p;
-            *lastLoss = float(acc / batch);
-        }
+        std::vector<float> parts(batch);
+        FH_HIP(hipMemcpyAsync(parts.data(), c->mlp.red.p, sizeof(float) * batch, hipMemcpyDeviceToHost, c->stream));
         FH_HIP(hipStreamSynchronize(c->stream));
+        double acc = 0.0;
+        for (float p : parts) acc += p;
+        if (lastLoss) *lastLoss = float(acc / batch);
     });
 }
 

@@ -316,8 +316,20 @@ class FacetEngine:
         self._check(self._lib.fh_mlp_loss_grad(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)))
         return float(loss.value), g
 
-    def mlpTrainDevice(self, d_x: int, d_t: int, n: int, batch: int, epochs: int, lr: float, seed: int = 0) -> float:
+    def mlpTrain(self, x, target, batch: int, epochs: int, lr: float, lr_decay: float = 1.0, seed: int = 0) -> float:
+        x = _f32(x)
+        t = _f32(target)
         loss = C.c_float(0.0)
-        self._check(self._lib.fh_mlp_train(self._h, C.c_void_p(d_x), C.c_void_p(d_t), n, batch, epochs, lr, seed,
+        self._check(self._lib.fh_mlp_train(self._h, _ptr(x), _ptr(t), x.shape[0], batch, epochs, lr, lr_decay, seed,
                                            C.byref(loss)))
         return float(loss.value)
+
+    # -- timing ------------------------------------------------------------------
+    def benchmarkSweep(self, nb: int, precision: int, reps: int = 50) -> float:
+        """Average SpMM sweep kernel duration in seconds (HIP events)."""
+        us = C.c_float(0.0)
+        self._check(self._lib.fh_benchmark_sweep(self._h, nb, precision, reps, C.byref(us)))
+        return us.value * 1e-6
+
+    def synchronize(self) -> None:
+        self._check(self._lib.fh_synchronize(self._h))

@@ -289,3 +289,33 @@ def test_mlp_gradient_matches_oracle(oracle):
     np.testing.assert_allclose(g, og, rtol=1e-3, atol=1e-5 * np.max(np.abs(og)))
     y = eng.mlpForward(x)
     np.testing.assert_allclose(y, oracle.mlp_forward(P, x, H), rtol=1e-4, atol=1e-5)
+
+
+def test_surrogate_warm_start_gain():
+    # the committed network (trained on the device by fh_mlp_train on forest
+    # and urban scenes) warm-starts a held-out row crop: the coupled iteration
+    # still runs to 0.01 K, so the converged state agrees with the plain run,
+    # and it needs no more coupling iterations than the plain warm start
+    print("surrogate gain case", flush=True)
+    from pathlib import Path
+    from streamsim_amd._lib import FH_WARM_PREVIOUS, FH_WARM_SURROGATE
+    path = Path(__file__).resolve().parents[1] / "streamsim_amd" / "data" / "surrogate.npz"
+    z = np.load(path, allow_pickle=False)
+    sc = scenes.forest(seed=91, trees_x=2, leaves=800, cells=30, size=20.0)
+    eng = _engine(R=256, K=16, seed=5)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    win = faceteb.ShortwaveWindows.for_bands(8)
+    faceteb.setup_scene(eng, sc, win)
+    eng.mlpSetParams(int(z["hidden"]), z["params"], z["feature_mean"], z["feature_std"], float(z["target_mean"]),
+                     float(z["target_std"]))
+    met = faceteb.diurnal_meteo(12)
+    cfg = faceteb.EbConfig()
+    s0 = faceteb.run_diurnal(eng, met, win, cfg, warm=FH_WARM_PREVIOUS)
+    T0 = eng.ebGetState()["temperature"]
+    s1 = faceteb.run_diurnal(eng, met, win, cfg, warm=FH_WARM_SURROGATE)
+    T1 = eng.ebGetState()["temperature"]
+    assert all(s0.converged) and all(s1.converged)
+    assert np.max(np.abs(T0 - T1)) < 0.05
+    print(f"coupling iterations plain {sum(s0.coupling)} surrogate {sum(s1.coupling)}", flush=True)
+    assert sum(s1.coupling) <= sum(s0.coupling)
