@@ -181,6 +181,14 @@ def main() -> None:
     work = scene.facet_count * nb_total * steps
     value = work / t_fast
     plain_value = work / t_plain
+    # ablations: each of the two changes alone
+    ablation = {}
+    t_b, st_b = run(FH_PRECISION_BF16_DC, FH_WARM_PREVIOUS)
+    ablation["bf16_only"] = {"value": work / t_b, "seconds": t_b, **st_b.summary()}
+    if have_sur:
+        t_s, st_s = run(FH_PRECISION_FP32, 1)
+        ablation["surrogate_only"] = {"value": work / t_s, "seconds": t_s, **st_s.summary()}
+    log(f"[bench] ablation {json.dumps({k: round(v['seconds'], 4) for k, v in ablation.items()})}")
 
     # roofline of the dominant kernel: the shortwave SpMM sweep
     nb = windows.count
@@ -233,6 +241,7 @@ def main() -> None:
         "plain_fp32": {"value": plain_value, "seconds": t_plain, **st_plain.summary()},
         "fast": {"value": value, "seconds": t_fast, **st_fast.summary()},
         "speedup_vs_plain": t_plain / t_fast,
+        "ablation": ablation,
         "form_factors": {"seconds": t_graph, "rays_per_s": rays_per_s, "directed_edges": nnz,
                          "bvh_host_seconds": t_bvh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

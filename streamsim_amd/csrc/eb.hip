@@ -18,6 +18,7 @@
 #include <cmath>
 
 #include "solver.hpp"
+#include "sweep_common.hpp"
 
 namespace fh {
 
@@ -161,117 +162,249 @@ __global__ __launch_bounds__(256) void k_absorb(uint32_t S, uint32_t nb, uint32_
         for (int k = 0; k < 3; ++k) part[size_t(blockIdx.x) * 4 + k] = sh[k][0];
 }
 
-// Fluxes, residual and relaxed Newton update (FP64), one lane per facet.
-// Writes per-block (max dT, max residual-equivalent) and the FP64 partial of
-// sum A Rn over energy-balance sides.
-__global__ __launch_bounds__(256) void k_newton(uint32_t F, EbConst k, const uint32_t* __restrict__ mat,
-                                                const fh_eb_material* __restrict__ mats,
-                                                const float* __restrict__ absorbed, const float* __restrict__ H,
-                                                const float* __restrict__ e, const float* __restrict__ eps,
-                                                float* __restrict__ T, const float* __restrict__ Tprev, int update,
-                                                float* __restrict__ Rn_out, float* __restrict__ H_out,
-                                                float* __restrict__ LE_out, float* __restrict__ G_out,
-                                                const double* __restrict__ area, float2* __restrict__ part,
-                                                double* __restrict__ dpart) {
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    float myDT = 0.0f, myEq = 0.0f;
-    double rnSum = 0.0;
-    if (f < F) {
-        const fh_eb_material m = mats[mat[f]];
-        const uint32_t s0 = 2u * f;
-        double rn[2];
-        for (int q = 0; q < 2; ++q) {
-            const uint32_t s = s0 + q;
-            rn[q] = double(absorbed[s]) + double(eps[s]) * H[s] - double(e[s]);
+// LW emission of a side at temperature T (FP64, stored FP32)
+__device__ __forceinline__ float lw_emission(float eps, float T) {
+    const double Tc = clampd(double(T), 1.0, 1000.0);
+    return float(double(eps) * kSigma * Tc * Tc * Tc * Tc);
+}
+
+struct NewtonOut {
+    float dT, eq;
+    double rn;
+};
+
+// Fluxes, residual and relaxed Newton update of one facet (FP64) given the
+// incident LW H0, H1 of its two sides.  Leaf facets share side 0's T.
+__device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_material& m,
+                                  const float* __restrict__ absorbed, float H0, float H1, const float* __restrict__ eps,
+                                  float* __restrict__ T, const float* __restrict__ Tprev, bool update,
+                                  float* __restrict__ Rn_out, float* __restrict__ H_out, float* __restrict__ LE_out,
+                                  float* __restrict__ G_out, const double* __restrict__ area) {
+    NewtonOut o{0.0f, 0.0f, 0.0};
+    const uint32_t s0 = 2u * f;
+    const float Hs[2] = {H0, H1};
+    double rn[2];
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t s = s0 + q;
+        rn[q] = double(absorbed[s]) + double(eps[s]) * Hs[q] - double(lw_emission(eps[s], T[s]));
+    }
+    if (f < k.leafCount) {
+        const double Tl = T[s0];
+        const double net = rn[0] + rn[1];
+        const double hs = 2.0 * k.ha * m.convectiveScale * (Tl - k.Ta);
+        double rs = m.surfaceResistance;
+        if (k.vegMethod == 1u) {
+            const double par = fmax(0.0, double(absorbed[s0]) + absorbed[s0 + 1]);
+            const double open = 0.25 + 0.75 * par / (par + 180.0);
+            const double tc = (Tl - 273.15 - 25.0) / 18.0;
+            rs = clampd(m.surfaceResistance / fmax(0.08, open * exp(-(tc * tc))), 40.0, 100000.0);
         }
-        if (f < k.leafCount) {
-            const double Tl = T[s0];
-            const double net = rn[0] + rn[1];
-            const double hs = 2.0 * k.ha * m.convectiveScale * (Tl - k.Ta);
-            double rs = m.surfaceResistance;
-            if (k.vegMethod == 1u) {
-                const double par = fmax(0.0, double(absorbed[s0]) + absorbed[s0 + 1]);
-                const double open = 0.25 + 0.75 * par / (par + 180.0);
-                const double tc = (Tl - 273.15 - 25.0) / 18.0;
-                const double resp = exp(-(tc * tc));
-                rs = clampd(m.surfaceResistance / fmax(0.08, open * resp), 40.0, 100000.0);
-            }
-            double le = 0.0, dle = 0.0;
-            if (m.vegetation) {
-                latent(Tl, k, rs, le, dle);
-                le *= 2.0;
-                dle *= 2.0;
-            }
-            const double gd = fmax(0.0, double(m.heatCapacityPerArea)) / k.dTime;
-            const double g = gd * (Tl - double(Tprev[s0]));
-            const double res = net - hs - le - g;
-            const double der = 8.0 * double(eps[s0]) * kSigma * Tl * Tl * Tl + 2.0 * k.ha * m.convectiveScale + dle + gd;
-            myEq = float(fabs(res) / fmax(1.0, der));
-            rnSum += area[f] * net;
+        double le = 0.0, dle = 0.0;
+        if (m.vegetation) {
+            latent(Tl, k, rs, le, dle);
+            le *= 2.0;
+            dle *= 2.0;
+        }
+        const double gd = fmax(0.0, double(m.heatCapacityPerArea)) / k.dTime;
+        const double g = gd * (Tl - double(Tprev[s0]));
+        const double res = net - hs - le - g;
+        const double der = 8.0 * double(eps[s0]) * kSigma * Tl * Tl * Tl + 2.0 * k.ha * m.convectiveScale + dle + gd;
+        o.eq = float(fabs(res) / fmax(1.0, der));
+        o.rn = area[f] * net;
+        if (Rn_out)
             for (int q = 0; q < 2; ++q) {
                 Rn_out[s0 + q] = float(net);
                 H_out[s0 + q] = float(hs);
                 LE_out[s0 + q] = float(le);
                 G_out[s0 + q] = float(g);
             }
-            if (update) {
-                const double cand = clampd(Tl + res / fmax(1.0, der), 220.0, 360.0);
-                const double upd = Tl + k.relax * (cand - Tl);
-                myDT = float(fabs(upd - Tl));
-                T[s0] = float(upd);
-                T[s0 + 1] = float(upd);
+        if (update) {
+            const double cand = clampd(Tl + res / fmax(1.0, der), 220.0, 360.0);
+            const double upd = Tl + k.relax * (cand - Tl);
+            o.dT = float(fabs(upd - Tl));
+            T[s0] = float(upd);
+            T[s0 + 1] = float(upd);
+        }
+    } else {
+        const bool dry = m.soilMoisture < 0.03f;
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t s = s0 + q;
+            const double Ts = T[s];
+            const double hs = k.ha * (Ts - k.Ta);
+            double le = 0.0, dle = 0.0;
+            if (!dry) latent(Ts, k, m.surfaceResistance, le, dle);
+            double g, gd;
+            if (m.soilTemperatureMethod == 0) {
+                gd = 0.0;
+                g = 0.35 * rn[q];
+            } else {
+                gd = soil_storage(m, k.dTime) * (m.soilTemperatureMethod == 2 ? 1.5 : 1.0);
+                g = gd * (Ts - double(Tprev[s]));
             }
-        } else {
-            const bool dry = m.soilMoisture < 0.03f;
-            for (int q = 0; q < 2; ++q) {
-                const uint32_t s = s0 + q;
-                const double Ts = T[s];
-                const double hs = k.ha * (Ts - k.Ta);
-                double le = 0.0, dle = 0.0;
-                if (!dry) latent(Ts, k, m.surfaceResistance, le, dle);
-                double g, gd;
-                if (m.soilTemperatureMethod == 0) {
-                    gd = 0.0;
-                    g = 0.35 * rn[q];
-                } else {
-                    gd = soil_storage(m, k.dTime) * (m.soilTemperatureMethod == 2 ? 1.5 : 1.0);
-                    g = gd * (Ts - double(Tprev[s]));
-                }
-                const double res = rn[q] - hs - le - g;
-                const double der = 4.0 * double(eps[s]) * kSigma * Ts * Ts * Ts + k.ha + dle + gd;
-                myEq = fmaxf(myEq, float(fabs(res) / fmax(1.0, der)));
-                rnSum += area[f] * rn[q];
+            const double res = rn[q] - hs - le - g;
+            const double der = 4.0 * double(eps[s]) * kSigma * Ts * Ts * Ts + k.ha + dle + gd;
+            o.eq = fmaxf(o.eq, float(fabs(res) / fmax(1.0, der)));
+            o.rn += area[f] * rn[q];
+            if (Rn_out) {
                 Rn_out[s] = float(rn[q]);
                 H_out[s] = float(hs);
                 LE_out[s] = float(le);
                 G_out[s] = float(g);
-                if (update) {
-                    const double cand = clampd(Ts + res / fmax(1.0, der), 220.0, 360.0);
-                    const double upd = Ts + k.relax * (cand - Ts);
-                    myDT = fmaxf(myDT, float(fabs(upd - Ts)));
-                    T[s] = float(upd);
-                }
+            }
+            if (update) {
+                const double cand = clampd(Ts + res / fmax(1.0, der), 220.0, 360.0);
+                const double upd = Ts + k.relax * (cand - Ts);
+                o.dT = fmaxf(o.dT, float(fabs(upd - Ts)));
+                T[s] = float(upd);
             }
         }
     }
+    return o;
+}
+
+// Newton step over all facets from the incident LW H (one lane per facet).
+// Writes per-block (max dT, max residual-equivalent) and the FP64 partial of
+// sum A Rn over energy-balance sides.
+__global__ __launch_bounds__(256) void k_newton(uint32_t F, EbConst k, const uint32_t* __restrict__ mat,
+                                                const fh_eb_material* __restrict__ mats,
+                                                const float* __restrict__ absorbed, const float* __restrict__ H,
+                                                const float* __restrict__ eps, float* __restrict__ T,
+                                                const float* __restrict__ Tprev, int update, float* __restrict__ Rn_out,
+                                                float* __restrict__ H_out, float* __restrict__ LE_out,
+                                                float* __restrict__ G_out, const double* __restrict__ area,
+                                                float2* __restrict__ part, double* __restrict__ dpart) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    float v[2] = {0.0f, 0.0f};
+    double rnSum = 0.0;
+    if (f < F) {
+        const NewtonOut o = newton_facet(f, k, mats[mat[f]], absorbed, H[2u * f], H[2u * f + 1], eps, T, Tprev,
+                                         update != 0, Rn_out, H_out, LE_out, G_out, area);
+        v[0] = o.dT;
+        v[1] = o.eq;
+        rnSum = o.rn;
+    }
 #pragma unroll
-    for (int msk = 32; msk > 0; msk >>= 1) {
-        myDT = fmaxf(myDT, __shfl_xor(myDT, msk, 64));
-        myEq = fmaxf(myEq, __shfl_xor(myEq, msk, 64));
-        rnSum += __shfl_xor(rnSum, msk, 64);
-    }
-    __shared__ float sdt[4], seq[4];
+    for (int msk = 32; msk > 0; msk >>= 1) rnSum += __shfl_xor(rnSum, msk, 64);
     __shared__ double srn[4];
-    if ((threadIdx.x & 63) == 0) {
-        sdt[threadIdx.x >> 6] = myDT;
-        seq[threadIdx.x >> 6] = myEq;
-        srn[threadIdx.x >> 6] = rnSum;
-    }
-    __syncthreads();
+    if ((threadIdx.x & 63) == 0) srn[threadIdx.x >> 6] = rnSum;
+    float out[2];
+    block_max<2>(v, out);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = make_float2(fmaxf(fmaxf(sdt[0], sdt[1]), fmaxf(sdt[2], sdt[3])),
-                                       fmaxf(fmaxf(seq[0], seq[1]), fmaxf(seq[2], seq[3])));
+        part[blockIdx.x] = make_float2(out[0], out[1]);
         dpart[size_t(blockIdx.x) * 4 + 3] = (srn[0] + srn[1]) + (srn[2] + srn[3]);
+    }
+}
+
+// ---- fused coupled sweep ----------------------------------------------------
+// One nonlinear Jacobi-Newton iteration of the coupled LW radiosity and
+// energy balance per launch: each facet team gathers G = F B for its two
+// sides, forms the incident LW H = max(0, fsky L + G), takes the relaxed
+// Newton step of the facet's temperature from H (FP64), recomputes the LW
+// emission at the new temperature and writes the Jacobi update
+// B' = e(T') + r (fsky_s L + G_s) + t (fsky_o L + G_o).  The fixed point is
+// the reference's converged coupling (a LW solve converged for T and a zero
+// energy residual); the loop stops when max dT, max residual-equivalent and
+// the largest relative change of B are all within tolerance.
+struct CoupledArgs {
+    const uint32_t* rowptr;
+    const uint32_t* col;
+    const float* w32;
+    const uint16_t* w16;
+    const float* Bin;
+    float* Bout;
+    const float* fsky;
+    const float* r;
+    const float* t;
+    const float* eps;
+    const float* absorbed;
+    float* T;
+    const float* Tprev;
+    float* Hout;
+    const uint32_t* mat;
+    const fh_eb_material* mats;
+    const double* area;
+    float L;
+    uint32_t F;
+    EbConst k;
+    SolveCtrl* ctrl;
+    float4* partial;
+};
+
+template <int TEAM, bool W16>
+__global__ __launch_bounds__(256) void k_coupled(CoupledArgs a) {
+    if (*reinterpret_cast<volatile uint32_t*>(&a.ctrl->done)) return;
+    const int lane = threadIdx.x % TEAM;
+    const uint32_t f = (blockIdx.x * 256u + threadIdx.x) / TEAM;
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // |dB|, |dB|/|B|, dT, residual-equivalent
+    if (f < a.F) {
+        const uint32_t s0 = 2u * f;
+        const uint32_t e0 = a.rowptr[s0], e1 = a.rowptr[s0 + 1], e2 = a.rowptr[s0 + 2];
+        float g[2][1] = {{0.0f}, {0.0f}};
+        for (uint32_t e = e0 + lane; e < e1; e += TEAM)
+            gather<1>(a.Bin + a.col[e], W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g[0]);
+        for (uint32_t e = e1 + lane; e < e2; e += TEAM)
+            gather<1>(a.Bin + a.col[e], W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g[1]);
+        team_reduce<1, TEAM>(g[0]);
+        team_reduce<1, TEAM>(g[1]);
+        if (lane == 0) {
+            const float L = a.L;
+            const float in0 = a.fsky[s0] * L + g[0][0];
+            const float in1 = a.fsky[s0 + 1] * L + g[1][0];
+            const float H0 = fmaxf(in0, 0.0f), H1 = fmaxf(in1, 0.0f);
+            a.Hout[s0] = H0;
+            a.Hout[s0 + 1] = H1;
+            const NewtonOut o = newton_facet(f, a.k, a.mats[a.mat[f]], a.absorbed, H0, H1, a.eps, a.T, a.Tprev, true,
+                                             nullptr, nullptr, nullptr, nullptr, a.area);
+            v[2] = o.dT;
+            v[3] = o.eq;
+            const float ins[2] = {in0, in1};
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const uint32_t s = s0 + q;
+                const float em = lw_emission(a.eps[s], a.T[s]);
+                const float nb = em + a.r[s] * ins[q] + a.t[s] * ins[q ^ 1];
+                const float old = a.Bin[s];
+                a.Bout[s] = nb;
+                const float d = fabsf(nb - old);
+                v[0] = fmaxf(v[0], d);
+                v[1] = fmaxf(v[1], d / fmaxf(fabsf(nb), 1.0e-30f));
+            }
+        }
+    }
+    float out[4];
+    block_max<4>(v, out);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = make_float4(out[0], out[1], out[2], out[3]);
+}
+
+// Reduces the coupled-sweep partials; done when max dT <= tolT, the max
+// residual-equivalent <= tolT and max |dB|/|B| <= tolRel (or the budget ends).
+__global__ __launch_bounds__(1024) void k_coupled_finalize(SolveCtrl* ctrl, const float4* __restrict__ part,
+                                                           uint32_t n, float tolT, float* __restrict__ last4) {
+    if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
+        const float4 p = part[i];
+        m[0] = fmaxf(m[0], p.x);
+        m[1] = fmaxf(m[1], p.y);
+        m[2] = fmaxf(m[2], p.z);
+        m[3] = fmaxf(m[3], p.w);
+    }
+    __shared__ float sh[1024][4];
+    for (int q = 0; q < 4; ++q) sh[threadIdx.x][q] = m[q];
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (int(threadIdx.x) < off)
+            for (int q = 0; q < 4; ++q) sh[threadIdx.x][q] = fmaxf(sh[threadIdx.x][q], sh[threadIdx.x + off][q]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const uint32_t it = ctrl->iter + 1u;
+        ctrl->iter = it;
+        ctrl->lastAbs = sh[0][0];
+        ctrl->lastRel = sh[0][1];
+        for (int q = 0; q < 4; ++q) last4[q] = sh[0][q];
+        const bool conv = sh[0][2] <= tolT && sh[0][3] <= tolT && sh[0][1] <= ctrl->tolRel;
+        if (conv || it >= ctrl->maxIter) ctrl->done = 1u;
     }
 }
 
@@ -382,7 +515,7 @@ static void newton(fh_context* c, const EbConst& k, int update, float& maxDT, fl
     c->partial.ensure(nbk);
     e.dpart.ensure(size_t(nbk) * 4u);
     hipLaunchKernelGGL(k_newton, dim3(nbk), dim3(256), 0, c->stream, F, k, e.material.p, e.mats.p, e.absorbed.p,
-                       e.Hlw.p, e.elw.p, e.eps.p, e.T.p, e.Tprev.p, update, e.Rn.p, e.Hs.p, e.LE.p, e.G.p, e.area.p,
+                       e.Hlw.p, e.eps.p, e.T.p, e.Tprev.p, update, e.Rn.p, e.Hs.p, e.LE.p, e.G.p, e.area.p,
                        c->partial.p, e.dpart.p);
     hipLaunchKernelGGL(k_reduce_max2, dim3(1), dim3(1024), 0, c->stream, c->partial.p, nbk, e.fout.p);
     if (rnSum) hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(256), 0, c->stream, e.dpart.p, nbk, 3u, 4u, e.dsum.p);
@@ -398,7 +531,7 @@ static void newton(fh_context* c, const EbConst& k, int update, float& maxDT, fl
 }
 
 // LW: source from T, warm or zero start, solve, incident (into e.Hlw)
-static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p) {
+static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p, uint32_t precision) {
     EbState& e = c->eb;
     const uint32_t S = c->S;
     hipLaunchKernelGGL(k_lw_source, blocks_for(S), dim3(256), 0, c->stream, S, e.T.p, e.eps.p, e.rlw.p, e.tlw.p,
@@ -409,7 +542,7 @@ static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p) {
     set_sky(c, &L, 1u, 1u);
     JacobiOptions o;
     o.ldb = 1u;
-    o.precision = p.precision;
+    o.precision = precision;
     o.maxIter = p.maxSweeps;
     o.tolRel = p.radiosityTolerance;
     float* bufs[2] = {e.Blw0.p, e.Blw1.p};
@@ -419,6 +552,66 @@ static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p) {
     e.lwValid = true;
     run_incident(c, 1u, e.Blw0.p, e.zero.p, e.Hlw.p);
     return st.iterations;
+}
+
+// Fused coupled sweeps until the three tolerances hold or maxIter; the
+// result is left in Blw0 and the last (|dB|, |dB|/|B|, dT, residual-eq).
+static uint32_t coupled_loop(fh_context* c, const fh_eb_node_params& p, const EbConst& k, bool w16,
+                             uint32_t maxIter, float last[4]) {
+    EbState& e = c->eb;
+    const int team = sweep_team(c);
+    const uint32_t grid = uint32_t((uint64_t(c->F) * uint64_t(team) + 255u) / 256u);
+    e.part4.ensure(grid);
+    e.last4.ensure(4);
+    launch_ctrl_reset(c, maxIter ? maxIter : 1u, p.radiosityTolerance, -1.0f, 1u);
+    CoupledArgs a{};
+    a.rowptr = c->rowptr.p;
+    a.col = c->col.p;
+    a.w32 = c->val32.p;
+    a.w16 = c->val16.p;
+    a.fsky = c->fsky.p;
+    a.r = e.rlw.p;
+    a.t = e.tlw.p;
+    a.eps = e.eps.p;
+    a.absorbed = e.absorbed.p;
+    a.T = e.T.p;
+    a.Tprev = e.Tprev.p;
+    a.Hout = e.Hlw.p;
+    a.mat = e.material.p;
+    a.mats = e.mats.p;
+    a.area = e.area.p;
+    a.L = p.skyLongwave;
+    a.F = c->F;
+    a.k = k;
+    a.ctrl = c->ctrl.p;
+    a.partial = e.part4.p;
+    uint64_t launched = 0;
+    uint32_t batch = 4;
+    while (true) {
+        for (uint32_t q = 0; q < batch; ++q) {
+            a.Bin = (launched & 1u) ? e.Blw1.p : e.Blw0.p;
+            a.Bout = (launched & 1u) ? e.Blw0.p : e.Blw1.p;
+#define FH_COUPLED(T, W) hipLaunchKernelGGL((k_coupled<T, W>), dim3(grid), dim3(256), 0, c->stream, a)
+            if (team == 4) { if (w16) FH_COUPLED(4, true); else FH_COUPLED(4, false); }
+            else if (team == 8) { if (w16) FH_COUPLED(8, true); else FH_COUPLED(8, false); }
+            else if (team == 16) { if (w16) FH_COUPLED(16, true); else FH_COUPLED(16, false); }
+            else if (team == 32) { if (w16) FH_COUPLED(32, true); else FH_COUPLED(32, false); }
+            else { if (w16) FH_COUPLED(64, true); else FH_COUPLED(64, false); }
+#undef FH_COUPLED
+            hipLaunchKernelGGL(k_coupled_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, e.part4.p, grid,
+                               p.temperatureTolerance, e.last4.p);
+            ++launched;
+        }
+        FH_HIP(hipGetLastError());
+        FH_HIP(hipMemcpyAsync(c->hctrl, c->ctrl.p, sizeof(SolveCtrl), hipMemcpyDeviceToHost, c->stream));
+        FH_HIP(hipMemcpyAsync(last, e.last4.p, sizeof(float) * 4, hipMemcpyDeviceToHost, c->stream));
+        FH_HIP(hipStreamSynchronize(c->stream));
+        if (c->hctrl->done) break;
+        if (batch < 16u) batch *= 2u;
+    }
+    const uint32_t iters = c->hctrl->iter;
+    if (iters & 1u) e.Blw0.swap(e.Blw1);
+    return iters;
 }
 
 void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost, const float* diffHost,
@@ -500,19 +693,26 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         surrogate_guess(c, k, p.windSpeed, p.skyLongwave, p.airTemperatureK);
     else if (p.recordFeatures)
         eb_features(c, p.airTemperatureK, p.windSpeed, p.skyLongwave);
-    // --- coupling loop -------------------------------------------------------
-    float maxDT = 0.0f, maxEq = 0.0f;
-    uint32_t iters = 0;
-    for (uint32_t it = 0; it < std::max(1u, p.maxCoupling); ++it) {
-        st.lwSweeps += lw_solve(c, p);
-        newton(c, k, 1, maxDT, maxEq, nullptr);
-        iters = it + 1u;
-        if (maxDT <= p.temperatureTolerance && maxEq <= p.temperatureTolerance) break;
-    }
-    st.lwSweeps += lw_solve(c, p);
-    float dtFinal = 0.0f;
+    // --- coupling: fused Jacobi-Newton sweeps, FP32 refinement, final check ---
+    if (!e.lwValid || p.zeroStart) st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
+    float last[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    const bool bf16 = p.precision == FH_PRECISION_BF16_DC;
+    uint32_t iters = coupled_loop(c, p, k, bf16, std::max(1u, p.maxCoupling), last);
+    float maxDT = last[2];
+    // final LW solve with the converged temperatures and the residual check
+    // (core:3834-3846), always with FP32 form factors: in BF16 mode this is
+    // the defect check, and a failed check continues with FP32 sweeps
+    st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
+    float dtFinal = 0.0f, maxEq = 0.0f;
     double rn = 0.0;
     newton(c, k, 0, dtFinal, maxEq, &rn);
+    if (bf16 && !(maxDT <= p.temperatureTolerance && maxEq <= p.temperatureTolerance)) {
+        iters += coupled_loop(c, p, k, false, std::max(1u, p.maxCoupling), last);
+        maxDT = last[2];
+        st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
+        newton(c, k, 0, dtFinal, maxEq, &rn);
+    }
+    st.lwSweeps += iters;
     st.couplingIterations = iters;
     st.maxTemperatureDelta = maxDT;
     st.maxResidualEquivalent = maxEq;

@@ -95,7 +95,8 @@ struct SolveCtrl {
     float lastAbs;
     float lastRel;
     uint32_t refreshes;
-    uint32_t pad[7];
+    uint32_t innerDone;   // BF16 inner sweeps reached tolRel: skip until the next refresh
+    uint32_t pad[6];
 };
 
 uint32_t band_stride(uint32_t nb);
@@ -122,6 +123,8 @@ struct EbState {
     DBuf<double> dpart;            // per-block FP64 partial sums
     DBuf<double> dsum;             // reduced FP64 sums
     DBuf<float2> fout;             // reduced (max dT, max residual-equivalent)
+    DBuf<float4> part4;            // coupled-sweep per-block partials
+    DBuf<float> last4;             // last coupled-sweep maxima
     DBuf<uint32_t> lit;
     // surrogate inputs / outputs per energy-balance unit (leaf facet or
     // ground side): unit u < leafCount is facet u, else ground side

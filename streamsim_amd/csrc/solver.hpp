@@ -37,7 +37,7 @@ struct JacobiOptions {
     float tolRel = -1.0f;
     float tolAbs = -1.0f;
     uint32_t checkEvery = 1;
-    uint32_t innerBf16 = 6;  // BF16 sweeps between FP32 refreshes
+    uint32_t innerBf16 = 12;  // most BF16 sweeps between FP32 refreshes
 };
 int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const float* rr, const float* tt,
                float* bufs[2], fh_solve_stats* stats);
@@ -46,6 +46,9 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
 void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, float* H);
 
 void set_sky(fh_context* c, const float* sky, uint32_t nb, uint32_t ldb);
+
+// lanes per facet team of the SpMM kernels for the current graph
+int sweep_team(const fh_context* c);
 
 // One sweep, no host synchronisation (timing entry).
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,

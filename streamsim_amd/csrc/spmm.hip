@@ -28,8 +28,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <string>
 
 #include "solver.hpp"
+#include "sweep_common.hpp"
 
 namespace fh {
 
@@ -58,42 +60,25 @@ struct SweepArgs {
     float relax;
     uint32_t judge;  // 1: this sweep may end the solve on tolRel
     uint32_t standalone;  // 1: timing launch, ignores the control block
+    float* buf0;  // when set: input / output are buf[iter & 1] / buf[~iter & 1],
+    float* buf1;  // iter = sweeps executed so far in this solve (device side)
     SolveCtrl* ctrl;
     float2* partial;  // per-block (max |dB|, max |dB|/|B|)
 };
-
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
-
-template <int NBC>
-__device__ __forceinline__ void gather(const float* __restrict__ p, float w, float (&acc)[NBC]) {
-    if constexpr (NBC == 1) {
-        acc[0] += w * p[0];
-    } else {
-#pragma unroll
-        for (int q = 0; q < NBC / 4; ++q) {
-            const float4 v = *reinterpret_cast<const float4*>(p + 4 * q);
-            acc[4 * q + 0] += w * v.x;
-            acc[4 * q + 1] += w * v.y;
-            acc[4 * q + 2] += w * v.z;
-            acc[4 * q + 3] += w * v.w;
-        }
-    }
-}
-
-template <int NBC, int TEAM>
-__device__ __forceinline__ void team_reduce(float (&acc)[NBC]) {
-#pragma unroll
-    for (int j = 0; j < NBC; ++j) {
-#pragma unroll
-        for (int m = TEAM / 2; m > 0; m >>= 1) acc[j] += __shfl_xor(acc[j], m, TEAM);
-    }
-}
 
 template <int NBC, int TEAM, int MODE, bool W16>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     SolveCtrl* ctrl = a.ctrl;
     if (MODE != MODE_INCIDENT && !a.standalone) {
         if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
+        if (MODE == MODE_SWEEP && W16 && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return;
+    }
+    const float* __restrict__ Bin = a.Bin;
+    float* __restrict__ Bout = a.Bout;
+    if (MODE != MODE_INCIDENT && !a.standalone && a.buf0) {
+        const uint32_t it = *reinterpret_cast<volatile uint32_t*>(&ctrl->iter);
+        Bin = (it & 1u) ? a.buf1 : a.buf0;
+        Bout = (it & 1u) ? a.buf0 : a.buf1;
     }
     const int lane = threadIdx.x % TEAM;
     const uint32_t f = (blockIdx.x * 256u + threadIdx.x) / TEAM;
@@ -109,7 +94,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
             for (int j = 0; j < NBC; ++j) g0[j] = g1[j] = q0[j] = q1[j] = 0.0f;
             for (uint32_t e = e0 + lane; e < e1; e += TEAM) {
                 const uint32_t n = a.col[e];
-                const float* p = a.Bin + size_t(n) * a.ldb + b0;
+                const float* p = Bin + size_t(n) * a.ldb + b0;
                 if (MODE == MODE_REFRESH) {
                     gather<NBC>(p, a.w32[e], g0);
                     gather<NBC>(p, bf16_to_f32(a.w16[e]), q0);
@@ -119,7 +104,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
             }
             for (uint32_t e = e1 + lane; e < e2; e += TEAM) {
                 const uint32_t n = a.col[e];
-                const float* p = a.Bin + size_t(n) * a.ldb + b0;
+                const float* p = Bin + size_t(n) * a.ldb + b0;
                 if (MODE == MODE_REFRESH) {
                     gather<NBC>(p, a.w32[e], g1);
                     gather<NBC>(p, bf16_to_f32(a.w16[e]), q1);
@@ -156,11 +141,11 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
                     a.c2[i0] = c0 + r0 * (g0[j] - q0[j]) + t0 * (g1[j] - q1[j]);
                     a.c2[i1] = c1 + r1 * (g1[j] - q1[j]) + t1 * (g0[j] - q0[j]);
                 }
-                const float o0 = a.Bin[i0], o1 = a.Bin[i1];
+                const float o0 = Bin[i0], o1 = Bin[i1];
                 const float x0 = o0 + a.relax * (n0 - o0);
                 const float x1 = o1 + a.relax * (n1 - o1);
-                a.Bout[i0] = x0;
-                a.Bout[i1] = x1;
+                Bout[i0] = x0;
+                Bout[i1] = x1;
                 const float d0 = fabsf(x0 - o0), d1 = fabsf(x1 - o1);
                 myAbs = fmaxf(myAbs, fmaxf(d0, d1));
                 myRel = fmaxf(myRel, fmaxf(d0 / fmaxf(fabsf(x0), 1.0e-30f), d1 / fmaxf(fabsf(x1), 1.0e-30f)));
@@ -189,11 +174,118 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     }
 }
 
+// Band-parallel variant for multi-band B (ldb >= 4): a team of BT lanes owns
+// one facet and each lane owns KB bands (b = chunk + lane + k BT), so the
+// gather of one CSR entry reads BT * KB contiguous floats of B[col] with
+// coalesced loads.  The team loads BT entries (col, w) at a time, one per
+// lane, and broadcasts each by cross-lane shuffle.  Bands beyond BT * KB are
+// processed in chunks.
+template <int BT, int KB, int MODE, bool W16>
+__global__ __launch_bounds__(256) void k_sweep_bp(SweepArgs a) {
+    SolveCtrl* ctrl = a.ctrl;
+    if (MODE != MODE_INCIDENT && !a.standalone) {
+        if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
+        if (MODE == MODE_SWEEP && W16 && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return;
+    }
+    const float* __restrict__ Bin = a.Bin;
+    float* __restrict__ Bout = a.Bout;
+    if (MODE != MODE_INCIDENT && !a.standalone && a.buf0) {
+        const uint32_t it = *reinterpret_cast<volatile uint32_t*>(&ctrl->iter);
+        Bin = (it & 1u) ? a.buf1 : a.buf0;
+        Bout = (it & 1u) ? a.buf0 : a.buf1;
+    }
+    const int lane = threadIdx.x % BT;
+    const uint32_t f = (blockIdx.x * 256u + threadIdx.x) / BT;
+    const uint32_t ldb = a.ldb;
+    float myAbs = 0.0f, myRel = 0.0f;
+    if (f < a.F) {
+        const uint32_t s0 = 2u * f;
+        const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
+        for (uint32_t cb = 0; cb < ldb; cb += BT * KB) {
+            float g[2][KB], q[2][KB];
+#pragma unroll
+            for (int k = 0; k < KB; ++k) g[0][k] = g[1][k] = q[0][k] = q[1][k] = 0.0f;
+            const float* __restrict__ Bl = Bin + cb + lane;
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const uint32_t end = rb[side + 1];
+                // U entries per step: all column / weight loads, then all
+                // U * KB gathers, then the FMAs (entries past the row end use
+                // column 0 with weight 0)
+                constexpr int U = 8;
+                for (uint32_t base = rb[side]; base < end; base += U) {
+                    uint32_t n[U];
+                    float w[U], wq[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        const uint32_t e = base + uint32_t(u);
+                        const bool ok = e < end;
+                        n[u] = ok ? a.col[e] : 0u;
+                        if (MODE == MODE_REFRESH) {
+                            w[u] = ok ? a.w32[e] : 0.0f;
+                            wq[u] = ok ? bf16_to_f32(a.w16[e]) : 0.0f;
+                        } else {
+                            w[u] = ok ? (W16 ? bf16_to_f32(a.w16[e]) : a.w32[e]) : 0.0f;
+                        }
+                    }
+                    float v[U][KB];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int k = 0; k < KB; ++k)
+                            v[u][k] = cb + uint32_t(lane + k * BT) < ldb ? Bl[size_t(n[u]) * ldb + k * BT] : 0.0f;
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int k = 0; k < KB; ++k) {
+                            g[side][k] += w[u] * v[u][k];
+                            if (MODE == MODE_REFRESH) q[side][k] += wq[u] * v[u][k];
+                        }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < KB; ++k) {
+                const uint32_t b = cb + uint32_t(lane) + uint32_t(k * BT);
+                if (b >= ldb) continue;
+                const size_t i0 = size_t(s0) * ldb + b, i1 = i0 + ldb;
+                if (MODE == MODE_INCIDENT) {
+                    const float sk = a.skyv[b];
+                    a.H[i0] = fmaxf(a.fsky[s0] * sk + g[0][k] + a.D[i0], 0.0f);
+                    a.H[i1] = fmaxf(a.fsky[s0 + 1] * sk + g[1][k] + a.D[i1], 0.0f);
+                    continue;
+                }
+                const float r0 = a.r[i0], t0 = a.t[i0], r1 = a.r[i1], t1 = a.t[i1];
+                const float* csrc = MODE == MODE_REFRESH ? a.cBase : a.c;
+                const float c0 = csrc[i0], c1 = csrc[i1];
+                const float n0 = c0 + r0 * g[0][k] + t0 * g[1][k];
+                const float n1 = c1 + r1 * g[1][k] + t1 * g[0][k];
+                if (MODE == MODE_REFRESH) {
+                    a.c2[i0] = c0 + r0 * (g[0][k] - q[0][k]) + t0 * (g[1][k] - q[1][k]);
+                    a.c2[i1] = c1 + r1 * (g[1][k] - q[1][k]) + t1 * (g[0][k] - q[0][k]);
+                }
+                const float o0 = Bin[i0], o1 = Bin[i1];
+                const float x0 = o0 + a.relax * (n0 - o0);
+                const float x1 = o1 + a.relax * (n1 - o1);
+                Bout[i0] = x0;
+                Bout[i1] = x1;
+                const float d0 = fabsf(x0 - o0), d1 = fabsf(x1 - o1);
+                myAbs = fmaxf(myAbs, fmaxf(d0, d1));
+                myRel = fmaxf(myRel, fmaxf(d0 / fmaxf(fabsf(x0), 1.0e-30f), d1 / fmaxf(fabsf(x1), 1.0e-30f)));
+            }
+        }
+    }
+    if (MODE == MODE_INCIDENT) return;
+    float v[2] = {myAbs, myRel}, out[2];
+    block_max<2>(v, out);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = make_float2(out[0], out[1]);
+}
+
 // Reduces the per-block residuals of the sweep just run and advances the
 // control block: iteration count, last residuals, done flag.
 __global__ __launch_bounds__(1024) void k_finalize(SolveCtrl* ctrl, const float2* __restrict__ partial, uint32_t n,
-                                                   uint32_t judge, uint32_t refresh) {
+                                                   uint32_t judge, uint32_t refresh, uint32_t inner) {
     if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
+    if (inner && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return;
     float mA = 0.0f, mR = 0.0f;
     for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
         const float2 p = partial[i];
@@ -221,7 +313,11 @@ __global__ __launch_bounds__(1024) void k_finalize(SolveCtrl* ctrl, const float2
         ctrl->iter = it;
         ctrl->lastAbs = absd;
         ctrl->lastRel = reld;
-        if (refresh) ctrl->refreshes += 1u;
+        if (refresh) {
+            ctrl->refreshes += 1u;
+            ctrl->innerDone = 0u;
+        }
+        if (inner && ctrl->tolRel >= 0.0f && reld <= ctrl->tolRel) ctrl->innerDone = 1u;
         bool stop = it >= ctrl->maxIter;
         if (judge && ctrl->tolRel >= 0.0f && reld <= ctrl->tolRel) stop = true;
         if (ctrl->tolAbs >= 0.0f && (it % ctrl->checkEvery) == 0u && absd <= ctrl->tolAbs) stop = true;
@@ -271,6 +367,8 @@ __global__ void k_split_optics(const float4* __restrict__ o, uint32_t S, float* 
     D[s] = v.w;
 }
 
+int sweep_team(const fh_context* c);
+
 uint32_t band_stride(uint32_t nb) { return nb <= 1u ? 1u : (nb + 3u) & ~3u; }
 
 static int pick_nbc(uint32_t ldb) {
@@ -282,7 +380,7 @@ static int pick_nbc(uint32_t ldb) {
     return 4;
 }
 
-static int pick_team(const fh_context* c) {
+int sweep_team(const fh_context* c) {
     // FH_SPMM_TEAM=4|16|64 overrides the choice (tuning knob)
     static const int forced = [] {
         const char* e = std::getenv("FH_SPMM_TEAM");
@@ -320,9 +418,75 @@ static void launch_team(int team, int mode, bool w16, const SweepArgs& a, hipStr
     else launch_mode<NBC, 64>(mode, w16, a, g, st);
 }
 
+template <int BT, int KB>
+static void launch_bp_mode(int mode, bool w16, const SweepArgs& a, uint32_t grid, hipStream_t st) {
+    if (mode == MODE_SWEEP && !w16)
+        hipLaunchKernelGGL((k_sweep_bp<BT, KB, MODE_SWEEP, false>), dim3(grid), dim3(256), 0, st, a);
+    else if (mode == MODE_SWEEP)
+        hipLaunchKernelGGL((k_sweep_bp<BT, KB, MODE_SWEEP, true>), dim3(grid), dim3(256), 0, st, a);
+    else if (mode == MODE_REFRESH)
+        hipLaunchKernelGGL((k_sweep_bp<BT, KB, MODE_REFRESH, false>), dim3(grid), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_sweep_bp<BT, KB, MODE_INCIDENT, false>), dim3(grid), dim3(256), 0, st, a);
+}
+
+// lanes per facet (BT) and bands per lane (KB) of the band-parallel kernel
+static void pick_bp(uint32_t ldb, int& bt, int& kb) {
+    if (ldb % 16u == 0u) {
+        bt = 16;
+        kb = ldb >= 64u ? 4 : int(ldb / 16u);
+    } else if (ldb % 8u == 0u) {
+        bt = 8;
+        kb = 1;
+    } else {
+        bt = 4;
+        kb = ldb == 12u ? 3 : 1;
+    }
+}
+
+// FH_SPMM_KERNEL=entry|band forces one of the two kernels (tuning knob)
+static bool band_parallel(uint32_t ldb) {
+    static const int forced = [] {
+        const char* e = std::getenv("FH_SPMM_KERNEL");
+        return e ? (std::string(e) == "entry" ? 1 : (std::string(e) == "band" ? 2 : 0)) : 0;
+    }();
+    if (forced == 1) return false;
+    if (forced == 2) return ldb >= 4u;
+    // measured on gfx950: entry-parallel wins up to 8 bands, band-parallel
+    // from 16 (fewer cache-line requests per gather instruction)
+    return ldb >= 16u;
+}
+
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
+    if (band_parallel(a.ldb)) {
+        int bt = 16, kb = 1;
+        pick_bp(a.ldb, bt, kb);
+        a.F = c->F;
+        a.rowptr = c->rowptr.p;
+        a.col = c->col.p;
+        a.w32 = c->val32.p;
+        a.w16 = c->val16.p;
+        a.fsky = c->fsky.p;
+        a.ctrl = c->ctrl.p;
+        const uint32_t grid = uint32_t((uint64_t(c->F) * uint64_t(bt) + 255u) / 256u);
+        c->partial.ensure(grid ? grid : 1u);
+        a.partial = c->partial.p;
+        const uint32_t g = grid ? grid : 1u;
+        if (bt == 16 && kb == 1) launch_bp_mode<16, 1>(mode, w16, a, g, c->stream);
+        else if (bt == 16 && kb == 2) launch_bp_mode<16, 2>(mode, w16, a, g, c->stream);
+        else if (bt == 16 && kb == 3) launch_bp_mode<16, 3>(mode, w16, a, g, c->stream);
+        else if (bt == 16) launch_bp_mode<16, 4>(mode, w16, a, g, c->stream);
+        else if (bt == 8) launch_bp_mode<8, 1>(mode, w16, a, g, c->stream);
+        else if (kb == 3) launch_bp_mode<4, 3>(mode, w16, a, g, c->stream);
+        else launch_bp_mode<4, 1>(mode, w16, a, g, c->stream);
+        if (mode != MODE_INCIDENT && !a.standalone)
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, g, a.judge,
+                               uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16));
+        FH_HIP(hipGetLastError());
+        return;
+    }
     const int nbc = pick_nbc(a.ldb);
-    const int team = pick_team(c);
+    const int team = sweep_team(c);
     a.nchunks = (a.ldb + uint32_t(nbc) - 1u) / uint32_t(nbc);
     a.F = c->F;
     a.rowptr = c->rowptr.p;
@@ -343,7 +507,7 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     }
     if (mode != MODE_INCIDENT && !a.standalone)
         hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, grid ? grid : 1u,
-                           a.judge, uint32_t(mode == MODE_REFRESH));
+                           a.judge, uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16));
     FH_HIP(hipGetLastError());
 }
 
@@ -437,23 +601,27 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     const bool bf16 = o.precision == FH_PRECISION_BF16_DC;
     if (bf16) c->c2.ensure(size_t(c->S) * o.ldb);
     while (true) {
+        a.buf0 = bufs[0];
+        a.buf1 = bufs[1];
         for (uint32_t k = 0; k < batch; ++k) {
-            a.Bin = bufs[launched & 1u];
-            a.Bout = bufs[(launched + 1u) & 1u];
             if (!bf16) {
                 a.c = cc;
                 a.judge = 1u;
                 launch_sweep(c, MODE_SWEEP, false, a);
             } else {
+                // groups of BF16 sweeps (stopping early once their relative
+                // change is within tolRel) closed by an FP32 refresh that
+                // judges convergence and recomputes the defect correction;
+                // the first group runs on the uncorrected source
                 const uint32_t period = o.innerBf16 + 1u;
-                if (launched % period == 0u) {
+                if (launched % period == period - 1u) {
                     a.c = cc;
                     a.cBase = cc;
                     a.c2 = c->c2.p;
                     a.judge = 1u;
                     launch_sweep(c, MODE_REFRESH, false, a);
                 } else {
-                    a.c = c->c2.p;
+                    a.c = launched < period ? cc : c->c2.p;
                     a.judge = 0u;
                     launch_sweep(c, MODE_SWEEP, true, a);
                 }

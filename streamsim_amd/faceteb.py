@@ -217,6 +217,7 @@ class RunStats:
 
     def summary(self) -> dict:
         return dict(nodes=self.nodes, coupling_iterations=int(np.sum(self.coupling)),
+                    coupling_per_node=[int(x) for x in self.coupling],
                     sw_sweeps=int(np.sum(self.sw_sweeps)), lw_sweeps=int(np.sum(self.lw_sweeps)),
                     all_converged=bool(all(self.converged)),
                     max_sw_closure=float(np.max(np.abs(self.closure))) if self.closure else 0.0)

@@ -319,3 +319,28 @@ def test_surrogate_warm_start_gain():
     assert np.max(np.abs(T0 - T1)) < 0.05
     print(f"coupling iterations plain {sum(s0.coupling)} surrogate {sum(s1.coupling)}", flush=True)
     assert sum(s1.coupling) <= sum(s0.coupling)
+
+
+@pytest.mark.parametrize("nb", [5, 12, 32, 47, 77])
+def test_many_bands_match_oracle(oracle, nb):
+    # every band layout of the SpMM kernels (band stride 8, 12, 32, 48 and a
+    # partial 64-band chunk at 80) against the FP64 oracle
+    print(f"many bands nb={nb}", flush=True)
+    sc = scenes.row_crop(leaves_per_row=60, cells=8)
+    eng = _engine(R=64, seed=6)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    rng = np.random.default_rng(nb)
+    S = sc.surface_count
+    r = rng.uniform(0.0, 0.5, (S, nb)).astype(np.float32)
+    t = (rng.uniform(0.0, 1.0, (S, nb)) * (1.0 - r) * 0.8).astype(np.float32)
+    e = rng.uniform(0.0, 5.0, (S, nb)).astype(np.float32)
+    D = rng.uniform(0.0, 300.0, (S, nb)).astype(np.float32)
+    sky = rng.uniform(10.0, 80.0, nb).astype(np.float32)
+    ref, _, _ = oracle.solve_bands(og, r, t, e, D, sky, tol_rel=1e-9)
+    for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC):
+        res = eng.solveBands(r, t, e, D, sky, tolRel=1e-7, precision=prec)
+        assert _rel(res.radiosity, ref) < RTOL, prec
+    H = eng.incidentBands(ref, D, sky)
+    assert _rel(H, oracle.incident_bands(og, ref, D, sky)) < RTOL
