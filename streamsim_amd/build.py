@@ -22,6 +22,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libfacet_hip.so"
+CLI = PKG / "facet_engine"
 ARCH = os.environ.get("FH_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
@@ -66,8 +67,15 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # command-line engine (host C++ over the C-ABI), rpath to this folder
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(PKG / "host" / "facet_engine.cpp"),
+           f"-L{PKG}", "-lfacet_hip", "-Wl,-rpath,$ORIGIN", "-o", str(CLI)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"facet_engine build failed:\n{res.stdout}\n{res.stderr}")
     if verbose:
-        print(f"built {LIB}", file=sys.stderr)
+        print(f"built {LIB} and {CLI}", file=sys.stderr)
     return LIB
 
 

@@ -46,3 +46,17 @@ def test_status_strings_without_gpu():
     assert lib.fh_status_string(1) == b"invalid argument"
     assert lib.fh_band_stride(1) == 1 and lib.fh_band_stride(3) == 4 and lib.fh_band_stride(48) == 48
     assert lib.fh_mlp_param_count(128) == 128 * 9 + 128 + 128 * 128 + 128 + 128 + 1
+
+
+def test_cli_usage_and_input_errors():
+    # main.cpp:54-79 exit codes: 2 usage; 1 failure (no GPU needed)
+    import subprocess
+    exe = ROOT / "streamsim_amd" / "facet_engine"
+    if not exe.exists():
+        pytest.fail("facet_engine is not built (python -m streamsim_amd.build)")
+    assert subprocess.run([str(exe)], capture_output=True).returncode == 2
+    assert subprocess.run([str(exe), "eVoxelRT", "x.obj"], capture_output=True).returncode == 2
+    r = subprocess.run([str(exe), "eFacetRT", "project.json"], capture_output=True, text=True)
+    assert r.returncode == 1 and "facetrt:" in r.stderr
+    r = subprocess.run([str(exe), "eFacetEB", "/nonexistent/scene.obj"], capture_output=True, text=True)
+    assert r.returncode == 1 and "faceteb:" in r.stderr
