@@ -7,7 +7,7 @@ solve, absorbed shortwave, and the longwave / energy-balance coupling run
 until max dT <= 0.01 K and every radiosity solve's relative change <= 1e-5.
 facet-band-steps = facets x (shortwave bands + 1 longwave band) x steps.
 
-  python bench.py [--gpus 1] [--steps 24] [--warmup 2] [--workload forest]
+  python bench.py [--gpus 1] [--steps 24] [--warmup 2] [--workload terrain]
 
 Prints one JSON line.  The CPU baseline is the oracle's restatement of the
 reference CPU path on one node of the same workload, timed on 1 host core.
@@ -50,10 +50,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="forest", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="terrain", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--surrogate", default=str(ROOT / "streamsim_amd" / "data" / "surrogate.npz"))
     ap.add_argument("--roofline-reps", type=int, default=50)
+    ap.add_argument("--mode", default="run", choices=["run", "sweep"],
+                    help="sweep: only the roofline kernels (form factors, SpMM sweeps, copy calibration)")
     return ap.parse_args()
 
 
@@ -119,6 +121,38 @@ def cpu_baseline(eng, scene, windows, meteo, cfg, nb_total: int, budget_s: float
     return scene.facet_count * nb_total / dt, dt, f"1 converged daytime node (hour {met.hour:.1f}, {it} SW sweeps)"
 
 
+def cpu_baseline_sampled(eng, scene, windows, nb_total: int, st_plain):
+    """For graphs too large to converge a node on one core within the bench:
+    the oracle (1 core) times one full-scene FP64 sweep of all shortwave
+    bands and one longwave sweep; a node is then priced with the plain GPU
+    run's sweep counts per node.  Returns facet-band-steps/s."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_ref as O
+
+    g = eng.getGraph()
+    og = O.Graph(g.rowOffsets, g.neighbors, g.counts, g.denominators, g.skyCounts)
+    S = scene.surface_count
+    nb = windows.count
+    rng = np.random.default_rng(0)
+    r = rng.uniform(0.0, 0.4, (S, nb)).astype(np.float32)
+    t = (0.5 * r).astype(np.float32)
+    z = np.zeros((S, nb), np.float32)
+    D = rng.uniform(0.0, 300.0, (S, nb)).astype(np.float32)
+    sky = np.full(nb, 20.0, np.float32)
+    t0 = time.perf_counter()
+    O.solve_bands(og, r, t, z, D, sky, max_iter=1, tol_rel=-1.0)
+    t_sw = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.solve_bands(og, r[:, :1], t[:, :1], z[:, :1], D[:, :1], sky[:1], max_iter=1, tol_rel=-1.0)
+    t_lw = time.perf_counter() - t0
+    nodes = st_plain.nodes
+    per_node = (sum(st_plain.sw_sweeps) * t_sw + sum(st_plain.lw_sweeps) * t_lw) / nodes
+    sample = (f"1 FP64 sweep of {nb} bands ({t_sw:.1f}s) + 1 LW sweep ({t_lw:.1f}s) over the full graph on 1 core, "
+              f"priced at the plain run's {sum(st_plain.sw_sweeps) / nodes:.1f} SW + "
+              f"{sum(st_plain.lw_sweeps) / nodes:.1f} LW sweeps per node")
+    return scene.facet_count * nb_total / per_node, t_sw + t_lw, sample
+
+
 def load_surrogate(eng, path: str) -> bool:
     if not os.path.exists(path):
         return False
@@ -158,6 +192,20 @@ def main() -> None:
 
     windows = faceteb.ShortwaveWindows.for_bands(wl["sw_bands"])
     nb_total = windows.count + 1
+    if a.mode == "sweep":
+        # counter runs: the SW sweep in FP32 and BF16, the LW sweep, and a
+        # 1 GiB copy as the byte calibration of the traffic counters
+        nb = windows.count
+        out = {"workload": a.workload, "sides": S, "nnz": diag.directedEdgeCount}
+        for name, nbk, prec in (("sw_fp32", nb, FH_PRECISION_FP32), ("sw_bf16", nb, FH_PRECISION_BF16_DC),
+                                ("lw_fp32", 1, FH_PRECISION_FP32)):
+            sec = time_sweep(eng, nbk, prec, a.roofline_reps)
+            alg = sweep_bytes(S, diag.directedEdgeCount, nbk, 4 if prec == FH_PRECISION_FP32 else 2)
+            out[name] = {"us": sec * 1e6, "alg_bytes": alg, "GBps": alg / sec / 1e9}
+        cs = eng.benchmarkCopy(1 << 30, 20)
+        out["copy_1GiB"] = {"us": cs * 1e6, "GBps": 2.0 * (1 << 30) / cs / 1e9}
+        print(json.dumps(out), flush=True)
+        return
     faceteb.setup_scene(eng, scene, windows)
     cfg = faceteb.EbConfig()
     meteo = faceteb.diurnal_meteo(24)
@@ -207,7 +255,10 @@ def main() -> None:
 
     cpu = None
     if not a.no_cpu_baseline:
-        v, dt, sample = cpu_baseline(eng, scene, windows, meteo, cfg, nb_total)
+        if nnz <= 50_000_000:
+            v, dt, sample = cpu_baseline(eng, scene, windows, meteo, cfg, nb_total)
+        else:
+            v, dt, sample = cpu_baseline_sampled(eng, scene, windows, nb_total, st_plain)
         cpu = {"value": v, "unit": "facet-band-steps/s", "cores": 1, "kind": "port",
                "sample": f"{sample}; {dt:.1f}s on 1 core"}
         log(f"[bench] cpu baseline {v:.4g} facet-band-steps/s ({dt:.1f}s)")

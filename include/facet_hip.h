@@ -184,6 +184,10 @@ void *fh_stream(const fh_context *ctx);
  * current graph with nb bands, on engine-owned HBM inputs. */
 int fh_benchmark_sweep(fh_context *ctx, uint32_t nb, uint32_t precision, uint32_t reps,
                        float *microsecondsPerSweep);
+/* Average duration (microseconds) of a plain device-to-device copy kernel of
+ * `bytes` bytes (reads `bytes`, writes `bytes`) on the context's stream: the
+ * calibration point for hardware traffic counters and achievable bandwidth. */
+int fh_benchmark_copy(fh_context *ctx, uint64_t bytes, uint32_t reps, float *microsecondsPerCopy);
 /* Waits for all work queued on the context's stream. */
 int fh_synchronize(fh_context *ctx);
 

@@ -571,6 +571,38 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
     });
 }
 
+__global__ void k_copy_calibration(const float4* __restrict__ a, float4* __restrict__ b, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        b[i] = a[i];
+}
+
+int fh_benchmark_copy(fh_context* c, uint64_t bytes, uint32_t reps, float* us) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        if (bytes < 16u || reps == 0u || !us) fail(FH_INVALID_ARGUMENT, "invalid benchmark arguments");
+        const size_t n = size_t(bytes / 16u);
+        fh::DBuf<float4> a, b;
+        a.ensure(n);
+        b.ensure(n);
+        fh::launch_fill_f32(c, reinterpret_cast<float*>(a.p), n * 4u, 1.0f);
+        const dim3 grid(2048), block(256);
+        hipLaunchKernelGGL(k_copy_calibration, grid, block, 0, c->stream, a.p, b.p, n);
+        hipEvent_t e0, e1;
+        FH_HIP(hipEventCreate(&e0));
+        FH_HIP(hipEventCreate(&e1));
+        FH_HIP(hipEventRecord(e0, c->stream));
+        for (uint32_t i = 0; i < reps; ++i)
+            hipLaunchKernelGGL(k_copy_calibration, grid, block, 0, c->stream, a.p, b.p, n);
+        FH_HIP(hipEventRecord(e1, c->stream));
+        FH_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        FH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *us = ms * 1000.0f / float(reps);
+    });
+}
+
 int fh_synchronize(fh_context* c) {
     if (!c) return FH_INVALID_ARGUMENT;
     return guarded(c, [&] { FH_HIP(hipStreamSynchronize(c->stream)); });

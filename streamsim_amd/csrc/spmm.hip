@@ -64,6 +64,7 @@ struct SweepArgs {
     float* buf1;  // iter = sweeps executed so far in this solve (device side)
     SolveCtrl* ctrl;
     float2* partial;  // per-block (max |dB|, max |dB|/|B|)
+    uint32_t xcd;     // 1: XCD-contiguous block order (xcd_block)
 };
 
 template <int NBC, int TEAM, int MODE, bool W16>
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
         Bout = (it & 1u) ? a.buf0 : a.buf1;
     }
     const int lane = threadIdx.x % TEAM;
-    const uint32_t f = (blockIdx.x * 256u + threadIdx.x) / TEAM;
+    const uint32_t f = (xcd_block(a.xcd) * 256u + threadIdx.x) / TEAM;
     float myAbs = 0.0f, myRel = 0.0f;
     if (f < a.F) {
         const uint32_t s0 = 2u * f;
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(256) void k_sweep_bp(SweepArgs a) {
         Bout = (it & 1u) ? a.buf0 : a.buf1;
     }
     const int lane = threadIdx.x % BT;
-    const uint32_t f = (blockIdx.x * 256u + threadIdx.x) / BT;
+    const uint32_t f = (xcd_block(a.xcd) * 256u + threadIdx.x) / BT;
     const uint32_t ldb = a.ldb;
     float myAbs = 0.0f, myRel = 0.0f;
     if (f < a.F) {
@@ -457,7 +458,17 @@ static bool band_parallel(uint32_t ldb) {
     return ldb >= 16u;
 }
 
+// FH_XCD_REMAP=0|1 forces the block order (tuning knob)
+static uint32_t xcd_remap() {
+    static const uint32_t v = [] {
+        const char* e = std::getenv("FH_XCD_REMAP");
+        return e ? uint32_t(std::atoi(e) != 0) : 0u;
+    }();
+    return v;
+}
+
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
+    a.xcd = xcd_remap();
     if (band_parallel(a.ldb)) {
         int bt = 16, kb = 1;
         pick_bp(a.ldb, bt, kb);

@@ -7,6 +7,17 @@
 
 namespace fh {
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  With remap set, hardware block b runs logical block xcd_block(), so
+// each XCD works through one contiguous range of rows and the columns that
+// neighbouring rows gather stay in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t remap) {
+    const uint32_t b = blockIdx.x;
+    if (!remap) return b;
+    const uint32_t G = gridDim.x, x = b & 7u, l = b >> 3, q = G >> 3, rem = G & 7u;
+    return x < rem ? x * (q + 1u) + l : rem * (q + 1u) + (x - rem) * q + l;
+}
+
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 template <int NBC>

@@ -331,5 +331,11 @@ class FacetEngine:
         self._check(self._lib.fh_benchmark_sweep(self._h, nb, precision, reps, C.byref(us)))
         return us.value * 1e-6
 
+    def benchmarkCopy(self, nbytes: int, reps: int = 20) -> float:
+        """Average duration in seconds of a plain copy of nbytes (read + write)."""
+        us = C.c_float(0.0)
+        self._check(self._lib.fh_benchmark_copy(self._h, nbytes, reps, C.byref(us)))
+        return us.value * 1e-6
+
     def synchronize(self) -> None:
         self._check(self._lib.fh_synchronize(self._h))
