@@ -91,6 +91,19 @@ void download(fh_context* c, T* dst, const T* src, size_t n) {
     FH_HIP(hipStreamSynchronize(c->stream));
 }
 
+// CSR entry arrays with kCsrPad zero entries past nnz (aligned group loads)
+void alloc_csr(fh_context* c, uint32_t nnz) {
+    const size_t n = size_t(nnz) + fh::kCsrPad;
+    c->col.ensure(n);
+    c->cnt.ensure(n);
+    c->val32.ensure(n);
+    c->val16.ensure(n);
+    FH_HIP(hipMemsetAsync(c->col.p + nnz, 0, fh::kCsrPad * sizeof(uint32_t), c->stream));
+    FH_HIP(hipMemsetAsync(c->cnt.p + nnz, 0, fh::kCsrPad * sizeof(uint32_t), c->stream));
+    FH_HIP(hipMemsetAsync(c->val32.p + nnz, 0, fh::kCsrPad * sizeof(float), c->stream));
+    FH_HIP(hipMemsetAsync(c->val16.p + nnz, 0, fh::kCsrPad * sizeof(uint16_t), c->stream));
+}
+
 // Shared tail of fh_solve / fh_solve_accelerated: optics AoS -> r,t,e,D;
 // fold; Jacobi; download.
 void solve_single(fh_context* c, const fh_surface_optics* optics, float sky, const float* init,
@@ -276,10 +289,7 @@ int fh_build_form_factors(fh_context* c, fh_graph_diagnostics* out) {
         uint32_t nnz = 0;
         download(c, &nnz, c->rowptr.p + S, 1);
         c->nnz = nnz;
-        c->col.ensure(nnz);
-        c->cnt.ensure(nnz);
-        c->val32.ensure(nnz);
-        c->val16.ensure(nnz);
+        alloc_csr(c, nnz);
         c->fsky.ensure(S);
         fh::launch_compact(c, R);
         c->misc.ensure(4);
@@ -322,12 +332,9 @@ int fh_set_graph(fh_context* c, uint32_t nnz, const uint32_t* rowOffsets, const 
         c->hasGraph = false;
         c->nnz = nnz;
         c->rowptr.ensure(size_t(S) + 1u);
-        c->col.ensure(nnz);
-        c->cnt.ensure(nnz);
+        alloc_csr(c, nnz);
         c->den.ensure(S);
         c->sky.ensure(S);
-        c->val32.ensure(nnz);
-        c->val16.ensure(nnz);
         c->fsky.ensure(S);
         upload(c, c->rowptr.p, rowOffsets, size_t(S) + 1u);
         upload(c, c->col.p, neighbors, nnz);
@@ -554,13 +561,25 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
         fh::launch_fill_f32(c, B0, n, 50.0f);
         fh::launch_fill_f32(c, B1, n, 0.0f);
         const bool w16 = precision == FH_PRECISION_BF16_DC;
-        for (int i = 0; i < 3; ++i) fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16);
+        // BF16 solves of band-parallel strides gather BF16 copies of B
+        fh::DBuf<uint16_t> h16;
+        uint16_t* H0 = nullptr;
+        uint16_t* H1 = nullptr;
+        if (w16 && fh::uses_bf16_gather(ldb)) {
+            const size_t nh = size_t(c->S) * fh::bf16_stride(ldb);
+            h16.ensure(2 * nh);
+            H0 = h16.p;
+            H1 = h16.p + nh;
+            fh::launch_fill_u32(c, reinterpret_cast<uint32_t*>(h16.p), nh, 0x42484248u);  // 50.0 in BF16
+        }
+        for (int i = 0; i < 3; ++i) fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1);
         hipEvent_t e0, e1;
         FH_HIP(hipEventCreate(&e0));
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c->stream));
         for (uint32_t i = 0; i < reps; ++i)
-            fh::launch_single_sweep(c, ldb, cc, rr, tt, (i & 1u) ? B1 : B0, (i & 1u) ? B0 : B1, w16);
+            fh::launch_single_sweep(c, ldb, cc, rr, tt, (i & 1u) ? B1 : B0, (i & 1u) ? B0 : B1, w16,
+                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1);
         FH_HIP(hipEventRecord(e1, c->stream));
         FH_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

@@ -15,7 +15,10 @@
 // scene-level energy sums in FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
+#include <cstdlib>
 
 #include "solver.hpp"
 #include "sweep_common.hpp"
@@ -98,10 +101,12 @@ __global__ void k_lw_source(uint32_t S, const float* __restrict__ T, const float
 
 // SW window optics per side and band (makeEnergyShortwaveOptics,
 // core:952-1003, angular strength 0, no reflectance texture)
+// Internal band b is window order[b]; dni is given in internal order.
 __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_t* __restrict__ mat,
                             const float* __restrict__ refl, const float* __restrict__ trans,
-                            const float4* __restrict__ nrm, const float* __restrict__ sunlit, float sx, float sy,
-                            float sz, const float* __restrict__ dni, float* __restrict__ r, float* __restrict__ t,
+                            const uint32_t* __restrict__ order, const float4* __restrict__ nrm,
+                            const float* __restrict__ sunlit, float sx, float sy, float sz,
+                            const float* __restrict__ dni, float* __restrict__ r, float* __restrict__ t,
                             float* __restrict__ D) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(S) * ldb) return;
@@ -112,9 +117,9 @@ __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_
         D[i] = 0.0f;
         return;
     }
-    const uint32_t m = mat[s >> 1];
-    const float rr = fminf(fmaxf(refl[size_t(m) * nb + b], 0.0f), 1.0f);
-    const float tt = fminf(fmaxf(trans[size_t(m) * nb + b], 0.0f), 1.0f - rr);
+    const uint32_t m = mat[s >> 1], w = order[b];
+    const float rr = fminf(fmaxf(refl[size_t(m) * nb + w], 0.0f), 1.0f);
+    const float tt = fminf(fmaxf(trans[size_t(m) * nb + w], 0.0f), 1.0f - rr);
     r[i] = rr;
     t[i] = fminf(tt, 1.0f - rr);
     const float4 n = nrm[s >> 1];
@@ -125,26 +130,52 @@ __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_
 
 // absorbed SW = sum_b alpha H, incident = sum_b H; FP64 block partials of
 // sum A (D + fsky S), sum A alpha H, sum A fsky B for the energy closure.
+// A team of 16 lanes owns one side, each lane 4 contiguous bands per pass
+// (16-byte loads of the [S][ldb] rows), summed over the team by shuffles.
 __global__ __launch_bounds__(256) void k_absorb(uint32_t S, uint32_t nb, uint32_t ldb, const float* __restrict__ r,
                                                 const float* __restrict__ t, const float* __restrict__ H,
                                                 const float* __restrict__ D, const float* __restrict__ B,
                                                 const float* __restrict__ fsky, const float* __restrict__ skyv,
                                                 const double* __restrict__ area, float* __restrict__ absorbed,
                                                 float* __restrict__ incident, double* __restrict__ part) {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t team = threadIdx.x >> 4, lane = threadIdx.x & 15u;
+    const uint32_t s = blockIdx.x * 16u + team;
     double pin = 0.0, pabs = 0.0, pesc = 0.0;
+    float a = 0.0f, h = 0.0f;
     if (s < S) {
-        float a = 0.0f, h = 0.0f;
         const double A = area[s >> 1];
-        for (uint32_t b = 0; b < nb; ++b) {
-            const size_t i = size_t(s) * ldb + b;
-            const float alpha = fminf(fmaxf(1.0f - r[i] - t[i], 0.0f), 1.0f);
-            a += alpha * H[i];
-            h += H[i];
-            pin += A * (double(D[i]) + double(fsky[s]) * skyv[b]);
-            pabs += A * double(alpha) * H[i];
-            pesc += A * double(fsky[s]) * B[i];
+        const float fs = fsky[s];
+        // ldb is 1 or a multiple of 4 (band_stride); the single band is lane 0's
+        const bool vec = ldb >= 4u;
+        for (uint32_t b0 = 4u * lane; b0 < nb; b0 += 64u) {
+            const size_t i = size_t(s) * ldb + b0;
+            const float4 R4 = vec ? *reinterpret_cast<const float4*>(r + i) : make_float4(r[i], 0.f, 0.f, 0.f);
+            const float4 T4 = vec ? *reinterpret_cast<const float4*>(t + i) : make_float4(t[i], 0.f, 0.f, 0.f);
+            const float4 H4 = vec ? *reinterpret_cast<const float4*>(H + i) : make_float4(H[i], 0.f, 0.f, 0.f);
+            const float4 D4 = vec ? *reinterpret_cast<const float4*>(D + i) : make_float4(D[i], 0.f, 0.f, 0.f);
+            const float4 B4 = vec ? *reinterpret_cast<const float4*>(B + i) : make_float4(B[i], 0.f, 0.f, 0.f);
+            const float rr[4] = {R4.x, R4.y, R4.z, R4.w}, tt[4] = {T4.x, T4.y, T4.z, T4.w};
+            const float hh[4] = {H4.x, H4.y, H4.z, H4.w}, dd[4] = {D4.x, D4.y, D4.z, D4.w};
+            const float bb[4] = {B4.x, B4.y, B4.z, B4.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t b = b0 + uint32_t(j);
+                if (b >= nb) break;
+                const float alpha = fminf(fmaxf(1.0f - rr[j] - tt[j], 0.0f), 1.0f);
+                a += alpha * hh[j];
+                h += hh[j];
+                pin += A * (double(dd[j]) + double(fs) * skyv[b]);
+                pabs += A * double(alpha) * hh[j];
+                pesc += A * double(fs) * bb[j];
+            }
         }
+    }
+#pragma unroll
+    for (int m = 8; m > 0; m >>= 1) {
+        a += __shfl_xor(a, m, 16);
+        h += __shfl_xor(h, m, 16);
+    }
+    if (lane == 0 && s < S) {
         absorbed[s] = a;
         incident[s] = h;
     }
@@ -340,10 +371,8 @@ __global__ __launch_bounds__(256) void k_coupled(CoupledArgs a) {
         const uint32_t s0 = 2u * f;
         const uint32_t e0 = a.rowptr[s0], e1 = a.rowptr[s0 + 1], e2 = a.rowptr[s0 + 2];
         float g[2][1] = {{0.0f}, {0.0f}};
-        for (uint32_t e = e0 + lane; e < e1; e += TEAM)
-            gather<1>(a.Bin + a.col[e], W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g[0]);
-        for (uint32_t e = e1 + lane; e < e2; e += TEAM)
-            gather<1>(a.Bin + a.col[e], W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g[1]);
+        team_rows_dot2<TEAM, (TEAM >= 32 ? 4 : 8), W16>(a.col, a.w32, a.w16, a.Bin, e0, e1, e1, e2, lane, g[0][0],
+                                                        g[1][0]);
         team_reduce<1, TEAM>(g[0]);
         team_reduce<1, TEAM>(g[1]);
         if (lane == 0) {
@@ -484,6 +513,23 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
         FH_HIP(hipMemcpyAsync(e.swRefl.p, refl, sizeof(float) * nMat * swNb, hipMemcpyHostToDevice, c->stream));
         FH_HIP(hipMemcpyAsync(e.swTrans.p, trans, sizeof(float) * nMat * swNb, hipMemcpyHostToDevice, c->stream));
     }
+    // internal band order: windows by decreasing largest single-scattering
+    // albedo r + t over the materials (the slowest to converge first), so
+    // the bands a solve still iterates form a prefix
+    std::vector<float> key(swNb, 0.0f);
+    for (uint32_t b = 0; b < swNb; ++b)
+        for (uint32_t m = 0; m < nMat; ++m) {
+            const float r = std::min(std::max(refl[size_t(m) * swNb + b], 0.0f), 1.0f);
+            const float t = std::min(std::max(trans[size_t(m) * swNb + b], 0.0f), 1.0f - r);
+            key[b] = std::max(key[b], r + t);
+        }
+    e.swOrder.resize(swNb);
+    for (uint32_t b = 0; b < swNb; ++b) e.swOrder[b] = b;
+    std::stable_sort(e.swOrder.begin(), e.swOrder.end(), [&](uint32_t x, uint32_t y) { return key[x] > key[y]; });
+    e.swOrderD.ensure(swNb ? swNb : 1u);
+    if (swNb)
+        FH_HIP(hipMemcpyAsync(e.swOrderD.p, e.swOrder.data(), sizeof(uint32_t) * swNb, hipMemcpyHostToDevice,
+                              c->stream));
     for (DBuf<float>* b : {&e.T, &e.Tprev, &e.absorbed, &e.incident, &e.sunlit, &e.rlw, &e.tlw, &e.eps, &e.elw,
                            &e.clw, &e.Hlw, &e.zero, &e.Blw0, &e.Blw1, &e.Rn, &e.Hs, &e.LE, &e.G})
         b->ensure(S);
@@ -505,6 +551,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     FH_HIP(hipGetLastError());
     FH_HIP(hipStreamSynchronize(c->stream));
     e.swValid = e.lwValid = e.tValid = false;
+    e.swRhoValid = false;
     e.ready = true;
 }
 
@@ -631,15 +678,25 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
     }
     if (e.swNb && swTotal > 0.0) {
         const uint32_t nb = e.swNb, ldb = e.swLdb;
-        std::vector<float> dpad(ldb, 0.0f);
-        for (uint32_t b = 0; b < nb; ++b) dpad[b] = p.sun.y > 0.0f ? dniHost[b] : 0.0f;
+        // per-band inputs in the internal band order
+        std::vector<float> dpad(ldb, 0.0f), diffInt(nb);
+        for (uint32_t b = 0; b < nb; ++b) {
+            dpad[b] = p.sun.y > 0.0f ? dniHost[e.swOrder[b]] : 0.0f;
+            diffInt[b] = diffHost[e.swOrder[b]];
+        }
         FH_HIP(hipMemcpyAsync(e.dni.p, dpad.data(), sizeof(float) * ldb, hipMemcpyHostToDevice, c->stream));
         const size_t n = size_t(S) * ldb;
         hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p, e.swRefl.p,
-                           e.swTrans.p, c->facetNormal.p, e.sunlit.p, p.sun.x, p.sun.y, p.sun.z, e.dni.p, e.rsw.p,
-                           e.tsw.p, e.Dsw.p);
+                           e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, p.sun.x, p.sun.y, p.sun.z,
+                           e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p);
         FH_HIP(hipGetLastError());
-        set_sky(c, diffHost, nb, ldb);
+        set_sky(c, diffInt.data(), nb, ldb);
+        if (!e.swRhoValid) {
+            // spectral radius per band of this scene's SW operator (Chebyshev)
+            e.swRho2.ensure(ldb);
+            estimate_rho2(c, ldb, nb, e.rsw.p, e.tsw.p, e.Bsw1.p, e.Hsw.p, e.csw.p, e.swRho2.p);
+            e.swRhoValid = true;
+        }
         launch_fill_f32(c, e.Hsw.p, n, 0.0f);  // emission is zero in the SW
         launch_fold(c, e.Hsw.p, e.rsw.p, e.tsw.p, e.Dsw.p, ldb, e.csw.p);
         if (!e.swValid || p.zeroStart) launch_fill_f32(c, e.Bsw0.p, n, 0.0f);
@@ -648,14 +705,22 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         o.precision = p.precision;
         o.maxIter = p.maxSweeps;
         o.tolRel = p.radiosityTolerance;
+        o.H = e.Hsw.p;
+        o.D = e.Dsw.p;
+        static const bool noCheb = [] {
+            const char* v = std::getenv("FH_NO_CHEBYSHEV");
+            return v && std::atoi(v) != 0;
+        }();
+        if (!noCheb) o.rho2 = e.swRho2.p;
         float* bufs[2] = {e.Bsw0.p, e.Bsw1.p};
         fh_solve_stats ss{};
-        const int fin = run_jacobi(c, o, e.csw.p, e.rsw.p, e.tsw.p, bufs, &ss);
+        bool hDone = false;
+        const int fin = run_jacobi(c, o, e.csw.p, e.rsw.p, e.tsw.p, bufs, &ss, &hDone);
         if (fin == 1) e.Bsw0.swap(e.Bsw1);
         st.swSweeps = ss.iterations;
         st.swRefreshes = ss.refreshes;
-        run_incident(c, ldb, e.Bsw0.p, e.Dsw.p, e.Hsw.p);
-        const uint32_t nbk = (S + 255u) / 256u;
+        if (!hDone) run_incident(c, ldb, e.Bsw0.p, e.Dsw.p, e.Hsw.p);
+        const uint32_t nbk = (S + 15u) / 16u;  // 16 sides per block
         e.dpart.ensure(size_t(nbk) * 4u);
         hipLaunchKernelGGL(k_absorb, dim3(nbk), dim3(256), 0, c->stream, S, nb, ldb, e.rsw.p, e.tsw.p, e.Hsw.p,
                            e.Dsw.p, e.Bsw0.p, c->fsky.p, c->skyv.p, e.area.p, e.absorbed.p, e.incident.p, e.dpart.p);
@@ -696,7 +761,15 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
     // --- coupling: fused Jacobi-Newton sweeps, FP32 refinement, final check ---
     if (!e.lwValid || p.zeroStart) st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
     float last[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    const bool bf16 = p.precision == FH_PRECISION_BF16_DC;
+    // The single-band LW sweep is bound by gather latency, not bytes: BF16
+    // form factors only add coupling iterations there, so FH_PRECISION_BF16_DC
+    // applies to the multi-band SW operator and the coupling runs in FP32
+    // (FH_LW_BF16=1 forces BF16 coupled sweeps, a tuning knob).
+    static const bool lwBf16 = [] {
+        const char* v = std::getenv("FH_LW_BF16");
+        return v && std::atoi(v) != 0;
+    }();
+    const bool bf16 = p.precision == FH_PRECISION_BF16_DC && lwBf16;
     uint32_t iters = coupled_loop(c, p, k, bf16, std::max(1u, p.maxCoupling), last);
     float maxDT = last[2];
     // final LW solve with the converged temperatures and the residual check
@@ -812,11 +885,16 @@ int fh_eb_get_state(fh_context* c, float* T, float* absorbed, float* Blw, float*
         get(LE, e.LE.p, S);
         get(G, e.G.p, S);
         if (swRadiosity && e.swNb) {
+            // [S][swNb] in window order (internal band b is window swOrder[b])
+            const uint32_t nb = e.swNb;
             fh::DBuf<float> tmp;
-            tmp.ensure(S * e.swNb);
-            fh::launch_unpad(c, e.Bsw0.p, e.swNb, e.swLdb, tmp.p);
-            get(swRadiosity, tmp.p, S * e.swNb);
+            tmp.ensure(S * nb);
+            fh::launch_unpad(c, e.Bsw0.p, nb, e.swLdb, tmp.p);
+            std::vector<float> h(S * nb);
+            FH_HIP(hipMemcpyAsync(h.data(), tmp.p, h.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
             FH_HIP(hipStreamSynchronize(c->stream));
+            for (size_t s = 0; s < S; ++s)
+                for (uint32_t b = 0; b < nb; ++b) swRadiosity[s * nb + e.swOrder[b]] = h[s * nb + b];
         }
         FH_HIP(hipStreamSynchronize(c->stream));
     });

@@ -95,11 +95,21 @@ struct SolveCtrl {
     float lastAbs;
     float lastRel;
     uint32_t refreshes;
-    uint32_t innerDone;   // BF16 inner sweeps reached tolRel: skip until the next refresh
-    uint32_t pad[6];
+    uint32_t innerDone;   // BF16 inner sweeps reached innerTol: skip until the next refresh
+    uint32_t nAct;        // band retirement: bands [0, nAct) are still iterated
+    uint32_t nActPrev;    // bands [nAct, nActPrev) retired by the last judge: copied once
+    float innerTol;       // relative change that ends a group of BF16 sweeps
+    uint32_t chebStart;   // sweep index that started the current Chebyshev sequence
+    uint32_t pad[2];
 };
+static_assert(sizeof(SolveCtrl) == 64, "control block is one 64-byte line");
 
 uint32_t band_stride(uint32_t nb);
+
+// CSR column / weight arrays are allocated kCsrPad entries beyond nnz (zero)
+// so that the sweep kernels load aligned groups of 8 entries with vector
+// loads; the row offsets need no alignment.
+constexpr uint32_t kCsrPad = 16;
 
 // FacetEB device state (one per context).
 struct EbState {
@@ -120,6 +130,13 @@ struct EbState {
     DBuf<float> Rn, Hs, LE, G;
     // SW [S][swLdb]
     DBuf<float> Bsw0, Bsw1, rsw, tsw, Dsw, csw, Hsw, dni;
+    // internal SW band j is window swOrder[j]: windows sorted by the largest
+    // r + t over the materials (slowest to converge first), so the bands
+    // still iterating form a prefix
+    std::vector<uint32_t> swOrder;
+    DBuf<uint32_t> swOrderD;
+    DBuf<float> swRho2;            // [swLdb] spectral radius^2 per internal band
+    bool swRhoValid = false;
     DBuf<double> dpart;            // per-block FP64 partial sums
     DBuf<double> dsum;             // reduced FP64 sums
     DBuf<float2> fout;             // reduced (max dT, max residual-equivalent)
@@ -183,6 +200,10 @@ struct fh_context {
     fh::DBuf<float> B0, B1, cc, rr, tt, DD, c2, HH, skyv;
     fh::DBuf<fh::SolveCtrl> ctrl;
     fh::DBuf<float2> partial;        // per-block residual partials
+    fh::DBuf<uint32_t> partAct;      // per-block 1 + highest unconverged band
+    fh::DBuf<uint16_t> X16a, X16b;   // BF16 copies of the radiosity gathered by BF16 sweeps
+    fh::DBuf<float> omega;           // Chebyshev weights per band
+    fh::DBuf<uint32_t> bandMax;      // per-band maxima (float bits) of the rho estimate
     fh::SolveCtrl* hctrl = nullptr;  // pinned mirror
     fh::DBuf<uint32_t> misc;         // small counters
     fh::DBuf<float> fbuf;            // small float scratch

@@ -19,6 +19,7 @@
 #include <cfloat>
 
 #include "engine.hpp"
+#include "sweep_common.hpp"
 
 namespace fh {
 
@@ -416,12 +417,6 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, uint32_t n, const uint32_
 }
 
 // ---- CSR fill: one wavefront per side --------------------------------------
-__device__ __forceinline__ uint16_t f32_to_bf16(float x) {
-    const uint32_t u = __float_as_uint(x);
-    const uint32_t r = u + 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
-    return uint16_t(r >> 16);
-}
-
 __global__ __launch_bounds__(256) void k_compact(const uint32_t* __restrict__ hits, const uint16_t* __restrict__ cnt16,
                                                  const uint32_t* __restrict__ rowptr, uint32_t S, uint32_t R,
                                                  uint32_t* __restrict__ col, uint32_t* __restrict__ cnt,

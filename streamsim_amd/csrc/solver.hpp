@@ -22,7 +22,8 @@ void launch_fold(fh_context* c, const float* e, const float* r, const float* t, 
 void launch_pad(fh_context* c, const float* src, uint32_t nb, uint32_t ldb, float* dst);
 void launch_unpad(fh_context* c, const float* src, uint32_t nb, uint32_t ldb, float* dst);
 void launch_split_optics(fh_context* c, const float4* o, float* r, float* t, float* e, float* D);
-void launch_ctrl_reset(fh_context* c, uint32_t maxIter, float tolRel, float tolAbs, uint32_t checkEvery);
+void launch_ctrl_reset(fh_context* c, uint32_t maxIter, float tolRel, float tolAbs, uint32_t checkEvery,
+                       uint32_t nAct = 1u, float innerTol = -1.0f);
 void launch_fill_u32(fh_context* c, uint32_t* p, size_t n, uint32_t v);
 void launch_fill_f32(fh_context* c, float* p, size_t n, float v);
 
@@ -37,21 +38,47 @@ struct JacobiOptions {
     float tolRel = -1.0f;
     float tolAbs = -1.0f;
     uint32_t checkEvery = 1;
-    uint32_t innerBf16 = 12;  // most BF16 sweeps between FP32 refreshes
+    uint32_t innerBf16 = 12;  // most BF16 sweeps between FP32 refreshes / before the FP32 phase
+    // BF16-gather solves (band-parallel layouts): the BF16 phase ends at
+    // relative change firstInner (the BF16 operator's fixed point is only
+    // about that close), then FP32 sweeps finish the solve
+    float firstInner = 1.0e-2f;
+    bool retire = true;       // per-band retirement (tolRel >= 0, band-parallel)
+    // optional incident H = max(0, fsky S + G + D) written by the judge
+    // sweeps (band-parallel with retirement); *incidentDone reports whether
+    // every band converged, i.e. H is complete
+    float* H = nullptr;
+    const float* D = nullptr;
+    // squared spectral radius per band (device, ldb): plain FP32 band-parallel
+    // solves then run the Chebyshev semi-iteration for [-rho_b, rho_b]
+    const float* rho2 = nullptr;
 };
+
+// Per-band spectral radius squared of the SW operator (r, t [S][ldb]) by
+// power iteration from B = 1: max(B_m+2) / max(B_m) per band into rho2
+// (device, ldb).  Uses scratch arrays v0, v1, zero ([S][ldb]).
+void estimate_rho2(fh_context* c, uint32_t ldb, uint32_t nb, const float* rr, const float* tt, float* v0, float* v1,
+                   float* zero, float* rho2);
 int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const float* rr, const float* tt,
-               float* bufs[2], fh_solve_stats* stats);
+               float* bufs[2], fh_solve_stats* stats, bool* incidentDone = nullptr);
 
 // H = max(0, fsky S + G(B) + D) for all bands ([S][ldb])
 void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, float* H);
 
 void set_sky(fh_context* c, const float* sky, uint32_t nb, uint32_t ldb);
 
+// true when sweeps of this band stride use the band-parallel kernels (BF16
+// solves then gather BF16 copies of B with row stride bf16_stride(ldb))
+bool uses_bf16_gather(uint32_t ldb);
+uint32_t bf16_stride(uint32_t ldb);
+
 // lanes per facet team of the SpMM kernels for the current graph
 int sweep_team(const fh_context* c);
 
-// One sweep, no host synchronisation (timing entry).
+// One sweep, no host synchronisation (timing entry).  With w16 and BF16
+// copies given (band-parallel strides), the BF16-gather sweep h16in -> h16out.
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
-                         const float* Bin, float* Bout, bool w16);
+                         const float* Bin, float* Bout, bool w16, uint16_t* h16in = nullptr,
+                         uint16_t* h16out = nullptr);
 
 }  // namespace fh

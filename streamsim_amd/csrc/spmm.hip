@@ -27,8 +27,12 @@
 // iteration count and raises `done`, after which queued sweeps exit at once.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "solver.hpp"
 #include "sweep_common.hpp"
@@ -65,6 +69,22 @@ struct SweepArgs {
     SolveCtrl* ctrl;
     float2* partial;  // per-block (max |dB|, max |dB|/|B|)
     uint32_t xcd;     // 1: XCD-contiguous block order (xcd_block)
+    // band retirement (band-parallel kernels): bands whose relative change
+    // in a judge sweep is within tolRel stop iterating; ctrl->nAct bounds
+    // the bands still iterated, partAct[block] = 1 + highest band above tol
+    uint32_t retire;
+    uint32_t* partAct;
+    // BF16 copies of B (row stride ldh, a multiple of 64 bands = 128 bytes):
+    // BF16 sweeps gather Bin16; sweeps of a BF16 solve also write Bout16.
+    // Ping-pong between h0 / h1 by ctrl->iter like buf0 / buf1.
+    uint16_t* h0;
+    uint16_t* h1;
+    uint32_t ldh;
+    uint32_t qRound;  // REFRESH: the BF16 sum uses round_bf16(B) (matches Bin16)
+    uint32_t bt;      // band-parallel kernels: lanes per facet team
+    uint32_t fuseH;   // judge sweeps also write H = max(0, fsky S + G + D) (band-parallel)
+    const float* omega;  // Chebyshev weights per band for this sweep (band-parallel FP32), or null
+    const float* rho2;   // squared spectral radius estimate per band (Chebyshev)
 };
 
 template <int NBC, int TEAM, int MODE, bool W16>
@@ -93,6 +113,16 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
             float q0[NBC], q1[NBC];  // BF16 sums in REFRESH mode
 #pragma unroll
             for (int j = 0; j < NBC; ++j) g0[j] = g1[j] = q0[j] = q1[j] = 0.0f;
+            if constexpr (NBC == 1) {
+                // single band: both rows with U entries per lane in flight
+                constexpr int U = TEAM >= 32 ? 4 : 8;
+                if (MODE == MODE_REFRESH) {
+                    team_rows_dot2<TEAM, U, false>(a.col, a.w32, a.w16, Bin, e0, e1, e1, e2, lane, g0[0], g1[0]);
+                    team_rows_dot2<TEAM, U, true>(a.col, a.w32, a.w16, Bin, e0, e1, e1, e2, lane, q0[0], q1[0]);
+                } else {
+                    team_rows_dot2<TEAM, U, W16>(a.col, a.w32, a.w16, Bin, e0, e1, e1, e2, lane, g0[0], g1[0]);
+                }
+            } else {
             for (uint32_t e = e0 + lane; e < e1; e += TEAM) {
                 const uint32_t n = a.col[e];
                 const float* p = Bin + size_t(n) * a.ldb + b0;
@@ -112,6 +142,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
                 } else {
                     gather<NBC>(p, W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g1);
                 }
+            }
             }
             team_reduce<NBC, TEAM>(g0);
             team_reduce<NBC, TEAM>(g1);
@@ -175,150 +206,434 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     }
 }
 
-// Band-parallel variant for multi-band B (ldb >= 4): a team of BT lanes owns
-// one facet and each lane owns KB bands (b = chunk + lane + k BT), so the
-// gather of one CSR entry reads BT * KB contiguous floats of B[col] with
-// coalesced loads.  The team loads BT entries (col, w) at a time, one per
-// lane, and broadcasts each by cross-lane shuffle.  Bands beyond BT * KB are
+// Band-parallel sweeps (ldb >= 16).  A team of bt lanes owns one facet and
+// every lane owns one 16-byte vector of bands: 4 FP32 bands (k_sweep_v) or
+// 8 BF16 bands of the BF16 copy (k_sweep_hv), so the gather of one CSR entry
+// is a single load instruction per team covering the row's contiguous bands
+// (2 lines of 128 bytes for 48 FP32 bands, 1 line for up to 64 BF16 bands).
+// Every lane of a team loads the same U entries (col, w) (one address per
+// team), then issues the U gathers before the FMAs.  Teams do not straddle
+// lanes' work: a block holds 256 / bt teams; bands beyond one team pass are
 // processed in chunks.
-template <int BT, int KB, int MODE, bool W16>
-__global__ __launch_bounds__(256) void k_sweep_bp(SweepArgs a) {
+//
+// With band retirement only bands [0, nAct) are gathered and updated; bands
+// [nAct, nActPrev), retired by the previous judge sweep, are copied from Bin
+// to Bout once so that both ping-pong buffers hold their final value.
+constexpr int kU = 8;  // CSR entries in flight per lane (one aligned group)
+
+// The 8-aligned group of CSR entries [base, base + 8): columns and weights
+// with 16-byte vector loads (one request per team and vector; the CSR arrays
+// are padded by kCsrPad zero entries), entries outside [start, end) get
+// weight 0 and ok[u] = false.  WQ: also the BF16 weights (REFRESH).
+template <bool W16, bool WQ>
+__device__ __forceinline__ void load_group(const SweepArgs& a, uint32_t base, uint32_t start, uint32_t end,
+                                           uint32_t (&n)[kU], float (&w)[kU], float (&wq)[kU], bool (&ok)[kU]) {
+    const uint4 cA = *reinterpret_cast<const uint4*>(a.col + base);
+    const uint4 cB = *reinterpret_cast<const uint4*>(a.col + base + 4);
+    n[0] = cA.x; n[1] = cA.y; n[2] = cA.z; n[3] = cA.w;
+    n[4] = cB.x; n[5] = cB.y; n[6] = cB.z; n[7] = cB.w;
+    float h[kU];
+    if (W16 || WQ) {
+        const uint4 q = *reinterpret_cast<const uint4*>(a.w16 + base);
+        const uint32_t qq[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            h[2 * j] = __uint_as_float(qq[j] << 16);
+            h[2 * j + 1] = __uint_as_float(qq[j] & 0xffff0000u);
+        }
+    }
+    float f[kU];
+    if (!W16) {
+        const float4 wA = *reinterpret_cast<const float4*>(a.w32 + base);
+        const float4 wB = *reinterpret_cast<const float4*>(a.w32 + base + 4);
+        f[0] = wA.x; f[1] = wA.y; f[2] = wA.z; f[3] = wA.w;
+        f[4] = wB.x; f[5] = wB.y; f[6] = wB.z; f[7] = wB.w;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+        const uint32_t e = base + uint32_t(u);
+        ok[u] = e >= start && e < end;
+        w[u] = ok[u] ? (W16 ? h[u] : f[u]) : 0.0f;
+        if (WQ) wq[u] = ok[u] ? h[u] : 0.0f;
+    }
+}
+
+struct BandCtx {
+    uint32_t nAct, nPrev;
+    float tolRel;
+    const float* Bin;
+    float* Bout;
+    const uint16_t* Bin16;
+    uint16_t* Bout16;
+};
+
+// control-block state of a band-parallel sweep; false: the sweep is skipped
+template <int MODE>
+__device__ __forceinline__ bool band_ctx(const SweepArgs& a, bool inner, BandCtx& x) {
     SolveCtrl* ctrl = a.ctrl;
-    if (MODE != MODE_INCIDENT && !a.standalone) {
-        if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
-        if (MODE == MODE_SWEEP && W16 && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return;
+    x.nAct = x.nPrev = a.ldb;
+    x.tolRel = -1.0f;
+    x.Bin = a.Bin;
+    x.Bout = a.Bout;
+    x.Bin16 = a.h0;  // standalone: h0 -> h1
+    x.Bout16 = a.h1;
+    if (MODE == MODE_INCIDENT || a.standalone) return true;
+    if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return false;
+    if (inner && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return false;
+    if (a.retire) {
+        x.nAct = *reinterpret_cast<volatile uint32_t*>(&ctrl->nAct);
+        x.nPrev = *reinterpret_cast<volatile uint32_t*>(&ctrl->nActPrev);
+        x.tolRel = ctrl->tolRel;
     }
-    const float* __restrict__ Bin = a.Bin;
-    float* __restrict__ Bout = a.Bout;
-    if (MODE != MODE_INCIDENT && !a.standalone && a.buf0) {
+    if (a.buf0) {
         const uint32_t it = *reinterpret_cast<volatile uint32_t*>(&ctrl->iter);
-        Bin = (it & 1u) ? a.buf1 : a.buf0;
-        Bout = (it & 1u) ? a.buf0 : a.buf1;
+        x.Bin = (it & 1u) ? a.buf1 : a.buf0;
+        x.Bout = (it & 1u) ? a.buf0 : a.buf1;
+        x.Bin16 = a.h0 ? ((it & 1u) ? a.h1 : a.h0) : nullptr;
+        x.Bout16 = a.h0 ? ((it & 1u) ? a.h0 : a.h1) : nullptr;
     }
-    const int lane = threadIdx.x % BT;
-    const uint32_t f = (xcd_block(a.xcd) * 256u + threadIdx.x) / BT;
+    return true;
+}
+
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ void st4(float* p, const float (&v)[4]) {
+    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void arr4(const float4 v, float (&o)[4]) {
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+
+// Update of side pair (s0, s0 + 1) at the 4 bands b0 .. b0 + 3 (b0 a
+// multiple of 4, 16-byte row accesses) from the gathered sums: bands below
+// nAct get the Jacobi (or Chebyshev) update, bands in [nAct, nActPrev) are
+// copied, bands from nActPrev on hold the same value in both buffers and are
+// rewritten unchanged.  Accumulates the residual maxima and the highest band
+// above tolRel.
+template <int MODE>
+__device__ __forceinline__ void band_update4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
+                                             const float (&g0)[4], const float (&g1)[4], const float (&q0)[4],
+                                             const float (&q1)[4], float& myAbs, float& myRel, float& myAct) {
+    if (b0 >= x.nPrev) return;
     const uint32_t ldb = a.ldb;
-    float myAbs = 0.0f, myRel = 0.0f;
-    if (f < a.F) {
+    const size_t i0 = size_t(s0) * ldb + b0, i1 = i0 + ldb;
+    float sk[4];
+    if (MODE == MODE_INCIDENT || (a.fuseH && a.judge)) arr4(ld4(a.skyv + b0), sk);
+    if (MODE == MODE_INCIDENT) {
+        float D0[4], D1[4], H0[4], H1[4];
+        arr4(ld4(a.D + i0), D0);
+        arr4(ld4(a.D + i1), D1);
+        const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            H0[j] = fmaxf(f0 * sk[j] + g0[j] + D0[j], 0.0f);
+            H1[j] = fmaxf(f1 * sk[j] + g1[j] + D1[j], 0.0f);
+        }
+        st4(a.H + i0, H0);
+        st4(a.H + i1, H1);
+        return;
+    }
+    float o0[4], o1[4];
+    arr4(ld4(x.Bin + i0), o0);
+    arr4(ld4(x.Bin + i1), o1);
+    if (b0 >= x.nAct) {  // retired by the last judge sweep: copied once
+        st4(x.Bout + i0, o0);
+        st4(x.Bout + i1, o1);
+        return;
+    }
+    bool act[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) act[j] = b0 + uint32_t(j) < x.nAct;
+    float r0[4], r1[4], t0[4], t1[4], c0[4], c1[4], n0[4], n1[4];
+    arr4(ld4(a.r + i0), r0);
+    arr4(ld4(a.r + i1), r1);
+    arr4(ld4(a.t + i0), t0);
+    arr4(ld4(a.t + i1), t1);
+    const float* csrc = MODE == MODE_REFRESH ? a.cBase : a.c;
+    arr4(ld4(csrc + i0), c0);
+    arr4(ld4(csrc + i1), c1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        n0[j] = c0[j] + r0[j] * g0[j] + t0[j] * g1[j];
+        n1[j] = c1[j] + r1[j] * g1[j] + t1[j] * g0[j];
+    }
+    if (MODE == MODE_REFRESH) {  // only active bands' c2 is ever read
+        float k0[4], k1[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            k0[j] = c0[j] + r0[j] * (g0[j] - q0[j]) + t0[j] * (g1[j] - q1[j]);
+            k1[j] = c1[j] + r1[j] * (g1[j] - q1[j]) + t1[j] * (g0[j] - q0[j]);
+        }
+        st4(a.c2 + i0, k0);
+        st4(a.c2 + i1, k1);
+    }
+    if (a.fuseH && a.judge) {  // incident of the FP32 gather (judge sweeps gather FP32)
+        float D0[4], D1[4], H0[4], H1[4];
+        arr4(ld4(a.D + i0), D0);
+        arr4(ld4(a.D + i1), D1);
+        const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            H0[j] = fmaxf(f0 * sk[j] + g0[j] + D0[j], 0.0f);
+            H1[j] = fmaxf(f1 * sk[j] + g1[j] + D1[j], 0.0f);
+        }
+        if (act[3]) {
+            st4(a.H + i0, H0);
+            st4(a.H + i1, H1);
+        } else {  // retired bands keep the incident of their last judge sweep
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (act[j]) {
+                    a.H[i0 + j] = H0[j];
+                    a.H[i1 + j] = H1[j];
+                }
+        }
+    }
+    float y0[4], y1[4], d0[4], d1[4], e0[4], e1[4];
+    if (a.omega) {
+        // Chebyshev semi-iteration: y = p + omega_b (J(B) - p) with p the
+        // iterate before B, which Bout still holds (ping-pong); the judge
+        // measures the Jacobi change |J(B) - B| / |J(B)|
+        float w[4];
+        arr4(ld4(a.omega + b0), w);
+        const bool anyW = w[0] != 1.0f || w[1] != 1.0f || w[2] != 1.0f || w[3] != 1.0f;
+        float p0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, p1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (anyW) {
+            arr4(ld4(x.Bout + i0), p0);
+            arr4(ld4(x.Bout + i1), p1);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            y0[j] = w[j] == 1.0f ? n0[j] : p0[j] + w[j] * (n0[j] - p0[j]);
+            y1[j] = w[j] == 1.0f ? n1[j] : p1[j] + w[j] * (n1[j] - p1[j]);
+            d0[j] = fabsf(n0[j] - o0[j]);
+            d1[j] = fabsf(n1[j] - o1[j]);
+            e0[j] = d0[j] / fmaxf(fabsf(n0[j]), 1.0e-30f);
+            e1[j] = d1[j] / fmaxf(fabsf(n1[j]), 1.0e-30f);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            y0[j] = o0[j] + a.relax * (n0[j] - o0[j]);
+            y1[j] = o1[j] + a.relax * (n1[j] - o1[j]);
+            d0[j] = fabsf(y0[j] - o0[j]);
+            d1[j] = fabsf(y1[j] - o1[j]);
+            e0[j] = d0[j] / fmaxf(fabsf(y0[j]), 1.0e-30f);
+            e1[j] = d1[j] / fmaxf(fabsf(y1[j]), 1.0e-30f);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (!act[j]) {  // beyond nAct inside this vector: unchanged
+            y0[j] = o0[j];
+            y1[j] = o1[j];
+            continue;
+        }
+        myAbs = fmaxf(myAbs, fmaxf(d0[j], d1[j]));
+        myRel = fmaxf(myRel, fmaxf(e0[j], e1[j]));
+        if (fmaxf(e0[j], e1[j]) > x.tolRel) myAct = fmaxf(myAct, float(b0 + uint32_t(j) + 1u));
+    }
+    st4(x.Bout + i0, y0);
+    st4(x.Bout + i1, y1);
+    if (x.Bout16) {
+        const uint32_t h0 = uint32_t(f32_to_bf16(y0[0])) | (uint32_t(f32_to_bf16(y0[1])) << 16);
+        const uint32_t h1 = uint32_t(f32_to_bf16(y0[2])) | (uint32_t(f32_to_bf16(y0[3])) << 16);
+        const uint32_t h2 = uint32_t(f32_to_bf16(y1[0])) | (uint32_t(f32_to_bf16(y1[1])) << 16);
+        const uint32_t h3 = uint32_t(f32_to_bf16(y1[2])) | (uint32_t(f32_to_bf16(y1[3])) << 16);
+        *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0) * a.ldh + b0) = make_uint2(h0, h1);
+        *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0 + 1) * a.ldh + b0) = make_uint2(h2, h3);
+    }
+}
+
+__device__ __forceinline__ void band_partials(const SweepArgs& a, float myAbs, float myRel, float myAct) {
+    float v[3] = {myAbs, myRel, myAct}, out[3];
+    block_max<3>(v, out);
+    if (threadIdx.x == 0) {
+        a.partial[blockIdx.x] = make_float2(out[0], out[1]);
+        if (a.partAct) a.partAct[blockIdx.x] = uint32_t(out[2]);
+    }
+}
+
+// FP32 gathers, 4 bands per lane.  MODE_SWEEP with FP32 (W16 false) or BF16
+// weights; MODE_REFRESH sums both (the BF16 sum of round_bf16(B) when
+// qRound, matching the BF16 copy the BF16 sweeps gather); MODE_INCIDENT.
+template <int MODE, bool W16>
+__global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
+    BandCtx x;
+    if (!band_ctx<MODE>(a, MODE == MODE_SWEEP && W16, x)) return;
+    const uint32_t bt = a.bt, tpb = 256u / bt, ldb = a.ldb;
+    const uint32_t team = threadIdx.x / bt, lane = threadIdx.x - team * bt;
+    const uint32_t f = xcd_block(a.xcd) * tpb + team;
+    float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
+    if (team < tpb && f < a.F) {
         const uint32_t s0 = 2u * f;
         const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
-        for (uint32_t cb = 0; cb < ldb; cb += BT * KB) {
-            float g[2][KB], q[2][KB];
+        for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
+            const uint32_t b0 = cb + 4u * lane;
+            float g[2][4] = {}, q[2][4] = {};
+            if (b0 < x.nAct) {
+                const float* __restrict__ Bl = x.Bin + b0;
 #pragma unroll
-            for (int k = 0; k < KB; ++k) g[0][k] = g[1][k] = q[0][k] = q[1][k] = 0.0f;
-            const float* __restrict__ Bl = Bin + cb + lane;
+                for (int side = 0; side < 2; ++side) {
+                    const uint32_t start = rb[side], end = rb[side + 1];
+                    for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += kU) {
+                        uint32_t n[kU];
+                        float w[kU], wq[kU];
+                        bool ok[kU];
+                        load_group<W16, MODE == MODE_REFRESH>(a, base, start, end, n, w, wq, ok);
+                        float4 v[kU];
 #pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                const uint32_t end = rb[side + 1];
-                // U entries per step: all column / weight loads, then all
-                // U * KB gathers, then the FMAs (entries past the row end use
-                // column 0 with weight 0)
-                constexpr int U = 8;
-                for (uint32_t base = rb[side]; base < end; base += U) {
-                    uint32_t n[U];
-                    float w[U], wq[U];
+                        for (int u = 0; u < kU; ++u)
+                            v[u] = ok[u] ? *reinterpret_cast<const float4*>(Bl + size_t(n[u]) * ldb)
+                                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 #pragma unroll
-                    for (int u = 0; u < U; ++u) {
-                        const uint32_t e = base + uint32_t(u);
-                        const bool ok = e < end;
-                        n[u] = ok ? a.col[e] : 0u;
-                        if (MODE == MODE_REFRESH) {
-                            w[u] = ok ? a.w32[e] : 0.0f;
-                            wq[u] = ok ? bf16_to_f32(a.w16[e]) : 0.0f;
-                        } else {
-                            w[u] = ok ? (W16 ? bf16_to_f32(a.w16[e]) : a.w32[e]) : 0.0f;
+                        for (int u = 0; u < kU; ++u) {
+                            const float vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                g[side][j] += w[u] * vv[j];
+                                if (MODE == MODE_REFRESH)
+                                    q[side][j] += wq[u] * (a.qRound ? round_bf16(vv[j]) : vv[j]);
+                            }
                         }
                     }
-                    float v[U][KB];
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-#pragma unroll
-                        for (int k = 0; k < KB; ++k)
-                            v[u][k] = cb + uint32_t(lane + k * BT) < ldb ? Bl[size_t(n[u]) * ldb + k * BT] : 0.0f;
-#pragma unroll
-                    for (int u = 0; u < U; ++u)
-#pragma unroll
-                        for (int k = 0; k < KB; ++k) {
-                            g[side][k] += w[u] * v[u][k];
-                            if (MODE == MODE_REFRESH) q[side][k] += wq[u] * v[u][k];
-                        }
                 }
             }
-#pragma unroll
-            for (int k = 0; k < KB; ++k) {
-                const uint32_t b = cb + uint32_t(lane) + uint32_t(k * BT);
-                if (b >= ldb) continue;
-                const size_t i0 = size_t(s0) * ldb + b, i1 = i0 + ldb;
-                if (MODE == MODE_INCIDENT) {
-                    const float sk = a.skyv[b];
-                    a.H[i0] = fmaxf(a.fsky[s0] * sk + g[0][k] + a.D[i0], 0.0f);
-                    a.H[i1] = fmaxf(a.fsky[s0 + 1] * sk + g[1][k] + a.D[i1], 0.0f);
-                    continue;
-                }
-                const float r0 = a.r[i0], t0 = a.t[i0], r1 = a.r[i1], t1 = a.t[i1];
-                const float* csrc = MODE == MODE_REFRESH ? a.cBase : a.c;
-                const float c0 = csrc[i0], c1 = csrc[i1];
-                const float n0 = c0 + r0 * g[0][k] + t0 * g[1][k];
-                const float n1 = c1 + r1 * g[1][k] + t1 * g[0][k];
-                if (MODE == MODE_REFRESH) {
-                    a.c2[i0] = c0 + r0 * (g[0][k] - q[0][k]) + t0 * (g[1][k] - q[1][k]);
-                    a.c2[i1] = c1 + r1 * (g[1][k] - q[1][k]) + t1 * (g[0][k] - q[0][k]);
-                }
-                const float o0 = Bin[i0], o1 = Bin[i1];
-                const float x0 = o0 + a.relax * (n0 - o0);
-                const float x1 = o1 + a.relax * (n1 - o1);
-                Bout[i0] = x0;
-                Bout[i1] = x1;
-                const float d0 = fabsf(x0 - o0), d1 = fabsf(x1 - o1);
-                myAbs = fmaxf(myAbs, fmaxf(d0, d1));
-                myRel = fmaxf(myRel, fmaxf(d0 / fmaxf(fabsf(x0), 1.0e-30f), d1 / fmaxf(fabsf(x1), 1.0e-30f)));
-            }
+            band_update4<MODE>(a, x, s0, b0, g[0], g[1], q[0], q[1], myAbs, myRel, myAct);
         }
     }
     if (MODE == MODE_INCIDENT) return;
-    float v[2] = {myAbs, myRel}, out[2];
-    block_max<2>(v, out);
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = make_float2(out[0], out[1]);
+    band_partials(a, myAbs, myRel, myAct);
+}
+
+// BF16 sweep: gathers the BF16 copy Bin16 (row stride ldh, a multiple of 64
+// bands) 8 bands per lane with BF16 weights; accumulates, updates and writes
+// B in FP32 and refreshes Bout16.
+__global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
+    BandCtx x;
+    if (!band_ctx<MODE_SWEEP>(a, true, x)) return;
+    const uint32_t bt = a.bt, tpb = 256u / bt, ldh = a.ldh;
+    const uint32_t team = threadIdx.x / bt, lane = threadIdx.x - team * bt;
+    const uint32_t f = xcd_block(a.xcd) * tpb + team;
+    float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
+    if (team < tpb && f < a.F) {
+        const uint32_t s0 = 2u * f;
+        const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
+        for (uint32_t cb = 0; cb < x.nPrev; cb += 8u * bt) {
+            const uint32_t b0 = cb + 8u * lane;
+            float g[2][8] = {};
+            if (b0 < x.nAct) {
+                const uint16_t* __restrict__ Hl = x.Bin16 + b0;
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    const uint32_t start = rb[side], end = rb[side + 1];
+                    for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += kU) {
+                        uint32_t n[kU];
+                        float w[kU], wq[kU];
+                        bool ok[kU];
+                        load_group<true, false>(a, base, start, end, n, w, wq, ok);
+                        uint4 v[kU];
+#pragma unroll
+                        for (int u = 0; u < kU; ++u)
+                            v[u] = ok[u] ? *reinterpret_cast<const uint4*>(Hl + size_t(n[u]) * ldh)
+                                         : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                        for (int u = 0; u < kU; ++u) {
+                            const uint32_t vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                g[side][2 * j] += w[u] * __uint_as_float(vv[j] << 16);
+                                g[side][2 * j + 1] += w[u] * __uint_as_float(vv[j] & 0xffff0000u);
+                            }
+                        }
+                    }
+                }
+            }
+            // two groups of 4 bands (the second may lie beyond ldb)
+            const float z[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            float lo0[4], lo1[4], hi0[4], hi1[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                lo0[j] = g[0][j];
+                lo1[j] = g[1][j];
+                hi0[j] = g[0][4 + j];
+                hi1[j] = g[1][4 + j];
+            }
+            band_update4<MODE_SWEEP>(a, x, s0, b0, lo0, lo1, z, z, myAbs, myRel, myAct);
+            if (b0 + 4u < a.ldb) band_update4<MODE_SWEEP>(a, x, s0, b0 + 4u, hi0, hi1, z, z, myAbs, myRel, myAct);
+        }
+    }
+    band_partials(a, myAbs, myRel, 0.0f);
+}
+
+// B -> BF16 copy with row stride ldh (bands >= ldb are zero)
+__global__ void k_to_bf16(const float* __restrict__ B, uint32_t S, uint32_t ldb, uint32_t ldh,
+                          uint16_t* __restrict__ H) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(S) * ldh) return;
+    const size_t s = i / ldh, b = i - s * ldh;
+    H[i] = b < ldb ? f32_to_bf16(B[s * ldb + b]) : uint16_t(0);
 }
 
 // Reduces the per-block residuals of the sweep just run and advances the
 // control block: iteration count, last residuals, done flag.
-__global__ __launch_bounds__(1024) void k_finalize(SolveCtrl* ctrl, const float2* __restrict__ partial, uint32_t n,
-                                                   uint32_t judge, uint32_t refresh, uint32_t inner) {
+__global__ __launch_bounds__(1024) void k_finalize(SolveCtrl* ctrl, const float2* __restrict__ partial,
+                                                   const uint32_t* __restrict__ partAct, uint32_t n, uint32_t judge,
+                                                   uint32_t refresh, uint32_t inner, const float* __restrict__ rho2,
+                                                   float* __restrict__ omega, uint32_t ldb) {
     if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
+    // Chebyshev weights for the next sweep (interval [-rho_b, rho_b]):
+    // omega_1 = 1, omega_2 = 1 / (1 - rho^2 / 2), omega_k+1 = 1 / (1 - rho^2 omega_k / 4)
+    if (omega)
+        for (uint32_t b = threadIdx.x; b < ldb; b += 1024u) {
+            const float r2 = rho2[b], w = omega[b];
+            omega[b] = ctrl->iter == ctrl->chebStart ? 1.0f / (1.0f - 0.5f * r2) : 1.0f / (1.0f - 0.25f * r2 * w);
+        }
     if (inner && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return;
     float mA = 0.0f, mR = 0.0f;
+    uint32_t act = 0u;
     for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
         const float2 p = partial[i];
         mA = fmaxf(mA, p.x);
         mR = fmaxf(mR, p.y);
+        if (partAct) act = max(act, partAct[i]);
     }
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) {
         mA = fmaxf(mA, __shfl_xor(mA, m, 64));
         mR = fmaxf(mR, __shfl_xor(mR, m, 64));
+        act = max(act, uint32_t(__shfl_xor(int(act), m, 64)));
     }
     __shared__ float sA[16], sR[16];
+    __shared__ uint32_t sN[16];
     if ((threadIdx.x & 63) == 0) {
         sA[threadIdx.x >> 6] = mA;
         sR[threadIdx.x >> 6] = mR;
+        sN[threadIdx.x >> 6] = act;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         float absd = 0.0f, reld = 0.0f;
+        uint32_t na = 0u;
         for (int w = 0; w < 16; ++w) {
             absd = fmaxf(absd, sA[w]);
             reld = fmaxf(reld, sR[w]);
+            na = max(na, sN[w]);
         }
         const uint32_t it = ctrl->iter + 1u;
         ctrl->iter = it;
         ctrl->lastAbs = absd;
         ctrl->lastRel = reld;
+        const uint32_t old = ctrl->nAct;
+        ctrl->nActPrev = old;
+        if (judge && partAct && ctrl->tolRel >= 0.0f) ctrl->nAct = min(old, na);
         if (refresh) {
             ctrl->refreshes += 1u;
             ctrl->innerDone = 0u;
         }
-        if (inner && ctrl->tolRel >= 0.0f && reld <= ctrl->tolRel) ctrl->innerDone = 1u;
+        if (inner && ctrl->innerTol >= 0.0f && reld <= ctrl->innerTol) ctrl->innerDone = 1u;
         bool stop = it >= ctrl->maxIter;
         if (judge && ctrl->tolRel >= 0.0f && reld <= ctrl->tolRel) stop = true;
         if (ctrl->tolAbs >= 0.0f && (it % ctrl->checkEvery) == 0u && absd <= ctrl->tolAbs) stop = true;
@@ -419,32 +734,6 @@ static void launch_team(int team, int mode, bool w16, const SweepArgs& a, hipStr
     else launch_mode<NBC, 64>(mode, w16, a, g, st);
 }
 
-template <int BT, int KB>
-static void launch_bp_mode(int mode, bool w16, const SweepArgs& a, uint32_t grid, hipStream_t st) {
-    if (mode == MODE_SWEEP && !w16)
-        hipLaunchKernelGGL((k_sweep_bp<BT, KB, MODE_SWEEP, false>), dim3(grid), dim3(256), 0, st, a);
-    else if (mode == MODE_SWEEP)
-        hipLaunchKernelGGL((k_sweep_bp<BT, KB, MODE_SWEEP, true>), dim3(grid), dim3(256), 0, st, a);
-    else if (mode == MODE_REFRESH)
-        hipLaunchKernelGGL((k_sweep_bp<BT, KB, MODE_REFRESH, false>), dim3(grid), dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_sweep_bp<BT, KB, MODE_INCIDENT, false>), dim3(grid), dim3(256), 0, st, a);
-}
-
-// lanes per facet (BT) and bands per lane (KB) of the band-parallel kernel
-static void pick_bp(uint32_t ldb, int& bt, int& kb) {
-    if (ldb % 16u == 0u) {
-        bt = 16;
-        kb = ldb >= 64u ? 4 : int(ldb / 16u);
-    } else if (ldb % 8u == 0u) {
-        bt = 8;
-        kb = 1;
-    } else {
-        bt = 4;
-        kb = ldb == 12u ? 3 : 1;
-    }
-}
-
 // FH_SPMM_KERNEL=entry|band forces one of the two kernels (tuning knob)
 static bool band_parallel(uint32_t ldb) {
     static const int forced = [] {
@@ -467,11 +756,19 @@ static uint32_t xcd_remap() {
     return v;
 }
 
+bool uses_bf16_gather(uint32_t ldb) { return band_parallel(ldb); }
+
+uint32_t bf16_stride(uint32_t ldb) { return (ldb + 63u) & ~63u; }
+
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.xcd = xcd_remap();
     if (band_parallel(a.ldb)) {
-        int bt = 16, kb = 1;
-        pick_bp(a.ldb, bt, kb);
+        // lanes per team: one 16-byte vector of the row each, at most 64
+        // bands per team pass
+        const bool half = mode == MODE_SWEEP && w16 && a.h0;
+        const uint32_t vw = half ? 8u : 4u;
+        a.bt = std::min((a.ldb + vw - 1u) / vw, 64u / vw);
+        const uint32_t tpb = 256u / a.bt;
         a.F = c->F;
         a.rowptr = c->rowptr.p;
         a.col = c->col.p;
@@ -479,25 +776,33 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         a.w16 = c->val16.p;
         a.fsky = c->fsky.p;
         a.ctrl = c->ctrl.p;
-        const uint32_t grid = uint32_t((uint64_t(c->F) * uint64_t(bt) + 255u) / 256u);
-        c->partial.ensure(grid ? grid : 1u);
+        const uint32_t g = std::max(1u, (c->F + tpb - 1u) / tpb);
+        c->partial.ensure(g);
         a.partial = c->partial.p;
-        const uint32_t g = grid ? grid : 1u;
-        if (bt == 16 && kb == 1) launch_bp_mode<16, 1>(mode, w16, a, g, c->stream);
-        else if (bt == 16 && kb == 2) launch_bp_mode<16, 2>(mode, w16, a, g, c->stream);
-        else if (bt == 16 && kb == 3) launch_bp_mode<16, 3>(mode, w16, a, g, c->stream);
-        else if (bt == 16) launch_bp_mode<16, 4>(mode, w16, a, g, c->stream);
-        else if (bt == 8) launch_bp_mode<8, 1>(mode, w16, a, g, c->stream);
-        else if (kb == 3) launch_bp_mode<4, 3>(mode, w16, a, g, c->stream);
-        else launch_bp_mode<4, 1>(mode, w16, a, g, c->stream);
+        if (a.retire) {
+            c->partAct.ensure(g);
+            a.partAct = c->partAct.p;
+        } else {
+            a.partAct = nullptr;
+        }
+        hipStream_t st = c->stream;
+        if (half) hipLaunchKernelGGL(k_sweep_hv, dim3(g), dim3(256), 0, st, a);
+        else if (mode == MODE_SWEEP && !w16) hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, false>), dim3(g), dim3(256), 0, st, a);
+        else if (mode == MODE_SWEEP) hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, true>), dim3(g), dim3(256), 0, st, a);
+        else if (mode == MODE_REFRESH) hipLaunchKernelGGL((k_sweep_v<MODE_REFRESH, false>), dim3(g), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL((k_sweep_v<MODE_INCIDENT, false>), dim3(g), dim3(256), 0, st, a);
         if (mode != MODE_INCIDENT && !a.standalone)
-            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, g, a.judge,
-                               uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16));
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, a.partAct, g,
+                               a.judge, uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16),
+                               a.rho2, const_cast<float*>(a.omega), a.ldb);
         FH_HIP(hipGetLastError());
         return;
     }
     const int nbc = pick_nbc(a.ldb);
     const int team = sweep_team(c);
+    a.retire = 0u;
+    a.partAct = nullptr;
+    a.omega = a.rho2 = nullptr;
     a.nchunks = (a.ldb + uint32_t(nbc) - 1u) / uint32_t(nbc);
     a.F = c->F;
     a.rowptr = c->rowptr.p;
@@ -517,8 +822,10 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         default: launch_team<16>(team, mode, w16, a, c->stream); break;
     }
     if (mode != MODE_INCIDENT && !a.standalone)
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, grid ? grid : 1u,
-                           a.judge, uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16));
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p,
+                           static_cast<const uint32_t*>(nullptr), grid ? grid : 1u, a.judge,
+                           uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16),
+                           static_cast<const float*>(nullptr), static_cast<float*>(nullptr), a.ldb);
     FH_HIP(hipGetLastError());
 }
 
@@ -548,15 +855,29 @@ void launch_split_optics(fh_context* c, const float4* o, float* r, float* t, flo
 }
 
 
-__global__ void k_ctrl_reset(SolveCtrl* ctrl, uint32_t maxIter, float tolRel, float tolAbs, uint32_t checkEvery) {
+__global__ void k_ctrl_reset(SolveCtrl* ctrl, uint32_t maxIter, float tolRel, float tolAbs, uint32_t checkEvery,
+                             uint32_t nAct, float innerTol) {
     if (threadIdx.x == 0) {
         SolveCtrl z{};
         z.maxIter = maxIter;
         z.tolRel = tolRel;
         z.tolAbs = tolAbs;
         z.checkEvery = checkEvery ? checkEvery : 1u;
+        z.nAct = nAct;
+        z.nActPrev = nAct;
+        z.innerTol = innerTol;
         *ctrl = z;
     }
+}
+
+__global__ void k_ctrl_inner(SolveCtrl* ctrl, float innerTol) {
+    if (threadIdx.x == 0) ctrl->innerTol = innerTol;
+}
+
+// restarts the Chebyshev sequence at the next sweep (omega = 1)
+__global__ void k_cheb_restart(SolveCtrl* ctrl, float* omega, uint32_t ldb) {
+    for (uint32_t b = threadIdx.x; b < ldb; b += blockDim.x) omega[b] = 1.0f;
+    if (threadIdx.x == 0) ctrl->chebStart = ctrl->iter;
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -569,8 +890,10 @@ __global__ void k_fill_f32(float* p, size_t n, float v) {
     if (i < n) p[i] = v;
 }
 
-void launch_ctrl_reset(fh_context* c, uint32_t maxIter, float tolRel, float tolAbs, uint32_t checkEvery) {
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, c->stream, c->ctrl.p, maxIter, tolRel, tolAbs, checkEvery);
+void launch_ctrl_reset(fh_context* c, uint32_t maxIter, float tolRel, float tolAbs, uint32_t checkEvery,
+                       uint32_t nAct, float innerTol) {
+    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(64), 0, c->stream, c->ctrl.p, maxIter, tolRel, tolAbs, checkEvery,
+                       nAct, innerTol);
     FH_HIP(hipGetLastError());
 }
 
@@ -599,49 +922,112 @@ void set_sky(fh_context* c, const float* sky, uint32_t nb, uint32_t ldb) {
     FH_HIP(hipStreamSynchronize(c->stream));
 }
 
+static float env_float(const char* name, float dflt) {
+    const char* e = std::getenv(name);
+    return e ? float(std::atof(e)) : dflt;
+}
+
 int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const float* rr, const float* tt,
-               float* bufs[2], fh_solve_stats* stats) {
-    launch_ctrl_reset(c, o.maxIter ? o.maxIter : 1u, o.tolRel, o.tolAbs, o.checkEvery);
+               float* bufs[2], fh_solve_stats* stats, bool* incidentDone) {
+    const bool bf16 = o.precision == FH_PRECISION_BF16_DC;
+    const bool bp = band_parallel(o.ldb);
+    const bool retire = o.retire && bp && o.tolRel >= 0.0f;
+    const bool half = bf16 && bp;
+    // FH_DC_FIRST overrides the end of the BF16 phase (tuning knob)
+    const float firstInner = env_float("FH_DC_FIRST", o.firstInner);
+    launch_ctrl_reset(c, o.maxIter ? o.maxIter : 1u, o.tolRel, o.tolAbs, o.checkEvery, o.ldb,
+                      half ? std::max(firstInner, o.tolRel) : o.tolRel);
     SweepArgs a{};
     a.r = rr;
     a.t = tt;
     a.ldb = o.ldb;
     a.relax = o.relax;
+    a.retire = retire ? 1u : 0u;
+    a.buf0 = bufs[0];
+    a.buf1 = bufs[1];
+    // Chebyshev acceleration of the band-parallel solves (FP32, and the BF16
+    // phase of BF16 solves)
+    if (bp && o.rho2) {
+        c->omega.ensure(o.ldb);
+        launch_fill_f32(c, c->omega.p, o.ldb, 1.0f);
+        a.omega = c->omega.p;
+        a.rho2 = o.rho2;
+    }
+    // incident fused into the judge sweeps: every band's H is that of its
+    // last judge sweep, complete once every band converged (retirement)
+    const bool fuse = retire && o.H && o.D;
+    if (fuse) {
+        a.fuseH = 1u;
+        a.H = o.H;
+        a.D = o.D;
+        a.skyv = c->skyv.p;
+    }
     uint64_t launched = 0;
     uint32_t batch = 4;
-    const bool bf16 = o.precision == FH_PRECISION_BF16_DC;
     if (bf16) c->c2.ensure(size_t(c->S) * o.ldb);
-    while (true) {
-        a.buf0 = bufs[0];
-        a.buf1 = bufs[1];
-        for (uint32_t k = 0; k < batch; ++k) {
-            if (!bf16) {
-                a.c = cc;
-                a.judge = 1u;
-                launch_sweep(c, MODE_SWEEP, false, a);
-            } else {
-                // groups of BF16 sweeps (stopping early once their relative
-                // change is within tolRel) closed by an FP32 refresh that
-                // judges convergence and recomputes the defect correction;
-                // the first group runs on the uncorrected source
-                const uint32_t period = o.innerBf16 + 1u;
-                if (launched % period == period - 1u) {
-                    a.c = cc;
-                    a.cBase = cc;
-                    a.c2 = c->c2.p;
-                    a.judge = 1u;
-                    launch_sweep(c, MODE_REFRESH, false, a);
-                } else {
-                    a.c = launched < period ? cc : c->c2.p;
-                    a.judge = 0u;
-                    launch_sweep(c, MODE_SWEEP, true, a);
-                }
-            }
-            ++launched;
+    if (half) {
+        // Phase 1: BF16 sweeps gathering BF16 copies of B (X16a = bf16(bufs[0])
+        // first), ending at relative change firstInner (about the distance of
+        // the BF16 operator's fixed point) or after innerBf16 sweeps.
+        // Phase 2: FP32 sweeps from there, which judge convergence per band
+        // and remove the BF16 defect (the FP32 operator's fixed point).
+        const uint32_t ldh = bf16_stride(o.ldb);
+        const size_t nh = size_t(c->S) * ldh;
+        c->X16a.ensure(nh);
+        c->X16b.ensure(nh);
+        hipLaunchKernelGGL(k_to_bf16, dim3(uint32_t((nh + 255) / 256)), dim3(256), 0, c->stream, bufs[0], c->S, o.ldb,
+                           ldh, c->X16a.p);
+        FH_HIP(hipGetLastError());
+        a.h0 = c->X16a.p;
+        a.h1 = c->X16b.p;
+        a.ldh = ldh;
+        a.c = cc;
+        a.judge = 0u;
+        for (uint32_t k = 0; k < o.innerBf16; ++k) launch_sweep(c, MODE_SWEEP, true, a);
+        a.h0 = a.h1 = nullptr;
+        if (a.omega) {
+            hipLaunchKernelGGL(k_cheb_restart, dim3(1), dim3(64), 0, c->stream, c->ctrl.p, c->omega.p, o.ldb);
+            FH_HIP(hipGetLastError());
         }
-        read_ctrl(c);
-        if (c->hctrl->done) break;
-        if (batch < 32u) batch *= 2u;
+        a.judge = 1u;
+        while (true) {
+            for (uint32_t k = 0; k < batch; ++k) launch_sweep(c, MODE_SWEEP, false, a);
+            read_ctrl(c);
+            if (c->hctrl->done) break;
+            if (batch < 32u) batch *= 2u;
+        }
+    } else {
+        while (true) {
+            for (uint32_t k = 0; k < batch; ++k) {
+                if (!bf16) {
+                    a.c = cc;
+                    a.judge = 1u;
+                    launch_sweep(c, MODE_SWEEP, false, a);
+                } else {
+                    // groups of BF16-weight sweeps (stopping early once their
+                    // relative change is within tolRel) closed by an FP32
+                    // refresh that judges convergence and recomputes the
+                    // defect correction; the first group runs on the
+                    // uncorrected source
+                    const uint32_t period = o.innerBf16 + 1u;
+                    if (launched % period == period - 1u) {
+                        a.c = cc;
+                        a.cBase = cc;
+                        a.c2 = c->c2.p;
+                        a.judge = 1u;
+                        launch_sweep(c, MODE_REFRESH, false, a);
+                    } else {
+                        a.c = launched < period ? cc : c->c2.p;
+                        a.judge = 0u;
+                        launch_sweep(c, MODE_SWEEP, true, a);
+                    }
+                }
+                ++launched;
+            }
+            read_ctrl(c);
+            if (c->hctrl->done) break;
+            if (batch < 32u) batch *= 2u;
+        }
     }
     if (stats) {
         stats->iterations = c->hctrl->iter;
@@ -649,7 +1035,59 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
         stats->maxRelDelta = c->hctrl->lastRel;
         stats->refreshes = c->hctrl->refreshes;
     }
+    if (incidentDone) *incidentDone = fuse && c->hctrl->nAct == 0u;
     return int(c->hctrl->iter & 1u);
+}
+
+// per-band maximum of a non-negative [S][ldb] array (float bits compared as
+// unsigned integers) into out[ldb] (zeroed by the caller), ldb <= 4096
+__global__ __launch_bounds__(256) void k_band_max(const float* __restrict__ v, uint32_t S, uint32_t ldb,
+                                                  uint32_t* __restrict__ out) {
+    __shared__ uint32_t sh[4096];
+    for (uint32_t b = threadIdx.x; b < ldb; b += 256u) sh[b] = 0u;
+    __syncthreads();
+    const size_t n = size_t(S) * ldb;
+    for (size_t i = size_t(blockIdx.x) * 256u + threadIdx.x; i < n; i += size_t(gridDim.x) * 256u)
+        atomicMax(&sh[i % ldb], __float_as_uint(fmaxf(v[i], 0.0f)));
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < ldb; b += 256u)
+        if (sh[b]) atomicMax(&out[b], sh[b]);
+}
+
+void estimate_rho2(fh_context* c, uint32_t ldb, uint32_t nb, const float* rr, const float* tt, float* v0, float* v1,
+                   float* zero, float* rho2) {
+    if (ldb > 4096u) fail(FH_INVALID_ARGUMENT, "too many bands for the spectral estimate");
+    const size_t n = size_t(c->S) * ldb;
+    launch_fill_f32(c, v0, n, 1.0f);
+    launch_fill_f32(c, zero, n, 0.0f);
+    c->bandMax.ensure(2u * ldb);
+    launch_fill_u32(c, c->bandMax.p, 2u * ldb, 0u);
+    // B_k+1 = K B_k (c = 0); maxima after 8 and 10 sweeps (both land in v0)
+    constexpr int m = 8;
+    for (int k = 0; k < m + 2; ++k) {
+        launch_single_sweep(c, ldb, zero, rr, tt, (k & 1) ? v1 : v0, (k & 1) ? v0 : v1, false);
+        if (k == m - 1 || k == m + 1) {
+            hipLaunchKernelGGL(k_band_max, dim3(512), dim3(256), 0, c->stream, v0, c->S, ldb,
+                               c->bandMax.p + (k == m - 1 ? 0u : ldb));
+            FH_HIP(hipGetLastError());
+        }
+    }
+    std::vector<uint32_t> h(2u * ldb);
+    FH_HIP(hipMemcpyAsync(h.data(), c->bandMax.p, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipStreamSynchronize(c->stream));
+    std::vector<float> r2(ldb, 0.0f);
+    for (uint32_t b = 0; b < ldb && b < nb; ++b) {
+        float a, z;
+        std::memcpy(&a, &h[b], 4);
+        std::memcpy(&z, &h[ldb + b], 4);
+        if (a > 0.0f && z > 0.0f && std::isfinite(a) && std::isfinite(z)) {
+            // two-step ratio (eigenvalues come in +-lambda pairs), 2 % margin
+            const float rho = std::min(0.98f, 1.02f * std::sqrt(z / a));
+            r2[b] = rho * rho;
+        }
+    }
+    FH_HIP(hipMemcpyAsync(rho2, r2.data(), sizeof(float) * ldb, hipMemcpyHostToDevice, c->stream));
+    FH_HIP(hipStreamSynchronize(c->stream));
 }
 
 void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, float* H) {
@@ -663,8 +1101,11 @@ void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, f
 }
 
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
-                         const float* Bin, float* Bout, bool w16) {
+                         const float* Bin, float* Bout, bool w16, uint16_t* h16in, uint16_t* h16out) {
     SweepArgs a{};
+    a.h0 = h16in;
+    a.h1 = h16out;
+    a.ldh = bf16_stride(ldb);
     a.c = cc;
     a.r = rr;
     a.t = tt;

@@ -20,6 +20,15 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t remap) {
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
+// round to nearest even (finite inputs)
+__device__ __forceinline__ uint16_t f32_to_bf16(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+// x rounded to BF16 and widened again: the value a BF16 copy of x holds
+__device__ __forceinline__ float round_bf16(float x) { return bf16_to_f32(f32_to_bf16(x)); }
+
 template <int NBC>
 __device__ __forceinline__ void gather(const float* __restrict__ p, float w, float (&acc)[NBC]) {
     if constexpr (NBC == 1) {
@@ -32,6 +41,43 @@ __device__ __forceinline__ void gather(const float* __restrict__ p, float w, flo
             acc[4 * q + 1] += w * v.y;
             acc[4 * q + 2] += w * v.z;
             acc[4 * q + 3] += w * v.w;
+        }
+    }
+}
+
+// Single-band gathers of the two rows [b0, e0) and [b1, e1) by the lanes of
+// a team: lane handles entries lane + k TEAM of both rows, U of each row per
+// step with all column / weight loads issued, then all 2 U gathers, then the
+// FMAs, so that a row of ~U TEAM entries costs one round of memory latency.
+// Per-lane partial sums; the caller reduces them over the team.
+template <int TEAM, int U, bool W16>
+__device__ __forceinline__ void team_rows_dot2(const uint32_t* __restrict__ col, const float* __restrict__ w32,
+                                               const uint16_t* __restrict__ w16, const float* __restrict__ x,
+                                               uint32_t b0, uint32_t e0, uint32_t b1, uint32_t e1, int lane,
+                                               float& g0, float& g1) {
+    const uint32_t len = max(e0 - b0, e1 - b1);
+    for (uint32_t k = uint32_t(lane); k < len; k += uint32_t(TEAM * U)) {
+        uint32_t n0[U], n1[U];
+        float w0[U], w1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i0 = b0 + k + uint32_t(u * TEAM), i1 = b1 + k + uint32_t(u * TEAM);
+            const bool ok0 = i0 < e0, ok1 = i1 < e1;
+            n0[u] = ok0 ? col[i0] : 0u;
+            n1[u] = ok1 ? col[i1] : 0u;
+            w0[u] = ok0 ? (W16 ? bf16_to_f32(w16[i0]) : w32[i0]) : 0.0f;
+            w1[u] = ok1 ? (W16 ? bf16_to_f32(w16[i1]) : w32[i1]) : 0.0f;
+        }
+        float v0[U], v1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v0[u] = x[n0[u]];
+            v1[u] = x[n1[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            g0 += w0[u] * v0[u];
+            g1 += w1[u] * v1[u];
         }
     }
 }

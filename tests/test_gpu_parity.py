@@ -199,19 +199,22 @@ def test_error_paths():
     assert res.iterations == 8
 
 
-def _eb_case(oracle, leaves=40, cells=6, R=128, K=32):
+def _eb_case(oracle, leaves=40, cells=6, R=128, K=32, nb=4):
     sc = scenes.row_crop(leaves_per_row=leaves, cells=cells)
     eng = _engine(R=R, K=K, seed=4)
     eng.setGeometry(sc.vertices)
     eng.buildVisibilityGraph()
-    win = faceteb.ShortwaveWindows.for_bands(4)
+    win = faceteb.ShortwaveWindows.for_bands(nb)
     faceteb.setup_scene(eng, sc, win)
     return sc, eng, win
 
 
-def test_eb_node_matches_oracle(oracle):
-    print("eb node case", flush=True)
-    sc, eng, win = _eb_case(oracle)
+# 4 windows: entry-parallel sweeps; 24 and 47: band-parallel sweeps with the
+# windows sorted internally, per-band retirement and (BF16) BF16 gathers
+@pytest.mark.parametrize("nb", [4, 24, 47])
+def test_eb_node_matches_oracle(oracle, nb):
+    print(f"eb node case nb={nb}", flush=True)
+    sc, eng, win = _eb_case(oracle, nb=nb)
     og = _as_oracle_graph(oracle, eng.getGraph())
     cfg = faceteb.EbConfig(tolerance=1e-3, radiosity_tolerance=1e-7, max_coupling=200)
     met = faceteb.diurnal_meteo(24)[12]
@@ -229,6 +232,8 @@ def test_eb_node_matches_oracle(oracle):
     cos = np.maximum(0.0, optics.side_normals(sc) @ sun).astype(np.float32)
     D = (np.clip(s["sunlit"], 0, 1) * cos)[:, None] * dni[None, :]
     sw, _, _ = oracle.solve_bands(og, r, t, np.zeros_like(r), D, dif, tol_rel=1e-9)
+    # radiosity returned per window in window order
+    assert _rel(s["shortwaveRadiosity"][sw > 1e-3], sw[sw > 1e-3]) < 1e-4
     H = oracle.incident_bands(og, sw, D, dif)
     absorbed = np.sum(np.clip(1 - r - t, 0, 1) * H, axis=1)
     assert _rel(s["absorbedShortwave"][absorbed > 1], absorbed[absorbed > 1]) < 1e-4
@@ -259,10 +264,11 @@ def test_eb_node_matches_oracle(oracle):
     assert _rel(s["longwaveRadiosity"], Blw) < 1e-4
 
 
-def test_eb_diurnal_plain_vs_bf16(oracle):
+@pytest.mark.parametrize("nb", [4, 24])
+def test_eb_diurnal_plain_vs_bf16(oracle, nb):
     # the BF16 + defect-correction path reaches the same converged state
-    print("eb diurnal case", flush=True)
-    sc, eng, win = _eb_case(oracle, leaves=30, cells=5, R=64, K=16)
+    print(f"eb diurnal case nb={nb}", flush=True)
+    sc, eng, win = _eb_case(oracle, leaves=30, cells=5, R=64, K=16, nb=nb)
     met = faceteb.diurnal_meteo(6)
     cfg = faceteb.EbConfig(tolerance=0.01)
     s1 = faceteb.run_diurnal(eng, met, win, cfg, precision=FH_PRECISION_FP32)
