@@ -148,8 +148,9 @@ uint32_t fh_surface_count(const fh_context *ctx);
 
 /* ---- multi-band extension (fuses the band loop of core:1690-1763) ------ */
 /* Arrays are [surfaceCount][nb] (band fastest); sky is [nb]; init may be
- * NULL.  Runs until the largest relative change of any side and band is
- * <= tolRel (checked on every FP32 sweep) or maxIter sweeps. */
+ * NULL.  Runs until, for every band, the largest relative change over the
+ * sides in an FP32 sweep is <= tolRel (from 16 bands on, a band that meets
+ * it stops iterating while the others continue), or maxIter sweeps. */
 int fh_solve_bands(fh_context *ctx, uint32_t nb, const float *reflectance,
                    const float *transmittance, const float *emission, const float *direct,
                    const float *sky, const float *init, uint32_t maxIter, float tolRel,
@@ -169,9 +170,9 @@ int fh_incident_bands(fh_context *ctx, uint32_t nb, const float *radiosity, cons
 
 /* One Jacobi sweep on device pointers, launched as a single kernel with no
  * convergence bookkeeping and no synchronisation (benchmark / roofline entry).
- * d_c, d_r, d_t, d_Bin, d_Bout are [surfaceCount][ldb]; ldb = nb rounded
- * up to a multiple of 4 (1 when nb == 1).  precision selects the F values
- * read (FP32 or BF16). */
+ * d_c, d_r, d_t, d_Bin, d_Bout are [surfaceCount][ldb] with ldb =
+ * fh_band_stride(nb): nb rounded up to a multiple of 4 (1 when nb == 1).
+ * precision selects the F values read (FP32 or BF16). */
 int fh_sweep_device(fh_context *ctx, uint32_t nb, const float *d_c, const float *d_r,
                     const float *d_t, const float *d_Bin, float *d_Bout, uint32_t precision);
 uint32_t fh_band_stride(uint32_t nb);

@@ -359,45 +359,61 @@ struct CoupledArgs {
     EbConst k;
     SolveCtrl* ctrl;
     float4* partial;
+    float* Iin;  // per side: unclamped incident fsky L + G of the gather half
 };
 
+// Gather half of a coupled sweep: a team of TEAM lanes per facet sums
+// G = F B over both sides' rows and stores the unclamped incident
+// fsky L + G per side in Iin.
 template <int TEAM, bool W16>
-__global__ __launch_bounds__(256) void k_coupled(CoupledArgs a) {
+__global__ __launch_bounds__(256) void k_coupled_gather(CoupledArgs a) {
     if (*reinterpret_cast<volatile uint32_t*>(&a.ctrl->done)) return;
     const int lane = threadIdx.x % TEAM;
     const uint32_t f = (blockIdx.x * 256u + threadIdx.x) / TEAM;
+    if (f >= a.F) return;
+    const uint32_t s0 = 2u * f;
+    const uint32_t e0 = a.rowptr[s0], e1 = a.rowptr[s0 + 1], e2 = a.rowptr[s0 + 2];
+    float g[2][1] = {{0.0f}, {0.0f}};
+    team_rows_dot2<TEAM, (TEAM >= 32 ? 4 : 8), W16>(a.col, a.w32, a.w16, a.Bin, e0, e1, e1, e2, lane, g[0][0],
+                                                    g[1][0]);
+    team_reduce<1, TEAM>(g[0]);
+    team_reduce<1, TEAM>(g[1]);
+    if (lane == 0) {
+        a.Iin[s0] = a.fsky[s0] * a.L + g[0][0];
+        a.Iin[s0 + 1] = a.fsky[s0 + 1] * a.L + g[1][0];
+    }
+}
+
+// Update half, one thread per facet (all lanes busy with the FP64 Newton
+// step): H = max(0, incident), relaxed Newton step of the facet's
+// temperature, LW emission at the new temperature and the Jacobi update
+// B' = e(T') + r in_s + t in_o; per-block maxima of |dB|, |dB|/|B|, dT and
+// the residual-equivalent.
+__global__ __launch_bounds__(256) void k_coupled_update(CoupledArgs a) {
+    if (*reinterpret_cast<volatile uint32_t*>(&a.ctrl->done)) return;
+    const uint32_t f = blockIdx.x * 256u + threadIdx.x;
     float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // |dB|, |dB|/|B|, dT, residual-equivalent
     if (f < a.F) {
         const uint32_t s0 = 2u * f;
-        const uint32_t e0 = a.rowptr[s0], e1 = a.rowptr[s0 + 1], e2 = a.rowptr[s0 + 2];
-        float g[2][1] = {{0.0f}, {0.0f}};
-        team_rows_dot2<TEAM, (TEAM >= 32 ? 4 : 8), W16>(a.col, a.w32, a.w16, a.Bin, e0, e1, e1, e2, lane, g[0][0],
-                                                        g[1][0]);
-        team_reduce<1, TEAM>(g[0]);
-        team_reduce<1, TEAM>(g[1]);
-        if (lane == 0) {
-            const float L = a.L;
-            const float in0 = a.fsky[s0] * L + g[0][0];
-            const float in1 = a.fsky[s0 + 1] * L + g[1][0];
-            const float H0 = fmaxf(in0, 0.0f), H1 = fmaxf(in1, 0.0f);
-            a.Hout[s0] = H0;
-            a.Hout[s0 + 1] = H1;
-            const NewtonOut o = newton_facet(f, a.k, a.mats[a.mat[f]], a.absorbed, H0, H1, a.eps, a.T, a.Tprev, true,
-                                             nullptr, nullptr, nullptr, nullptr, a.area);
-            v[2] = o.dT;
-            v[3] = o.eq;
-            const float ins[2] = {in0, in1};
+        const float in0 = a.Iin[s0], in1 = a.Iin[s0 + 1];
+        const float H0 = fmaxf(in0, 0.0f), H1 = fmaxf(in1, 0.0f);
+        a.Hout[s0] = H0;
+        a.Hout[s0 + 1] = H1;
+        const NewtonOut o = newton_facet(f, a.k, a.mats[a.mat[f]], a.absorbed, H0, H1, a.eps, a.T, a.Tprev, true,
+                                         nullptr, nullptr, nullptr, nullptr, a.area);
+        v[2] = o.dT;
+        v[3] = o.eq;
+        const float ins[2] = {in0, in1};
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                const uint32_t s = s0 + q;
-                const float em = lw_emission(a.eps[s], a.T[s]);
-                const float nb = em + a.r[s] * ins[q] + a.t[s] * ins[q ^ 1];
-                const float old = a.Bin[s];
-                a.Bout[s] = nb;
-                const float d = fabsf(nb - old);
-                v[0] = fmaxf(v[0], d);
-                v[1] = fmaxf(v[1], d / fmaxf(fabsf(nb), 1.0e-30f));
-            }
+        for (int q = 0; q < 2; ++q) {
+            const uint32_t s = s0 + q;
+            const float em = lw_emission(a.eps[s], a.T[s]);
+            const float nb = em + a.r[s] * ins[q] + a.t[s] * ins[q ^ 1];
+            const float old = a.Bin[s];
+            a.Bout[s] = nb;
+            const float d = fabsf(nb - old);
+            v[0] = fmaxf(v[0], d);
+            v[1] = fmaxf(v[1], d / fmaxf(fabsf(nb), 1.0e-30f));
         }
     }
     float out[4];
@@ -608,8 +624,10 @@ static uint32_t coupled_loop(fh_context* c, const fh_eb_node_params& p, const Eb
     EbState& e = c->eb;
     const int team = sweep_team(c);
     const uint32_t grid = uint32_t((uint64_t(c->F) * uint64_t(team) + 255u) / 256u);
-    e.part4.ensure(grid);
+    const uint32_t gridU = (c->F + 255u) / 256u;
+    e.part4.ensure(gridU);
     e.last4.ensure(4);
+    e.Iin.ensure(c->S);
     launch_ctrl_reset(c, maxIter ? maxIter : 1u, p.radiosityTolerance, -1.0f, 1u);
     CoupledArgs a{};
     a.rowptr = c->rowptr.p;
@@ -632,20 +650,22 @@ static uint32_t coupled_loop(fh_context* c, const fh_eb_node_params& p, const Eb
     a.k = k;
     a.ctrl = c->ctrl.p;
     a.partial = e.part4.p;
+    a.Iin = e.Iin.p;
     uint64_t launched = 0;
     uint32_t batch = 4;
     while (true) {
         for (uint32_t q = 0; q < batch; ++q) {
             a.Bin = (launched & 1u) ? e.Blw1.p : e.Blw0.p;
             a.Bout = (launched & 1u) ? e.Blw0.p : e.Blw1.p;
-#define FH_COUPLED(T, W) hipLaunchKernelGGL((k_coupled<T, W>), dim3(grid), dim3(256), 0, c->stream, a)
+#define FH_COUPLED(T, W) hipLaunchKernelGGL((k_coupled_gather<T, W>), dim3(grid), dim3(256), 0, c->stream, a)
             if (team == 4) { if (w16) FH_COUPLED(4, true); else FH_COUPLED(4, false); }
             else if (team == 8) { if (w16) FH_COUPLED(8, true); else FH_COUPLED(8, false); }
             else if (team == 16) { if (w16) FH_COUPLED(16, true); else FH_COUPLED(16, false); }
             else if (team == 32) { if (w16) FH_COUPLED(32, true); else FH_COUPLED(32, false); }
             else { if (w16) FH_COUPLED(64, true); else FH_COUPLED(64, false); }
 #undef FH_COUPLED
-            hipLaunchKernelGGL(k_coupled_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, e.part4.p, grid,
+            hipLaunchKernelGGL(k_coupled_update, dim3(gridU), dim3(256), 0, c->stream, a);
+            hipLaunchKernelGGL(k_coupled_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, e.part4.p, gridU,
                                p.temperatureTolerance, e.last4.p);
             ++launched;
         }

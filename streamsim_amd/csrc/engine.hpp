@@ -141,6 +141,7 @@ struct EbState {
     DBuf<double> dsum;             // reduced FP64 sums
     DBuf<float2> fout;             // reduced (max dT, max residual-equivalent)
     DBuf<float4> part4;            // coupled-sweep per-block partials
+    DBuf<float> Iin;               // coupled sweep: incident before clamping, per side
     DBuf<float> last4;             // last coupled-sweep maxima
     DBuf<uint32_t> lit;
     // surrogate inputs / outputs per energy-balance unit (leaf facet or

@@ -266,7 +266,20 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_sun(const BvhNode* __rest
     ray_setup(r);
     float tbest;
     const uint32_t hf = traverse<true>(nodes, tris, r, f, tmin, stack + threadIdx.x, tbest);
-    if (hf == kSky) atomicAdd(&lit[side], 1u);
+    const bool sky = hf == kSky;
+    if (64u % K == 0u) {
+        // the K rays of a side are K consecutive lanes of one wave (all
+        // active: the facing test is per side): one add per side
+        const uint64_t m = __ballot(sky);
+        const uint32_t lane = threadIdx.x & 63u, first = lane - lane % K;
+        if (lane == first) {
+            const uint64_t bits = K == 64u ? m : (m >> first) & ((uint64_t(1) << K) - 1u);
+            const uint32_t n = uint32_t(__popcll(bits));
+            if (n) atomicAdd(&lit[side], n);
+        }
+    } else if (sky) {
+        atomicAdd(&lit[side], 1u);
+    }
 }
 
 __global__ void k_sun_fraction(const uint32_t* __restrict__ lit, uint32_t S, uint32_t K, float* __restrict__ frac) {

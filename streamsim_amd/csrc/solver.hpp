@@ -42,7 +42,7 @@ struct JacobiOptions {
     // BF16-gather solves (band-parallel layouts): the BF16 phase ends at
     // relative change firstInner (the BF16 operator's fixed point is only
     // about that close), then FP32 sweeps finish the solve
-    float firstInner = 1.0e-2f;
+    float firstInner = 2.0e-2f;
     bool retire = true;       // per-band retirement (tolRel >= 0, band-parallel)
     // optional incident H = max(0, fsky S + G + D) written by the judge
     // sweeps (band-parallel with retirement); *incidentDone reports whether

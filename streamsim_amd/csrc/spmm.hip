@@ -685,6 +685,9 @@ __global__ void k_split_optics(const float4* __restrict__ o, uint32_t S, float* 
 
 int sweep_team(const fh_context* c);
 
+// Row stride of the [S][ldb] band arrays: 1 band, else a multiple of 4
+// (16-byte vectors).  Padding wider rows to whole 128-byte lines measured
+// slower on gfx950 (more row-stream work for the same gathered lines).
 uint32_t band_stride(uint32_t nb) { return nb <= 1u ? 1u : (nb + 3u) & ~3u; }
 
 static int pick_nbc(uint32_t ldb) {
