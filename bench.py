@@ -59,10 +59,13 @@ def parse():
     return ap.parse_args()
 
 
-def sweep_bytes(S: int, nnz: int, nb: int, w: int) -> int:
+def sweep_bytes(S: int, nnz: int, nb: int, w: int, bf16_copy: bool = False) -> int:
     """Algorithmic HBM bytes of one sweep: row offsets, (col + value) per
-    entry, fsky, and per side-band c, r, t read + B read + B write."""
-    return 4 * (S + 1) + (4 + w) * nnz + 4 * S + 20 * S * nb
+    entry, fsky, and per side-band c, r, t read + B read + B write; the BF16
+    sweep also reads the BF16 copy of B it gathers once and writes the new
+    copy (2 + 2 bytes per side-band)."""
+    b = 4 * (S + 1) + (4 + w) * nnz + 4 * S + 20 * S * nb
+    return b + (4 * S * nb if bf16_copy else 0)
 
 
 def time_sweep(eng, nb: int, precision: int, reps: int) -> float:
@@ -72,7 +75,7 @@ def time_sweep(eng, nb: int, precision: int, reps: int) -> float:
 
 
 def cpu_baseline(eng, scene, windows, meteo, cfg, nb_total: int, budget_s: float = 25.0):
-    """The oracle (1 core) on one daytime node of the same graph: multi-band
+    """The oracle (its current thread count) on one daytime node of the same graph: multi-band
     SW solve, incident, absorbed, then the coupled FP64 energy balance, both
     to the benchmark's tolerances.  Returns facet-band-steps/s."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -123,7 +126,7 @@ def cpu_baseline(eng, scene, windows, meteo, cfg, nb_total: int, budget_s: float
 
 def cpu_baseline_sampled(eng, scene, windows, nb_total: int, st_plain):
     """For graphs too large to converge a node on one core within the bench:
-    the oracle (1 core) times one full-scene FP64 sweep of all shortwave
+    the oracle (its current thread count) times one full-scene FP64 sweep of all shortwave
     bands and one longwave sweep; a node is then priced with the plain GPU
     run's sweep counts per node.  Returns facet-band-steps/s."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -147,7 +150,7 @@ def cpu_baseline_sampled(eng, scene, windows, nb_total: int, st_plain):
     t_lw = time.perf_counter() - t0
     nodes = st_plain.nodes
     per_node = (sum(st_plain.sw_sweeps) * t_sw + sum(st_plain.lw_sweeps) * t_lw) / nodes
-    sample = (f"1 FP64 sweep of {nb} bands ({t_sw:.1f}s) + 1 LW sweep ({t_lw:.1f}s) over the full graph on 1 core, "
+    sample = (f"1 FP64 sweep of {nb} bands ({t_sw:.1f}s) + 1 LW sweep ({t_lw:.1f}s) over the full graph, "
               f"priced at the plain run's {sum(st_plain.sw_sweeps) / nodes:.1f} SW + "
               f"{sum(st_plain.lw_sweeps) / nodes:.1f} LW sweeps per node")
     return scene.facet_count * nb_total / per_node, t_sw + t_lw, sample
@@ -200,7 +203,8 @@ def main() -> None:
         for name, nbk, prec in (("sw_fp32", nb, FH_PRECISION_FP32), ("sw_bf16", nb, FH_PRECISION_BF16_DC),
                                 ("lw_fp32", 1, FH_PRECISION_FP32)):
             sec = time_sweep(eng, nbk, prec, a.roofline_reps)
-            alg = sweep_bytes(S, diag.directedEdgeCount, nbk, 4 if prec == FH_PRECISION_FP32 else 2)
+            alg = sweep_bytes(S, diag.directedEdgeCount, nbk, 4 if prec == FH_PRECISION_FP32 else 2,
+                              bf16_copy=prec != FH_PRECISION_FP32 and nbk >= 16)
             out[name] = {"us": sec * 1e6, "alg_bytes": alg, "GBps": alg / sec / 1e9}
         cs = eng.benchmarkCopy(1 << 30, 20)
         out["copy_1GiB"] = {"us": cs * 1e6, "GBps": 2.0 * (1 << 30) / cs / 1e9}
@@ -238,30 +242,43 @@ def main() -> None:
         ablation["surrogate_only"] = {"value": work / t_s, "seconds": t_s, **st_s.summary()}
     log(f"[bench] ablation {json.dumps({k: round(v['seconds'], 4) for k, v in ablation.items()})}")
 
-    # roofline of the dominant kernel: the shortwave SpMM sweep
+    # roofline of the dominant kernel, the FP32 shortwave SpMM sweep (all
+    # bands active), and of the BF16 sweep of the fast path's first phase
     nb = windows.count
     nnz = diag.directedEdgeCount
-    sec = time_sweep(eng, nb, FH_PRECISION_BF16_DC, a.roofline_reps)
+    sec16 = time_sweep(eng, nb, FH_PRECISION_BF16_DC, a.roofline_reps)
     sec32 = time_sweep(eng, nb, FH_PRECISION_FP32, a.roofline_reps)
-    alg = sweep_bytes(S, nnz, nb, 2)
+    alg16 = sweep_bytes(S, nnz, nb, 2, bf16_copy=True)
     alg32 = sweep_bytes(S, nnz, nb, 4)
-    achieved = alg / sec / 1e9
-    log(f"[bench] SW sweep nb={nb}: bf16 {sec * 1e6:.1f} us ({achieved:.0f} GB/s), "
-        f"fp32 {sec32 * 1e6:.1f} us ({alg32 / sec32 / 1e9:.0f} GB/s)")
-    traffic = None
+    achieved = alg32 / sec32 / 1e9
+    log(f"[bench] SW sweep nb={nb}: fp32 {sec32 * 1e6:.1f} us ({achieved:.0f} GB/s), "
+        f"bf16 {sec16 * 1e6:.1f} us ({alg16 / sec16 / 1e9:.0f} GB/s)")
+    traffic32 = traffic16 = None
     tf = ROOT / "profiles" / f"sweep_traffic_{a.workload}.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        tj = json.loads(tf.read_text())
+        traffic32 = tj.get("k_sweep_v", {}).get("bytes_per_launch")
+        traffic16 = tj.get("k_sweep_hv", {}).get("bytes_per_launch")
 
-    cpu = None
+    cpu = cpu_all = None
     if not a.no_cpu_baseline:
-        if nnz <= 50_000_000:
-            v, dt, sample = cpu_baseline(eng, scene, windows, meteo, cfg, nb_total)
-        else:
-            v, dt, sample = cpu_baseline_sampled(eng, scene, windows, nb_total, st_plain)
-        cpu = {"value": v, "unit": "facet-band-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{sample}; {dt:.1f}s on 1 core"}
-        log(f"[bench] cpu baseline {v:.4g} facet-band-steps/s ({dt:.1f}s)")
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_ref as O
+        ncores = O.max_threads()
+        for threads in (1, ncores):
+            O.set_threads(threads)
+            if nnz <= 50_000_000:
+                v, dt, sample = cpu_baseline(eng, scene, windows, meteo, cfg, nb_total)
+            else:
+                v, dt, sample = cpu_baseline_sampled(eng, scene, windows, nb_total, st_plain)
+            rec = {"value": v, "unit": "facet-band-steps/s", "cores": threads, "kind": "port",
+                   "sample": f"{sample}; {dt:.1f}s on {threads} core{'s' if threads > 1 else ''}"}
+            log(f"[bench] cpu baseline ({threads} cores) {v:.4g} facet-band-steps/s ({dt:.1f}s)")
+            if threads == 1:
+                cpu = rec
+            else:
+                cpu_all = rec
+        O.set_threads(ncores)
 
     out = {
         "metric": "converged facet-band-steps/s at 0.01 K: surrogate+BF16 vs plain FP32; HBM GB/s",
@@ -286,7 +303,8 @@ def main() -> None:
             "time_steps": steps,
             "tolerance_K": cfg.tolerance,
             "radiosity_rel_tol": cfg.radiosity_tolerance,
-            "fast_path": "bf16-F + fp32 defect correction" + (" + surrogate warm start" if have_sur else ""),
+            "fast_path": "shortwave: BF16 F and BF16-gathered B, then FP32 sweeps to the FP32 fixed point"
+                         + (" + surrogate warm start" if have_sur else ""),
             "parallelism": "single GPU",
         },
         "plain_fp32": {"value": plain_value, "seconds": t_plain, **st_plain.summary()},
@@ -296,11 +314,14 @@ def main() -> None:
         "form_factors": {"seconds": t_graph, "rays_per_s": rays_per_s, "directed_edges": nnz,
                          "bvh_host_seconds": t_bvh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"k_sweep shortwave nb={nb} bf16", "bytes_per_launch": alg,
-                     "us_per_launch": sec * 1e6, "fp32_us_per_launch": sec32 * 1e6,
-                     "fp32_achieved": alg32 / sec32 / 1e9},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic32,
+                     "kernel": f"k_sweep_v: FP32 shortwave sweep, {nb} bands", "bytes_per_launch": alg32,
+                     "us_per_launch": sec32 * 1e6,
+                     "bf16_sweep": {"kernel": f"k_sweep_hv: BF16-gather shortwave sweep, {nb} bands",
+                                    "achieved": alg16 / sec16 / 1e9, "bytes_per_launch": alg16,
+                                    "us_per_launch": sec16 * 1e6, "traffic": traffic16}},
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(out), flush=True)
     eng.destroy()

@@ -12,6 +12,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define OR_SKY 0xffffffffu
 
@@ -1003,12 +1006,17 @@ uint32_t or_solve_bands(const or_graph *g, uint32_t nb, const float *r, const fl
     const size_t n = (size_t)S * nb;
     float *cur = (float *)calloc(n, sizeof(float));
     float *nxt = (float *)calloc(n, sizeof(float));
-    double *in = (double *)malloc(sizeof(double) * 2u * nb);
     if (init) memcpy(cur, init, sizeof(float) * n);
     uint32_t it = 0;
     double lastRel = 0.0;
     while (it < maxIter) {
         double maxRel = 0.0;
+        /* facets are independent within a sweep: the threads split them, every
+           value is computed as in the single-threaded loop */
+#pragma omp parallel reduction(max : maxRel)
+        {
+        double *in = (double *)malloc(sizeof(double) * 2u * nb);
+#pragma omp for schedule(dynamic, 256)
         for (uint32_t f = 0; f < S / 2u; ++f) {
             for (uint32_t k = 0; k < 2u; ++k) {
                 const uint32_t s = 2u * f + k;
@@ -1036,6 +1044,8 @@ uint32_t or_solve_bands(const or_graph *g, uint32_t nb, const float *r, const fl
                 if (d1 > maxRel) maxRel = d1;
             }
         }
+        free(in);
+        }
         float *tmp = cur;
         cur = nxt;
         nxt = tmp;
@@ -1047,8 +1057,25 @@ uint32_t or_solve_bands(const or_graph *g, uint32_t nb, const float *r, const fl
     if (lastRelOut) *lastRelOut = lastRel;
     free(cur);
     free(nxt);
-    free(in);
     return it;
+}
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
+
+int or_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* core:1531-1559 */

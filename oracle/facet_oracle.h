@@ -121,6 +121,12 @@ uint32_t or_solve_bands(const or_graph *g, uint32_t nb, const float *r,
                         uint32_t maxIter, double tolRel, float relaxation,
                         float *B, double *lastRel);
 
+/* Threads of the multi-band solve (OpenMP over facets; the results do not
+ * depend on the count).  The single-threaded run is the restatement of the
+ * reference's single-threaded CPU route; all cores are its parallel variant. */
+void or_set_threads(int n);
+int or_max_threads(void);
+
 /* core:1531-1559 */
 void or_incident(const or_graph *g, const float *B, const or_optics *optics,
                  float sky, float *H);

@@ -164,6 +164,7 @@ struct MlpState {
     uint64_t adamStep = 0;         // Adam steps taken since the parameters were set
     // training workspace
     DBuf<float> A1, H1, A2, H2, y, dA2, dA1, grad, m, v, red;
+    DBuf<float> W2T;               // layer-2 weights transposed (matrix-core inference)
 };
 
 }  // namespace fh

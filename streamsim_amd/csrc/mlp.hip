@@ -184,12 +184,102 @@ __global__ __launch_bounds__(128) void k_infer(const float* __restrict__ X, uint
     y[b] = out;
 }
 
+// ---- inference on the matrix cores (hidden width 128) -----------------------
+// W2T[q][j] = W2[j][q]: the layer-2 B operand read row-contiguously
+__global__ void k_transpose_w2(const float* __restrict__ params, uint32_t H, float* __restrict__ W2T) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= H * H) return;
+    const float* W2 = params + H * kIn + H;
+    const uint32_t q = i / H, j = i - q * H;
+    W2T[i] = W2[j * H + q];
+}
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// A block of 256 threads (4 waves) evaluates 96 units.  Layer 1 (9 -> 128)
+// on the vector lanes into LDS; layer 2 (96 x 128 times 128 x 128) on the
+// matrix cores with v_mfma_f32_32x32x2f32 (FP32 in, FP32 accumulate): wave
+// w owns output columns [32 w, 32 w + 32) for the block's 3 row tiles of 32
+// units, loading each B operand once for the 3 tiles; the epilogue applies
+// b2, GELU and the output layer per column, sums the 32 columns across the
+// lanes and the 4 column tiles through LDS.
+// Operand layout of the 32x32x2 instruction (wave64): lane l supplies
+// A[l % 32][l / 32] and B[l / 32][l % 32]; accumulator register r of lane l
+// holds C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+constexpr int kMfmaUnits = 96;
+__global__ __launch_bounds__(256) void k_infer_mfma(const float* __restrict__ X, uint32_t n,
+                                                    const float* __restrict__ params, const float* __restrict__ W2T,
+                                                    float* __restrict__ y) {
+    constexpr int H = 128, TU = kMfmaUnits, RT = TU / 32;
+    __shared__ float sH1[TU][H + 1];
+    __shared__ float sOut[TU][4];
+    const float* W1 = params;
+    const float* b1 = W1 + H * kIn;
+    const float* b2 = b1 + H + H * H;
+    const float* W3 = b2 + H;
+    const float b3 = W3[H];
+    const uint32_t u0 = blockIdx.x * TU;
+    for (uint32_t i = threadIdx.x; i < uint32_t(TU * H); i += 256u) {
+        const uint32_t u = i / H, j = i - u * H, unit = u0 + u;
+        float a = b1[j];
+        if (unit < n) {
+#pragma unroll
+            for (int q = 0; q < kIn; ++q) a += W1[j * kIn + q] * X[size_t(unit) * kIn + q];
+        }
+        sH1[u][j] = unit < n ? gelu_f(a) : 0.0f;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 31, lk = lane >> 5;
+    f32x16 acc[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
+    for (int k = 0; k < H; k += 2) {
+        const float bv = W2T[(k + lk) * H + w * 32 + li];
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(sH1[t * 32 + li][k + lk], bv, acc[t], 0, 0, 0);
+    }
+    const int j = w * 32 + li;
+    const float bj = b2[j], wj = W3[j];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        float part[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) part[r] = wj * gelu_f(acc[t][r] + bj);
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+#pragma unroll
+            for (int m = 16; m > 0; m >>= 1) part[r] += __shfl_xor(part[r], m, 64);
+        if (li == 0)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) sOut[t * 32 + 8 * (r >> 2) + 4 * lk + (r & 3)][w] = part[r];
+    }
+    __syncthreads();
+    if (threadIdx.x < uint32_t(TU)) {
+        const uint32_t unit = u0 + threadIdx.x;
+        if (unit < n) y[unit] = b3 + sOut[threadIdx.x][0] + sOut[threadIdx.x][1] + sOut[threadIdx.x][2] +
+                                sOut[threadIdx.x][3];
+    }
+}
+
 static void infer(fh_context* c, const float* X, uint32_t n, float* y) {
     const uint32_t H = c->mlp.hidden;
+    if (H == 128) {
+        c->mlp.W2T.ensure(size_t(H) * H);
+        hipLaunchKernelGGL(k_transpose_w2, dim3((H * H + 255u) / 256u), dim3(256), 0, c->stream, c->mlp.params.p, H,
+                           c->mlp.W2T.p);
+        const uint32_t blocks = (n + kMfmaUnits - 1u) / kMfmaUnits;
+        hipLaunchKernelGGL(k_infer_mfma, dim3(blocks ? blocks : 1u), dim3(256), 0, c->stream, X, n, c->mlp.params.p,
+                           c->mlp.W2T.p, y);
+        FH_HIP(hipGetLastError());
+        return;
+    }
     const dim3 grid((n + 127u) / 128u ? (n + 127u) / 128u : 1u);
     if (H == 32) hipLaunchKernelGGL(k_infer<32>, grid, dim3(128), 0, c->stream, X, n, c->mlp.params.p, y);
     else if (H == 64) hipLaunchKernelGGL(k_infer<64>, grid, dim3(128), 0, c->stream, X, n, c->mlp.params.p, y);
-    else if (H == 128) hipLaunchKernelGGL(k_infer<128>, grid, dim3(128), 0, c->stream, X, n, c->mlp.params.p, y);
     else fail(FH_INVALID_ARGUMENT, "inference supports hidden widths 32, 64 and 128");
     FH_HIP(hipGetLastError());
 }

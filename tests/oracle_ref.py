@@ -64,6 +64,8 @@ def load():
         "or_ray_generate": (None, [_P, _U, _U, _U, _P, _P]),
         "or_solve": (None, [C.POINTER(OrGraph), _P, _F, _U, _F, _P, _P]),
         "or_solve_bands": (_U, [C.POINTER(OrGraph), _U, _P, _P, _P, _P, _P, _P, _U, _D, _F, _P, _P]),
+        "or_set_threads": (None, [_I]),
+        "or_max_threads": (_I, []),
         "or_incident": (None, [C.POINTER(OrGraph), _P, _P, _F, _P]),
         "or_incident_bands": (None, [C.POINTER(OrGraph), _U, _P, _P, _P, _P]),
         "or_solve_direct": (_I, [C.POINTER(OrGraph), _P, _F, _P]),
@@ -283,3 +285,12 @@ def mlp_loss_grad(params, x, target, hidden: int):
     g = np.zeros(p.size, np.float64)
     loss = load().or_mlp_loss_grad(_p(p), fin, hidden, _p(x), _p(np.ascontiguousarray(target, np.float32)), n, _p(g))
     return float(loss), g
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's multi-band solve (results do not depend on it)."""
+    load().or_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(load().or_max_threads())

@@ -297,6 +297,22 @@ def test_mlp_gradient_matches_oracle(oracle):
     np.testing.assert_allclose(y, oracle.mlp_forward(P, x, H), rtol=1e-4, atol=1e-5)
 
 
+def test_mlp_forward_width128_matrix_cores(oracle):
+    # width 128 runs layer 2 on the matrix cores (v_mfma_f32_32x32x2f32):
+    # against the FP64-accumulating oracle on a unit count that leaves a
+    # partial block of 96
+    print("mlp forward width 128 case", flush=True)
+    eng = _engine()
+    rng = np.random.default_rng(11)
+    H = 128
+    P = rng.normal(0, 0.15, eng.mlpParamCount(H)).astype(np.float32)
+    eng.mlpSetParams(H, P)
+    x = rng.normal(0, 1, (1000, 9)).astype(np.float32)
+    y = eng.mlpForward(x)
+    ref = oracle.mlp_forward(P, x, H)
+    np.testing.assert_allclose(y, ref, rtol=1e-4, atol=1e-4 * np.max(np.abs(ref)))
+
+
 def test_surrogate_warm_start_gain():
     # the committed network (trained on the device by fh_mlp_train on forest
     # and urban scenes) warm-starts a held-out row crop: the coupled iteration

@@ -226,6 +226,30 @@ def test_solve_matches_direct_lu(oracle):
     np.testing.assert_allclose(Bb[:, 1], Bd, rtol=1e-7)
 
 
+def test_solve_bands_thread_count_independent(oracle):
+    # the all-core variant (OpenMP over facets) computes every value as the
+    # single-threaded restatement does
+    sc = scenes.row_crop(leaves_per_row=25, cells=5)
+    g = oracle.ray_graph(sc.vertices, 3, 64)
+    rng = np.random.default_rng(1)
+    S, nb = g.S, 5
+    r = rng.uniform(0.0, 0.5, (S, nb)).astype(np.float32)
+    t = (rng.uniform(0.0, 1.0, (S, nb)) * (1.0 - r) * 0.8).astype(np.float32)
+    e = rng.uniform(0.0, 5.0, (S, nb)).astype(np.float32)
+    D = rng.uniform(0.0, 300.0, (S, nb)).astype(np.float32)
+    sky = rng.uniform(10.0, 80.0, nb).astype(np.float32)
+    n = oracle.max_threads()
+    try:
+        oracle.set_threads(1)
+        B1, it1, l1 = oracle.solve_bands(g, r, t, e, D, sky, tol_rel=1e-8)
+        oracle.set_threads(max(n, 4))
+        Bn, itn, ln = oracle.solve_bands(g, r, t, e, D, sky, tol_rel=1e-8)
+    finally:
+        oracle.set_threads(n)
+    assert it1 == itn and l1 == ln
+    assert np.array_equal(B1, Bn)
+
+
 def test_incident_definition(oracle):
     # H = max(0, (sky S + sum count B) / den + D)
     sc = scenes.row_crop(leaves_per_row=10, cells=3)
