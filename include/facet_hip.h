@@ -255,7 +255,7 @@ typedef struct {
     float maxTemperatureDelta;
     float maxResidualEquivalent;
     uint32_t swRefreshes;
-    uint32_t reserved;
+    float maxEnergyResidual;  /* max |energy residual| of the final check, W m-2 */
     double swInput;      /* sum A (D + fsky S) over sides and bands, W     */
     double swAbsorbed;   /* sum A alpha H                                  */
     double swEscaped;    /* sum A fsky B                                   */

@@ -83,7 +83,7 @@ class FhEbNodeStats(C.Structure):
     _fields_ = [
         ("couplingIterations", C.c_uint32), ("converged", C.c_uint32), ("swSweeps", C.c_uint32),
         ("lwSweeps", C.c_uint32), ("maxTemperatureDelta", C.c_float), ("maxResidualEquivalent", C.c_float),
-        ("swRefreshes", C.c_uint32), ("reserved", C.c_uint32), ("swInput", C.c_double),
+        ("swRefreshes", C.c_uint32), ("maxEnergyResidual", C.c_float), ("swInput", C.c_double),
         ("swAbsorbed", C.c_double), ("swEscaped", C.c_double), ("netRadiation", C.c_double),
     ]
 

@@ -200,7 +200,8 @@ __device__ __forceinline__ float lw_emission(float eps, float T) {
 }
 
 struct NewtonOut {
-    float dT, eq;
+    float dT, eq;  // temperature step, |residual| / max(1, thermal response)
+    float res;     // |residual|, W m-2
     double rn;
 };
 
@@ -211,7 +212,7 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
                                   float* __restrict__ T, const float* __restrict__ Tprev, bool update,
                                   float* __restrict__ Rn_out, float* __restrict__ H_out, float* __restrict__ LE_out,
                                   float* __restrict__ G_out, const double* __restrict__ area) {
-    NewtonOut o{0.0f, 0.0f, 0.0};
+    NewtonOut o{0.0f, 0.0f, 0.0f, 0.0};
     const uint32_t s0 = 2u * f;
     const float Hs[2] = {H0, H1};
     double rn[2];
@@ -241,6 +242,7 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
         const double res = net - hs - le - g;
         const double der = 8.0 * double(eps[s0]) * kSigma * Tl * Tl * Tl + 2.0 * k.ha * m.convectiveScale + dle + gd;
         o.eq = float(fabs(res) / fmax(1.0, der));
+        o.res = float(fabs(res));
         o.rn = area[f] * net;
         if (Rn_out)
             for (int q = 0; q < 2; ++q) {
@@ -275,6 +277,7 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
             const double res = rn[q] - hs - le - g;
             const double der = 4.0 * double(eps[s]) * kSigma * Ts * Ts * Ts + k.ha + dle + gd;
             o.eq = fmaxf(o.eq, float(fabs(res) / fmax(1.0, der)));
+            o.res = fmaxf(o.res, float(fabs(res)));
             o.rn += area[f] * rn[q];
             if (Rn_out) {
                 Rn_out[s] = float(rn[q]);
@@ -303,25 +306,26 @@ __global__ __launch_bounds__(256) void k_newton(uint32_t F, EbConst k, const uin
                                                 const float* __restrict__ Tprev, int update, float* __restrict__ Rn_out,
                                                 float* __restrict__ H_out, float* __restrict__ LE_out,
                                                 float* __restrict__ G_out, const double* __restrict__ area,
-                                                float2* __restrict__ part, double* __restrict__ dpart) {
+                                                float4* __restrict__ part, double* __restrict__ dpart) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    float v[2] = {0.0f, 0.0f};
+    float v[3] = {0.0f, 0.0f, 0.0f};
     double rnSum = 0.0;
     if (f < F) {
         const NewtonOut o = newton_facet(f, k, mats[mat[f]], absorbed, H[2u * f], H[2u * f + 1], eps, T, Tprev,
                                          update != 0, Rn_out, H_out, LE_out, G_out, area);
         v[0] = o.dT;
         v[1] = o.eq;
+        v[2] = o.res;
         rnSum = o.rn;
     }
 #pragma unroll
     for (int msk = 32; msk > 0; msk >>= 1) rnSum += __shfl_xor(rnSum, msk, 64);
     __shared__ double srn[4];
     if ((threadIdx.x & 63) == 0) srn[threadIdx.x >> 6] = rnSum;
-    float out[2];
-    block_max<2>(v, out);
+    float out[3];
+    block_max<3>(v, out);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = make_float2(out[0], out[1]);
+        part[blockIdx.x] = make_float4(out[0], out[1], out[2], 0.0f);
         dpart[size_t(blockIdx.x) * 4 + 3] = (srn[0] + srn[1]) + (srn[2] + srn[3]);
     }
 }
@@ -453,6 +457,23 @@ __global__ __launch_bounds__(1024) void k_coupled_finalize(SolveCtrl* ctrl, cons
     }
 }
 
+// max over n float4 partials, component-wise, into out[0..3]
+__global__ __launch_bounds__(1024) void k_reduce_max4(const float4* __restrict__ part, uint32_t n,
+                                                      float* __restrict__ out) {
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t i = threadIdx.x; i < n; i += 1024u) {
+        const float4 p = part[i];
+        m[0] = fmaxf(m[0], p.x);
+        m[1] = fmaxf(m[1], p.y);
+        m[2] = fmaxf(m[2], p.z);
+        m[3] = fmaxf(m[3], p.w);
+    }
+    float r[4];
+    block_max_1024<4>(m, r);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) out[q] = r[q];
+}
+
 __global__ __launch_bounds__(1024) void k_reduce_max2(const float2* __restrict__ part, uint32_t n,
                                                       float2* __restrict__ out) {
     float a = 0.0f, b = 0.0f;
@@ -571,25 +592,28 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     e.ready = true;
 }
 
-static void newton(fh_context* c, const EbConst& k, int update, float& maxDT, float& maxEq, double* rnSum) {
+static void newton(fh_context* c, const EbConst& k, int update, float& maxDT, float& maxEq, double* rnSum,
+                   float* maxRes = nullptr) {
     EbState& e = c->eb;
     const uint32_t F = c->F;
     const uint32_t nbk = (F + 255u) / 256u;
-    c->partial.ensure(nbk);
+    e.part4.ensure(nbk);
+    e.last4.ensure(4);
     e.dpart.ensure(size_t(nbk) * 4u);
     hipLaunchKernelGGL(k_newton, dim3(nbk), dim3(256), 0, c->stream, F, k, e.material.p, e.mats.p, e.absorbed.p,
                        e.Hlw.p, e.eps.p, e.T.p, e.Tprev.p, update, e.Rn.p, e.Hs.p, e.LE.p, e.G.p, e.area.p,
-                       c->partial.p, e.dpart.p);
-    hipLaunchKernelGGL(k_reduce_max2, dim3(1), dim3(1024), 0, c->stream, c->partial.p, nbk, e.fout.p);
+                       e.part4.p, e.dpart.p);
+    hipLaunchKernelGGL(k_reduce_max4, dim3(1), dim3(1024), 0, c->stream, e.part4.p, nbk, e.last4.p);
     if (rnSum) hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(256), 0, c->stream, e.dpart.p, nbk, 3u, 4u, e.dsum.p);
     FH_HIP(hipGetLastError());
-    float2 h;
-    FH_HIP(hipMemcpyAsync(&h, e.fout.p, sizeof(float2), hipMemcpyDeviceToHost, c->stream));
+    float h[4];
+    FH_HIP(hipMemcpyAsync(h, e.last4.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
     double d4[4] = {0, 0, 0, 0};
     if (rnSum) FH_HIP(hipMemcpyAsync(d4, e.dsum.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
     FH_HIP(hipStreamSynchronize(c->stream));
-    maxDT = h.x;
-    maxEq = h.y;
+    maxDT = h[0];
+    maxEq = h[1];
+    if (maxRes) *maxRes = h[2];
     if (rnSum) *rnSum = d4[3];
 }
 
@@ -798,17 +822,19 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
     st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
     float dtFinal = 0.0f, maxEq = 0.0f;
     double rn = 0.0;
-    newton(c, k, 0, dtFinal, maxEq, &rn);
+    float maxRes = 0.0f;
+    newton(c, k, 0, dtFinal, maxEq, &rn, &maxRes);
     if (bf16 && !(maxDT <= p.temperatureTolerance && maxEq <= p.temperatureTolerance)) {
         iters += coupled_loop(c, p, k, false, std::max(1u, p.maxCoupling), last);
         maxDT = last[2];
         st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
-        newton(c, k, 0, dtFinal, maxEq, &rn);
+        newton(c, k, 0, dtFinal, maxEq, &rn, &maxRes);
     }
     st.lwSweeps += iters;
     st.couplingIterations = iters;
     st.maxTemperatureDelta = maxDT;
     st.maxResidualEquivalent = maxEq;
+    st.maxEnergyResidual = maxRes;
     st.converged = (maxDT <= p.temperatureTolerance && maxEq <= p.temperatureTolerance) ? 1u : 0u;
     st.netRadiation = rn;
     if (out) *out = st;

@@ -108,4 +108,24 @@ __device__ __forceinline__ void block_max(float (&v)[N], float (&out)[N]) {
     for (int q = 0; q < N; ++q) out[q] = fmaxf(fmaxf(sh[0][q], sh[1][q]), fmaxf(sh[2][q], sh[3][q]));
 }
 
+// max-reduce n floats of a 1024-thread block; every thread gets the result
+template <int N>
+__device__ __forceinline__ void block_max_1024(float (&v)[N], float (&out)[N]) {
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) v[q] = fmaxf(v[q], __shfl_xor(v[q], m, 64));
+    __shared__ float sh[16][N];
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < N; ++q) sh[threadIdx.x >> 6][q] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        float r = sh[0][q];
+        for (int w = 1; w < 16; ++w) r = fmaxf(r, sh[w][q]);
+        out[q] = r;
+    }
+}
+
 }  // namespace fh

@@ -21,6 +21,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <ctime>
+#include <cstdlib>
 #include <filesystem>
 #include <fstream>
 #include <iomanip>
@@ -240,9 +242,326 @@ int runFacetRT(const std::string& input, const std::string& outFile) try {
     return 1;
 }
 
+// ---- FacetEB run settings ---------------------------------------------------
+// The reference reads these from the input's XML tags (core:3193-3240); an
+// OBJ input carries none, so they come from the environment here, with the
+// reference's defaults and clamps:
+//   FACET_EB_START / FACET_EB_END      startTimeNode / endTimeNode (0 / 1)
+//   FACET_EB_DTIME                     dTime, s (1800)
+//   FACET_EB_METEO                     meteorology file (filePath): a header
+//                                      line, then rows "julianTime Tair[C]
+//                                      vapourPressure[hPa] pressure[hPa]
+//                                      wind[m/s] shortwave[W/m2] longwave[W/m2]"
+//   FACET_EB_LATITUDE / _LONGITUDE     Latitude / Longitude (40 / 116)
+//   FACET_EB_PROCESS                   energy | radiation | both
+//                                      (isEnergyProcess / isRadiationProcess)
+//   FACET_EB_TOLERANCE, FACET_EB_COUPLING, FACET_EB_RELAXATION
+//                                      temperatureTolerance (0.05), couplingIterations (20),
+//                                      temperatureRelaxation (0.5)
+//   FACET_EB_SKY_TEMPERATURE           skyTemperature, K (250)
+//   FACET_EB_DIRECT_RATIO              directScatteringRatio (0.8)
+//   FACET_EB_SPECTRAL_WIDTH            spectralAccelerationWidth, 1-nm samples (100)
+//   STREAMSIM_FACET_STEP_SYNC=1        FACET_STEP / FACET_ACK back-pressure (core:3438-3441)
+double envNumber(const char* name, double fallback) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return fallback;
+    char* end = nullptr;
+    const double x = std::strtod(v, &end);
+    return (end != v && std::isfinite(x)) ? x : fallback;
+}
+
+struct EbSettings {
+    int startNode = 0, endNode = 1;
+    float dTime = 1800.0f;
+    float latitude = 40.0f, longitude = 116.0f;
+    float skyTemperature = 250.0f, directRatio = 0.8f;
+    float tolerance = 0.05f, relaxation = 0.5f;
+    uint32_t couplingIterations = 20;
+    uint32_t spectralWidth = 100;
+    bool saveEnergy = false, saveRadiation = false, stepSync = false;
+    std::string meteoFile;
+};
+
+EbSettings ebSettings() {
+    EbSettings s;
+    s.startNode = std::max(0, int(std::lround(envNumber("FACET_EB_START", 0.0))));
+    s.endNode = std::max(s.startNode + 1, int(std::lround(envNumber("FACET_EB_END", s.startNode + 1))));
+    s.dTime = std::max(1.0f, float(envNumber("FACET_EB_DTIME", 1800.0)));
+    s.latitude = float(envNumber("FACET_EB_LATITUDE", 40.0));
+    s.longitude = float(envNumber("FACET_EB_LONGITUDE", 116.0));
+    s.skyTemperature = std::clamp(float(envNumber("FACET_EB_SKY_TEMPERATURE", 250.0)), 150.0f, 350.0f);
+    s.directRatio = std::clamp(float(envNumber("FACET_EB_DIRECT_RATIO", 0.8)), 0.0f, 1.0f);
+    s.tolerance = std::clamp(float(envNumber("FACET_EB_TOLERANCE", 0.05)), 0.001f, 10.0f);
+    s.relaxation = std::clamp(float(envNumber("FACET_EB_RELAXATION", 0.5)), 0.05f, 1.0f);
+    s.couplingIterations = uint32_t(std::clamp(int(std::lround(envNumber("FACET_EB_COUPLING", 20.0))), 1, 100));
+    s.spectralWidth = uint32_t(std::clamp(int(std::lround(envNumber("FACET_EB_SPECTRAL_WIDTH", 100.0))), 1, 1000));
+    if (const char* v = std::getenv("FACET_EB_PROCESS")) {
+        const std::string p = v;
+        s.saveEnergy = p == "energy" || p == "both";
+        s.saveRadiation = p == "radiation" || p == "both";
+    }
+    if (const char* v = std::getenv("FACET_EB_METEO")) s.meteoFile = v;
+    const char* sync = std::getenv("STREAMSIM_FACET_STEP_SYNC");
+    s.stepSync = sync && std::string(sync) == "1";
+    return s;
+}
+
+struct MeteoStep {  // core:3172-3181
+    int node = 0;
+    float julianTime = 0.0f;
+    float airTemperature = 25.0f;
+    float vaporPressure = 15.0f;
+    float pressure = 839.0f;
+    float windSpeed = 1.0f;
+    float shortwave = 0.0f;
+    float longwave = 320.0f;
+};
+
+// meteorology rows (core:3240-3268): a file, else defaults at node times
+std::vector<MeteoStep> loadMeteorology(const EbSettings& s) {
+    std::vector<MeteoStep> rows;
+    if (!s.meteoFile.empty()) {
+        std::ifstream in(s.meteoFile);
+        if (!in) throw std::runtime_error("FacetEB cannot open meteorology: " + s.meteoFile);
+        std::string line;
+        std::getline(in, line);
+        int row = 0;
+        while (std::getline(in, line)) {
+            std::istringstream v(line);
+            MeteoStep m;
+            m.node = row++;
+            if (!(v >> m.julianTime >> m.airTemperature >> m.vaporPressure >> m.pressure >> m.windSpeed >>
+                  m.shortwave >> m.longwave))
+                continue;
+            rows.push_back(m);
+        }
+    }
+    if (rows.empty()) {
+        rows.resize(size_t(s.endNode));
+        for (int n = 0; n < s.endNode; ++n) {
+            rows[size_t(n)].node = n;
+            rows[size_t(n)].julianTime = 1.0f + float(n) * s.dTime / 86400.0f;
+        }
+    }
+    if (s.endNode > int(rows.size())) throw std::runtime_error("FacetEB time range exceeds meteorology rows");
+    return rows;
+}
+
+// "DOY<day>_HH-MM" (core:3276-3288)
+std::string timeToken(float julianTime) {
+    int day = int(std::floor(julianTime));
+    int minutes = int(std::lround((double(julianTime) - day) * 1440.0));
+    if (minutes >= 1440) {
+        day += minutes / 1440;
+        minutes %= 1440;
+    }
+    std::ostringstream t;
+    t << "DOY" << day << '_' << std::setw(2) << std::setfill('0') << minutes / 60 << '-' << std::setw(2)
+      << std::setfill('0') << minutes % 60;
+    return t.str();
+}
+
+// Solar zenith and azimuth (degrees; azimuth eastward from north) for a
+// local clock time in a time zone hours east of UTC.  Low-precision solar
+// coordinates after Meeus, Astronomical Algorithms (2nd ed.), ch. 7, 12,
+// 25 (mean longitude and anomaly, equation of the centre, apparent longitude,
+// obliquity, Greenwich mean sidereal time), with the atmospheric refraction
+// correction of Reda & Andreas, Solar Position Algorithm (NREL/TP-560-34302,
+// eq. 42).  The reference runs the full SPA (thirdparty/spa.cpp) with
+// timezone 8 and elevation 100 m; this form agrees with SPA's published
+// example to about 0.01 degree (test_host.py).
+struct SunAngles {
+    double zenith, azimuth;
+};
+
+SunAngles solarPosition(int year, int month, int day, int hour, int minute, double second, double timezone,
+                        double longitude, double latitude, double pressure, double temperature) {
+    constexpr double d2r = 3.14159265358979323846 / 180.0;
+    int y = year, m = month;
+    if (m <= 2) {
+        y -= 1;
+        m += 12;
+    }
+    const double a = std::floor(y / 100.0), b = 2.0 - a + std::floor(a / 4.0);
+    const double dayFrac = day + (hour - timezone + minute / 60.0 + second / 3600.0) / 24.0;
+    const double jd = std::floor(365.25 * (y + 4716)) + std::floor(30.6001 * (m + 1)) + dayFrac + b - 1524.5;
+    const double t = (jd - 2451545.0) / 36525.0;
+    const double l0 = std::fmod(280.46646 + t * (36000.76983 + t * 0.0003032), 360.0);
+    const double ma = 357.52911 + t * (35999.05029 - 0.0001537 * t);
+    const double c = (1.914602 - t * (0.004817 + 0.000014 * t)) * std::sin(ma * d2r) +
+                     (0.019993 - 0.000101 * t) * std::sin(2.0 * ma * d2r) + 0.000289 * std::sin(3.0 * ma * d2r);
+    const double omega = 125.04 - 1934.136 * t;
+    const double lambda = l0 + c - 0.00569 - 0.00478 * std::sin(omega * d2r);
+    const double eps0 = 23.0 + (26.0 + (21.448 - t * (46.815 + t * (0.00059 - t * 0.001813))) / 60.0) / 60.0;
+    const double eps = eps0 + 0.00256 * std::cos(omega * d2r);
+    const double ra = std::atan2(std::cos(eps * d2r) * std::sin(lambda * d2r), std::cos(lambda * d2r));
+    const double dec = std::asin(std::sin(eps * d2r) * std::sin(lambda * d2r));
+    const double gmst = 280.46061837 + 360.98564736629 * (jd - 2451545.0) + t * t * (0.000387933 - t / 38710000.0);
+    const double h = (gmst + longitude) * d2r - ra;
+    const double lat = latitude * d2r;
+    const double sinE = std::sin(lat) * std::sin(dec) + std::cos(lat) * std::cos(dec) * std::cos(h);
+    const double e0 = std::asin(std::clamp(sinE, -1.0, 1.0)) / d2r;
+    double de = 0.0;
+    if (e0 >= -(0.26667 + 0.5667))
+        de = (pressure / 1010.0) * (283.0 / (273.0 + temperature)) * 1.02 /
+             (60.0 * std::tan((e0 + 10.3 / (e0 + 5.11)) * d2r));
+    const double az = std::atan2(std::sin(h), std::cos(h) * std::sin(lat) - std::tan(dec) * std::cos(lat)) / d2r;
+    return {90.0 - (e0 + de), std::fmod(az + 180.0 + 360.0, 360.0)};
+}
+
+// calendar day of a 2019 day of year (the reference's Utils::calculateMonthAndDay(2019, ...))
+void monthAndDay(int dayOfYear, int& month, int& day) {
+    static const int len[12] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+    int d = std::max(1, dayOfYear);
+    month = 1;
+    while (month <= 12 && d > len[month - 1]) {
+        d -= len[month - 1];
+        ++month;
+    }
+    if (month > 12) {
+        month = 12;
+        d = 31;
+    }
+    day = d;
+}
+
+// updateSolarPosition (core:3290-3330): whole-minute local time, timezone 8
+SunAngles nodeSun(const MeteoStep& m, const EbSettings& s) {
+    const int doy = int(std::floor(m.julianTime));
+    int month = 1, day = 1;
+    monthAndDay(doy, month, day);
+    const double hours = (double(m.julianTime) - doy) * 24.0;
+    const int hour = int(std::floor(hours));
+    const int minute = int(std::floor((hours - hour) * 60.0));
+    return solarPosition(2019, month, day, std::clamp(hour, 0, 23), std::clamp(minute, 0, 59), 0.0, 8.0,
+                         s.longitude, s.latitude, std::max(100.0f, m.pressure), m.airTemperature);
+}
+
+std::string utcTimestamp() {
+    const std::time_t now = std::time(nullptr);
+    std::tm utc{};
+    gmtime_r(&now, &utc);
+    std::ostringstream t;
+    t << std::put_time(&utc, "%Y-%m-%dT%H:%M:%SZ");
+    return t.str();
+}
+
+std::string jsonString(const std::string& v) {
+    std::string o = "\"";
+    for (char ch : v) {
+        if (ch == '"' || ch == '\\') o += '\\';
+        o += ch;
+    }
+    return o + "\"";
+}
+
+// faceteb_run.json (core:3213-3222, 4030-4042, 4143-4146): the index of this
+// invocation's products and completed nodes, rewritten after every node
+struct RunIndex {
+    std::filesystem::path path;
+    std::string startedAt, input;
+    int startNode = 0, endNode = 1;
+    float tolerance = 0.05f;
+    uint32_t budget = 20;
+    std::string status = "running", finishedAt;
+    std::vector<std::string> products, completed;
+    RunIndex() = default;
+    RunIndex(const RunIndex&) = delete;
+    RunIndex& operator=(const RunIndex&) = delete;
+    ~RunIndex() {  // a run that ends by an exception is marked failed (core:66-72)
+        if (status != "running" || path.empty()) return;
+        status = "failed";
+        finishedAt = utcTimestamp();
+        try {
+            write();
+        } catch (...) {
+        }
+    }
+    void write() const {
+        std::ofstream o(path, std::ios::trunc);
+        o << std::setprecision(9) << "{\n  \"kind\": \"faceteb-run\",\n  \"schemaVersion\": 1,\n  \"status\": "
+          << jsonString(status) << ",\n  \"startedAt\": " << jsonString(startedAt) << ",\n";
+        if (!finishedAt.empty()) o << "  \"finishedAt\": " << jsonString(finishedAt) << ",\n";
+        o << "  \"input\": " << jsonString(input) << ",\n  \"requested\": {\"startNode\": " << startNode
+          << ", \"endNodeExclusive\": " << endNode << "},\n  \"convergence\": {\"temperatureTolerance_K\": "
+          << tolerance << ", \"iterationBudget\": " << budget
+          << ", \"criterion\": \"temperatureDelta and residual/thermal-response <= temperatureTolerance\"},\n"
+          << "  \"completed\": [";
+        for (size_t i = 0; i < completed.size(); ++i) o << (i ? ",\n    " : "\n    ") << completed[i];
+        o << (completed.empty() ? "" : "\n  ") << "],\n  \"products\": [";
+        for (size_t i = 0; i < products.size(); ++i) o << (i ? ", " : "") << jsonString(products[i]);
+        o << "]\n}\n";
+        if (!o) throw std::runtime_error("FacetEB cannot write run index");
+    }
+};
+
+// process/<model>_T=<token>.{bin,json} (core:3937-4019): 5 floats per side for
+// energy [LE, H, G, T, Rn], 3 for radiation [B_SW, B_LW, Rn]
+void writeProcess(const std::filesystem::path& dir, const std::string& type, int node, float julianTime,
+                  const std::string& token, const SunAngles& sun, const Scene& scene, uint32_t iterations,
+                  float dT, bool converged, float residual, float residualEq, const EbSettings& set,
+                  const std::vector<float>* cols[5], RunIndex& index) {
+    const std::string model = type == "radiation" ? "facetrt" : "faceteb";
+    const std::string stem = model + "_T=" + token;
+    const std::filesystem::path pdir = dir / "process";
+    std::filesystem::create_directories(pdir);
+    const int nf = type == "energy" ? 5 : 3;
+    std::ofstream bin(pdir / (stem + ".bin"), std::ios::binary | std::ios::trunc);
+    const size_t S = cols[0]->size();
+    std::vector<float> rec(size_t(nf) * S);
+    for (size_t s = 0; s < S; ++s)
+        for (int k = 0; k < nf; ++k) rec[s * nf + k] = (*cols[k])[s];
+    bin.write(reinterpret_cast<const char*>(rec.data()), std::streamsize(rec.size() * sizeof(float)));
+    bin.close();
+    if (!bin) throw std::runtime_error("FacetEB cannot finish process binary");
+    std::ofstream js(pdir / (stem + ".json"), std::ios::trunc);
+    js << std::setprecision(9) << "{\n  \"kind\": \"facet-" << type << "-process\",\n  \"model\": \"" << model
+       << "\",\n  \"processType\": \"" << type << "\",\n  \"geometry\": \"facet\",\n  \"node\": " << node
+       << ",\n  \"julianTime\": " << julianTime << ",\n  \"time\": \"" << token << "\",\n  \"solarZenith\": "
+       << float(sun.zenith) << ",\n  \"solarAzimuth\": " << float(sun.azimuth) << ",\n  \"facetCount\": "
+       << scene.facetCount() << ",\n  \"surfaceCount\": " << S
+       << ",\n  \"geometryFile\": \"../faceteb.json\",\n  \"componentFile\": \"../.facet_steps/components.bin\",\n"
+       << "  \"dataFile\": \"" << stem << ".bin\",\n  \"dataType\": \"float32-little-endian\",\n"
+       << "  \"layout\": \"surface-interleaved\",\n  \"recordFloats\": " << nf << ",\n  \"couplingIterations\": "
+       << iterations << ",\n  \"temperatureDelta\": " << dT << ",\n  \"converged\": "
+       << (converged ? "true" : "false") << ",\n  \"energyResidual\": " << residual
+       << ",\n  \"energyResidualUnits\": \"W m-2\",\n  \"energyResidualTemperatureEquivalent\": " << residualEq
+       << ",\n  \"temperatureTolerance\": " << set.tolerance << ",\n  \"iterationBudget\": "
+       << set.couplingIterations << ",\n  \"fields\": ";
+    if (type == "radiation")
+        js << "[{\"id\":\"shortwaveRadiation\",\"label\":\"shortwave radiosity [W m-2]\",\"offset\":0},"
+              "{\"id\":\"longwaveRadiation\",\"label\":\"longwave radiosity [W m-2]\",\"offset\":1},"
+              "{\"id\":\"netRadiation\",\"label\":\"net radiation [W m-2]\",\"offset\":2}]\n}\n";
+    else
+        js << "[{\"id\":\"latentHeat\",\"label\":\"latent heat [W m-2]\",\"offset\":0},"
+              "{\"id\":\"sensibleHeat\",\"label\":\"sensible heat [W m-2]\",\"offset\":1},"
+              "{\"id\":\"surfaceHeatFlux\",\"label\":\"surface heat flux [W m-2]\",\"offset\":2},"
+              "{\"id\":\"temperature\",\"label\":\"temperature [K]\",\"offset\":3},"
+              "{\"id\":\"netRadiation\",\"label\":\"net radiation [W m-2]\",\"offset\":4}]\n}\n";
+    index.products.push_back("process/" + stem + ".bin");
+    index.products.push_back("process/" + stem + ".json");
+}
+
 int runFacetEB(const std::string& input, const std::string& outFile) try {
-    const std::string result = outputPath(input, outFile, "faceteb.json");
+    const EbSettings set = ebSettings();
     const Scene scene = loadObj(input);
+    const std::vector<MeteoStep> meteo = loadMeteorology(set);
+    const std::string result = outputPath(input, outFile, "faceteb.json");
+    const std::filesystem::path outDir = std::filesystem::path(result).parent_path().empty()
+                                             ? std::filesystem::path(".")
+                                             : std::filesystem::path(result).parent_path();
+    std::filesystem::create_directories(outDir);
+    RunIndex index;
+    index.path = outDir / "faceteb_run.json";
+    index.startedAt = utcTimestamp();
+    index.input = input;
+    index.startNode = set.startNode;
+    index.endNode = set.endNode;
+    index.tolerance = set.tolerance;
+    index.budget = set.couplingIterations;
+    index.write();
+    std::cout << "PROGRESS\t3\tLoading the facet scene and meteorology" << std::endl;
+
     facethip::FacetrtHip model;
     model.initialize(engineConfig());
     model.setGeometry(scene.vertices);
@@ -263,52 +582,189 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         mats[i].soilTemperatureMethod = 1;
         mats[i].vegetation = 1;
     }
-    const float swR[2] = {0.10f, 0.20f}, swT[2] = {0.05f, 0.0f};
-    facethip::check(fh_eb_set_scene(ctx, scene.leafFacetCount, scene.material.data(), 2, mats, 1, swR, swT), ctx);
-    fh_eb_node_params p{};
-    p.sun = {0.0f, -1.0f, 0.0f};  // DOY 1 00:00: the sun is below the horizon
-    p.airTemperatureK = std::clamp(25.0f + 273.15f, 220.0f, 340.0f);
-    p.vaporPressure = 15.0f;
-    p.pressure = 839.0f;
-    p.windSpeed = 1.0f;
-    p.skyLongwave = 320.0f;
-    p.dTime = 1800.0f;
-    p.temperatureTolerance = 0.05f;
-    p.temperatureRelaxation = 0.5f;
-    p.maxCoupling = 20;
-    p.radiosityTolerance = 1.0e-5f;
-    p.maxSweeps = 2000;
-    p.precision = FH_PRECISION_FP32;
-    p.warmStart = FH_WARM_PREVIOUS;
-    p.firstNode = 1;
-    const float zero = 0.0f;
-    fh_eb_node_stats st{};
-    std::cout << "PROGRESS\t10\tCoupled facet radiation and energy balance" << std::endl;
-    facethip::check(fh_eb_run_node(ctx, &p, &zero, &zero, &st), ctx);
+    // shortwave windows of the flat per-micrometre spectrum the reference
+    // falls back to without Esun/Esky files (core:2522-2529, 3495-3565):
+    // N1 = 2001 samples of 1 nm, windows of spectralAccelerationWidth samples
+    constexpr uint32_t kSamples = 2001;
+    std::vector<std::pair<uint32_t, uint32_t>> windows;
+    for (uint32_t f = 0; f + 1 < kSamples; f += set.spectralWidth)
+        windows.push_back({f, std::min(f + set.spectralWidth, kSamples - 1)});
+    const uint32_t nw = uint32_t(windows.size());
+    const float dLevel = 0.5f * set.directRatio, qLevel = 0.5f * (1.0f - set.directRatio);
+    const auto integral = [](float level, uint32_t a, uint32_t b) { return float(double(level) * (b - a) * 0.001); };
+    const float directIntegral = integral(dLevel, 0, kSamples - 1);
+    std::vector<float> swR(2 * size_t(nw)), swT(2 * size_t(nw));
+    for (uint32_t w = 0; w < nw; ++w) {
+        swR[w] = 0.10f;
+        swT[w] = 0.05f;
+        swR[nw + w] = 0.20f;
+        swT[nw + w] = 0.0f;
+    }
+    facethip::check(
+        fh_eb_set_scene(ctx, scene.leafFacetCount, scene.material.data(), 2, mats, nw, swR.data(), swT.data()), ctx);
+
+    const std::filesystem::path stepDir = outDir / ".facet_steps";
+    std::filesystem::create_directories(stepDir);
+    {  // components.bin (core:3418-3429): 1 for ground, the object's output component (0) otherwise
+        std::ofstream comp(stepDir / "components.bin", std::ios::binary | std::ios::trunc);
+        for (uint32_t f = 0; f < scene.facetCount(); ++f) {
+            const uint8_t c = f >= scene.leafFacetCount ? 1 : 0;
+            comp.write(reinterpret_cast<const char*>(&c), 1);
+        }
+        if (!comp) throw std::runtime_error("FacetEB cannot write component identities");
+    }
+    {  // no photovoltaic materials in an OBJ scene: header only (core:3433-3436)
+        std::ofstream pv(outDir / "photovoltaic_summary.csv", std::ios::trunc);
+        pv << "node,time,area_m2,temperature_C,incident_W,absorbed_W,electric_W,net_longwave_W,sensible_W,"
+              "storage_W,max_residual_W_m2,energy_kWh\n";
+    }
+    index.products.push_back("photovoltaic_summary.csv");
+    index.products.push_back(".facet_steps/components.bin");
+    if (set.stepSync) {  // shared geometry before the first node (core:3442-3462)
+        std::ofstream g(result, std::ios::binary | std::ios::trunc);
+        writeHeader(g, "HIP GPU Facet RT-EB coupled", scene);
+        g << "  \"componentFile\": \".facet_steps/components.bin\",\n  \"facetCount\": " << scene.facetCount()
+          << ",\n  \"leafFacetCount\": " << scene.leafFacetCount << ",\n  \"vertexPositions\": [";
+        for (size_t i = 0; i < scene.vertices.size(); ++i) {
+            if (i) g << ',';
+            const auto& v = scene.vertices[i];
+            g << v.position[0] << ',' << v.position[1] << ',' << v.position[2];
+        }
+        g << "]\n}\n";
+        if (!g) throw std::runtime_error("FacetEB cannot write shared geometry");
+    }
+
     const size_t S = scene.facetCount() * size_t(2);
-    std::vector<float> T(S), Blw(S), sunlit(S), Rn(S), H(S), LE(S), G(S);
-    facethip::check(fh_eb_get_state(ctx, T.data(), nullptr, Blw.data(), sunlit.data(), Rn.data(), H.data(),
-                                    LE.data(), G.data(), nullptr),
-                    ctx);
-    std::ofstream out(result, std::ios::binary);
+    std::vector<float> T(S), Blw(S), sunlit(S), Rn(S), H(S), LE(S), G(S), sw(S * nw), Bsw(S), Btot(S);
+    bool allConverged = true;
+    fh_eb_node_stats last{};
+    std::string lastToken;
+    for (int node = set.startNode; node < set.endNode; ++node) {
+        const MeteoStep& m = meteo[size_t(node)];
+        const SunAngles sun = nodeSun(m, set);
+        const std::string token = timeToken(m.julianTime);
+        const facethip::Direction sd = sunDirection(float(sun.zenith), float(sun.azimuth));
+        const float cz = std::cos(float(sun.zenith) * kPi / 180.0f);
+        const float total = std::max(0.0f, m.shortwave);
+        const float scale = (cz > 1.0e-4f && directIntegral > 1.0e-8f)
+                                ? std::min(1400.0f, total * directIntegral / cz) / directIntegral
+                                : 0.0f;
+        std::vector<float> dni(nw, 0.0f), dif(nw, 0.0f);
+        for (uint32_t w = 0; w < nw && total > 0.0f; ++w) {
+            dni[w] = scale * integral(dLevel, windows[w].first, windows[w].second);
+            dif[w] = total * integral(qLevel, windows[w].first, windows[w].second);
+        }
+        fh_eb_node_params p{};
+        p.sun = {sd.x, sd.y, sd.z};
+        p.airTemperatureK = std::clamp(m.airTemperature + 273.15f, 220.0f, 340.0f);
+        p.vaporPressure = m.vaporPressure;
+        p.pressure = m.pressure;
+        p.windSpeed = m.windSpeed;
+        p.skyLongwave = m.longwave > 0.0f ? m.longwave
+                                           : float(5.670374419e-8 * std::pow(double(set.skyTemperature), 4.0));
+        p.dTime = set.dTime;
+        p.temperatureTolerance = set.tolerance;
+        p.temperatureRelaxation = set.relaxation;
+        p.maxCoupling = set.couplingIterations;
+        p.radiosityTolerance = 1.0e-5f;
+        p.maxSweeps = 2000;
+        p.precision = FH_PRECISION_FP32;
+        p.warmStart = FH_WARM_PREVIOUS;
+        p.firstNode = node == set.startNode ? 1u : 0u;
+        fh_eb_node_stats st{};
+        facethip::check(fh_eb_run_node(ctx, &p, dni.data(), dif.data(), &st), ctx);
+        facethip::check(fh_eb_get_state(ctx, T.data(), nullptr, Blw.data(), sunlit.data(), Rn.data(), H.data(),
+                                        LE.data(), G.data(), sw.data()),
+                        ctx);
+        for (size_t s = 0; s < S; ++s) {
+            float b = 0.0f;
+            for (uint32_t w = 0; w < nw; ++w) b += sw[s * nw + w];
+            Bsw[s] = b;
+            Btot[s] = b + Blw[s];
+        }
+        const bool converged = st.converged != 0;
+        allConverged = allConverged && converged;
+        // .facet_steps/energy_T=<token>.bin (core:3910-3935): per side
+        // [T, B_total, B_SW, B_LW, Rn, H, LE, G, sunlit], no sensor bands
+        {
+            std::vector<float> rec(S * 9);
+            for (size_t s = 0; s < S; ++s) {
+                const float v[9] = {T[s], Btot[s], Bsw[s], Blw[s], Rn[s], H[s], LE[s], G[s], sunlit[s]};
+                std::copy(v, v + 9, rec.begin() + std::ptrdiff_t(s * 9));
+            }
+            std::ofstream bin(stepDir / ("energy_T=" + token + ".bin"), std::ios::binary | std::ios::trunc);
+            bin.write(reinterpret_cast<const char*>(rec.data()), std::streamsize(rec.size() * sizeof(float)));
+            if (!bin) throw std::runtime_error("FacetEB cannot write time-step energy file");
+        }
+        if (set.saveRadiation) {
+            const std::vector<float>* cols[5] = {&Bsw, &Blw, &Rn, nullptr, nullptr};
+            writeProcess(outDir, "radiation", node, m.julianTime, token, sun, scene, st.couplingIterations,
+                         st.maxTemperatureDelta, converged, st.maxEnergyResidual, st.maxResidualEquivalent, set,
+                         cols, index);
+        }
+        if (set.saveEnergy) {
+            const std::vector<float>* cols[5] = {&LE, &H, &G, &T, &Rn};
+            writeProcess(outDir, "energy", node, m.julianTime, token, sun, scene, st.couplingIterations,
+                         st.maxTemperatureDelta, converged, st.maxEnergyResidual, st.maxResidualEquivalent, set,
+                         cols, index);
+        }
+        {
+            std::ofstream pv(outDir / ("photovoltaic_node_" + std::to_string(node) + ".csv"), std::ios::trunc);
+            pv << "area_m2,temperature_C,incident_W_m2,absorbed_W_m2,electric_W_m2,residual_W_m2,latent_W_m2,"
+                  "effective_W_m2,net_longwave_W_m2,sensible_W_m2,storage_W_m2,facet_index\n";
+        }
+        index.products.push_back(".facet_steps/energy_T=" + token + ".bin");
+        index.products.push_back("photovoltaic_node_" + std::to_string(node) + ".csv");
+        {
+            std::ostringstream c;
+            c << std::setprecision(9) << "{\"node\": " << node << ", \"token\": " << jsonString(token)
+              << ", \"julianTime\": " << m.julianTime << ", \"converged\": " << (converged ? "true" : "false")
+              << ", \"iterations\": " << st.couplingIterations << ", \"iterationBudget\": " << set.couplingIterations
+              << ", \"temperatureDelta\": " << st.maxTemperatureDelta
+              << ", \"temperatureTolerance\": " << set.tolerance << ", \"energyResidual\": " << st.maxEnergyResidual
+              << ", \"energyResidualUnits\": \"W m-2\", \"energyResidualTemperatureEquivalent\": "
+              << st.maxResidualEquivalent << "}";
+            index.completed.push_back(c.str());
+        }
+        index.write();
+        const int progress = 10 + int(86.0 * (node - set.startNode + 1) / (set.endNode - set.startNode));
+        std::cout << "PROGRESS\t" << progress << "\tFacet RT-EB node " << (node - set.startNode + 1) << "/"
+                  << (set.endNode - set.startNode) << " " << token << ", iterations=" << st.couplingIterations
+                  << ", max dT=" << st.maxTemperatureDelta << " K" << std::endl;
+        std::cout << "FACET_CONVERGENCE\t" << node << '\t' << (converged ? 1 : 0) << '\t' << st.couplingIterations
+                  << '\t' << st.maxTemperatureDelta << '\t' << st.maxEnergyResidual << '\t'
+                  << st.maxResidualEquivalent << std::endl;
+        if (set.stepSync) {  // back-pressure (core:4072-4082)
+            std::cout << "FACET_STEP\t" << node << '\t' << token << '\t' << std::setprecision(9) << m.julianTime
+                      << std::endl;
+            std::string ack;
+            if (!std::getline(std::cin, ack) || ack != "FACET_ACK\t" + std::to_string(node))
+                throw std::runtime_error("FacetEB time-step output was interrupted or failed");
+        }
+        last = st;
+        lastToken = token;
+    }
+
+    std::ofstream out(result, std::ios::binary | std::ios::trunc);
     if (!out) throw std::runtime_error("FacetEB cannot write result: " + result);
     writeHeader(out, "HIP GPU Facet RT-EB coupled", scene);
     out << "  \"componentFile\": \".facet_steps/components.bin\",\n"
         << "  \"facetCount\": " << scene.facetCount() << ",\n  \"leafFacetCount\": " << scene.leafFacetCount
-        << ",\n  \"time\": \"DOY1_00-00\",\n";
+        << ",\n  \"time\": \"" << lastToken << "\",\n";
     writeVertices(out, scene);
     out << "  \"graph\": {\"directedEdges\":" << graph.directedEdgeCount << ",\"closureError\":"
         << graph.maxClosureError << ",\"fragmentOverflow\":0,\"hashOverflow\":0},\n"
-        << "  \"iterations\": " << st.lwSweeps << ",\n  \"maxDelta\": 0,\n"
-        << "  \"couplingIterations\": " << st.couplingIterations << ",\n"
-        << "  \"temperatureDelta\": " << st.maxTemperatureDelta << ",\n"
-        << "  \"converged\": " << (st.converged ? "true" : "false") << ",\n"
+        << "  \"iterations\": " << last.lwSweeps << ",\n  \"maxDelta\": 0,\n"
+        << "  \"couplingIterations\": " << last.couplingIterations << ",\n"
+        << "  \"temperatureDelta\": " << last.maxTemperatureDelta << ",\n"
+        << "  \"converged\": " << (allConverged ? "true" : "false") << ",\n"
+        << "  \"energyResidual\": " << last.maxEnergyResidual << ",\n"
         << "  \"energyResidualUnits\": \"W m-2\",\n"
-        << "  \"energyResidualTemperatureEquivalent\": " << st.maxResidualEquivalent << ",\n"
-        << "  \"temperatureTolerance\": " << p.temperatureTolerance << ",\n"
-        << "  \"iterationBudget\": " << p.maxCoupling << ",\n";
+        << "  \"energyResidualTemperatureEquivalent\": " << last.maxResidualEquivalent << ",\n"
+        << "  \"temperatureTolerance\": " << set.tolerance << ",\n"
+        << "  \"iterationBudget\": " << set.couplingIterations << ",\n";
     writeArray(out, "sunlit", sunlit, true);
-    writeArray(out, "radiosity", Blw, true);
+    writeArray(out, "radiosity", Btot, true);
     writeArray(out, "lightEnhancement", std::vector<float>(S, 0.0f), true);
     writeArray(out, "temperature", T, true);
     writeArray(out, "netRadiation", Rn, true);
@@ -318,19 +774,41 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
     writeArray(out, "photovoltaicPower", std::vector<float>(S, 0.0f), false);
     out << "}\n";
     out.close();
-    std::cout << "FACET_CONVERGENCE\t0\t" << (st.converged ? 1 : 0) << '\t' << st.couplingIterations << '\t'
-              << st.maxTemperatureDelta << '\t' << st.maxResidualEquivalent << std::endl;
+    if (!out) throw std::runtime_error("FacetEB cannot finish result output");
+    index.products.push_back(std::filesystem::path(result).filename().generic_string());
+    index.status = allConverged ? "completed" : "non_converged";
+    index.finishedAt = utcTimestamp();
+    index.write();
+    std::cout << "PROGRESS\t100\t"
+              << (allConverged ? "Facet radiation and energy balance converged"
+                               : "Facet run finished with non-converged nodes; results kept")
+              << std::endl;
     std::cout << "RESULT\t" << result << std::endl;
-    return st.converged ? 0 : 2;
+    return allConverged ? 0 : 2;
 } catch (const std::exception& e) {
     std::cerr << "faceteb: " << e.what() << '\n';
     return 1;
+}
+
+// --solar-position Y M D h m s timezone longitude latitude pressure temperature
+int runSolarPosition(int argc, char** argv) {
+    if (argc != 13) {
+        std::cerr << "Usage: facet_engine --solar-position Y M D h m s tz lon lat pressure temperature\n";
+        return 2;
+    }
+    double v[11];
+    for (int i = 0; i < 11; ++i) v[i] = std::atof(argv[2 + i]);
+    const SunAngles a = solarPosition(int(v[0]), int(v[1]), int(v[2]), int(v[3]), int(v[4]), v[5], v[6], v[7], v[8],
+                                      v[9], v[10]);
+    std::cout << std::setprecision(10) << a.zenith << ' ' << a.azimuth << std::endl;
+    return 0;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "";
+    if (mode == "--solar-position") return runSolarPosition(argc, argv);
     const std::string input = argc > 2 ? argv[2] : "";
     const std::string output = argc > 3 ? argv[3] : "";
     if (mode != "eFacetRT" && mode != "eFacetEB") {

@@ -60,3 +60,25 @@ def test_cli_usage_and_input_errors():
     assert r.returncode == 1 and "facetrt:" in r.stderr
     r = subprocess.run([str(exe), "eFacetEB", "/nonexistent/scene.obj"], capture_output=True, text=True)
     assert r.returncode == 1 and "faceteb:" in r.stderr
+
+
+def test_cli_solar_position_matches_spa_example():
+    # the published example of NREL's Solar Position Algorithm (Reda &
+    # Andreas, NREL/TP-560-34302): 2003-10-17 12:30:30 local, timezone -7,
+    # 105.1786 W, 39.742476 N, 820 hPa, 11 C -> zenith 50.111622 deg,
+    # azimuth 194.340241 deg.  The low-precision form is within 0.001 deg.
+    import subprocess
+    exe = ROOT / "streamsim_amd" / "facet_engine"
+    r = subprocess.run([str(exe), "--solar-position", "2003", "10", "17", "12", "30", "30", "-7", "-105.1786",
+                        "39.742476", "820", "11"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    zen, az = map(float, r.stdout.split())
+    assert abs(zen - 50.111622) < 1e-3 and abs(az - 194.340241) < 1e-3
+    # sun below the horizon at local midnight, near the zenith at a tropical noon
+    r = subprocess.run([str(exe), "--solar-position", "2019", "1", "1", "0", "0", "0", "8", "116", "40", "839", "25"],
+                       capture_output=True, text=True)
+    assert float(r.stdout.split()[0]) > 90.0
+    r = subprocess.run([str(exe), "--solar-position", "2019", "6", "21", "12", "0", "0", "0", "0", "23.44", "1013",
+                        "15"], capture_output=True, text=True)
+    assert float(r.stdout.split()[0]) < 1.0
+

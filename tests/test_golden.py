@@ -107,3 +107,78 @@ def test_cli_facetrt_on_validation_obj(tmp_path):
     e = json.loads((tmp_path / "eb.json").read_text())
     T = np.asarray(e["temperature"])
     assert T.size == 2 * 4746 and np.all((T > 220) & (T < 360))
+
+
+@pytest.mark.gpu
+def test_cli_faceteb_nodes_step_outputs_and_backpressure(tmp_path):
+    # eFacetEB over three meteorological nodes (morning, noon, night) with the
+    # per-node outputs of the reference (core:3410-4082): components.bin,
+    # .facet_steps/energy_T=<token>.bin [T, B, B_SW, B_LW, Rn, H, LE, G, sunlit]
+    # per side, process files, faceteb_run.json, and the FACET_STEP /
+    # FACET_ACK back-pressure on stdin / stdout
+    print("golden cli eb nodes", flush=True)
+    import os
+    exe = Path(__file__).resolve().parents[1] / "streamsim_amd" / "facet_engine"
+    meteo = tmp_path / "meteo.txt"
+    meteo.write_text("julian Tair e P u SW LW\n172.25 18 14 1000 2 250 330\n172.5 27 16 1000 2 850 360\n"
+                     "172.875 21 15 1000 1 0 340\n")
+    env = dict(os.environ, FACET_RAYS_PER_SIDE="64", FACET_SUN_RAYS_PER_SIDE="16", FACET_EB_METEO=str(meteo),
+               FACET_EB_START="0", FACET_EB_END="3", FACET_EB_PROCESS="both", STREAMSIM_FACET_STEP_SYNC="1",
+               FACET_EB_TOLERANCE="0.01", FACET_EB_COUPLING="60")
+    out = tmp_path / "eb" / "faceteb.json"
+    p = subprocess.Popen([str(exe), "eFacetEB", str(HERE / "urban_tree_building.obj"), str(out)], env=env,
+                         stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    steps, conv = [], []
+    for line in p.stdout:
+        f = line.rstrip("\n").split("\t")
+        if f[0] == "FACET_STEP":
+            steps.append((int(f[1]), f[2]))
+            # the shared geometry is on disk before the first node's record
+            assert json.loads(out.read_text())["facetCount"] == 4746
+            p.stdin.write(f"FACET_ACK\t{f[1]}\n")
+            p.stdin.flush()
+        elif f[0] == "FACET_CONVERGENCE":
+            conv.append(int(f[2]))
+    rc = p.wait(timeout=300)
+    assert rc == 0, p.stderr.read()
+    assert [s[1] for s in steps] == ["DOY172_06-00", "DOY172_12-00", "DOY172_21-00"]
+    assert conv == [1, 1, 1]
+    d = tmp_path / "eb"
+    F, S = 4746, 2 * 4746
+    assert (d / ".facet_steps" / "components.bin").stat().st_size == F
+    comp = np.fromfile(d / ".facet_steps" / "components.bin", np.uint8)
+    assert comp[:138].max() == 0 and comp[138:].min() == 1
+    recs = []
+    for _, tok in steps:
+        r = np.fromfile(d / ".facet_steps" / f"energy_T={tok}.bin", np.float32).reshape(S, 9)
+        assert np.all((r[:, 0] > 220) & (r[:, 0] < 360))
+        np.testing.assert_allclose(r[:, 1], r[:, 2] + r[:, 3], rtol=1e-6, atol=1e-3)
+        assert np.all((r[:, 8] >= 0) & (r[:, 8] <= 1))
+        recs.append(r)
+        e = np.fromfile(d / "process" / f"faceteb_T={tok}.bin", np.float32).reshape(S, 5)
+        np.testing.assert_array_equal(e[:, 3], r[:, 0])
+        q = np.fromfile(d / "process" / f"facetrt_T={tok}.bin", np.float32).reshape(S, 3)
+        np.testing.assert_array_equal(q[:, 0], r[:, 2])
+        meta = json.loads((d / "process" / f"faceteb_T={tok}.json").read_text())
+        assert meta["recordFloats"] == 5 and meta["surfaceCount"] == S and meta["converged"] is True
+    # noon is sunlit and brighter in the shortwave than the morning; the night node has none
+    assert recs[1][:, 8].mean() > 0.2 and recs[1][:, 2].mean() > recs[0][:, 2].mean() > 0
+    assert recs[2][:, 8].max() == 0 and recs[2][:, 2].max() == 0
+    idx = json.loads((d / "faceteb_run.json").read_text())
+    assert idx["kind"] == "faceteb-run" and idx["status"] == "completed"
+    assert [c["node"] for c in idx["completed"]] == [0, 1, 2] and all(c["converged"] for c in idx["completed"])
+    assert ".facet_steps/energy_T=DOY172_12-00.bin" in idx["products"] and "faceteb.json" in idx["products"]
+    res = json.loads(out.read_text())
+    assert res["time"] == "DOY172_21-00" and res["converged"] is True
+    np.testing.assert_array_equal(np.asarray(res["temperature"], np.float32), recs[2][:, 0])
+    # an acknowledgement for the wrong node ends the run with status 1
+    p = subprocess.Popen([str(exe), "eFacetEB", str(HERE / "urban_tree_building.obj"), str(tmp_path / "x.json")],
+                         env=env, stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
+    for line in p.stdout:
+        if line.startswith("FACET_STEP"):
+            p.stdin.write("FACET_ACK\t7\n")
+            p.stdin.flush()
+    assert p.wait(timeout=300) == 1
+    idx = json.loads((tmp_path / "faceteb_run.json").read_text())
+    assert idx["status"] == "failed"
+
