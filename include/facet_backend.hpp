@@ -40,12 +40,15 @@ struct SurfaceOptics {  // facetvk::SurfaceOptics (facetrt_vulkan.h:30-35)
     float directIrradiance{};
 };
 
-struct Config {  // facetvk::Config (facetrt_vulkan.h:37-47), ray fields
+struct Config {  // facetvk::Config (facetrt_vulkan.h:37-47), ray fields + periodic fields
     uint32_t raysPerSide{256};
     uint32_t sunRaysPerSide{64};
     uint32_t seed{1};
     uint32_t gpuIndex{0};
     float rayEpsilon{1.0e-5f};
+    uint32_t periodicNeighborCount{0};
+    float periodicSizeX{1.0f};
+    float periodicSizeZ{1.0f};
 };
 
 struct GraphDiagnostics {  // facetvk::GraphDiagnostics (facetrt_vulkan.h:49-56)
@@ -94,6 +97,9 @@ public:
         c.seed = config.seed;
         c.gpuIndex = config.gpuIndex;
         c.rayEpsilon = config.rayEpsilon;
+        c.periodicNeighborCount = config.periodicNeighborCount;
+        c.periodicSizeX = config.periodicSizeX;
+        c.periodicSizeZ = config.periodicSizeZ;
         check(fh_create(&c, &m_ctx), nullptr);
     }
 

@@ -57,15 +57,20 @@ typedef struct {
 } fh_surface_optics;
 
 /* facetvk::Config (vkh:37-47).  The raster fields are replaced by the
- * ray-cast form-factor fields. */
+ * ray-cast form-factor fields; the periodic fields keep their meaning
+ * (sh/visibility.vert:15-35): the scene is repeated at the first
+ * periodicNeighborCount tiles of the neighbour list (+-X, +-Z, corners,
+ * then the ring at distance 2), offsets periodicSizeX / periodicSizeZ, and
+ * rays see every copy (a hit on a copy counts for the original facet). */
 typedef struct {
     uint32_t raysPerSide;    /* cosine-weighted form-factor rays per side  */
     uint32_t sunRaysPerSide; /* sun visibility rays per side               */
     uint32_t seed;           /* counter-RNG seed                           */
     uint32_t gpuIndex;       /* HIP device ordinal                         */
     float rayEpsilon;        /* minimum hit distance (m)                   */
-    uint32_t reorder;        /* 1: Morton-order facets inside the engine   */
-    uint32_t reserved[2];
+    uint32_t periodicNeighborCount; /* 0 .. 20                             */
+    float periodicSizeX;     /* tile period along X (m), > 0 when periodic */
+    float periodicSizeZ;     /* tile period along Z (m)                    */
 } fh_config;
 
 /* facetvk::GraphDiagnostics (vkh:49-56) */

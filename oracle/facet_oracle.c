@@ -789,12 +789,13 @@ static int box_hit(const wray_t *r, const obvh_node *n, float tlimit)
     return tn <= tf;
 }
 
-/* closest: mode 0; any hit: mode 1 */
-static uint32_t trace(const or_vertex *v, uint32_t F, const obvh_t *bvh, const wray_t *r,
-                      uint32_t skip, float tmin, int anyhit, float *tHit, uint32_t *facetOut)
+/* closest: mode 0; any hit: mode 1.  The search starts from the best hit
+   (*tHit, *facetOut) found so far when `resume` is set (periodic tiles). */
+static uint32_t trace_from(const or_vertex *v, uint32_t F, const obvh_t *bvh, const wray_t *r,
+                           uint32_t skip, float tmin, int anyhit, int resume, float *tHit, uint32_t *facetOut)
 {
-    float best = FLT_MAX;
-    uint32_t bestF = OR_SKY;
+    float best = resume ? *tHit : FLT_MAX;
+    uint32_t bestF = resume ? *facetOut : OR_SKY;
     if (!bvh) {
         for (uint32_t f = 0; f < F; ++f) {
             if (f == skip) continue;
@@ -842,6 +843,41 @@ static uint32_t trace(const or_vertex *v, uint32_t F, const obvh_t *bvh, const w
     return bestF * 2u + (dd > 0.0f ? 1u : 0u);
 }
 
+static uint32_t trace(const or_vertex *v, uint32_t F, const obvh_t *bvh, const wray_t *r,
+                      uint32_t skip, float tmin, int anyhit, float *tHit, uint32_t *facetOut)
+{
+    float t = FLT_MAX;
+    uint32_t f = OR_SKY;
+    const uint32_t side = trace_from(v, F, bvh, r, skip, tmin, anyhit, 0, &t, &f);
+    if (tHit) *tHit = t;
+    if (facetOut) *facetOut = f;
+    return side;
+}
+
+/* Periodic copies (sh/visibility.vert:15-35): tile 0 is the scene, tiles
+   1..count the neighbour list of tile_offset(); the ray is traced once per
+   tile with its origin moved by minus the tile offset (float ops as
+   written), the copies of the origin facet are ordinary surfaces, and the
+   best (t, facet) carries over the tiles in order. */
+static uint32_t trace_periodic(const or_vertex *v, uint32_t F, const obvh_t *bvh, const float o[3],
+                               const float d[3], uint32_t skip, float tmin, int anyhit, uint32_t count, float px,
+                               float pz, uint32_t *facetOut)
+{
+    float t = FLT_MAX;
+    uint32_t f = OR_SKY, side = OR_SKY;
+    for (uint32_t k = 0; k <= count; ++k) {
+        if (anyhit && f != OR_SKY) break;
+        int off[2];
+        tile_offset(k, off);
+        const float ok[3] = {o[0] - (float)off[0] * px, o[1], o[2] - (float)off[1] * pz};
+        wray_t r;
+        wray_init(&r, k ? ok : o, d);
+        side = trace_from(v, F, bvh, &r, k ? OR_SKY : skip, tmin, anyhit, 1, &t, &f);
+    }
+    if (facetOut) *facetOut = f;
+    return side;
+}
+
 uint32_t or_trace_one(const or_vertex *v, uint32_t F, const float o[3],
                       const float d[3], uint32_t skipFacet, float tmin, float *tHit)
 {
@@ -859,6 +895,12 @@ static int u32_cmp(const void *a, const void *b)
 int or_ray_graph(const or_vertex *v, uint32_t F, uint32_t seed, uint32_t R,
                  float tmin, int brute, or_graph *out, uint32_t *hits)
 {
+    return or_ray_graph_periodic(v, F, seed, R, tmin, brute, 0u, 1.0f, 1.0f, out, hits);
+}
+
+int or_ray_graph_periodic(const or_vertex *v, uint32_t F, uint32_t seed, uint32_t R, float tmin, int brute,
+                          uint32_t periodic, float px, float pz, or_graph *out, uint32_t *hits)
+{
     const uint32_t S = 2u * F;
     obvh_t bvh;
     memset(&bvh, 0, sizeof(bvh));
@@ -872,9 +914,8 @@ int or_ray_graph(const or_vertex *v, uint32_t F, uint32_t seed, uint32_t R,
         for (uint32_t j = 0; j < R; ++j) {
             float o[3], d[3];
             or_ray_generate(v, s, j, seed, o, d);
-            wray_t r;
-            wray_init(&r, o, d);
-            all[(size_t)s * R + j] = trace(v, F, brute ? NULL : &bvh, &r, s >> 1, tmin, 0, NULL, NULL);
+            all[(size_t)s * R + j] =
+                trace_periodic(v, F, brute ? NULL : &bvh, o, d, s >> 1, tmin, 0, periodic, px, pz, NULL);
         }
     }
     if (hits) memcpy(hits, all, sizeof(uint32_t) * (size_t)S * R);
@@ -922,6 +963,12 @@ int or_ray_graph(const or_vertex *v, uint32_t F, uint32_t seed, uint32_t R,
 int or_ray_sunlit(const or_vertex *v, uint32_t F, const float sun[3],
                   uint32_t seed, uint32_t K, float tmin, int brute, float *fraction)
 {
+    return or_ray_sunlit_periodic(v, F, sun, seed, K, tmin, brute, 0u, 1.0f, 1.0f, fraction);
+}
+
+int or_ray_sunlit_periodic(const or_vertex *v, uint32_t F, const float sun[3], uint32_t seed, uint32_t K,
+                           float tmin, int brute, uint32_t periodic, float px, float pz, float *fraction)
+{
     obvh_t bvh;
     memset(&bvh, 0, sizeof(bvh));
     if (!brute && obvh_build(&bvh, v, F)) return -2;
@@ -939,10 +986,8 @@ int or_ray_sunlit(const or_vertex *v, uint32_t F, const float sun[3],
         for (uint32_t j = 0; j < K; ++j) {
             float o[3];
             triangle_point(v, f, u01(ray_rand(key, j, 0u)), u01(ray_rand(key, j, 1u)), o);
-            wray_t r;
-            wray_init(&r, o, sun);
             uint32_t hitF;
-            trace(v, F, brute ? NULL : &bvh, &r, f, tmin, 1, NULL, &hitF);
+            trace_periodic(v, F, brute ? NULL : &bvh, o, sun, f, tmin, 1, periodic, px, pz, &hitF);
             if (hitF == OR_SKY) ++lit;
         }
         fraction[s] = (float)lit / (float)K;

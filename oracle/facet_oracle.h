@@ -98,6 +98,16 @@ int or_ray_sunlit(const or_vertex *v, uint32_t F, const float sun[3],
                   uint32_t seed, uint32_t K, float tmin, int brute,
                   float *fraction);
 
+/* The same with periodic copies of the scene (sh/visibility.vert:15-35):
+ * tiles 1..periodic of the neighbour list, offsets (dx px, 0, dz pz); every
+ * ray is traced against each tile in order with its origin moved by minus
+ * the offset, keeping the best (t, facet); copies of the origin facet are
+ * ordinary surfaces. */
+int or_ray_graph_periodic(const or_vertex *v, uint32_t F, uint32_t seed, uint32_t R, float tmin, int brute,
+                          uint32_t periodic, float px, float pz, or_graph *out, uint32_t *hits);
+int or_ray_sunlit_periodic(const or_vertex *v, uint32_t F, const float sun[3], uint32_t seed, uint32_t K,
+                           float tmin, int brute, uint32_t periodic, float px, float pz, float *fraction);
+
 /* Single ray query (debug / tests): closest hit side or 0xffffffff. */
 uint32_t or_trace_one(const or_vertex *v, uint32_t F, const float o[3],
                       const float d[3], uint32_t skipFacet, float tmin,

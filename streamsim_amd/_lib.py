@@ -30,8 +30,9 @@ class FhConfig(C.Structure):
         ("seed", C.c_uint32),
         ("gpuIndex", C.c_uint32),
         ("rayEpsilon", C.c_float),
-        ("reorder", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("periodicNeighborCount", C.c_uint32),
+        ("periodicSizeX", C.c_float),
+        ("periodicSizeZ", C.c_float),
     ]
 
 

@@ -191,6 +191,12 @@ int fh_create(const fh_config* config, fh_context** out) {
             fail(FH_INVALID_ARGUMENT, "sunRaysPerSide must be in [1, 65536]");
         if (!std::isfinite(config->rayEpsilon) || config->rayEpsilon < 0.0f)
             fail(FH_INVALID_ARGUMENT, "rayEpsilon must be finite and non-negative");
+        if (config->periodicNeighborCount > 20u)
+            fail(FH_INVALID_ARGUMENT, "periodicNeighborCount must be in [0, 20]");
+        if (config->periodicNeighborCount &&
+            !(std::isfinite(config->periodicSizeX) && std::isfinite(config->periodicSizeZ) &&
+              config->periodicSizeX > 0.0f && config->periodicSizeZ > 0.0f))
+            fail(FH_INVALID_ARGUMENT, "periodic tile sizes must be finite and positive");
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count == 0) fail(FH_RUNTIME_ERROR, "no HIP device available");
         if (int(config->gpuIndex) >= count) fail(FH_RUNTIME_ERROR, "gpuIndex out of range");

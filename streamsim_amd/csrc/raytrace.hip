@@ -138,9 +138,7 @@ __device__ __forceinline__ bool slab(const Ray& r, float lx, float hx, float ly,
 // kSky.  stk points at this lane's column of the LDS stack (stride kTraceBlock).
 template <bool ANY>
 __device__ uint32_t traverse(const BvhNode* __restrict__ nodes, const float4* __restrict__ tris, const Ray& r,
-                             uint32_t skip, float tmin, int* stk, float& tbest) {
-    uint32_t bestF = kSky;
-    tbest = FLT_MAX;
+                             uint32_t skip, float tmin, int* stk, float& tbest, uint32_t bestF = kSky) {
     int sp = 0;
     int cur = 0;
     while (true) {
@@ -190,12 +188,43 @@ __device__ uint32_t traverse(const BvhNode* __restrict__ nodes, const float4* __
     return bestF;
 }
 
+// Periodic tiles (sh/visibility.vert:15-26): tile 0 is the scene, then the
+// neighbour list; tile t holds a copy shifted by (dx sizeX, 0, dz sizeZ).
+__constant__ int kTileOffset[21][2] = {{0, 0},   {-1, 0},  {1, 0},   {0, -1}, {0, 1},  {-1, -1}, {-1, 1},
+                                       {1, -1},  {1, 1},   {-2, 0},  {2, 0},  {0, -2}, {0, 2},   {-2, -1},
+                                       {-2, 1},  {2, -1},  {2, 1},   {-1, -2}, {1, -2}, {-1, 2},  {1, 2}};
+
+struct Periodic {
+    uint32_t tiles;  // 1 + neighbour count
+    float sx, sz;
+};
+
+// Closest (or any) hit over the periodic copies: the ray is traced against
+// the scene once per tile with its origin moved by minus the tile offset
+// (the copies of the origin facet are ordinary surfaces), keeping the
+// running best (t, facet) across tiles; the root boxes cull tiles the ray
+// does not reach before the best hit.
+template <bool ANY>
+__device__ uint32_t trace_tiles(const BvhNode* __restrict__ nodes, const float4* __restrict__ tris, const Ray& r,
+                                uint32_t facet, float tmin, int* stk, const Periodic& p, float& tbest) {
+    tbest = FLT_MAX;
+    uint32_t best = traverse<ANY>(nodes, tris, r, facet, tmin, stk, tbest);
+    for (uint32_t t = 1; t < p.tiles; ++t) {
+        if (ANY && best != kSky) break;
+        Ray rt = r;
+        rt.o[0] = r.o[0] - float(kTileOffset[t][0]) * p.sx;
+        rt.o[2] = r.o[2] - float(kTileOffset[t][1]) * p.sz;
+        best = traverse<ANY>(nodes, tris, rt, kSky, tmin, stk, tbest, best);
+    }
+    return best;
+}
+
 // ---- form-factor rays: one lane per (side, ray) ---------------------------
 __global__ __launch_bounds__(kTraceBlock) void k_trace_graph(const BvhNode* __restrict__ nodes,
                                                              const float4* __restrict__ tris,
                                                              const float* __restrict__ fverts,
                                                              const float4* __restrict__ fnorm, uint32_t S,
-                                                             uint32_t R, uint32_t seed, float tmin,
+                                                             uint32_t R, uint32_t seed, float tmin, Periodic per,
                                                              uint32_t* __restrict__ hits) {
     __shared__ int stack[kBvhMaxDepth * kTraceBlock];
     const uint64_t gid = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
@@ -230,7 +259,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_graph(const BvhNode* __re
     for (int i = 0; i < 3; ++i) r.d[i] = (x * t1[i] + y * t2[i]) + z * n[i];
     ray_setup(r);
     float tbest;
-    const uint32_t hf = traverse<false>(nodes, tris, r, f, tmin, stack + threadIdx.x, tbest);
+    const uint32_t hf = trace_tiles<false>(nodes, tris, r, f, tmin, stack + threadIdx.x, per, tbest);
     uint32_t out = kSky;
     if (hf != kSky) {
         const float4 hn = fnorm[hf];
@@ -246,7 +275,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_sun(const BvhNode* __rest
                                                            const float* __restrict__ fverts,
                                                            const float4* __restrict__ fnorm, uint32_t S,
                                                            uint32_t K, uint32_t seed, float sx, float sy, float sz,
-                                                           float tmin, uint32_t* __restrict__ lit) {
+                                                           float tmin, Periodic per, uint32_t* __restrict__ lit) {
     __shared__ int stack[kBvhMaxDepth * kTraceBlock];
     const uint64_t gid = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
     if (gid >= uint64_t(S) * K) return;
@@ -265,7 +294,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_sun(const BvhNode* __rest
     r.d[2] = sz;
     ray_setup(r);
     float tbest;
-    const uint32_t hf = traverse<true>(nodes, tris, r, f, tmin, stack + threadIdx.x, tbest);
+    const uint32_t hf = trace_tiles<true>(nodes, tris, r, f, tmin, stack + threadIdx.x, per, tbest);
     const bool sky = hf == kSky;
     if (64u % K == 0u) {
         // the K rays of a side are K consecutive lanes of one wave (all
@@ -475,11 +504,16 @@ __global__ __launch_bounds__(256) void k_weights(const uint32_t* __restrict__ ro
 }
 
 // ---- host launchers ---------------------------------------------------------
+static Periodic periodic(const fh_context* c) {
+    Periodic p{1u + c->cfg.periodicNeighborCount, c->cfg.periodicSizeX, c->cfg.periodicSizeZ};
+    return p;
+}
+
 void launch_trace_graph(fh_context* c, uint32_t R, uint32_t seed, float tmin) {
     const uint64_t n = uint64_t(c->S) * R;
     const uint64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
     hipLaunchKernelGGL(k_trace_graph, dim3(uint32_t(blocks)), dim3(kTraceBlock), 0, c->stream, c->nodes.p, c->tris.p,
-                       c->facetVerts.p, c->facetNormal.p, c->S, R, seed, tmin, c->hits.p);
+                       c->facetVerts.p, c->facetNormal.p, c->S, R, seed, tmin, periodic(c), c->hits.p);
     FH_HIP(hipGetLastError());
 }
 
@@ -523,7 +557,8 @@ void launch_sun(fh_context* c, float sx, float sy, float sz, uint32_t* lit, floa
     const uint64_t n = uint64_t(c->S) * K;
     const uint64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
     hipLaunchKernelGGL(k_trace_sun, dim3(uint32_t(blocks)), dim3(kTraceBlock), 0, c->stream, c->nodes.p, c->tris.p,
-                       c->facetVerts.p, c->facetNormal.p, c->S, K, c->cfg.seed, sx, sy, sz, c->cfg.rayEpsilon, lit);
+                       c->facetVerts.p, c->facetNormal.p, c->S, K, c->cfg.seed, sx, sy, sz, c->cfg.rayEpsilon,
+                       periodic(c), lit);
     hipLaunchKernelGGL(k_sun_fraction, dim3((c->S + 255u) / 256u), dim3(256), 0, c->stream, lit, c->S, K, frac);
     FH_HIP(hipGetLastError());
 }

@@ -68,7 +68,8 @@ class FacetEngine:
     """One GPU context: geometry, form factors, radiosity solves, sunlit fraction."""
 
     def __init__(self, rays_per_side: int = 256, sun_rays_per_side: int = 64, seed: int = 1,
-                 gpu_index: int = 0, ray_epsilon: float = 1.0e-5):
+                 gpu_index: int = 0, ray_epsilon: float = 1.0e-5, periodic_neighbors: int = 0,
+                 periodic_size: tuple[float, float] = (1.0, 1.0)):
         self._lib = _lib.load()
         cfg = FhConfig()
         cfg.raysPerSide = rays_per_side
@@ -76,7 +77,8 @@ class FacetEngine:
         cfg.seed = seed
         cfg.gpuIndex = gpu_index
         cfg.rayEpsilon = ray_epsilon
-        cfg.reorder = 0
+        cfg.periodicNeighborCount = periodic_neighbors
+        cfg.periodicSizeX, cfg.periodicSizeZ = periodic_size
         self.config = cfg
         h = C.c_void_p()
         st = self._lib.fh_create(C.byref(cfg), C.byref(h))

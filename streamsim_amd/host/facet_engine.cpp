@@ -49,6 +49,7 @@ struct Scene {
     std::vector<facethip::Vertex> vertices;
     std::vector<uint32_t> material;
     uint32_t leafFacetCount = 0;
+    float extentX = 1.0f, extentZ = 1.0f;  // soil grid extent (the periodic tile)
     uint32_t facetCount() const { return uint32_t(vertices.size() / 3); }
 };
 
@@ -114,6 +115,8 @@ Scene loadObj(const std::string& path) {
         if (appendTriangle(s, pos[t[0]], pos[t[1]], pos[t[2]], 0u)) ++s.leafFacetCount;
     const float margin = 0.5f * std::max(hi[0] - lo[0], hi[2] - lo[2]);
     const float x0 = lo[0] - margin, x1 = hi[0] + margin, z0 = lo[2] - margin, z1 = hi[2] + margin;
+    s.extentX = x1 - x0;
+    s.extentZ = z1 - z0;
     for (uint32_t row = 0; row < kSoilGrid; ++row) {
         const float za = z0 + (z1 - z0) * float(row) / float(kSoilGrid);
         const float zb = z0 + (z1 - z0) * float(row + 1) / float(kSoilGrid);
@@ -182,10 +185,19 @@ std::string outputPath(const std::string& input, const std::string& requested, c
     return (std::filesystem::path(input).parent_path() / name).lexically_normal().string();
 }
 
-facethip::Config engineConfig() {
+// FACET_RAYS_PER_SIDE, FACET_SUN_RAYS_PER_SIDE: ray counts; FACET_PERIODIC_NEIGHBORS:
+// periodic copies of the scene (0..20, the reference's periodicNeighborCount)
+// with the soil grid's extent as the period (periodicSizeX/Z = scene width /
+// depth, core:2890-2894)
+facethip::Config engineConfig(const Scene& s) {
     facethip::Config c;
     if (const char* e = std::getenv("FACET_RAYS_PER_SIDE")) c.raysPerSide = uint32_t(std::atoi(e));
     if (const char* e = std::getenv("FACET_SUN_RAYS_PER_SIDE")) c.sunRaysPerSide = uint32_t(std::atoi(e));
+    if (const char* e = std::getenv("FACET_PERIODIC_NEIGHBORS")) {
+        c.periodicNeighborCount = uint32_t(std::clamp(std::atoi(e), 0, 20));
+        c.periodicSizeX = s.extentX;
+        c.periodicSizeZ = s.extentZ;
+    }
     return c;
 }
 
@@ -195,7 +207,7 @@ int runFacetRT(const std::string& input, const std::string& outFile) try {
     const auto t0 = Clock::now();
     std::cout << "PROGRESS\t3\tCreating HIP device context" << std::endl;
     facethip::FacetrtHip model;
-    model.initialize(engineConfig());
+    model.initialize(engineConfig(scene));
     model.setGeometry(scene.vertices);
     std::cout << "PROGRESS\t10\tRay-cast form factors" << std::endl;
     const auto tv = Clock::now();
@@ -563,7 +575,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
     std::cout << "PROGRESS\t3\tLoading the facet scene and meteorology" << std::endl;
 
     facethip::FacetrtHip model;
-    model.initialize(engineConfig());
+    model.initialize(engineConfig(scene));
     model.setGeometry(scene.vertices);
     const facethip::GraphDiagnostics graph = model.buildVisibilityGraph({facethip::Direction{}});
     fh_context* ctx = model.handle();

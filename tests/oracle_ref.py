@@ -60,6 +60,8 @@ def load():
         "or_raster_sunlit": (_I, [_P, _U, _P, _P, _U, _U, _U, _F, _F, _P, _P]),
         "or_ray_graph": (_I, [_P, _U, _U, _U, _F, _I, C.POINTER(OrGraph), _P]),
         "or_ray_sunlit": (_I, [_P, _U, _P, _U, _U, _F, _I, _P]),
+        "or_ray_graph_periodic": (_I, [_P, _U, _U, _U, _F, _I, _U, _F, _F, C.POINTER(OrGraph), _P]),
+        "or_ray_sunlit_periodic": (_I, [_P, _U, _P, _U, _U, _F, _I, _U, _F, _F, _P]),
         "or_trace_one": (_U, [_P, _U, _P, _P, _U, _F, _P]),
         "or_ray_generate": (None, [_P, _U, _U, _U, _P, _P]),
         "or_solve": (None, [C.POINTER(OrGraph), _P, _F, _U, _F, _P, _P]),
@@ -168,23 +170,27 @@ def raster_sunlit(vertices, bmin, bmax, sun, raster=1024, layers=64) -> np.ndarr
     return out
 
 
-def ray_graph(vertices, seed: int, rays: int, tmin: float = 1.0e-5, brute: bool = False, hits: bool = False):
+def ray_graph(vertices, seed: int, rays: int, tmin: float = 1.0e-5, brute: bool = False, hits: bool = False,
+              periodic: int = 0, size=(1.0, 1.0)):
     v = np.ascontiguousarray(vertices, np.float32)
     F = v.shape[0] // 3
     h = np.zeros(2 * F * rays, np.uint32) if hits else None
     g = OrGraph()
-    rc = load().or_ray_graph(_p(v), F, seed, rays, tmin, int(brute), C.byref(g), _p(h))
+    rc = load().or_ray_graph_periodic(_p(v), F, seed, rays, tmin, int(brute), periodic, size[0], size[1], C.byref(g),
+                                      _p(h))
     if rc:
         raise MemoryError("oracle allocation failed")
     out = _take(g)
     return (out, h) if hits else out
 
 
-def ray_sunlit(vertices, sun, seed: int, K: int, tmin: float = 1.0e-5, brute: bool = False) -> np.ndarray:
+def ray_sunlit(vertices, sun, seed: int, K: int, tmin: float = 1.0e-5, brute: bool = False, periodic: int = 0,
+               size=(1.0, 1.0)) -> np.ndarray:
     v = np.ascontiguousarray(vertices, np.float32)
     F = v.shape[0] // 3
     out = np.zeros(2 * F, np.float32)
-    load().or_ray_sunlit(_p(v), F, _p(np.asarray(sun, np.float32)), seed, K, tmin, int(brute), _p(out))
+    load().or_ray_sunlit_periodic(_p(v), F, _p(np.asarray(sun, np.float32)), seed, K, tmin, int(brute), periodic,
+                                  size[0], size[1], _p(out))
     return out
 
 

@@ -82,3 +82,12 @@ def test_cli_solar_position_matches_spa_example():
                         "15"], capture_output=True, text=True)
     assert float(r.stdout.split()[0]) < 1.0
 
+
+def test_periodic_config_validation():
+    # fh_create rejects the periodic fields before it looks for a device
+    from streamsim_amd import FacetEngine
+    with pytest.raises(ValueError, match="periodicNeighborCount"):
+        FacetEngine(periodic_neighbors=21, periodic_size=(1.0, 1.0))
+    with pytest.raises(ValueError, match="periodic tile sizes"):
+        FacetEngine(periodic_neighbors=4, periodic_size=(0.0, 1.0))
+

@@ -76,6 +76,32 @@ def test_sunlit_bitexact(oracle):
         assert np.array_equal(f, of), (zen, az)
 
 
+@pytest.mark.parametrize("neighbors", [4, 20])
+def test_periodic_graph_and_sunlit_bitexact(oracle, neighbors):
+    # periodic copies of a row crop (10 x 10 m tile): form-factor CSR and
+    # sunlit fractions equal the oracle's replay of the same tiles
+    print(f"periodic case neighbors={neighbors}", flush=True)
+    from streamsim_amd import FacetEngine
+    sc = scenes.row_crop(leaves_per_row=60, cells=8)
+    eng = FacetEngine(rays_per_side=64, sun_rays_per_side=32, seed=4, periodic_neighbors=neighbors,
+                      periodic_size=(10.0, 10.0))
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    g = eng.getGraph()
+    og = oracle.ray_graph(sc.vertices, 4, 64, periodic=neighbors, size=(10.0, 10.0))
+    for k in ("rowOffsets", "neighbors", "counts", "denominators", "skyCounts"):
+        assert np.array_equal(getattr(g, k), getattr(og, k)), k
+    # the neighbours close part of the sky seen by the ground
+    g0 = oracle.ray_graph(sc.vertices, 4, 64)
+    ground = np.arange(2 * sc.leaf_count, sc.surface_count, 2)
+    assert og.skyCounts[ground].sum() < g0.skyCounts[ground].sum()
+    for zen, az in ((60.0, 100.0), (75.0, 270.0)):
+        sun = oracle.sun_direction(zen, az)
+        f = eng.computeSunlitFraction(sun)
+        of = oracle.ray_sunlit(sc.vertices, sun, 4, 32, periodic=neighbors, size=(10.0, 10.0))
+        assert np.array_equal(f, of), (zen, az)
+
+
 def test_reference_solve_on_raster_graph(oracle):
     # the reference's own graph (A-buffer restatement) uploaded through
     # fh_set_graph; fixed 64 sweeps from zero vs solveCpu (FP64 gather)

@@ -116,6 +116,41 @@ def test_ray_graph_brute_equals_bvh(oracle):
     assert np.all(g1.denominators == 48)
 
 
+def test_ray_graph_periodic_brute_equals_bvh_and_closes_the_gap(oracle):
+    # periodic copies (sh/visibility.vert:15-35): a 1 x 1 ground tile under
+    # a 1 x 1 roof 0.1 above it.  Alone, the ground sees the roof with the
+    # parallel-squares view factor; tiled with the 20 neighbours of the
+    # reference's list, rays leaving the tile hit the copies and the view
+    # factor goes to 1 (only rays flatter than ~87 deg get past 2 tiles)
+    ground = _square_grid(4)
+    roof = _square_grid(4, y=0.1, down=True)
+    v = scenes.triangles_to_vertices(np.concatenate([roof, ground]))
+    nr, R = roof.shape[0], 256
+    kw = dict(periodic=20, size=(1.0, 1.0))
+    gb, hb = oracle.ray_graph(v, 5, R, brute=True, hits=True, **kw)
+    gv, hv = oracle.ray_graph(v, 5, R, hits=True, **kw)
+    assert np.array_equal(hb, hv)
+    g0 = oracle.ray_graph(v, 5, R)
+
+    def roof_fraction(g):
+        rows = np.repeat(np.arange(g.S), np.diff(g.rowOffsets))
+        src = (rows % 2 == 0) & (rows // 2 >= nr)  # upward sides of the ground
+        return g.counts[src & (g.neighbors // 2 < nr)].sum() / (ground.shape[0] * R)
+
+    f0, fp = roof_fraction(g0), roof_fraction(gv)
+    sigma = math.sqrt(0.25 / (ground.shape[0] * R))
+    assert f0 == pytest.approx(_vf_parallel_squares(1.0, 0.1), abs=5 * sigma)
+    assert fp > 0.99
+    # a low sun from +X shades the whole ground only with the periodic roof
+    sun = np.array([0.5, 0.1, 0.0], np.float32)
+    sun /= np.linalg.norm(sun)
+    s0 = oracle.ray_sunlit(v, sun, 3, 32)
+    sp = oracle.ray_sunlit(v, sun, 3, 32, **kw)
+    assert np.array_equal(sp, oracle.ray_sunlit(v, sun, 3, 32, brute=True, **kw))
+    ground_up = np.arange(2 * nr, 2 * (nr + ground.shape[0]), 2)
+    assert s0[ground_up].mean() > 0.3 and sp[ground_up].max() == 0.0
+
+
 def test_ray_trace_one_hits_plane(oracle):
     v = scenes.triangles_to_vertices(_square_grid(1, y=1.0, down=True))
     side, t = oracle.trace_one(v, [0.3, 0.0, 0.4], [0.0, 2.0, 0.0], 0xFFFFFFFF)
