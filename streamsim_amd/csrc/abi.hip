@@ -6,6 +6,8 @@
 // fh_status codes instead of exceptions.
 #include <hip/hip_runtime.h>
 
+#include <cfloat>
+
 #include <cmath>
 #include <cstdio>
 #include <memory>
@@ -259,6 +261,38 @@ int fh_set_geometry(fh_context* c, const fh_vertex* v, uint32_t vertexCount) {
         }
         fh::BvhHost bvh;
         fh::build_bvh(tri9.data(), F, bvh);
+        // sweep schedule: facets in Morton order of their centroids, so the
+        // rows a sweep processes together see overlapping columns
+        {
+            float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+            std::vector<float> cen(size_t(F) * 3u);
+            for (uint32_t f = 0; f < F; ++f)
+                for (int q = 0; q < 3; ++q) {
+                    const float x = (tri9[size_t(f) * 9u + q] + tri9[size_t(f) * 9u + 3u + q] +
+                                     tri9[size_t(f) * 9u + 6u + q]) / 3.0f;
+                    cen[size_t(f) * 3u + q] = x;
+                    lo[q] = std::min(lo[q], x);
+                    hi[q] = std::max(hi[q], x);
+                }
+            const float ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1.0e-20f});
+            std::vector<std::pair<uint64_t, uint32_t>> key(F);
+            for (uint32_t f = 0; f < F; ++f) {
+                uint64_t code = 0;
+                uint32_t cell[3];
+                for (int q = 0; q < 3; ++q)
+                    cell[q] = uint32_t(std::min(1023.0f, std::max(0.0f, (cen[size_t(f) * 3u + q] - lo[q]) / ext *
+                                                                             1023.0f)));
+                for (int b = 0; b < 10; ++b)
+                    for (int q = 0; q < 3; ++q) code |= uint64_t((cell[q] >> b) & 1u) << (3 * b + q);
+                key[f] = {code, f};
+            }
+            std::stable_sort(key.begin(), key.end(),
+                             [](const auto& a, const auto& b) { return a.first < b.first; });
+            std::vector<uint32_t> order(F);
+            for (uint32_t i = 0; i < F; ++i) order[i] = key[i].second;
+            c->sched.ensure(F);
+            upload(c, c->sched.p, order.data(), order.size());
+        }
         c->F = F;
         c->S = 2u * F;
         c->facetVerts.ensure(tri9.size());

@@ -364,6 +364,7 @@ struct CoupledArgs {
     SolveCtrl* ctrl;
     float4* partial;
     float* Iin;  // per side: unclamped incident fsky L + G of the gather half
+    const uint32_t* sched;  // facet gathered by logical team i (null: facet i)
 };
 
 // Gather half of a coupled sweep: a team of TEAM lanes per facet sums
@@ -373,8 +374,9 @@ template <int TEAM, bool W16>
 __global__ __launch_bounds__(256) void k_coupled_gather(CoupledArgs a) {
     if (*reinterpret_cast<volatile uint32_t*>(&a.ctrl->done)) return;
     const int lane = threadIdx.x % TEAM;
-    const uint32_t f = (blockIdx.x * 256u + threadIdx.x) / TEAM;
-    if (f >= a.F) return;
+    const uint32_t fl = (blockIdx.x * 256u + threadIdx.x) / TEAM;
+    if (fl >= a.F) return;
+    const uint32_t f = a.sched ? a.sched[fl] : fl;
     const uint32_t s0 = 2u * f;
     const uint32_t e0 = a.rowptr[s0], e1 = a.rowptr[s0 + 1], e2 = a.rowptr[s0 + 2];
     float g[2][1] = {{0.0f}, {0.0f}};
@@ -675,6 +677,7 @@ static uint32_t coupled_loop(fh_context* c, const fh_eb_node_params& p, const Eb
     a.ctrl = c->ctrl.p;
     a.partial = e.part4.p;
     a.Iin = e.Iin.p;
+    a.sched = sweep_schedule(c);
     uint64_t launched = 0;
     uint32_t batch = 4;
     while (true) {

@@ -191,6 +191,7 @@ struct fh_context {
     uint32_t nnz = 0;
     float closure = 0.0f;
     fh::DBuf<uint32_t> rowptr, col, cnt, den, sky;
+    fh::DBuf<uint32_t> sched;        // processing order of facets in the sweeps (Morton order)
     fh::DBuf<float> val32, fsky;
     fh::DBuf<uint16_t> val16;
 

@@ -72,6 +72,9 @@ void set_sky(fh_context* c, const float* sky, uint32_t nb, uint32_t ldb);
 bool uses_bf16_gather(uint32_t ldb);
 uint32_t bf16_stride(uint32_t ldb);
 
+// facet processing order of the sweeps (null: natural order)
+const uint32_t* sweep_schedule(const fh_context* c);
+
 // lanes per facet team of the SpMM kernels for the current graph
 int sweep_team(const fh_context* c);
 

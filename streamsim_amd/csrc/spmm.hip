@@ -83,6 +83,7 @@ struct SweepArgs {
     uint32_t qRound;  // REFRESH: the BF16 sum uses round_bf16(B) (matches Bin16)
     uint32_t bt;      // band-parallel kernels: lanes per facet team
     uint32_t fuseH;   // judge sweeps also write H = max(0, fsky S + G + D) (band-parallel)
+    const uint32_t* sched;  // facet processed by logical team i (null: facet i)
     const float* omega;  // Chebyshev weights per band for this sweep (band-parallel FP32), or null
     const float* rho2;   // squared spectral radius estimate per band (Chebyshev)
 };
@@ -102,9 +103,10 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
         Bout = (it & 1u) ? a.buf0 : a.buf1;
     }
     const int lane = threadIdx.x % TEAM;
-    const uint32_t f = (xcd_block(a.xcd) * 256u + threadIdx.x) / TEAM;
+    const uint32_t fl = (xcd_block(a.xcd) * 256u + threadIdx.x) / TEAM;
     float myAbs = 0.0f, myRel = 0.0f;
-    if (f < a.F) {
+    if (fl < a.F) {
+        const uint32_t f = a.sched ? a.sched[fl] : fl;
         const uint32_t s0 = 2u * f;
         const uint32_t e0 = a.rowptr[s0], e1 = a.rowptr[s0 + 1], e2 = a.rowptr[s0 + 2];
         for (uint32_t ch = 0; ch < a.nchunks; ++ch) {
@@ -465,9 +467,10 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
     if (!band_ctx<MODE>(a, MODE == MODE_SWEEP && W16, x)) return;
     const uint32_t bt = a.bt, tpb = 256u / bt, ldb = a.ldb;
     const uint32_t team = threadIdx.x / bt, lane = threadIdx.x - team * bt;
-    const uint32_t f = xcd_block(a.xcd) * tpb + team;
+    const uint32_t fl = xcd_block(a.xcd) * tpb + team;
     float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
-    if (team < tpb && f < a.F) {
+    if (team < tpb && fl < a.F) {
+        const uint32_t f = a.sched ? a.sched[fl] : fl;
         const uint32_t s0 = 2u * f;
         const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
         for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
@@ -516,9 +519,10 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
     if (!band_ctx<MODE_SWEEP>(a, true, x)) return;
     const uint32_t bt = a.bt, tpb = 256u / bt, ldh = a.ldh;
     const uint32_t team = threadIdx.x / bt, lane = threadIdx.x - team * bt;
-    const uint32_t f = xcd_block(a.xcd) * tpb + team;
+    const uint32_t fl = xcd_block(a.xcd) * tpb + team;
     float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
-    if (team < tpb && f < a.F) {
+    if (team < tpb && fl < a.F) {
+        const uint32_t f = a.sched ? a.sched[fl] : fl;
         const uint32_t s0 = 2u * f;
         const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
         for (uint32_t cb = 0; cb < x.nPrev; cb += 8u * bt) {
@@ -763,8 +767,23 @@ bool uses_bf16_gather(uint32_t ldb) { return band_parallel(ldb); }
 
 uint32_t bf16_stride(uint32_t ldb) { return (ldb + 63u) & ~63u; }
 
+// Facets are processed in Morton order of their centroids (rows processed
+// together see overlapping columns: -7% FP32 shortwave sweep, -3% LW sweep on
+// the terrain forest; the order does not change any computed value).
+// FH_SCHED=0 processes them in index order (tuning knob).
+static bool use_sched() {
+    static const bool v = [] {
+        const char* e = std::getenv("FH_SCHED");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return v;
+}
+
+const uint32_t* sweep_schedule(const fh_context* c) { return use_sched() && c->sched.p ? c->sched.p : nullptr; }
+
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.xcd = xcd_remap();
+    a.sched = sweep_schedule(c);
     if (band_parallel(a.ldb)) {
         // lanes per team: one 16-byte vector of the row each, at most 64
         // bands per team pass
