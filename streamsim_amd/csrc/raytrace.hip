@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "engine.hpp"
@@ -226,7 +227,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_graph(const BvhNode* __re
                                                              const float4* __restrict__ fnorm, uint32_t S,
                                                              uint32_t R, uint32_t seed, float tmin, Periodic per,
                                                              uint32_t* __restrict__ hits) {
-    __shared__ int stack[kBvhMaxDepth * kTraceBlock];
+    // traversal stack in LDS, (BVH depth + 1) x kTraceBlock ints sized at
+    // launch: a shallower tree leaves room for more resident waves
+    extern __shared__ int stack[];
     const uint64_t gid = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
     if (gid >= uint64_t(S) * R) return;
     const uint32_t side = uint32_t(gid / R);
@@ -276,7 +279,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_sun(const BvhNode* __rest
                                                            const float4* __restrict__ fnorm, uint32_t S,
                                                            uint32_t K, uint32_t seed, float sx, float sy, float sz,
                                                            float tmin, Periodic per, uint32_t* __restrict__ lit) {
-    __shared__ int stack[kBvhMaxDepth * kTraceBlock];
+    // traversal stack in LDS, (BVH depth + 1) x kTraceBlock ints sized at
+    // launch: a shallower tree leaves room for more resident waves
+    extern __shared__ int stack[];
     const uint64_t gid = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
     if (gid >= uint64_t(S) * K) return;
     const uint32_t side = uint32_t(gid / K);
@@ -509,10 +514,17 @@ static Periodic periodic(const fh_context* c) {
     return p;
 }
 
+// LDS bytes of the traversal stack: one entry per internal level of the tree
+static size_t stack_bytes(const fh_context* c) {
+    const int levels = std::min(std::max(c->bvhDepth + 1, 1), kBvhMaxDepth);
+    return size_t(levels) * kTraceBlock * sizeof(int);
+}
+
 void launch_trace_graph(fh_context* c, uint32_t R, uint32_t seed, float tmin) {
     const uint64_t n = uint64_t(c->S) * R;
     const uint64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL(k_trace_graph, dim3(uint32_t(blocks)), dim3(kTraceBlock), 0, c->stream, c->nodes.p, c->tris.p,
+    hipLaunchKernelGGL(k_trace_graph, dim3(uint32_t(blocks)), dim3(kTraceBlock), stack_bytes(c), c->stream, c->nodes.p,
+                       c->tris.p,
                        c->facetVerts.p, c->facetNormal.p, c->S, R, seed, tmin, periodic(c), c->hits.p);
     FH_HIP(hipGetLastError());
 }
@@ -556,7 +568,8 @@ void launch_sun(fh_context* c, float sx, float sy, float sz, uint32_t* lit, floa
     const uint32_t K = c->cfg.sunRaysPerSide;
     const uint64_t n = uint64_t(c->S) * K;
     const uint64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL(k_trace_sun, dim3(uint32_t(blocks)), dim3(kTraceBlock), 0, c->stream, c->nodes.p, c->tris.p,
+    hipLaunchKernelGGL(k_trace_sun, dim3(uint32_t(blocks)), dim3(kTraceBlock), stack_bytes(c), c->stream, c->nodes.p,
+                       c->tris.p,
                        c->facetVerts.p, c->facetNormal.p, c->S, K, c->cfg.seed, sx, sy, sz, c->cfg.rayEpsilon,
                        periodic(c), lit);
     hipLaunchKernelGGL(k_sun_fraction, dim3((c->S + 255u) / 256u), dim3(256), 0, c->stream, lit, c->S, K, frac);
