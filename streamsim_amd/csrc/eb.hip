@@ -591,6 +591,20 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     FH_HIP(hipStreamSynchronize(c->stream));
     e.swValid = e.lwValid = e.tValid = false;
     e.swRhoValid = false;
+    if (swNb && c->hasGraph) {
+        // spectral radius per internal band of this scene's SW operator (the
+        // Chebyshev parameter): r and t do not depend on the sun, so the
+        // estimate runs here, once per scene, with the direct term zero
+        const uint32_t ldb = e.swLdb;
+        launch_fill_f32(c, e.dni.p, ldb, 0.0f);
+        hipLaunchKernelGGL(k_sw_optics, blocks_for(nsw), dim3(256), 0, c->stream, S, swNb, ldb, e.material.p,
+                           e.swRefl.p, e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, 0.0f, 1.0f, 0.0f,
+                           e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p);
+        FH_HIP(hipGetLastError());
+        e.swRho2.ensure(ldb);
+        estimate_rho2(c, ldb, swNb, e.rsw.p, e.tsw.p, e.Bsw1.p, e.Hsw.p, e.csw.p, e.swRho2.p);
+        e.swRhoValid = true;
+    }
     e.ready = true;
 }
 

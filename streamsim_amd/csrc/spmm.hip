@@ -80,7 +80,6 @@ struct SweepArgs {
     uint16_t* h0;
     uint16_t* h1;
     uint32_t ldh;
-    uint32_t qRound;  // REFRESH: the BF16 sum uses round_bf16(B) (matches Bin16)
     uint32_t bt;      // band-parallel kernels: lanes per facet team
     uint32_t fuseH;   // judge sweeps also write H = max(0, fsky S + G + D) (band-parallel)
     const uint32_t* sched;  // facet processed by logical team i (null: facet i)
@@ -226,16 +225,16 @@ constexpr int kU = 8;  // CSR entries in flight per lane (one aligned group)
 // The 8-aligned group of CSR entries [base, base + 8): columns and weights
 // with 16-byte vector loads (one request per team and vector; the CSR arrays
 // are padded by kCsrPad zero entries), entries outside [start, end) get
-// weight 0 and ok[u] = false.  WQ: also the BF16 weights (REFRESH).
-template <bool W16, bool WQ>
+// weight 0 and ok[u] = false.
+template <bool W16>
 __device__ __forceinline__ void load_group(const SweepArgs& a, uint32_t base, uint32_t start, uint32_t end,
-                                           uint32_t (&n)[kU], float (&w)[kU], float (&wq)[kU], bool (&ok)[kU]) {
+                                           uint32_t (&n)[kU], float (&w)[kU], bool (&ok)[kU]) {
     const uint4 cA = *reinterpret_cast<const uint4*>(a.col + base);
     const uint4 cB = *reinterpret_cast<const uint4*>(a.col + base + 4);
     n[0] = cA.x; n[1] = cA.y; n[2] = cA.z; n[3] = cA.w;
     n[4] = cB.x; n[5] = cB.y; n[6] = cB.z; n[7] = cB.w;
     float h[kU];
-    if (W16 || WQ) {
+    if (W16) {
         const uint4 q = *reinterpret_cast<const uint4*>(a.w16 + base);
         const uint32_t qq[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
@@ -256,7 +255,6 @@ __device__ __forceinline__ void load_group(const SweepArgs& a, uint32_t base, ui
         const uint32_t e = base + uint32_t(u);
         ok[u] = e >= start && e < end;
         w[u] = ok[u] ? (W16 ? h[u] : f[u]) : 0.0f;
-        if (WQ) wq[u] = ok[u] ? h[u] : 0.0f;
     }
 }
 
@@ -316,8 +314,8 @@ __device__ __forceinline__ void arr4(const float4 v, float (&o)[4]) {
 // above tolRel.
 template <int MODE>
 __device__ __forceinline__ void band_update4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
-                                             const float (&g0)[4], const float (&g1)[4], const float (&q0)[4],
-                                             const float (&q1)[4], float& myAbs, float& myRel, float& myAct) {
+                                             const float (&g0)[4], const float (&g1)[4], float& myAbs, float& myRel,
+                                             float& myAct) {
     if (b0 >= x.nPrev) return;
     const uint32_t ldb = a.ldb;
     const size_t i0 = size_t(s0) * ldb + b0, i1 = i0 + ldb;
@@ -353,23 +351,12 @@ __device__ __forceinline__ void band_update4(const SweepArgs& a, const BandCtx& 
     arr4(ld4(a.r + i1), r1);
     arr4(ld4(a.t + i0), t0);
     arr4(ld4(a.t + i1), t1);
-    const float* csrc = MODE == MODE_REFRESH ? a.cBase : a.c;
-    arr4(ld4(csrc + i0), c0);
-    arr4(ld4(csrc + i1), c1);
+    arr4(ld4(a.c + i0), c0);
+    arr4(ld4(a.c + i1), c1);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         n0[j] = c0[j] + r0[j] * g0[j] + t0[j] * g1[j];
         n1[j] = c1[j] + r1[j] * g1[j] + t1[j] * g0[j];
-    }
-    if (MODE == MODE_REFRESH) {  // only active bands' c2 is ever read
-        float k0[4], k1[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            k0[j] = c0[j] + r0[j] * (g0[j] - q0[j]) + t0[j] * (g1[j] - q1[j]);
-            k1[j] = c1[j] + r1[j] * (g1[j] - q1[j]) + t1[j] * (g0[j] - q0[j]);
-        }
-        st4(a.c2 + i0, k0);
-        st4(a.c2 + i1, k1);
     }
     if (a.fuseH && a.judge) {  // incident of the FP32 gather (judge sweeps gather FP32)
         float D0[4], D1[4], H0[4], H1[4];
@@ -458,9 +445,8 @@ __device__ __forceinline__ void band_partials(const SweepArgs& a, float myAbs, f
     }
 }
 
-// FP32 gathers, 4 bands per lane.  MODE_SWEEP with FP32 (W16 false) or BF16
-// weights; MODE_REFRESH sums both (the BF16 sum of round_bf16(B) when
-// qRound, matching the BF16 copy the BF16 sweeps gather); MODE_INCIDENT.
+// FP32 gathers, 4 bands per lane: MODE_SWEEP with FP32 (W16 false) or BF16
+// weights, or MODE_INCIDENT.
 template <int MODE, bool W16>
 __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
     BandCtx x;
@@ -475,7 +461,7 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
         const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
         for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
             const uint32_t b0 = cb + 4u * lane;
-            float g[2][4] = {}, q[2][4] = {};
+            float g[2][4] = {};
             if (b0 < x.nAct) {
                 const float* __restrict__ Bl = x.Bin + b0;
 #pragma unroll
@@ -483,9 +469,9 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
                     const uint32_t start = rb[side], end = rb[side + 1];
                     for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += kU) {
                         uint32_t n[kU];
-                        float w[kU], wq[kU];
+                        float w[kU];
                         bool ok[kU];
-                        load_group<W16, MODE == MODE_REFRESH>(a, base, start, end, n, w, wq, ok);
+                        load_group<W16>(a, base, start, end, n, w, ok);
                         float4 v[kU];
 #pragma unroll
                         for (int u = 0; u < kU; ++u)
@@ -495,16 +481,12 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
                         for (int u = 0; u < kU; ++u) {
                             const float vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                g[side][j] += w[u] * vv[j];
-                                if (MODE == MODE_REFRESH)
-                                    q[side][j] += wq[u] * (a.qRound ? round_bf16(vv[j]) : vv[j]);
-                            }
+                            for (int j = 0; j < 4; ++j) g[side][j] += w[u] * vv[j];
                         }
                     }
                 }
             }
-            band_update4<MODE>(a, x, s0, b0, g[0], g[1], q[0], q[1], myAbs, myRel, myAct);
+            band_update4<MODE>(a, x, s0, b0, g[0], g[1], myAbs, myRel, myAct);
         }
     }
     if (MODE == MODE_INCIDENT) return;
@@ -535,9 +517,9 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
                     const uint32_t start = rb[side], end = rb[side + 1];
                     for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += kU) {
                         uint32_t n[kU];
-                        float w[kU], wq[kU];
+                        float w[kU];
                         bool ok[kU];
-                        load_group<true, false>(a, base, start, end, n, w, wq, ok);
+                        load_group<true>(a, base, start, end, n, w, ok);
                         uint4 v[kU];
 #pragma unroll
                         for (int u = 0; u < kU; ++u)
@@ -556,7 +538,6 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
                 }
             }
             // two groups of 4 bands (the second may lie beyond ldb)
-            const float z[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             float lo0[4], lo1[4], hi0[4], hi1[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -565,8 +546,8 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
                 hi0[j] = g[0][4 + j];
                 hi1[j] = g[1][4 + j];
             }
-            band_update4<MODE_SWEEP>(a, x, s0, b0, lo0, lo1, z, z, myAbs, myRel, myAct);
-            if (b0 + 4u < a.ldb) band_update4<MODE_SWEEP>(a, x, s0, b0 + 4u, hi0, hi1, z, z, myAbs, myRel, myAct);
+            band_update4<MODE_SWEEP>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
+            if (b0 + 4u < a.ldb) band_update4<MODE_SWEEP>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
         }
     }
     band_partials(a, myAbs, myRel, 0.0f);
@@ -811,7 +792,7 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         if (half) hipLaunchKernelGGL(k_sweep_hv, dim3(g), dim3(256), 0, st, a);
         else if (mode == MODE_SWEEP && !w16) hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, false>), dim3(g), dim3(256), 0, st, a);
         else if (mode == MODE_SWEEP) hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, true>), dim3(g), dim3(256), 0, st, a);
-        else if (mode == MODE_REFRESH) hipLaunchKernelGGL((k_sweep_v<MODE_REFRESH, false>), dim3(g), dim3(256), 0, st, a);
+        else if (mode == MODE_REFRESH) fail(FH_RUNTIME_ERROR, "refresh sweeps run on the entry-parallel layouts only");
         else hipLaunchKernelGGL((k_sweep_v<MODE_INCIDENT, false>), dim3(g), dim3(256), 0, st, a);
         if (mode != MODE_INCIDENT && !a.standalone)
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, a.partAct, g,
@@ -986,7 +967,7 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     }
     uint64_t launched = 0;
     uint32_t batch = 4;
-    if (bf16) c->c2.ensure(size_t(c->S) * o.ldb);
+    if (bf16 && !half) c->c2.ensure(size_t(c->S) * o.ldb);
     if (half) {
         // Phase 1: BF16 sweeps gathering BF16 copies of B (X16a = bf16(bufs[0])
         // first), ending at relative change firstInner (about the distance of
@@ -1084,8 +1065,9 @@ void estimate_rho2(fh_context* c, uint32_t ldb, uint32_t nb, const float* rr, co
     launch_fill_f32(c, zero, n, 0.0f);
     c->bandMax.ensure(2u * ldb);
     launch_fill_u32(c, c->bandMax.p, 2u * ldb, 0u);
-    // B_k+1 = K B_k (c = 0); maxima after 8 and 10 sweeps (both land in v0)
-    constexpr int m = 8;
+    // B_k+1 = K B_k (c = 0); maxima after m and m + 2 sweeps (both land in
+    // v0); FH_RHO_SWEEPS overrides m (tuning knob)
+    const int m = std::max(2, int(env_float("FH_RHO_SWEEPS", 8.0f))) & ~1;
     for (int k = 0; k < m + 2; ++k) {
         launch_single_sweep(c, ldb, zero, rr, tt, (k & 1) ? v1 : v0, (k & 1) ? v0 : v1, false);
         if (k == m - 1 || k == m + 1) {
@@ -1104,7 +1086,7 @@ void estimate_rho2(fh_context* c, uint32_t ldb, uint32_t nb, const float* rr, co
         std::memcpy(&z, &h[ldb + b], 4);
         if (a > 0.0f && z > 0.0f && std::isfinite(a) && std::isfinite(z)) {
             // two-step ratio (eigenvalues come in +-lambda pairs), 2 % margin
-            const float rho = std::min(0.98f, 1.02f * std::sqrt(z / a));
+            const float rho = std::min(0.98f, env_float("FH_RHO_MARGIN", 1.02f) * std::sqrt(z / a));
             r2[b] = rho * rho;
         }
     }

@@ -26,9 +26,6 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float x) {
     return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
-// x rounded to BF16 and widened again: the value a BF16 copy of x holds
-__device__ __forceinline__ float round_bf16(float x) { return bf16_to_f32(f32_to_bf16(x)); }
-
 template <int NBC>
 __device__ __forceinline__ void gather(const float* __restrict__ p, float w, float (&acc)[NBC]) {
     if constexpr (NBC == 1) {

@@ -392,3 +392,91 @@ def test_many_bands_match_oracle(oracle, nb):
         assert _rel(res.radiosity, ref) < RTOL, prec
     H = eng.incidentBands(ref, D, sky)
     assert _rel(H, oracle.incident_bands(og, ref, D, sky)) < RTOL
+
+
+class _Hip:
+    """hipMalloc / hipMemcpy / hipFree through ctypes, for the device-pointer entry points."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.lib = C.CDLL("libamdhip64.so")
+        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipFree.argtypes = [C.c_void_p]
+        self.lib.hipDeviceSynchronize.argtypes = []
+        self.held = []
+
+    def put(self, a):
+        C = self.C
+        a = np.ascontiguousarray(a, np.float32)
+        p = C.c_void_p()
+        assert self.lib.hipMalloc(C.byref(p), max(a.nbytes, 4)) == 0
+        assert self.lib.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+        self.held.append(p)
+        return p
+
+    def get(self, p, shape):
+        out = np.empty(shape, np.float32)
+        assert self.lib.hipDeviceSynchronize() == 0
+        assert self.lib.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0
+        return out
+
+    def free(self):
+        for p in self.held:
+            self.lib.hipFree(p)
+
+
+@pytest.mark.parametrize("nb", [1, 8, 47])
+def test_device_pointer_entry_points(oracle, nb):
+    # fh_sweep_device: one Jacobi sweep on device arrays [S][ldb] equals
+    # c + r G_s + t G_o with G = F B from the graph's own weights;
+    # fh_solve_bands_device equals fh_solve_bands on the same inputs
+    print(f"device pointers nb={nb}", flush=True)
+    import ctypes as C
+    from streamsim_amd import _lib
+    sc = scenes.row_crop(leaves_per_row=60, cells=8)
+    eng = _engine(R=64, seed=2)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    g = eng.getGraph()
+    S = sc.surface_count
+    lib = _lib.load()
+    ldb = int(lib.fh_band_stride(nb))
+    rng = np.random.default_rng(nb)
+    c = rng.uniform(0.0, 50.0, (S, ldb)).astype(np.float32)
+    r = rng.uniform(0.0, 0.5, (S, ldb)).astype(np.float32)
+    t = (rng.uniform(0.0, 0.4, (S, ldb))).astype(np.float32)
+    B = rng.uniform(0.0, 80.0, (S, ldb)).astype(np.float32)
+    hip = _Hip()
+    try:
+        dc, dr, dt, dB, dO = hip.put(c), hip.put(r), hip.put(t), hip.put(B), hip.put(np.zeros((S, ldb)))
+        st = lib.fh_sweep_device(eng.handle, nb, dc, dr, dt, dB, dO, _lib.FH_PRECISION_FP32)
+        assert st == 0
+        out = hip.get(dO, (S, ldb))
+        rows = np.repeat(np.arange(S), np.diff(g.rowOffsets))
+        w = (g.counts / np.maximum(g.denominators[rows], 1)).astype(np.float32).astype(np.float64)
+        G = np.zeros((S, ldb))
+        np.add.at(G, rows, w[:, None] * B[g.neighbors].astype(np.float64))
+        Go = G[np.arange(S) ^ 1]
+        ref = c + r * G + t * Go
+        np.testing.assert_allclose(out[:, :nb], ref[:, :nb], rtol=2e-5, atol=1e-3)
+        # the multi-band solve from device arrays [S][nb] equals the host-array call
+        rr = r[:, :nb].copy()
+        tt = (t[:, :nb] * (1.0 - rr)).astype(np.float32)
+        e = rng.uniform(0.0, 5.0, (S, nb)).astype(np.float32)
+        D = rng.uniform(0.0, 300.0, (S, nb)).astype(np.float32)
+        sky = rng.uniform(10.0, 80.0, nb).astype(np.float32)
+        host = eng.solveBands(rr, tt, e, D, sky, tolRel=1e-6)
+        dout = hip.put(np.zeros((S, nb)))
+        stats = _lib.FhSolveStats()
+        st = lib.fh_solve_bands_device(eng.handle, nb, hip.put(rr), hip.put(tt), hip.put(e), hip.put(D),
+                                       sky.ctypes.data, None, 1000, 1e-6, 1.0,
+                                       _lib.FH_PRECISION_FP32, dout, C.byref(stats))
+        assert st == 0
+        dev = hip.get(dout, (S, nb))
+        np.testing.assert_array_equal(dev, np.asarray(host.radiosity).reshape(S, nb))
+        assert stats.iterations == host.iterations
+    finally:
+        hip.free()
+
