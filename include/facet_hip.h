@@ -250,6 +250,9 @@ typedef struct {
     uint32_t firstNode;        /* 1: T = airTemperatureK (core:3470-3479)  */
     uint32_t zeroStart;        /* 1: every solve starts from B = 0         */
     uint32_t recordFeatures;   /* 1: keep the surrogate features (training) */
+    uint32_t adaptiveRelaxation; /* 1: full Newton steps; temperatureRelaxation only for a
+                                    unit whose step reverses direction (same fixed point;
+                                    the stop test then bounds the full step) */
 } fh_eb_node_params;
 
 typedef struct {

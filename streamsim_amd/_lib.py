@@ -76,7 +76,7 @@ class FhEbNodeParams(C.Structure):
         ("temperatureTolerance", C.c_float), ("temperatureRelaxation", C.c_float), ("maxCoupling", C.c_uint32),
         ("vegetationMethod", C.c_uint32), ("radiosityTolerance", C.c_float), ("maxSweeps", C.c_uint32),
         ("precision", C.c_uint32), ("warmStart", C.c_uint32), ("firstNode", C.c_uint32),
-        ("zeroStart", C.c_uint32), ("recordFeatures", C.c_uint32),
+        ("zeroStart", C.c_uint32), ("recordFeatures", C.c_uint32), ("adaptiveRelaxation", C.c_uint32),
     ]
 
 

@@ -32,7 +32,19 @@ struct EbConst {
     double Ta, ea, P, ha, rho, ra, dTime, relax;
     uint32_t vegMethod;
     uint32_t leafCount;
+    uint32_t adaptive;  // 1: full Newton steps, relax only where a step reverses direction
 };
+
+// Relaxation of one Newton step: the configured one, or with adaptive
+// relaxation 1 unless the unit's full step changed sign since its previous
+// step (an oscillation), which then takes the configured relaxation; prev
+// holds the unit's previous full step.
+__device__ __forceinline__ double step_relaxation(const EbConst& k, float* prev, double step) {
+    if (!k.adaptive || !prev) return k.relax;
+    const double w = double(*prev) * step < 0.0 ? k.relax : 1.0;
+    *prev = float(step);
+    return w;
+}
 
 __device__ __forceinline__ double clampd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
@@ -211,7 +223,8 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
                                   const float* __restrict__ absorbed, float H0, float H1, const float* __restrict__ eps,
                                   float* __restrict__ T, const float* __restrict__ Tprev, bool update,
                                   float* __restrict__ Rn_out, float* __restrict__ H_out, float* __restrict__ LE_out,
-                                  float* __restrict__ G_out, const double* __restrict__ area) {
+                                  float* __restrict__ G_out, const double* __restrict__ area,
+                                  float* __restrict__ stepPrev = nullptr) {
     NewtonOut o{0.0f, 0.0f, 0.0f, 0.0};
     const uint32_t s0 = 2u * f;
     const float Hs[2] = {H0, H1};
@@ -253,7 +266,8 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
             }
         if (update) {
             const double cand = clampd(Tl + res / fmax(1.0, der), 220.0, 360.0);
-            const double upd = Tl + k.relax * (cand - Tl);
+            const double w = step_relaxation(k, stepPrev ? stepPrev + s0 : nullptr, cand - Tl);
+            const double upd = Tl + w * (cand - Tl);
             o.dT = float(fabs(upd - Tl));
             T[s0] = float(upd);
             T[s0 + 1] = float(upd);
@@ -287,7 +301,8 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
             }
             if (update) {
                 const double cand = clampd(Ts + res / fmax(1.0, der), 220.0, 360.0);
-                const double upd = Ts + k.relax * (cand - Ts);
+                const double w = step_relaxation(k, stepPrev ? stepPrev + s : nullptr, cand - Ts);
+                const double upd = Ts + w * (cand - Ts);
                 o.dT = fmaxf(o.dT, float(fabs(upd - Ts)));
                 T[s] = float(upd);
             }
@@ -364,6 +379,7 @@ struct CoupledArgs {
     SolveCtrl* ctrl;
     float4* partial;
     float* Iin;  // per side: unclamped incident fsky L + G of the gather half
+    float* stepPrev;  // per side: previous full Newton step (adaptive relaxation)
     const uint32_t* sched;  // facet gathered by logical team i (null: facet i)
 };
 
@@ -406,7 +422,7 @@ __global__ __launch_bounds__(256) void k_coupled_update(CoupledArgs a) {
         a.Hout[s0] = H0;
         a.Hout[s0 + 1] = H1;
         const NewtonOut o = newton_facet(f, a.k, a.mats[a.mat[f]], a.absorbed, H0, H1, a.eps, a.T, a.Tprev, true,
-                                         nullptr, nullptr, nullptr, nullptr, a.area);
+                                         nullptr, nullptr, nullptr, nullptr, a.area, a.stepPrev);
         v[2] = o.dT;
         v[3] = o.eq;
         const float ins[2] = {in0, in1};
@@ -591,6 +607,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     FH_HIP(hipStreamSynchronize(c->stream));
     e.swValid = e.lwValid = e.tValid = false;
     e.swRhoValid = false;
+    e.guessPrevValid = false;
     if (swNb && c->hasGraph) {
         // spectral radius per internal band of this scene's SW operator (the
         // Chebyshev parameter): r and t do not depend on the sun, so the
@@ -692,6 +709,9 @@ static uint32_t coupled_loop(fh_context* c, const fh_eb_node_params& p, const Eb
     a.partial = e.part4.p;
     a.Iin = e.Iin.p;
     a.sched = sweep_schedule(c);
+    e.stepPrev.ensure(c->S);
+    launch_fill_f32(c, e.stepPrev.p, c->S, 0.0f);
+    a.stepPrev = e.stepPrev.p;
     uint64_t launched = 0;
     uint32_t batch = 4;
     while (true) {
@@ -810,15 +830,21 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
     k.ra = k.rho * 1005.0 / std::max(0.1, k.ha);
     k.dTime = std::max(1.0, double(p.dTime));
     k.relax = p.temperatureRelaxation;
+    k.adaptive = p.adaptiveRelaxation ? 1u : 0u;
     k.vegMethod = p.vegetationMethod;
     k.leafCount = e.leafCount;
-    if (p.firstNode || !e.tValid) launch_fill_f32(c, e.T.p, S, p.airTemperatureK);
+    if (p.firstNode || !e.tValid) {
+        launch_fill_f32(c, e.T.p, S, p.airTemperatureK);
+        e.guessPrevValid = false;
+    }
     copy_f32(c, e.T.p, e.Tprev.p, S);
     e.tValid = true;
-    if (p.warmStart == FH_WARM_SURROGATE)
+    if (p.warmStart == FH_WARM_SURROGATE) {
         surrogate_guess(c, k, p.windSpeed, p.skyLongwave, p.airTemperatureK);
-    else if (p.recordFeatures)
-        eb_features(c, p.airTemperatureK, p.windSpeed, p.skyLongwave);
+    } else {
+        e.guessPrevValid = false;
+        if (p.recordFeatures) eb_features(c, p.airTemperatureK, p.windSpeed, p.skyLongwave);
+    }
     // --- coupling: fused Jacobi-Newton sweeps, FP32 refinement, final check ---
     if (!e.lwValid || p.zeroStart) st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
     float last[4] = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -972,6 +998,7 @@ int fh_eb_set_temperature(fh_context* c, const float* T) {
         FH_HIP(hipMemcpyAsync(c->eb.T.p, T, sizeof(float) * c->S, hipMemcpyHostToDevice, c->stream));
         FH_HIP(hipStreamSynchronize(c->stream));
         c->eb.tValid = true;
+        c->eb.guessPrevValid = false;  // T no longer follows the warm-started chain
     });
 }
 

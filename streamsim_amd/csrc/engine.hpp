@@ -142,6 +142,7 @@ struct EbState {
     DBuf<float2> fout;             // reduced (max dT, max residual-equivalent)
     DBuf<float4> part4;            // coupled-sweep per-block partials
     DBuf<float> Iin;               // coupled sweep: incident before clamping, per side
+    DBuf<float> stepPrev;          // coupled sweep: previous full Newton step per side
     DBuf<float> last4;             // last coupled-sweep maxima
     DBuf<uint32_t> lit;
     // surrogate inputs / outputs per energy-balance unit (leaf facet or
@@ -151,6 +152,9 @@ struct EbState {
     DBuf<float> feat;              // units x 9 raw features
     DBuf<float> featN;             // units x 9 normalised
     DBuf<float> guess;             // units
+    DBuf<float> guessPrev;         // units: the previous node's network output
+    float guessPrevTa = 0.0f;      // air temperature of that node
+    bool guessPrevValid = false;   // the previous node was warm-started by the network
     bool featValid = false;
 };
 

@@ -353,6 +353,24 @@ __global__ void k_apply_guess(uint32_t units, uint32_t leafCount, const float* _
     }
 }
 
+// Residual form: T += (Ta - TaPrev) + sd (y - yPrev), the change the network
+// predicts between the previous node and this one, applied to the previous
+// node's converged state (the network's own bias per unit cancels)
+__global__ void k_apply_delta(uint32_t units, uint32_t leafCount, const float* __restrict__ y,
+                              const float* __restrict__ yPrev, float dTa, float sd, float* __restrict__ T) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= units) return;
+    const float d = dTa + sd * (y[u] - yPrev[u]);
+    if (u < leafCount) {
+        const float t = fminf(fmaxf(T[2u * u] + d, 220.0f), 360.0f);
+        T[2u * u] = t;
+        T[2u * u + 1u] = t;
+    } else {
+        const uint32_t s = 2u * leafCount + (u - leafCount);
+        T[s] = fminf(fmaxf(T[s] + d, 220.0f), 360.0f);
+    }
+}
+
 static uint32_t unit_count(const fh_context* c) { return c->eb.leafCount + 2u * (c->F - c->eb.leafCount); }
 
 // Computes the raw + normalised features of every unit for the current node
@@ -389,9 +407,20 @@ void surrogate_guess(fh_context* c, const EbConst&, float windSpeed, float skyLo
     const uint32_t units = unit_count(c);
     eb_features(c, Ta, windSpeed, skyLongwave);
     infer(c, e.featN.p, units, e.guess.p);
-    hipLaunchKernelGGL(k_apply_guess, dim3((units + 255u) / 256u), dim3(256), 0, c->stream, units, e.leafCount,
-                       e.guess.p, Ta, c->mlp.outMean, c->mlp.outStd, e.T.p);
+    const dim3 grid((units + 255u) / 256u);
+    if (e.guessPrevValid) {
+        // the previous node was also warm-started: predict the change
+        hipLaunchKernelGGL(k_apply_delta, grid, dim3(256), 0, c->stream, units, e.leafCount, e.guess.p,
+                           e.guessPrev.p, Ta - e.guessPrevTa, c->mlp.outStd, e.T.p);
+    } else {
+        hipLaunchKernelGGL(k_apply_guess, grid, dim3(256), 0, c->stream, units, e.leafCount, e.guess.p, Ta,
+                           c->mlp.outMean, c->mlp.outStd, e.T.p);
+    }
     FH_HIP(hipGetLastError());
+    e.guessPrev.ensure(units);
+    FH_HIP(hipMemcpyAsync(e.guessPrev.p, e.guess.p, sizeof(float) * units, hipMemcpyDeviceToDevice, c->stream));
+    e.guessPrevTa = Ta;
+    e.guessPrevValid = true;
 }
 
 // ---- training driver -----------------------------------------------------------

@@ -168,6 +168,7 @@ class EbConfig:
     radiosity_tolerance: float = 1.0e-5
     max_sweeps: int = 2000
     dtime: float = 3600.0
+    adaptive_relaxation: bool = True
 
 
 def node_params(m: Meteo, cfg: EbConfig, first: bool, precision: int = 0, warm: int = 0, zero_start: bool = False,
@@ -192,6 +193,7 @@ def node_params(m: Meteo, cfg: EbConfig, first: bool, precision: int = 0, warm: 
     p.firstNode = 1 if first else 0
     p.zeroStart = 1 if zero_start else 0
     p.recordFeatures = 1 if record else 0
+    p.adaptiveRelaxation = 1 if cfg.adaptive_relaxation else 0
     return p
 
 
