@@ -59,12 +59,16 @@ def parse():
     return ap.parse_args()
 
 
-def sweep_bytes(S: int, nnz: int, nb: int, w: int, bf16_copy: bool = False) -> int:
+def sweep_bytes(S: int, nnz: int, nb: int, w: int, bf16_copy: bool = False, increments: bool = False) -> int:
     """Algorithmic HBM bytes of one sweep: row offsets, (col + value) per
-    entry, fsky, and per side-band c, r, t read + B read + B write; the BF16
-    sweep also reads the BF16 copy of B it gathers once and writes the new
-    copy (2 + 2 bytes per side-band)."""
+    entry, fsky, and per side-band c, r, t read + B read + B write; a BF16
+    sweep also reads the BF16 array it gathers once and writes the new one
+    (2 + 2 bytes per side-band); the increment sweep of BF16 solves besides
+    reads and writes the FP32 accumulated gathers and carries (16 bytes per
+    side-band)."""
     b = 4 * (S + 1) + (4 + w) * nnz + 4 * S + 20 * S * nb
+    if increments:
+        return b + 20 * S * nb
     return b + (4 * S * nb if bf16_copy else 0)
 
 
@@ -196,15 +200,16 @@ def main() -> None:
     windows = faceteb.ShortwaveWindows.for_bands(wl["sw_bands"])
     nb_total = windows.count + 1
     if a.mode == "sweep":
-        # counter runs: the SW sweep in FP32 and BF16, the LW sweep, and a
-        # 1 GiB copy as the byte calibration of the traffic counters
+        # counter runs: the SW sweep in FP32 and the increment sweep of BF16
+        # solves, the LW sweep, and a 1 GiB copy as the byte calibration of
+        # the traffic counters
         nb = windows.count
         out = {"workload": a.workload, "sides": S, "nnz": diag.directedEdgeCount}
         for name, nbk, prec in (("sw_fp32", nb, FH_PRECISION_FP32), ("sw_bf16", nb, FH_PRECISION_BF16_DC),
                                 ("lw_fp32", 1, FH_PRECISION_FP32)):
             sec = time_sweep(eng, nbk, prec, a.roofline_reps)
-            alg = sweep_bytes(S, diag.directedEdgeCount, nbk, 4 if prec == FH_PRECISION_FP32 else 2,
-                              bf16_copy=prec != FH_PRECISION_FP32 and nbk >= 16)
+            alg = sweep_bytes(S, diag.directedEdgeCount, nbk, 4,
+                              increments=prec != FH_PRECISION_FP32 and nbk >= 16)
             out[name] = {"us": sec * 1e6, "alg_bytes": alg, "GBps": alg / sec / 1e9}
         cs = eng.benchmarkCopy(1 << 30, 20)
         out["copy_1GiB"] = {"us": cs * 1e6, "GBps": 2.0 * (1 << 30) / cs / 1e9}
@@ -242,23 +247,24 @@ def main() -> None:
         ablation["surrogate_only"] = {"value": work / t_s, "seconds": t_s, **st_s.summary()}
     log(f"[bench] ablation {json.dumps({k: round(v['seconds'], 4) for k, v in ablation.items()})}")
 
-    # roofline of the dominant kernel, the FP32 shortwave SpMM sweep (all
-    # bands active), and of the BF16 sweep of the fast path's first phase
+    # roofline of the fast path's dominant kernel, the increment sweep of
+    # the BF16 shortwave solves (all bands active), and of the plain path's,
+    # the FP32 sweep
     nb = windows.count
     nnz = diag.directedEdgeCount
     sec16 = time_sweep(eng, nb, FH_PRECISION_BF16_DC, a.roofline_reps)
     sec32 = time_sweep(eng, nb, FH_PRECISION_FP32, a.roofline_reps)
-    alg16 = sweep_bytes(S, nnz, nb, 2, bf16_copy=True)
+    alg16 = sweep_bytes(S, nnz, nb, 4, increments=True)
     alg32 = sweep_bytes(S, nnz, nb, 4)
-    achieved = alg32 / sec32 / 1e9
-    log(f"[bench] SW sweep nb={nb}: fp32 {sec32 * 1e6:.1f} us ({achieved:.0f} GB/s), "
-        f"bf16 {sec16 * 1e6:.1f} us ({alg16 / sec16 / 1e9:.0f} GB/s)")
+    achieved = alg16 / sec16 / 1e9
+    log(f"[bench] SW sweep nb={nb}: increments {sec16 * 1e6:.1f} us ({achieved:.0f} GB/s), "
+        f"fp32 {sec32 * 1e6:.1f} us ({alg32 / sec32 / 1e9:.0f} GB/s)")
     traffic32 = traffic16 = None
     tf = ROOT / "profiles" / f"sweep_traffic_{a.workload}.json"
     if tf.exists():
         tj = json.loads(tf.read_text())
         traffic32 = tj.get("k_sweep_v", {}).get("bytes_per_launch")
-        traffic16 = tj.get("k_sweep_hv", {}).get("bytes_per_launch")
+        traffic16 = tj.get("k_sweep_inc", {}).get("bytes_per_launch")
 
     cpu = cpu_all = None
     if not a.no_cpu_baseline:
@@ -303,8 +309,8 @@ def main() -> None:
             "time_steps": steps,
             "tolerance_K": cfg.tolerance,
             "radiosity_rel_tol": cfg.radiosity_tolerance,
-            "fast_path": "shortwave: BF16 F and BF16-gathered B, then FP32 sweeps to the FP32 fixed point"
-                         + (" + surrogate warm start" if have_sur else ""),
+            "fast_path": "shortwave: BF16 increments of B gathered, FP32 weights and accumulation (FP32 "
+                         "fixed point)" + (" + surrogate warm start" if have_sur else ""),
             "parallelism": "single GPU",
         },
         "plain_fp32": {"value": plain_value, "seconds": t_plain, **st_plain.summary()},
@@ -314,12 +320,12 @@ def main() -> None:
         "form_factors": {"seconds": t_graph, "rays_per_s": rays_per_s, "directed_edges": nnz,
                          "bvh_host_seconds": t_bvh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic32,
-                     "kernel": f"k_sweep_v: FP32 shortwave sweep, {nb} bands", "bytes_per_launch": alg32,
-                     "us_per_launch": sec32 * 1e6,
-                     "bf16_sweep": {"kernel": f"k_sweep_hv: BF16-gather shortwave sweep, {nb} bands",
-                                    "achieved": alg16 / sec16 / 1e9, "bytes_per_launch": alg16,
-                                    "us_per_launch": sec16 * 1e6, "traffic": traffic16}},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic16,
+                     "kernel": f"k_sweep_hv<2>: increment sweep of the BF16 shortwave solves, {nb} bands",
+                     "bytes_per_launch": alg16, "us_per_launch": sec16 * 1e6,
+                     "fp32_sweep": {"kernel": f"k_sweep_v<0, false>: FP32 shortwave sweep of the plain path, {nb} bands",
+                                    "achieved": alg32 / sec32 / 1e9, "bytes_per_launch": alg32,
+                                    "us_per_launch": sec32 * 1e6, "traffic": traffic32}},
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
     }

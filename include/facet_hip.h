@@ -155,7 +155,10 @@ uint32_t fh_surface_count(const fh_context *ctx);
 /* Arrays are [surfaceCount][nb] (band fastest); sky is [nb]; init may be
  * NULL.  Runs until, for every band, the largest relative change over the
  * sides in an FP32 sweep is <= tolRel (from 16 bands on, a band that meets
- * it stops iterating while the others continue), or maxIter sweeps. */
+ * it stops iterating while the others continue), or maxIter sweeps.
+ * FH_PRECISION_BF16_DC from 16 bands on gathers BF16 increments of the
+ * radiosity with FP32 weights and accumulation, reaching the FP32 fixed
+ * point; its stopping test uses max(tolRel, 3e-7). */
 int fh_solve_bands(fh_context *ctx, uint32_t nb, const float *reflectance,
                    const float *transmittance, const float *emission, const float *direct,
                    const float *sky, const float *init, uint32_t maxIter, float tolRel,
@@ -186,8 +189,11 @@ uint32_t fh_band_stride(uint32_t nb);
 void *fh_stream(const fh_context *ctx);
 
 /* Average duration (microseconds, HIP events on the context's stream) of
- * `reps` back-to-back single-kernel SpMM sweeps (fh_sweep_device) over the
- * current graph with nb bands, on engine-owned HBM inputs. */
+ * `reps` back-to-back single-kernel SpMM sweeps over the current graph with
+ * nb bands, on engine-owned HBM inputs: the sweep of fh_sweep_device, except
+ * that BF16 precision at band-parallel strides (nb >= 16) times the
+ * increment sweep the BF16 solves run (BF16 increments of B gathered with
+ * FP32 weights, FP32 accumulated gathers and carries). */
 int fh_benchmark_sweep(fh_context *ctx, uint32_t nb, uint32_t precision, uint32_t reps,
                        float *microsecondsPerSweep);
 /* Average duration (microseconds) of a plain device-to-device copy kernel of

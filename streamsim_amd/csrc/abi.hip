@@ -601,25 +601,34 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
         fh::launch_fill_f32(c, B0, n, 50.0f);
         fh::launch_fill_f32(c, B1, n, 0.0f);
         const bool w16 = precision == FH_PRECISION_BF16_DC;
-        // BF16 solves of band-parallel strides gather BF16 copies of B
+        // BF16 solves of band-parallel strides run increment sweeps: BF16
+        // increments gathered, accumulated gathers and carries in FP32
         fh::DBuf<uint16_t> h16;
+        fh::DBuf<float> acc;
         uint16_t* H0 = nullptr;
         uint16_t* H1 = nullptr;
+        float* G = nullptr;
+        float* K = nullptr;
         if (w16 && fh::uses_bf16_gather(ldb)) {
             const size_t nh = size_t(c->S) * fh::bf16_stride(ldb);
             h16.ensure(2 * nh);
             H0 = h16.p;
             H1 = h16.p + nh;
-            fh::launch_fill_u32(c, reinterpret_cast<uint32_t*>(h16.p), nh, 0x42484248u);  // 50.0 in BF16
+            fh::launch_fill_u32(c, reinterpret_cast<uint32_t*>(h16.p), nh, 0x3c233c23u);  // 0.01 in BF16
+            acc.ensure(2 * n);
+            G = acc.p;
+            K = acc.p + n;
+            fh::launch_fill_f32(c, G, n, 100.0f);
+            fh::launch_fill_f32(c, K, n, 0.0f);
         }
-        for (int i = 0; i < 3; ++i) fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1);
+        for (int i = 0; i < 3; ++i) fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1, G, K);
         hipEvent_t e0, e1;
         FH_HIP(hipEventCreate(&e0));
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c->stream));
         for (uint32_t i = 0; i < reps; ++i)
             fh::launch_single_sweep(c, ldb, cc, rr, tt, (i & 1u) ? B1 : B0, (i & 1u) ? B0 : B1, w16,
-                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1);
+                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1, G, K);
         FH_HIP(hipEventRecord(e1, c->stream));
         FH_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

@@ -208,7 +208,8 @@ struct fh_context {
     fh::DBuf<fh::SolveCtrl> ctrl;
     fh::DBuf<float2> partial;        // per-block residual partials
     fh::DBuf<uint32_t> partAct;      // per-block 1 + highest unconverged band
-    fh::DBuf<uint16_t> X16a, X16b;   // BF16 copies of the radiosity gathered by BF16 sweeps
+    fh::DBuf<uint16_t> X16a, X16b;   // BF16 copies of the radiosity (or its increments) gathered by BF16 sweeps
+    fh::DBuf<float> gacc, carry;     // increment sweeps: accumulated gathers, B minus the gathered state
     fh::DBuf<float> omega;           // Chebyshev weights per band
     fh::DBuf<uint32_t> bandMax;      // per-band maxima (float bits) of the rho estimate
     fh::SolveCtrl* hctrl = nullptr;  // pinned mirror

@@ -79,9 +79,11 @@ const uint32_t* sweep_schedule(const fh_context* c);
 int sweep_team(const fh_context* c);
 
 // One sweep, no host synchronisation (timing entry).  With w16 and BF16
-// copies given (band-parallel strides), the BF16-gather sweep h16in -> h16out.
+// copies given (band-parallel strides), the BF16-gather sweep h16in -> h16out;
+// with gacc and carry ([S][ldb]) also given, the increment sweep of BF16
+// solves (h16in / h16out hold BF16 increments).
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
                          const float* Bin, float* Bout, bool w16, uint16_t* h16in = nullptr,
-                         uint16_t* h16out = nullptr);
+                         uint16_t* h16out = nullptr, float* gacc = nullptr, float* carry = nullptr);
 
 }  // namespace fh

@@ -85,6 +85,12 @@ struct SweepArgs {
     const uint32_t* sched;  // facet processed by logical team i (null: facet i)
     const float* omega;  // Chebyshev weights per band for this sweep (band-parallel FP32), or null
     const float* rho2;   // squared spectral radius estimate per band (Chebyshev)
+    // increment sweeps (band-parallel BF16 solves): 1 = base sweep, 2 =
+    // increment sweep, 0 = neither; gacc = accumulated gathers M Q, carry =
+    // B - Q per side-band ([S][ldb] FP32), Q the state the gathers have seen
+    uint32_t inc;
+    float* gacc;
+    float* carry;
 };
 
 template <int NBC, int TEAM, int MODE, bool W16>
@@ -436,6 +442,130 @@ __device__ __forceinline__ void band_update4(const SweepArgs& a, const BandCtx& 
     }
 }
 
+// Row update of an increment sweep at the 4 bands b0 .. b0 + 3 of the side
+// pair (s0, s0 + 1).  The gathers see Q, a copy of B that only ever changes
+// by BF16 increments: sweep k gathers the increments dq_k (BF16) and
+// accumulates G = M Q_k in FP32; each side keeps carry = B - Q exactly.  The
+// update from G is the FP32 one (Jacobi or Chebyshev), and the next increment
+// is dq_k+1 = bf16(B_k+1 - Q_k) with carry_k+1 = (B_k+1 - Q_k) - dq_k+1, so
+// what rounding drops in one sweep is sent in the next and Q -> B as B
+// converges: the fixed point is the FP32 one.  The base sweep (BASE) gathers
+// Q_0 = bf16(B_0) itself.
+template <bool BASE>
+__device__ __forceinline__ void band_update_inc4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
+                                                 float (&g0)[4], float (&g1)[4], float& myAbs, float& myRel,
+                                                 float& myAct) {
+    if (b0 >= x.nPrev) return;
+    const uint32_t ldb = a.ldb;
+    const size_t i0 = size_t(s0) * ldb + b0, i1 = i0 + ldb;
+    float o0[4], o1[4];
+    arr4(ld4(x.Bin + i0), o0);
+    arr4(ld4(x.Bin + i1), o1);
+    if (b0 >= x.nAct) {  // retired by the last judge sweep: copied once
+        st4(x.Bout + i0, o0);
+        st4(x.Bout + i1, o1);
+        return;
+    }
+    float k0[4], k1[4];
+    if (BASE) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            k0[j] = o0[j] - bf16_to_f32(f32_to_bf16(o0[j]));
+            k1[j] = o1[j] - bf16_to_f32(f32_to_bf16(o1[j]));
+        }
+    } else {
+        float p0[4], p1[4];
+        arr4(ld4(a.gacc + i0), p0);
+        arr4(ld4(a.gacc + i1), p1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            g0[j] += p0[j];
+            g1[j] += p1[j];
+        }
+        arr4(ld4(a.carry + i0), k0);
+        arr4(ld4(a.carry + i1), k1);
+    }
+    st4(a.gacc + i0, g0);
+    st4(a.gacc + i1, g1);
+    bool act[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) act[j] = b0 + uint32_t(j) < x.nAct;
+    float r0[4], r1[4], t0[4], t1[4], c0[4], c1[4], n0[4], n1[4];
+    arr4(ld4(a.r + i0), r0);
+    arr4(ld4(a.r + i1), r1);
+    arr4(ld4(a.t + i0), t0);
+    arr4(ld4(a.t + i1), t1);
+    arr4(ld4(a.c + i0), c0);
+    arr4(ld4(a.c + i1), c1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        n0[j] = c0[j] + r0[j] * g0[j] + t0[j] * g1[j];
+        n1[j] = c1[j] + r1[j] * g1[j] + t1[j] * g0[j];
+    }
+    if (a.fuseH && a.judge) {  // incident of this sweep's gather
+        float sk[4], D0[4], D1[4], H0[4], H1[4];
+        arr4(ld4(a.skyv + b0), sk);
+        arr4(ld4(a.D + i0), D0);
+        arr4(ld4(a.D + i1), D1);
+        const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            H0[j] = fmaxf(f0 * sk[j] + g0[j] + D0[j], 0.0f);
+            H1[j] = fmaxf(f1 * sk[j] + g1[j] + D1[j], 0.0f);
+        }
+        if (act[3]) {
+            st4(a.H + i0, H0);
+            st4(a.H + i1, H1);
+        } else {  // retired bands keep the incident of their last judge sweep
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (act[j]) {
+                    a.H[i0 + j] = H0[j];
+                    a.H[i1 + j] = H1[j];
+                }
+        }
+    }
+    float y0[4], y1[4];
+    float w[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    if (a.omega) arr4(ld4(a.omega + b0), w);
+    const bool anyW = w[0] != 1.0f || w[1] != 1.0f || w[2] != 1.0f || w[3] != 1.0f;
+    float p0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, p1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (anyW) {  // Chebyshev: the iterate before B, which Bout still holds
+        arr4(ld4(x.Bout + i0), p0);
+        arr4(ld4(x.Bout + i1), p1);
+    }
+    uint32_t q[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        y0[j] = w[j] == 1.0f ? n0[j] : p0[j] + w[j] * (n0[j] - p0[j]);
+        y1[j] = w[j] == 1.0f ? n1[j] : p1[j] + w[j] * (n1[j] - p1[j]);
+        const float d0 = fabsf(n0[j] - o0[j]), d1 = fabsf(n1[j] - o1[j]);
+        if (!act[j]) {  // beyond nAct inside this vector: B unchanged
+            y0[j] = o0[j];
+            y1[j] = o1[j];
+        } else {
+            const float e0 = d0 / fmaxf(fabsf(n0[j]), 1.0e-30f), e1 = d1 / fmaxf(fabsf(n1[j]), 1.0e-30f);
+            myAbs = fmaxf(myAbs, fmaxf(d0, d1));
+            myRel = fmaxf(myRel, fmaxf(e0, e1));
+            if (fmaxf(e0, e1) > x.tolRel) myAct = fmaxf(myAct, float(b0 + uint32_t(j) + 1u));
+        }
+        // next increment and the part of it that BF16 does not carry
+        const float v0 = (y0[j] - o0[j]) + k0[j], v1 = (y1[j] - o1[j]) + k1[j];
+        const uint16_t h0 = f32_to_bf16(v0), h1 = f32_to_bf16(v1);
+        k0[j] = v0 - bf16_to_f32(h0);
+        k1[j] = v1 - bf16_to_f32(h1);
+        q[j] = uint32_t(h0) | (uint32_t(h1) << 16);
+    }
+    st4(x.Bout + i0, y0);
+    st4(x.Bout + i1, y1);
+    st4(a.carry + i0, k0);
+    st4(a.carry + i1, k1);
+    *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0) * a.ldh + b0) =
+        make_uint2((q[0] & 0xffffu) | (q[1] << 16), (q[2] & 0xffffu) | (q[3] << 16));
+    *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0 + 1) * a.ldh + b0) =
+        make_uint2((q[0] >> 16) | (q[1] & 0xffff0000u), (q[2] >> 16) | (q[3] & 0xffff0000u));
+}
+
 __device__ __forceinline__ void band_partials(const SweepArgs& a, float myAbs, float myRel, float myAct) {
     float v[3] = {myAbs, myRel, myAct}, out[3];
     block_max<3>(v, out);
@@ -495,10 +625,13 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
 
 // BF16 sweep: gathers the BF16 copy Bin16 (row stride ldh, a multiple of 64
 // bands) 8 bands per lane with BF16 weights; accumulates, updates and writes
-// B in FP32 and refreshes Bout16.
+// B in FP32 and refreshes Bout16.  INC = 1 / 2: base / increment sweep of the
+// increment scheme (band_update_inc4), FP32 weights, Bin16 / Bout16 hold the
+// BF16 increments.
+template <int INC>
 __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
     BandCtx x;
-    if (!band_ctx<MODE_SWEEP>(a, true, x)) return;
+    if (!band_ctx<MODE_SWEEP>(a, INC == 0, x)) return;
     const uint32_t bt = a.bt, tpb = 256u / bt, ldh = a.ldh;
     const uint32_t team = threadIdx.x / bt, lane = threadIdx.x - team * bt;
     const uint32_t fl = xcd_block(a.xcd) * tpb + team;
@@ -519,7 +652,7 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
                         uint32_t n[kU];
                         float w[kU];
                         bool ok[kU];
-                        load_group<true>(a, base, start, end, n, w, ok);
+                        load_group<INC == 0>(a, base, start, end, n, w, ok);
                         uint4 v[kU];
 #pragma unroll
                         for (int u = 0; u < kU; ++u)
@@ -546,11 +679,16 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
                 hi0[j] = g[0][4 + j];
                 hi1[j] = g[1][4 + j];
             }
-            band_update4<MODE_SWEEP>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
-            if (b0 + 4u < a.ldb) band_update4<MODE_SWEEP>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
+            if constexpr (INC == 0) {
+                band_update4<MODE_SWEEP>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
+                if (b0 + 4u < a.ldb) band_update4<MODE_SWEEP>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
+            } else {
+                band_update_inc4<INC == 1>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
+                if (b0 + 4u < a.ldb) band_update_inc4<INC == 1>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
+            }
         }
     }
-    band_partials(a, myAbs, myRel, 0.0f);
+    band_partials(a, myAbs, myRel, INC == 0 ? 0.0f : myAct);
 }
 
 // B -> BF16 copy with row stride ldh (bands >= ldb are zero)
@@ -760,6 +898,16 @@ static bool use_sched() {
     return v;
 }
 
+// BF16 band-parallel solves gather BF16 increments (FH_INCREMENTS=0: a BF16
+// phase, then FP32 sweeps; tuning knob)
+static bool use_increments() {
+    static const bool v = [] {
+        const char* e = std::getenv("FH_INCREMENTS");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return v;
+}
+
 const uint32_t* sweep_schedule(const fh_context* c) { return use_sched() && c->sched.p ? c->sched.p : nullptr; }
 
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
@@ -768,7 +916,7 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     if (band_parallel(a.ldb)) {
         // lanes per team: one 16-byte vector of the row each, at most 64
         // bands per team pass
-        const bool half = mode == MODE_SWEEP && w16 && a.h0;
+        const bool half = (mode == MODE_SWEEP && w16 && a.h0) || a.inc;
         const uint32_t vw = half ? 8u : 4u;
         a.bt = std::min((a.ldb + vw - 1u) / vw, 64u / vw);
         const uint32_t tpb = 256u / a.bt;
@@ -789,14 +937,16 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
             a.partAct = nullptr;
         }
         hipStream_t st = c->stream;
-        if (half) hipLaunchKernelGGL(k_sweep_hv, dim3(g), dim3(256), 0, st, a);
+        if (a.inc == 1u) hipLaunchKernelGGL(k_sweep_hv<1>, dim3(g), dim3(256), 0, st, a);
+        else if (a.inc) hipLaunchKernelGGL(k_sweep_hv<2>, dim3(g), dim3(256), 0, st, a);
+        else if (half) hipLaunchKernelGGL(k_sweep_hv<0>, dim3(g), dim3(256), 0, st, a);
         else if (mode == MODE_SWEEP && !w16) hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, false>), dim3(g), dim3(256), 0, st, a);
         else if (mode == MODE_SWEEP) hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, true>), dim3(g), dim3(256), 0, st, a);
         else if (mode == MODE_REFRESH) fail(FH_RUNTIME_ERROR, "refresh sweeps run on the entry-parallel layouts only");
         else hipLaunchKernelGGL((k_sweep_v<MODE_INCIDENT, false>), dim3(g), dim3(256), 0, st, a);
         if (mode != MODE_INCIDENT && !a.standalone)
             hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, a.partAct, g,
-                               a.judge, uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16),
+                               a.judge, uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16 && !a.inc),
                                a.rho2, const_cast<float*>(a.omega), a.ldb);
         FH_HIP(hipGetLastError());
         return;
@@ -938,7 +1088,13 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     const bool half = bf16 && bp;
     // FH_DC_FIRST overrides the end of the BF16 phase (tuning knob)
     const float firstInner = env_float("FH_DC_FIRST", o.firstInner);
-    launch_ctrl_reset(c, o.maxIter ? o.maxIter : 1u, o.tolRel, o.tolAbs, o.checkEvery, o.ldb,
+    // the accumulated gathers of increment solves are a running FP32 sum, so
+    // their last-ulp changes do not settle: such solves resolve relative
+    // changes down to kIncrementTolFloor (2.5 FP32 ulps)
+    constexpr float kIncrementTolFloor = 3.0e-7f;
+    const bool incr = half && use_increments();
+    const float tolRel = incr && o.tolRel >= 0.0f ? std::max(o.tolRel, kIncrementTolFloor) : o.tolRel;
+    launch_ctrl_reset(c, o.maxIter ? o.maxIter : 1u, tolRel, o.tolAbs, o.checkEvery, o.ldb,
                       half ? std::max(firstInner, o.tolRel) : o.tolRel);
     SweepArgs a{};
     a.r = rr;
@@ -968,7 +1124,37 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     uint64_t launched = 0;
     uint32_t batch = 4;
     if (bf16 && !half) c->c2.ensure(size_t(c->S) * o.ldb);
-    if (half) {
+    if (incr) {
+        // every sweep gathers BF16: the base sweep Q_0 = bf16(B_0), then the
+        // BF16 increments of B with their rounding carried to the next one
+        // (band_update_inc4); FP32 weights and accumulation, FP32 fixed point
+        const uint32_t ldh = bf16_stride(o.ldb);
+        const size_t nh = size_t(c->S) * ldh, nf = size_t(c->S) * o.ldb;
+        c->X16a.ensure(nh);
+        c->X16b.ensure(nh);
+        c->gacc.ensure(nf);
+        c->carry.ensure(nf);
+        hipLaunchKernelGGL(k_to_bf16, dim3(uint32_t((nh + 255) / 256)), dim3(256), 0, c->stream, bufs[0], c->S, o.ldb,
+                           ldh, c->X16a.p);
+        FH_HIP(hipGetLastError());
+        a.h0 = c->X16a.p;
+        a.h1 = c->X16b.p;
+        a.ldh = ldh;
+        a.gacc = c->gacc.p;
+        a.carry = c->carry.p;
+        a.c = cc;
+        a.judge = 1u;
+        while (true) {
+            for (uint32_t k = 0; k < batch; ++k) {
+                a.inc = launched == 0 ? 1u : 2u;
+                launch_sweep(c, MODE_SWEEP, false, a);
+                ++launched;
+            }
+            read_ctrl(c);
+            if (c->hctrl->done) break;
+            if (batch < 32u) batch *= 2u;
+        }
+    } else if (half) {
         // Phase 1: BF16 sweeps gathering BF16 copies of B (X16a = bf16(bufs[0])
         // first), ending at relative change firstInner (about the distance of
         // the BF16 operator's fixed point) or after innerBf16 sweeps.
@@ -1105,8 +1291,15 @@ void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, f
 }
 
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
-                         const float* Bin, float* Bout, bool w16, uint16_t* h16in, uint16_t* h16out) {
+                         const float* Bin, float* Bout, bool w16, uint16_t* h16in, uint16_t* h16out, float* gacc,
+                         float* carry) {
     SweepArgs a{};
+    if (w16 && h16in && gacc && carry && band_parallel(ldb)) {
+        a.inc = 2u;
+        a.gacc = gacc;
+        a.carry = carry;
+        w16 = false;
+    }
     a.h0 = h16in;
     a.h1 = h16out;
     a.ldh = bf16_stride(ldb);

@@ -394,6 +394,34 @@ def test_many_bands_match_oracle(oracle, nb):
     assert _rel(H, oracle.incident_bands(og, ref, D, sky)) < RTOL
 
 
+@pytest.mark.parametrize("nb", [24, 47])
+def test_bf16_increments_follow_fp32(oracle, nb):
+    # BF16 solves at band-parallel strides gather BF16 increments of B and
+    # carry what rounding drops into the next increment: from the same warm
+    # start they take the FP32 solve's sweeps and reach its fixed point (their
+    # stopping test resolves relative changes down to 3e-7, 2.5 FP32 ulps)
+    print(f"bf16 increments nb={nb}", flush=True)
+    sc = scenes.row_crop(leaves_per_row=60, cells=8)
+    eng = _engine(R=64, seed=9)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    rng = np.random.default_rng(100 + nb)
+    S = sc.surface_count
+    r = rng.uniform(0.0, 0.5, (S, nb)).astype(np.float32)
+    t = (rng.uniform(0.0, 1.0, (S, nb)) * (1.0 - r) * 0.8).astype(np.float32)
+    e = np.zeros((S, nb), np.float32)
+    D = rng.uniform(0.0, 300.0, (S, nb)).astype(np.float32)
+    sky = rng.uniform(10.0, 80.0, nb).astype(np.float32)
+    ref, _, _ = oracle.solve_bands(og, r, t, e, D, sky, tol_rel=1e-9)
+    init = (ref * rng.uniform(0.5, 1.5, ref.shape)).astype(np.float32)
+    f32 = eng.solveBands(r, t, e, D, sky, init=init, tolRel=1e-7, precision=FH_PRECISION_FP32)
+    b16 = eng.solveBands(r, t, e, D, sky, init=init, tolRel=1e-7, precision=FH_PRECISION_BF16_DC)
+    assert f32.iterations - 2 <= b16.iterations <= f32.iterations, (b16.iterations, f32.iterations)
+    assert _rel(b16.radiosity, f32.radiosity) < 2e-6
+    assert _rel(b16.radiosity, ref) < RTOL
+
+
 class _Hip:
     """hipMalloc / hipMemcpy / hipFree through ctypes, for the device-pointer entry points."""
 
