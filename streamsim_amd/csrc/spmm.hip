@@ -691,6 +691,17 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
     band_partials(a, myAbs, myRel, INC == 0 ? 0.0f : myAct);
 }
 
+// H = max(0, fsky S + G + D) from the accumulated gathers G of an increment
+// solve (each band's G is that of its last sweep)
+__global__ void k_incident_gathered(const float* __restrict__ G, const float* __restrict__ D,
+                                    const float* __restrict__ fsky, const float* __restrict__ skyv, uint32_t S,
+                                    uint32_t ldb, float* __restrict__ H) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(S) * ldb) return;
+    const uint32_t s = uint32_t(i / ldb), b = uint32_t(i - size_t(s) * ldb);
+    H[i] = fmaxf(fsky[s] * skyv[b] + G[i] + D[i], 0.0f);
+}
+
 // B -> BF16 copy with row stride ldh (bands >= ldb are zero)
 __global__ void k_to_bf16(const float* __restrict__ B, uint32_t S, uint32_t ldb, uint32_t ldh,
                           uint16_t* __restrict__ H) {
@@ -1114,7 +1125,9 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     }
     // incident fused into the judge sweeps: every band's H is that of its
     // last judge sweep, complete once every band converged (retirement)
-    const bool fuse = retire && o.H && o.D;
+    // (increment solves hold the gathers of every band's last sweep in gacc
+    // and derive H from them once after the solve)
+    const bool fuse = retire && o.H && o.D && !incr;
     if (fuse) {
         a.fuseH = 1u;
         a.H = o.H;
@@ -1153,6 +1166,12 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
             read_ctrl(c);
             if (c->hctrl->done) break;
             if (batch < 32u) batch *= 2u;
+        }
+        if (o.H && o.D) {
+            const size_t n = size_t(c->S) * o.ldb;
+            hipLaunchKernelGGL(k_incident_gathered, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, c->stream,
+                               c->gacc.p, o.D, c->fsky.p, c->skyv.p, c->S, o.ldb, o.H);
+            FH_HIP(hipGetLastError());
         }
     } else if (half) {
         // Phase 1: BF16 sweeps gathering BF16 copies of B (X16a = bf16(bufs[0])
@@ -1224,7 +1243,7 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
         stats->maxRelDelta = c->hctrl->lastRel;
         stats->refreshes = c->hctrl->refreshes;
     }
-    if (incidentDone) *incidentDone = fuse && c->hctrl->nAct == 0u;
+    if (incidentDone) *incidentDone = (fuse && c->hctrl->nAct == 0u) || (incr && o.H && o.D);
     return int(c->hctrl->iter & 1u);
 }
 
