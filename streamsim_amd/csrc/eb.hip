@@ -119,10 +119,16 @@ __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_
                             const uint32_t* __restrict__ order, const float4* __restrict__ nrm,
                             const float* __restrict__ sunlit, float sx, float sy, float sz,
                             const float* __restrict__ dni, float* __restrict__ r, float* __restrict__ t,
-                            float* __restrict__ D) {
+                            float* __restrict__ D, float* __restrict__ dir) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(S) * ldb) return;
     const uint32_t s = uint32_t(i / ldb), b = uint32_t(i - size_t(s) * ldb);
+    if (b == 0) {
+        const float4 n = nrm[s >> 1];
+        const float sg = (s & 1u) ? -1.0f : 1.0f;
+        const float cosv = fmaxf(0.0f, (sg * n.x) * sx + (sg * n.y) * sy + (sg * n.z) * sz);
+        dir[s] = __fmul_rn(fminf(fmaxf(sunlit[s], 0.0f), 1.0f), cosv);
+    }
     if (b >= nb) {
         r[i] = 0.0f;
         t[i] = 0.0f;
@@ -137,7 +143,7 @@ __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_
     const float4 n = nrm[s >> 1];
     const float sg = (s & 1u) ? -1.0f : 1.0f;
     const float cosv = fmaxf(0.0f, (sg * n.x) * sx + (sg * n.y) * sy + (sg * n.z) * sz);
-    D[i] = fminf(fmaxf(sunlit[s], 0.0f), 1.0f) * cosv * dni[b];
+    D[i] = __fmul_rn(__fmul_rn(fminf(fmaxf(sunlit[s], 0.0f), 1.0f), cosv), dni[b]);
 }
 
 // absorbed SW = sum_b alpha H, incident = sum_b H; FP64 block partials of
@@ -590,6 +596,25 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
         b->ensure(S);
     const size_t nsw = size_t(S) * (e.swLdb ? e.swLdb : 1u);
     for (DBuf<float>* b : {&e.Bsw0, &e.Bsw1, &e.rsw, &e.tsw, &e.Dsw, &e.csw, &e.Hsw}) b->ensure(nsw);
+    e.swDir.ensure(S);
+    if (swNb) {
+        // r, t per material in the internal band order, clamped as the
+        // per-side arrays of k_sw_optics (the band-parallel sweeps read these)
+        const uint32_t ldb = e.swLdb;
+        std::vector<float> rt(size_t(nMat) * ldb, 0.0f), tt(size_t(nMat) * ldb, 0.0f);
+        for (uint32_t m = 0; m < nMat; ++m)
+            for (uint32_t b = 0; b < swNb; ++b) {
+                const uint32_t w = e.swOrder[b];
+                const float r = std::fmin(std::fmax(refl[size_t(m) * swNb + w], 0.0f), 1.0f);
+                const float t = std::fmin(std::fmax(trans[size_t(m) * swNb + w], 0.0f), 1.0f - r);
+                rt[size_t(m) * ldb + b] = r;
+                tt[size_t(m) * ldb + b] = std::fmin(t, 1.0f - r);
+            }
+        e.swRtab.ensure(rt.size());
+        e.swTtab.ensure(tt.size());
+        FH_HIP(hipMemcpyAsync(e.swRtab.p, rt.data(), sizeof(float) * rt.size(), hipMemcpyHostToDevice, c->stream));
+        FH_HIP(hipMemcpyAsync(e.swTtab.p, tt.data(), sizeof(float) * tt.size(), hipMemcpyHostToDevice, c->stream));
+    }
     e.dni.ensure(e.swLdb ? e.swLdb : 1u);
     e.lit.ensure(S);
     e.dsum.ensure(4);
@@ -616,7 +641,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
         launch_fill_f32(c, e.dni.p, ldb, 0.0f);
         hipLaunchKernelGGL(k_sw_optics, blocks_for(nsw), dim3(256), 0, c->stream, S, swNb, ldb, e.material.p,
                            e.swRefl.p, e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, 0.0f, 1.0f, 0.0f,
-                           e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p);
+                           e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
         FH_HIP(hipGetLastError());
         e.swRho2.ensure(ldb);
         estimate_rho2(c, ldb, swNb, e.rsw.p, e.tsw.p, e.Bsw1.p, e.Hsw.p, e.csw.p, e.swRho2.p);
@@ -769,7 +794,7 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         const size_t n = size_t(S) * ldb;
         hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p, e.swRefl.p,
                            e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, p.sun.x, p.sun.y, p.sun.z,
-                           e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p);
+                           e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
         FH_HIP(hipGetLastError());
         set_sky(c, diffInt.data(), nb, ldb);
         if (!e.swRhoValid) {
@@ -788,6 +813,17 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         o.tolRel = p.radiosityTolerance;
         o.H = e.Hsw.p;
         o.D = e.Dsw.p;
+        static const bool noTables = [] {
+            const char* v = std::getenv("FH_SW_ARRAYS");
+            return v && std::atoi(v) != 0;
+        }();
+        if (!noTables) {
+            o.fmat = e.material.p;
+            o.rtab = e.swRtab.p;
+            o.ttab = e.swTtab.p;
+            o.dirf = e.swDir.p;
+            o.dniv = e.dni.p;
+        }
         static const bool noCheb = [] {
             const char* v = std::getenv("FH_NO_CHEBYSHEV");
             return v && std::atoi(v) != 0;

@@ -123,6 +123,8 @@ struct EbState {
     DBuf<uint32_t> material;       // F
     DBuf<fh_eb_material> mats;     // nMat
     DBuf<float> swRefl, swTrans;   // nMat x swNb
+    DBuf<float> swRtab, swTtab;    // nMat x swLdb, clamped, internal band order
+    DBuf<float> swDir;             // [S] clamp(sunlit, 0, 1) max(0, cos(normal, sun)) of the current node
     DBuf<double> area;             // F
     // per side
     DBuf<float> T, Tprev, absorbed, incident, sunlit, rlw, tlw, eps, elw, clw, Hlw, zero;

@@ -52,6 +52,14 @@ struct JacobiOptions {
     // squared spectral radius per band (device, ldb): plain FP32 band-parallel
     // solves then run the Chebyshev semi-iteration for [-rho_b, rho_b]
     const float* rho2 = nullptr;
+    // energy-balance shortwave solves at band-parallel strides: r, t and c
+    // from material tables and per-side direct factors (cc, rr, tt are then
+    // read by entry-parallel sweeps and the spectral estimate only)
+    const uint32_t* fmat = nullptr;  // material per facet
+    const float* rtab = nullptr;     // [materials][ldb]
+    const float* ttab = nullptr;     // [materials][ldb]
+    const float* dirf = nullptr;     // [S] clamp(sunlit, 0, 1) max(0, cos)
+    const float* dniv = nullptr;     // [ldb]
 };
 
 // Per-band spectral radius squared of the SW operator (r, t [S][ldb]) by

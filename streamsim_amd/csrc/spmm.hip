@@ -91,6 +91,14 @@ struct SweepArgs {
     uint32_t inc;
     float* gacc;
     float* carry;
+    // band-parallel sweeps of energy-balance shortwave solves: r, t from the
+    // facet's material tables and c, D from per-side direct factors instead
+    // of the [S][ldb] arrays (null fmat: the arrays)
+    const uint32_t* fmat;  // material per facet
+    const float* rtab;     // [materials][ldb] reflectance per internal band
+    const float* ttab;     // [materials][ldb] transmittance
+    const float* dirf;     // [S] clamp(sunlit, 0, 1) max(0, cos(normal, sun))
+    const float* dniv;     // [ldb] direct normal irradiance per internal band
 };
 
 template <int NBC, int TEAM, int MODE, bool W16>
@@ -312,6 +320,45 @@ __device__ __forceinline__ void arr4(const float4 v, float (&o)[4]) {
     o[3] = v.w;
 }
 
+// r, t, c and D of the side pair (s0, s0 + 1) at the bands b0 .. b0 + 3:
+// from the [S][ldb] arrays, or (fmat set) from the facet's material row and
+// the direct factors, D = dir dni and c = r (fsky_s S + D_s) + t (fsky_o S + D_o)
+// (both sides of a facet share its material).  D only when wantD.
+__device__ __forceinline__ void load_rtc(const SweepArgs& a, uint32_t s0, uint32_t b0, size_t i0, size_t i1,
+                                         bool wantD, float (&r0)[4], float (&r1)[4], float (&t0)[4], float (&t1)[4],
+                                         float (&c0)[4], float (&c1)[4], float (&D0)[4], float (&D1)[4]) {
+    if (a.fmat) {
+        const size_t m = size_t(a.fmat[s0 >> 1]) * a.ldb + b0;
+        float sk[4], dn[4];
+        arr4(ld4(a.rtab + m), r0);
+        arr4(ld4(a.ttab + m), t0);
+        arr4(ld4(a.skyv + b0), sk);
+        arr4(ld4(a.dniv + b0), dn);
+        const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1], d0 = a.dirf[s0], d1 = a.dirf[s0 + 1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            r1[j] = r0[j];
+            t1[j] = t0[j];
+            D0[j] = __fmul_rn(d0, dn[j]);
+            D1[j] = __fmul_rn(d1, dn[j]);
+            const float h0 = fmaf(f0, sk[j], D0[j]), h1 = fmaf(f1, sk[j], D1[j]);
+            c0[j] = fmaf(t0[j], h1, __fmul_rn(r0[j], h0));
+            c1[j] = fmaf(t0[j], h0, __fmul_rn(r0[j], h1));
+        }
+        return;
+    }
+    arr4(ld4(a.r + i0), r0);
+    arr4(ld4(a.r + i1), r1);
+    arr4(ld4(a.t + i0), t0);
+    arr4(ld4(a.t + i1), t1);
+    arr4(ld4(a.c + i0), c0);
+    arr4(ld4(a.c + i1), c1);
+    if (wantD) {
+        arr4(ld4(a.D + i0), D0);
+        arr4(ld4(a.D + i1), D1);
+    }
+}
+
 // Update of side pair (s0, s0 + 1) at the 4 bands b0 .. b0 + 3 (b0 a
 // multiple of 4, 16-byte row accesses) from the gathered sums: bands below
 // nAct get the Jacobi (or Chebyshev) update, bands in [nAct, nActPrev) are
@@ -352,22 +399,16 @@ __device__ __forceinline__ void band_update4(const SweepArgs& a, const BandCtx& 
     bool act[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) act[j] = b0 + uint32_t(j) < x.nAct;
-    float r0[4], r1[4], t0[4], t1[4], c0[4], c1[4], n0[4], n1[4];
-    arr4(ld4(a.r + i0), r0);
-    arr4(ld4(a.r + i1), r1);
-    arr4(ld4(a.t + i0), t0);
-    arr4(ld4(a.t + i1), t1);
-    arr4(ld4(a.c + i0), c0);
-    arr4(ld4(a.c + i1), c1);
+    const bool fuse = a.fuseH && a.judge;
+    float r0[4], r1[4], t0[4], t1[4], c0[4], c1[4], n0[4], n1[4], D0[4], D1[4];
+    load_rtc(a, s0, b0, i0, i1, fuse, r0, r1, t0, t1, c0, c1, D0, D1);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         n0[j] = c0[j] + r0[j] * g0[j] + t0[j] * g1[j];
         n1[j] = c1[j] + r1[j] * g1[j] + t1[j] * g0[j];
     }
-    if (a.fuseH && a.judge) {  // incident of the FP32 gather (judge sweeps gather FP32)
-        float D0[4], D1[4], H0[4], H1[4];
-        arr4(ld4(a.D + i0), D0);
-        arr4(ld4(a.D + i1), D1);
+    if (fuse) {  // incident of the FP32 gather (judge sweeps gather FP32)
+        float H0[4], H1[4];
         const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -490,23 +531,17 @@ __device__ __forceinline__ void band_update_inc4(const SweepArgs& a, const BandC
     bool act[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) act[j] = b0 + uint32_t(j) < x.nAct;
-    float r0[4], r1[4], t0[4], t1[4], c0[4], c1[4], n0[4], n1[4];
-    arr4(ld4(a.r + i0), r0);
-    arr4(ld4(a.r + i1), r1);
-    arr4(ld4(a.t + i0), t0);
-    arr4(ld4(a.t + i1), t1);
-    arr4(ld4(a.c + i0), c0);
-    arr4(ld4(a.c + i1), c1);
+    const bool fuse = a.fuseH && a.judge;
+    float r0[4], r1[4], t0[4], t1[4], c0[4], c1[4], n0[4], n1[4], D0[4], D1[4];
+    load_rtc(a, s0, b0, i0, i1, fuse, r0, r1, t0, t1, c0, c1, D0, D1);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         n0[j] = c0[j] + r0[j] * g0[j] + t0[j] * g1[j];
         n1[j] = c1[j] + r1[j] * g1[j] + t1[j] * g0[j];
     }
-    if (a.fuseH && a.judge) {  // incident of this sweep's gather
-        float sk[4], D0[4], D1[4], H0[4], H1[4];
+    if (fuse) {  // incident of this sweep's gather
+        float sk[4], H0[4], H1[4];
         arr4(ld4(a.skyv + b0), sk);
-        arr4(ld4(a.D + i0), D0);
-        arr4(ld4(a.D + i1), D1);
         const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -1125,6 +1160,14 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     }
     // incident fused into the judge sweeps: every band's H is that of its
     // last judge sweep, complete once every band converged (retirement)
+    if (bp && o.fmat) {
+        a.fmat = o.fmat;
+        a.rtab = o.rtab;
+        a.ttab = o.ttab;
+        a.dirf = o.dirf;
+        a.dniv = o.dniv;
+        a.skyv = c->skyv.p;
+    }
     // (increment solves hold the gathers of every band's last sweep in gacc
     // and derive H from them once after the solve)
     const bool fuse = retire && o.H && o.D && !incr;
