@@ -690,12 +690,15 @@ static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p, uint32_t pre
     o.precision = precision;
     o.maxIter = p.maxSweeps;
     o.tolRel = p.radiosityTolerance;
+    o.H = e.Hlw.p;
+    o.D = e.zero.p;
     float* bufs[2] = {e.Blw0.p, e.Blw1.p};
     fh_solve_stats st{};
-    const int fin = run_jacobi(c, o, e.clw.p, e.rlw.p, e.tlw.p, bufs, &st);
+    bool hDone = false;
+    const int fin = run_jacobi(c, o, e.clw.p, e.rlw.p, e.tlw.p, bufs, &st, &hDone);
     if (fin == 1) e.Blw0.swap(e.Blw1);
     e.lwValid = true;
-    run_incident(c, 1u, e.Blw0.p, e.zero.p, e.Hlw.p);
+    if (!hDone) run_incident(c, 1u, e.Blw0.p, e.zero.p, e.Hlw.p);
     return st.iterations;
 }
 

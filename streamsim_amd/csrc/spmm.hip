@@ -188,6 +188,13 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
                     a.c2[i0] = c0 + r0 * (g0[j] - q0[j]) + t0 * (g1[j] - q1[j]);
                     a.c2[i1] = c1 + r1 * (g1[j] - q1[j]) + t1 * (g0[j] - q0[j]);
                 }
+                if (MODE == MODE_SWEEP && a.fuseH && a.judge) {  // incident of this sweep's gather
+                    const float sk = a.skyv[b];
+                    const float h0 = a.fsky[s0] * sk + g0[j] + a.D[i0];
+                    const float h1 = a.fsky[s0 + 1] * sk + g1[j] + a.D[i1];
+                    a.H[i0] = fmaxf(h0, 0.0f);
+                    a.H[i1] = fmaxf(h1, 0.0f);
+                }
                 const float o0 = Bin[i0], o1 = Bin[i1];
                 const float x0 = o0 + a.relax * (n0 - o0);
                 const float x1 = o1 + a.relax * (n1 - o1);
@@ -1170,7 +1177,9 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     }
     // (increment solves hold the gathers of every band's last sweep in gacc
     // and derive H from them once after the solve)
-    const bool fuse = retire && o.H && o.D && !incr;
+    // (entry-parallel FP32 solves: every sweep judges, so the last one run
+    // leaves H of its gather)
+    const bool fuse = o.H && o.D && !incr && (retire || (!bp && !bf16));
     if (fuse) {
         a.fuseH = 1u;
         a.H = o.H;
@@ -1286,7 +1295,8 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
         stats->maxRelDelta = c->hctrl->lastRel;
         stats->refreshes = c->hctrl->refreshes;
     }
-    if (incidentDone) *incidentDone = (fuse && c->hctrl->nAct == 0u) || (incr && o.H && o.D);
+    if (incidentDone)
+        *incidentDone = (fuse && (!retire || c->hctrl->nAct == 0u)) || (incr && o.H && o.D);
     return int(c->hctrl->iter & 1u);
 }
 
