@@ -334,7 +334,8 @@ int fh_build_form_factors(fh_context* c, fh_graph_diagnostics* out) {
         fh::launch_compact(c, R);
         c->misc.ensure(4);
         fh::launch_fill_u32(c, c->misc.p, 4, 0u);
-        fh::launch_weights(c, c->misc.p);
+        // a count never exceeds its row's den, which never exceeds the rays of a side
+        fh::launch_weights(c, c->misc.p, R, R <= 65535u);
         uint32_t bits = 0;
         download(c, &bits, c->misc.p, 1);
         std::memcpy(&c->closure, &bits, sizeof(float));
@@ -383,7 +384,16 @@ int fh_set_graph(fh_context* c, uint32_t nnz, const uint32_t* rowOffsets, const 
         upload(c, c->sky.p, skyCounts, S);
         c->misc.ensure(4);
         fh::launch_fill_u32(c, c->misc.p, 4, 0u);
-        fh::launch_weights(c, c->misc.p);
+        uint32_t maxCount = 0;
+        bool packable = true;
+        for (uint32_t s = 0; s < S; ++s) {
+            packable = packable && denominators[s] <= 65535u;
+            for (uint32_t e = rowOffsets[s]; e < rowOffsets[s + 1u]; ++e) {
+                maxCount = std::max(maxCount, counts[e]);
+                packable = packable && counts[e] <= denominators[s];
+            }
+        }
+        fh::launch_weights(c, c->misc.p, maxCount, packable);
         uint32_t bits = 0;
         download(c, &bits, c->misc.p, 1);
         std::memcpy(&c->closure, &bits, sizeof(float));

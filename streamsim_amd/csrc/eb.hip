@@ -387,6 +387,9 @@ struct CoupledArgs {
     float* Iin;  // per side: unclamped incident fsky L + G of the gather half
     float* stepPrev;  // per side: previous full Newton step (adaptive relaxation)
     const uint32_t* sched;  // facet gathered by logical team i (null: facet i)
+    const uint32_t* pk;     // packed CSR entries (FP32 gathers), or null
+    const uint32_t* den;
+    uint32_t pkBits;
 };
 
 // Gather half of a coupled sweep: a team of TEAM lanes per facet sums
@@ -402,8 +405,12 @@ __global__ __launch_bounds__(256) void k_coupled_gather(CoupledArgs a) {
     const uint32_t s0 = 2u * f;
     const uint32_t e0 = a.rowptr[s0], e1 = a.rowptr[s0 + 1], e2 = a.rowptr[s0 + 2];
     float g[2][1] = {{0.0f}, {0.0f}};
-    team_rows_dot2<TEAM, (TEAM >= 32 ? 4 : 8), W16>(a.col, a.w32, a.w16, a.Bin, e0, e1, e1, e2, lane, g[0][0],
-                                                    g[1][0]);
+    if (!W16 && a.pk)
+        team_rows_dot2_pk<TEAM, (TEAM >= 32 ? 4 : 8)>(a.pk, a.pkBits, row_den(a.den[s0]), row_den(a.den[s0 + 1]),
+                                                      a.Bin, e0, e1, e1, e2, lane, g[0][0], g[1][0]);
+    else
+        team_rows_dot2<TEAM, (TEAM >= 32 ? 4 : 8), W16>(a.col, a.w32, a.w16, a.Bin, e0, e1, e1, e2, lane, g[0][0],
+                                                        g[1][0]);
     team_reduce<1, TEAM>(g[0]);
     team_reduce<1, TEAM>(g[1]);
     if (lane == 0) {
@@ -737,6 +744,9 @@ static uint32_t coupled_loop(fh_context* c, const fh_eb_node_params& p, const Eb
     a.partial = e.part4.p;
     a.Iin = e.Iin.p;
     a.sched = sweep_schedule(c);
+    a.pk = c->pkBits && use_packed() ? c->pk.p : nullptr;
+    a.den = c->den.p;
+    a.pkBits = c->pkBits;
     e.stepPrev.ensure(c->S);
     launch_fill_f32(c, e.stepPrev.p, c->S, 0.0f);
     a.stepPrev = e.stepPrev.p;

@@ -195,6 +195,11 @@ struct fh_context {
 
     // graph (CSR over sides)
     uint32_t nnz = 0;
+    // packed CSR entries col | cnt << pkBits (4 bytes; the weight cnt / den
+    // of the row is recomputed in the kernels), when column and count fit in
+    // 32 bits; pkBits = 0: no packed array
+    fh::DBuf<uint32_t> pk;
+    uint32_t pkBits = 0;
     float closure = 0.0f;
     fh::DBuf<uint32_t> rowptr, col, cnt, den, sky;
     fh::DBuf<uint32_t> sched;        // processing order of facets in the sweeps (Morton order)

@@ -558,10 +558,38 @@ void launch_compact(fh_context* c, uint32_t R) {
     FH_HIP(hipGetLastError());
 }
 
-void launch_weights(fh_context* c, uint32_t* closureBits) {
+// packed entries col | cnt << bits
+__global__ void k_pack(const uint32_t* __restrict__ col, const uint32_t* __restrict__ cnt, size_t n, uint32_t bits,
+                       uint32_t* __restrict__ pk) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) pk[i] = col[i] | (cnt[i] << bits);
+}
+
+static uint32_t bits_for(uint64_t v) {  // bits to hold 0 .. v
+    uint32_t b = 1;
+    while (b < 32 && (uint64_t(1) << b) <= v) ++b;
+    return b;
+}
+
+void launch_weights(fh_context* c, uint32_t* closureBits, uint32_t maxCount, bool packable) {
     hipLaunchKernelGGL(k_weights, dim3((c->S + 3u) / 4u), dim3(256), 0, c->stream, c->rowptr.p, c->cnt.p, c->den.p,
                        c->sky.p, c->S, c->val32.p, c->val16.p, c->fsky.p, closureBits);
     FH_HIP(hipGetLastError());
+    const uint32_t colBits = bits_for(c->S ? c->S - 1u : 0u), cntBits = bits_for(maxCount);
+    c->pkBits = 0;
+    if (packable && colBits + cntBits <= 32u) {
+        const size_t n = size_t(c->nnz);
+        c->pk.ensure(n + kCsrPad);
+        FH_HIP(hipMemsetAsync(c->pk.p + n, 0, kCsrPad * sizeof(uint32_t), c->stream));
+        if (n) {
+            hipLaunchKernelGGL(k_pack, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, c->stream, c->col.p, c->cnt.p,
+                               n, colBits, c->pk.p);
+            FH_HIP(hipGetLastError());
+        }
+        c->pkBits = colBits;
+    } else {
+        c->pk.release();
+    }
 }
 
 void launch_sun(fh_context* c, float sx, float sy, float sz, uint32_t* lit, float* frac) {

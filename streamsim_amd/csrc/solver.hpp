@@ -13,7 +13,10 @@ void launch_trace_graph(fh_context* c, uint32_t R, uint32_t seed, float tmin);
 void launch_rows(fh_context* c, uint32_t R);
 void launch_scan(fh_context* c, const uint32_t* in, uint32_t n, uint32_t* out);
 void launch_compact(fh_context* c, uint32_t R);
-void launch_weights(fh_context* c, uint32_t* closureBits);
+// weights cnt / den, fsky and the closure; packed entries when packable
+// (every count <= its row's den <= 65535) and column and count (<= maxCount)
+// fit in 32 bits
+void launch_weights(fh_context* c, uint32_t* closureBits, uint32_t maxCount, bool packable);
 void launch_sun(fh_context* c, float sx, float sy, float sz, uint32_t* lit, float* frac);
 
 // spmm.hip
@@ -79,6 +82,9 @@ void set_sky(fh_context* c, const float* sky, uint32_t nb, uint32_t ldb);
 // solves then gather BF16 copies of B with row stride bf16_stride(ldb))
 bool uses_bf16_gather(uint32_t ldb);
 uint32_t bf16_stride(uint32_t ldb);
+
+// whether FP32-weight gathers read the packed CSR entries (when present)
+bool use_packed();
 
 // facet processing order of the sweeps (null: natural order)
 const uint32_t* sweep_schedule(const fh_context* c);

@@ -99,6 +99,13 @@ struct SweepArgs {
     const float* ttab;     // [materials][ldb] transmittance
     const float* dirf;     // [S] clamp(sunlit, 0, 1) max(0, cos(normal, sun))
     const float* dniv;     // [ldb] direct normal irradiance per internal band
+    // packed CSR entries (col | cnt << pkBits) read instead of col / w32 by
+    // the entry-parallel FP32-weight gathers, weight cnt / den[row]; null:
+    // col / w32.  (The band-parallel kernels keep col / w32: every lane of a
+    // team would recompute the same weights.)
+    const uint32_t* pk;
+    const uint32_t* den;
+    uint32_t pkBits;
 };
 
 template <int NBC, int TEAM, int MODE, bool W16>
@@ -134,28 +141,35 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
                 if (MODE == MODE_REFRESH) {
                     team_rows_dot2<TEAM, U, false>(a.col, a.w32, a.w16, Bin, e0, e1, e1, e2, lane, g0[0], g1[0]);
                     team_rows_dot2<TEAM, U, true>(a.col, a.w32, a.w16, Bin, e0, e1, e1, e2, lane, q0[0], q1[0]);
+                } else if (!W16 && a.pk) {
+                    team_rows_dot2_pk<TEAM, U>(a.pk, a.pkBits, row_den(a.den[s0]), row_den(a.den[s0 + 1]), Bin, e0,
+                                               e1, e1, e2, lane, g0[0], g1[0]);
                 } else {
                     team_rows_dot2<TEAM, U, W16>(a.col, a.w32, a.w16, Bin, e0, e1, e1, e2, lane, g0[0], g1[0]);
                 }
             } else {
+            const bool packed = MODE != MODE_REFRESH && !W16 && a.pk;
+            const RowDen d0 = row_den(packed ? a.den[s0] : 0u), d1 = row_den(packed ? a.den[s0 + 1] : 0u);
             for (uint32_t e = e0 + lane; e < e1; e += TEAM) {
-                const uint32_t n = a.col[e];
+                const uint32_t q = packed ? a.pk[e] : 0u;
+                const uint32_t n = packed ? pk_col(q, a.pkBits) : a.col[e];
                 const float* p = Bin + size_t(n) * a.ldb + b0;
                 if (MODE == MODE_REFRESH) {
                     gather<NBC>(p, a.w32[e], g0);
                     gather<NBC>(p, bf16_to_f32(a.w16[e]), q0);
                 } else {
-                    gather<NBC>(p, W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g0);
+                    gather<NBC>(p, packed ? pk_weight(q, a.pkBits, d0) : W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g0);
                 }
             }
             for (uint32_t e = e1 + lane; e < e2; e += TEAM) {
-                const uint32_t n = a.col[e];
+                const uint32_t q = packed ? a.pk[e] : 0u;
+                const uint32_t n = packed ? pk_col(q, a.pkBits) : a.col[e];
                 const float* p = Bin + size_t(n) * a.ldb + b0;
                 if (MODE == MODE_REFRESH) {
                     gather<NBC>(p, a.w32[e], g1);
                     gather<NBC>(p, bf16_to_f32(a.w16[e]), q1);
                 } else {
-                    gather<NBC>(p, W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g1);
+                    gather<NBC>(p, packed ? pk_weight(q, a.pkBits, d1) : W16 ? bf16_to_f32(a.w16[e]) : a.w32[e], g1);
                 }
             }
             }
@@ -951,6 +965,16 @@ static bool use_sched() {
     return v;
 }
 
+// FP32-weight gathers read the packed CSR entries when the graph has them
+// (FH_CSR_PACKED=0: column and FP32 weight arrays; tuning knob)
+bool use_packed() {
+    static const bool v = [] {
+        const char* e = std::getenv("FH_CSR_PACKED");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return v;
+}
+
 // BF16 band-parallel solves gather BF16 increments (FH_INCREMENTS=0: a BF16
 // phase, then FP32 sweeps; tuning knob)
 static bool use_increments() {
@@ -966,6 +990,9 @@ const uint32_t* sweep_schedule(const fh_context* c) { return use_sched() && c->s
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.xcd = xcd_remap();
     a.sched = sweep_schedule(c);
+    a.pk = c->pkBits && use_packed() ? c->pk.p : nullptr;
+    a.den = c->den.p;
+    a.pkBits = c->pkBits;
     if (band_parallel(a.ldb)) {
         // lanes per team: one 16-byte vector of the row each, at most 64
         // bands per team pass

@@ -26,6 +26,28 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float x) {
     return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+// Packed CSR entry: column in the low `bits` bits, ray count above.  The
+// weight count / den of the row equals the FP32 weight array's correctly
+// rounded quotient: with rcp = RN(1 / den), q = RN(count rcp) and the exact
+// residual r = count - q den (one FMA), RN(q + r rcp) is the correctly
+// rounded quotient (Markstein's correction step; checked exhaustively for
+// 0 <= count <= den <= 65535, the range packed graphs are built for).
+// den 0 passes rcp 0 and gives weight 0.
+struct RowDen {
+    float den, rcp;
+};
+__device__ __forceinline__ RowDen row_den(uint32_t d) {
+    const float f = float(d);
+    return {f, d ? __frcp_rn(f) : 0.0f};
+}
+__device__ __forceinline__ uint32_t pk_col(uint32_t p, uint32_t bits) { return p & ((1u << bits) - 1u); }
+__device__ __forceinline__ float pk_weight(uint32_t p, uint32_t bits, RowDen d) {
+    const float c = float(p >> bits);
+    const float q = __fmul_rn(c, d.rcp);
+    const float r = fmaf(-q, d.den, c);
+    return fmaf(r, d.rcp, q);
+}
+
 template <int NBC>
 __device__ __forceinline__ void gather(const float* __restrict__ p, float w, float (&acc)[NBC]) {
     if constexpr (NBC == 1) {
@@ -75,6 +97,35 @@ __device__ __forceinline__ void team_rows_dot2(const uint32_t* __restrict__ col,
         for (int u = 0; u < U; ++u) {
             g0 += w0[u] * v0[u];
             g1 += w1[u] * v1[u];
+        }
+    }
+}
+
+// team_rows_dot2 over packed entries (pk_col / pk_weight with the rows'
+// denominators d0, d1)
+template <int TEAM, int U>
+__device__ __forceinline__ void team_rows_dot2_pk(const uint32_t* __restrict__ pk, uint32_t bits, RowDen d0, RowDen d1,
+                                                  const float* __restrict__ x, uint32_t b0, uint32_t e0, uint32_t b1,
+                                                  uint32_t e1, int lane, float& g0, float& g1) {
+    const uint32_t len = max(e0 - b0, e1 - b1);
+    for (uint32_t k = uint32_t(lane); k < len; k += uint32_t(TEAM * U)) {
+        uint32_t p0[U], p1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i0 = b0 + k + uint32_t(u * TEAM), i1 = b1 + k + uint32_t(u * TEAM);
+            p0[u] = i0 < e0 ? pk[i0] : 0u;
+            p1[u] = i1 < e1 ? pk[i1] : 0u;
+        }
+        float v0[U], v1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            v0[u] = x[pk_col(p0[u], bits)];
+            v1[u] = x[pk_col(p1[u], bits)];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            g0 += pk_weight(p0[u], bits, d0) * v0[u];
+            g1 += pk_weight(p1[u], bits, d1) * v1[u];
         }
     }
 }
