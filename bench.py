@@ -59,17 +59,15 @@ def parse():
     return ap.parse_args()
 
 
-def sweep_bytes(S: int, nnz: int, nb: int, w: int, bf16_copy: bool = False, increments: bool = False) -> int:
-    """Algorithmic HBM bytes of one sweep: row offsets, (col + value) per
-    entry, fsky, and per side-band c, r, t read + B read + B write; a BF16
-    sweep also reads the BF16 array it gathers once and writes the new one
-    (2 + 2 bytes per side-band); the increment sweep of BF16 solves besides
-    reads and writes the FP32 accumulated gathers and carries (16 bytes per
-    side-band)."""
-    b = 4 * (S + 1) + (4 + w) * nnz + 4 * S + 20 * S * nb
-    if increments:
-        return b + 20 * S * nb
-    return b + (4 * S * nb if bf16_copy else 0)
+def sweep_bytes(S: int, nnz: int, nb: int, entry: int, increments: bool = False) -> int:
+    """Algorithmic HBM bytes of one sweep: row offsets, `entry` bytes per CSR
+    entry (8: column + FP32 weight; 4: packed column | count), fsky, and per
+    side-band c, r, t read + B read + B write; the increment sweep of BF16
+    solves besides reads the BF16 increments it gathers once and writes the
+    next ones (2 + 2 bytes per side-band) and reads and writes the FP32
+    accumulated gathers and carries (16 bytes per side-band)."""
+    b = 4 * (S + 1) + entry * nnz + 4 * S + 20 * S * nb
+    return b + (20 * S * nb if increments else 0)
 
 
 def time_sweep(eng, nb: int, precision: int, reps: int) -> float:
@@ -208,7 +206,8 @@ def main() -> None:
         for name, nbk, prec in (("sw_fp32", nb, FH_PRECISION_FP32), ("sw_bf16", nb, FH_PRECISION_BF16_DC),
                                 ("lw_fp32", 1, FH_PRECISION_FP32)):
             sec = time_sweep(eng, nbk, prec, a.roofline_reps)
-            alg = sweep_bytes(S, diag.directedEdgeCount, nbk, 4,
+            # single-band sweeps read the packed entries
+            alg = sweep_bytes(S, diag.directedEdgeCount, nbk, 8 if nbk >= 16 else 4,
                               increments=prec != FH_PRECISION_FP32 and nbk >= 16)
             out[name] = {"us": sec * 1e6, "alg_bytes": alg, "GBps": alg / sec / 1e9}
         cs = eng.benchmarkCopy(1 << 30, 20)
@@ -254,8 +253,8 @@ def main() -> None:
     nnz = diag.directedEdgeCount
     sec16 = time_sweep(eng, nb, FH_PRECISION_BF16_DC, a.roofline_reps)
     sec32 = time_sweep(eng, nb, FH_PRECISION_FP32, a.roofline_reps)
-    alg16 = sweep_bytes(S, nnz, nb, 4, increments=True)
-    alg32 = sweep_bytes(S, nnz, nb, 4)
+    alg16 = sweep_bytes(S, nnz, nb, 8, increments=True)
+    alg32 = sweep_bytes(S, nnz, nb, 8)
     achieved = alg16 / sec16 / 1e9
     log(f"[bench] SW sweep nb={nb}: increments {sec16 * 1e6:.1f} us ({achieved:.0f} GB/s), "
         f"fp32 {sec32 * 1e6:.1f} us ({alg32 / sec32 / 1e9:.0f} GB/s)")

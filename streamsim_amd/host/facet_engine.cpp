@@ -274,6 +274,9 @@ int runFacetRT(const std::string& input, const std::string& outFile) try {
 //   FACET_EB_DIRECT_RATIO              directScatteringRatio (0.8)
 //   FACET_EB_SPECTRAL_WIDTH            spectralAccelerationWidth, 1-nm samples (100)
 //   STREAMSIM_FACET_STEP_SYNC=1        FACET_STEP / FACET_ACK back-pressure (core:3438-3441)
+// and two switches of this engine (both off: the reference's numerics):
+//   FACET_EB_ADAPTIVE_RELAXATION=1     full Newton steps, the relaxation only where a step reverses
+//   FACET_EB_BF16=1                    shortwave solves gather BF16 increments (FP32 fixed point)
 double envNumber(const char* name, double fallback) {
     const char* v = std::getenv(name);
     if (!v || !*v) return fallback;
@@ -291,6 +294,7 @@ struct EbSettings {
     uint32_t couplingIterations = 20;
     uint32_t spectralWidth = 100;
     bool saveEnergy = false, saveRadiation = false, stepSync = false;
+    bool adaptiveRelaxation = false, bf16 = false;
     std::string meteoFile;
 };
 
@@ -315,6 +319,8 @@ EbSettings ebSettings() {
     if (const char* v = std::getenv("FACET_EB_METEO")) s.meteoFile = v;
     const char* sync = std::getenv("STREAMSIM_FACET_STEP_SYNC");
     s.stepSync = sync && std::string(sync) == "1";
+    s.adaptiveRelaxation = envNumber("FACET_EB_ADAPTIVE_RELAXATION", 0.0) != 0.0;
+    s.bf16 = envNumber("FACET_EB_BF16", 0.0) != 0.0;
     return s;
 }
 
@@ -680,7 +686,8 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         p.maxCoupling = set.couplingIterations;
         p.radiosityTolerance = 1.0e-5f;
         p.maxSweeps = 2000;
-        p.precision = FH_PRECISION_FP32;
+        p.precision = set.bf16 ? FH_PRECISION_BF16_DC : FH_PRECISION_FP32;
+        p.adaptiveRelaxation = set.adaptiveRelaxation ? 1u : 0u;
         p.warmStart = FH_WARM_PREVIOUS;
         p.firstNode = node == set.startNode ? 1u : 0u;
         fh_eb_node_stats st{};
