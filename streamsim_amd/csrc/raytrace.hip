@@ -316,6 +316,113 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_sun(const BvhNode* __rest
     }
 }
 
+// ---- sun rays, one packet per wave -----------------------------------------
+// The K rays of a side are parallel and start on one facet, and the 64 / K
+// sides of a wave are neighbours: the wave walks the BVH as one packet (one
+// node stream, its stack in LDS, a child visited when the box test of any
+// searching lane passes, triangles tested by the lanes whose own box test
+// passed; a lane stops searching at its first hit).  Boxes are conservative
+// and the triangle test exact, so each ray's answer (some triangle beyond
+// tmin, own facet excluded) is the one of its own traversal.
+constexpr int kSunBlock = 256;
+
+__device__ void packet_any(const BvhNode* __restrict__ nodes, const float4* __restrict__ tris, const Ray& r,
+                           uint32_t skip, float tmin, int* stk, bool& on, bool& hit) {
+    const float limit = FLT_MAX * 1.0001f + 1.0e-6f;  // no hit yet (any-hit traversal)
+    int sp = 0;
+    int cur = 0;
+    while (true) {
+        cur = __builtin_amdgcn_readfirstlane(cur);
+        const float4 nx = nodes[cur].x, ny = nodes[cur].y, nz = nodes[cur].z;
+        const int4 meta = nodes[cur].meta;
+        float tn0, tn1;
+        bool h0 = on && (meta.z > 0 || meta.x >= 0) && slab(r, nx.x, nx.y, ny.x, ny.y, nz.x, nz.y, limit, tn0);
+        bool h1 = on && (meta.w > 0 || meta.y >= 0) && slab(r, nx.z, nx.w, ny.z, ny.w, nz.z, nz.w, limit, tn1);
+        bool any0 = __ballot(h0) != 0u, any1 = __ballot(h1) != 0u;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const bool anyc = c == 0 ? any0 : any1;
+            const int cnt = c == 0 ? meta.z : meta.w;
+            if (anyc && cnt > 0) {
+                const bool hc = c == 0 ? h0 : h1;
+                const int first = c == 0 ? meta.x : meta.y;
+                for (int i = first; i < first + cnt; ++i) {
+                    const float4 p0 = tris[3 * size_t(i)];
+                    const uint32_t f = __float_as_uint(p0.w);
+                    float t;
+                    if (hc && on && f != skip &&
+                        watertight(r, p0, tris[3 * size_t(i) + 1], tris[3 * size_t(i) + 2], tmin, t)) {
+                        hit = true;
+                        on = false;
+                    }
+                }
+                if (__ballot(on) == 0u) return;
+                if (c == 0) any0 = false; else any1 = false;
+            }
+        }
+        if (any0 && any1) {
+            stk[sp] = meta.y;
+            ++sp;
+            cur = meta.x;
+        } else if (any0) {
+            cur = meta.x;
+        } else if (any1) {
+            cur = meta.y;
+        } else {
+            if (sp == 0) break;
+            --sp;
+            cur = stk[sp];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSunBlock) void k_trace_sun_packet(const BvhNode* __restrict__ nodes,
+                                                                const float4* __restrict__ tris,
+                                                                const float* __restrict__ fverts,
+                                                                const float4* __restrict__ fnorm, uint32_t S,
+                                                                uint32_t K, uint32_t seed, float sx, float sy,
+                                                                float sz, float tmin, Periodic per,
+                                                                uint32_t* __restrict__ lit) {
+    __shared__ int stacks[kSunBlock / 64][kBvhMaxDepth];
+    int* stk = stacks[threadIdx.x >> 6];
+    const uint64_t gid = uint64_t(blockIdx.x) * kSunBlock + threadIdx.x;
+    bool on = gid < uint64_t(S) * K;
+    const uint32_t side = on ? uint32_t(gid / K) : S - 1u;
+    const uint32_t j = uint32_t(gid % K);
+    const uint32_t f = side >> 1;
+    const float4 nf = fnorm[f];
+    const float sg = (side & 1u) ? -1.0f : 1.0f;
+    const float ns[3] = {sg * nf.x, sg * nf.y, sg * nf.z};
+    on = on && ns[0] * sx + ns[1] * sy + ns[2] * sz > 0.0f;
+    const uint32_t key = side_key(seed ^ 0x5bd1e995u, side);
+    Ray r;
+    triangle_point(fverts, f, u01(ray_rand(key, j, 0u)), u01(ray_rand(key, j, 1u)), r.o);
+    r.d[0] = sx;
+    r.d[1] = sy;
+    r.d[2] = sz;
+    ray_setup(r);
+    bool hit = false;
+    const bool lane = on;  // sides facing the sun
+    for (uint32_t t = 0; t < per.tiles; ++t) {
+        if (__ballot(on) == 0u) break;
+        Ray rt = r;
+        if (t) {
+            rt.o[0] = r.o[0] - float(kTileOffset[t][0]) * per.sx;
+            rt.o[2] = r.o[2] - float(kTileOffset[t][1]) * per.sz;
+        }
+        packet_any(nodes, tris, rt, t == 0 ? f : kSky, tmin, stk, on, hit);
+    }
+    const bool sky = lane && !hit;
+    // K consecutive lanes hold the rays of one side: one add per side
+    const uint64_t m = __ballot(sky);
+    const uint32_t l = threadIdx.x & 63u, firstLane = l - l % K;
+    if (l == firstLane && gid < uint64_t(S) * K) {
+        const uint64_t bits = K == 64u ? m : (m >> firstLane) & ((uint64_t(1) << K) - 1u);
+        const uint32_t n = uint32_t(__popcll(bits));
+        if (n) atomicAdd(&lit[side], n);
+    }
+}
+
 __global__ void k_sun_fraction(const uint32_t* __restrict__ lit, uint32_t S, uint32_t K, float* __restrict__ frac) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < S) frac[s] = float(lit[s]) / float(K);
@@ -595,11 +702,21 @@ void launch_weights(fh_context* c, uint32_t* closureBits, uint32_t maxCount, boo
 void launch_sun(fh_context* c, float sx, float sy, float sz, uint32_t* lit, float* frac) {
     const uint32_t K = c->cfg.sunRaysPerSide;
     const uint64_t n = uint64_t(c->S) * K;
-    const uint64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL(k_trace_sun, dim3(uint32_t(blocks)), dim3(kTraceBlock), stack_bytes(c), c->stream, c->nodes.p,
-                       c->tris.p,
-                       c->facetVerts.p, c->facetNormal.p, c->S, K, c->cfg.seed, sx, sy, sz, c->cfg.rayEpsilon,
-                       periodic(c), lit);
+    static const bool packets = [] {
+        const char* e = std::getenv("FH_SUN_PACKETS");  // 0: one traversal per ray (tuning knob)
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    if (packets && 64u % K == 0u && c->bvhDepth < kBvhMaxDepth) {
+        const uint64_t blocks = (n + kSunBlock - 1) / kSunBlock;
+        hipLaunchKernelGGL(k_trace_sun_packet, dim3(uint32_t(blocks)), dim3(kSunBlock), 0, c->stream, c->nodes.p,
+                           c->tris.p, c->facetVerts.p, c->facetNormal.p, c->S, K, c->cfg.seed, sx, sy, sz,
+                           c->cfg.rayEpsilon, periodic(c), lit);
+    } else {
+        const uint64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
+        hipLaunchKernelGGL(k_trace_sun, dim3(uint32_t(blocks)), dim3(kTraceBlock), stack_bytes(c), c->stream,
+                           c->nodes.p, c->tris.p, c->facetVerts.p, c->facetNormal.p, c->S, K, c->cfg.seed, sx, sy, sz,
+                           c->cfg.rayEpsilon, periodic(c), lit);
+    }
     hipLaunchKernelGGL(k_sun_fraction, dim3((c->S + 255u) / 256u), dim3(256), 0, c->stream, lit, c->S, K, frac);
     FH_HIP(hipGetLastError());
 }
