@@ -251,7 +251,8 @@ typedef struct {
     uint32_t vegetationMethod; /* 0 | 1                                    */
     float radiosityTolerance;  /* relative change, SW and LW solves        */
     uint32_t maxSweeps;        /* per radiosity solve                      */
-    uint32_t precision;        /* fh_precision of the SW and LW solves     */
+    uint32_t precision;        /* fh_precision of the SW solves from 16 windows on (BF16
+                                  increments); below 16 windows and for the LW: FP32 */
     uint32_t warmStart;        /* fh_warm_start                            */
     uint32_t firstNode;        /* 1: T = airTemperatureK (core:3470-3479)  */
     uint32_t zeroStart;        /* 1: every solve starts from B = 0         */

@@ -821,7 +821,10 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         if (!e.swValid || p.zeroStart) launch_fill_f32(c, e.Bsw0.p, n, 0.0f);
         JacobiOptions o;
         o.ldb = ldb;
-        o.precision = p.precision;
+        // BF16 gathers only where they save lines: below 16 windows a side's
+        // FP32 row is at most one 64-byte gather, and the entry-parallel
+        // BF16 scheme (defect correction) takes more sweeps than FP32
+        o.precision = uses_bf16_gather(ldb) ? p.precision : FH_PRECISION_FP32;
         o.maxIter = p.maxSweeps;
         o.tolRel = p.radiosityTolerance;
         o.H = e.Hsw.p;

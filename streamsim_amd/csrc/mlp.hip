@@ -26,6 +26,16 @@ __device__ __forceinline__ float gelu_f(float x) {
     const float k = 0.7978845608028654f;
     return 0.5f * x * (1.0f + tanhf(k * (x + 0.044715f * x * x * x)));
 }
+// GELU for inference on the matrix-core path: the same tanh form with
+// tanh(u) = 1 - 2 / (1 + e^(2u)) on the hardware exp2 and reciprocal
+// (about 1e-6 relative; the model is a warm-start predictor)
+__device__ __forceinline__ float gelu_fast(float x) {
+    const float k = 0.7978845608028654f;
+    const float u = k * (x + 0.044715f * x * x * x);
+    const float e = __builtin_amdgcn_exp2f(fminf(2.8853900817779268f * u, 126.0f));  // e^(2u)
+    const float th = 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + e);
+    return 0.5f * x * (1.0f + th);
+}
 __device__ __forceinline__ float gelu_d(float x) {
     const float k = 0.7978845608028654f;
     const float th = tanhf(k * (x + 0.044715f * x * x * x));
@@ -219,28 +229,42 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const float* __restrict__ X,
     const float* W3 = b2 + H;
     const float b3 = W3[H];
     const uint32_t u0 = blockIdx.x * TU;
-    for (uint32_t i = threadIdx.x; i < uint32_t(TU * H); i += 256u) {
-        const uint32_t u = i / H, j = i - u * H, unit = u0 + u;
-        float a = b1[j];
-        if (unit < n) {
+    // layer 1: thread t computes hidden unit j = t % 128 of units t / 128,
+    // t / 128 + 2, ...: its row of W1 stays in registers
+    {
+        const uint32_t j = threadIdx.x & uint32_t(H - 1);
+        float w1[kIn];
 #pragma unroll
-            for (int q = 0; q < kIn; ++q) a += W1[j * kIn + q] * X[size_t(unit) * kIn + q];
+        for (int q = 0; q < kIn; ++q) w1[q] = W1[j * kIn + q];
+        const float b1j = b1[j];
+        for (uint32_t u = threadIdx.x / uint32_t(H); u < uint32_t(TU); u += 256u / uint32_t(H)) {
+            const uint32_t unit = u0 + u;
+            float a = b1j;
+            if (unit < n) {
+#pragma unroll
+                for (int q = 0; q < kIn; ++q) a += w1[q] * X[size_t(unit) * kIn + q];
+            }
+            sH1[u][j] = unit < n ? gelu_fast(a) : 0.0f;
         }
-        sH1[u][j] = unit < n ? gelu_f(a) : 0.0f;
     }
-    __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 31, lk = lane >> 5;
+    // this lane's B operands for all 64 k-steps, loaded before the matrix
+    // loop so that no global load latency sits between the MFMAs
+    float bv[H / 2];
+#pragma unroll
+    for (int k = 0; k < H / 2; ++k) bv[k] = W2T[(2 * k + lk) * H + w * 32 + li];
+    __syncthreads();
     f32x16 acc[RT];
 #pragma unroll
     for (int t = 0; t < RT; ++t)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[t][r] = 0.0f;
-    for (int k = 0; k < H; k += 2) {
-        const float bv = W2T[(k + lk) * H + w * 32 + li];
+#pragma unroll 8
+    for (int k = 0; k < H / 2; ++k) {
 #pragma unroll
         for (int t = 0; t < RT; ++t)
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(sH1[t * 32 + li][k + lk], bv, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(sH1[t * 32 + li][2 * k + lk], bv[k], acc[t], 0, 0, 0);
     }
     const int j = w * 32 + li;
     const float bj = b2[j], wj = W3[j];
@@ -248,7 +272,7 @@ __global__ __launch_bounds__(256) void k_infer_mfma(const float* __restrict__ X,
     for (int t = 0; t < RT; ++t) {
         float part[16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) part[r] = wj * gelu_f(acc[t][r] + bj);
+        for (int r = 0; r < 16; ++r) part[r] = wj * gelu_fast(acc[t][r] + bj);
 #pragma unroll
         for (int r = 0; r < 16; ++r)
 #pragma unroll

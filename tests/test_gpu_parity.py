@@ -64,15 +64,18 @@ def test_form_factor_graph_forest_crown(oracle):
     assert np.array_equal(g.skyCounts, og.skyCounts)
 
 
-def test_sunlit_bitexact(oracle):
-    print("sunlit case", flush=True)
+# 32 and 16 rays per side: one packet of 2 or 4 sides per wave; 24: one
+# traversal per ray (24 does not divide the wave)
+@pytest.mark.parametrize("K", [32, 16, 24])
+def test_sunlit_bitexact(oracle, K):
+    print(f"sunlit case K={K}", flush=True)
     sc = scenes.row_crop(leaves_per_row=60, cells=10)
-    eng = _engine(R=32, K=32, seed=9)
+    eng = _engine(R=32, K=K, seed=9)
     eng.setGeometry(sc.vertices)
-    for zen, az in ((30.0, 135.0), (60.0, 250.0), (0.0, 0.0)):
+    for zen, az in ((30.0, 135.0), (60.0, 250.0), (0.0, 0.0), (85.0, 20.0)):
         sun = oracle.sun_direction(zen, az)
         f = eng.computeSunlitFraction(sun)
-        of = oracle.ray_sunlit(sc.vertices, sun, 9, 32)
+        of = oracle.ray_sunlit(sc.vertices, sun, 9, K)
         assert np.array_equal(f, of), (zen, az)
 
 
