@@ -34,44 +34,20 @@
 
 #include "facet_backend.hpp"
 #include "facet_hip.h"
+#include "project.hpp"
+#include "scene.hpp"
 
 namespace {
 
-using Vec3 = std::array<float, 3>;
+using facetcli::appendTriangle;
+using facetcli::Scene;
+using facetcli::Vec3;
 using Clock = std::chrono::steady_clock;
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kBeam = 700.0f;
 constexpr float kSky = 100.0f;
 constexpr uint32_t kSoilGrid = 48;
-
-struct Scene {
-    std::vector<facethip::Vertex> vertices;
-    std::vector<uint32_t> material;
-    uint32_t leafFacetCount = 0;
-    float extentX = 1.0f, extentZ = 1.0f;  // soil grid extent (the periodic tile)
-    uint32_t facetCount() const { return uint32_t(vertices.size() / 3); }
-};
-
-// appendTriangle (core:288-306): normal of cross(b - a, c - a) in float
-bool appendTriangle(Scene& s, const Vec3& a, const Vec3& b, const Vec3& c, uint32_t mat) {
-    const Vec3 u{b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-    const Vec3 v{c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-    const Vec3 n{u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
-    const float dd = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-    if (dd <= 1.0e-16f) return false;
-    const float inv = 1.0f / std::sqrt(dd);
-    for (const Vec3* p : {&a, &b, &c}) {
-        facethip::Vertex vx;
-        for (int k = 0; k < 3; ++k) {
-            vx.position[k] = (*p)[k];
-            vx.normal[k] = n[k] * inv;
-        }
-        s.vertices.push_back(vx);
-    }
-    s.material.push_back(mat);
-    return true;
-}
 
 // loadTreeWithSoil (core:362-473) for an OBJ without a terrain object
 Scene loadObj(const std::string& path) {
@@ -189,12 +165,14 @@ std::string outputPath(const std::string& input, const std::string& requested, c
 // periodic copies of the scene (0..20, the reference's periodicNeighborCount)
 // with the soil grid's extent as the period (periodicSizeX/Z = scene width /
 // depth, core:2890-2894)
-facethip::Config engineConfig(const Scene& s) {
+// (a project's periodicNeighborCount is the default of the variable)
+facethip::Config engineConfig(const Scene& s, uint32_t periodic = 0) {
     facethip::Config c;
     if (const char* e = std::getenv("FACET_RAYS_PER_SIDE")) c.raysPerSide = uint32_t(std::atoi(e));
     if (const char* e = std::getenv("FACET_SUN_RAYS_PER_SIDE")) c.sunRaysPerSide = uint32_t(std::atoi(e));
-    if (const char* e = std::getenv("FACET_PERIODIC_NEIGHBORS")) {
-        c.periodicNeighborCount = uint32_t(std::clamp(std::atoi(e), 0, 20));
+    if (const char* e = std::getenv("FACET_PERIODIC_NEIGHBORS")) periodic = uint32_t(std::clamp(std::atoi(e), 0, 20));
+    if (periodic) {
+        c.periodicNeighborCount = periodic;
         c.periodicSizeX = s.extentX;
         c.periodicSizeZ = s.extentZ;
     }
@@ -295,8 +273,42 @@ struct EbSettings {
     uint32_t spectralWidth = 100;
     bool saveEnergy = false, saveRadiation = false, stepSync = false;
     bool adaptiveRelaxation = false, bf16 = false;
+    int vegetationMethod = 0;
     std::string meteoFile;
 };
+
+int envSwitches(EbSettings& s);
+
+// the settings a project.json gives (facetInputView's tags); step sync and
+// this engine's switches from the environment as for OBJ input
+EbSettings projectSettings(const facetcli::ProjectRun& r) {
+    EbSettings s;
+    s.startNode = r.startNode;
+    s.endNode = r.endNode;
+    s.dTime = r.dTime;
+    s.latitude = r.latitude;
+    s.longitude = r.longitude;
+    s.skyTemperature = r.skyTemperature;
+    s.directRatio = r.directRatio;
+    s.tolerance = r.tolerance;
+    s.relaxation = r.relaxation;
+    s.couplingIterations = r.couplingIterations;
+    s.spectralWidth = r.spectralWidth;
+    s.saveEnergy = r.saveEnergy;
+    s.saveRadiation = r.saveRadiation;
+    s.meteoFile = r.meteoFile;
+    s.vegetationMethod = r.vegetationMethod;
+    envSwitches(s);
+    return s;
+}
+
+int envSwitches(EbSettings& s) {
+    const char* sync = std::getenv("STREAMSIM_FACET_STEP_SYNC");
+    s.stepSync = sync && std::string(sync) == "1";
+    s.adaptiveRelaxation = envNumber("FACET_EB_ADAPTIVE_RELAXATION", 0.0) != 0.0;
+    s.bf16 = envNumber("FACET_EB_BF16", 0.0) != 0.0;
+    return 0;
+}
 
 EbSettings ebSettings() {
     EbSettings s;
@@ -317,10 +329,7 @@ EbSettings ebSettings() {
         s.saveRadiation = p == "radiation" || p == "both";
     }
     if (const char* v = std::getenv("FACET_EB_METEO")) s.meteoFile = v;
-    const char* sync = std::getenv("STREAMSIM_FACET_STEP_SYNC");
-    s.stepSync = sync && std::string(sync) == "1";
-    s.adaptiveRelaxation = envNumber("FACET_EB_ADAPTIVE_RELAXATION", 0.0) != 0.0;
-    s.bf16 = envNumber("FACET_EB_BF16", 0.0) != 0.0;
+    envSwitches(s);
     return s;
 }
 
@@ -560,9 +569,21 @@ void writeProcess(const std::filesystem::path& dir, const std::string& type, int
     index.products.push_back("process/" + stem + ".json");
 }
 
+bool isProject(const std::string& input) {
+    std::string e = std::filesystem::path(input).extension().string();
+    for (char& c : e) c = char(std::tolower(static_cast<unsigned char>(c)));
+    return e == ".json";
+}
+
 int runFacetEB(const std::string& input, const std::string& outFile) try {
-    const EbSettings set = ebSettings();
-    const Scene scene = loadObj(input);
+    // project.json: scene, materials and settings from the project; OBJ:
+    // loadTreeWithSoil's scene with the default materials, settings from
+    // the environment
+    const bool project = isProject(input);
+    facetcli::Project prj;
+    if (project) prj = facetcli::loadProject(input);
+    const EbSettings set = project ? projectSettings(prj.run) : ebSettings();
+    const Scene scene = project ? prj.scene : loadObj(input);
     const std::vector<MeteoStep> meteo = loadMeteorology(set);
     const std::string result = outputPath(input, outFile, "faceteb.json");
     const std::filesystem::path outDir = std::filesystem::path(result).parent_path().empty()
@@ -581,7 +602,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
     std::cout << "PROGRESS\t3\tLoading the facet scene and meteorology" << std::endl;
 
     facethip::FacetrtHip model;
-    model.initialize(engineConfig(scene));
+    model.initialize(engineConfig(scene, project ? prj.run.periodicNeighbors : 0u));
     model.setGeometry(scene.vertices);
     const facethip::GraphDiagnostics graph = model.buildVisibilityGraph({facethip::Direction{}});
     fh_context* ctx = model.handle();
@@ -611,6 +632,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
     const float dLevel = 0.5f * set.directRatio, qLevel = 0.5f * (1.0f - set.directRatio);
     const auto integral = [](float level, uint32_t a, uint32_t b) { return float(double(level) * (b - a) * 0.001); };
     const float directIntegral = integral(dLevel, 0, kSamples - 1);
+    std::vector<fh_eb_material> matList(mats, mats + 2);
     std::vector<float> swR(2 * size_t(nw)), swT(2 * size_t(nw));
     for (uint32_t w = 0; w < nw; ++w) {
         swR[w] = 0.10f;
@@ -618,15 +640,46 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         swR[nw + w] = 0.20f;
         swT[nw + w] = 0.0f;
     }
-    facethip::check(
-        fh_eb_set_scene(ctx, scene.leafFacetCount, scene.material.data(), 2, mats, nw, swR.data(), swT.data()), ctx);
+    if (project) {
+        // a project's materials: objects' meshes, then the ground; window r,
+        // t at the window's centre sample (core:3510-3529, 952-1003)
+        const size_t nm = prj.materials.size();
+        matList.assign(nm, fh_eb_material{});
+        swR.assign(nm * nw, 0.0f);
+        swT.assign(nm * nw, 0.0f);
+        for (size_t m = 0; m < nm; ++m) {
+            const facetcli::ProjectMaterial& pm = prj.materials[m];
+            fh_eb_material& e = matList[m];
+            e.tirReflectance = pm.tirR;
+            e.tirTransmittance = pm.tirT;
+            e.surfaceResistance = pm.rss;
+            e.specificHeat = pm.cs;
+            e.density = pm.rhos;
+            e.thermalConductivity = pm.lambdas;
+            e.soilMoisture = pm.smc;
+            e.heatCapacityPerArea = pm.heatCapacity;
+            e.convectiveScale = pm.convectiveScale;
+            e.soilTemperatureMethod = pm.soilMethod;
+            e.vegetation = pm.vegetation ? 1 : 0;
+            const bool leaf = m + 1 < nm;  // the last material is the ground's
+            for (uint32_t w = 0; w < nw; ++w) {
+                const size_t centre = (size_t(windows[w].first) + windows[w].second) / 2u;
+                const float r = pm.swR(centre, leaf);
+                swR[m * nw + w] = r;
+                swT[m * nw + w] = pm.swT(centre, leaf, r);
+            }
+        }
+    }
+    facethip::check(fh_eb_set_scene(ctx, scene.leafFacetCount, scene.material.data(), uint32_t(matList.size()),
+                                    matList.data(), nw, swR.data(), swT.data()),
+                    ctx);
 
     const std::filesystem::path stepDir = outDir / ".facet_steps";
     std::filesystem::create_directories(stepDir);
     {  // components.bin (core:3418-3429): 1 for ground, the object's output component (0) otherwise
         std::ofstream comp(stepDir / "components.bin", std::ios::binary | std::ios::trunc);
         for (uint32_t f = 0; f < scene.facetCount(); ++f) {
-            const uint8_t c = f >= scene.leafFacetCount ? 1 : 0;
+            const uint8_t c = project ? prj.facetComponent[f] : f >= scene.leafFacetCount ? 1 : 0;
             comp.write(reinterpret_cast<const char*>(&c), 1);
         }
         if (!comp) throw std::runtime_error("FacetEB cannot write component identities");
@@ -688,6 +741,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         p.maxSweeps = 2000;
         p.precision = set.bf16 ? FH_PRECISION_BF16_DC : FH_PRECISION_FP32;
         p.adaptiveRelaxation = set.adaptiveRelaxation ? 1u : 0u;
+        p.vegetationMethod = uint32_t(std::clamp(set.vegetationMethod, 0, 1));
         p.warmStart = FH_WARM_PREVIOUS;
         p.firstNode = node == set.startNode ? 1u : 0u;
         fh_eb_node_stats st{};
@@ -810,6 +864,23 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
 }
 
 // --solar-position Y M D h m s timezone longitude latitude pressure temperature
+// facet_engine --scene-obj project.json scene.obj: the assembled facet scene
+// of a project (object facets, then the ground) as OBJ
+int runSceneObj(int argc, char** argv) try {
+    if (argc < 4) {
+        std::cerr << "Usage: facet_engine --scene-obj project.json scene.obj\n";
+        return 2;
+    }
+    const facetcli::Project prj = facetcli::loadProject(argv[2]);
+    facetcli::writeSceneObj(prj.scene, argv[3]);
+    std::cout << "facets " << prj.scene.facetCount() << " objects " << prj.scene.leafFacetCount << " materials "
+              << prj.materials.size() << std::endl;
+    return 0;
+} catch (const std::exception& e) {
+    std::cerr << "facet_engine: " << e.what() << '\n';
+    return 1;
+}
+
 int runSolarPosition(int argc, char** argv) {
     if (argc != 13) {
         std::cerr << "Usage: facet_engine --solar-position Y M D h m s tz lon lat pressure temperature\n";
@@ -828,6 +899,7 @@ int runSolarPosition(int argc, char** argv) {
 int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "";
     if (mode == "--solar-position") return runSolarPosition(argc, argv);
+    if (mode == "--scene-obj") return runSceneObj(argc, argv);
     const std::string input = argc > 2 ? argv[2] : "";
     const std::string output = argc > 3 ? argv[3] : "";
     if (mode != "eFacetRT" && mode != "eFacetEB") {
@@ -839,9 +911,12 @@ int main(int argc, char** argv) {
         return 1;
     }
     const std::string ext = std::filesystem::path(input).extension().string();
-    if (ext != ".obj" && ext != ".OBJ") {
-        std::cerr << (mode == "eFacetRT" ? "facetrt" : "faceteb")
-                  << ": only OBJ scenes are read by this engine (project.json is not supported)\n";
+    if (mode == "eFacetRT" && isProject(input)) {
+        std::cerr << "facetrt: project.json scenes run in eFacetEB here; eFacetRT reads OBJ scenes\n";
+        return 1;
+    }
+    if (ext != ".obj" && ext != ".OBJ" && !isProject(input)) {
+        std::cerr << (mode == "eFacetRT" ? "facetrt" : "faceteb") << ": the input must be an OBJ scene or project.json\n";
         return 1;
     }
     return mode == "eFacetRT" ? runFacetRT(input, output) : runFacetEB(input, output);

@@ -182,3 +182,46 @@ def test_cli_faceteb_nodes_step_outputs_and_backpressure(tmp_path):
     idx = json.loads((tmp_path / "faceteb_run.json").read_text())
     assert idx["status"] == "failed"
 
+
+
+@pytest.mark.gpu
+def test_cli_faceteb_project(tmp_path):
+    # eFacetEB on a project.json: the reference's validation project (trees
+    # of a Vegetation object, a Building object, soil ground) with a
+    # meteorology block added; materials, components and settings come from
+    # the project (control, meteo), not from the environment
+    print("golden cli eb project", flush=True)
+    import os
+    import shutil
+    exe = Path(__file__).resolve().parents[1] / "streamsim_amd" / "facet_engine"
+    src = HERE / "urban_tree_project"
+    prj = tmp_path / "prj"
+    shutil.copytree(src, prj)
+    (prj / "meteo.txt").write_text("julian Tair e P u SW LW\n172.5 27 16 1000 2 850 360\n172.875 21 15 1000 1 0 340\n")
+    d = json.loads((src / "project.json").read_text())
+    c = d["configuration"]
+    c["meteo"] = {"path": "meteo.txt", "start": 0, "end": 2, "dTime": 3600, "latitude": 40, "longitude": 116}
+    c.setdefault("control", {}).update({"temperatureTolerance": 0.01, "couplingIterations": 60,
+                                        "spectralAccelerationWidth": 250})
+    (prj / "project.json").write_text(json.dumps(d))
+    env = dict(os.environ, FACET_RAYS_PER_SIDE="64", FACET_SUN_RAYS_PER_SIDE="16")
+    for k in ("FACET_EB_METEO", "FACET_EB_START", "FACET_EB_END", "STREAMSIM_FACET_STEP_SYNC"):
+        env.pop(k, None)
+    out = tmp_path / "eb" / "faceteb.json"
+    r = subprocess.run([str(exe), "eFacetEB", str(prj / "project.json"), str(out)], env=env, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    conv = [int(l.split("\t")[2]) for l in r.stdout.splitlines() if l.startswith("FACET_CONVERGENCE")]
+    assert conv == [1, 1]
+    res = json.loads(out.read_text())
+    assert res["facetCount"] == 4138 and res["leafFacetCount"] == 138
+    comp = np.fromfile(tmp_path / "eb" / ".facet_steps" / "components.bin", np.uint8)
+    # 7 trees of 18 facets (Vegetation: 2), the building's 12 (Building: 3), ground 1
+    assert np.all(comp[:126] == 2) and np.all(comp[126:138] == 3) and np.all(comp[138:] == 1)
+    noon = np.fromfile(tmp_path / "eb" / ".facet_steps" / "energy_T=DOY172_12-00.bin", np.float32).reshape(-1, 9)
+    T = noon[:, 0]
+    assert np.all((T > 220) & (T < 360))
+    # latent heat: the leaves (leaf_c3, Vegetation) transpire, the building
+    # (soil_dry, not Vegetation) does not
+    LE = noon[:, 6]
+    assert np.all(LE[:2 * 126].reshape(-1, 2)[:, 0] > 0) and np.all(LE[2 * 126:2 * 138] == 0)
