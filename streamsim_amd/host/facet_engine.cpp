@@ -179,7 +179,11 @@ facethip::Config engineConfig(const Scene& s, uint32_t periodic = 0) {
     return c;
 }
 
+bool isProject(const std::string& input);
+int runFacetRTProject(const std::string& input, const std::string& outFile);
+
 int runFacetRT(const std::string& input, const std::string& outFile) try {
+    if (isProject(input)) return runFacetRTProject(input, outFile);
     const std::string result = outputPath(input, outFile, "facetrt.json");
     const Scene scene = loadObj(input);
     const auto t0 = Clock::now();
@@ -229,6 +233,151 @@ int runFacetRT(const std::string& input, const std::string& outFile) try {
     return 0;
 } catch (const std::exception& e) {
     std::cerr << "facetrt: " << e.what() << '\n';
+    return 1;
+}
+
+// planckRadiance (core:524-536): W m-2 sr-1 um-1
+float planckRadiance(float wavelengthNm, float temperature) {
+    constexpr double c1 = 1.1910439340652e8, c2 = 14388.291040407;
+    const double um = wavelengthNm > 50.0f ? double(wavelengthNm) / 1000.0 : double(wavelengthNm);
+    const double x = c2 / (std::max(1.0, double(temperature)) * std::max(1.0e-6, um));
+    return float(c1 / (std::pow(um, 5.0) * std::expm1(x)));
+}
+
+// eFacetRT on a project.json (runGpu's band loop, core:1672-1774): per
+// sensor band the optics of makeSpectralOptics (core:1057-1126; Lambertian
+// background, no texture): optical bands r, t of the band, D = sunlit
+// max(0, n.s) 700, sky 50; thermal bands (> 2500 nm) r, t = ref_TIR,
+// tau_TIR, emission e = eps (f Planck(T_sunlit) + (1 - f) Planck(T_shaded)),
+// sky Planck(T_sky).  All bands in one 64-sweep multi-band solve (the
+// reference's kIterations sweeps per band); optical results as apparent
+// reflectance against 700 cos(zenith) + 50, clamped to [0, 1]
+int runFacetRTProject(const std::string& input, const std::string& outFile) try {
+    const facetcli::Project prj = facetcli::loadProject(input);
+    const Scene& scene = prj.scene;
+    const std::vector<float>& bands = prj.run.bands;
+    const uint32_t nb = uint32_t(bands.size());
+    const std::string result = outputPath(input, outFile, "facetrt.json");
+    const auto t0 = Clock::now();
+    std::cout << "PROGRESS\t3\tCreating HIP device context" << std::endl;
+    facethip::FacetrtHip model;
+    model.initialize(engineConfig(scene, prj.run.periodicNeighbors));
+    model.setGeometry(scene.vertices);
+    std::cout << "PROGRESS\t10\tRay-cast form factors" << std::endl;
+    const auto tv = Clock::now();
+    const facethip::GraphDiagnostics graph = model.buildVisibilityGraph({facethip::Direction{}});
+    const auto tv1 = Clock::now();
+    const facethip::Direction sun = sunDirection(prj.run.sunZenith, prj.run.sunAzimuth);
+    const std::vector<float> sunlit = model.computeSunlitFraction(sun);
+    const auto ts1 = Clock::now();
+    const size_t S = size_t(scene.facetCount()) * 2u;
+    std::vector<float> r(S * nb), t(S * nb), e(S * nb, 0.0f), D(S * nb, 0.0f), sky(nb);
+    for (uint32_t b = 0; b < nb; ++b)
+        sky[b] = bands[b] > 2500.0f ? planckRadiance(bands[b], prj.run.skyTemperature) : kSky * 0.5f;
+    for (uint32_t f = 0; f < scene.facetCount(); ++f) {
+        const facetcli::ProjectMaterial& m = prj.materials[scene.material[f]];
+        const bool object = f < scene.leafFacetCount;
+        const auto& vx = scene.vertices[3 * size_t(f)];
+        for (uint32_t ls = 0; ls < 2; ++ls) {
+            const size_t side = 2 * size_t(f) + ls;
+            const float sg = ls == 0 ? 1.0f : -1.0f;
+            const float cosv =
+                std::max(0.0f, sg * vx.normal[0] * sun.x + sg * vx.normal[1] * sun.y + sg * vx.normal[2] * sun.z);
+            for (uint32_t b = 0; b < nb; ++b) {
+                const size_t i = side * nb + b;
+                const bool thermal = bands[b] > 2500.0f;
+                float refl, trans;
+                if (thermal) {
+                    refl = std::clamp(m.tirR, 0.0f, 1.0f);
+                    trans = std::clamp(m.tirT, 0.0f, 1.0f - refl);
+                } else {
+                    const float rb = m.bandR.empty() ? m.r : m.bandR[std::min<size_t>(b, m.bandR.size() - 1)];
+                    const float tb = m.bandT.empty() ? m.t : m.bandT[std::min<size_t>(b, m.bandT.size() - 1)];
+                    refl = std::clamp(rb, 0.0f, 1.0f);
+                    trans = std::clamp(tb, 0.0f, 1.0f - refl);
+                }
+                const float eps = std::max(0.0f, 1.0f - refl - trans);
+                if (!object && thermal) refl = std::clamp(1.0f - trans - eps, 0.0f, 1.0f);
+                r[i] = refl;
+                t[i] = std::min(trans, 1.0f - refl);
+                if (thermal) {
+                    const float ill = std::clamp(sunlit[side], 0.0f, 1.0f);
+                    e[i] = eps * (ill * planckRadiance(bands[b], m.sunlitT) +
+                                  (1.0f - ill) * planckRadiance(bands[b], m.shadedT));
+                } else {
+                    D[i] = sunlit[side] * cosv * kBeam;
+                }
+            }
+        }
+    }
+    std::cout << "PROGRESS\t82\tHIP Jacobi radiosity: " << nb << " bands, 64 iterations" << std::endl;
+    fh_context* ctx = model.handle();
+    std::vector<float> out(S * nb);
+    fh_solve_stats st{};
+    facethip::check(fh_solve_bands(ctx, nb, r.data(), t.data(), e.data(), D.data(), sky.data(), nullptr, 64u, -1.0f,
+                                   1.0f, FH_PRECISION_FP32, out.data(), &st),
+                    ctx);
+    const float ref = kBeam * std::max(0.0f, sun.y) + kSky * 0.5f;
+    const auto toReflectance = [&](float v) { return std::clamp(v / ref, 0.0f, 1.0f); };
+    bool anyThermal = false, allThermal = true;
+    std::vector<float> band0(S), enh(S, 0.0f), bandRad;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const bool thermal = bands[b] > 2500.0f;
+        anyThermal = anyThermal || thermal;
+        allThermal = allThermal && thermal;
+    }
+    if (nb > 1) bandRad.resize(S * nb);
+    for (uint32_t b = 0; b < nb; ++b) {
+        const bool conv = bands[b] <= 2500.0f && ref > 1.0e-8f;
+        for (size_t s = 0; s < S; ++s) {
+            float v = out[s * nb + b];
+            if (conv) v = toReflectance(v);
+            if (b == 0) band0[s] = v;
+            if (nb > 1) bandRad[size_t(b) * S + s] = v;
+        }
+    }
+    if (bands[0] <= 2500.0f) {  // light enhancement of the first band (a diffuse-only solve)
+        std::vector<float> r0(S), t0(S), e0(S), D0(S, 0.0f), d(S);
+        for (size_t s = 0; s < S; ++s) {
+            r0[s] = r[s * nb];
+            t0[s] = t[s * nb];
+            e0[s] = e[s * nb];
+        }
+        fh_solve_stats sd{};
+        facethip::check(fh_solve_bands(ctx, 1u, r0.data(), t0.data(), e0.data(), D0.data(), sky.data(), nullptr, 64u,
+                                       -1.0f, 1.0f, FH_PRECISION_FP32, d.data(), &sd),
+                        ctx);
+        for (size_t s = 0; s < S; ++s) enh[s] = std::max(0.0f, band0[s] - (ref > 1.0e-8f ? toReflectance(d[s]) : d[s]));
+    }
+    const auto t1 = Clock::now();
+    std::filesystem::create_directories(std::filesystem::path(result).parent_path().empty()
+                                            ? std::filesystem::path(".")
+                                            : std::filesystem::path(result).parent_path());
+    std::ofstream o(result, std::ios::binary);
+    if (!o) throw std::runtime_error("Cannot write web result: " + result);
+    writeHeader(o, "HIP GPU", scene);
+    o << "  \"quantity\": \""
+      << (allThermal ? "spectral radiance" : !anyThermal ? "reflectance" : "reflectance and spectral radiance")
+      << "\",\n  \"units\": \"" << (allThermal ? "W m-2 sr-1 um-1" : !anyThermal ? "1" : "band-dependent")
+      << "\",\n  \"facetCount\": " << scene.facetCount() << ",\n  \"leafFacetCount\": " << scene.leafFacetCount
+      << ",\n";
+    writeVertices(o, scene);
+    o << "  \"graph\": {\"directedEdges\":" << graph.directedEdgeCount << ",\"closureError\":" << graph.maxClosureError
+      << ",\"fragmentOverflow\":" << graph.fragmentOverflow << ",\"hashOverflow\":" << graph.hashOverflow << "},\n"
+      << "  \"timing\": {\"totalMs\":" << ms(t0, t1) << ",\"visibilityMs\":" << ms(tv, tv1)
+      << ",\"sunlightMs\":" << ms(tv1, ts1) << ",\"solveMs\":" << ms(ts1, t1) << "},\n"
+      << "  \"iterations\": " << st.iterations << ",\n  \"maxDelta\": " << st.maxDelta << ",\n";
+    writeArray(o, "wavelengths", bands, true);
+    writeArray(o, "sunlit", sunlit, true);
+    writeArray(o, "radiosity", band0, true);
+    writeArray(o, "lightEnhancement", enh, !bandRad.empty());
+    if (!bandRad.empty()) writeArray(o, "bandRadiosity", bandRad, false);
+    o << "}\n";
+    o.close();
+    std::cout << "RESULT\t" << result << std::endl;
+    return 0;
+} catch (const std::exception& ex) {
+    std::cerr << "facetrt: " << ex.what() << '\n';
     return 1;
 }
 
@@ -911,10 +1060,6 @@ int main(int argc, char** argv) {
         return 1;
     }
     const std::string ext = std::filesystem::path(input).extension().string();
-    if (mode == "eFacetRT" && isProject(input)) {
-        std::cerr << "facetrt: project.json scenes run in eFacetEB here; eFacetRT reads OBJ scenes\n";
-        return 1;
-    }
     if (ext != ".obj" && ext != ".OBJ" && !isProject(input)) {
         std::cerr << (mode == "eFacetRT" ? "facetrt" : "faceteb") << ": the input must be an OBJ scene or project.json\n";
         return 1;

@@ -44,6 +44,7 @@ constexpr uint32_t kEnergySamples = 2001;  // 1-nm samples from 400 nm (core N1)
 struct ProjectMaterial {
     std::string label;                    // spectral / thermal / energy names
     std::vector<float> energyR, energyT;  // per 1-nm sample from 400 nm; empty: r, t
+    std::vector<float> bandR, bandT;      // per sensor band (optical bands); empty: r, t
     float r = 0.10f, t = 0.05f;
     float tirR = 0.03f, tirT = 0.0f;
     float sunlitT = 300.0f, shadedT = 296.0f;
@@ -225,6 +226,16 @@ inline void surfaceParameters(const json::Value& cfg, const std::string& spectra
             const float ti = interpolate(wl, T, lambda, 0.0f);
             m.energyR[k] = r;
             m.energyT[k] = std::min(std::clamp(std::isfinite(ti) ? ti : 0.0f, 0.0f, 1.0f), 1.0f - r);
+        }
+        // per sensor band (resolveSpectrals over the sensor wavelengths)
+        m.bandR.assign(bands.size(), 0.0f);
+        m.bandT.assign(bands.size(), 0.0f);
+        for (size_t b = 0; b < bands.size(); ++b) {
+            const float ri = interpolate(wl, R, bands[b], 0.2f);
+            const float r = std::clamp(std::isfinite(ri) ? ri : 0.2f, 0.0f, 1.0f);
+            const float ti = interpolate(wl, T, bands[b], 0.0f);
+            m.bandR[b] = r;
+            m.bandT[b] = std::min(std::clamp(std::isfinite(ti) ? ti : 0.0f, 0.0f, 1.0f), 1.0f - r);
         }
         m.r = m.energyR.front();
         m.t = m.energyT.front();

@@ -225,3 +225,87 @@ def test_cli_faceteb_project(tmp_path):
     # (soil_dry, not Vegetation) does not
     LE = noon[:, 6]
     assert np.all(LE[:2 * 126].reshape(-1, 2)[:, 0] > 0) and np.all(LE[2 * 126:2 * 138] == 0)
+
+
+@pytest.mark.gpu
+def test_cli_facetrt_project_bands(oracle, tmp_path):
+    # eFacetRT on the validation project (sensor bands 550, 850, 10500 nm),
+    # against the oracle: the same assembled scene (--scene-obj), its ray
+    # graph and sunlit fractions replayed on the host, the band optics of
+    # makeSpectralOptics (core:1057-1126) built here from the project's
+    # entries for the scene's four materials, 64 FP64 Jacobi sweeps per band,
+    # optical bands as apparent reflectance against 700 cos(30 deg) + 50
+    print("golden cli rt project", flush=True)
+    import os
+    from streamsim_amd.optics import planck_radiance
+    exe = Path(__file__).resolve().parents[1] / "streamsim_amd" / "facet_engine"
+    prj = HERE / "urban_tree_project" / "project.json"
+    env = dict(os.environ, FACET_RAYS_PER_SIDE="64", FACET_SUN_RAYS_PER_SIDE="16")
+    out = tmp_path / "rt" / "facetrt.json"
+    r = subprocess.run([str(exe), "eFacetRT", str(prj), str(out)], env=env, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    res = json.loads(out.read_text())
+    F = res["facetCount"]
+    S = 2 * F
+    assert F == 4138 and res["wavelengths"] == [550, 850, 10500]
+    assert res["quantity"] == "reflectance and spectral radiance"
+    got = np.array(res["bandRadiosity"], np.float64).reshape(3, S).T
+    # the assembled scene and its material per facet
+    sobj = tmp_path / "scene.obj"
+    subprocess.run([str(exe), "--scene-obj", str(prj), str(sobj)], check=True, capture_output=True)
+    verts, mat, cur = [], [], None
+    for line in open(sobj):
+        t = line.split()
+        if t and t[0] == "v":
+            verts.append([float(x) for x in t[1:4]])
+        elif t and t[0] == "usemtl":
+            cur = int(t[1][1:])
+        elif t and t[0] == "f":
+            mat.append(cur)
+    pos = np.array(verts, np.float32)
+    mat = np.array(mat)
+    assert len(mat) == F
+    # vertices with the facet normal as appendTriangle computes it (float32
+    # cross(b - a, c - a) times 1 / sqrt of its squared length)
+    a, b, c = pos[0::3], pos[1::3], pos[2::3]
+    u, w = b - a, c - a
+    nrm = np.stack([u[:, 1] * w[:, 2] - u[:, 2] * w[:, 1], u[:, 2] * w[:, 0] - u[:, 0] * w[:, 2],
+                    u[:, 0] * w[:, 1] - u[:, 1] * w[:, 0]], axis=1).astype(np.float32)
+    dd = nrm[:, 0] * nrm[:, 0] + nrm[:, 1] * nrm[:, 1] + nrm[:, 2] * nrm[:, 2]
+    nrm = nrm * (np.float32(1.0) / np.sqrt(dd))[:, None]
+    v = np.concatenate([pos, np.repeat(nrm, 3, axis=0)], axis=1).astype(np.float32)
+    # material -> (r, t at 550 / 850 nm, ref_TIR, tau_TIR, T_sunlit, T_shaded):
+    # trees green_leaf (two listed values for three bands: the first, as
+    # interpolateValue does) with vegetation_temperature; the building's wall
+    # and roof concrete with building_temperature; ground soil, soil_temperature
+    table = {0: (0.08, 0.04, 0.03, 0.0, 300.0, 296.0), 1: (0.35, 0.0, 0.08, 0.0, 310.0, 300.0),
+             2: (0.35, 0.0, 0.08, 0.0, 310.0, 300.0), 3: (0.20, 0.0, 0.05, 0.0, 305.0, 295.0)}
+    g = oracle.ray_graph(v, 1, 64)
+    sun = oracle.sun_direction(30.0, 135.0)
+    sl = oracle.ray_sunlit(v, sun, 1, 16)
+    assert np.array_equal(np.array(res["sunlit"], np.float32), sl)
+    sn = np.repeat(nrm.astype(np.float64), 2, axis=0) * np.tile([1.0, -1.0], F)[:, None]
+    cos = np.maximum(0.0, sn @ np.asarray(sun, np.float64))
+    side_mat = np.repeat(mat, 2)
+    p = np.array([table[m] for m in side_mat])
+    nb = 3
+    R = np.zeros((S, nb)); T = np.zeros((S, nb)); E = np.zeros((S, nb)); D = np.zeros((S, nb))
+    for b in range(2):
+        R[:, b] = p[:, 0]
+        T[:, b] = np.minimum(p[:, 1], 1 - p[:, 0])
+        D[:, b] = sl * cos * 700.0
+    R[:, 2] = p[:, 2]
+    T[:, 2] = p[:, 3]
+    eps = np.maximum(0.0, 1 - R[:, 2] - T[:, 2])
+    ill = np.clip(sl, 0, 1)
+    ps = np.array([planck_radiance(10500, x) for x in p[:, 4]])
+    pd = np.array([planck_radiance(10500, x) for x in p[:, 5]])
+    E[:, 2] = eps * (ill * ps + (1 - ill) * pd)
+    sky = [50.0, 50.0, planck_radiance(10500, 250.0)]
+    ref, it, _ = oracle.solve_bands(g, R, T, E, D, sky, max_iter=64, tol_rel=-1.0)
+    assert it == 64
+    refl = 700.0 * np.cos(np.radians(30.0)) + 50.0
+    want = ref.astype(np.float64)
+    want[:, :2] = np.clip(want[:, :2] / refl, 0, 1)
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-3)) < 1e-4

@@ -161,10 +161,3 @@ def test_project_errors(tmp_path, case, message):
         p = p.rename(p.with_name("scene.json"))
     r = subprocess.run([str(EXE), "--scene-obj", str(p), str(tmp_path / "o.obj")], capture_output=True, text=True)
     assert r.returncode == 1 and message in r.stderr, r.stderr
-
-
-def test_facetrt_reads_obj_only(tmp_path):
-    _need_exe()
-    p = _write_project(tmp_path, [])
-    r = subprocess.run([str(EXE), "eFacetRT", str(p)], capture_output=True, text=True)
-    assert r.returncode == 1 and "eFacetRT reads OBJ scenes" in r.stderr
