@@ -261,8 +261,11 @@ int fh_set_geometry(fh_context* c, const fh_vertex* v, uint32_t vertexCount) {
         }
         fh::BvhHost bvh;
         fh::build_bvh(tri9.data(), F, bvh);
-        // sweep schedule: facets in Morton order of their centroids, so the
-        // rows a sweep processes together see overlapping columns
+        // sweep schedule: facets in the leaf order of the BVH (spatially
+        // compact runs of facets), so the rows a sweep processes together see
+        // overlapping columns; Morton or Hilbert order of the centroids by
+        // FH_SCHED_KIND = 0 / 1 (measured: BVH order 1% faster than Morton,
+        // Hilbert 0.5%, on the terrain forest)
         {
             float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
             std::vector<float> cen(size_t(F) * 3u);
@@ -275,6 +278,11 @@ int fh_set_geometry(fh_context* c, const fh_vertex* v, uint32_t vertexCount) {
                     hi[q] = std::max(hi[q], x);
                 }
             const float ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2], 1.0e-20f});
+            // FH_SCHED_KIND: 0 Morton, 1 Hilbert, 2 BVH leaf order (default; tuning knob)
+            static const int kind = [] {
+                const char* e = std::getenv("FH_SCHED_KIND");
+                return e ? std::atoi(e) : 2;
+            }();
             std::vector<std::pair<uint64_t, uint32_t>> key(F);
             for (uint32_t f = 0; f < F; ++f) {
                 uint64_t code = 0;
@@ -282,14 +290,40 @@ int fh_set_geometry(fh_context* c, const fh_vertex* v, uint32_t vertexCount) {
                 for (int q = 0; q < 3; ++q)
                     cell[q] = uint32_t(std::min(1023.0f, std::max(0.0f, (cen[size_t(f) * 3u + q] - lo[q]) / ext *
                                                                              1023.0f)));
-                for (int b = 0; b < 10; ++b)
-                    for (int q = 0; q < 3; ++q) code |= uint64_t((cell[q] >> b) & 1u) << (3 * b + q);
+                if (kind == 1) {
+                    // Hilbert index (Skilling's transpose form, 10 bits per axis)
+                    uint32_t X[3] = {cell[0], cell[1], cell[2]};
+                    for (uint32_t Q = 1u << 9; Q > 1; Q >>= 1) {
+                        const uint32_t P = Q - 1;
+                        for (int i = 0; i < 3; ++i) {
+                            if (X[i] & Q) {
+                                X[0] ^= P;
+                            } else {
+                                const uint32_t t = (X[0] ^ X[i]) & P;
+                                X[0] ^= t;
+                                X[i] ^= t;
+                            }
+                        }
+                    }
+                    for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+                    uint32_t t = 0;
+                    for (uint32_t Q = 1u << 9; Q > 1; Q >>= 1)
+                        if (X[2] & Q) t ^= Q - 1;
+                    for (int i = 0; i < 3; ++i) X[i] ^= t;
+                    for (int b = 9; b >= 0; --b)
+                        for (int q = 0; q < 3; ++q) code = (code << 1) | ((X[q] >> b) & 1u);
+                } else {
+                    for (int b = 0; b < 10; ++b)
+                        for (int q = 0; q < 3; ++q) code |= uint64_t((cell[q] >> b) & 1u) << (3 * b + q);
+                }
                 key[f] = {code, f};
             }
             std::stable_sort(key.begin(), key.end(),
                              [](const auto& a, const auto& b) { return a.first < b.first; });
             std::vector<uint32_t> order(F);
             for (uint32_t i = 0; i < F; ++i) order[i] = key[i].second;
+            if (kind == 2 && bvh.primFacet.size() == F)
+                for (uint32_t i = 0; i < F; ++i) order[i] = bvh.primFacet[i];
             c->sched.ensure(F);
             upload(c, c->sched.p, order.data(), order.size());
         }
