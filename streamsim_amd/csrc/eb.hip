@@ -150,12 +150,39 @@ __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_
 // sum A (D + fsky S), sum A alpha H, sum A fsky B for the energy closure.
 // A team of 16 lanes owns one side, each lane 4 contiguous bands per pass
 // (16-byte loads of the [S][ldb] rows), summed over the team by shuffles.
+// per-side direct factor clamp(sunlit, 0, 1) max(0, cos(normal, sun)) (the
+// b = 0 part of k_sw_optics), for solves that read the material tables
+__global__ void k_sun_dir(uint32_t S, const float4* __restrict__ nrm, const float* __restrict__ sunlit, float sx,
+                          float sy, float sz, float* __restrict__ dir) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const float4 n = nrm[s >> 1];
+    const float sg = (s & 1u) ? -1.0f : 1.0f;
+    const float cosv = fmaxf(0.0f, (sg * n.x) * sx + (sg * n.y) * sy + (sg * n.z) * sz);
+    dir[s] = __fmul_rn(fminf(fmaxf(sunlit[s], 0.0f), 1.0f), cosv);
+}
+
+// Absorbed / incident SW per side.  TAB: r, t from the material tables and
+// D = dir dni (the values k_sw_optics would store); FROMG: the incident
+// H = max(0, fsky S + G + D) from the accumulated gathers G of an increment
+// solve (else the H array).
+struct AbsorbTables {
+    const uint32_t* mat;
+    const float* rtab;
+    const float* ttab;
+    const float* dir;
+    const float* dniv;
+    const float* G;
+};
+
+template <bool TAB, bool FROMG>
 __global__ __launch_bounds__(256) void k_absorb(uint32_t S, uint32_t nb, uint32_t ldb, const float* __restrict__ r,
                                                 const float* __restrict__ t, const float* __restrict__ H,
                                                 const float* __restrict__ D, const float* __restrict__ B,
                                                 const float* __restrict__ fsky, const float* __restrict__ skyv,
                                                 const double* __restrict__ area, float* __restrict__ absorbed,
-                                                float* __restrict__ incident, double* __restrict__ part) {
+                                                float* __restrict__ incident, double* __restrict__ part,
+                                                AbsorbTables tb) {
     const uint32_t team = threadIdx.x >> 4, lane = threadIdx.x & 15u;
     const uint32_t s = blockIdx.x * 16u + team;
     double pin = 0.0, pabs = 0.0, pesc = 0.0;
@@ -164,16 +191,43 @@ __global__ __launch_bounds__(256) void k_absorb(uint32_t S, uint32_t nb, uint32_
         const double A = area[s >> 1];
         const float fs = fsky[s];
         // ldb is 1 or a multiple of 4 (band_stride); the single band is lane 0's
+        // (TAB: band-parallel strides, ldb >= 16)
         const bool vec = ldb >= 4u;
+        const size_t mrow = TAB ? size_t(tb.mat[s >> 1]) * ldb : 0u;
+        const float dsd = TAB ? tb.dir[s] : 0.0f;
         for (uint32_t b0 = 4u * lane; b0 < nb; b0 += 64u) {
             const size_t i = size_t(s) * ldb + b0;
-            const float4 R4 = vec ? *reinterpret_cast<const float4*>(r + i) : make_float4(r[i], 0.f, 0.f, 0.f);
-            const float4 T4 = vec ? *reinterpret_cast<const float4*>(t + i) : make_float4(t[i], 0.f, 0.f, 0.f);
-            const float4 H4 = vec ? *reinterpret_cast<const float4*>(H + i) : make_float4(H[i], 0.f, 0.f, 0.f);
-            const float4 D4 = vec ? *reinterpret_cast<const float4*>(D + i) : make_float4(D[i], 0.f, 0.f, 0.f);
+            float rr[4], tt[4], hh[4], dd[4];
+            if constexpr (TAB) {
+                const float4 R4 = *reinterpret_cast<const float4*>(tb.rtab + mrow + b0);
+                const float4 T4 = *reinterpret_cast<const float4*>(tb.ttab + mrow + b0);
+                const float4 N4 = *reinterpret_cast<const float4*>(tb.dniv + b0);
+                const float nn[4] = {N4.x, N4.y, N4.z, N4.w};
+                rr[0] = R4.x; rr[1] = R4.y; rr[2] = R4.z; rr[3] = R4.w;
+                tt[0] = T4.x; tt[1] = T4.y; tt[2] = T4.z; tt[3] = T4.w;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) dd[j] = __fmul_rn(dsd, nn[j]);
+            } else {
+                const float4 R4 = vec ? *reinterpret_cast<const float4*>(r + i) : make_float4(r[i], 0.f, 0.f, 0.f);
+                const float4 T4 = vec ? *reinterpret_cast<const float4*>(t + i) : make_float4(t[i], 0.f, 0.f, 0.f);
+                const float4 D4 = vec ? *reinterpret_cast<const float4*>(D + i) : make_float4(D[i], 0.f, 0.f, 0.f);
+                rr[0] = R4.x; rr[1] = R4.y; rr[2] = R4.z; rr[3] = R4.w;
+                tt[0] = T4.x; tt[1] = T4.y; tt[2] = T4.z; tt[3] = T4.w;
+                dd[0] = D4.x; dd[1] = D4.y; dd[2] = D4.z; dd[3] = D4.w;
+            }
+            if constexpr (FROMG) {
+                const float4 G4 = *reinterpret_cast<const float4*>(tb.G + i);
+                const float gg[4] = {G4.x, G4.y, G4.z, G4.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const uint32_t b = b0 + uint32_t(j);
+                    hh[j] = b < nb ? fmaxf(fs * skyv[b] + gg[j] + dd[j], 0.0f) : 0.0f;
+                }
+            } else {
+                const float4 H4 = vec ? *reinterpret_cast<const float4*>(H + i) : make_float4(H[i], 0.f, 0.f, 0.f);
+                hh[0] = H4.x; hh[1] = H4.y; hh[2] = H4.z; hh[3] = H4.w;
+            }
             const float4 B4 = vec ? *reinterpret_cast<const float4*>(B + i) : make_float4(B[i], 0.f, 0.f, 0.f);
-            const float rr[4] = {R4.x, R4.y, R4.z, R4.w}, tt[4] = {T4.x, T4.y, T4.z, T4.w};
-            const float hh[4] = {H4.x, H4.y, H4.z, H4.w}, dd[4] = {D4.x, D4.y, D4.z, D4.w};
             const float bb[4] = {B4.x, B4.y, B4.z, B4.w};
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -805,10 +859,32 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         }
         FH_HIP(hipMemcpyAsync(e.dni.p, dpad.data(), sizeof(float) * ldb, hipMemcpyHostToDevice, c->stream));
         const size_t n = size_t(S) * ldb;
-        hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p, e.swRefl.p,
-                           e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, p.sun.x, p.sun.y, p.sun.z,
-                           e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
-        FH_HIP(hipGetLastError());
+        static const bool noTables = [] {
+            const char* v = std::getenv("FH_SW_ARRAYS");
+            return v && std::atoi(v) != 0;
+        }();
+        // band-parallel solves read r, t from the material tables and form c,
+        // D from the per-side direct factors, and the absorbed pass does the
+        // same: the [S][ldb] optics arrays and the folded source are only
+        // built for the entry-parallel layouts (and the spectral estimate)
+        const bool tab = !noTables && uses_bf16_gather(ldb);
+        const uint32_t precision = uses_bf16_gather(ldb) ? p.precision : FH_PRECISION_FP32;
+        // an increment solve keeps the gathers of every band's last sweep:
+        // the absorbed pass forms the incident from them
+        const bool fromG = tab && increment_solve(ldb, precision);
+        const auto optics_arrays = [&] {
+            hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p,
+                               e.swRefl.p, e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, p.sun.x, p.sun.y,
+                               p.sun.z, e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
+            FH_HIP(hipGetLastError());
+        };
+        if (!tab || !e.swRhoValid) {
+            optics_arrays();
+        } else {
+            hipLaunchKernelGGL(k_sun_dir, blocks_for(S), dim3(256), 0, c->stream, S, c->facetNormal.p, e.sunlit.p,
+                               p.sun.x, p.sun.y, p.sun.z, e.swDir.p);
+            FH_HIP(hipGetLastError());
+        }
         set_sky(c, diffInt.data(), nb, ldb);
         if (!e.swRhoValid) {
             // spectral radius per band of this scene's SW operator (Chebyshev)
@@ -816,24 +892,24 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
             estimate_rho2(c, ldb, nb, e.rsw.p, e.tsw.p, e.Bsw1.p, e.Hsw.p, e.csw.p, e.swRho2.p);
             e.swRhoValid = true;
         }
-        launch_fill_f32(c, e.Hsw.p, n, 0.0f);  // emission is zero in the SW
-        launch_fold(c, e.Hsw.p, e.rsw.p, e.tsw.p, e.Dsw.p, ldb, e.csw.p);
+        if (!tab) {
+            launch_fill_f32(c, e.Hsw.p, n, 0.0f);  // emission is zero in the SW
+            launch_fold(c, e.Hsw.p, e.rsw.p, e.tsw.p, e.Dsw.p, ldb, e.csw.p);
+        }
         if (!e.swValid || p.zeroStart) launch_fill_f32(c, e.Bsw0.p, n, 0.0f);
         JacobiOptions o;
         o.ldb = ldb;
         // BF16 gathers only where they save lines: below 16 windows a side's
         // FP32 row is at most one 64-byte gather, and the entry-parallel
         // BF16 scheme (defect correction) takes more sweeps than FP32
-        o.precision = uses_bf16_gather(ldb) ? p.precision : FH_PRECISION_FP32;
+        o.precision = precision;
         o.maxIter = p.maxSweeps;
         o.tolRel = p.radiosityTolerance;
-        o.H = e.Hsw.p;
-        o.D = e.Dsw.p;
-        static const bool noTables = [] {
-            const char* v = std::getenv("FH_SW_ARRAYS");
-            return v && std::atoi(v) != 0;
-        }();
-        if (!noTables) {
+        if (!fromG) {
+            o.H = e.Hsw.p;
+            o.D = e.Dsw.p;
+        }
+        if (tab) {
             o.fmat = e.material.p;
             o.rtab = e.swRtab.p;
             o.ttab = e.swTtab.p;
@@ -852,11 +928,25 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         if (fin == 1) e.Bsw0.swap(e.Bsw1);
         st.swSweeps = ss.iterations;
         st.swRefreshes = ss.refreshes;
-        if (!hDone) run_incident(c, ldb, e.Bsw0.p, e.Dsw.p, e.Hsw.p);
+        if (!fromG && !hDone) {
+            if (tab) optics_arrays();  // the incident sweep reads the D array
+            run_incident(c, ldb, e.Bsw0.p, e.Dsw.p, e.Hsw.p);
+        }
         const uint32_t nbk = (S + 15u) / 16u;  // 16 sides per block
         e.dpart.ensure(size_t(nbk) * 4u);
-        hipLaunchKernelGGL(k_absorb, dim3(nbk), dim3(256), 0, c->stream, S, nb, ldb, e.rsw.p, e.tsw.p, e.Hsw.p,
-                           e.Dsw.p, e.Bsw0.p, c->fsky.p, c->skyv.p, e.area.p, e.absorbed.p, e.incident.p, e.dpart.p);
+        AbsorbTables tb{e.material.p, e.swRtab.p, e.swTtab.p, e.swDir.p, e.dni.p, c->gacc.p};
+        if (fromG)
+            hipLaunchKernelGGL((k_absorb<true, true>), dim3(nbk), dim3(256), 0, c->stream, S, nb, ldb, e.rsw.p,
+                               e.tsw.p, e.Hsw.p, e.Dsw.p, e.Bsw0.p, c->fsky.p, c->skyv.p, e.area.p, e.absorbed.p,
+                               e.incident.p, e.dpart.p, tb);
+        else if (tab)
+            hipLaunchKernelGGL((k_absorb<true, false>), dim3(nbk), dim3(256), 0, c->stream, S, nb, ldb, e.rsw.p,
+                               e.tsw.p, e.Hsw.p, e.Dsw.p, e.Bsw0.p, c->fsky.p, c->skyv.p, e.area.p, e.absorbed.p,
+                               e.incident.p, e.dpart.p, tb);
+        else
+            hipLaunchKernelGGL((k_absorb<false, false>), dim3(nbk), dim3(256), 0, c->stream, S, nb, ldb, e.rsw.p,
+                               e.tsw.p, e.Hsw.p, e.Dsw.p, e.Bsw0.p, c->fsky.p, c->skyv.p, e.area.p, e.absorbed.p,
+                               e.incident.p, e.dpart.p, tb);
         hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(256), 0, c->stream, e.dpart.p, nbk, 0u, 3u, e.dsum.p);
         FH_HIP(hipGetLastError());
         double d4[4];

@@ -86,6 +86,10 @@ uint32_t bf16_stride(uint32_t ldb);
 // whether FP32-weight gathers read the packed CSR entries (when present)
 bool use_packed();
 
+// whether a solve of this band stride and precision runs increment sweeps
+// (its accumulated gathers are then in c->gacc after the solve)
+bool increment_solve(uint32_t ldb, uint32_t precision);
+
 // facet processing order of the sweeps (null: natural order)
 const uint32_t* sweep_schedule(const fh_context* c);
 

@@ -986,6 +986,10 @@ static bool use_increments() {
     return v;
 }
 
+bool increment_solve(uint32_t ldb, uint32_t precision) {
+    return precision == FH_PRECISION_BF16_DC && band_parallel(ldb) && use_increments();
+}
+
 const uint32_t* sweep_schedule(const fh_context* c) { return use_sched() && c->sched.p ? c->sched.p : nullptr; }
 
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
