@@ -54,8 +54,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--surrogate", default=str(ROOT / "streamsim_amd" / "data" / "surrogate.npz"))
     ap.add_argument("--roofline-reps", type=int, default=50)
-    ap.add_argument("--mode", default="run", choices=["run", "sweep"],
-                    help="sweep: only the roofline kernels (form factors, SpMM sweeps, copy calibration)")
+    ap.add_argument("--mode", default="run", choices=["run", "sweep", "precision"],
+                    help="sweep: only the roofline kernels (form factors, SpMM sweeps, copy calibration); "
+                         "precision: per-band error of the FP32 and BF16 shortwave solves against the FP64 oracle")
     return ap.parse_args()
 
 
@@ -124,6 +125,56 @@ def cpu_baseline(eng, scene, windows, meteo, cfg, nb_total: int, budget_s: float
     O.eb_node(og, scene.facet_count, scene.leaf_count, scene.material, mats, absorbed, p, T0, T0)
     dt = time.perf_counter() - t0
     return scene.facet_count * nb_total / dt, dt, f"1 converged daytime node (hour {met.hour:.1f}, {it} SW sweeps)"
+
+
+def precision_report(eng, scene, windows, meteo, cfg) -> dict:
+    """Per-band error of the GPU shortwave multi-band solve on the brightest
+    node against the FP64 oracle on the same graph and inputs: FP32 sweeps
+    and the BF16 solve (BF16 increments gathered, FP32 weights and
+    accumulation), each to relative change 1e-7 and to the run's 1e-5; the
+    oracle to 1e-9 (FP64 accumulation, its threads).  Error per band: max
+    |B - B_ref| / B_ref over the sides with B_ref above 1e-3 of the band's
+    maximum."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_ref as O
+    from streamsim_amd import faceteb, optics
+    from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32
+
+    g = eng.getGraph()
+    og = O.Graph(g.rowOffsets, g.neighbors, g.counts, g.denominators, g.skyCounts)
+    k = max(range(len(meteo)), key=lambda i: meteo[i].shortwave)
+    met = meteo[k]
+    cz = math.cos(math.radians(met.zenith))
+    dni, dif = windows.irradiance(met.shortwave, cz)
+    sun = faceteb.sun_vector(met.zenith, met.azimuth)
+    sl = eng.computeSunlitFraction(sun)
+    r_tab, t_tab = faceteb.material_tables(["leaf", "soil", "building"], windows)
+    mat = np.repeat(scene.material, 2)
+    r = r_tab[mat].astype(np.float32)
+    t = np.minimum(np.clip(t_tab[mat], 0, 1 - r), 1 - r).astype(np.float32)
+    cos = np.maximum(0.0, optics.side_normals(scene) @ sun).astype(np.float32)
+    D = ((np.clip(sl, 0, 1) * cos)[:, None] * dni[None, :]).astype(np.float32)
+    e = np.zeros_like(r)
+    t0 = time.perf_counter()
+    ref, it_ref, _ = O.solve_bands(og, r, t, e, D, dif, max_iter=400, tol_rel=1e-9)
+    t_ref = time.perf_counter() - t0
+    ref = ref.astype(np.float64)
+    mask = ref > 1e-3 * ref.max(axis=0, keepdims=True)
+    out = {"node_hour": met.hour, "bands": int(windows.count), "sides": int(scene.surface_count),
+           "oracle": {"iterations": it_ref, "seconds": t_ref, "threads": O.max_threads()}}
+    for name, prec in (("fp32", FH_PRECISION_FP32), ("bf16", FH_PRECISION_BF16_DC)):
+        for tol in (1e-7, cfg.radiosity_tolerance):
+            eng.synchronize()
+            t1 = time.perf_counter()
+            res = eng.solveBands(r, t, e, D, dif, tolRel=tol, precision=prec)
+            eng.synchronize()
+            dt = time.perf_counter() - t1
+            got = res.radiosity.astype(np.float64)
+            err = [float(np.max(np.abs(got[mask[:, b], b] - ref[mask[:, b], b]) / ref[mask[:, b], b]))
+                   for b in range(windows.count)]
+            out[f"{name}_tol{tol:g}"] = {"iterations": res.iterations, "seconds": dt, "max_rel_error": max(err),
+                                         "max_rel_error_per_band": err}
+    return out
 
 
 def cpu_baseline_sampled(eng, scene, windows, nb_total: int, st_plain):
@@ -197,6 +248,11 @@ def main() -> None:
 
     windows = faceteb.ShortwaveWindows.for_bands(wl["sw_bands"])
     nb_total = windows.count + 1
+    if a.mode == "precision":
+        rep = precision_report(eng, scene, windows, faceteb.diurnal_meteo(24), faceteb.EbConfig())
+        rep["workload"] = a.workload
+        print(json.dumps(rep), flush=True)
+        return
     if a.mode == "sweep":
         # counter runs: the SW sweep in FP32 and the increment sweep of BF16
         # solves, the LW sweep, and a 1 GiB copy as the byte calibration of
