@@ -385,15 +385,20 @@ __global__ __launch_bounds__(kSunBlock) void k_trace_sun_packet(const BvhNode* _
                                                                 uint32_t* __restrict__ lit) {
     __shared__ int stacks[kSunBlock / 64][kBvhMaxDepth];
     int* stk = stacks[threadIdx.x >> 6];
+    // lanes run over (facet, sample): at most one side of a facet faces the
+    // sun (n.s > 0: side 2f, -n.s > 0: side 2f + 1), so every lane of a wave
+    // traces a ray of a facing side instead of half of them idling on the
+    // sides that face away (their fraction stays 0)
+    const uint32_t F = S >> 1;
     const uint64_t gid = uint64_t(blockIdx.x) * kSunBlock + threadIdx.x;
-    bool on = gid < uint64_t(S) * K;
-    const uint32_t side = on ? uint32_t(gid / K) : S - 1u;
+    bool on = gid < uint64_t(F) * K;
+    const uint32_t f = on ? uint32_t(gid / K) : F - 1u;
     const uint32_t j = uint32_t(gid % K);
-    const uint32_t f = side >> 1;
     const float4 nf = fnorm[f];
-    const float sg = (side & 1u) ? -1.0f : 1.0f;
-    const float ns[3] = {sg * nf.x, sg * nf.y, sg * nf.z};
-    on = on && ns[0] * sx + ns[1] * sy + ns[2] * sz > 0.0f;
+    const float d0 = nf.x * sx + nf.y * sy + nf.z * sz;
+    const float d1 = (-nf.x) * sx + (-nf.y) * sy + (-nf.z) * sz;
+    const uint32_t side = 2u * f + (d0 > 0.0f ? 0u : 1u);
+    on = on && (d0 > 0.0f || d1 > 0.0f);
     const uint32_t key = side_key(seed ^ 0x5bd1e995u, side);
     Ray r;
     triangle_point(fverts, f, u01(ray_rand(key, j, 0u)), u01(ray_rand(key, j, 1u)), r.o);
@@ -416,7 +421,7 @@ __global__ __launch_bounds__(kSunBlock) void k_trace_sun_packet(const BvhNode* _
     // K consecutive lanes hold the rays of one side: one add per side
     const uint64_t m = __ballot(sky);
     const uint32_t l = threadIdx.x & 63u, firstLane = l - l % K;
-    if (l == firstLane && gid < uint64_t(S) * K) {
+    if (l == firstLane && gid < uint64_t(F) * K) {
         const uint64_t bits = K == 64u ? m : (m >> firstLane) & ((uint64_t(1) << K) - 1u);
         const uint32_t n = uint32_t(__popcll(bits));
         if (n) atomicAdd(&lit[side], n);
@@ -707,7 +712,7 @@ void launch_sun(fh_context* c, float sx, float sy, float sz, uint32_t* lit, floa
         return e ? std::atoi(e) != 0 : true;
     }();
     if (packets && 64u % K == 0u && c->bvhDepth < kBvhMaxDepth) {
-        const uint64_t blocks = (n + kSunBlock - 1) / kSunBlock;
+        const uint64_t blocks = (uint64_t(c->F) * K + kSunBlock - 1) / kSunBlock;  // lanes per (facet, sample)
         hipLaunchKernelGGL(k_trace_sun_packet, dim3(uint32_t(blocks)), dim3(kSunBlock), 0, c->stream, c->nodes.p,
                            c->tris.p, c->facetVerts.p, c->facetNormal.p, c->S, K, c->cfg.seed, sx, sy, sz,
                            c->cfg.rayEpsilon, periodic(c), lit);
