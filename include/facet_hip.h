@@ -158,7 +158,11 @@ uint32_t fh_surface_count(const fh_context *ctx);
  * it stops iterating while the others continue), or maxIter sweeps.
  * FH_PRECISION_BF16_DC from 16 bands on gathers BF16 increments of the
  * radiosity with FP32 weights and accumulation, reaching the FP32 fixed
- * point; its stopping test uses max(tolRel, 3e-7). */
+ * point; its stopping test uses max(tolRel, 3e-7).  From 16 bands on,
+ * facets that are each other's strongest partner (at least 0.2 of a side's
+ * row weight) are updated together with the entries between the two
+ * solved exactly (block Jacobi sweep): same fixed point and stopping test
+ * on the Jacobi change, fewer sweeps where leaves face each other closely. */
 int fh_solve_bands(fh_context *ctx, uint32_t nb, const float *reflectance,
                    const float *transmittance, const float *emission, const float *direct,
                    const float *sky, const float *init, uint32_t maxIter, float tolRel,
@@ -189,11 +193,13 @@ uint32_t fh_band_stride(uint32_t nb);
 void *fh_stream(const fh_context *ctx);
 
 /* Average duration (microseconds, HIP events on the context's stream) of
- * `reps` back-to-back single-kernel SpMM sweeps over the current graph with
- * nb bands, on engine-owned HBM inputs: the sweep of fh_sweep_device, except
- * that BF16 precision at band-parallel strides (nb >= 16) times the
- * increment sweep the BF16 solves run (BF16 increments of B gathered with
- * FP32 weights, FP32 accumulated gathers and carries). */
+ * `reps` back-to-back SpMM sweeps over the current graph with nb bands, on
+ * engine-owned HBM inputs, as the solves run them: the sweep of
+ * fh_sweep_device, except that at band-parallel strides (nb >= 16) the
+ * coupled facet pairs are updated as blocks by a second kernel on a second
+ * stream beside the first (joined before the next sweep), and BF16
+ * precision times the increment sweep the BF16 solves run (BF16 increments
+ * of B gathered with FP32 weights, FP32 accumulated gathers and carries). */
 int fh_benchmark_sweep(fh_context *ctx, uint32_t nb, uint32_t precision, uint32_t reps,
                        float *microsecondsPerSweep);
 /* Average duration (microseconds) of a plain device-to-device copy kernel of

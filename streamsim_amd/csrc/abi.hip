@@ -224,7 +224,11 @@ void fh_destroy(fh_context* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->pairStream) (void)hipStreamSynchronize(c->pairStream);
     if (c->hctrl) (void)hipHostFree(c->hctrl);
+    if (c->pairFork) (void)hipEventDestroy(c->pairFork);
+    if (c->pairJoin) (void)hipEventDestroy(c->pairJoin);
+    if (c->pairStream) (void)hipStreamDestroy(c->pairStream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -370,6 +374,7 @@ int fh_build_form_factors(fh_context* c, fh_graph_diagnostics* out) {
         fh::launch_fill_u32(c, c->misc.p, 4, 0u);
         // a count never exceeds its row's den, which never exceeds the rays of a side
         fh::launch_weights(c, c->misc.p, R, R <= 65535u);
+        fh::build_pairs(c);
         uint32_t bits = 0;
         download(c, &bits, c->misc.p, 1);
         std::memcpy(&c->closure, &bits, sizeof(float));
@@ -428,6 +433,7 @@ int fh_set_graph(fh_context* c, uint32_t nnz, const uint32_t* rowOffsets, const 
             }
         }
         fh::launch_weights(c, c->misc.p, maxCount, packable);
+        fh::build_pairs(c);
         uint32_t bits = 0;
         download(c, &bits, c->misc.p, 1);
         std::memcpy(&c->closure, &bits, sizeof(float));
@@ -665,14 +671,14 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
             fh::launch_fill_f32(c, G, n, 100.0f);
             fh::launch_fill_f32(c, K, n, 0.0f);
         }
-        for (int i = 0; i < 3; ++i) fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1, G, K);
+        for (int i = 0; i < 3; ++i) fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1, G, K, true);
         hipEvent_t e0, e1;
         FH_HIP(hipEventCreate(&e0));
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c->stream));
         for (uint32_t i = 0; i < reps; ++i)
             fh::launch_single_sweep(c, ldb, cc, rr, tt, (i & 1u) ? B1 : B0, (i & 1u) ? B0 : B1, w16,
-                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1, G, K);
+                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1, G, K, true);
         FH_HIP(hipEventRecord(e1, c->stream));
         FH_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

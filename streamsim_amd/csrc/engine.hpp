@@ -203,6 +203,20 @@ struct fh_context {
     float closure = 0.0f;
     fh::DBuf<uint32_t> rowptr, col, cnt, den, sky;
     fh::DBuf<uint32_t> sched;        // processing order of facets in the sweeps (Morton order)
+    // pair blocks of the band-parallel sweeps: units of two strongly
+    // coupled facets in sweep order, per unit the weights of the CSR
+    // entries from each facet's sides to the other's (w[f0->g0], w[f0->g1],
+    // w[f1->g0], w[f1->g1]; then g -> f), and the other facets in sweep
+    // order; unitCount = 0: no pair blocks
+    fh::DBuf<uint2> units;
+    fh::DBuf<float4> unitW;
+    fh::DBuf<uint32_t> singles;
+    uint32_t unitCount = 0;
+    uint32_t singleCount = 0;
+    // the pair launch of a sweep runs beside the singles launch: second
+    // stream (default priority) and the fork / join events
+    hipStream_t pairStream = nullptr;
+    hipEvent_t pairFork = nullptr, pairJoin = nullptr;
     fh::DBuf<float> val32, fsky;
     fh::DBuf<uint16_t> val16;
 

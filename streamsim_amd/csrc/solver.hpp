@@ -86,6 +86,11 @@ uint32_t bf16_stride(uint32_t ldb);
 // whether FP32-weight gathers read the packed CSR entries (when present)
 bool use_packed();
 
+// Pair blocks of the band-parallel sweeps, built with the graph's weights:
+// facets matched with their most strongly coupled facet, units of two
+// facets in sweep order (c->units, c->unitW)
+void build_pairs(fh_context* c);
+
 // whether a solve of this band stride and precision runs increment sweeps
 // (its accumulated gathers are then in c->gacc after the solve)
 bool increment_solve(uint32_t ldb, uint32_t precision);
@@ -99,9 +104,11 @@ int sweep_team(const fh_context* c);
 // One sweep, no host synchronisation (timing entry).  With w16 and BF16
 // copies given (band-parallel strides), the BF16-gather sweep h16in -> h16out;
 // with gacc and carry ([S][ldb]) also given, the increment sweep of BF16
-// solves (h16in / h16out hold BF16 increments).
+// solves (h16in / h16out hold BF16 increments).  pairs: the sweep of the
+// solves (pair blocks where the graph has them); else one Jacobi sweep.
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
                          const float* Bin, float* Bout, bool w16, uint16_t* h16in = nullptr,
-                         uint16_t* h16out = nullptr, float* gacc = nullptr, float* carry = nullptr);
+                         uint16_t* h16out = nullptr, float* gacc = nullptr, float* carry = nullptr,
+                         bool pairs = false);
 
 }  // namespace fh

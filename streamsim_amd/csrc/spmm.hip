@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -106,7 +107,17 @@ struct SweepArgs {
     const uint32_t* pk;
     const uint32_t* den;
     uint32_t pkBits;
+    // pair blocks (band-parallel FP32 and increment sweeps): team i updates
+    // the two facets of units[i] together, with the CSR entries between them
+    // solved exactly (unitW); null: one facet per team
+    const uint2* units;
+    const float4* unitW;
+    uint32_t nUnits;
+    uint32_t jacobiOnly;  // 1: plain Jacobi sweep (single-sweep entry points)
+    uint32_t partOff;     // band-parallel: first partial of this launch (a sweep may be two launches)
 };
+
+constexpr uint32_t kNoFacet = 0xffffffffu;  // no partner facet
 
 template <int NBC, int TEAM, int MODE, bool W16>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
@@ -256,6 +267,10 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
 // [nAct, nActPrev), retired by the previous judge sweep, are copied from Bin
 // to Bout once so that both ping-pong buffers hold their final value.
 constexpr int kU = 8;  // CSR entries in flight per lane (one aligned group)
+#ifndef FH_PAIR_GROUPS
+#define FH_PAIR_GROUPS 2
+#endif
+constexpr int kPairGroups = FH_PAIR_GROUPS;  // aligned groups in flight per lane in the pair gathers
 
 // The 8-aligned group of CSR entries [base, base + 8): columns and weights
 // with 16-byte vector loads (one request per team and vector; the CSR arrays
@@ -380,23 +395,201 @@ __device__ __forceinline__ void load_rtc(const SweepArgs& a, uint32_t s0, uint32
     }
 }
 
-// Update of side pair (s0, s0 + 1) at the 4 bands b0 .. b0 + 3 (b0 a
-// multiple of 4, 16-byte row accesses) from the gathered sums: bands below
-// nAct get the Jacobi (or Chebyshev) update, bands in [nAct, nActPrev) are
-// copied, bands from nActPrev on hold the same value in both buffers and are
-// rewritten unchanged.  Accumulates the residual maxima and the highest band
-// above tolRel.
+// Per-facet state of a 4-band row update (bands b0 .. b0 + 3, b0 a multiple
+// of 4, 16-byte row accesses) between the gathers and the stores.
+struct Upd4 {
+    size_t i0, i1;        // [S][ldb] offsets of the sides s0, s0 + 1
+    float o0[4], o1[4];   // B of this sweep
+    float n0[4], n1[4];   // Jacobi value c + r g_s + t g_o from the gathers
+    float r0[4], r1[4], t0[4], t1[4];
+    float k0[4], k1[4];   // increment sweeps: carry B - Q; else 0
+    bool act[4];          // band below nAct
+};
+
+// First half of the update of the side pair (s0, s0 + 1) from the gathered
+// sums g.  Bands in [nAct, nActPrev), retired by the last judge sweep, are
+// copied once so both ping-pong buffers hold their final value, and bands
+// from nActPrev on are left alone (false for both); otherwise the Jacobi
+// value n and, on judge sweeps of fused solves, the incident of this
+// sweep's gather.  INC: 0 FP32 sweep, 1 / 2 base / increment sweep.
+//
+// Increment sweeps: the gathers see Q, a copy of B that only ever changes
+// by BF16 increments: sweep k gathers the increments dq_k (BF16) and
+// accumulates G = M Q_k in FP32 (gacc); each side keeps carry = B - Q
+// exactly.  The update from G is the FP32 one (Jacobi or Chebyshev), and the
+// next increment is dq_k+1 = bf16(B_k+1 - Q_k) with carry_k+1 = (B_k+1 - Q_k)
+// - dq_k+1, so what rounding drops in one sweep is sent in the next and
+// Q -> B as B converges: the fixed point is the FP32 one.  The base sweep
+// gathers Q_0 = bf16(B_0) itself.
+template <int INC>
+__device__ __forceinline__ bool upd_begin(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
+                                          float (&g0)[4], float (&g1)[4], Upd4& u) {
+    if (b0 >= x.nPrev) return false;
+    const uint32_t ldb = a.ldb;
+    u.i0 = size_t(s0) * ldb + b0;
+    u.i1 = u.i0 + ldb;
+    const size_t i0 = u.i0, i1 = u.i1;
+    arr4(ld4(x.Bin + i0), u.o0);
+    arr4(ld4(x.Bin + i1), u.o1);
+    if (b0 >= x.nAct) {
+        st4(x.Bout + i0, u.o0);
+        st4(x.Bout + i1, u.o1);
+        return false;
+    }
+    if (INC == 1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            u.k0[j] = u.o0[j] - bf16_to_f32(f32_to_bf16(u.o0[j]));
+            u.k1[j] = u.o1[j] - bf16_to_f32(f32_to_bf16(u.o1[j]));
+        }
+    } else if (INC == 2) {
+        float p0[4], p1[4];
+        arr4(ld4(a.gacc + i0), p0);
+        arr4(ld4(a.gacc + i1), p1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            g0[j] += p0[j];
+            g1[j] += p1[j];
+        }
+        arr4(ld4(a.carry + i0), u.k0);
+        arr4(ld4(a.carry + i1), u.k1);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) u.k0[j] = u.k1[j] = 0.0f;
+    }
+    if (INC) {
+        st4(a.gacc + i0, g0);
+        st4(a.gacc + i1, g1);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) u.act[j] = b0 + uint32_t(j) < x.nAct;
+    const bool fuse = a.fuseH && a.judge;
+    float c0[4], c1[4], D0[4], D1[4];
+    load_rtc(a, s0, b0, i0, i1, fuse, u.r0, u.r1, u.t0, u.t1, c0, c1, D0, D1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        u.n0[j] = c0[j] + u.r0[j] * g0[j] + u.t0[j] * g1[j];
+        u.n1[j] = c1[j] + u.r1[j] * g1[j] + u.t1[j] * g0[j];
+    }
+    if (fuse) {
+        float sk[4], H0[4], H1[4];
+        arr4(ld4(a.skyv + b0), sk);
+        const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            H0[j] = fmaxf(f0 * sk[j] + g0[j] + D0[j], 0.0f);
+            H1[j] = fmaxf(f1 * sk[j] + g1[j] + D1[j], 0.0f);
+        }
+        if (u.act[3]) {
+            st4(a.H + i0, H0);
+            st4(a.H + i1, H1);
+        } else {  // retired bands keep the incident of their last judge sweep
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (u.act[j]) {
+                    a.H[i0 + j] = H0[j];
+                    a.H[i1 + j] = H1[j];
+                }
+        }
+    }
+    return true;
+}
+
+// Second half: the new iterate from m, the sweep's value before
+// acceleration (m = n with one facet per team) — Chebyshev y = p +
+// omega_b (m - p) with p the iterate before B, which Bout still holds
+// (ping-pong), or y = B + relax (m - B); the residual maxima of the Jacobi
+// change |n - B| (relaxed: of relax (n - B)), the reference's criterion;
+// the highest band above tolRel; the stores (and the next increment).
+template <int INC>
+__device__ __forceinline__ void upd_end(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0, Upd4& u,
+                                        const float (&m0)[4], const float (&m1)[4], float& myAbs, float& myRel,
+                                        float& myAct) {
+    const size_t i0 = u.i0, i1 = u.i1;
+    float y0[4], y1[4], d0[4], d1[4], e0[4], e1[4];
+    if (INC || a.omega) {
+        float w[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+        if (a.omega) arr4(ld4(a.omega + b0), w);
+        const bool anyW = w[0] != 1.0f || w[1] != 1.0f || w[2] != 1.0f || w[3] != 1.0f;
+        float p0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, p1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (anyW) {
+            arr4(ld4(x.Bout + i0), p0);
+            arr4(ld4(x.Bout + i1), p1);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            y0[j] = w[j] == 1.0f ? m0[j] : p0[j] + w[j] * (m0[j] - p0[j]);
+            y1[j] = w[j] == 1.0f ? m1[j] : p1[j] + w[j] * (m1[j] - p1[j]);
+            d0[j] = fabsf(u.n0[j] - u.o0[j]);
+            d1[j] = fabsf(u.n1[j] - u.o1[j]);
+            e0[j] = d0[j] / fmaxf(fabsf(u.n0[j]), 1.0e-30f);
+            e1[j] = d1[j] / fmaxf(fabsf(u.n1[j]), 1.0e-30f);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float j0 = u.o0[j] + a.relax * (u.n0[j] - u.o0[j]);  // relaxed Jacobi iterate
+            const float j1 = u.o1[j] + a.relax * (u.n1[j] - u.o1[j]);
+            y0[j] = u.o0[j] + a.relax * (m0[j] - u.o0[j]);
+            y1[j] = u.o1[j] + a.relax * (m1[j] - u.o1[j]);
+            d0[j] = fabsf(j0 - u.o0[j]);
+            d1[j] = fabsf(j1 - u.o1[j]);
+            e0[j] = d0[j] / fmaxf(fabsf(j0), 1.0e-30f);
+            e1[j] = d1[j] / fmaxf(fabsf(j1), 1.0e-30f);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (!u.act[j]) {  // beyond nAct inside this vector: unchanged
+            y0[j] = u.o0[j];
+            y1[j] = u.o1[j];
+            continue;
+        }
+        myAbs = fmaxf(myAbs, fmaxf(d0[j], d1[j]));
+        myRel = fmaxf(myRel, fmaxf(e0[j], e1[j]));
+        if (fmaxf(e0[j], e1[j]) > x.tolRel) myAct = fmaxf(myAct, float(b0 + uint32_t(j) + 1u));
+    }
+    st4(x.Bout + i0, y0);
+    st4(x.Bout + i1, y1);
+    if (INC) {
+        // next increment and the part of it that BF16 does not carry
+        uint32_t q[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float v0 = (y0[j] - u.o0[j]) + u.k0[j], v1 = (y1[j] - u.o1[j]) + u.k1[j];
+            const uint16_t h0 = f32_to_bf16(v0), h1 = f32_to_bf16(v1);
+            u.k0[j] = v0 - bf16_to_f32(h0);
+            u.k1[j] = v1 - bf16_to_f32(h1);
+            q[j] = uint32_t(h0) | (uint32_t(h1) << 16);
+        }
+        st4(a.carry + i0, u.k0);
+        st4(a.carry + i1, u.k1);
+        *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0) * a.ldh + b0) =
+            make_uint2((q[0] & 0xffffu) | (q[1] << 16), (q[2] & 0xffffu) | (q[3] << 16));
+        *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0 + 1) * a.ldh + b0) =
+            make_uint2((q[0] >> 16) | (q[1] & 0xffff0000u), (q[2] >> 16) | (q[3] & 0xffff0000u));
+    } else if (x.Bout16) {
+        const uint32_t h0 = uint32_t(f32_to_bf16(y0[0])) | (uint32_t(f32_to_bf16(y0[1])) << 16);
+        const uint32_t h1 = uint32_t(f32_to_bf16(y0[2])) | (uint32_t(f32_to_bf16(y0[3])) << 16);
+        const uint32_t h2 = uint32_t(f32_to_bf16(y1[0])) | (uint32_t(f32_to_bf16(y1[1])) << 16);
+        const uint32_t h3 = uint32_t(f32_to_bf16(y1[2])) | (uint32_t(f32_to_bf16(y1[3])) << 16);
+        *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0) * a.ldh + b0) = make_uint2(h0, h1);
+        *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0 + 1) * a.ldh + b0) = make_uint2(h2, h3);
+    }
+}
+
+// Update of the side pair (s0, s0 + 1) at the 4 bands b0 .. b0 + 3 from the
+// gathered sums (one facet per team): MODE_SWEEP the Jacobi (or Chebyshev)
+// update, MODE_INCIDENT the incident H = max(0, fsky S + G + D).
 template <int MODE>
 __device__ __forceinline__ void band_update4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
-                                             const float (&g0)[4], const float (&g1)[4], float& myAbs, float& myRel,
+                                             float (&g0)[4], float (&g1)[4], float& myAbs, float& myRel,
                                              float& myAct) {
     if (b0 >= x.nPrev) return;
-    const uint32_t ldb = a.ldb;
-    const size_t i0 = size_t(s0) * ldb + b0, i1 = i0 + ldb;
-    float sk[4];
-    if (MODE == MODE_INCIDENT || (a.fuseH && a.judge)) arr4(ld4(a.skyv + b0), sk);
     if (MODE == MODE_INCIDENT) {
-        float D0[4], D1[4], H0[4], H1[4];
+        const size_t i0 = size_t(s0) * a.ldb + b0, i1 = i0 + a.ldb;
+        float sk[4], D0[4], D1[4], H0[4], H1[4];
+        arr4(ld4(a.skyv + b0), sk);
         arr4(ld4(a.D + i0), D0);
         arr4(ld4(a.D + i1), D1);
         const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
@@ -409,231 +602,193 @@ __device__ __forceinline__ void band_update4(const SweepArgs& a, const BandCtx& 
         st4(a.H + i1, H1);
         return;
     }
-    float o0[4], o1[4];
-    arr4(ld4(x.Bin + i0), o0);
-    arr4(ld4(x.Bin + i1), o1);
-    if (b0 >= x.nAct) {  // retired by the last judge sweep: copied once
-        st4(x.Bout + i0, o0);
-        st4(x.Bout + i1, o1);
-        return;
-    }
-    bool act[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) act[j] = b0 + uint32_t(j) < x.nAct;
-    const bool fuse = a.fuseH && a.judge;
-    float r0[4], r1[4], t0[4], t1[4], c0[4], c1[4], n0[4], n1[4], D0[4], D1[4];
-    load_rtc(a, s0, b0, i0, i1, fuse, r0, r1, t0, t1, c0, c1, D0, D1);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        n0[j] = c0[j] + r0[j] * g0[j] + t0[j] * g1[j];
-        n1[j] = c1[j] + r1[j] * g1[j] + t1[j] * g0[j];
-    }
-    if (fuse) {  // incident of the FP32 gather (judge sweeps gather FP32)
-        float H0[4], H1[4];
-        const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            H0[j] = fmaxf(f0 * sk[j] + g0[j] + D0[j], 0.0f);
-            H1[j] = fmaxf(f1 * sk[j] + g1[j] + D1[j], 0.0f);
-        }
-        if (act[3]) {
-            st4(a.H + i0, H0);
-            st4(a.H + i1, H1);
-        } else {  // retired bands keep the incident of their last judge sweep
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (act[j]) {
-                    a.H[i0 + j] = H0[j];
-                    a.H[i1 + j] = H1[j];
-                }
-        }
-    }
-    float y0[4], y1[4], d0[4], d1[4], e0[4], e1[4];
-    if (a.omega) {
-        // Chebyshev semi-iteration: y = p + omega_b (J(B) - p) with p the
-        // iterate before B, which Bout still holds (ping-pong); the judge
-        // measures the Jacobi change |J(B) - B| / |J(B)|
-        float w[4];
-        arr4(ld4(a.omega + b0), w);
-        const bool anyW = w[0] != 1.0f || w[1] != 1.0f || w[2] != 1.0f || w[3] != 1.0f;
-        float p0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, p1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (anyW) {
-            arr4(ld4(x.Bout + i0), p0);
-            arr4(ld4(x.Bout + i1), p1);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            y0[j] = w[j] == 1.0f ? n0[j] : p0[j] + w[j] * (n0[j] - p0[j]);
-            y1[j] = w[j] == 1.0f ? n1[j] : p1[j] + w[j] * (n1[j] - p1[j]);
-            d0[j] = fabsf(n0[j] - o0[j]);
-            d1[j] = fabsf(n1[j] - o1[j]);
-            e0[j] = d0[j] / fmaxf(fabsf(n0[j]), 1.0e-30f);
-            e1[j] = d1[j] / fmaxf(fabsf(n1[j]), 1.0e-30f);
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            y0[j] = o0[j] + a.relax * (n0[j] - o0[j]);
-            y1[j] = o1[j] + a.relax * (n1[j] - o1[j]);
-            d0[j] = fabsf(y0[j] - o0[j]);
-            d1[j] = fabsf(y1[j] - o1[j]);
-            e0[j] = d0[j] / fmaxf(fabsf(y0[j]), 1.0e-30f);
-            e1[j] = d1[j] / fmaxf(fabsf(y1[j]), 1.0e-30f);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        if (!act[j]) {  // beyond nAct inside this vector: unchanged
-            y0[j] = o0[j];
-            y1[j] = o1[j];
-            continue;
-        }
-        myAbs = fmaxf(myAbs, fmaxf(d0[j], d1[j]));
-        myRel = fmaxf(myRel, fmaxf(e0[j], e1[j]));
-        if (fmaxf(e0[j], e1[j]) > x.tolRel) myAct = fmaxf(myAct, float(b0 + uint32_t(j) + 1u));
-    }
-    st4(x.Bout + i0, y0);
-    st4(x.Bout + i1, y1);
-    if (x.Bout16) {
-        const uint32_t h0 = uint32_t(f32_to_bf16(y0[0])) | (uint32_t(f32_to_bf16(y0[1])) << 16);
-        const uint32_t h1 = uint32_t(f32_to_bf16(y0[2])) | (uint32_t(f32_to_bf16(y0[3])) << 16);
-        const uint32_t h2 = uint32_t(f32_to_bf16(y1[0])) | (uint32_t(f32_to_bf16(y1[1])) << 16);
-        const uint32_t h3 = uint32_t(f32_to_bf16(y1[2])) | (uint32_t(f32_to_bf16(y1[3])) << 16);
-        *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0) * a.ldh + b0) = make_uint2(h0, h1);
-        *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0 + 1) * a.ldh + b0) = make_uint2(h2, h3);
-    }
+    Upd4 u;
+    if (upd_begin<0>(a, x, s0, b0, g0, g1, u)) upd_end<0>(a, x, s0, b0, u, u.n0, u.n1, myAbs, myRel, myAct);
 }
 
-// Row update of an increment sweep at the 4 bands b0 .. b0 + 3 of the side
-// pair (s0, s0 + 1).  The gathers see Q, a copy of B that only ever changes
-// by BF16 increments: sweep k gathers the increments dq_k (BF16) and
-// accumulates G = M Q_k in FP32; each side keeps carry = B - Q exactly.  The
-// update from G is the FP32 one (Jacobi or Chebyshev), and the next increment
-// is dq_k+1 = bf16(B_k+1 - Q_k) with carry_k+1 = (B_k+1 - Q_k) - dq_k+1, so
-// what rounding drops in one sweep is sent in the next and Q -> B as B
-// converges: the fixed point is the FP32 one.  The base sweep (BASE) gathers
-// Q_0 = bf16(B_0) itself.
+// Row update of an increment sweep (one facet per team); BASE: base sweep.
 template <bool BASE>
 __device__ __forceinline__ void band_update_inc4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
                                                  float (&g0)[4], float (&g1)[4], float& myAbs, float& myRel,
                                                  float& myAct) {
-    if (b0 >= x.nPrev) return;
-    const uint32_t ldb = a.ldb;
-    const size_t i0 = size_t(s0) * ldb + b0, i1 = i0 + ldb;
-    float o0[4], o1[4];
-    arr4(ld4(x.Bin + i0), o0);
-    arr4(ld4(x.Bin + i1), o1);
-    if (b0 >= x.nAct) {  // retired by the last judge sweep: copied once
-        st4(x.Bout + i0, o0);
-        st4(x.Bout + i1, o1);
+    Upd4 u;
+    if (upd_begin<BASE ? 1 : 2>(a, x, s0, b0, g0, g1, u))
+        upd_end<BASE ? 1 : 2>(a, x, s0, b0, u, u.n0, u.n1, myAbs, myRel, myAct);
+}
+
+// Pair blocks.  With the CSR entries between the two facets (f, g) of a
+// unit taken at the new values, the changes D = B' - B satisfy
+// Df = jf + Af Dg and Dg = jg + Ag Df: j = n - B (plus A carry of the
+// partner in increment sweeps, whose gathers saw Q = B - carry), and A
+// (2 x 2 per band) maps the partner's side changes into this facet's
+// Jacobi value through the unit weights W of its rows: A00 = r0 W00 +
+// t0 W10, A01 = r0 W01 + t0 W11, A10 = r1 W10 + t1 W00, A11 = r1 W11 +
+// t1 W01.  So (I - Af Ag) Df = jf + Af jg, then Dg; a near-singular block
+// (det <= 1e-6) keeps the Jacobi values.  The new value is written as the
+// Jacobi value plus the pair correction, B'f = nf + Af (Dg + carry_g),
+// and a correction within 2^-21 of nf (a few ulps) is dropped: near the
+// fixed point the sweep is then exactly the FP32 Jacobi map, whose fixed
+// point it reaches (B' = B + D would let partners flip each other's last
+// bit indefinitely).  To keep the register footprint of a team at two
+// facets small, a facet's state between the gathers and the solve is only
+// its Jacobi change, A and carry; B and the carry are read again for the
+// stores.
+struct Pair4 {
+    float j0[4], j1[4];                      // Jacobi change n - B
+    float a00[4], a01[4], a10[4], a11[4];    // A
+    float k0[4], k1[4];                      // carry (increment sweeps)
+};
+
+template <int INC>
+__device__ __forceinline__ bool pair_begin(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
+                                           float (&g0)[4], float (&g1)[4], const float4 W, Pair4& p) {
+    Upd4 u;
+    if (!upd_begin<INC>(a, x, s0, b0, g0, g1, u)) return false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        p.j0[j] = u.n0[j] - u.o0[j];
+        p.j1[j] = u.n1[j] - u.o1[j];
+        p.a00[j] = u.r0[j] * W.x + u.t0[j] * W.z;
+        p.a01[j] = u.r0[j] * W.y + u.t0[j] * W.w;
+        p.a10[j] = u.r1[j] * W.z + u.t1[j] * W.x;
+        p.a11[j] = u.r1[j] * W.w + u.t1[j] * W.y;
+        p.k0[j] = u.k0[j];
+        p.k1[j] = u.k1[j];
+    }
+    return true;
+}
+
+// pair corrections c = A (D + carry) of the partner, per facet side
+__device__ __forceinline__ void pair_solve4(const Pair4& f, const Pair4& g, float (&cf0)[4], float (&cf1)[4],
+                                            float (&cg0)[4], float (&cg1)[4]) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float jf0 = f.j0[j] + f.a00[j] * g.k0[j] + f.a01[j] * g.k1[j];
+        const float jf1 = f.j1[j] + f.a10[j] * g.k0[j] + f.a11[j] * g.k1[j];
+        const float jg0 = g.j0[j] + g.a00[j] * f.k0[j] + g.a01[j] * f.k1[j];
+        const float jg1 = g.j1[j] + g.a10[j] * f.k0[j] + g.a11[j] * f.k1[j];
+        const float p00 = f.a00[j] * g.a00[j] + f.a01[j] * g.a10[j];
+        const float p01 = f.a00[j] * g.a01[j] + f.a01[j] * g.a11[j];
+        const float p10 = f.a10[j] * g.a00[j] + f.a11[j] * g.a10[j];
+        const float p11 = f.a10[j] * g.a01[j] + f.a11[j] * g.a11[j];
+        const float det = (1.0f - p00) * (1.0f - p11) - p01 * p10;
+        cf0[j] = cf1[j] = cg0[j] = cg1[j] = 0.0f;
+        if (det > 1.0e-6f) {
+            const float h0 = jf0 + f.a00[j] * jg0 + f.a01[j] * jg1;
+            const float h1 = jf1 + f.a10[j] * jg0 + f.a11[j] * jg1;
+            const float inv = 1.0f / det;
+            const float df0 = ((1.0f - p11) * h0 + p01 * h1) * inv;
+            const float df1 = (p10 * h0 + (1.0f - p00) * h1) * inv;
+            const float dg0 = jg0 + g.a00[j] * df0 + g.a01[j] * df1;
+            const float dg1 = jg1 + g.a10[j] * df0 + g.a11[j] * df1;
+            const float ug0 = dg0 + g.k0[j], ug1 = dg1 + g.k1[j], uf0 = df0 + f.k0[j], uf1 = df1 + f.k1[j];
+            cf0[j] = f.a00[j] * ug0 + f.a01[j] * ug1;
+            cf1[j] = f.a10[j] * ug0 + f.a11[j] * ug1;
+            cg0[j] = g.a00[j] * uf0 + g.a01[j] * uf1;
+            cg1[j] = g.a10[j] * uf0 + g.a11[j] * uf1;
+        }
+    }
+}
+
+// stores of one facet of a pair: B and the carry read again, n = B + j
+// (exact where B and n are within a factor 2), m = n + c beyond 2^-21 |n|
+template <int INC>
+__device__ __forceinline__ void pair_end(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
+                                         const float (&j0)[4], const float (&j1)[4], const float (&c0)[4],
+                                         const float (&c1)[4], float& myAbs, float& myRel, float& myAct) {
+    Upd4 u;
+    u.i0 = size_t(s0) * a.ldb + b0;
+    u.i1 = u.i0 + a.ldb;
+    arr4(ld4(x.Bin + u.i0), u.o0);
+    arr4(ld4(x.Bin + u.i1), u.o1);
+    if (INC == 1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            u.k0[j] = u.o0[j] - bf16_to_f32(f32_to_bf16(u.o0[j]));
+            u.k1[j] = u.o1[j] - bf16_to_f32(f32_to_bf16(u.o1[j]));
+        }
+    } else if (INC == 2) {
+        arr4(ld4(a.carry + u.i0), u.k0);
+        arr4(ld4(a.carry + u.i1), u.k1);
+    }
+    constexpr float kDrop = 4.76837158e-07f;  // 2^-21
+    float m0[4], m1[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        u.act[j] = b0 + uint32_t(j) < x.nAct;
+        u.n0[j] = u.o0[j] + j0[j];
+        u.n1[j] = u.o1[j] + j1[j];
+        m0[j] = fabsf(c0[j]) > kDrop * fabsf(u.n0[j]) ? u.n0[j] + c0[j] : u.n0[j];
+        m1[j] = fabsf(c1[j]) > kDrop * fabsf(u.n1[j]) ? u.n1[j] + c1[j] : u.n1[j];
+    }
+    upd_end<INC>(a, x, s0, b0, u, m0, m1, myAbs, myRel, myAct);
+}
+
+// Update of a unit's two facets (side pairs s0 and s1) at 4 bands; INC as
+// upd_begin.  Units whose facets have no entries between them update both
+// facets as single ones.
+template <int INC>
+__device__ __forceinline__ void pair_update4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t s1,
+                                             uint32_t b0, bool coupled, const float4 Wf, const float4 Wg,
+                                             float (&gf0)[4], float (&gf1)[4], float (&gg0)[4], float (&gg1)[4],
+                                             float& myAbs, float& myRel, float& myAct) {
+    if (!coupled) {
+        Upd4 u;
+        if (upd_begin<INC>(a, x, s0, b0, gf0, gf1, u)) upd_end<INC>(a, x, s0, b0, u, u.n0, u.n1, myAbs, myRel, myAct);
+        if (upd_begin<INC>(a, x, s1, b0, gg0, gg1, u)) upd_end<INC>(a, x, s1, b0, u, u.n0, u.n1, myAbs, myRel, myAct);
         return;
     }
-    float k0[4], k1[4];
-    if (BASE) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            k0[j] = o0[j] - bf16_to_f32(f32_to_bf16(o0[j]));
-            k1[j] = o1[j] - bf16_to_f32(f32_to_bf16(o1[j]));
-        }
-    } else {
-        float p0[4], p1[4];
-        arr4(ld4(a.gacc + i0), p0);
-        arr4(ld4(a.gacc + i1), p1);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            g0[j] += p0[j];
-            g1[j] += p1[j];
-        }
-        arr4(ld4(a.carry + i0), k0);
-        arr4(ld4(a.carry + i1), k1);
-    }
-    st4(a.gacc + i0, g0);
-    st4(a.gacc + i1, g1);
-    bool act[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) act[j] = b0 + uint32_t(j) < x.nAct;
-    const bool fuse = a.fuseH && a.judge;
-    float r0[4], r1[4], t0[4], t1[4], c0[4], c1[4], n0[4], n1[4], D0[4], D1[4];
-    load_rtc(a, s0, b0, i0, i1, fuse, r0, r1, t0, t1, c0, c1, D0, D1);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        n0[j] = c0[j] + r0[j] * g0[j] + t0[j] * g1[j];
-        n1[j] = c1[j] + r1[j] * g1[j] + t1[j] * g0[j];
-    }
-    if (fuse) {  // incident of this sweep's gather
-        float sk[4], H0[4], H1[4];
-        arr4(ld4(a.skyv + b0), sk);
-        const float f0 = a.fsky[s0], f1 = a.fsky[s0 + 1];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            H0[j] = fmaxf(f0 * sk[j] + g0[j] + D0[j], 0.0f);
-            H1[j] = fmaxf(f1 * sk[j] + g1[j] + D1[j], 0.0f);
-        }
-        if (act[3]) {
-            st4(a.H + i0, H0);
-            st4(a.H + i1, H1);
-        } else {  // retired bands keep the incident of their last judge sweep
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (act[j]) {
-                    a.H[i0 + j] = H0[j];
-                    a.H[i1 + j] = H1[j];
-                }
-        }
-    }
-    float y0[4], y1[4];
-    float w[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-    if (a.omega) arr4(ld4(a.omega + b0), w);
-    const bool anyW = w[0] != 1.0f || w[1] != 1.0f || w[2] != 1.0f || w[3] != 1.0f;
-    float p0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, p1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (anyW) {  // Chebyshev: the iterate before B, which Bout still holds
-        arr4(ld4(x.Bout + i0), p0);
-        arr4(ld4(x.Bout + i1), p1);
-    }
-    uint32_t q[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        y0[j] = w[j] == 1.0f ? n0[j] : p0[j] + w[j] * (n0[j] - p0[j]);
-        y1[j] = w[j] == 1.0f ? n1[j] : p1[j] + w[j] * (n1[j] - p1[j]);
-        const float d0 = fabsf(n0[j] - o0[j]), d1 = fabsf(n1[j] - o1[j]);
-        if (!act[j]) {  // beyond nAct inside this vector: B unchanged
-            y0[j] = o0[j];
-            y1[j] = o1[j];
-        } else {
-            const float e0 = d0 / fmaxf(fabsf(n0[j]), 1.0e-30f), e1 = d1 / fmaxf(fabsf(n1[j]), 1.0e-30f);
-            myAbs = fmaxf(myAbs, fmaxf(d0, d1));
-            myRel = fmaxf(myRel, fmaxf(e0, e1));
-            if (fmaxf(e0, e1) > x.tolRel) myAct = fmaxf(myAct, float(b0 + uint32_t(j) + 1u));
-        }
-        // next increment and the part of it that BF16 does not carry
-        const float v0 = (y0[j] - o0[j]) + k0[j], v1 = (y1[j] - o1[j]) + k1[j];
-        const uint16_t h0 = f32_to_bf16(v0), h1 = f32_to_bf16(v1);
-        k0[j] = v0 - bf16_to_f32(h0);
-        k1[j] = v1 - bf16_to_f32(h1);
-        q[j] = uint32_t(h0) | (uint32_t(h1) << 16);
-    }
-    st4(x.Bout + i0, y0);
-    st4(x.Bout + i1, y1);
-    st4(a.carry + i0, k0);
-    st4(a.carry + i1, k1);
-    *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0) * a.ldh + b0) =
-        make_uint2((q[0] & 0xffffu) | (q[1] << 16), (q[2] & 0xffffu) | (q[3] << 16));
-    *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0 + 1) * a.ldh + b0) =
-        make_uint2((q[0] >> 16) | (q[1] & 0xffff0000u), (q[2] >> 16) | (q[3] & 0xffff0000u));
+    Pair4 pf, pg;
+    const bool okf = pair_begin<INC>(a, x, s0, b0, gf0, gf1, Wf, pf);
+    const bool okg = pair_begin<INC>(a, x, s1, b0, gg0, gg1, Wg, pg);  // same bands: same outcome as f
+    if (!okf || !okg) return;
+    float cf0[4], cf1[4], cg0[4], cg1[4];
+    pair_solve4(pf, pg, cf0, cf1, cg0, cg1);
+    pair_end<INC>(a, x, s0, b0, pf.j0, pf.j1, cf0, cf1, myAbs, myRel, myAct);
+    pair_end<INC>(a, x, s1, b0, pg.j0, pg.j1, cg0, cg1, myAbs, myRel, myAct);
 }
 
 __device__ __forceinline__ void band_partials(const SweepArgs& a, float myAbs, float myRel, float myAct) {
     float v[3] = {myAbs, myRel, myAct}, out[3];
     block_max<3>(v, out);
     if (threadIdx.x == 0) {
-        a.partial[blockIdx.x] = make_float2(out[0], out[1]);
-        if (a.partAct) a.partAct[blockIdx.x] = uint32_t(out[2]);
+        a.partial[a.partOff + blockIdx.x] = make_float2(out[0], out[1]);
+        if (a.partAct) a.partAct[a.partOff + blockIdx.x] = uint32_t(out[2]);
+    }
+}
+
+// Sums of the two rows of a facet (entries [rb[0], rb[1]) and [rb[1],
+// rb[2])) at the 4 FP32 bands of this lane, Bl = Bin + b0.
+template <bool W16>
+__device__ __forceinline__ void gather_v(const SweepArgs& a, const float* __restrict__ Bl, uint32_t ldb,
+                                         const uint32_t (&rb)[3], float (&g)[2][4]) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t start = rb[side], end = rb[side + 1];
+        for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += kU) {
+            uint32_t n[kU];
+            float w[kU];
+            bool ok[kU];
+            load_group<W16>(a, base, start, end, n, w, ok);
+            float4 v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                v[u] = ok[u] ? *reinterpret_cast<const float4*>(Bl + size_t(n[u]) * ldb)
+                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const float vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) g[side][j] += w[u] * vv[j];
+            }
+        }
     }
 }
 
 // FP32 gathers, 4 bands per lane: MODE_SWEEP with FP32 (W16 false) or BF16
-// weights, or MODE_INCIDENT.
-template <int MODE, bool W16>
+// weights, or MODE_INCIDENT.  PAIR (MODE_SWEEP, FP32 weights): team i
+// updates the two facets of units[i] with the pair block solve.
+template <int MODE, bool W16, bool PAIR>
 __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
     BandCtx x;
     if (!band_ctx<MODE>(a, MODE == MODE_SWEEP && W16, x)) return;
@@ -641,50 +796,120 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
     const uint32_t team = threadIdx.x / bt, lane = threadIdx.x - team * bt;
     const uint32_t fl = xcd_block(a.xcd) * tpb + team;
     float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
-    if (team < tpb && fl < a.F) {
-        const uint32_t f = a.sched ? a.sched[fl] : fl;
+    if (team < tpb && fl < (PAIR ? a.nUnits : a.F)) {
+        uint32_t f, h = kNoFacet;
+        if (PAIR) {
+            const uint2 un = a.units[fl];
+            f = un.x;
+            h = un.y;
+        } else {
+            f = a.sched ? a.sched[fl] : fl;
+        }
         const uint32_t s0 = 2u * f;
         const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
-        for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
-            const uint32_t b0 = cb + 4u * lane;
-            float g[2][4] = {};
-            if (b0 < x.nAct) {
-                const float* __restrict__ Bl = x.Bin + b0;
-#pragma unroll
-                for (int side = 0; side < 2; ++side) {
-                    const uint32_t start = rb[side], end = rb[side + 1];
-                    for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += kU) {
-                        uint32_t n[kU];
-                        float w[kU];
-                        bool ok[kU];
-                        load_group<W16>(a, base, start, end, n, w, ok);
-                        float4 v[kU];
-#pragma unroll
-                        for (int u = 0; u < kU; ++u)
-                            v[u] = ok[u] ? *reinterpret_cast<const float4*>(Bl + size_t(n[u]) * ldb)
-                                         : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-#pragma unroll
-                        for (int u = 0; u < kU; ++u) {
-                            const float vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) g[side][j] += w[u] * vv[j];
-                        }
-                    }
-                }
+        if (!PAIR || h == kNoFacet) {
+            for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
+                const uint32_t b0 = cb + 4u * lane;
+                float g[2][4] = {};
+                if (b0 < x.nAct) gather_v<W16>(a, x.Bin + b0, ldb, rb, g);
+                band_update4<MODE>(a, x, s0, b0, g[0], g[1], myAbs, myRel, myAct);
             }
-            band_update4<MODE>(a, x, s0, b0, g[0], g[1], myAbs, myRel, myAct);
+        } else {
+            const uint32_t s1 = 2u * h;
+            const uint32_t rc[3] = {a.rowptr[s1], a.rowptr[s1 + 1], a.rowptr[s1 + 2]};
+            const float4 Wf = a.unitW[2u * fl], Wg = a.unitW[2u * fl + 1u];
+            const bool coupled = Wf.x + Wf.y + Wf.z + Wf.w + Wg.x + Wg.y + Wg.z + Wg.w > 0.0f;
+            for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
+                const uint32_t b0 = cb + 4u * lane;
+                float gf[2][4] = {}, gg[2][4] = {};
+                if (b0 < x.nAct) {
+                    gather_v<W16>(a, x.Bin + b0, ldb, rb, gf);
+                    gather_v<W16>(a, x.Bin + b0, ldb, rc, gg);
+                }
+                pair_update4<0>(a, x, s0, s1, b0, coupled, Wf, Wg, gf[0], gf[1], gg[0], gg[1], myAbs, myRel, myAct);
+            }
         }
     }
     if (MODE == MODE_INCIDENT) return;
     band_partials(a, myAbs, myRel, myAct);
 }
 
+// Sums of the two rows of a facet at the 8 BF16 bands of this lane,
+// Hl = Bin16 + b0 (row stride ldh), BF16 weights (W16) or FP32.
+template <bool W16>
+__device__ __forceinline__ void gather_h(const SweepArgs& a, const uint16_t* __restrict__ Hl, uint32_t ldh,
+                                         const uint32_t (&rb)[3], float (&g)[2][8]) {
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t start = rb[side], end = rb[side + 1];
+        for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += kU) {
+            uint32_t n[kU];
+            float w[kU];
+            bool ok[kU];
+            load_group<W16>(a, base, start, end, n, w, ok);
+            uint4 v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                v[u] = ok[u] ? *reinterpret_cast<const uint4*>(Hl + size_t(n[u]) * ldh) : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const uint32_t vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    g[side][2 * j] += w[u] * __uint_as_float(vv[j] << 16);
+                    g[side][2 * j + 1] += w[u] * __uint_as_float(vv[j] & 0xffff0000u);
+                }
+            }
+        }
+    }
+}
+
+// Sums of the two rows of a facet at 4 BF16 bands of this lane (8-byte
+// loads), FP32 weights: the pair blocks' increment sweeps, which hold two
+// facets' state per lane and so run fewer waves per SIMD: each lane keeps
+// 8 G entries in flight (G aligned groups; the CSR padding of kCsrPad = 16
+// entries covers the reads past a row's end).
+template <int G>
+__device__ __forceinline__ void gather_h4(const SweepArgs& a, const uint16_t* __restrict__ Hl, uint32_t ldh,
+                                          const uint32_t (&rb)[3], float (&g)[2][4]) {
+    static_assert(G * kU <= int(kCsrPad) + 1, "reads past the CSR padding");
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const uint32_t start = rb[side], end = rb[side + 1];
+        for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += uint32_t(G * kU)) {
+            uint32_t n[G][kU];
+            float w[G][kU];
+            bool ok[G][kU];
+#pragma unroll
+            for (int q = 0; q < G; ++q) load_group<false>(a, base + uint32_t(q * kU), start, end, n[q], w[q], ok[q]);
+            uint2 v[G][kU];
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    v[q][u] = ok[q][u] ? *reinterpret_cast<const uint2*>(Hl + size_t(n[q][u]) * ldh) : make_uint2(0u, 0u);
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const uint32_t vv[2] = {v[q][u].x, v[q][u].y};
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        g[side][2 * j] += w[q][u] * __uint_as_float(vv[j] << 16);
+                        g[side][2 * j + 1] += w[q][u] * __uint_as_float(vv[j] & 0xffff0000u);
+                    }
+                }
+        }
+    }
+}
+
 // BF16 sweep: gathers the BF16 copy Bin16 (row stride ldh, a multiple of 64
 // bands) 8 bands per lane with BF16 weights; accumulates, updates and writes
 // B in FP32 and refreshes Bout16.  INC = 1 / 2: base / increment sweep of the
-// increment scheme (band_update_inc4), FP32 weights, Bin16 / Bout16 hold the
-// BF16 increments.
-template <int INC>
+// increment scheme (upd_begin), FP32 weights, Bin16 / Bout16 hold the BF16
+// increments; PAIR (INC 1 / 2): team i updates the two facets of units[i]
+// with the pair block solve, 4 bands per lane.
+template <int INC, bool PAIR>
 __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
     BandCtx x;
     if (!band_ctx<MODE_SWEEP>(a, INC == 0, x)) return;
@@ -692,55 +917,61 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
     const uint32_t team = threadIdx.x / bt, lane = threadIdx.x - team * bt;
     const uint32_t fl = xcd_block(a.xcd) * tpb + team;
     float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
-    if (team < tpb && fl < a.F) {
-        const uint32_t f = a.sched ? a.sched[fl] : fl;
+    if (team < tpb && fl < (PAIR ? a.nUnits : a.F)) {
+        uint32_t f, h = kNoFacet;
+        if (PAIR) {
+            const uint2 un = a.units[fl];
+            f = un.x;
+            h = un.y;
+        } else {
+            f = a.sched ? a.sched[fl] : fl;
+        }
         const uint32_t s0 = 2u * f;
         const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
-        for (uint32_t cb = 0; cb < x.nPrev; cb += 8u * bt) {
-            const uint32_t b0 = cb + 8u * lane;
-            float g[2][8] = {};
-            if (b0 < x.nAct) {
-                const uint16_t* __restrict__ Hl = x.Bin16 + b0;
+        if (!PAIR) {
+            for (uint32_t cb = 0; cb < x.nPrev; cb += 8u * bt) {
+                const uint32_t b0 = cb + 8u * lane;
+                float g[2][8] = {};
+                if (b0 < x.nAct) gather_h<INC == 0>(a, x.Bin16 + b0, ldh, rb, g);
+                // two groups of 4 bands (the second may lie beyond ldb)
+                float lo0[4], lo1[4], hi0[4], hi1[4];
 #pragma unroll
-                for (int side = 0; side < 2; ++side) {
-                    const uint32_t start = rb[side], end = rb[side + 1];
-                    for (uint32_t base = start & ~uint32_t(kU - 1); base < end; base += kU) {
-                        uint32_t n[kU];
-                        float w[kU];
-                        bool ok[kU];
-                        load_group<INC == 0>(a, base, start, end, n, w, ok);
-                        uint4 v[kU];
-#pragma unroll
-                        for (int u = 0; u < kU; ++u)
-                            v[u] = ok[u] ? *reinterpret_cast<const uint4*>(Hl + size_t(n[u]) * ldh)
-                                         : make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-                        for (int u = 0; u < kU; ++u) {
-                            const uint32_t vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                g[side][2 * j] += w[u] * __uint_as_float(vv[j] << 16);
-                                g[side][2 * j + 1] += w[u] * __uint_as_float(vv[j] & 0xffff0000u);
-                            }
-                        }
-                    }
+                for (int j = 0; j < 4; ++j) {
+                    lo0[j] = g[0][j];
+                    lo1[j] = g[1][j];
+                    hi0[j] = g[0][4 + j];
+                    hi1[j] = g[1][4 + j];
+                }
+                if constexpr (INC == 0) {
+                    band_update4<MODE_SWEEP>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
+                    if (b0 + 4u < a.ldb) band_update4<MODE_SWEEP>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
+                } else {
+                    band_update_inc4<INC == 1>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
+                    if (b0 + 4u < a.ldb) band_update_inc4<INC == 1>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
                 }
             }
-            // two groups of 4 bands (the second may lie beyond ldb)
-            float lo0[4], lo1[4], hi0[4], hi1[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                lo0[j] = g[0][j];
-                lo1[j] = g[1][j];
-                hi0[j] = g[0][4 + j];
-                hi1[j] = g[1][4 + j];
-            }
-            if constexpr (INC == 0) {
-                band_update4<MODE_SWEEP>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
-                if (b0 + 4u < a.ldb) band_update4<MODE_SWEEP>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
-            } else {
-                band_update_inc4<INC == 1>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
-                if (b0 + 4u < a.ldb) band_update_inc4<INC == 1>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
+        } else {
+            // 4 bands per lane; a unit without a second facet (odd facet
+            // count) updates its facet alone
+            const uint32_t s1 = 2u * (h == kNoFacet ? f : h);
+            const uint32_t rc[3] = {a.rowptr[s1], a.rowptr[s1 + 1], a.rowptr[s1 + 2]};
+            const float4 Wf = a.unitW[2u * fl], Wg = a.unitW[2u * fl + 1u];
+            const bool coupled = Wf.x + Wf.y + Wf.z + Wf.w + Wg.x + Wg.y + Wg.z + Wg.w > 0.0f;
+            for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
+                const uint32_t b0 = cb + 4u * lane;
+                float gf[2][4] = {}, gg[2][4] = {};
+                if (b0 < x.nAct) {
+                    gather_h4<kPairGroups>(a, x.Bin16 + b0, ldh, rb, gf);
+                    if (h != kNoFacet) gather_h4<kPairGroups>(a, x.Bin16 + b0, ldh, rc, gg);
+                }
+                if (h == kNoFacet) {
+                    Upd4 u;
+                    if (upd_begin<INC>(a, x, s0, b0, gf[0], gf[1], u))
+                        upd_end<INC>(a, x, s0, b0, u, u.n0, u.n1, myAbs, myRel, myAct);
+                } else {
+                    pair_update4<INC>(a, x, s0, s1, b0, coupled, Wf, Wg, gf[0], gf[1], gg[0], gg[1], myAbs, myRel,
+                                      myAct);
+                }
             }
         }
     }
@@ -992,6 +1223,172 @@ bool increment_solve(uint32_t ldb, uint32_t precision) {
 
 const uint32_t* sweep_schedule(const fh_context* c) { return use_sched() && c->sched.p ? c->sched.p : nullptr; }
 
+// ---- pair blocks ------------------------------------------------------------
+// Leaves of a crown often come in close, facing pairs whose mutual exchange
+// is the slowest mode of the Jacobi sweep (eigenvalues +-(r + t) F of the
+// pair).  The band-parallel sweeps update two facets per team and solve the
+// CSR entries between the two exactly (a block-Jacobi sweep with 4-side
+// blocks: same fixed point, same Jacobi-residual stopping rule; measured on
+// a 70 k-facet terrain forest at 47 bands: 8 -> 6 Chebyshev sweeps for the
+// slowest band).  FH_PAIRS=0 keeps one facet per team (tuning knob).
+// A facet is matched only with a partner that takes at least this share of
+// one of its rows' weight (terrain forest, 1.0 M facets, 47 bands: 4 % of
+// the facets matched, 138 -> 118 shortwave sweeps; all mutual best
+// partners, 57 % of the facets: 116).
+#ifndef FH_PAIR_MIN_WEIGHT
+#define FH_PAIR_MIN_WEIGHT 0.2f
+#endif
+constexpr float kPairMinWeight = FH_PAIR_MIN_WEIGHT;
+
+static bool use_pairs() {
+    static const bool v = [] {
+        const char* e = std::getenv("FH_PAIRS");
+        return e ? std::atoi(e) != 0 : true;
+    }();
+    return v;
+}
+
+
+// Strongest unmatched partner of each unmatched facet: the facet whose two
+// sides collect the largest weight sum in one of f's rows, if that sum is
+// at least minStrength (entries of one column facet are adjacent in a
+// sorted row; an unsorted row only splits the sums).  Ties go to the
+// smaller index.
+__global__ void k_pair_best(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                            const float* __restrict__ w, const uint32_t* __restrict__ pairOf, uint32_t F,
+                            float minStrength, uint32_t* __restrict__ best) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    if (pairOf[f] != kNoFacet) {
+        best[f] = kNoFacet;
+        return;
+    }
+    float bw = 0.0f;
+    uint32_t bh = kNoFacet;
+    auto consider = [&](uint32_t h, float s) {
+        if (h == kNoFacet || h == f || pairOf[h] != kNoFacet || s < minStrength) return;
+        if (s > bw || (s == bw && s > 0.0f && h < bh)) {
+            bw = s;
+            bh = h;
+        }
+    };
+    for (uint32_t s = 2u * f; s < 2u * f + 2u; ++s) {
+        uint32_t cur = kNoFacet;
+        float sum = 0.0f;
+        for (uint32_t e = rowptr[s], e1 = rowptr[s + 1]; e < e1; ++e) {
+            const uint32_t h = col[e] >> 1;
+            if (h != cur) {
+                consider(cur, sum);
+                cur = h;
+                sum = 0.0f;
+            }
+            sum += w[e];
+        }
+        consider(cur, sum);
+    }
+    best[f] = bh;
+}
+
+// mutual choices become pairs (symmetric: f -> h exactly when h -> f)
+__global__ void k_pair_match(const uint32_t* __restrict__ best, uint32_t F, uint32_t* __restrict__ pairOf) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F || pairOf[f] != kNoFacet) return;
+    const uint32_t h = best[f];
+    if (h != kNoFacet && best[h] == f) pairOf[f] = h;
+}
+
+// weights of the entries of facet a's rows whose column is a side of facet
+// b: (a0 -> b0, a0 -> b1, a1 -> b0, a1 -> b1), duplicates summed
+__device__ float4 block_weights(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                const float* __restrict__ w, uint32_t a, uint32_t b) {
+    float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t q = 0; q < 2u; ++q) {
+        const uint32_t s = 2u * a + q;
+        for (uint32_t e = rowptr[s], e1 = rowptr[s + 1]; e < e1; ++e) {
+            const uint32_t c = col[e];
+            if ((c >> 1) == b) v[2u * q + (c & 1u)] += w[e];
+        }
+    }
+    return make_float4(v[0], v[1], v[2], v[3]);
+}
+
+__global__ void k_unit_weights(const uint32_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                               const float* __restrict__ w, const uint2* __restrict__ units, uint32_t U,
+                               float4* __restrict__ unitW) {
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= U) return;
+    const uint2 p = units[u];
+    const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    unitW[2u * u] = p.y == kNoFacet ? z : block_weights(rowptr, col, w, p.x, p.y);
+    unitW[2u * u + 1u] = p.y == kNoFacet ? z : block_weights(rowptr, col, w, p.y, p.x);
+}
+
+void build_pairs(fh_context* c) {
+    c->units.release();
+    c->unitW.release();
+    c->singles.release();
+    c->unitCount = 0u;
+    c->singleCount = 0u;
+    const uint32_t F = c->F;
+    if (!use_pairs() || F < 2u) return;
+    DBuf<uint32_t> pairOf, best;
+    pairOf.ensure(F);
+    best.ensure(F);
+    launch_fill_u32(c, pairOf.p, F, kNoFacet);
+    const uint32_t grid = (F + 255u) / 256u;
+    for (int round = 0; round < 3; ++round) {  // later rounds match what earlier choices left
+        hipLaunchKernelGGL(k_pair_best, dim3(grid), dim3(256), 0, c->stream, c->rowptr.p, c->col.p, c->val32.p,
+                           pairOf.p, F, kPairMinWeight, best.p);
+        hipLaunchKernelGGL(k_pair_match, dim3(grid), dim3(256), 0, c->stream, best.p, F, pairOf.p);
+        FH_HIP(hipGetLastError());
+    }
+    std::vector<uint32_t> partner(F), order(F);
+    FH_HIP(hipMemcpyAsync(partner.data(), pairOf.p, sizeof(uint32_t) * F, hipMemcpyDeviceToHost, c->stream));
+    const uint32_t* sched = sweep_schedule(c);
+    if (sched)
+        FH_HIP(hipMemcpyAsync(order.data(), sched, sizeof(uint32_t) * F, hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipStreamSynchronize(c->stream));
+    if (!sched)
+        for (uint32_t f = 0; f < F; ++f) order[f] = f;
+    // in sweep order: a matched facet and its partner form a unit (at the
+    // position of the first of the two), every other facet stays single
+    std::vector<uint2> units;
+    std::vector<uint32_t> singles;
+    singles.reserve(F);
+    std::vector<uint8_t> placed(F, 0u);
+    for (uint32_t i = 0; i < F; ++i) {
+        const uint32_t f = order[i];
+        if (placed[f]) continue;
+        placed[f] = 1u;
+        const uint32_t p = partner[f];
+        if (p != kNoFacet && p < F && !placed[p] && partner[p] == f) {
+            placed[p] = 1u;
+            units.push_back(make_uint2(f, p));
+        } else {
+            singles.push_back(f);
+        }
+    }
+    const uint32_t U = uint32_t(units.size()), nS = uint32_t(singles.size());
+    if (U == 0u) return;
+    c->units.ensure(U);
+    c->unitW.ensure(2u * size_t(U));
+    c->singles.ensure(std::max(nS, 1u));
+    FH_HIP(hipMemcpyAsync(c->units.p, units.data(), sizeof(uint2) * U, hipMemcpyHostToDevice, c->stream));
+    if (nS)
+        FH_HIP(hipMemcpyAsync(c->singles.p, singles.data(), sizeof(uint32_t) * nS, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_unit_weights, dim3((U + 255u) / 256u), dim3(256), 0, c->stream, c->rowptr.p, c->col.p,
+                       c->val32.p, c->units.p, U, c->unitW.p);
+    FH_HIP(hipGetLastError());
+    FH_HIP(hipStreamSynchronize(c->stream));  // the host vectors back the copies
+    if (!c->pairStream) {
+        FH_HIP(hipStreamCreate(&c->pairStream));
+        FH_HIP(hipEventCreateWithFlags(&c->pairFork, hipEventDisableTiming));
+        FH_HIP(hipEventCreateWithFlags(&c->pairJoin, hipEventDisableTiming));
+    }
+    c->unitCount = U;
+    c->singleCount = nS;
+}
+
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.xcd = xcd_remap();
     a.sched = sweep_schedule(c);
@@ -1004,7 +1401,6 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         const bool half = (mode == MODE_SWEEP && w16 && a.h0) || a.inc;
         const uint32_t vw = half ? 8u : 4u;
         a.bt = std::min((a.ldb + vw - 1u) / vw, 64u / vw);
-        const uint32_t tpb = 256u / a.bt;
         a.F = c->F;
         a.rowptr = c->rowptr.p;
         a.col = c->col.p;
@@ -1012,27 +1408,61 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         a.w16 = c->val16.p;
         a.fsky = c->fsky.p;
         a.ctrl = c->ctrl.p;
-        const uint32_t g = std::max(1u, (c->F + tpb - 1u) / tpb);
-        c->partial.ensure(g);
-        a.partial = c->partial.p;
+        // pair blocks (FP32-weight and increment sweeps): the single facets
+        // in one launch, the coupled pairs in a second one (4 bands per
+        // lane) on the pair stream beside it (fork / join by events), each
+        // writing its own range of the per-block partials
+        const bool pair =
+            c->unitCount && use_pairs() && !a.jacobiOnly && (a.inc || (mode == MODE_SWEEP && !w16 && !half));
+        SweepArgs p = a;
+        uint32_t g = std::max(1u, (c->F + 256u / a.bt - 1u) / (256u / a.bt)), gp = 0u;
+        if (pair) {
+            a.sched = c->singles.p;
+            a.F = c->singleCount;
+            g = (a.F + 256u / a.bt - 1u) / (256u / a.bt);
+            p.bt = std::min((p.ldb + 3u) / 4u, 16u);
+            p.units = c->units.p;
+            p.unitW = c->unitW.p;
+            p.nUnits = c->unitCount;
+            gp = (p.nUnits + 256u / p.bt - 1u) / (256u / p.bt);
+            p.partOff = g;
+        }
+        const uint32_t parts = std::max(1u, g + gp);
+        c->partial.ensure(parts);
+        a.partial = p.partial = c->partial.p;
         if (a.retire) {
-            c->partAct.ensure(g);
-            a.partAct = c->partAct.p;
+            c->partAct.ensure(parts);
+            a.partAct = p.partAct = c->partAct.p;
         } else {
-            a.partAct = nullptr;
+            a.partAct = p.partAct = nullptr;
         }
         hipStream_t st = c->stream;
-        if (a.inc == 1u) hipLaunchKernelGGL(k_sweep_hv<1>, dim3(g), dim3(256), 0, st, a);
-        else if (a.inc) hipLaunchKernelGGL(k_sweep_hv<2>, dim3(g), dim3(256), 0, st, a);
-        else if (half) hipLaunchKernelGGL(k_sweep_hv<0>, dim3(g), dim3(256), 0, st, a);
-        else if (mode == MODE_SWEEP && !w16) hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, false>), dim3(g), dim3(256), 0, st, a);
-        else if (mode == MODE_SWEEP) hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, true>), dim3(g), dim3(256), 0, st, a);
-        else if (mode == MODE_REFRESH) fail(FH_RUNTIME_ERROR, "refresh sweeps run on the entry-parallel layouts only");
-        else hipLaunchKernelGGL((k_sweep_v<MODE_INCIDENT, false>), dim3(g), dim3(256), 0, st, a);
+        if (gp) {
+            hipStream_t ps = c->pairStream;
+            FH_HIP(hipEventRecord(c->pairFork, st));
+            FH_HIP(hipStreamWaitEvent(ps, c->pairFork, 0));
+            if (p.inc == 1u) hipLaunchKernelGGL((k_sweep_hv<1, true>), dim3(gp), dim3(256), 0, ps, p);
+            else if (p.inc) hipLaunchKernelGGL((k_sweep_hv<2, true>), dim3(gp), dim3(256), 0, ps, p);
+            else hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, false, true>), dim3(gp), dim3(256), 0, ps, p);
+            FH_HIP(hipEventRecord(c->pairJoin, ps));
+        }
+        if (g) {
+            if (a.inc == 1u) hipLaunchKernelGGL((k_sweep_hv<1, false>), dim3(g), dim3(256), 0, st, a);
+            else if (a.inc) hipLaunchKernelGGL((k_sweep_hv<2, false>), dim3(g), dim3(256), 0, st, a);
+            else if (half) hipLaunchKernelGGL((k_sweep_hv<0, false>), dim3(g), dim3(256), 0, st, a);
+            else if (mode == MODE_SWEEP && !w16)
+                hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, false, false>), dim3(g), dim3(256), 0, st, a);
+            else if (mode == MODE_SWEEP)
+                hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, true, false>), dim3(g), dim3(256), 0, st, a);
+            else if (mode == MODE_REFRESH) fail(FH_RUNTIME_ERROR, "refresh sweeps run on the entry-parallel layouts only");
+            else hipLaunchKernelGGL((k_sweep_v<MODE_INCIDENT, false, false>), dim3(g), dim3(256), 0, st, a);
+        }
+        if (gp) FH_HIP(hipStreamWaitEvent(st, c->pairJoin, 0));
         if (mode != MODE_INCIDENT && !a.standalone)
-            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, a.partAct, g,
-                               a.judge, uint32_t(mode == MODE_REFRESH), uint32_t(mode == MODE_SWEEP && w16 && !a.inc),
-                               a.rho2, const_cast<float*>(a.omega), a.ldb);
+            hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p, a.partAct,
+                               g + gp, a.judge, uint32_t(mode == MODE_REFRESH),
+                               uint32_t(mode == MODE_SWEEP && w16 && !a.inc), a.rho2, const_cast<float*>(a.omega),
+                               a.ldb);
         FH_HIP(hipGetLastError());
         return;
     }
@@ -1359,7 +1789,8 @@ void estimate_rho2(fh_context* c, uint32_t ldb, uint32_t nb, const float* rr, co
     // v0); FH_RHO_SWEEPS overrides m (tuning knob)
     const int m = std::max(2, int(env_float("FH_RHO_SWEEPS", 8.0f))) & ~1;
     for (int k = 0; k < m + 2; ++k) {
-        launch_single_sweep(c, ldb, zero, rr, tt, (k & 1) ? v1 : v0, (k & 1) ? v0 : v1, false);
+        launch_single_sweep(c, ldb, zero, rr, tt, (k & 1) ? v1 : v0, (k & 1) ? v0 : v1, false, nullptr, nullptr,
+                            nullptr, nullptr, true);  // the operator of the solves: pair blocks
         if (k == m - 1 || k == m + 1) {
             hipLaunchKernelGGL(k_band_max, dim3(512), dim3(256), 0, c->stream, v0, c->S, ldb,
                                c->bandMax.p + (k == m - 1 ? 0u : ldb));
@@ -1396,8 +1827,9 @@ void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, f
 
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
                          const float* Bin, float* Bout, bool w16, uint16_t* h16in, uint16_t* h16out, float* gacc,
-                         float* carry) {
+                         float* carry, bool pairs) {
     SweepArgs a{};
+    a.jacobiOnly = pairs ? 0u : 1u;
     if (w16 && h16in && gacc && carry && band_parallel(ldb)) {
         a.inc = 2u;
         a.gacc = gacc;

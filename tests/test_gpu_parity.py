@@ -372,6 +372,55 @@ def test_surrogate_warm_start_gain():
     assert sum(s1.coupling) <= sum(s0.coupling)
 
 
+def _facing_pairs_scene(pairs=120, gap=0.02, side=0.3, seed=11):
+    """Leaves in parallel, facing pairs `gap` apart over a ground patch: the
+    inner sides of a pair see each other with weights near 0.8."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform([0.5, 0.6, 0.5], [3.5, 2.0, 3.5], (pairs, 3))
+    n = scenes.sphere_normals(pairs, np.random.default_rng(seed + 1))
+    a = scenes.equilateral_leaves(c, n, side, np.random.default_rng(seed + 2))
+    b = scenes.equilateral_leaves(c + gap * n, n, side, np.random.default_rng(seed + 2))  # same rotation
+    leaves = np.concatenate([a, b])
+    return scenes._finish("pairs", leaves, scenes.grid_ground(0.0, 4.0, 0.0, 4.0, 8, 8))
+
+
+@pytest.mark.parametrize("nb", [24, 47])
+def test_pair_blocks_match_oracle(oracle, nb):
+    # strongly coupled facet pairs are solved as 2-facet blocks inside the
+    # band-parallel sweeps: same fixed point as the oracle's Jacobi solve,
+    # far fewer sweeps than Jacobi needs for the pairs' slow modes
+    print(f"pair blocks nb={nb}", flush=True)
+    sc = _facing_pairs_scene()
+    eng = _engine(R=128, seed=5)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    g = eng.getGraph()
+    og = _as_oracle_graph(oracle, g)
+    # precondition: most leaves have a partner facet holding > 0.2 of a
+    # side's row (the inner sides of the pairs)
+    S, F = sc.surface_count, sc.facet_count
+    rows = np.repeat(np.arange(S), np.diff(g.rowOffsets.astype(np.int64)))
+    w = g.counts / np.maximum(g.denominators[rows], 1)
+    key = rows * F + g.neighbors // 2
+    uk, inv = np.unique(key, return_inverse=True)
+    strongest = np.zeros(S)
+    np.maximum.at(strongest, uk // F, np.bincount(inv, weights=w))
+    assert np.mean(strongest.reshape(-1, 2).max(axis=1)[: sc.leaf_count] > 0.2) > 0.8
+    rng = np.random.default_rng(nb)
+    r = rng.uniform(0.35, 0.5, (S, nb)).astype(np.float32)
+    t = (0.95 - r) * rng.uniform(0.8, 1.0, (S, nb)).astype(np.float32)
+    e = rng.uniform(0.0, 5.0, (S, nb)).astype(np.float32)
+    D = rng.uniform(0.0, 300.0, (S, nb)).astype(np.float32)
+    sky = rng.uniform(10.0, 80.0, nb).astype(np.float32)
+    ref, _, _ = oracle.solve_bands(og, r, t, e, D, sky, tol_rel=1e-9)
+    _, it_jacobi, _ = oracle.solve_bands(og, r, t, e, D, sky, tol_rel=1e-7)
+    for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC):
+        res = eng.solveBands(r, t, e, D, sky, tolRel=1e-7, precision=prec)
+        print(f"  precision {prec}: {res.iterations} sweeps, Jacobi {it_jacobi}", flush=True)
+        assert _rel(res.radiosity, ref) < RTOL, prec
+        assert res.iterations < 0.85 * it_jacobi, prec
+
+
 @pytest.mark.parametrize("nb", [5, 12, 32, 47, 77])
 def test_many_bands_match_oracle(oracle, nb):
     # every band layout of the SpMM kernels (band stride 8, 12, 32, 48 and a
