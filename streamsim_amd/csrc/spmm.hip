@@ -725,19 +725,12 @@ __device__ __forceinline__ void pair_end(const SweepArgs& a, const BandCtx& x, u
 }
 
 // Update of a unit's two facets (side pairs s0 and s1) at 4 bands; INC as
-// upd_begin.  Units whose facets have no entries between them update both
-// facets as single ones.
+// upd_begin.
 template <int INC>
 __device__ __forceinline__ void pair_update4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t s1,
-                                             uint32_t b0, bool coupled, const float4 Wf, const float4 Wg,
-                                             float (&gf0)[4], float (&gf1)[4], float (&gg0)[4], float (&gg1)[4],
-                                             float& myAbs, float& myRel, float& myAct) {
-    if (!coupled) {
-        Upd4 u;
-        if (upd_begin<INC>(a, x, s0, b0, gf0, gf1, u)) upd_end<INC>(a, x, s0, b0, u, u.n0, u.n1, myAbs, myRel, myAct);
-        if (upd_begin<INC>(a, x, s1, b0, gg0, gg1, u)) upd_end<INC>(a, x, s1, b0, u, u.n0, u.n1, myAbs, myRel, myAct);
-        return;
-    }
+                                             uint32_t b0, const float4 Wf, const float4 Wg, float (&gf0)[4],
+                                             float (&gf1)[4], float (&gg0)[4], float (&gg1)[4], float& myAbs,
+                                             float& myRel, float& myAct) {
     Pair4 pf, pg;
     const bool okf = pair_begin<INC>(a, x, s0, b0, gf0, gf1, Wf, pf);
     const bool okg = pair_begin<INC>(a, x, s1, b0, gg0, gg1, Wg, pg);  // same bands: same outcome as f
@@ -807,7 +800,7 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
         }
         const uint32_t s0 = 2u * f;
         const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
-        if (!PAIR || h == kNoFacet) {
+        if (!PAIR) {
             for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
                 const uint32_t b0 = cb + 4u * lane;
                 float g[2][4] = {};
@@ -818,7 +811,6 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
             const uint32_t s1 = 2u * h;
             const uint32_t rc[3] = {a.rowptr[s1], a.rowptr[s1 + 1], a.rowptr[s1 + 2]};
             const float4 Wf = a.unitW[2u * fl], Wg = a.unitW[2u * fl + 1u];
-            const bool coupled = Wf.x + Wf.y + Wf.z + Wf.w + Wg.x + Wg.y + Wg.z + Wg.w > 0.0f;
             for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
                 const uint32_t b0 = cb + 4u * lane;
                 float gf[2][4] = {}, gg[2][4] = {};
@@ -826,7 +818,7 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
                     gather_v<W16>(a, x.Bin + b0, ldb, rb, gf);
                     gather_v<W16>(a, x.Bin + b0, ldb, rc, gg);
                 }
-                pair_update4<0>(a, x, s0, s1, b0, coupled, Wf, Wg, gf[0], gf[1], gg[0], gg[1], myAbs, myRel, myAct);
+                pair_update4<0>(a, x, s0, s1, b0, Wf, Wg, gf[0], gf[1], gg[0], gg[1], myAbs, myRel, myAct);
             }
         }
     }
@@ -951,27 +943,18 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
                 }
             }
         } else {
-            // 4 bands per lane; a unit without a second facet (odd facet
-            // count) updates its facet alone
-            const uint32_t s1 = 2u * (h == kNoFacet ? f : h);
+            // 4 bands per lane
+            const uint32_t s1 = 2u * h;
             const uint32_t rc[3] = {a.rowptr[s1], a.rowptr[s1 + 1], a.rowptr[s1 + 2]};
             const float4 Wf = a.unitW[2u * fl], Wg = a.unitW[2u * fl + 1u];
-            const bool coupled = Wf.x + Wf.y + Wf.z + Wf.w + Wg.x + Wg.y + Wg.z + Wg.w > 0.0f;
             for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
                 const uint32_t b0 = cb + 4u * lane;
                 float gf[2][4] = {}, gg[2][4] = {};
                 if (b0 < x.nAct) {
                     gather_h4<kPairGroups>(a, x.Bin16 + b0, ldh, rb, gf);
-                    if (h != kNoFacet) gather_h4<kPairGroups>(a, x.Bin16 + b0, ldh, rc, gg);
+                    gather_h4<kPairGroups>(a, x.Bin16 + b0, ldh, rc, gg);
                 }
-                if (h == kNoFacet) {
-                    Upd4 u;
-                    if (upd_begin<INC>(a, x, s0, b0, gf[0], gf[1], u))
-                        upd_end<INC>(a, x, s0, b0, u, u.n0, u.n1, myAbs, myRel, myAct);
-                } else {
-                    pair_update4<INC>(a, x, s0, s1, b0, coupled, Wf, Wg, gf[0], gf[1], gg[0], gg[1], myAbs, myRel,
-                                      myAct);
-                }
+                pair_update4<INC>(a, x, s0, s1, b0, Wf, Wg, gf[0], gf[1], gg[0], gg[1], myAbs, myRel, myAct);
             }
         }
     }
