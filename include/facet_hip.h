@@ -257,8 +257,9 @@ typedef struct {
     uint32_t vegetationMethod; /* 0 | 1                                    */
     float radiosityTolerance;  /* relative change, SW and LW solves        */
     uint32_t maxSweeps;        /* per radiosity solve                      */
-    uint32_t precision;        /* fh_precision of the SW solves from 16 windows on (BF16
-                                  increments); below 16 windows and for the LW: FP32 */
+    uint32_t precision;        /* fh_precision of the SW solves from 33 windows on (BF16
+                                  increments; FP32 rows then span more than one 128-byte
+                                  line); up to 32 windows and for the LW: FP32 */
     uint32_t warmStart;        /* fh_warm_start                            */
     uint32_t firstNode;        /* 1: T = airTemperatureK (core:3470-3479)  */
     uint32_t zeroStart;        /* 1: every solve starts from B = 0         */

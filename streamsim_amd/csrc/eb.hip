@@ -868,7 +868,7 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         // same: the [S][ldb] optics arrays and the folded source are only
         // built for the entry-parallel layouts (and the spectral estimate)
         const bool tab = !noTables && uses_bf16_gather(ldb);
-        const uint32_t precision = uses_bf16_gather(ldb) ? p.precision : FH_PRECISION_FP32;
+        const uint32_t precision = bf16_pays(ldb) ? p.precision : FH_PRECISION_FP32;
         // an increment solve keeps the gathers of every band's last sweep:
         // the absorbed pass forms the incident from them
         const bool fromG = tab && increment_solve(ldb, precision);

@@ -81,6 +81,9 @@ void set_sky(fh_context* c, const float* sky, uint32_t nb, uint32_t ldb);
 // true when sweeps of this band stride use the band-parallel kernels (BF16
 // solves then gather BF16 copies of B with row stride bf16_stride(ldb))
 bool uses_bf16_gather(uint32_t ldb);
+// whether the energy balance's BF16 mode runs its shortwave solves with BF16
+// increments at this band stride (FP32 rows longer than one 128-byte line)
+bool bf16_pays(uint32_t ldb);
 uint32_t bf16_stride(uint32_t ldb);
 
 // whether FP32-weight gathers read the packed CSR entries (when present)
