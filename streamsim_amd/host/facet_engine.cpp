@@ -823,6 +823,34 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
                                     matList.data(), nw, swR.data(), swT.data()),
                     ctx);
 
+    // sensor bands solved after every node (core:3862-3908): the project's
+    // sensor bands, else the default 550 nm (core:141, 2389-2400); per
+    // material the band r, t (custom spectra; defaults 0.10 / 0.05 for
+    // objects, 0.20 / 0.0 for the ground) and the thermal r, t
+    struct SensorMaterial {
+        std::vector<float> r, t;
+        float tirR = 0.03f, tirT = 0.0f;
+    };
+    std::vector<SensorMaterial> sensorMats(2);
+    sensorMats[0].r = {0.10f};
+    sensorMats[0].t = {0.05f};
+    sensorMats[1].r = {0.20f};
+    sensorMats[1].t = {0.0f};
+    sensorMats[1].tirR = 0.05f;
+    std::vector<float> sensorBands{550.0f};
+    if (project) {
+        sensorBands = prj.run.bands;
+        sensorMats.assign(prj.materials.size(), SensorMaterial{});
+        for (size_t m = 0; m < prj.materials.size(); ++m) {
+            const facetcli::ProjectMaterial& pm = prj.materials[m];
+            sensorMats[m].r = pm.bandR.empty() ? std::vector<float>{pm.r} : pm.bandR;
+            sensorMats[m].t = pm.bandT.empty() ? std::vector<float>{pm.t} : pm.bandT;
+            sensorMats[m].tirR = pm.tirR;
+            sensorMats[m].tirT = pm.tirT;
+        }
+    }
+    const uint32_t nbs = uint32_t(sensorBands.size());
+
     const std::filesystem::path stepDir = outDir / ".facet_steps";
     std::filesystem::create_directories(stepDir);
     {  // components.bin (core:3418-3429): 1 for ground, the object's output component (0) otherwise
@@ -906,13 +934,76 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         }
         const bool converged = st.converged != 0;
         allConverged = allConverged && converged;
+        // sensor products (core:3862-3908, makeStateSpectralOptics
+        // core:1135-1199 with no reflectance texture and no background
+        // angular strength): per band the optics at the node's temperatures,
+        // direct total * 0.5 directRatio * 0.001 and diffuse total * 0.5 (1 -
+        // directRatio) * 0.001 of the flat spectrum, sky radiance diffuse / 2
+        // (thermal: Planck at the sky temperature), 64 sweeps; optical bands
+        // as apparent reflectance against direct cos(zenith) + diffuse / 2,
+        // thermal bands as radiance
+        std::vector<float> bandOut(S * nbs, 0.0f);
+        if (nbs) {
+            std::vector<float> br(S * nbs), bt(S * nbs), be(S * nbs, 0.0f), bD(S * nbs, 0.0f), bsky(nbs),
+                dS(nbs, 0.0f), qS(nbs, 0.0f), bB(S * nbs);
+            std::vector<uint8_t> opt(nbs);
+            for (uint32_t b = 0; b < nbs; ++b) {  // isOpticalWavelength (core:1127-1133)
+                opt[b] = (sensorBands[b] <= 2.5f ? sensorBands[b] * 1000.0f : sensorBands[b]) <= 2500.0f;
+                dS[b] = opt[b] ? total * dLevel * 0.001f : 0.0f;
+                qS[b] = opt[b] ? total * qLevel * 0.001f : 0.0f;
+                bsky[b] = opt[b] ? qS[b] * 0.5f : planckRadiance(sensorBands[b], set.skyTemperature);
+            }
+            for (uint32_t f = 0; f < scene.facetCount(); ++f) {
+                const SensorMaterial& sm = sensorMats[std::min<size_t>(scene.material[f], sensorMats.size() - 1)];
+                const bool object = f < scene.leafFacetCount;
+                const auto& vx = scene.vertices[3 * size_t(f)];
+                for (uint32_t ls = 0; ls < 2; ++ls) {
+                    const size_t side = 2 * size_t(f) + ls;
+                    const float sg = ls == 0 ? 1.0f : -1.0f;
+                    const float cosv = std::max(0.0f, sg * vx.normal[0] * sd.x + sg * vx.normal[1] * sd.y +
+                                                          sg * vx.normal[2] * sd.z);
+                    for (uint32_t b = 0; b < nbs; ++b) {
+                        const size_t i = side * nbs + b;
+                        const bool optical = opt[b] != 0;
+                        const float meanR = optical ? sm.r[std::min<size_t>(b, sm.r.size() - 1)] : sm.tirR;
+                        float refl = std::clamp(meanR, 0.0f, 1.0f);
+                        const float trans = std::clamp(optical ? sm.t[std::min<size_t>(b, sm.t.size() - 1)] : sm.tirT,
+                                                       0.0f, 1.0f - refl);
+                        const float eps = std::max(0.0f, 1.0f - refl - trans);
+                        if (!object && optical) refl = std::clamp(refl, 0.0f, 0.999f);  // Hapke at strength 0
+                        else if (!object) refl = std::clamp(1.0f - trans - eps, 0.0f, 1.0f);
+                        br[i] = refl;
+                        bt[i] = std::min(trans, 1.0f - refl);
+                        if (optical) bD[i] = std::clamp(sunlit[side], 0.0f, 1.0f) * cosv * dS[b];
+                        else be[i] = eps * planckRadiance(sensorBands[b], std::clamp(T[side], 1.0f, 2000.0f));
+                    }
+                }
+            }
+            fh_solve_stats bs{};
+            facethip::check(fh_solve_bands(ctx, nbs, br.data(), bt.data(), be.data(), bD.data(), bsky.data(), nullptr,
+                                           64u, -1.0f, 1.0f, FH_PRECISION_FP32, bB.data(), &bs),
+                            ctx);
+            const float czs = std::max(0.0f, sd.y);
+            for (uint32_t b = 0; b < nbs; ++b) {
+                const bool optical = opt[b] != 0;
+                const float refIrr = dS[b] * czs + qS[b] * 0.5f;
+                for (size_t s = 0; s < S; ++s) {
+                    const float v = bB[s * nbs + b];
+                    if (!optical) bandOut[s * nbs + b] = v;
+                    else if (refIrr > 1.0e-8f) bandOut[s * nbs + b] = std::clamp(v / refIrr, 0.0f, 1.0f);
+                }
+            }
+        }
         // .facet_steps/energy_T=<token>.bin (core:3910-3935): per side
-        // [T, B_total, B_SW, B_LW, Rn, H, LE, G, sunlit], no sensor bands
+        // [T, B_total, B_SW, B_LW, Rn, H, LE, G, sunlit] + the sensor bands
         {
-            std::vector<float> rec(S * 9);
+            const size_t rs = 9 + size_t(nbs);
+            std::vector<float> rec(S * rs);
             for (size_t s = 0; s < S; ++s) {
                 const float v[9] = {T[s], Btot[s], Bsw[s], Blw[s], Rn[s], H[s], LE[s], G[s], sunlit[s]};
-                std::copy(v, v + 9, rec.begin() + std::ptrdiff_t(s * 9));
+                std::copy(v, v + 9, rec.begin() + std::ptrdiff_t(s * rs));
+                std::copy(bandOut.begin() + std::ptrdiff_t(s * nbs), bandOut.begin() + std::ptrdiff_t((s + 1) * nbs),
+                          rec.begin() + std::ptrdiff_t(s * rs + 9));
             }
             std::ofstream bin(stepDir / ("energy_T=" + token + ".bin"), std::ios::binary | std::ios::trunc);
             bin.write(reinterpret_cast<const char*>(rec.data()), std::streamsize(rec.size() * sizeof(float)));

@@ -114,7 +114,8 @@ def test_cli_faceteb_nodes_step_outputs_and_backpressure(tmp_path):
     # eFacetEB over three meteorological nodes (morning, noon, night) with the
     # per-node outputs of the reference (core:3410-4082): components.bin,
     # .facet_steps/energy_T=<token>.bin [T, B, B_SW, B_LW, Rn, H, LE, G, sunlit]
-    # per side, process files, faceteb_run.json, and the FACET_STEP /
+    # + the default sensor band (550 nm) per side, process files,
+    # faceteb_run.json, and the FACET_STEP /
     # FACET_ACK back-pressure on stdin / stdout
     print("golden cli eb nodes", flush=True)
     import os
@@ -150,8 +151,9 @@ def test_cli_faceteb_nodes_step_outputs_and_backpressure(tmp_path):
     assert comp[:138].max() == 0 and comp[138:].min() == 1
     recs = []
     for _, tok in steps:
-        r = np.fromfile(d / ".facet_steps" / f"energy_T={tok}.bin", np.float32).reshape(S, 9)
+        r = np.fromfile(d / ".facet_steps" / f"energy_T={tok}.bin", np.float32).reshape(S, 10)
         assert np.all((r[:, 0] > 220) & (r[:, 0] < 360))
+        assert np.all((r[:, 9] >= 0) & (r[:, 9] <= 1))  # 550 nm apparent reflectance
         np.testing.assert_allclose(r[:, 1], r[:, 2] + r[:, 3], rtol=1e-6, atol=1e-3)
         assert np.all((r[:, 8] >= 0) & (r[:, 8] <= 1))
         recs.append(r)
@@ -164,6 +166,7 @@ def test_cli_faceteb_nodes_step_outputs_and_backpressure(tmp_path):
     # noon is sunlit and brighter in the shortwave than the morning; the night node has none
     assert recs[1][:, 8].mean() > 0.2 and recs[1][:, 2].mean() > recs[0][:, 2].mean() > 0
     assert recs[2][:, 8].max() == 0 and recs[2][:, 2].max() == 0
+    assert recs[1][:, 9].mean() > 0.01 and recs[2][:, 9].max() == 0
     idx = json.loads((d / "faceteb_run.json").read_text())
     assert idx["kind"] == "faceteb-run" and idx["status"] == "completed"
     assert [c["node"] for c in idx["completed"]] == [0, 1, 2] and all(c["converged"] for c in idx["completed"])
@@ -185,11 +188,13 @@ def test_cli_faceteb_nodes_step_outputs_and_backpressure(tmp_path):
 
 
 @pytest.mark.gpu
-def test_cli_faceteb_project(tmp_path):
+def test_cli_faceteb_project(oracle, tmp_path):
     # eFacetEB on a project.json: the reference's validation project (trees
     # of a Vegetation object, a Building object, soil ground) with a
     # meteorology block added; materials, components and settings come from
-    # the project (control, meteo), not from the environment
+    # the project (control, meteo), not from the environment.  The step files
+    # carry the project's sensor bands (550, 850, 10500 nm) after the nine
+    # energy values, checked at noon against the oracle (core:3862-3908)
     print("golden cli eb project", flush=True)
     import os
     import shutil
@@ -203,6 +208,7 @@ def test_cli_faceteb_project(tmp_path):
     c["meteo"] = {"path": "meteo.txt", "start": 0, "end": 2, "dTime": 3600, "latitude": 40, "longitude": 116}
     c.setdefault("control", {}).update({"temperatureTolerance": 0.01, "couplingIterations": 60,
                                         "spectralAccelerationWidth": 250})
+    c["sensor"]["energyProcess"] = True  # process metadata with the node's sun angles
     (prj / "project.json").write_text(json.dumps(d))
     env = dict(os.environ, FACET_RAYS_PER_SIDE="64", FACET_SUN_RAYS_PER_SIDE="16")
     for k in ("FACET_EB_METEO", "FACET_EB_START", "FACET_EB_END", "STREAMSIM_FACET_STEP_SYNC"):
@@ -218,13 +224,82 @@ def test_cli_faceteb_project(tmp_path):
     comp = np.fromfile(tmp_path / "eb" / ".facet_steps" / "components.bin", np.uint8)
     # 7 trees of 18 facets (Vegetation: 2), the building's 12 (Building: 3), ground 1
     assert np.all(comp[:126] == 2) and np.all(comp[126:138] == 3) and np.all(comp[138:] == 1)
-    noon = np.fromfile(tmp_path / "eb" / ".facet_steps" / "energy_T=DOY172_12-00.bin", np.float32).reshape(-1, 9)
+    noon = np.fromfile(tmp_path / "eb" / ".facet_steps" / "energy_T=DOY172_12-00.bin", np.float32).reshape(-1, 12)
     T = noon[:, 0]
     assert np.all((T > 220) & (T < 360))
     # latent heat: the leaves (leaf_c3, Vegetation) transpire, the building
     # (soil_dry, not Vegetation) does not
     LE = noon[:, 6]
     assert np.all(LE[:2 * 126].reshape(-1, 2)[:, 0] > 0) and np.all(LE[2 * 126:2 * 138] == 0)
+    night = np.fromfile(tmp_path / "eb" / ".facet_steps" / "energy_T=DOY172_21-00.bin", np.float32).reshape(-1, 12)
+    assert np.all(night[:, 9:11] == 0) and np.all(night[:, 11] > 0)  # no light; thermal radiance
+    # the noon sensor bands against the oracle: makeStateSpectralOptics with
+    # the node's temperatures and sunlit fractions, the flat spectrum's
+    # direct / diffuse levels 0.5 * 0.8 and 0.5 * 0.2 per micrometre of the
+    # 850 W m-2 total, 64 FP64 Jacobi sweeps per band, optical bands as
+    # apparent reflectance against direct cos(zenith) + diffuse / 2
+    from streamsim_amd.optics import planck_radiance
+    v, nrm, mat = _project_scene(exe, prj / "project.json", tmp_path)
+    F, S = 4138, 2 * 4138
+    g = oracle.ray_graph(v, 1, 64)
+    meta = json.loads((tmp_path / "eb" / "process" / "faceteb_T=DOY172_12-00.json").read_text())
+    sun = oracle.sun_direction(meta["solarZenith"], meta["solarAzimuth"])
+    sl = noon[:, 8].astype(np.float64)
+    sn = np.repeat(nrm.astype(np.float64), 2, axis=0) * np.tile([1.0, -1.0], F)[:, None]
+    cos = np.maximum(0.0, sn @ np.asarray(sun, np.float64))
+    p = np.array([VALIDATION_MATERIALS[m] for m in np.repeat(mat, 2)])
+    ground = np.repeat(np.arange(F) >= 138, 2)
+    dS, qS = 850.0 * 0.4 * 0.001, 850.0 * 0.1 * 0.001
+    R = np.zeros((S, 3)); Tt = np.zeros((S, 3)); E = np.zeros((S, 3)); D = np.zeros((S, 3))
+    for b in range(2):
+        R[:, b] = np.where(ground, np.minimum(p[:, 0], 0.999), p[:, 0])
+        Tt[:, b] = np.minimum(p[:, 1], 1 - p[:, 0])
+        D[:, b] = np.clip(sl, 0, 1) * cos * dS
+    R[:, 2] = p[:, 2]
+    Tt[:, 2] = p[:, 3]
+    eps = np.maximum(0.0, 1 - R[:, 2] - Tt[:, 2])
+    E[:, 2] = eps * np.array([planck_radiance(10500, x) for x in np.clip(T.astype(np.float64), 1, 2000)])
+    sky = [qS * 0.5, qS * 0.5, planck_radiance(10500, 250.0)]
+    B, _, _ = oracle.solve_bands(g, R, Tt, E, D, sky, max_iter=64, tol_rel=-1.0)
+    ref = dS * max(0.0, float(sun[1])) + qS * 0.5
+    want = np.concatenate([np.clip(B[:, :2] / ref, 0, 1), B[:, 2:]], axis=1)
+    got = noon[:, 9:12].astype(np.float64)
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-3)) < 1e-3
+
+
+# material -> (r, t at 550 / 850 nm, ref_TIR, tau_TIR, T_sunlit, T_shaded) of
+# the validation project's four scene materials: trees green_leaf (two
+# listed values for three bands: the first, as interpolateValue does) with
+# vegetation_temperature; the building's wall and roof concrete with
+# building_temperature; ground soil, soil_temperature
+VALIDATION_MATERIALS = {0: (0.08, 0.04, 0.03, 0.0, 300.0, 296.0), 1: (0.35, 0.0, 0.08, 0.0, 310.0, 300.0),
+                        2: (0.35, 0.0, 0.08, 0.0, 310.0, 300.0), 3: (0.20, 0.0, 0.05, 0.0, 305.0, 295.0)}
+
+
+def _project_scene(exe, prj, tmp_path):
+    """The assembled scene of a project (facet_engine --scene-obj): vertices
+    with the facet normal as appendTriangle computes it (float32 cross(b - a,
+    c - a) times 1 / sqrt of its squared length), normals, material per facet."""
+    sobj = tmp_path / "scene.obj"
+    subprocess.run([str(exe), "--scene-obj", str(prj), str(sobj)], check=True, capture_output=True)
+    verts, mat, cur = [], [], None
+    for line in open(sobj):
+        t = line.split()
+        if t and t[0] == "v":
+            verts.append([float(x) for x in t[1:4]])
+        elif t and t[0] == "usemtl":
+            cur = int(t[1][1:])
+        elif t and t[0] == "f":
+            mat.append(cur)
+    pos = np.array(verts, np.float32)
+    a, b, c = pos[0::3], pos[1::3], pos[2::3]
+    u, w = b - a, c - a
+    nrm = np.stack([u[:, 1] * w[:, 2] - u[:, 2] * w[:, 1], u[:, 2] * w[:, 0] - u[:, 0] * w[:, 2],
+                    u[:, 0] * w[:, 1] - u[:, 1] * w[:, 0]], axis=1).astype(np.float32)
+    dd = nrm[:, 0] * nrm[:, 0] + nrm[:, 1] * nrm[:, 1] + nrm[:, 2] * nrm[:, 2]
+    nrm = nrm * (np.float32(1.0) / np.sqrt(dd))[:, None]
+    v = np.concatenate([pos, np.repeat(nrm, 3, axis=0)], axis=1).astype(np.float32)
+    return v, nrm, np.array(mat)
 
 
 @pytest.mark.gpu
@@ -252,35 +327,9 @@ def test_cli_facetrt_project_bands(oracle, tmp_path):
     assert res["quantity"] == "reflectance and spectral radiance"
     got = np.array(res["bandRadiosity"], np.float64).reshape(3, S).T
     # the assembled scene and its material per facet
-    sobj = tmp_path / "scene.obj"
-    subprocess.run([str(exe), "--scene-obj", str(prj), str(sobj)], check=True, capture_output=True)
-    verts, mat, cur = [], [], None
-    for line in open(sobj):
-        t = line.split()
-        if t and t[0] == "v":
-            verts.append([float(x) for x in t[1:4]])
-        elif t and t[0] == "usemtl":
-            cur = int(t[1][1:])
-        elif t and t[0] == "f":
-            mat.append(cur)
-    pos = np.array(verts, np.float32)
-    mat = np.array(mat)
+    v, nrm, mat = _project_scene(exe, prj, tmp_path)
     assert len(mat) == F
-    # vertices with the facet normal as appendTriangle computes it (float32
-    # cross(b - a, c - a) times 1 / sqrt of its squared length)
-    a, b, c = pos[0::3], pos[1::3], pos[2::3]
-    u, w = b - a, c - a
-    nrm = np.stack([u[:, 1] * w[:, 2] - u[:, 2] * w[:, 1], u[:, 2] * w[:, 0] - u[:, 0] * w[:, 2],
-                    u[:, 0] * w[:, 1] - u[:, 1] * w[:, 0]], axis=1).astype(np.float32)
-    dd = nrm[:, 0] * nrm[:, 0] + nrm[:, 1] * nrm[:, 1] + nrm[:, 2] * nrm[:, 2]
-    nrm = nrm * (np.float32(1.0) / np.sqrt(dd))[:, None]
-    v = np.concatenate([pos, np.repeat(nrm, 3, axis=0)], axis=1).astype(np.float32)
-    # material -> (r, t at 550 / 850 nm, ref_TIR, tau_TIR, T_sunlit, T_shaded):
-    # trees green_leaf (two listed values for three bands: the first, as
-    # interpolateValue does) with vegetation_temperature; the building's wall
-    # and roof concrete with building_temperature; ground soil, soil_temperature
-    table = {0: (0.08, 0.04, 0.03, 0.0, 300.0, 296.0), 1: (0.35, 0.0, 0.08, 0.0, 310.0, 300.0),
-             2: (0.35, 0.0, 0.08, 0.0, 310.0, 300.0), 3: (0.20, 0.0, 0.05, 0.0, 305.0, 295.0)}
+    table = VALIDATION_MATERIALS
     g = oracle.ray_graph(v, 1, 64)
     sun = oracle.sun_direction(30.0, 135.0)
     sl = oracle.ray_sunlit(v, sun, 1, 16)
