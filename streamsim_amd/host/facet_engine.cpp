@@ -310,13 +310,41 @@ int runFacetRTProject(const std::string& input, const std::string& outFile) try 
             }
         }
     }
-    std::cout << "PROGRESS\t82\tHIP Jacobi radiosity: " << nb << " bands, 64 iterations" << std::endl;
     fh_context* ctx = model.handle();
     std::vector<float> out(S * nb);
     fh_solve_stats st{};
-    facethip::check(fh_solve_bands(ctx, nb, r.data(), t.data(), e.data(), D.data(), sky.data(), nullptr, 64u, -1.0f,
-                                   1.0f, FH_PRECISION_FP32, out.data(), &st),
-                    ctx);
+    // one band's optics as the reference's AoS vector (accelerated solves)
+    const auto bandOptics = [&](uint32_t b, bool direct) {
+        std::vector<fh_surface_optics> o(S);
+        for (size_t s = 0; s < S; ++s)
+            o[s] = {r[s * nb + b], t[s * nb + b], e[s * nb + b], direct ? D[s * nb + b] : 0.0f};
+        return o;
+    };
+    if (prj.run.acceleratedSolver) {
+        // radiationSolver "accelerated" (core:1706-1724): per band
+        // solveAccelerated(optics, sky, previous band's radiosity when it is
+        // of the same kind, 64, batches of 8, |dB| <= 1e-4)
+        std::cout << "PROGRESS\t82\tHIP accelerated spectral solve: " << nb << " bands" << std::endl;
+        std::vector<float> prev, cur(S);
+        bool prevThermal = false;
+        for (uint32_t b = 0; b < nb; ++b) {
+            const bool thermal = bands[b] > 2500.0f;
+            const std::vector<fh_surface_optics> o = bandOptics(b, true);
+            const float* init = !prev.empty() && prevThermal == thermal ? prev.data() : nullptr;
+            fh_solve_stats sb{};
+            facethip::check(fh_solve_accelerated(ctx, o.data(), sky[b], init, 64u, 8u, 1.0e-4f, 1.0f, cur.data(), &sb),
+                            ctx);
+            for (size_t s = 0; s < S; ++s) out[s * nb + b] = cur[s];
+            if (b == 0) st = sb;
+            prev = cur;
+            prevThermal = thermal;
+        }
+    } else {
+        std::cout << "PROGRESS\t82\tHIP Jacobi radiosity: " << nb << " bands, 64 iterations" << std::endl;
+        facethip::check(fh_solve_bands(ctx, nb, r.data(), t.data(), e.data(), D.data(), sky.data(), nullptr, 64u,
+                                       -1.0f, 1.0f, FH_PRECISION_FP32, out.data(), &st),
+                        ctx);
+    }
     const float ref = kBeam * std::max(0.0f, sun.y) + kSky * 0.5f;
     const auto toReflectance = [&](float v) { return std::clamp(v / ref, 0.0f, 1.0f); };
     bool anyThermal = false, allThermal = true;
@@ -344,9 +372,15 @@ int runFacetRTProject(const std::string& input, const std::string& outFile) try 
             e0[s] = e[s * nb];
         }
         fh_solve_stats sd{};
-        facethip::check(fh_solve_bands(ctx, 1u, r0.data(), t0.data(), e0.data(), D0.data(), sky.data(), nullptr, 64u,
-                                       -1.0f, 1.0f, FH_PRECISION_FP32, d.data(), &sd),
-                        ctx);
+        if (prj.run.acceleratedSolver) {
+            const std::vector<fh_surface_optics> o = bandOptics(0, false);
+            facethip::check(fh_solve_accelerated(ctx, o.data(), sky[0], nullptr, 64u, 8u, 1.0e-4f, 1.0f, d.data(), &sd),
+                            ctx);
+        } else {
+            facethip::check(fh_solve_bands(ctx, 1u, r0.data(), t0.data(), e0.data(), D0.data(), sky.data(), nullptr,
+                                           64u, -1.0f, 1.0f, FH_PRECISION_FP32, d.data(), &sd),
+                            ctx);
+        }
         for (size_t s = 0; s < S; ++s) enh[s] = std::max(0.0f, band0[s] - (ref > 1.0e-8f ? toReflectance(d[s]) : d[s]));
     }
     const auto t1 = Clock::now();

@@ -81,6 +81,7 @@ struct ProjectRun {
     int vegetationMethod = 0;
     bool saveEnergy = false, saveRadiation = false;
     float sunZenith = 30.0f, sunAzimuth = 135.0f;
+    bool acceleratedSolver = false;  // control.radiationSolver == "accelerated" (fileio_json.cpp:208)
     uint32_t periodicNeighbors = 0;
     std::vector<float> bands;  // sensor bands, nm
     std::string outDir;
@@ -444,6 +445,7 @@ inline Project loadProject(const std::string& file) {
     run.spectralWidth =
         uint32_t(std::clamp(int(std::lround(json::num(control, "spectralAccelerationWidth", 100.0f))), 1, 1000));
     run.vegetationMethod = json::integer(control, "vegetationTemperatureMethod", 0);
+    run.acceleratedSolver = json::str(control, "radiationSolver", "traditional") == "accelerated";
     run.saveEnergy = json::boolean(sensor, "energyProcess", false);
     run.saveRadiation = json::boolean(sensor, "radiationProcess", false);
     run.sunZenith = json::num(light, "zenith", 30.0f);

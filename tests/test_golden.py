@@ -303,18 +303,29 @@ def _project_scene(exe, prj, tmp_path):
 
 
 @pytest.mark.gpu
-def test_cli_facetrt_project_bands(oracle, tmp_path):
+@pytest.mark.parametrize("solver", ["traditional", "accelerated"])
+def test_cli_facetrt_project_bands(oracle, tmp_path, solver):
     # eFacetRT on the validation project (sensor bands 550, 850, 10500 nm),
     # against the oracle: the same assembled scene (--scene-obj), its ray
     # graph and sunlit fractions replayed on the host, the band optics of
     # makeSpectralOptics (core:1057-1126) built here from the project's
     # entries for the scene's four materials, 64 FP64 Jacobi sweeps per band,
-    # optical bands as apparent reflectance against 700 cos(30 deg) + 50
-    print("golden cli rt project", flush=True)
+    # optical bands as apparent reflectance against 700 cos(30 deg) + 50.
+    # control.radiationSolver "accelerated" solves band by band until the
+    # absolute change is 1e-4, warm-started from the previous band of the
+    # same kind (core:1706-1724): the same fixed point in fewer sweeps
+    print(f"golden cli rt project {solver}", flush=True)
     import os
+    import shutil
     from streamsim_amd.optics import planck_radiance
     exe = Path(__file__).resolve().parents[1] / "streamsim_amd" / "facet_engine"
     prj = HERE / "urban_tree_project" / "project.json"
+    if solver == "accelerated":
+        shutil.copytree(HERE / "urban_tree_project", tmp_path / "prj")
+        prj = tmp_path / "prj" / "project.json"
+        d = json.loads(prj.read_text())
+        d["configuration"].setdefault("control", {})["radiationSolver"] = "accelerated"
+        prj.write_text(json.dumps(d))
     env = dict(os.environ, FACET_RAYS_PER_SIDE="64", FACET_SUN_RAYS_PER_SIDE="16")
     out = tmp_path / "rt" / "facetrt.json"
     r = subprocess.run([str(exe), "eFacetRT", str(prj), str(out)], env=env, capture_output=True, text=True,
@@ -358,3 +369,4 @@ def test_cli_facetrt_project_bands(oracle, tmp_path):
     want = ref.astype(np.float64)
     want[:, :2] = np.clip(want[:, :2] / refl, 0, 1)
     assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-3)) < 1e-4
+    assert res["iterations"] == 64 if solver == "traditional" else res["iterations"] < 64
