@@ -225,7 +225,10 @@ typedef struct {
     float convectiveScale;
     int32_t soilTemperatureMethod; /* 0: G = 0.35 Rn, 1: k dT, 2: 1.5 k dT */
     int32_t vegetation;            /* 1: leaf transpires (LE on) */
-    int32_t reserved;
+    int32_t photovoltaic;          /* 1: object facets generate electric power (core:3655-3663) */
+    float pvEta25;                 /* efficiency at 25 C */
+    float pvGamma;                 /* efficiency temperature coefficient, K-1 */
+    float pvBifaciality;           /* weight of the back side's incident shortwave */
 } fh_eb_material;
 
 /* Scene description for runFacetEBCore (core:3164-4154): facets
@@ -299,6 +302,15 @@ int fh_eb_run_node(fh_context *ctx, const fh_eb_node_params *params, const float
 int fh_eb_get_state(fh_context *ctx, float *temperature, float *absorbedShortwave,
                     float *longwaveRadiosity, float *sunlit, float *netRadiation, float *sensibleHeat,
                     float *latentHeat, float *storageHeat, float *swRadiosity);
+
+/* Per side, after fh_eb_run_node: the shortwave incident (sum of the windows'
+ * incident irradiance, core:3544), the absorbed shortwave and the electric
+ * power of photovoltaic object facets (W m-2, the same on both sides of a
+ * facet, 0 elsewhere; the reference's photovoltaicPower, core:3657-3667).
+ * Any pointer may be NULL.  No counterpart in the reference's backend API:
+ * the reference keeps these arrays in its host loop (core:3485). */
+int fh_eb_get_photovoltaic(fh_context *ctx, float *incidentShortwave, float *absorbedShortwave,
+                           float *power);
 /* Replaces the temperature state (surfaceCount values). */
 int fh_eb_set_temperature(fh_context *ctx, const float *temperature);
 

@@ -1287,6 +1287,7 @@ static double soil_storage(const or_eb_material *m, double dT)
 uint32_t or_eb_node(const or_graph *g, uint32_t F, uint32_t leafCount,
                     const uint32_t *material, const or_eb_material *mats,
                     const float *normalY, const float *absorbedSW,
+                    const float *incidentSW,
                     const or_eb_params *p, float *T, const float *Tprev,
                     float *Blw, int *convergedOut, float *maxDeltaTOut,
                     float *maxResEqOut)
@@ -1367,8 +1368,18 @@ uint32_t or_eb_node(const or_graph *g, uint32_t F, uint32_t leafCount,
                     const double Gd = fmax(0.0, (double)m->heatCapacityPerArea) / p->dTime; \
                     const double G = Gd * (Tl - (double)Tprev[s0]);                 \
                     const double eps = clampd(1.0 - (double)r[s0] - (double)tt[s0], 0.0, 1.0); \
-                    res[s0] = net - Hs - LE - G;                                    \
-                    der[s0] = 8.0 * eps * kSigma * Tl * Tl * Tl + 2.0 * ha * m->convectiveScale + LEd + Gd; \
+                    /* photovoltaic power and its slope (core:3655-3663, 3772-3778) */ \
+                    double Ppv = 0.0, Ppvd = 0.0;                                   \
+                    if (m->photovoltaic && incidentSW) {                            \
+                        const double inc = (double)incidentSW[s0] + (double)m->pvBifaciality * incidentSW[s0 + 1u]; \
+                        const double eta = (double)m->pvEta25 * (1.0 + (double)m->pvGamma * (Tl - 298.15)); \
+                        const double avail = (double)absorbedSW[s0] + absorbedSW[s0 + 1u]; \
+                        const double raw = eta * inc;                               \
+                        Ppv = clampd(raw, 0.0, avail);                              \
+                        Ppvd = raw > 0.0 && raw < avail ? (double)m->pvEta25 * m->pvGamma * inc : 0.0; \
+                    }                                                               \
+                    res[s0] = net - Hs - LE - G - Ppv;                              \
+                    der[s0] = 8.0 * eps * kSigma * Tl * Tl * Tl + 2.0 * ha * m->convectiveScale + LEd + Gd + Ppvd; \
                     const double eq = fabs(res[s0]) / fmax(1.0, der[s0]);           \
                     if (eq > maxEq) maxEq = eq;                                     \
                 } else {                                                            \

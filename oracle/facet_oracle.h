@@ -163,7 +163,10 @@ typedef struct {
     float convectiveScale;
     int32_t soilTemperatureMethod;
     int32_t vegetation;
-    int32_t pad;
+    int32_t photovoltaic;    /* object facets generate electric power */
+    float pvEta25;
+    float pvGamma;
+    float pvBifaciality;
 } or_eb_material;
 
 typedef struct {
@@ -184,11 +187,14 @@ typedef struct {
 /* One FacetEB node (core:3567-3846) in FP64: coupling loop of
  * LW optics -> LW solve -> incident -> fluxes -> relaxed Newton, then the
  * final LW solve and residual.  T (S) is in/out; Tprev (S) is the previous
- * node's state.  absorbedSW (S) is given.  Outputs per side may be NULL.
- * Returns the number of coupling iterations; *converged set. */
+ * node's state.  absorbedSW (S) is given; incidentSW (S, the shortwave
+ * incident per side, may be NULL) feeds the power of photovoltaic object
+ * facets (core:3655-3663).  Outputs per side may be NULL.  Returns the
+ * number of coupling iterations; *converged set. */
 uint32_t or_eb_node(const or_graph *g, uint32_t F, uint32_t leafCount,
                     const uint32_t *material, const or_eb_material *mats,
                     const float *normalY, const float *absorbedSW,
+                    const float *incidentSW,
                     const or_eb_params *p, float *T, const float *Tprev,
                     float *Blw, int *converged, float *maxDeltaT,
                     float *maxResidualEq);

@@ -65,7 +65,8 @@ class FhEbMaterial(C.Structure):
         ("tirReflectance", C.c_float), ("tirTransmittance", C.c_float), ("surfaceResistance", C.c_float),
         ("specificHeat", C.c_float), ("density", C.c_float), ("thermalConductivity", C.c_float),
         ("soilMoisture", C.c_float), ("heatCapacityPerArea", C.c_float), ("convectiveScale", C.c_float),
-        ("soilTemperatureMethod", C.c_int32), ("vegetation", C.c_int32), ("reserved", C.c_int32),
+        ("soilTemperatureMethod", C.c_int32), ("vegetation", C.c_int32), ("photovoltaic", C.c_int32),
+        ("pvEta25", C.c_float), ("pvGamma", C.c_float), ("pvBifaciality", C.c_float),
     ]
 
 
@@ -121,6 +122,7 @@ SIGNATURES = {
     "fh_eb_set_scene": (_I, [_P, _U32, _P, _U32, _P, _U32, _P, _P]),
     "fh_eb_run_node": (_I, [_P, C.POINTER(FhEbNodeParams), _P, _P, C.POINTER(FhEbNodeStats)]),
     "fh_eb_get_state": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "fh_eb_get_photovoltaic": (_I, [_P, _P, _P, _P]),
     "fh_eb_set_temperature": (_I, [_P, _P]),
     "fh_mlp_param_count": (_U32, [_U32]),
     "fh_mlp_set_params": (_I, [_P, _U32, _P, _P, _P, _F, _F]),

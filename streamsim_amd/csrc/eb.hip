@@ -279,11 +279,18 @@ struct NewtonOut {
 
 // Fluxes, residual and relaxed Newton update of one facet (FP64) given the
 // incident LW H0, H1 of its two sides.  Leaf facets share side 0's T.
+//
+// Photovoltaic object facets (core:3655-3663, 3772-3778): power P =
+// clamp(eta25 (1 + gamma (T - 298.15)) (I_0 + bifaciality I_1), 0, A_0 + A_1)
+// with I the incident and A the absorbed shortwave of the two sides leaves
+// the balance (residual - P), and while unclamped its slope eta25 gamma I
+// joins the Newton derivative.
 __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_material& m,
-                                  const float* __restrict__ absorbed, float H0, float H1, const float* __restrict__ eps,
-                                  float* __restrict__ T, const float* __restrict__ Tprev, bool update,
-                                  float* __restrict__ Rn_out, float* __restrict__ H_out, float* __restrict__ LE_out,
-                                  float* __restrict__ G_out, const double* __restrict__ area,
+                                  const float* __restrict__ absorbed, const float* __restrict__ incidentSW, float H0,
+                                  float H1, const float* __restrict__ eps, float* __restrict__ T,
+                                  const float* __restrict__ Tprev, bool update, float* __restrict__ Rn_out,
+                                  float* __restrict__ H_out, float* __restrict__ LE_out, float* __restrict__ G_out,
+                                  float* __restrict__ pv_out, const double* __restrict__ area,
                                   float* __restrict__ stepPrev = nullptr) {
     NewtonOut o{0.0f, 0.0f, 0.0f, 0.0};
     const uint32_t s0 = 2u * f;
@@ -312,8 +319,17 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
         }
         const double gd = fmax(0.0, double(m.heatCapacityPerArea)) / k.dTime;
         const double g = gd * (Tl - double(Tprev[s0]));
-        const double res = net - hs - le - g;
-        const double der = 8.0 * double(eps[s0]) * kSigma * Tl * Tl * Tl + 2.0 * k.ha * m.convectiveScale + dle + gd;
+        double pv = 0.0, dpv = 0.0;
+        if (m.photovoltaic) {
+            const double inc = double(incidentSW[s0]) + double(m.pvBifaciality) * double(incidentSW[s0 + 1]);
+            const double avail = double(absorbed[s0]) + double(absorbed[s0 + 1]);
+            const double raw = double(m.pvEta25) * (1.0 + double(m.pvGamma) * (Tl - 298.15)) * inc;
+            pv = clampd(raw, 0.0, avail);
+            dpv = raw > 0.0 && raw < avail ? double(m.pvEta25) * double(m.pvGamma) * inc : 0.0;
+        }
+        const double res = net - hs - le - g - pv;
+        const double der =
+            8.0 * double(eps[s0]) * kSigma * Tl * Tl * Tl + 2.0 * k.ha * m.convectiveScale + dle + gd + dpv;
         o.eq = float(fabs(res) / fmax(1.0, der));
         o.res = float(fabs(res));
         o.rn = area[f] * net;
@@ -324,6 +340,7 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
                 LE_out[s0 + q] = float(le);
                 G_out[s0 + q] = float(g);
             }
+        if (pv_out) pv_out[s0] = pv_out[s0 + 1] = float(pv);
         if (update) {
             const double cand = clampd(Tl + res / fmax(1.0, der), 220.0, 360.0);
             const double w = step_relaxation(k, stepPrev ? stepPrev + s0 : nullptr, cand - Tl);
@@ -359,6 +376,7 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
                 LE_out[s] = float(le);
                 G_out[s] = float(g);
             }
+            if (pv_out) pv_out[s] = 0.0f;
             if (update) {
                 const double cand = clampd(Ts + res / fmax(1.0, der), 220.0, 360.0);
                 const double w = step_relaxation(k, stepPrev ? stepPrev + s : nullptr, cand - Ts);
@@ -376,18 +394,20 @@ __device__ NewtonOut newton_facet(uint32_t f, const EbConst& k, const fh_eb_mate
 // sum A Rn over energy-balance sides.
 __global__ __launch_bounds__(256) void k_newton(uint32_t F, EbConst k, const uint32_t* __restrict__ mat,
                                                 const fh_eb_material* __restrict__ mats,
-                                                const float* __restrict__ absorbed, const float* __restrict__ H,
+                                                const float* __restrict__ absorbed,
+                                                const float* __restrict__ incidentSW, const float* __restrict__ H,
                                                 const float* __restrict__ eps, float* __restrict__ T,
                                                 const float* __restrict__ Tprev, int update, float* __restrict__ Rn_out,
                                                 float* __restrict__ H_out, float* __restrict__ LE_out,
-                                                float* __restrict__ G_out, const double* __restrict__ area,
-                                                float4* __restrict__ part, double* __restrict__ dpart) {
+                                                float* __restrict__ G_out, float* __restrict__ pv_out,
+                                                const double* __restrict__ area, float4* __restrict__ part,
+                                                double* __restrict__ dpart) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     float v[3] = {0.0f, 0.0f, 0.0f};
     double rnSum = 0.0;
     if (f < F) {
-        const NewtonOut o = newton_facet(f, k, mats[mat[f]], absorbed, H[2u * f], H[2u * f + 1], eps, T, Tprev,
-                                         update != 0, Rn_out, H_out, LE_out, G_out, area);
+        const NewtonOut o = newton_facet(f, k, mats[mat[f]], absorbed, incidentSW, H[2u * f], H[2u * f + 1], eps, T,
+                                         Tprev, update != 0, Rn_out, H_out, LE_out, G_out, pv_out, area);
         v[0] = o.dT;
         v[1] = o.eq;
         v[2] = o.res;
@@ -427,6 +447,7 @@ struct CoupledArgs {
     const float* t;
     const float* eps;
     const float* absorbed;
+    const float* incidentSW;  // incident shortwave per side (photovoltaic power)
     float* T;
     const float* Tprev;
     float* Hout;
@@ -488,8 +509,9 @@ __global__ __launch_bounds__(256) void k_coupled_update(CoupledArgs a) {
         const float H0 = fmaxf(in0, 0.0f), H1 = fmaxf(in1, 0.0f);
         a.Hout[s0] = H0;
         a.Hout[s0 + 1] = H1;
-        const NewtonOut o = newton_facet(f, a.k, a.mats[a.mat[f]], a.absorbed, H0, H1, a.eps, a.T, a.Tprev, true,
-                                         nullptr, nullptr, nullptr, nullptr, a.area, a.stepPrev);
+        const NewtonOut o = newton_facet(f, a.k, a.mats[a.mat[f]], a.absorbed, a.incidentSW, H0, H1, a.eps, a.T,
+                                         a.Tprev, true, nullptr, nullptr, nullptr, nullptr, nullptr, a.area,
+                                         a.stepPrev);
         v[2] = o.dT;
         v[3] = o.eq;
         const float ins[2] = {in0, in1};
@@ -652,8 +674,8 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     if (swNb)
         FH_HIP(hipMemcpyAsync(e.swOrderD.p, e.swOrder.data(), sizeof(uint32_t) * swNb, hipMemcpyHostToDevice,
                               c->stream));
-    for (DBuf<float>* b : {&e.T, &e.Tprev, &e.absorbed, &e.incident, &e.sunlit, &e.rlw, &e.tlw, &e.eps, &e.elw,
-                           &e.clw, &e.Hlw, &e.zero, &e.Blw0, &e.Blw1, &e.Rn, &e.Hs, &e.LE, &e.G})
+    for (DBuf<float>* b : {&e.T, &e.Tprev, &e.absorbed, &e.incident, &e.pvPower, &e.sunlit, &e.rlw, &e.tlw, &e.eps,
+                           &e.elw, &e.clw, &e.Hlw, &e.zero, &e.Blw0, &e.Blw1, &e.Rn, &e.Hs, &e.LE, &e.G})
         b->ensure(S);
     const size_t nsw = size_t(S) * (e.swLdb ? e.swLdb : 1u);
     for (DBuf<float>* b : {&e.Bsw0, &e.Bsw1, &e.rsw, &e.tsw, &e.Dsw, &e.csw, &e.Hsw}) b->ensure(nsw);
@@ -686,6 +708,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     launch_fill_f32(c, e.zero.p, S, 0.0f);
     launch_fill_f32(c, e.absorbed.p, S, 0.0f);
     launch_fill_f32(c, e.incident.p, S, 0.0f);
+    launch_fill_f32(c, e.pvPower.p, S, 0.0f);
     launch_fill_f32(c, e.Bsw0.p, nsw, 0.0f);
     launch_fill_f32(c, e.Blw0.p, S, 0.0f);
     for (DBuf<float>* b : {&e.Rn, &e.Hs, &e.LE, &e.G, &e.sunlit}) launch_fill_f32(c, b->p, S, 0.0f);
@@ -720,8 +743,8 @@ static void newton(fh_context* c, const EbConst& k, int update, float& maxDT, fl
     e.last4.ensure(4);
     e.dpart.ensure(size_t(nbk) * 4u);
     hipLaunchKernelGGL(k_newton, dim3(nbk), dim3(256), 0, c->stream, F, k, e.material.p, e.mats.p, e.absorbed.p,
-                       e.Hlw.p, e.eps.p, e.T.p, e.Tprev.p, update, e.Rn.p, e.Hs.p, e.LE.p, e.G.p, e.area.p,
-                       e.part4.p, e.dpart.p);
+                       e.incident.p, e.Hlw.p, e.eps.p, e.T.p, e.Tprev.p, update, e.Rn.p, e.Hs.p, e.LE.p, e.G.p,
+                       e.pvPower.p, e.area.p, e.part4.p, e.dpart.p);
     hipLaunchKernelGGL(k_reduce_max4, dim3(1), dim3(1024), 0, c->stream, e.part4.p, nbk, e.last4.p);
     if (rnSum) hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(256), 0, c->stream, e.dpart.p, nbk, 3u, 4u, e.dsum.p);
     FH_HIP(hipGetLastError());
@@ -785,6 +808,7 @@ static uint32_t coupled_loop(fh_context* c, const fh_eb_node_params& p, const Eb
     a.t = e.tlw.p;
     a.eps = e.eps.p;
     a.absorbed = e.absorbed.p;
+    a.incidentSW = e.incident.p;
     a.T = e.T.p;
     a.Tprev = e.Tprev.p;
     a.Hout = e.Hlw.p;
@@ -1127,6 +1151,21 @@ int fh_eb_get_state(fh_context* c, float* T, float* absorbed, float* Blw, float*
             for (size_t s = 0; s < S; ++s)
                 for (uint32_t b = 0; b < nb; ++b) swRadiosity[s * nb + e.swOrder[b]] = h[s * nb + b];
         }
+        FH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int fh_eb_get_photovoltaic(fh_context* c, float* incidentShortwave, float* absorbedShortwave, float* power) {
+    return guarded_eb(c, [&] {
+        fh::EbState& e = c->eb;
+        if (!e.ready) fail(FH_LOGIC_ERROR, "fh_eb_set_scene() must be called before fh_eb_get_photovoltaic()");
+        const size_t S = c->S;
+        auto get = [&](float* dst, const float* src) {
+            if (dst) FH_HIP(hipMemcpyAsync(dst, src, S * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        };
+        get(incidentShortwave, e.incident.p);
+        get(absorbedShortwave, e.absorbed.p);
+        get(power, e.pvPower.p);
         FH_HIP(hipStreamSynchronize(c->stream));
     });
 }

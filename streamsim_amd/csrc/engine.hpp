@@ -130,6 +130,7 @@ struct EbState {
     DBuf<float> T, Tprev, absorbed, incident, sunlit, rlw, tlw, eps, elw, clw, Hlw, zero;
     DBuf<float> Blw0, Blw1;
     DBuf<float> Rn, Hs, LE, G;
+    DBuf<float> pvPower;  // electric power of photovoltaic object facets, W m-2 (both sides)
     // SW [S][swLdb]
     DBuf<float> Bsw0, Bsw1, rsw, tsw, Dsw, csw, Hsw, dni;
     // internal SW band j is window swOrder[j]: windows sorted by the largest

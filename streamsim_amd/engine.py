@@ -269,6 +269,14 @@ class FacetEngine:
             out["shortwaveRadiosity"] = sw
         return out
 
+    def ebGetPhotovoltaic(self) -> dict:
+        """Per side: incident and absorbed shortwave, photovoltaic power."""
+        S = self.surfaceCount()
+        out = {k: np.zeros(S, np.float32) for k in ("incidentShortwave", "absorbedShortwave", "power")}
+        self._check(self._lib.fh_eb_get_photovoltaic(self._h, _ptr(out["incidentShortwave"]),
+                                                     _ptr(out["absorbedShortwave"]), _ptr(out["power"])))
+        return out
+
     def ebSetTemperature(self, T) -> None:
         T = _f32(T)
         self._check(self._lib.fh_eb_set_temperature(self._h, _ptr(T)))

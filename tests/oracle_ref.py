@@ -31,7 +31,7 @@ class OrEbMaterial(C.Structure):
     _fields_ = [("tirReflectance", _F), ("tirTransmittance", _F), ("surfaceResistance", _F), ("specificHeat", _F),
                 ("density", _F), ("thermalConductivity", _F), ("soilMoisture", _F), ("heatCapacityPerArea", _F),
                 ("convectiveScale", _F), ("soilTemperatureMethod", C.c_int32), ("vegetation", C.c_int32),
-                ("pad", C.c_int32)]
+                ("photovoltaic", C.c_int32), ("pvEta25", _F), ("pvGamma", _F), ("pvBifaciality", _F)]
 
 
 class OrEbParams(C.Structure):
@@ -71,8 +71,8 @@ def load():
         "or_incident": (None, [C.POINTER(OrGraph), _P, _P, _F, _P]),
         "or_incident_bands": (None, [C.POINTER(OrGraph), _U, _P, _P, _P, _P]),
         "or_solve_direct": (_I, [C.POINTER(OrGraph), _P, _F, _P]),
-        "or_eb_node": (_U, [C.POINTER(OrGraph), _U, _U, _P, _P, _P, _P, C.POINTER(OrEbParams), _P, _P, _P, _P, _P,
-                            _P]),
+        "or_eb_node": (_U, [C.POINTER(OrGraph), _U, _U, _P, _P, _P, _P, _P, C.POINTER(OrEbParams), _P, _P, _P, _P,
+                            _P, _P]),
         "or_mlp_forward": (None, [_P, _U, _U, _P, _U, _P]),
         "or_mlp_loss_grad": (_D, [_P, _U, _U, _P, _P, _U, _P]),
     }
@@ -260,7 +260,8 @@ def solve_direct(g: Graph, optics, sky: float) -> np.ndarray:
     return out
 
 
-def eb_node(g: Graph, F: int, leaf_count: int, material, mats, absorbed, params: OrEbParams, T, Tprev):
+def eb_node(g: Graph, F: int, leaf_count: int, material, mats, absorbed, params: OrEbParams, T, Tprev,
+            incident=None):
     h = _Held(g)
     mat = np.ascontiguousarray(material, np.uint32)
     marr = (OrEbMaterial * len(mats))(*mats)
@@ -270,8 +271,10 @@ def eb_node(g: Graph, F: int, leaf_count: int, material, mats, absorbed, params:
     conv = C.c_int(0)
     mdt = np.zeros(1, np.float32)
     meq = np.zeros(1, np.float32)
+    inc = None if incident is None else np.ascontiguousarray(incident, np.float32)
     it = load().or_eb_node(h.ref(), F, leaf_count, _p(mat), C.cast(marr, _P), None,
-                           _p(np.ascontiguousarray(absorbed, np.float32)), C.byref(params), _p(T), _p(Tp), _p(Blw),
+                           _p(np.ascontiguousarray(absorbed, np.float32)), _p(inc), C.byref(params), _p(T), _p(Tp),
+                           _p(Blw),
                            C.cast(C.pointer(conv), _P), _p(mdt), _p(meq))
     return T, Blw, int(it), bool(conv.value), float(mdt[0]), float(meq[0])
 

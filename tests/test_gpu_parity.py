@@ -293,6 +293,63 @@ def test_eb_node_matches_oracle(oracle, nb):
     assert _rel(s["longwaveRadiosity"], Blw) < 1e-4
 
 
+@pytest.mark.parametrize("nb", [4, 24])
+def test_eb_node_photovoltaic_matches_oracle(oracle, nb):
+    # photovoltaic object facets (core:3655-3663, 3772-3778): the leaves as
+    # bifacial panels; the coupled balance with their electric power against
+    # the FP64 oracle given the same incident and absorbed shortwave
+    print(f"eb photovoltaic case nb={nb}", flush=True)
+    sc, eng, win = _eb_case(oracle, nb=nb)
+    pv_leaf = dict(faceteb.LEAF, vegetation=0, heatCapacityPerArea=12000.0, photovoltaic=1, pvEta25=0.22,
+                   pvGamma=-0.0035, pvBifaciality=0.3)
+    r_tab, t_tab = faceteb.material_tables(["leaf", "soil", "building"], win)
+    eng.ebSetScene(sc.leaf_count, sc.material, [pv_leaf, faceteb.SOIL, faceteb.BUILDING], r_tab, t_tab)
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    cfg = faceteb.EbConfig(tolerance=1e-3, radiosity_tolerance=1e-7, max_coupling=200)
+    met = faceteb.diurnal_meteo(24)[12]
+    dni, dif = win.irradiance(met.shortwave, math.cos(math.radians(met.zenith)))
+    st = eng.ebRunNode(faceteb.node_params(met, cfg, first=True), dni, dif)
+    assert st.converged
+    s = eng.ebGetState()
+    pv = eng.ebGetPhotovoltaic()
+    np.testing.assert_array_equal(pv["absorbedShortwave"], s["absorbedShortwave"])
+    mats = []
+    for m in (pv_leaf, faceteb.SOIL, faceteb.BUILDING):
+        om = oracle.OrEbMaterial()
+        for k, v in m.items():
+            setattr(om, k, v)
+        mats.append(om)
+    op = oracle.OrEbParams()
+    op.airTemperatureK = met.air_temperature_k
+    op.vaporPressure = met.vapor_pressure
+    op.pressure = met.pressure
+    op.windSpeed = met.wind
+    op.skyLongwave = met.longwave
+    op.dTime = cfg.dtime
+    op.tolerance = 1e-3
+    op.relaxation = cfg.relaxation
+    op.maxCoupling = 200
+    op.lwMaxSweeps = 200
+    op.lwTolRel = 1e-9
+    T0 = np.full(sc.surface_count, met.air_temperature_k, np.float32)
+    T, Blw, it, conv, _, _ = oracle.eb_node(og, sc.facet_count, sc.leaf_count, sc.material, mats,
+                                            s["absorbedShortwave"], op, T0, T0, incident=pv["incidentShortwave"])
+    assert conv
+    assert np.max(np.abs(s["temperature"] - T)) < 0.01
+    assert _rel(s["longwaveRadiosity"], Blw) < 1e-4
+    # power of every leaf: the model at the oracle's temperatures
+    L = sc.leaf_count
+    inc = pv["incidentShortwave"].astype(np.float64).reshape(-1, 2)[:L]
+    ab = pv["absorbedShortwave"].astype(np.float64).reshape(-1, 2)[:L]
+    Tl = T.astype(np.float64).reshape(-1, 2)[:L, 0]
+    want = np.clip(0.22 * (1 - 0.0035 * (Tl - 298.15)) * (inc[:, 0] + 0.3 * inc[:, 1]), 0, ab.sum(axis=1))
+    got = pv["power"].reshape(-1, 2)
+    np.testing.assert_array_equal(got[:L, 0], got[:L, 1])
+    assert np.all(got[L:] == 0)
+    assert want.max() > 10.0
+    np.testing.assert_allclose(got[:L, 0], want, rtol=1e-3, atol=1e-3)
+
+
 @pytest.mark.parametrize("nb", [4, 24, 47])
 def test_eb_diurnal_plain_vs_bf16(oracle, nb):
     # the BF16 path reaches the same converged state (shortwave BF16
