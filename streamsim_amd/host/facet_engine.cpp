@@ -712,7 +712,7 @@ void writeProcess(const std::filesystem::path& dir, const std::string& type, int
                   float dT, bool converged, float residual, float residualEq, const EbSettings& set,
                   const std::vector<float>* cols[5], RunIndex& index) {
     const std::string model = type == "radiation" ? "facetrt" : "faceteb";
-    const std::string stem = model + "_T=" + token;
+    const std::string stem = model + (type == "photovoltaic" ? "_pv_T=" : "_T=") + token;
     const std::filesystem::path pdir = dir / "process";
     std::filesystem::create_directories(pdir);
     const int nf = type == "energy" ? 5 : 3;
@@ -738,7 +738,11 @@ void writeProcess(const std::filesystem::path& dir, const std::string& type, int
        << ",\n  \"energyResidualUnits\": \"W m-2\",\n  \"energyResidualTemperatureEquivalent\": " << residualEq
        << ",\n  \"temperatureTolerance\": " << set.tolerance << ",\n  \"iterationBudget\": "
        << set.couplingIterations << ",\n  \"fields\": ";
-    if (type == "radiation")
+    if (type == "photovoltaic")
+        js << "[{\"id\":\"photovoltaicPower\",\"label\":\"photovoltaic power [W m-2]\",\"offset\":0},"
+              "{\"id\":\"temperature\",\"label\":\"temperature [K]\",\"offset\":1},"
+              "{\"id\":\"absorbedShortwave\",\"label\":\"absorbed shortwave of the side [W m-2]\",\"offset\":2}]\n}\n";
+    else if (type == "radiation")
         js << "[{\"id\":\"shortwaveRadiation\",\"label\":\"shortwave radiosity [W m-2]\",\"offset\":0},"
               "{\"id\":\"longwaveRadiation\",\"label\":\"longwave radiosity [W m-2]\",\"offset\":1},"
               "{\"id\":\"netRadiation\",\"label\":\"net radiation [W m-2]\",\"offset\":2}]\n}\n";
@@ -844,6 +848,10 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
             e.convectiveScale = pm.convectiveScale;
             e.soilTemperatureMethod = pm.soilMethod;
             e.vegetation = pm.vegetation ? 1 : 0;
+            e.photovoltaic = pm.pv ? 1 : 0;
+            e.pvEta25 = pm.pvEta25;
+            e.pvGamma = pm.pvGamma;
+            e.pvBifaciality = pm.pvBifaciality;
             const bool leaf = m + 1 < nm;  // the last material is the ground's
             for (uint32_t w = 0; w < nw; ++w) {
                 const size_t centre = (size_t(windows[w].first) + windows[w].second) / 2u;
@@ -895,10 +903,26 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         }
         if (!comp) throw std::runtime_error("FacetEB cannot write component identities");
     }
-    {  // no photovoltaic materials in an OBJ scene: header only (core:3433-3436)
-        std::ofstream pv(outDir / "photovoltaic_summary.csv", std::ios::trunc);
-        pv << "node,time,area_m2,temperature_C,incident_W,absorbed_W,electric_W,net_longwave_W,sensible_W,"
-              "storage_W,max_residual_W_m2,energy_kWh\n";
+    // photovoltaic summary, one row per node with panels (core:3433-3436)
+    std::ofstream pvSummary(outDir / "photovoltaic_summary.csv", std::ios::trunc);
+    pvSummary << "node,time,area_m2,temperature_C,incident_W,absorbed_W,electric_W,net_longwave_W,sensible_W,"
+                 "storage_W,max_residual_W_m2,energy_kWh\n";
+    double pvEnergyKwh = 0.0;
+    std::vector<uint8_t> pvFacet(scene.facetCount(), 0);
+    std::vector<float> pvArea(scene.facetCount(), 0.0f), pvBif(scene.facetCount(), 0.0f);
+    bool anyPv = false;
+    for (uint32_t f = 0; f < scene.leafFacetCount && project; ++f) {
+        const facetcli::ProjectMaterial& pm = prj.materials[scene.material[f]];
+        if (!pm.pv) continue;
+        anyPv = true;
+        pvFacet[f] = 1;
+        pvBif[f] = pm.pvBifaciality;
+        const auto& a = scene.vertices[3 * size_t(f)].position;
+        const auto& b = scene.vertices[3 * size_t(f) + 1].position;
+        const auto& c = scene.vertices[3 * size_t(f) + 2].position;
+        const float u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, w[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        const float x = u[1] * w[2] - u[2] * w[1], y = u[2] * w[0] - u[0] * w[2], z = u[0] * w[1] - u[1] * w[0];
+        pvArea[f] = 0.5f * std::sqrt(x * x + y * y + z * z);  // facetArea (core:333-341)
     }
     index.products.push_back("photovoltaic_summary.csv");
     index.products.push_back(".facet_steps/components.bin");
@@ -918,6 +942,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
 
     const size_t S = scene.facetCount() * size_t(2);
     std::vector<float> T(S), Blw(S), sunlit(S), Rn(S), H(S), LE(S), G(S), sw(S * nw), Bsw(S), Btot(S);
+    std::vector<float> absorbedSW(S), pvInc(S), pvPow(S, 0.0f);
     bool allConverged = true;
     fh_eb_node_stats last{};
     std::string lastToken;
@@ -957,8 +982,8 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         p.firstNode = node == set.startNode ? 1u : 0u;
         fh_eb_node_stats st{};
         facethip::check(fh_eb_run_node(ctx, &p, dni.data(), dif.data(), &st), ctx);
-        facethip::check(fh_eb_get_state(ctx, T.data(), nullptr, Blw.data(), sunlit.data(), Rn.data(), H.data(),
-                                        LE.data(), G.data(), sw.data()),
+        facethip::check(fh_eb_get_state(ctx, T.data(), absorbedSW.data(), Blw.data(), sunlit.data(), Rn.data(),
+                                        H.data(), LE.data(), G.data(), sw.data()),
                         ctx);
         for (size_t s = 0; s < S; ++s) {
             float b = 0.0f;
@@ -1055,10 +1080,44 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
                          st.maxTemperatureDelta, converged, st.maxEnergyResidual, st.maxResidualEquivalent, set,
                          cols, index);
         }
-        {
+        {  // photovoltaic facets of the node and their summary row (core:3848-3859)
             std::ofstream pv(outDir / ("photovoltaic_node_" + std::to_string(node) + ".csv"), std::ios::trunc);
             pv << "area_m2,temperature_C,incident_W_m2,absorbed_W_m2,electric_W_m2,residual_W_m2,latent_W_m2,"
                   "effective_W_m2,net_longwave_W_m2,sensible_W_m2,storage_W_m2,facet_index\n";
+            double pa = 0, pt = 0, pg = 0, pabs = 0, pp = 0, plw = 0, ph = 0, ps = 0, pres = 0;
+            if (anyPv) {
+                facethip::check(fh_eb_get_photovoltaic(ctx, pvInc.data(), nullptr, pvPow.data()), ctx);
+                for (uint32_t f = 0; f < scene.leafFacetCount; ++f) {
+                    if (!pvFacet[f]) continue;
+                    const size_t k = 2 * size_t(f);
+                    const float a = pvArea[f], ab = absorbedSW[k] + absorbedSW[k + 1];
+                    const float inc = pvInc[k] + pvInc[k + 1], power = pvPow[k];
+                    const float lw = Rn[k] - ab, res = Rn[k] - H[k] - LE[k] - G[k] - power;
+                    pv << a << ',' << T[k] - 273.15f << ',' << inc << ',' << ab << ',' << power << ',' << res << ','
+                       << LE[k] << ',' << pvInc[k] + pvBif[f] * pvInc[k + 1] << ',' << lw << ',' << H[k] << ','
+                       << G[k] << ',' << f << '\n';
+                    pa += a;
+                    pt += a * (T[k] - 273.15f);
+                    pg += a * inc;
+                    pabs += a * ab;
+                    pp += a * power;
+                    plw += a * lw;
+                    ph += a * H[k];
+                    ps += a * G[k];
+                    pres = std::max(pres, double(std::abs(res)));
+                }
+            }
+            if (pa > 0) {
+                pvEnergyKwh += pp * set.dTime / 3600000.0;
+                pvSummary << node << ',' << token << ',' << pa << ',' << pt / pa << ',' << pg << ',' << pabs << ','
+                          << pp << ',' << plw << ',' << ph << ',' << ps << ',' << pres << ',' << pvEnergyKwh << '\n';
+                pvSummary.flush();
+                const std::vector<float>* cols[5] = {&pvPow, &T, &absorbedSW, nullptr, nullptr};
+                writeProcess(outDir, "photovoltaic", node, m.julianTime, token, sun, scene, st.couplingIterations,
+                             st.maxTemperatureDelta, converged, st.maxEnergyResidual, st.maxResidualEquivalent, set,
+                             cols, index);
+            }
+            if (!pv || !pvSummary) throw std::runtime_error("FacetEB cannot finish time-step output");
         }
         index.products.push_back(".facet_steps/energy_T=" + token + ".bin");
         index.products.push_back("photovoltaic_node_" + std::to_string(node) + ".csv");

@@ -9,8 +9,8 @@
 // voxel-size cells; one material per object mesh from the project's
 // spectra (custom spectra over the sensor bands), thermal, surface-energy
 // and soil entries; the FacetEB settings the project's control, light and
-// meteo blocks give.  Not read: DEM terrain, reflectance textures,
-// photovoltaic materials (rejected with a message), Hapke background
+// meteo blocks give (photovoltaic surface-energy entries included).  Not
+// read: DEM terrain, reflectance textures, Hapke background
 // heterogeneity (Lambert is used); PROSPECT, BSM and file spectra are
 // replaced by their entry's own listed values taken as a custom spectrum
 // (stated on stderr).  Fire, Fog and medium objects are skipped as the
@@ -52,7 +52,9 @@ struct ProjectMaterial {
     float heatCapacity = 60000.0f, convectiveScale = 1.0f;
     int soilMethod = 1;
     bool vegetation = true;
-    uint8_t component = 0;  // 0 unknown, 1 soil, 2 vegetation, 3 building, 5 water
+    bool pv = false;  // energyModel "photovoltaic"
+    float pvEta25 = 0.22f, pvGamma = -0.0035f, pvBifaciality = 0.0f;
+    uint8_t component = 0;  // 0 unknown, 1 soil, 2 vegetation, 3 building, 4 photovoltaic, 5 water
 
     // shortwave reflectance / transmittance of an energy window whose centre
     // is the 1-nm sample k (makeEnergyShortwaveOptics, core:952-1003, with a
@@ -553,13 +555,24 @@ inline Project loadProject(const std::string& file) {
             pd::surfaceParameters(cfg, sp, th, run.bands, m);
             const std::string en = energy.empty() ? std::string{} : energy[std::min(i, energy.size() - 1)];
             if (const json::Value* e = pd::named(mats, en)) {
-                // surfaceEnergy (core:1994-2003, 2925-2940)
+                // surfaceEnergy (core:2015-2026, 2953-2973)
                 const json::Value& pr = json::member(*e, "params", json::Value::Object);
                 const bool pv = json::str(*e, "energyModel") == "photovoltaic";
-                if (pv) throw std::runtime_error("Photovoltaic materials are not modelled by this engine: " + en);
-                m.vegetation = json::str(*e, "type") == "Vegetation" && json::str(*e, "energyModel") != "wood";
-                m.heatCapacity = json::num(pr, "heatCapacityPerArea", 60000.0f);
+                m.pv = pv;
+                m.vegetation = json::str(*e, "type") == "Vegetation" && json::str(*e, "energyModel") != "wood" && !pv;
+                m.heatCapacity = json::num(pr, "heatCapacityPerArea", pv ? 12000.0f : 60000.0f);
                 m.convectiveScale = json::num(pr, "convectiveScale", 1.0f);
+                m.pvEta25 = json::num(pr, "eta25", 0.22f);
+                m.pvGamma = json::num(pr, "gamma", -0.0035f);
+                m.pvBifaciality = json::num(pr, "bifaciality", 0.0f);
+                if (pv && (!(m.pvEta25 >= 0.0f && m.pvEta25 <= 0.5f) || !(m.pvGamma >= -0.02f && m.pvGamma <= 0.0f) ||
+                           !(m.pvBifaciality >= 0.0f && m.pvBifaciality <= 1.0f) || !(m.heatCapacity >= 0.0f) ||
+                           !(m.convectiveScale > 0.0f)))
+                    throw std::runtime_error("Invalid photovoltaic material parameters: " + en);
+                if (pv && (m.tirT > 1e-6f ||
+                           std::any_of(m.energyT.begin(), m.energyT.end(), [](float v) { return v > 1e-6f; })))
+                    throw std::runtime_error("Photovoltaic surface must be opaque: " + en);
+                if (pv) m.component = 4;
             }
             m.label = sp + "/" + th + "/" + en;
             bindings.push_back({meshNames.empty() ? std::string{} : meshNames[std::min(i, meshNames.size() - 1)],

@@ -140,18 +140,49 @@ def test_obj_groups_pick_their_mesh_material(tmp_path):
     assert min(t[:, 1].min() for t in tris) == 0.0
 
 
+_PANEL = [{"name": "panel", "type": "Building", "dimensions": [1, 1, 0.1], "spectralName": "pv_flat",
+           "materialName": "pv_solid"}]
+_PV_SPECTRA = [{"name": "soil", "model": "custom", "reflectance": "0.25", "transmittance": "0"},
+               {"name": "pv_flat", "model": "custom", "reflectance": "0.06", "transmittance": "0", "refTir": 0.05}]
+
+
+def test_photovoltaic_material_loads(tmp_path):
+    # a photovoltaic surface-energy entry is accepted on a scene object
+    # (core:2015-2026, 2953-2973): component 4 for its facets
+    _need_exe()
+    p = _write_project(tmp_path, _PANEL, spectra=_PV_SPECTRA,
+                       materials=[{"name": "soilset", "type": "Soil"},
+                                  {"name": "pv_solid", "type": "Building", "energyModel": "photovoltaic",
+                                   "params": {"eta25": 0.2, "gamma": -0.004, "bifaciality": 0.5}}])
+    r = subprocess.run([str(EXE), "--scene-obj", str(p), str(tmp_path / "o.obj")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 @pytest.mark.parametrize("case, message", [
-    ("pv", "Photovoltaic materials are not modelled"),
+    ("pv_ground", "Photovoltaic panels must be scene objects"),
+    ("pv_params", "Invalid photovoltaic material parameters: pv_solid"),
+    ("pv_opaque", "Photovoltaic surface must be opaque: pv_solid"),
     ("dem", "DEM terrain is not read"),
     ("missing", "Cannot open scene OBJ"),
     ("name", "accepts project.json only"),
 ])
 def test_project_errors(tmp_path, case, message):
+    # the reference's rules (core:1895-1900, 2963-2969) and this engine's limits
     _need_exe()
-    if case == "pv":
-        p = _write_project(tmp_path, [{"name": "panel", "type": "Building", "dimensions": [1, 1, 0.1],
-                                       "materialName": "pv_solid"}],
-                           materials=[{"name": "pv_solid", "type": "Building", "energyModel": "photovoltaic"}])
+    if case == "pv_ground":
+        p = _write_project(tmp_path, _PANEL, spectra=_PV_SPECTRA,
+                           materials=[{"name": "soilset", "type": "Soil", "energyModel": "photovoltaic"},
+                                      {"name": "pv_solid", "type": "Building", "energyModel": "photovoltaic"}])
+    elif case == "pv_params":
+        p = _write_project(tmp_path, _PANEL, spectra=_PV_SPECTRA,
+                           materials=[{"name": "soilset", "type": "Soil"},
+                                      {"name": "pv_solid", "type": "Building", "energyModel": "photovoltaic",
+                                       "params": {"eta25": 0.9}}])
+    elif case == "pv_opaque":
+        spectra = [dict(_PV_SPECTRA[0]), dict(_PV_SPECTRA[1], tauTir=0.1)]
+        p = _write_project(tmp_path, _PANEL, spectra=spectra,
+                           materials=[{"name": "soilset", "type": "Soil"},
+                                      {"name": "pv_solid", "type": "Building", "energyModel": "photovoltaic"}])
     elif case == "dem":
         p = _write_project(tmp_path, [], scene={"terrain": True, "demFile": "terrain/dem.tif"})
     elif case == "missing":
