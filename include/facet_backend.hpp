@@ -9,7 +9,10 @@
 //   * buildVisibilityGraph() casts Config::raysPerSide seeded cosine-weighted
 //     rays per side instead of rasterising the given directions (the
 //     argument is accepted and only checked for being non-empty);
-//   * computeSunlitFraction() samples Config::sunRaysPerSide points per side.
+//   * computeSunlitFraction() samples Config::sunRaysPerSide points per side;
+//   * with Config::rasterWidth > 0 both use the reference's raster method
+//     instead (rasterWidth x rasterHeight pixels, maxFragmentsPerPixel
+//     fragments per pixel, the given directions), with its results.
 #pragma once
 
 #include <cmath>
@@ -49,6 +52,11 @@ struct Config {  // facetvk::Config (facetrt_vulkan.h:37-47), ray fields + perio
     uint32_t periodicNeighborCount{0};
     float periodicSizeX{1.0f};
     float periodicSizeZ{1.0f};
+    // the reference's raster fields (facetrt_vulkan.h:38-40); rasterWidth 0
+    // selects the ray-cast graph and sunlit fraction
+    uint32_t rasterWidth{0};
+    uint32_t rasterHeight{0};
+    uint32_t maxFragmentsPerPixel{32};
 };
 
 struct GraphDiagnostics {  // facetvk::GraphDiagnostics (facetrt_vulkan.h:49-56)
@@ -101,6 +109,7 @@ public:
         c.periodicSizeX = config.periodicSizeX;
         c.periodicSizeZ = config.periodicSizeZ;
         check(fh_create(&c, &m_ctx), nullptr);
+        m_config = config;
     }
 
     void setGeometry(const std::vector<Vertex>& vertices) {
@@ -114,7 +123,16 @@ public:
         require();
         if (directions.empty()) throw std::invalid_argument("At least one visibility direction is required");
         fh_graph_diagnostics d{};
-        check(fh_build_form_factors(m_ctx, &d), m_ctx);
+        if (m_config.rasterWidth) {
+            std::vector<fh_direction> dirs;
+            dirs.reserve(directions.size());
+            for (const Direction& v : directions) dirs.push_back({v.x, v.y, v.z});
+            check(fh_build_raster_graph(m_ctx, dirs.data(), uint32_t(dirs.size()), m_config.rasterWidth,
+                                        m_config.rasterHeight, m_config.maxFragmentsPerPixel, &d),
+                  m_ctx);
+        } else {
+            check(fh_build_form_factors(m_ctx, &d), m_ctx);
+        }
         return {d.facetCount, d.surfaceCount, d.directedEdgeCount, d.fragmentOverflow, d.hashOverflow,
                 d.maxClosureError};
     }
@@ -173,9 +191,17 @@ public:
         const float len = std::sqrt(sunDirection.x * sunDirection.x + sunDirection.y * sunDirection.y +
                                     sunDirection.z * sunDirection.z);
         if (!(len > 1.0e-8f)) throw std::runtime_error("Zero-length direction or degenerate triangle");
+        std::vector<float> out(surfaceCount());
+        if (m_config.rasterWidth) {
+            // makeProjection takes the direction as given (vk:1071)
+            const fh_direction raw{sunDirection.x, sunDirection.y, sunDirection.z};
+            check(fh_compute_raster_sunlit_fraction(m_ctx, raw, m_config.rasterWidth, m_config.rasterHeight,
+                                                    m_config.maxFragmentsPerPixel, out.data()),
+                  m_ctx);
+            return out;
+        }
         const float inv = 1.0f / len;
         fh_direction d{sunDirection.x * inv, sunDirection.y * inv, sunDirection.z * inv};
-        std::vector<float> out(surfaceCount());
         check(fh_compute_sunlit_fraction(m_ctx, d, out.data()), m_ctx);
         return out;
     }
@@ -200,6 +226,7 @@ private:
     }
 
     fh_context* m_ctx{nullptr};
+    Config m_config{};
 };
 
 }  // namespace facethip

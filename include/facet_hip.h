@@ -147,6 +147,26 @@ int fh_incident_irradiance(fh_context *ctx, const float *radiosity,
  * config.sunRaysPerSide seeded sample points of a side that see the sun. */
 int fh_compute_sunlit_fraction(fh_context *ctx, fh_direction sun, float *fraction);
 
+/* buildVisibilityGraph(directions) (vkh:80, vk:890-1061) by the reference's
+ * raster method, in place of fh_build_form_factors: per direction an
+ * orthographic A-buffer of rasterWidth x rasterHeight pixels and at most
+ * maxFragmentsPerPixel fragments per pixel (Config, vkh:38-40) over the
+ * scene and its periodic copies; in each pixel the fragments in depth order
+ * add to both sides' denominators, the first camera-facing side and the
+ * last back side see the sky, and consecutive surfaces of different facets
+ * exchange (both directions).  Fragment overflow fails with
+ * FH_RUNTIME_ERROR (diagnostics still filled) and leaves no graph. */
+int fh_build_raster_graph(fh_context *ctx, const fh_direction *directions, uint32_t directionCount,
+                          uint32_t rasterWidth, uint32_t rasterHeight, uint32_t maxFragmentsPerPixel,
+                          fh_graph_diagnostics *out);
+
+/* computeSunlitFraction(direction) (vkh:98, vk:1064-1157) by the raster
+ * method: per side, covered pixels where it is the nearest surface over
+ * covered pixels (0 where it covers none). */
+int fh_compute_raster_sunlit_fraction(fh_context *ctx, fh_direction sun, uint32_t rasterWidth,
+                                      uint32_t rasterHeight, uint32_t maxFragmentsPerPixel,
+                                      float *fraction);
+
 /* facetCount() / surfaceCount() (vkh:103-104) */
 uint32_t fh_facet_count(const fh_context *ctx);
 uint32_t fh_surface_count(const fh_context *ctx);

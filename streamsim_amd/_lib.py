@@ -110,6 +110,8 @@ SIGNATURES = {
     "fh_solve_accelerated": (_I, [_P, _P, _F, _P, _U32, _U32, _F, _F, _P, C.POINTER(FhSolveStats)]),
     "fh_incident_irradiance": (_I, [_P, _P, _P, _F, _P]),
     "fh_compute_sunlit_fraction": (_I, [_P, FhDirection, _P]),
+    "fh_build_raster_graph": (_I, [_P, _P, _U32, _U32, _U32, _U32, C.POINTER(FhGraphDiagnostics)]),
+    "fh_compute_raster_sunlit_fraction": (_I, [_P, FhDirection, _U32, _U32, _U32, _P]),
     "fh_facet_count": (_U32, [_P]),
     "fh_surface_count": (_U32, [_P]),
     "fh_solve_bands": (_I, [_P, _U32, _P, _P, _P, _P, _P, _P, _U32, _F, _F, _U32, _P, C.POINTER(FhSolveStats)]),

@@ -166,6 +166,8 @@ void solve_bands_impl(fh_context* c, uint32_t nb, const float* dr, const float* 
 
 }  // namespace
 
+void fh::alloc_csr_device(fh_context* c, uint32_t nnz) { alloc_csr(c, nnz); }
+
 extern "C" {
 
 const char* fh_status_string(int status) {
@@ -333,6 +335,16 @@ int fh_set_geometry(fh_context* c, const fh_vertex* v, uint32_t vertexCount) {
         }
         c->F = F;
         c->S = 2u * F;
+        // scene bounds of the raster projection (vk:456-464)
+        for (int q = 0; q < 3; ++q) {
+            c->boundsMin[q] = FLT_MAX;
+            c->boundsMax[q] = -FLT_MAX;
+        }
+        for (uint32_t i = 0; i < vertexCount; ++i)
+            for (int q = 0; q < 3; ++q) {
+                c->boundsMin[q] = std::min(c->boundsMin[q], v[i].position[q]);
+                c->boundsMax[q] = std::max(c->boundsMax[q], v[i].position[q]);
+            }
         c->facetVerts.ensure(tri9.size());
         c->facetNormal.ensure(F);
         c->nodes.ensure(bvh.nodes.size());
@@ -543,6 +555,75 @@ int fh_compute_sunlit_fraction(fh_context* c, fh_direction sun, float* fraction)
         fh::launch_fill_u32(c, c->misc.p, S, 0u);
         fh::launch_sun(c, sun.x, sun.y, sun.z, c->misc.p, c->HH.p);
         download(c, fraction, c->HH.p, S);
+    });
+}
+
+// buildVisibilityGraph(directions) (vk:890-1061) by the reference's own
+// method: an orthographic A-buffer per direction, exchanges between
+// consecutive surfaces of each pixel's depth-sorted fragments
+int fh_build_raster_graph(fh_context* c, const fh_direction* directions, uint32_t directionCount,
+                          uint32_t rasterWidth, uint32_t rasterHeight, uint32_t maxFragmentsPerPixel,
+                          fh_graph_diagnostics* out) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_geometry(c, "buildVisibilityGraph()");
+        if (!directions || directionCount == 0u) fail(FH_INVALID_ARGUMENT, "Visibility directions must not be empty");
+        c->hasGraph = false;
+        std::vector<float> dirs(size_t(directionCount) * 3u);
+        for (uint32_t d = 0; d < directionCount; ++d) {
+            dirs[3u * d] = directions[d].x;
+            dirs[3u * d + 1u] = directions[d].y;
+            dirs[3u * d + 2u] = directions[d].z;
+        }
+        const uint32_t overflow =
+            fh::launch_raster_graph(c, dirs.data(), directionCount, rasterWidth, rasterHeight, maxFragmentsPerPixel);
+        if (out) {
+            out->facetCount = c->F;
+            out->surfaceCount = c->S;
+            out->directedEdgeCount = c->nnz;
+            out->fragmentOverflow = overflow;
+            out->hashOverflow = 0u;
+            out->maxClosureError = 0.0f;
+        }
+        if (overflow) fail(FH_RUNTIME_ERROR, "Visibility A-buffer overflow; increase Config::maxFragmentsPerPixel");
+        const uint32_t S = c->S;
+        std::vector<uint32_t> den(S);
+        download(c, den.data(), c->den.p, S);
+        bool any = false;
+        for (uint32_t s = 0; s < S; ++s) any = any || den[s] > 0u;
+        if (!any) fail(FH_RUNTIME_ERROR, "Visibility raster has no surface samples; radiation results are invalid");
+        c->fsky.ensure(S);
+        c->misc.ensure(4);
+        fh::launch_fill_u32(c, c->misc.p, 4, 0u);
+        // counts never exceed their row's den (each exchange adds to both
+        // sides' den); packable while every den fits 16 bits
+        uint32_t maxDen = 0;
+        for (uint32_t s = 0; s < S; ++s) maxDen = std::max(maxDen, den[s]);
+        fh::launch_weights(c, c->misc.p, maxDen, maxDen <= 65535u);
+        fh::build_pairs(c);
+        uint32_t bits = 0;
+        download(c, &bits, c->misc.p, 1);
+        std::memcpy(&c->closure, &bits, sizeof(float));
+        c->hasGraph = true;
+        if (out) out->maxClosureError = c->closure;
+    });
+}
+
+// computeSunlitFraction(direction) (vk:1064-1157) by the reference's raster
+// method: per side, the pixels where it is the nearest surface over the
+// pixels it covers
+int fh_compute_raster_sunlit_fraction(fh_context* c, fh_direction sun, uint32_t rasterWidth, uint32_t rasterHeight,
+                                      uint32_t maxFragmentsPerPixel, float* fraction) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_geometry(c, "computeSunlitFraction()");
+        if (!fraction) fail(FH_INVALID_ARGUMENT, "fraction must not be null");
+        const float dir[3] = {sun.x, sun.y, sun.z};
+        c->HH.ensure(c->S);
+        const uint32_t overflow =
+            fh::launch_raster_sunlit(c, dir, rasterWidth, rasterHeight, maxFragmentsPerPixel, c->HH.p);
+        if (overflow) fail(FH_RUNTIME_ERROR, "Direct-light A-buffer overflow; increase Config::maxFragmentsPerPixel");
+        download(c, fraction, c->HH.p, c->S);
     });
 }
 

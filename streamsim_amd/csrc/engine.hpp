@@ -202,6 +202,8 @@ struct fh_context {
     fh::DBuf<uint32_t> pk;
     uint32_t pkBits = 0;
     float closure = 0.0f;
+    float boundsMin[3] = {0.0f, 0.0f, 0.0f};  // vertex bounds (raster projection)
+    float boundsMax[3] = {0.0f, 0.0f, 0.0f};
     fh::DBuf<uint32_t> rowptr, col, cnt, den, sky;
     fh::DBuf<uint32_t> sched;        // processing order of facets in the sweeps (Morton order)
     // pair blocks of the band-parallel sweeps: units of two strongly

@@ -19,6 +19,18 @@ void launch_compact(fh_context* c, uint32_t R);
 void launch_weights(fh_context* c, uint32_t* closureBits, uint32_t maxCount, bool packable);
 void launch_sun(fh_context* c, float sx, float sy, float sz, uint32_t* lit, float* frac);
 
+// raster.hip: the reference's raster visibility graph (CSR, den, sky and the
+// weights' inputs in c; returns the fragment overflow count, the graph is
+// complete only at 0) and raster sunlit fraction (device frac [S]; returns
+// the overflow count)
+uint32_t launch_raster_graph(fh_context* c, const float* dirs, uint32_t ndirs, uint32_t W, uint32_t H,
+                             uint32_t layers);
+uint32_t launch_raster_sunlit(fh_context* c, const float sun[3], uint32_t W, uint32_t H, uint32_t layers,
+                              float* frac);
+
+// abi.hip: CSR entry arrays for nnz entries plus the zero pad
+void alloc_csr_device(fh_context* c, uint32_t nnz);
+
 // spmm.hip
 void launch_fold(fh_context* c, const float* e, const float* r, const float* t, const float* D, uint32_t ldb,
                  float* out);

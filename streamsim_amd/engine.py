@@ -129,6 +129,17 @@ class FacetEngine:
         self._check(self._lib.fh_build_form_factors(self._h, C.byref(d)))
         return _diag(d)
 
+    def buildRasterGraph(self, directions, rasterWidth=512, rasterHeight=512,
+                         maxFragmentsPerPixel=32) -> GraphDiagnostics:
+        """buildVisibilityGraph(directions) by the reference's raster method
+        (facetrt_vulkan.cpp:890-1061); the Config defaults (vkh:38-40)."""
+        d3 = _f32(directions).reshape(-1, 3)
+        dirs = (FhDirection * d3.shape[0])(*[FhDirection(float(x), float(y), float(z)) for x, y, z in d3])
+        d = FhGraphDiagnostics()
+        self._check(self._lib.fh_build_raster_graph(self._h, dirs, d3.shape[0], rasterWidth, rasterHeight,
+                                                    maxFragmentsPerPixel, C.byref(d)))
+        return _diag(d)
+
     def setGraph(self, rowOffsets, neighbors, counts, denominators, skyCounts) -> GraphDiagnostics:
         ro = np.ascontiguousarray(rowOffsets, dtype=np.uint32)
         nb = np.ascontiguousarray(neighbors, dtype=np.uint32)
@@ -157,6 +168,17 @@ class FacetEngine:
         out = np.zeros(S, np.float32)
         d = FhDirection(float(direction[0]), float(direction[1]), float(direction[2]))
         self._check(self._lib.fh_compute_sunlit_fraction(self._h, d, _ptr(out)))
+        return out
+
+    def computeRasterSunlitFraction(self, direction, rasterWidth=512, rasterHeight=512,
+                                    maxFragmentsPerPixel=32) -> np.ndarray:
+        """computeSunlitFraction(direction) by the raster method
+        (facetrt_vulkan.cpp:1064-1157)."""
+        S = self.surfaceCount()
+        out = np.zeros(S, np.float32)
+        d = FhDirection(float(direction[0]), float(direction[1]), float(direction[2]))
+        self._check(self._lib.fh_compute_raster_sunlit_fraction(self._h, d, rasterWidth, rasterHeight,
+                                                                maxFragmentsPerPixel, _ptr(out)))
         return out
 
     # -- radiosity -----------------------------------------------------------

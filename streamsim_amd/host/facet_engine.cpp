@@ -166,8 +166,19 @@ std::string outputPath(const std::string& input, const std::string& requested, c
 // with the soil grid's extent as the period (periodicSizeX/Z = scene width /
 // depth, core:2890-2894)
 // (a project's periodicNeighborCount is the default of the variable)
+// FACET_VISIBILITY=raster: the reference's raster graph and sunlit fraction
+// (FACET_RASTER_SIZE pixels square, default 1024; FACET_RASTER_LAYERS
+// fragments per pixel, default 64: the reference's kRasterSize /
+// kRasterLayers, core:75-76, where its adaptive choice stays at 64,
+// core:168-187) over the 40 hemisphere directions
 facethip::Config engineConfig(const Scene& s, uint32_t periodic = 0) {
     facethip::Config c;
+    if (const char* e = std::getenv("FACET_VISIBILITY"); e && std::string(e) == "raster") {
+        c.rasterWidth = c.rasterHeight = 1024u;
+        c.maxFragmentsPerPixel = 64u;
+        if (const char* r = std::getenv("FACET_RASTER_SIZE")) c.rasterWidth = c.rasterHeight = uint32_t(std::atoi(r));
+        if (const char* l = std::getenv("FACET_RASTER_LAYERS")) c.maxFragmentsPerPixel = uint32_t(std::atoi(l));
+    }
     if (const char* e = std::getenv("FACET_RAYS_PER_SIDE")) c.raysPerSide = uint32_t(std::atoi(e));
     if (const char* e = std::getenv("FACET_SUN_RAYS_PER_SIDE")) c.sunRaysPerSide = uint32_t(std::atoi(e));
     if (const char* e = std::getenv("FACET_PERIODIC_NEIGHBORS")) periodic = uint32_t(std::clamp(std::atoi(e), 0, 20));
@@ -177,6 +188,22 @@ facethip::Config engineConfig(const Scene& s, uint32_t periodic = 0) {
         c.periodicSizeZ = s.extentZ;
     }
     return c;
+}
+
+// makeHemisphereDirections (core:475-489): 40 golden-angle directions over
+// the upper hemisphere
+std::vector<facethip::Direction> hemisphereDirections() {
+    const uint32_t n = 40u;
+    const float goldenAngle = 2.39996322972865332f;
+    std::vector<facethip::Direction> d;
+    d.reserve(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float y = (float(i) + 0.5f) / float(n);
+        const float radius = std::sqrt(std::max(0.0f, 1.0f - y * y));
+        const float azimuth = goldenAngle * float(i);
+        d.push_back({radius * std::cos(azimuth), y, radius * std::sin(azimuth)});
+    }
+    return d;
 }
 
 bool isProject(const std::string& input);
@@ -191,9 +218,9 @@ int runFacetRT(const std::string& input, const std::string& outFile) try {
     facethip::FacetrtHip model;
     model.initialize(engineConfig(scene));
     model.setGeometry(scene.vertices);
-    std::cout << "PROGRESS\t10\tRay-cast form factors" << std::endl;
+    std::cout << "PROGRESS\t10\tVisibility graph" << std::endl;
     const auto tv = Clock::now();
-    const facethip::GraphDiagnostics graph = model.buildVisibilityGraph({facethip::Direction{}});
+    const facethip::GraphDiagnostics graph = model.buildVisibilityGraph(hemisphereDirections());
     const auto tv1 = Clock::now();
     const facethip::Direction sun = sunDirection(30.0f, 135.0f);
     const std::vector<float> sunlit = model.computeSunlitFraction(sun);
@@ -263,9 +290,9 @@ int runFacetRTProject(const std::string& input, const std::string& outFile) try 
     facethip::FacetrtHip model;
     model.initialize(engineConfig(scene, prj.run.periodicNeighbors));
     model.setGeometry(scene.vertices);
-    std::cout << "PROGRESS\t10\tRay-cast form factors" << std::endl;
+    std::cout << "PROGRESS\t10\tVisibility graph" << std::endl;
     const auto tv = Clock::now();
-    const facethip::GraphDiagnostics graph = model.buildVisibilityGraph({facethip::Direction{}});
+    const facethip::GraphDiagnostics graph = model.buildVisibilityGraph(hemisphereDirections());
     const auto tv1 = Clock::now();
     const facethip::Direction sun = sunDirection(prj.run.sunZenith, prj.run.sunAzimuth);
     const std::vector<float> sunlit = model.computeSunlitFraction(sun);
@@ -791,7 +818,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
     facethip::FacetrtHip model;
     model.initialize(engineConfig(scene, project ? prj.run.periodicNeighbors : 0u));
     model.setGeometry(scene.vertices);
-    const facethip::GraphDiagnostics graph = model.buildVisibilityGraph({facethip::Direction{}});
+    const facethip::GraphDiagnostics graph = model.buildVisibilityGraph(hemisphereDirections());
     fh_context* ctx = model.handle();
     // object material = SurfaceParameters{}, ground = {0.20},{0.0},0.05 (core:113-143, 437-440)
     fh_eb_material mats[2] = {};

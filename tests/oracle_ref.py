@@ -159,12 +159,12 @@ def raster_graph(vertices, bmin, bmax, raster=1024, layers=64, periodic=0, px=1.
     return out
 
 
-def raster_sunlit(vertices, bmin, bmax, sun, raster=1024, layers=64) -> np.ndarray:
+def raster_sunlit(vertices, bmin, bmax, sun, raster=1024, layers=64, periodic=0, px=1.0, pz=1.0) -> np.ndarray:
     v = np.ascontiguousarray(vertices, np.float32)
     F = v.shape[0] // 3
     out = np.zeros(2 * F, np.float32)
     rc = load().or_raster_sunlit(_p(v), F, _p(np.asarray(bmin, np.float32)), _p(np.asarray(bmax, np.float32)),
-                                 raster, layers, 0, 1.0, 1.0, _p(np.asarray(sun, np.float32)), _p(out))
+                                 raster, layers, periodic, px, pz, _p(np.asarray(sun, np.float32)), _p(out))
     if rc:
         raise RuntimeError("CPU sunlight A-buffer overflow")
     return out

@@ -421,3 +421,45 @@ def test_cli_facetrt_project_bands(oracle, tmp_path, solver):
     want[:, :2] = np.clip(want[:, :2] / refl, 0, 1)
     assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-3)) < 1e-4
     assert res["iterations"] == 64 if solver == "traditional" else res["iterations"] < 64
+
+
+@pytest.mark.gpu
+def test_gpu_raster_graph_is_the_oracle_graph(oracle, raster):
+    # the device raster graph and sunlit fraction at the stored run's
+    # settings (1024 x 1024, 64 fragments, 40 hemisphere directions) equal
+    # the oracle's bit for bit, so the solve agrees with the stored output
+    print("golden gpu raster graph", flush=True)
+    from streamsim_amd import FacetEngine
+    sc, z, g, sun, sl = raster
+    eng = FacetEngine()
+    eng.setGeometry(sc.vertices)
+    d = eng.buildRasterGraph(oracle.hemisphere_directions(40), 1024, 1024, 64)
+    assert d.fragmentOverflow == 0
+    gg = eng.getGraph()
+    for k in ("rowOffsets", "neighbors", "counts", "denominators", "skyCounts"):
+        assert np.array_equal(getattr(gg, k), getattr(g, k)), k
+    f = eng.computeRasterSunlitFraction(sun, 1024, 1024, 64)
+    assert np.array_equal(f, sl)
+    res = eng.solve(optics.make_optics(sc, f, sun), 100.0, 64, 1.0)
+    _agreement(res.radiosity, z["radiosity"])
+
+
+@pytest.mark.gpu
+def test_cli_facetrt_raster_graph_matches_reference_output(tmp_path, raster):
+    # the command-line engine with the reference's raster graph
+    # (FACET_VISIBILITY=raster: 1024 x 1024, 64 fragments, 40 directions):
+    # sunlit fraction is the oracle's, radiosity agrees with the stored run
+    # per side as the oracle's raster path does
+    print("golden cli raster", flush=True)
+    import os
+    sc, z, g, sun, sl = raster
+    exe = Path(__file__).resolve().parents[1] / "streamsim_amd" / "facet_engine"
+    out = tmp_path / "facetrt.json"
+    env = dict(os.environ, FACET_VISIBILITY="raster")
+    r = subprocess.run([str(exe), "eFacetRT", str(HERE / "urban_tree_building.obj"), str(out)], capture_output=True,
+                       text=True, timeout=120, env=env)
+    assert r.returncode == 0, r.stderr
+    d = json.loads(out.read_text())
+    assert d["graph"]["directedEdges"] == g.nnz and d["graph"]["fragmentOverflow"] == 0
+    assert np.allclose(np.asarray(d["sunlit"], np.float32), sl, rtol=0, atol=1e-6)
+    _agreement(np.asarray(d["radiosity"], np.float32), z["radiosity"])

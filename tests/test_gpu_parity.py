@@ -105,6 +105,58 @@ def test_periodic_graph_and_sunlit_bitexact(oracle, neighbors):
         assert np.array_equal(f, of), (zen, az)
 
 
+def _vertex_bounds(sc):
+    p = np.asarray(sc.vertices, np.float32)[:, :3]
+    return p.min(axis=0), p.max(axis=0)
+
+
+# the reference's raster graph on the device: CSR, denominators and sky
+# counts equal the oracle's A-buffer restatement (the fragments of a pixel in
+# depth order, ties in submission order) for the same directions; 384 pixels
+# across with 8 periodic copies, 512 without
+@pytest.mark.parametrize("neighbors,size", [(0, 512), (8, 384)])
+def test_raster_graph_and_sunlit_bitexact(oracle, neighbors, size):
+    print(f"raster graph case neighbors={neighbors} size={size}", flush=True)
+    from streamsim_amd import FacetEngine
+    sc = scenes.row_crop(leaves_per_row=100, cells=12)
+    eng = FacetEngine(rays_per_side=32, sun_rays_per_side=16, seed=1, periodic_neighbors=neighbors,
+                      periodic_size=(10.0, 10.0))
+    eng.setGeometry(sc.vertices)
+    dirs = oracle.hemisphere_directions(24)
+    d = eng.buildRasterGraph(dirs, size, size, 64)
+    g = eng.getGraph()
+    lo, hi = _vertex_bounds(sc)
+    og = oracle.raster_graph(sc.vertices, lo, hi, raster=size, layers=64, periodic=neighbors, px=10.0, pz=10.0,
+                             dirs=dirs)
+    assert d.fragmentOverflow == 0 and d.directedEdgeCount == og.nnz > 0
+    for k in ("rowOffsets", "neighbors", "counts", "denominators", "skyCounts"):
+        assert np.array_equal(getattr(g, k), getattr(og, k)), k
+    assert abs(d.maxClosureError - og.closure) < 1e-6
+    for zen, az in ((30.0, 135.0), (70.0, 260.0)):
+        sun = oracle.sun_direction(zen, az)
+        f = eng.computeRasterSunlitFraction(sun, size, size, 64)
+        of = oracle.raster_sunlit(sc.vertices, lo, hi, sun, raster=size, layers=64, periodic=neighbors, px=10.0,
+                                  pz=10.0)
+        assert np.array_equal(f, of), (zen, az)
+
+
+def test_raster_graph_overflow_reported(oracle):
+    # two fragments allowed per pixel: the leaves over the ground overflow,
+    # the call fails and leaves no graph
+    print("raster overflow case", flush=True)
+    from streamsim_amd import FacetEngine
+    from streamsim_amd._lib import LogicError
+    sc = scenes.row_crop(leaves_per_row=60, cells=8)
+    eng = FacetEngine(rays_per_side=32, sun_rays_per_side=16, seed=1)
+    eng.setGeometry(sc.vertices)
+    with pytest.raises(RuntimeError, match="A-buffer overflow"):
+        eng.buildRasterGraph(oracle.hemisphere_directions(8), 256, 256, 1)
+    with pytest.raises(LogicError):
+        eng.getGraph()
+    with pytest.raises(RuntimeError, match="A-buffer overflow"):
+        eng.computeRasterSunlitFraction(oracle.sun_direction(10.0, 0.0), 256, 256, 1)
+
+
 def test_reference_solve_on_raster_graph(oracle):
     # the reference's own graph (A-buffer restatement) uploaded through
     # fh_set_graph; fixed 64 sweeps from zero vs solveCpu (FP64 gather)
