@@ -256,6 +256,13 @@ typedef struct {
  * ground (one temperature per side).  swReflectance / swTransmittance are
  * [materialCount][swBands]: the window-centre optics of each shortwave band
  * (makeEnergyShortwaveOptics, core:952-1003). */
+/* scene.backgroundAngularStrength (core:2904-2910, Hapke background,
+ * core:546-605): strength in [0, 1] (clamped) of the ground's directional
+ * shortwave reflectance and angular longwave emissivity in the energy
+ * balance; 0 (the default) keeps the ground Lambertian.  Kept across
+ * fh_eb_set_scene calls. */
+int fh_eb_set_background_angular_strength(fh_context *ctx, float strength);
+
 int fh_eb_set_scene(fh_context *ctx, uint32_t leafFacetCount, const uint32_t *facetMaterial,
                     uint32_t materialCount, const fh_eb_material *materials, uint32_t swBands,
                     const float *swReflectance, const float *swTransmittance);

@@ -146,6 +146,61 @@ void or_sun_direction(float zenith_deg, float azimuth_deg, float xyz[3])
 }
 
 /* ------------------------------------------------------------------ */
+/* Hapke background: core:546-605 */
+static float clampf_(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+
+static float hapke_h(float mu, float w)                       /* core:546-550 */
+{
+    const float g = sqrtf(fmaxf(1.0f - w, 1.0e-6f));
+    return (1.0f + 2.0f * mu) / (1.0f + 2.0f * mu * g);
+}
+
+static float hapke_kernel(float mu0, float mu, float phase, float w)  /* core:552-563 */
+{
+    const float pi = 3.14159265358979323846f;
+    const float surge = 1.0f / (1.0f + tanf(0.5f * phase) / 0.1f);
+    const float multiple = hapke_h(mu0, w) * hapke_h(mu, w) - 1.0f;
+    return w / (4.0f * pi) * mu0 / fmaxf(mu0 + mu, 1.0e-5f) * (1.0f + surge + multiple);
+}
+
+float or_hapke_reflectance(float reflectance, const float incoming[3], const float outgoing[3],
+                           const float normal[3], float strength)      /* core:565-587 */
+{
+    const float pi = 3.14159265358979323846f;
+    float n[3], in[3], out[3];
+    normalize3(normal, n);
+    normalize3(incoming, in);
+    normalize3(outgoing, out);
+    if (dot3(in, n) < 0.0f) for (int k = 0; k < 3; ++k) in[k] = in[k] * -1.0f;
+    if (dot3(out, n) < 0.0f) for (int k = 0; k < 3; ++k) out[k] = out[k] * -1.0f;
+    const float mu0 = fmaxf(dot3(n, in), 1.0e-4f);
+    const float mu = fmaxf(dot3(n, out), 1.0e-4f);
+    const float phase = acosf(clampf_(dot3(in, out), -1.0f, 1.0f));
+    const float base = clampf_(reflectance, 0.0f, 0.999f);
+    const float w = clampf_(4.0f * base / powf(1.0f + base, 2.0f), 1.0e-5f, 0.999f);
+    const float directional = hapke_kernel(mu0, mu, phase, w);
+    const float h1 = hapke_h(1.0f, w);
+    const float normalIncidence = w / (8.0f * pi) * h1 * h1;
+    const float value = clampf_(base * directional / fmaxf(normalIncidence, 1.0e-6f), 0.0f, 1.0f);
+    return base + (value - base) * clampf_(strength, 0.0f, 1.0f);
+}
+
+static float hapke_ir(float mu, float w)
+{
+    const float root = sqrtf(1.0f - w);
+    return root * (1.0f + 2.0f * mu / 1.5f) / (1.0f + 2.0f * mu / 1.5f * root);
+}
+
+float or_hapke_emissivity(float emissivity, float cosine, float strength)  /* core:589-605 */
+{
+    const float base = clampf_(emissivity, 0.0f, 1.0f);
+    const float w = clampf_(1.0f - base * base, 1.0e-5f, 0.999f);
+    const float value = clampf_(base * hapke_ir(clampf_(cosine, 1.0e-4f, 1.0f), w) /
+                                    fmaxf(hapke_ir(1.0f, w), 1.0e-6f), 0.0f, 1.0f);
+    return base + (value - base) * clampf_(strength, 0.0f, 1.0f);
+}
+
+/* ------------------------------------------------------------------ */
 /* A-buffer rasterizer: core:1212-1335 */
 typedef struct { float depth; uint32_t side; } fragment_t;
 
@@ -1293,7 +1348,6 @@ uint32_t or_eb_node(const or_graph *g, uint32_t F, uint32_t leafCount,
                     float *maxResEqOut)
 {
     const uint32_t S = 2u * F;
-    (void)normalY;
     float *r = (float *)malloc(sizeof(float) * S);
     float *tt = (float *)malloc(sizeof(float) * S);
     float *e = (float *)malloc(sizeof(float) * S);
@@ -1310,13 +1364,17 @@ uint32_t or_eb_node(const or_graph *g, uint32_t F, uint32_t leafCount,
     uint32_t iters = 0;
     double maxDT = 0.0, maxEq = 0.0;
 
-    /* LW optics from T (core:1005-1036), with Hapke strength 0 */
+    /* LW optics from T (core:1005-1036); ground sides with the Hapke
+     * emissivity at |n_y| (core:1025-1027) */
     #define LW_OPTICS()                                                             \
         for (uint32_t s = 0; s < S; ++s) {                                          \
             const or_eb_material *m = &mats[material[s >> 1]];                     \
             const float rr = fminf(fmaxf(m->tirReflectance, 0.0f), 1.0f);           \
             const float t0 = fminf(fmaxf(m->tirTransmittance, 0.0f), 1.0f - rr);    \
-            const float eps = fmaxf(0.0f, 1.0f - rr - t0);                          \
+            float eps = fmaxf(0.0f, 1.0f - rr - t0);                                \
+            if ((s >> 1) >= leafCount && p->backgroundAngularStrength > 0.0f)       \
+                eps = or_hapke_emissivity(eps, fabsf(normalY[s >> 1]),              \
+                                          p->backgroundAngularStrength);            \
             r[s] = fminf(fmaxf(1.0f - t0 - eps, 0.0f), 1.0f);                       \
             tt[s] = t0;                                                             \
             const double Tc = clampd(T[s], 1.0, 1000.0);                            \

@@ -182,7 +182,16 @@ typedef struct {
     uint32_t vegetationMethod;
     uint32_t lwMaxSweeps;    /* inner LW sweeps cap */
     float lwTolRel;          /* inner LW relative tolerance (<0: fixed sweeps) */
+    float backgroundAngularStrength; /* Hapke strength of the ground (0: Lambert) */
 } or_eb_params;
+
+/* Hapke background (core:546-605).  Shortwave reflectance of a ground side
+ * (normal n) lit from `incoming` and seen from `outgoing`, and the ground's
+ * longwave emissivity at cosine |n_y|; both blended with the Lambertian
+ * value by strength in [0, 1]. */
+float or_hapke_reflectance(float reflectance, const float incoming[3], const float outgoing[3],
+                           const float normal[3], float strength);
+float or_hapke_emissivity(float emissivity, float cosine, float strength);
 
 /* One FacetEB node (core:3567-3846) in FP64: coupling loop of
  * LW optics -> LW solve -> incident -> fluxes -> relaxed Newton, then the

@@ -121,6 +121,7 @@ SIGNATURES = {
     "fh_sweep_device": (_I, [_P, _U32, _P, _P, _P, _P, _P, _U32]),
     "fh_band_stride": (_U32, [_U32]),
     "fh_stream": (_P, [_P]),
+    "fh_eb_set_background_angular_strength": (_I, [_P, _F]),
     "fh_eb_set_scene": (_I, [_P, _U32, _P, _U32, _P, _U32, _P, _P]),
     "fh_eb_run_node": (_I, [_P, C.POINTER(FhEbNodeParams), _P, _P, C.POINTER(FhEbNodeStats)]),
     "fh_eb_get_state": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdlib>
 
+#include "hapke.hpp"
 #include "solver.hpp"
 #include "sweep_common.hpp"
 
@@ -86,7 +87,10 @@ __global__ void k_areas(const float* __restrict__ fv, uint32_t F, double* __rest
 
 // per-side LW optics constants (makeEnergyLongwaveOptics, core:1005-1036,
 // angular strength 0)
+// ground sides (facets from leafCount on) take the Hapke emissivity at
+// |n_y| when strength > 0 (core:1025-1027)
 __global__ void k_lw_setup(uint32_t S, const uint32_t* __restrict__ mat, const fh_eb_material* __restrict__ mats,
+                           uint32_t leafCount, const float4* __restrict__ nrm, float strength,
                            float* __restrict__ r, float* __restrict__ t, float* __restrict__ eps) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
@@ -94,6 +98,13 @@ __global__ void k_lw_setup(uint32_t S, const uint32_t* __restrict__ mat, const f
     const float rr = fminf(fmaxf(m.tirReflectance, 0.0f), 1.0f);
     const float tt = fminf(fmaxf(m.tirTransmittance, 0.0f), 1.0f - rr);
     const float e = fmaxf(0.0f, 1.0f - rr - tt);
+    if (strength > 0.0f && (s >> 1) >= leafCount) {
+        const float eh = hapke::emissivity(e, fabsf(nrm[s >> 1].y), strength);
+        r[s] = fminf(fmaxf(1.0f - tt - eh, 0.0f), 1.0f);
+        t[s] = tt;
+        eps[s] = eh;
+        return;
+    }
     r[s] = fminf(fmaxf(1.0f - tt - e, 0.0f), 1.0f);
     t[s] = tt;
     eps[s] = fminf(fmaxf(1.0f - r[s] - tt, 0.0f), 1.0f);
@@ -112,14 +123,16 @@ __global__ void k_lw_source(uint32_t S, const float* __restrict__ T, const float
 }
 
 // SW window optics per side and band (makeEnergyShortwaveOptics,
-// core:952-1003, angular strength 0, no reflectance texture)
+// core:952-1003, no reflectance texture); with strength > 0 the ground
+// sides' reflectance is the Hapke value for this sun (core:988-996).
 // Internal band b is window order[b]; dni is given in internal order.
 __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_t* __restrict__ mat,
                             const float* __restrict__ refl, const float* __restrict__ trans,
                             const uint32_t* __restrict__ order, const float4* __restrict__ nrm,
                             const float* __restrict__ sunlit, float sx, float sy, float sz,
-                            const float* __restrict__ dni, float* __restrict__ r, float* __restrict__ t,
-                            float* __restrict__ D, float* __restrict__ dir) {
+                            const float* __restrict__ dni, uint32_t leafCount, float strength,
+                            float* __restrict__ r, float* __restrict__ t, float* __restrict__ D,
+                            float* __restrict__ dir) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(S) * ldb) return;
     const uint32_t s = uint32_t(i / ldb), b = uint32_t(i - size_t(s) * ldb);
@@ -136,8 +149,15 @@ __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_
         return;
     }
     const uint32_t m = mat[s >> 1], w = order[b];
-    const float rr = fminf(fmaxf(refl[size_t(m) * nb + w], 0.0f), 1.0f);
+    float rr = fminf(fmaxf(refl[size_t(m) * nb + w], 0.0f), 1.0f);
     const float tt = fminf(fmaxf(trans[size_t(m) * nb + w], 0.0f), 1.0f - rr);
+    if (strength > 0.0f && (s >> 1) >= leafCount) {
+        const float4 nf = nrm[s >> 1];
+        const float sg = (s & 1u) ? -1.0f : 1.0f;
+        const float sn[3] = {nf.x * sg, nf.y * sg, nf.z * sg};
+        const float sun[3] = {sx, sy, sz};
+        rr = hapke::reflectance(rr, sun, sn, sn, strength);
+    }
     r[i] = rr;
     t[i] = fminf(tt, 1.0f - rr);
     const float4 n = nrm[s >> 1];
@@ -703,8 +723,8 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     e.dsum.ensure(4);
     e.fout.ensure(1);
     hipLaunchKernelGGL(k_areas, blocks_for(F), dim3(256), 0, c->stream, c->facetVerts.p, F, e.area.p);
-    hipLaunchKernelGGL(k_lw_setup, blocks_for(S), dim3(256), 0, c->stream, S, e.material.p, e.mats.p, e.rlw.p,
-                       e.tlw.p, e.eps.p);
+    hipLaunchKernelGGL(k_lw_setup, blocks_for(S), dim3(256), 0, c->stream, S, e.material.p, e.mats.p, leafCount,
+                       c->facetNormal.p, c->hapkeStrength, e.rlw.p, e.tlw.p, e.eps.p);
     launch_fill_f32(c, e.zero.p, S, 0.0f);
     launch_fill_f32(c, e.absorbed.p, S, 0.0f);
     launch_fill_f32(c, e.incident.p, S, 0.0f);
@@ -725,7 +745,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
         launch_fill_f32(c, e.dni.p, ldb, 0.0f);
         hipLaunchKernelGGL(k_sw_optics, blocks_for(nsw), dim3(256), 0, c->stream, S, swNb, ldb, e.material.p,
                            e.swRefl.p, e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, 0.0f, 1.0f, 0.0f,
-                           e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
+                           e.dni.p, e.leafCount, c->hapkeStrength, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
         FH_HIP(hipGetLastError());
         e.swRho2.ensure(ldb);
         estimate_rho2(c, ldb, swNb, e.rsw.p, e.tsw.p, e.Bsw1.p, e.Hsw.p, e.csw.p, e.swRho2.p);
@@ -891,7 +911,11 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         // D from the per-side direct factors, and the absorbed pass does the
         // same: the [S][ldb] optics arrays and the folded source are only
         // built for the entry-parallel layouts (and the spectral estimate)
-        const bool tab = !noTables && uses_bf16_gather(ldb);
+        // a Hapke ground's reflectance follows the sun: per-side arrays, and
+        // the spectral radius estimated again for every node
+        const bool hapkeOn = c->hapkeStrength > 0.0f;
+        if (hapkeOn) e.swRhoValid = false;
+        const bool tab = !noTables && !hapkeOn && uses_bf16_gather(ldb);
         const uint32_t precision = bf16_pays(ldb) ? p.precision : FH_PRECISION_FP32;
         // an increment solve keeps the gathers of every band's last sweep:
         // the absorbed pass forms the incident from them
@@ -899,7 +923,7 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         const auto optics_arrays = [&] {
             hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p,
                                e.swRefl.p, e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, p.sun.x, p.sun.y,
-                               p.sun.z, e.dni.p, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
+                               p.sun.z, e.dni.p, e.leafCount, c->hapkeStrength, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
             FH_HIP(hipGetLastError());
         };
         if (!tab || !e.swRhoValid) {
@@ -1073,6 +1097,23 @@ int guarded_eb(fh_context* c, Fn&& body) {
 }  // namespace
 
 extern "C" {
+
+// scene.backgroundAngularStrength (core:2904-2910), clamped to [0, 1]
+int fh_eb_set_background_angular_strength(fh_context* c, float strength) {
+    return guarded_eb(c, [&] {
+        if (!std::isfinite(strength)) fail(FH_INVALID_ARGUMENT, "background angular strength must be finite");
+        c->hapkeStrength = std::min(std::max(strength, 0.0f), 1.0f);
+        fh::EbState& e = c->eb;
+        if (e.ready) {
+            hipLaunchKernelGGL(fh::k_lw_setup, fh::blocks_for(c->S), dim3(256), 0, c->stream, c->S, e.material.p,
+                               e.mats.p, e.leafCount, c->facetNormal.p, c->hapkeStrength, e.rlw.p, e.tlw.p,
+                               e.eps.p);
+            FH_HIP(hipGetLastError());
+            FH_HIP(hipStreamSynchronize(c->stream));
+            e.swRhoValid = false;
+        }
+    });
+}
 
 int fh_eb_set_scene(fh_context* c, uint32_t leafFacetCount, const uint32_t* facetMaterial, uint32_t materialCount,
                     const fh_eb_material* materials, uint32_t swBands, const float* swReflectance,

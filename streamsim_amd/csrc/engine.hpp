@@ -202,6 +202,7 @@ struct fh_context {
     fh::DBuf<uint32_t> pk;
     uint32_t pkBits = 0;
     float closure = 0.0f;
+    float hapkeStrength = 0.0f;  // Hapke strength of the ground (energy balance optics)
     float boundsMin[3] = {0.0f, 0.0f, 0.0f};  // vertex bounds (raster projection)
     float boundsMax[3] = {0.0f, 0.0f, 0.0f};
     fh::DBuf<uint32_t> rowptr, col, cnt, den, sky;

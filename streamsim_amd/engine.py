@@ -253,6 +253,11 @@ class FacetEngine:
         return out
 
     # -- FacetEB ---------------------------------------------------------------
+    def ebSetBackgroundAngularStrength(self, strength: float) -> None:
+        """scene.backgroundAngularStrength (facetrtcore.cpp:2904-2910): the
+        Hapke ground of the energy balance, 0 = Lambertian."""
+        self._check(self._lib.fh_eb_set_background_angular_strength(self._h, float(strength)))
+
     def ebSetScene(self, leaf_count: int, facet_material, materials, sw_reflectance=None, sw_transmittance=None):
         """materials: sequence of dicts with the FhEbMaterial fields; sw tables
         [materialCount][swBands] (window-centre optics)."""

@@ -37,7 +37,8 @@ class OrEbMaterial(C.Structure):
 class OrEbParams(C.Structure):
     _fields_ = [("airTemperatureK", _F), ("vaporPressure", _F), ("pressure", _F), ("windSpeed", _F),
                 ("skyLongwave", _F), ("dTime", _F), ("tolerance", _F), ("relaxation", _F), ("maxCoupling", _U),
-                ("vegetationMethod", _U), ("lwMaxSweeps", _U), ("lwTolRel", _F)]
+                ("vegetationMethod", _U), ("lwMaxSweeps", _U), ("lwTolRel", _F),
+                ("backgroundAngularStrength", _F)]
 
 
 _lib = None
@@ -55,6 +56,8 @@ def load():
         "or_graph_copy": (None, [C.POINTER(OrGraph), _P, _P, _P, _P, _P]),
         "or_graph_from_arrays": (_I, [C.POINTER(OrGraph), _U, _U, _P, _P, _P, _P, _P]),
         "or_hemisphere_directions": (None, [_U, _P]),
+        "or_hapke_reflectance": (_F, [_F, _P, _P, _P, _F]),
+        "or_hapke_emissivity": (_F, [_F, _F, _F]),
         "or_sun_direction": (None, [_F, _F, _P]),
         "or_raster_graph": (_I, [_P, _U, _P, _P, _U, _U, _U, _F, _F, _P, _U, C.POINTER(OrGraph)]),
         "or_raster_sunlit": (_I, [_P, _U, _P, _P, _U, _U, _U, _F, _F, _P, _P]),
@@ -260,8 +263,18 @@ def solve_direct(g: Graph, optics, sky: float) -> np.ndarray:
     return out
 
 
+def hapke_reflectance(reflectance, incoming, outgoing, normal, strength) -> float:
+    f = lambda v: np.ascontiguousarray(v, np.float32)  # noqa: E731
+    return float(load().or_hapke_reflectance(reflectance, _p(f(incoming)), _p(f(outgoing)), _p(f(normal)),
+                                             strength))
+
+
+def hapke_emissivity(emissivity, cosine, strength) -> float:
+    return float(load().or_hapke_emissivity(emissivity, cosine, strength))
+
+
 def eb_node(g: Graph, F: int, leaf_count: int, material, mats, absorbed, params: OrEbParams, T, Tprev,
-            incident=None):
+            incident=None, normal_y=None):
     h = _Held(g)
     mat = np.ascontiguousarray(material, np.uint32)
     marr = (OrEbMaterial * len(mats))(*mats)
@@ -272,7 +285,10 @@ def eb_node(g: Graph, F: int, leaf_count: int, material, mats, absorbed, params:
     mdt = np.zeros(1, np.float32)
     meq = np.zeros(1, np.float32)
     inc = None if incident is None else np.ascontiguousarray(incident, np.float32)
-    it = load().or_eb_node(h.ref(), F, leaf_count, _p(mat), C.cast(marr, _P), None,
+    ny = None if normal_y is None else np.ascontiguousarray(normal_y, np.float32)
+    if params.backgroundAngularStrength > 0 and ny is None:
+        raise ValueError("the Hapke ground needs the facet normals' y components")
+    it = load().or_eb_node(h.ref(), F, leaf_count, _p(mat), C.cast(marr, _P), _p(ny),
                            _p(np.ascontiguousarray(absorbed, np.float32)), _p(inc), C.byref(params), _p(T), _p(Tp),
                            _p(Blw),
                            C.cast(C.pointer(conv), _P), _p(mdt), _p(meq))

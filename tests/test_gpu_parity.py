@@ -345,6 +345,86 @@ def test_eb_node_matches_oracle(oracle, nb):
     assert _rel(s["longwaveRadiosity"], Blw) < 1e-4
 
 
+# Hapke ground (core:546-605) at strength 0.7: the ground facets' normals
+# tilted so that |n_y| < 1 (the longwave emissivity then varies); shortwave
+# reflectance per ground side and band from the sun, longwave emissivity
+# per ground facet, against the oracle's restatement
+@pytest.mark.parametrize("nb", [4, 24])
+def test_eb_node_hapke_ground_matches_oracle(oracle, nb):
+    print(f"eb hapke case nb={nb}", flush=True)
+    strength = 0.7
+    sc = scenes.row_crop(leaves_per_row=40, cells=6)
+    v = np.array(sc.vertices, np.float32, copy=True)
+    ground = np.arange(sc.leaf_count, sc.facet_count)
+    n = np.stack([np.where(ground % 2 == 0, 0.35, -0.5), np.ones(ground.size), np.full(ground.size, 0.2)], 1)
+    n = (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+    for k in range(3):
+        v[3 * ground + k, 3:6] = n
+    sc.vertices = v
+    eng = _engine(R=128, K=32, seed=4)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    win = faceteb.ShortwaveWindows.for_bands(nb)
+    faceteb.setup_scene(eng, sc, win)
+    eng.ebSetBackgroundAngularStrength(strength)
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    cfg = faceteb.EbConfig(tolerance=1e-3, radiosity_tolerance=1e-7, max_coupling=200)
+    met = faceteb.diurnal_meteo(24)[10]
+    dni, dif = win.irradiance(met.shortwave, math.cos(math.radians(met.zenith)))
+    st = eng.ebRunNode(faceteb.node_params(met, cfg, first=True), dni, dif)
+    assert st.converged
+    s = eng.ebGetState()
+    r_tab, t_tab = faceteb.material_tables(["leaf", "soil", "building"], win)
+    mat = np.repeat(sc.material, 2)
+    r = r_tab[mat].copy()
+    t = np.minimum(np.clip(t_tab[mat], 0, 1 - r), 1 - r)
+    sun = faceteb.sun_vector(met.zenith, met.azimuth)
+    sn = optics.side_normals(sc)
+    for side in range(2 * sc.leaf_count, sc.surface_count):
+        for b in range(nb):
+            r[side, b] = oracle.hapke_reflectance(r[side, b], sun, sn[side], sn[side], strength)
+    t = np.minimum(t, 1 - r)
+    assert np.ptp(r[2 * sc.leaf_count:, 0]) > 1e-3  # the sides' reflectance follows their normal
+    cos = np.maximum(0.0, sn @ sun).astype(np.float32)
+    D = (np.clip(s["sunlit"], 0, 1) * cos)[:, None] * dni[None, :]
+    sw, _, _ = oracle.solve_bands(og, r, t, np.zeros_like(r), D, dif, tol_rel=1e-9)
+    assert _rel(s["shortwaveRadiosity"][sw > 1e-3], sw[sw > 1e-3]) < 1e-4
+    H = oracle.incident_bands(og, sw, D, dif)
+    absorbed = np.sum(np.clip(1 - r - t, 0, 1) * H, axis=1)
+    assert _rel(s["absorbedShortwave"][absorbed > 1], absorbed[absorbed > 1]) < 1e-4
+    mats = []
+    for m in (faceteb.LEAF, faceteb.SOIL, faceteb.BUILDING):
+        om = oracle.OrEbMaterial()
+        for k, val in m.items():
+            setattr(om, k, val)
+        mats.append(om)
+    op = oracle.OrEbParams()
+    op.airTemperatureK = met.air_temperature_k
+    op.vaporPressure = met.vapor_pressure
+    op.pressure = met.pressure
+    op.windSpeed = met.wind
+    op.skyLongwave = met.longwave
+    op.dTime = cfg.dtime
+    op.tolerance = 1e-3
+    op.relaxation = cfg.relaxation
+    op.maxCoupling = 200
+    op.lwMaxSweeps = 200
+    op.lwTolRel = 1e-9
+    op.backgroundAngularStrength = strength
+    T0 = np.full(sc.surface_count, met.air_temperature_k, np.float32)
+    T, Blw, it, conv, _, _ = oracle.eb_node(og, sc.facet_count, sc.leaf_count, sc.material, mats,
+                                            s["absorbedShortwave"], op, T0, T0,
+                                            normal_y=sc.facet_normals()[:, 1])
+    assert conv
+    assert np.max(np.abs(s["temperature"] - T)) < 0.01
+    assert _rel(s["longwaveRadiosity"], Blw) < 1e-4
+    # strength 0 again: the Lambertian ground's longwave radiosity differs
+    op.backgroundAngularStrength = 0.0
+    _, Blw0, _, _, _, _ = oracle.eb_node(og, sc.facet_count, sc.leaf_count, sc.material, mats,
+                                         s["absorbedShortwave"], op, T0, T0)
+    assert _rel(Blw0, Blw) > 1e-4
+
+
 @pytest.mark.parametrize("nb", [4, 24])
 def test_eb_node_photovoltaic_matches_oracle(oracle, nb):
     # photovoltaic object facets (core:3655-3663, 3772-3778): the leaves as

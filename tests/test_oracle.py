@@ -370,3 +370,37 @@ def test_mlp_gradient_matches_finite_differences(oracle):
         lm = np.mean((oracle.mlp_forward(Pm, x, H) - t) ** 2)
         fd = (lp - lm) / (float(Pp[i]) - float(Pm[i]))
         assert g[i] == pytest.approx(fd, rel=2e-3, abs=2e-5)
+
+
+def test_hapke_background_closed_forms(oracle):
+    # core:546-605.  Pins: strength 0 gives the Lambertian value (shortwave
+    # clamped to 0.999); at normal incidence and exit (mu0 = mu = 1, phase 0,
+    # full surge) the kernel reduces to w (1 + H(1)^2) / (8 pi), so the
+    # reflectance is base (1 + H^2) / H^2 with H = 3 / (1 + 2 sqrt(1 - w)),
+    # w = 4 b / (1 + b)^2; with the sun in the plane of the side mu0 hits its
+    # floor and the directional value vanishes; the longwave value is the
+    # Lambertian one at cosine 1 for every strength and follows the closed
+    # form of the angular factor elsewhere.
+    n = np.array([0.0, 1.0, 0.0], np.float32)
+    for b in (0.05, 0.2, 0.45):
+        assert oracle.hapke_reflectance(b, [0.3, 0.9, 0.1], n, n, 0.0) == np.float32(b)
+        w = 4 * b / (1 + b) ** 2
+        H = 3.0 / (1.0 + 2.0 * np.sqrt(1.0 - w))
+        want = min(1.0, b * (1 + H * H) / (H * H))
+        assert oracle.hapke_reflectance(b, n, n, n, 1.0) == pytest.approx(want, rel=2e-6)
+        # half strength: halfway between the Lambertian and the directional value
+        assert oracle.hapke_reflectance(b, n, n, n, 0.5) == pytest.approx(0.5 * (b + want), rel=2e-6)
+        # the sun below the side's plane is mirrored onto it (same value)
+        assert oracle.hapke_reflectance(b, -n, n, n, 1.0) == oracle.hapke_reflectance(b, n, n, n, 1.0)
+        assert oracle.hapke_reflectance(b, [1.0, 0.0, 0.0], n, n, 1.0) < 1e-3 * b
+    assert oracle.hapke_reflectance(1.0, n, n, n, 0.0) == np.float32(0.999)
+    for e in (0.9, 0.95, 0.98):
+        for s in (0.0, 0.5, 1.0):
+            assert oracle.hapke_emissivity(e, 1.0, s) == pytest.approx(e, rel=1e-7)
+        w = 1 - e * e
+        root = np.sqrt(1 - w)
+        f = lambda mu: root * (1 + 2 * mu / 1.5) / (1 + 2 * mu / 1.5 * root)  # noqa: E731
+        for mu in (0.2, 0.7):
+            want = min(1.0, e * f(mu) / f(1.0))
+            assert oracle.hapke_emissivity(e, mu, 1.0) == pytest.approx(want, rel=2e-6)
+            assert oracle.hapke_emissivity(e, mu, 0.3) == pytest.approx(e + 0.3 * (want - e), rel=2e-6)
