@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "facet_backend.hpp"
+#include "../csrc/hapke.hpp"
 #include "facet_hip.h"
 #include "project.hpp"
 #include "scene.hpp"
@@ -323,8 +324,17 @@ int runFacetRTProject(const std::string& input, const std::string& outFile) try 
                     refl = std::clamp(rb, 0.0f, 1.0f);
                     trans = std::clamp(tb, 0.0f, 1.0f - refl);
                 }
-                const float eps = std::max(0.0f, 1.0f - refl - trans);
-                if (!object && thermal) refl = std::clamp(1.0f - trans - eps, 0.0f, 1.0f);
+                float eps = std::max(0.0f, 1.0f - refl - trans);
+                if (!object) {  // the background's Hapke optics (core:1095-1105)
+                    const float sn[3] = {sg * vx.normal[0], sg * vx.normal[1], sg * vx.normal[2]};
+                    if (thermal) {
+                        eps = fh::hapke::emissivity(eps, std::fabs(sn[1]), scene.backgroundAngularStrength);
+                        refl = std::clamp(1.0f - trans - eps, 0.0f, 1.0f);
+                    } else {
+                        const float sv[3] = {sun.x, sun.y, sun.z};
+                        refl = fh::hapke::reflectance(refl, sv, sn, sn, scene.backgroundAngularStrength);
+                    }
+                }
                 r[i] = refl;
                 t[i] = std::min(trans, 1.0f - refl);
                 if (thermal) {
@@ -891,6 +901,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
     facethip::check(fh_eb_set_scene(ctx, scene.leafFacetCount, scene.material.data(), uint32_t(matList.size()),
                                     matList.data(), nw, swR.data(), swT.data()),
                     ctx);
+    facethip::check(fh_eb_set_background_angular_strength(ctx, scene.backgroundAngularStrength), ctx);
 
     // sensor bands solved after every node (core:3862-3908): the project's
     // sensor bands, else the default 550 nm (core:141, 2389-2400); per
@@ -1055,9 +1066,17 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
                         float refl = std::clamp(meanR, 0.0f, 1.0f);
                         const float trans = std::clamp(optical ? sm.t[std::min<size_t>(b, sm.t.size() - 1)] : sm.tirT,
                                                        0.0f, 1.0f - refl);
-                        const float eps = std::max(0.0f, 1.0f - refl - trans);
-                        if (!object && optical) refl = std::clamp(refl, 0.0f, 0.999f);  // Hapke at strength 0
-                        else if (!object) refl = std::clamp(1.0f - trans - eps, 0.0f, 1.0f);
+                        float eps = std::max(0.0f, 1.0f - refl - trans);
+                        if (!object) {  // the background's Hapke optics (core:1170-1180)
+                            const float sn[3] = {sg * vx.normal[0], sg * vx.normal[1], sg * vx.normal[2]};
+                            if (optical) {
+                                const float sv[3] = {sd.x, sd.y, sd.z};
+                                refl = fh::hapke::reflectance(refl, sv, sn, sn, scene.backgroundAngularStrength);
+                            } else {
+                                eps = fh::hapke::emissivity(eps, std::fabs(sn[1]), scene.backgroundAngularStrength);
+                                refl = std::clamp(1.0f - trans - eps, 0.0f, 1.0f);
+                            }
+                        }
                         br[i] = refl;
                         bt[i] = std::min(trans, 1.0f - refl);
                         if (optical) bD[i] = std::clamp(sunlit[side], 0.0f, 1.0f) * cosv * dS[b];

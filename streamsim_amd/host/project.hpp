@@ -10,8 +10,7 @@
 // spectra (custom spectra over the sensor bands), thermal, surface-energy
 // and soil entries; the FacetEB settings the project's control, light and
 // meteo blocks give (photovoltaic surface-energy entries included).  Not
-// read: DEM terrain, reflectance textures, Hapke background
-// heterogeneity (Lambert is used); PROSPECT, BSM and file spectra are
+// read: DEM terrain, reflectance textures; PROSPECT, BSM and file spectra are
 // replaced by their entry's own listed values taken as a custom spectrum
 // (stated on stderr).  Fire, Fog and medium objects are skipped as the
 // reference does.
@@ -478,9 +477,13 @@ inline Project loadProject(const std::string& file) {
         if (json::str(m, "energyModel") == "photovoltaic" &&
             json::str(m, "name") == json::str(background, "materialName", "soilset"))
             throw std::runtime_error("Photovoltaic panels must be scene objects, not background ground");
-    if (json::boolean(background, "heterogeneityEnabled", false) &&
-        json::num(background, "angularEffectStrength", 0.5f) > 0.0f)
-        std::cerr << "facet_engine: Hapke background heterogeneity is not modelled; the ground is Lambertian\n";
+    // the Hapke ground (core:1942-1943 writes the project's background as
+    // backgroundHeterogeneity Hapke / Lambert and backgroundAngularStrength;
+    // core:2904-2910 keeps the strength, clamped to [0, 1], for Hapke only)
+    prj.scene.backgroundAngularStrength =
+        json::boolean(background, "heterogeneityEnabled", false)
+            ? std::clamp(json::num(background, "angularEffectStrength", 0.5f), 0.0f, 1.0f)
+            : 0.0f;
     prj.scene.extentX = W;
     prj.scene.extentZ = D;
 

@@ -19,6 +19,7 @@ struct Scene {
     std::vector<uint32_t> material;
     uint32_t leafFacetCount = 0;
     float extentX = 1.0f, extentZ = 1.0f;  // ground extent (the periodic tile)
+    float backgroundAngularStrength = 0.0f;  // Hapke ground (core:2904-2910); 0: Lambert
     uint32_t facetCount() const { return uint32_t(vertices.size() / 3); }
 };
 
