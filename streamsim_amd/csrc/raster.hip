@@ -48,12 +48,15 @@ __device__ __forceinline__ float edgef(float ax, float ay, float bx, float by, f
     return (px - ax) * (by - ay) - (py - ay) * (bx - ax);
 }
 
-// one thread per (tile, facet) in submission order; fragments are
-// (depth bits, tile << 27 | camera side)
+// one wave per (tile, facet) in submission order, its lanes striding over
+// the pixels of the triangle's clamped bounding box (a building wall covers
+// 10^5 pixels, a leaf a few: per-thread triangles left one lane looping over
+// the wall); fragments are (depth bits, tile << 27 | camera side)
 __global__ __launch_bounds__(256) void k_raster(const float* __restrict__ fverts, const float4* __restrict__ fnorm,
                                                 uint32_t F, RasterView rv, uint32_t* __restrict__ counts,
                                                 uint2* __restrict__ frags, uint32_t* __restrict__ overflow) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t i = uint64_t(blockIdx.x) * (blockDim.x / 64u) + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
     if (i >= uint64_t(F) * rv.tiles) return;
     const uint32_t t = uint32_t(i / F), f = uint32_t(i - uint64_t(t) * F);
     const float tx = float(kRasterTiles[t][0]) * rv.px, tz = float(kRasterTiles[t][1]) * rv.pz;
@@ -81,26 +84,29 @@ __global__ __launch_bounds__(256) void k_raster(const float* __restrict__ fverts
     if (y0 < 0) y0 = 0;
     int y1 = int(ceilf(maxy));
     if (y1 > int(rv.H) - 1) y1 = int(rv.H) - 1;
+    if (x1 < x0 || y1 < y0) return;
+    const uint32_t bw = uint32_t(x1 - x0 + 1);
+    const uint32_t npx = bw * uint32_t(y1 - y0 + 1);  // <= W H <= 2^28
     const uint32_t key = (t << kTileShift) | side;
-    for (int py = y0; py <= y1; ++py) {
-        for (int pxl = x0; pxl <= x1; ++pxl) {
-            const float cx = float(pxl) + 0.5f, cy = float(py) + 0.5f;
-            const float w0 = edgef(x[1], y[1], x[2], y[2], cx, cy);
-            const float w1 = edgef(x[2], y[2], x[0], y[0], cx, cy);
-            const float w2 = edgef(x[0], y[0], x[1], y[1], cx, cy);
-            const bool inside = area > 0.0f ? (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)
-                                            : (w0 <= 0.0f && w1 <= 0.0f && w2 <= 0.0f);
-            if (!inside) continue;
-            const size_t pixel = size_t(py) * rv.W + size_t(pxl);
-            const uint32_t layer = atomicAdd(&counts[pixel], 1u);
-            if (layer >= rv.L) {
-                atomicAdd(overflow, 1u);
-                continue;
-            }
-            const float invArea = 1.0f / area;
-            const float depth = (w0 * d[0] + w1 * d[1] + w2 * d[2]) * invArea;
-            frags[pixel * rv.L + layer] = make_uint2(__float_as_uint(depth), key);
+    const float invArea = 1.0f / area;
+    for (uint32_t q = lane; q < npx; q += 64u) {
+        const uint32_t row = q / bw;
+        const int py = y0 + int(row), pxl = x0 + int(q - row * bw);
+        const float cx = float(pxl) + 0.5f, cy = float(py) + 0.5f;
+        const float w0 = edgef(x[1], y[1], x[2], y[2], cx, cy);
+        const float w1 = edgef(x[2], y[2], x[0], y[0], cx, cy);
+        const float w2 = edgef(x[0], y[0], x[1], y[1], cx, cy);
+        const bool inside = area > 0.0f ? (w0 >= 0.0f && w1 >= 0.0f && w2 >= 0.0f)
+                                        : (w0 <= 0.0f && w1 <= 0.0f && w2 <= 0.0f);
+        if (!inside) continue;
+        const size_t pixel = size_t(py) * rv.W + size_t(pxl);
+        const uint32_t layer = atomicAdd(&counts[pixel], 1u);
+        if (layer >= rv.L) {
+            atomicAdd(overflow, 1u);
+            continue;
         }
+        const float depth = (w0 * d[0] + w1 * d[1] + w2 * d[2]) * invArea;
+        frags[pixel * rv.L + layer] = make_uint2(__float_as_uint(depth), key);
     }
 }
 
@@ -290,8 +296,8 @@ uint32_t raster_pass(fh_context* c, const RasterView& rv, DBuf<uint32_t>& counts
     ovf.ensure(1);
     launch_fill_u32(c, counts.p, pixels, 0u);
     launch_fill_u32(c, ovf.p, 1, 0u);
-    const uint64_t n = uint64_t(c->F) * rv.tiles;
-    hipLaunchKernelGGL(k_raster, dim3(uint32_t((n + 255u) / 256u)), dim3(256), 0, c->stream, c->facetVerts.p,
+    const uint64_t n = uint64_t(c->F) * rv.tiles;  // one wave each, 4 per block
+    hipLaunchKernelGGL(k_raster, dim3(uint32_t((n + 3u) / 4u)), dim3(256), 0, c->stream, c->facetVerts.p,
                        c->facetNormal.p, c->F, rv, counts.p, frags.p, ovf.p);
     FH_HIP(hipGetLastError());
     uint32_t o = 0;
