@@ -54,9 +54,10 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--surrogate", default=str(ROOT / "streamsim_amd" / "data" / "surrogate.npz"))
     ap.add_argument("--roofline-reps", type=int, default=50)
-    ap.add_argument("--mode", default="run", choices=["run", "sweep", "precision"],
+    ap.add_argument("--mode", default="run", choices=["run", "sweep", "precision", "raster"],
                     help="sweep: only the roofline kernels (form factors, SpMM sweeps, copy calibration); "
-                         "precision: per-band error of the FP32 and BF16 shortwave solves against the FP64 oracle")
+                         "precision: per-band error of the FP32 and BF16 shortwave solves against the FP64 oracle; "
+                         "raster: the reference's raster graph and sunlit fraction on the urban validation scene")
     return ap.parse_args()
 
 
@@ -219,11 +220,46 @@ def load_surrogate(eng, path: str) -> bool:
     return True
 
 
+def raster_report(reps: int) -> None:
+    """The raster visibility graph (1024 x 1024 pixels, 64 fragments, the 40
+    hemisphere directions) and the raster sunlit fraction of the urban
+    validation scene (4746 facets), host to host, averaged over `reps`
+    calls after one warm-up call."""
+    from streamsim_amd import FacetEngine, scenes
+    sc = scenes.load_tree_with_soil(ROOT / "tests" / "golden" / "urban_tree_building.obj")
+    eng = FacetEngine()
+    eng.setGeometry(sc.vertices)
+    n = 40  # makeHemisphereDirections: golden-angle spiral over the upper hemisphere
+    i = np.arange(n, dtype=np.float32)
+    y = (i + np.float32(0.5)) / np.float32(n)
+    rad = np.sqrt(np.maximum(np.float32(0.0), np.float32(1.0) - y * y))
+    az = np.float32(2.39996322972865332) * i
+    dirs = np.stack([rad * np.cos(az), y, rad * np.sin(az)], 1).astype(np.float32)
+    sun = np.array([np.sin(np.radians(30.0)) * np.cos(np.radians(135.0)), np.cos(np.radians(30.0)),
+                    np.sin(np.radians(30.0)) * np.sin(np.radians(135.0))], np.float32)
+    d = eng.buildRasterGraph(dirs, 1024, 1024, 64)
+    eng.computeRasterSunlitFraction(sun, 1024, 1024, 64)
+    t = time.perf_counter()
+    for _ in range(reps):
+        eng.buildRasterGraph(dirs, 1024, 1024, 64)
+    tg = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        eng.computeRasterSunlitFraction(sun, 1024, 1024, 64)
+    ts = (time.perf_counter() - t) / reps
+    print(json.dumps({"workload": "urban validation scene, 4746 facets, raster 1024 x 1024 x 64, 40 directions",
+                      "directed_edges": d.directedEdgeCount, "graph_ms": tg * 1e3, "sunlit_ms": ts * 1e3,
+                      "reps": reps}), flush=True)
+
+
 def main() -> None:
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (no multi-GPU path is built)"}))
         sys.exit(2)
+    if a.mode == "raster":
+        raster_report(a.roofline_reps)
+        return
     wl = WORKLOADS[a.workload]
     steps = a.steps if a.steps is not None else wl["steps"]
     from streamsim_amd import FacetEngine, faceteb, scenes
