@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_raster(const float* __restrict__ fverts
             continue;
         }
         const float depth = (w0 * d[0] + w1 * d[1] + w2 * d[2]) * invArea;
-        frags[pixel * rv.L + layer] = make_uint2(__float_as_uint(depth), key);
+        frags[size_t(layer) * (size_t(rv.W) * rv.H) + pixel] = make_uint2(__float_as_uint(depth), key);
     }
 }
 
@@ -116,18 +116,20 @@ __device__ __forceinline__ bool frag_less(uint2 a, uint2 b) {
 }
 
 // sorts a pixel's fragments in place (insertion sort; lists are short)
+// (fragment i of a pixel at frags[i * pixels + pixel]: layer-major, so the
+// lanes of a wave, neighbouring pixels, read and write neighbouring entries)
 __device__ __forceinline__ uint32_t pixel_sorted(const uint32_t* __restrict__ counts, uint2* __restrict__ frags,
-                                                 size_t pixel, uint32_t L) {
+                                                 size_t pixel, size_t pixels, uint32_t L) {
     const uint32_t cnt = min(counts[pixel], L);
-    uint2* fr = frags + pixel * L;
+    uint2* fr = frags + pixel;
     for (uint32_t i = 1; i < cnt; ++i) {
-        const uint2 v = fr[i];
+        const uint2 v = fr[i * pixels];
         uint32_t j = i;
-        while (j > 0u && frag_less(v, fr[j - 1u])) {
-            fr[j] = fr[j - 1u];
+        while (j > 0u && frag_less(v, fr[(j - 1u) * pixels])) {
+            fr[j * pixels] = fr[(j - 1u) * pixels];
             --j;
         }
-        fr[j] = v;
+        fr[j * pixels] = v;
     }
     return cnt;
 }
@@ -160,18 +162,18 @@ __global__ __launch_bounds__(256) void k_graph_resolve(const uint32_t* __restric
                                                        unsigned long long cap) {
     const size_t pixel = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (pixel >= pixels) return;
-    const uint32_t cnt = pixel_sorted(counts, frags, pixel, L);
+    const uint32_t cnt = pixel_sorted(counts, frags, pixel, pixels, L);
     if (cnt == 0u) return;
-    const uint2* fr = frags + pixel * L;
+    const uint2* fr = frags + pixel;
     for (uint32_t i = 0; i < cnt; ++i) {
-        const uint32_t s = frag_side(fr[i]);
+        const uint32_t s = frag_side(fr[i * pixels]);
         atomicAdd(&den[s], 1u);
         atomicAdd(&den[s ^ 1u], 1u);
     }
     atomicAdd(&sky[frag_side(fr[0])], 1u);
-    atomicAdd(&sky[frag_side(fr[cnt - 1u]) ^ 1u], 1u);
+    atomicAdd(&sky[frag_side(fr[(cnt - 1u) * pixels]) ^ 1u], 1u);
     for (uint32_t i = 0; i + 1u < cnt; ++i) {
-        const uint32_t a = frag_side(fr[i]) ^ 1u, b = frag_side(fr[i + 1u]);
+        const uint32_t a = frag_side(fr[i * pixels]) ^ 1u, b = frag_side(fr[(i + 1u) * pixels]);
         if ((a >> 1) == (b >> 1)) continue;
         const unsigned long long slot = atomicAdd(cursor, 2ull);
         if (slot + 2ull <= cap) {  // cap is the exact bound: always true
@@ -191,11 +193,12 @@ __global__ __launch_bounds__(256) void k_sun_resolve(const uint32_t* __restrict_
     if (pixel >= pixels) return;
     const uint32_t cnt = min(counts[pixel], L);
     if (cnt == 0u) return;
-    const uint2* fr = frags + pixel * L;
+    const uint2* fr = frags + pixel;
     uint2 best = fr[0];
     for (uint32_t i = 0; i < cnt; ++i) {
-        atomicAdd(&total[frag_side(fr[i])], 1u);
-        if (frag_less(fr[i], best)) best = fr[i];
+        const uint2 v = fr[i * pixels];
+        atomicAdd(&total[frag_side(v)], 1u);
+        if (frag_less(v, best)) best = v;
     }
     atomicAdd(&visible[frag_side(best)], 1u);
 }
