@@ -5,14 +5,19 @@
 // reference's host loops (runGpu, the FacetEB lambdas) compile against it by
 // swapping the class name.  Header-only; link libfacet_hip.so.
 //
-// Differences a caller can observe:
-//   * buildVisibilityGraph() casts Config::raysPerSide seeded cosine-weighted
+// Config keeps the reference's fields, order and defaults (a default
+// Config rasterises 512 x 512 pixels with 32 fragments per pixel, as the
+// reference does); the fields after them select this engine's ray-cast form
+// factors.  Differences a caller can observe:
+//   * with Config::visibilityMethod == VisibilityMethod::RayCast,
+//     buildVisibilityGraph() casts Config::raysPerSide seeded cosine-weighted
 //     rays per side instead of rasterising the given directions (the
-//     argument is accepted and only checked for being non-empty);
-//   * computeSunlitFraction() samples Config::sunRaysPerSide points per side;
-//   * with Config::rasterWidth > 0 both use the reference's raster method
-//     instead (rasterWidth x rasterHeight pixels, maxFragmentsPerPixel
-//     fragments per pixel, the given directions), with its results.
+//     argument is accepted and only checked for being non-empty), and
+//     computeSunlitFraction() samples Config::sunRaysPerSide points per side;
+//   * edgeHashCapacity is accepted and unused (the graph is built from
+//     sorted pair lists, not a hash table, so it cannot overflow), and
+//     hashOverflow is always 0;
+//   * enableValidation is accepted and unused (no Vulkan layers).
 #pragma once
 
 #include <cmath>
@@ -43,20 +48,28 @@ struct SurfaceOptics {  // facetvk::SurfaceOptics (facetrt_vulkan.h:30-35)
     float directIrradiance{};
 };
 
-struct Config {  // facetvk::Config (facetrt_vulkan.h:37-47), ray fields + periodic fields
-    uint32_t raysPerSide{256};
-    uint32_t sunRaysPerSide{64};
-    uint32_t seed{1};
+// How buildVisibilityGraph() and computeSunlitFraction() see the scene.
+enum class VisibilityMethod : uint32_t {
+    Raster = 0,   // the reference's A-buffer over the given directions
+    RayCast = 1,  // seeded cosine-weighted form-factor rays and sun rays
+};
+
+struct Config {  // facetvk::Config (facetrt_vulkan.h:37-47), then the ray-cast fields
+    uint32_t rasterWidth{512};
+    uint32_t rasterHeight{512};
+    uint32_t maxFragmentsPerPixel{32};
+    uint32_t edgeHashCapacity{0};  // accepted, unused: no hash table
+    bool enableValidation{false};  // accepted, unused
     uint32_t gpuIndex{0};
-    float rayEpsilon{1.0e-5f};
     uint32_t periodicNeighborCount{0};
     float periodicSizeX{1.0f};
     float periodicSizeZ{1.0f};
-    // the reference's raster fields (facetrt_vulkan.h:38-40); rasterWidth 0
-    // selects the ray-cast graph and sunlit fraction
-    uint32_t rasterWidth{0};
-    uint32_t rasterHeight{0};
-    uint32_t maxFragmentsPerPixel{32};
+    // ray-cast form factors (this engine's extension)
+    VisibilityMethod visibilityMethod{VisibilityMethod::Raster};
+    uint32_t raysPerSide{256};
+    uint32_t sunRaysPerSide{64};
+    uint32_t seed{1};
+    float rayEpsilon{1.0e-5f};
 };
 
 struct GraphDiagnostics {  // facetvk::GraphDiagnostics (facetrt_vulkan.h:49-56)
@@ -123,7 +136,7 @@ public:
         require();
         if (directions.empty()) throw std::invalid_argument("At least one visibility direction is required");
         fh_graph_diagnostics d{};
-        if (m_config.rasterWidth) {
+        if (m_config.visibilityMethod == VisibilityMethod::Raster) {
             std::vector<fh_direction> dirs;
             dirs.reserve(directions.size());
             for (const Direction& v : directions) dirs.push_back({v.x, v.y, v.z});
@@ -192,7 +205,7 @@ public:
                                     sunDirection.z * sunDirection.z);
         if (!(len > 1.0e-8f)) throw std::runtime_error("Zero-length direction or degenerate triangle");
         std::vector<float> out(surfaceCount());
-        if (m_config.rasterWidth) {
+        if (m_config.visibilityMethod == VisibilityMethod::Raster) {
             // makeProjection takes the direction as given (vk:1071)
             const fh_direction raw{sunDirection.x, sunDirection.y, sunDirection.z};
             check(fh_compute_raster_sunlit_fraction(m_ctx, raw, m_config.rasterWidth, m_config.rasterHeight,

@@ -6,7 +6,7 @@
  * "vkh" below) and the host loops of models/histream/src/facetrt/
  * facetrtcore.cpp ("core").  Plain C types, pointers and sizes only.
  * Every entry point returns an fh_status; fh_last_error() holds the message.
- * The C++ shim (streamsim_amd/csrc/facet_backend.hpp) rethrows the codes as
+ * The C++ shim (include/facet_backend.hpp) rethrows the codes as
  * the exception types the reference throws (invalid_argument, logic_error,
  * runtime_error, overflow_error).
  *

@@ -257,6 +257,7 @@ int fh_set_geometry(fh_context* c, const fh_vertex* v, uint32_t vertexCount) {
         }
         c->hasGeometry = false;
         c->hasGraph = false;
+        fh::eb_geometry_changed(c);
         const uint32_t F = vertexCount / 3u;
         std::vector<float> tri9(size_t(F) * 9u);
         std::vector<float4> nrm(F);
@@ -366,6 +367,7 @@ int fh_build_form_factors(fh_context* c, fh_graph_diagnostics* out) {
     return guarded(c, [&] {
         require_geometry(c, "buildVisibilityGraph()");
         c->hasGraph = false;
+        fh::eb_graph_changed(c);
         const uint32_t S = c->S, R = c->cfg.raysPerSide;
         c->hits.ensure(size_t(S) * R);
         c->cnt16.ensure(size_t(S) * R);
@@ -422,6 +424,7 @@ int fh_set_graph(fh_context* c, uint32_t nnz, const uint32_t* rowOffsets, const 
         for (uint32_t s = 0; s < S; ++s) any = any || denominators[s] > 0u;
         if (!any) fail(FH_RUNTIME_ERROR, "Visibility raster has no surface samples; radiation results are invalid");
         c->hasGraph = false;
+        fh::eb_graph_changed(c);
         c->nnz = nnz;
         c->rowptr.ensure(size_t(S) + 1u);
         alloc_csr(c, nnz);
@@ -569,6 +572,7 @@ int fh_build_raster_graph(fh_context* c, const fh_direction* directions, uint32_
         require_geometry(c, "buildVisibilityGraph()");
         if (!directions || directionCount == 0u) fail(FH_INVALID_ARGUMENT, "Visibility directions must not be empty");
         c->hasGraph = false;
+        fh::eb_graph_changed(c);
         std::vector<float> dirs(size_t(directionCount) * 3u);
         for (uint32_t d = 0; d < directionCount; ++d) {
             dirs[3u * d] = directions[d].x;

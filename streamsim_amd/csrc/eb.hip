@@ -123,8 +123,9 @@ __global__ void k_lw_source(uint32_t S, const float* __restrict__ T, const float
 }
 
 // SW window optics per side and band (makeEnergyShortwaveOptics,
-// core:952-1003, no reflectance texture); with strength > 0 the ground
-// sides' reflectance is the Hapke value for this sun (core:988-996).
+// core:952-1003, no reflectance texture); the ground sides' reflectance is
+// the Hapke value for this sun (core:988-996): the Lambertian value clamped
+// to 0.999 at strength 0.
 // Internal band b is window order[b]; dni is given in internal order.
 __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_t* __restrict__ mat,
                             const float* __restrict__ refl, const float* __restrict__ trans,
@@ -151,12 +152,18 @@ __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_
     const uint32_t m = mat[s >> 1], w = order[b];
     float rr = fminf(fmaxf(refl[size_t(m) * nb + w], 0.0f), 1.0f);
     const float tt = fminf(fmaxf(trans[size_t(m) * nb + w], 0.0f), 1.0f - rr);
-    if (strength > 0.0f && (s >> 1) >= leafCount) {
-        const float4 nf = nrm[s >> 1];
-        const float sg = (s & 1u) ? -1.0f : 1.0f;
-        const float sn[3] = {nf.x * sg, nf.y * sg, nf.z * sg};
-        const float sun[3] = {sx, sy, sz};
-        rr = hapke::reflectance(rr, sun, sn, sn, strength);
+    if ((s >> 1) >= leafCount) {
+        // ground sides: backgroundHapkeReflectance for every side (core:989-992);
+        // at strength 0 it is the Lambertian value clamped to 0.999
+        if (strength > 0.0f) {
+            const float4 nf = nrm[s >> 1];
+            const float sg = (s & 1u) ? -1.0f : 1.0f;
+            const float sn[3] = {nf.x * sg, nf.y * sg, nf.z * sg};
+            const float sun[3] = {sx, sy, sz};
+            rr = hapke::reflectance(rr, sun, sn, sn, strength);
+        } else {
+            rr = fminf(rr, 0.999f);
+        }
     }
     r[i] = rr;
     t[i] = fminf(tt, 1.0f - rr);
@@ -662,6 +669,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     EbState& e = c->eb;
     const uint32_t F = c->F, S = c->S;
     e.ready = false;
+    e.facets = F;
     e.leafCount = leafCount;
     e.nMat = nMat;
     e.swNb = swNb;
@@ -687,6 +695,15 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
             const float t = std::min(std::max(trans[size_t(m) * swNb + b], 0.0f), 1.0f - r);
             key[b] = std::max(key[b], r + t);
         }
+    // a ground side's shortwave reflectance is clamped to 0.999 (the Hapke
+    // model's base, core:989-992); the material tables hold the values of
+    // leaves, so a ground material above the clamp takes the per-side arrays
+    std::vector<uint8_t> groundMat(nMat, 0u);
+    for (uint32_t f = leafCount; f < F; ++f) groundMat[facetMaterial[f]] = 1u;
+    e.groundClamp = false;
+    for (uint32_t m = 0; m < nMat; ++m)
+        for (uint32_t b = 0; b < swNb && groundMat[m]; ++b)
+            if (refl[size_t(m) * swNb + b] > 0.999f) e.groundClamp = true;
     e.swOrder.resize(swNb);
     for (uint32_t b = 0; b < swNb; ++b) e.swOrder[b] = b;
     std::stable_sort(e.swOrder.begin(), e.swOrder.end(), [&](uint32_t x, uint32_t y) { return key[x] > key[y]; });
@@ -915,7 +932,7 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         // the spectral radius estimated again for every node
         const bool hapkeOn = c->hapkeStrength > 0.0f;
         if (hapkeOn) e.swRhoValid = false;
-        const bool tab = !noTables && !hapkeOn && uses_bf16_gather(ldb);
+        const bool tab = !noTables && !hapkeOn && !e.groundClamp && uses_bf16_gather(ldb);
         const uint32_t precision = bf16_pays(ldb) ? p.precision : FH_PRECISION_FP32;
         // an increment solve keeps the gathers of every band's last sweep:
         // the absorbed pass forms the incident from them
@@ -1145,7 +1162,13 @@ int fh_eb_run_node(fh_context* c, const fh_eb_node_params* p, const float* swDir
                    fh_eb_node_stats* stats) {
     return guarded_eb(c, [&] {
         if (!c->eb.ready) fail(FH_LOGIC_ERROR, "fh_eb_set_scene() must be called before fh_eb_run_node()");
+        if (!c->hasGraph || c->eb.facets != c->F)
+            fail(FH_LOGIC_ERROR, "buildVisibilityGraph() and fh_eb_set_scene() must follow setGeometry()");
         if (!p) fail(FH_INVALID_ARGUMENT, "params must not be null");
+        if (!std::isfinite(p->sun.x) || !std::isfinite(p->sun.y) || !std::isfinite(p->sun.z))
+            fail(FH_INVALID_ARGUMENT, "sun direction must be finite");
+        if (c->hapkeStrength > 0.0f && !(p->sun.x * p->sun.x + p->sun.y * p->sun.y + p->sun.z * p->sun.z > 0.0f))
+            fail(FH_INVALID_ARGUMENT, "the Hapke ground needs a non-zero sun direction");
         if (c->eb.swNb && (!swDirectNormal || !swDiffuse)) fail(FH_INVALID_ARGUMENT, "shortwave inputs must be given");
         for (uint32_t b = 0; b < c->eb.swNb; ++b)
             if (!std::isfinite(swDirectNormal[b]) || !std::isfinite(swDiffuse[b]) || swDirectNormal[b] < 0.0f ||

@@ -114,9 +114,11 @@ constexpr uint32_t kCsrPad = 16;
 // FacetEB device state (one per context).
 struct EbState {
     bool ready = false;
+    uint32_t facets = 0;  // facet count of the scene fh_eb_set_scene was given
     uint32_t leafCount = 0;
     uint32_t nMat = 0;
     uint32_t swNb = 0, swLdb = 0;
+    bool groundClamp = false;  // a ground material's SW reflectance exceeds 0.999 (per-side arrays)
     bool swValid = false;  // Bsw holds the previous node's SW solution
     bool lwValid = false;  // Blw holds a LW solution
     bool tValid = false;   // T holds a state
@@ -244,3 +246,29 @@ struct fh_context {
     fh::EbState eb;
     fh::MlpState mlp;
 };
+
+namespace fh {
+
+// A new graph invalidates what the energy balance derived from the old one:
+// the spectral radius of the shortwave operator, the warm-start radiosities
+// and the ground-view fractions of the surrogate features.
+inline void eb_graph_changed(fh_context* c) {
+    EbState& e = c->eb;
+    e.swRhoValid = false;
+    e.swValid = false;
+    e.lwValid = false;
+    e.featValid = false;
+    e.guessPrevValid = false;
+    e.gview.release();
+}
+
+// New geometry invalidates the whole energy-balance scene (its per-facet
+// and per-side buffers are sized for the old facet count):
+// fh_eb_set_scene must be called again.
+inline void eb_geometry_changed(fh_context* c) {
+    eb_graph_changed(c);
+    c->eb.ready = false;
+    c->eb.tValid = false;
+}
+
+}  // namespace fh

@@ -174,7 +174,9 @@ std::string outputPath(const std::string& input, const std::string& requested, c
 // core:168-187) over the 40 hemisphere directions
 facethip::Config engineConfig(const Scene& s, uint32_t periodic = 0) {
     facethip::Config c;
+    c.visibilityMethod = facethip::VisibilityMethod::RayCast;
     if (const char* e = std::getenv("FACET_VISIBILITY"); e && std::string(e) == "raster") {
+        c.visibilityMethod = facethip::VisibilityMethod::Raster;
         c.rasterWidth = c.rasterHeight = 1024u;
         c.maxFragmentsPerPixel = 64u;
         if (const char* r = std::getenv("FACET_RASTER_SIZE")) c.rasterWidth = c.rasterHeight = uint32_t(std::atoi(r));

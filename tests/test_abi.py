@@ -91,3 +91,12 @@ def test_periodic_config_validation():
     with pytest.raises(ValueError, match="periodic tile sizes"):
         FacetEngine(periodic_neighbors=4, periodic_size=(0.0, 1.0))
 
+
+
+def test_backend_compiles_at_the_reference_call_sites():
+    # include/facet_backend.hpp after the type swap: the reference's Config
+    # fields, defaults and method calls compile unchanged
+    import subprocess
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", f"-I{ROOT / 'include'}",
+                        str(ROOT / "tests" / "backend_callsites.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

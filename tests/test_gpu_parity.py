@@ -280,6 +280,48 @@ def test_error_paths():
     assert res.iterations == 8
 
 
+def test_eb_state_follows_the_scene():
+    # a new geometry drops the energy-balance scene (its buffers are sized for
+    # the old facet count): the node fails cleanly before any kernel runs;
+    # non-finite sun directions, and a zero one under the Hapke ground, are
+    # rejected
+    print("eb scene guard", flush=True)
+    from streamsim_amd._lib import LogicError
+    sc = scenes.row_crop(leaves_per_row=20, cells=4)
+    eng = _engine(R=64, K=16, seed=2)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    win = faceteb.ShortwaveWindows.for_bands(4)
+    faceteb.setup_scene(eng, sc, win)
+    cfg = faceteb.EbConfig()
+    met = faceteb.diurnal_meteo(24)[12]
+    dni, dif = win.irradiance(met.shortwave, math.cos(math.radians(met.zenith)))
+    st = eng.ebRunNode(faceteb.node_params(met, cfg, first=True), dni, dif)
+    assert st.converged
+    bigger = scenes.row_crop(leaves_per_row=30, cells=5)
+    eng.setGeometry(bigger.vertices)
+    with pytest.raises(LogicError):
+        eng.ebRunNode(faceteb.node_params(met, cfg, first=False), dni, dif)
+    eng.buildVisibilityGraph()
+    with pytest.raises(LogicError):
+        eng.ebRunNode(faceteb.node_params(met, cfg, first=False), dni, dif)
+    with pytest.raises(LogicError):
+        eng.ebGetState()
+    faceteb.setup_scene(eng, bigger, win)
+    p = faceteb.node_params(met, cfg, first=True)
+    p.sun.x = float("nan")
+    with pytest.raises(ValueError):
+        eng.ebRunNode(p, dni, dif)
+    eng.ebSetBackgroundAngularStrength(0.5)
+    p = faceteb.node_params(met, cfg, first=True)
+    p.sun.x = p.sun.y = p.sun.z = 0.0
+    with pytest.raises(ValueError):
+        eng.ebRunNode(p, dni, dif)
+    st = eng.ebRunNode(faceteb.node_params(met, cfg, first=True), dni, dif)
+    assert st.converged
+    assert np.all(np.isfinite(eng.ebGetState()["temperature"]))
+
+
 def _eb_case(oracle, leaves=40, cells=6, R=128, K=32, nb=4):
     sc = scenes.row_crop(leaves_per_row=leaves, cells=cells)
     eng = _engine(R=R, K=K, seed=4)
