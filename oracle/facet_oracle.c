@@ -1303,37 +1303,37 @@ static double clampd(double x, double lo, double hi)
     return x < lo ? lo : (x > hi ? hi : x);
 }
 
-static double sat_vp(double T)
+double or_sat_vp(double T)
 {
     const double c = clampd(T - 273.15, -80.0, 80.0);
     return 6.107 * pow(10.0, 7.5 * c / (237.3 + c));
 }
 
-static double sat_vp_slope(double T)
+double or_sat_vp_slope(double T)
 {
     const double c = clampd(T - 273.15, -80.0, 80.0);
-    return sat_vp(T) * 2.3026 * 7.5 * 237.3 / ((237.3 + c) * (237.3 + c));
+    return or_sat_vp(T) * 2.3026 * 7.5 * 237.3 / ((237.3 + c) * (237.3 + c));
 }
 
-static double latent(double T, double rho, double P, double ea, double ra, double rs)
+double or_latent(double T, double rho, double P, double ea, double ra, double rs)
 {
     const double c = T - 273.15;
     const double lam = (2.501 - 0.002361 * c) * 1.0e6;
-    const double deficit = 0.622 * fmax(0.0, sat_vp(T) - ea) / fmax(100.0, P);
+    const double deficit = 0.622 * fmax(0.0, or_sat_vp(T) - ea) / fmax(100.0, P);
     return rho * lam * deficit / fmax(1.0, ra + rs);
 }
 
-static double latent_slope(double T, double rho, double P, double ea, double ra, double rs)
+double or_latent_slope(double T, double rho, double P, double ea, double ra, double rs)
 {
     const double c = T - 273.15;
     const double lam = (2.501 - 0.002361 * c) * 1.0e6;
-    const double deficit = 0.622 * fmax(0.0, sat_vp(T) - ea) / fmax(100.0, P);
+    const double deficit = 0.622 * fmax(0.0, or_sat_vp(T) - ea) / fmax(100.0, P);
     const double d = rho / fmax(1.0, ra + rs) *
-                     (-2361.0 * deficit + lam * 0.622 * sat_vp_slope(T) / fmax(100.0, P));
+                     (-2361.0 * deficit + lam * 0.622 * or_sat_vp_slope(T) / fmax(100.0, P));
     return fmax(0.0, d);
 }
 
-static double soil_storage(const or_eb_material *m, double dT)
+double or_soil_storage(const or_eb_material *m, double dT)
 {
     const double inertia = sqrt(fmax(0.0, (double)m->specificHeat * m->density * m->thermalConductivity));
     return 2.0 * inertia / sqrt(3.14159265358979323846 * fmax(1.0, dT));
@@ -1421,8 +1421,8 @@ uint32_t or_eb_node(const or_graph *g, uint32_t F, uint32_t leafCount,
                         rs = clampd(m->surfaceResistance / fmax(0.08, open * resp), 40.0, 100000.0); \
                     }                                                               \
                     const double vf = m->vegetation ? 2.0 : 0.0;                    \
-                    const double LE = vf * latent(Tl, rho, P, p->vaporPressure, ra, rs); \
-                    const double LEd = vf * latent_slope(Tl, rho, P, p->vaporPressure, ra, rs); \
+                    const double LE = vf * or_latent(Tl, rho, P, p->vaporPressure, ra, rs); \
+                    const double LEd = vf * or_latent_slope(Tl, rho, P, p->vaporPressure, ra, rs); \
                     const double Gd = fmax(0.0, (double)m->heatCapacityPerArea) / p->dTime; \
                     const double G = Gd * (Tl - (double)Tprev[s0]);                 \
                     const double eps = clampd(1.0 - (double)r[s0] - (double)tt[s0], 0.0, 1.0); \
@@ -1446,14 +1446,14 @@ uint32_t or_eb_node(const or_graph *g, uint32_t F, uint32_t leafCount,
                         const double Ts = T[s];                                     \
                         const double Hs = ha * (Ts - Ta);                           \
                         const int dry = m->soilMoisture < 0.03f;                    \
-                        const double LE = dry ? 0.0 : latent(Ts, rho, P, p->vaporPressure, ra, m->surfaceResistance); \
-                        const double LEd = dry ? 0.0 : latent_slope(Ts, rho, P, p->vaporPressure, ra, m->surfaceResistance); \
+                        const double LE = dry ? 0.0 : or_latent(Ts, rho, P, p->vaporPressure, ra, m->surfaceResistance); \
+                        const double LEd = dry ? 0.0 : or_latent_slope(Ts, rho, P, p->vaporPressure, ra, m->surfaceResistance); \
                         double G, Gd;                                               \
                         if (m->soilTemperatureMethod == 0) {                        \
                             Gd = 0.0;                                               \
                             G = 0.35 * Rn[s];                                       \
                         } else {                                                    \
-                            Gd = soil_storage(m, p->dTime) * (m->soilTemperatureMethod == 2 ? 1.5 : 1.0); \
+                            Gd = or_soil_storage(m, p->dTime) * (m->soilTemperatureMethod == 2 ? 1.5 : 1.0); \
                             G = Gd * (Ts - (double)Tprev[s]);                       \
                         }                                                           \
                         const double eps = clampd(1.0 - (double)r[s] - (double)tt[s], 0.0, 1.0); \

@@ -200,6 +200,17 @@ float or_hapke_emissivity(float emissivity, float cosine, float strength);
  * incident per side, may be NULL) feeds the power of photovoltaic object
  * facets (core:3655-3663).  Outputs per side may be NULL.  Returns the
  * number of coupling iterations; *converged set. */
+/* FacetEB flux helpers in FP64 (core:854-915): saturation vapour pressure
+ * (hPa) and its slope at T (K); latent heat flux and its T-derivative for air
+ * density rho, pressure P (hPa), vapour pressure ea (hPa), aerodynamic and
+ * surface resistances ra, rs (s m-1); soil storage coefficient
+ * 2 sqrt(c rho lambda) / sqrt(pi dT) (W m-2 K-1). */
+double or_sat_vp(double T);
+double or_sat_vp_slope(double T);
+double or_latent(double T, double rho, double P, double ea, double ra, double rs);
+double or_latent_slope(double T, double rho, double P, double ea, double ra, double rs);
+double or_soil_storage(const or_eb_material *m, double dT);
+
 uint32_t or_eb_node(const or_graph *g, uint32_t F, uint32_t leafCount,
                     const uint32_t *material, const or_eb_material *mats,
                     const float *normalY, const float *absorbedSW,

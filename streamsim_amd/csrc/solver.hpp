@@ -121,9 +121,12 @@ int sweep_team(const fh_context* c);
 // with gacc and carry ([S][ldb]) also given, the increment sweep of BF16
 // solves (h16in / h16out hold BF16 increments).  pairs: the sweep of the
 // solves (pair blocks where the graph has them); else one Jacobi sweep.
+// partials: device buffer of partialCap per-block residual partials the
+// sweep writes instead of the context's own (grown on demand) buffer; a
+// sweep that needs more fails with FH_INVALID_ARGUMENT before any launch.
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
                          const float* Bin, float* Bout, bool w16, uint16_t* h16in = nullptr,
                          uint16_t* h16out = nullptr, float* gacc = nullptr, float* carry = nullptr,
-                         bool pairs = false);
+                         bool pairs = false, float2* partials = nullptr, uint32_t partialCap = 0u);
 
 }  // namespace fh

@@ -114,6 +114,11 @@ struct SweepArgs {
     const float4* unitW;
     uint32_t nUnits;
     uint32_t jacobiOnly;  // 1: plain Jacobi sweep (single-sweep entry points)
+    uint32_t sides;       // sides of the scene (rows of the gathered arrays)
+    // per-block partials given by the caller (capacity partialCap); null:
+    // the context's partial buffer, grown as needed
+    float2* partialExt;
+    uint32_t partialCap;
     uint32_t partOff;     // band-parallel: first partial of this launch (a sweep may be two launches)
 };
 
@@ -778,6 +783,100 @@ __device__ __forceinline__ void gather_v(const SweepArgs& a, const float* __rest
     }
 }
 
+// Buffer resource over `bytes` bytes at `base` (raw buffer, no swizzle):
+// loads at byte offsets past `bytes` return zero without touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, int(bytes), 0x00020000);
+}
+constexpr uint32_t kNoRow = 0xfffffff0u;  // byte offset past any gathered array: loads zero
+
+// Gathers of both rows of a facet with the CSR loads software-pipelined.
+// The aligned groups of 8 entries of row 0 ([rb[0], rb[1])) and then of
+// row 1 ([rb[1], rb[2])) form one sequence; the columns and weights of the
+// next group are loaded while this group's gathers are in flight, so a
+// group costs one round of memory latency instead of two, and the first
+// group of row 1 overlaps the last one of row 0.  One accumulator: it is
+// handed to side 0 where row 1 begins.  Each gather is a 16-byte buffer
+// load at byte offset col * rowBytes + laneBytes of `rows`; entries outside
+// the row load at kNoRow, which returns zero without a memory access.  The
+// 16 bytes are 4 FP32 bands (BF16 false) or 8 BF16 bands.
+template <bool BF16>
+struct RowSums {
+    static constexpr int NB = BF16 ? 8 : 4;
+    float s0[NB], s1[NB];
+};
+
+template <bool BF16>
+__device__ __forceinline__ RowSums<BF16> gather_rows_pipe(const SweepArgs& a, const uint32_t (&rb)[3],
+                                                          __amdgpu_buffer_rsrc_t rows, uint32_t rowBytes,
+                                                          uint32_t laneBytes) {
+    constexpr int NB = RowSums<BF16>::NB;
+    const uint32_t lo0 = rb[0], lo1 = rb[1], hi1 = rb[2];
+    const uint32_t base0 = lo0 & ~uint32_t(kU - 1), base1 = lo1 & ~uint32_t(kU - 1);
+    const uint32_t n0 = lo1 > lo0 ? (lo1 - base0 + uint32_t(kU - 1)) / uint32_t(kU) : 0u;
+    const uint32_t n1 = hi1 > lo1 ? (hi1 - base1 + uint32_t(kU - 1)) / uint32_t(kU) : 0u;
+    const uint32_t n = n0 + n1;
+    float acc[NB], r0[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] = r0[j] = 0.0f;
+    uint32_t base = n0 ? base0 : base1;
+    uint4 cA = make_uint4(0u, 0u, 0u, 0u), cB = cA;
+    float4 wA = make_float4(0.0f, 0.0f, 0.0f, 0.0f), wB = wA;
+    if (n) {
+        cA = *reinterpret_cast<const uint4*>(a.col + base);
+        cB = *reinterpret_cast<const uint4*>(a.col + base + 4);
+        wA = *reinterpret_cast<const float4*>(a.w32 + base);
+        wB = *reinterpret_cast<const float4*>(a.w32 + base + 4);
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        // row 1 begins: the sums so far are row 0's
+        const bool hand = k == n0;
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            r0[j] = hand ? acc[j] : r0[j];
+            acc[j] = hand ? 0.0f : acc[j];
+        }
+        const uint32_t lo = k < n0 ? lo0 : lo1, hi = k < n0 ? lo1 : hi1;
+        const uint32_t c[kU] = {cA.x, cA.y, cA.z, cA.w, cB.x, cB.y, cB.z, cB.w};
+        const float w[kU] = {wA.x, wA.y, wA.z, wA.w, wB.x, wB.y, wB.z, wB.w};
+        uint4 v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t e = base + uint32_t(u);
+            const uint32_t off = e >= lo && e < hi ? __umul24(c[u], rowBytes) + laneBytes : kNoRow;
+            v[u] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rows, off, 0, 0));
+        }
+        if (k + 1u < n) {  // the next group lies within the CSR (and its padding)
+            base = k + 1u < n0 ? base0 + (k + 1u) * uint32_t(kU) : base1 + (k + 1u - n0) * uint32_t(kU);
+            cA = *reinterpret_cast<const uint4*>(a.col + base);
+            cB = *reinterpret_cast<const uint4*>(a.col + base + 4);
+            wA = *reinterpret_cast<const float4*>(a.w32 + base);
+            wB = *reinterpret_cast<const float4*>(a.w32 + base + 4);
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            const uint32_t vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+            if constexpr (BF16) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[2 * j] += w[u] * __uint_as_float(vv[j] << 16);
+                    acc[2 * j + 1] += w[u] * __uint_as_float(vv[j] & 0xffff0000u);
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j] += w[u] * __uint_as_float(vv[j]);
+            }
+        }
+    }
+    RowSums<BF16> out;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        out.s0[j] = n1 ? r0[j] : acc[j];
+        out.s1[j] = n1 ? acc[j] : 0.0f;
+    }
+    return out;
+}
+
 // FP32 gathers, 4 bands per lane: MODE_SWEEP with FP32 (W16 false) or BF16
 // weights, or MODE_INCIDENT.  PAIR (MODE_SWEEP, FP32 weights): team i
 // updates the two facets of units[i] with the pair block solve.
@@ -804,7 +903,19 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
             for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
                 const uint32_t b0 = cb + 4u * lane;
                 float g[2][4] = {};
-                if (b0 < x.nAct) gather_v<W16>(a, x.Bin + b0, ldb, rb, g);
+                if (b0 < x.nAct) {
+                    if constexpr (!W16) {
+                        const RowSums<false> rs =
+                            gather_rows_pipe<false>(a, rb, make_rsrc(x.Bin, a.sides * ldb * 4u), ldb * 4u, b0 * 4u);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            g[0][j] = rs.s0[j];
+                            g[1][j] = rs.s1[j];
+                        }
+                    } else {
+                        gather_v<W16>(a, x.Bin + b0, ldb, rb, g);
+                    }
+                }
                 band_update4<MODE>(a, x, s0, b0, g[0], g[1], myAbs, myRel, myAct);
             }
         } else {
@@ -815,8 +926,21 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
                 const uint32_t b0 = cb + 4u * lane;
                 float gf[2][4] = {}, gg[2][4] = {};
                 if (b0 < x.nAct) {
-                    gather_v<W16>(a, x.Bin + b0, ldb, rb, gf);
-                    gather_v<W16>(a, x.Bin + b0, ldb, rc, gg);
+                    if constexpr (!W16) {
+                        const __amdgpu_buffer_rsrc_t rows = make_rsrc(x.Bin, a.sides * ldb * 4u);
+                        const RowSums<false> rf = gather_rows_pipe<false>(a, rb, rows, ldb * 4u, b0 * 4u);
+                        const RowSums<false> rg = gather_rows_pipe<false>(a, rc, rows, ldb * 4u, b0 * 4u);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            gf[0][j] = rf.s0[j];
+                            gf[1][j] = rf.s1[j];
+                            gg[0][j] = rg.s0[j];
+                            gg[1][j] = rg.s1[j];
+                        }
+                    } else {
+                        gather_v<W16>(a, x.Bin + b0, ldb, rb, gf);
+                        gather_v<W16>(a, x.Bin + b0, ldb, rc, gg);
+                    }
                 }
                 pair_update4<0>(a, x, s0, s1, b0, Wf, Wg, gf[0], gf[1], gg[0], gg[1], myAbs, myRel, myAct);
             }
@@ -924,7 +1048,19 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
             for (uint32_t cb = 0; cb < x.nPrev; cb += 8u * bt) {
                 const uint32_t b0 = cb + 8u * lane;
                 float g[2][8] = {};
-                if (b0 < x.nAct) gather_h<INC == 0>(a, x.Bin16 + b0, ldh, rb, g);
+                if (b0 < x.nAct) {
+                    if constexpr (INC != 0) {
+                        const RowSums<true> rs =
+                            gather_rows_pipe<true>(a, rb, make_rsrc(x.Bin16, a.sides * ldh * 2u), ldh * 2u, b0 * 2u);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            g[0][j] = rs.s0[j];
+                            g[1][j] = rs.s1[j];
+                        }
+                    } else {
+                        gather_h<INC == 0>(a, x.Bin16 + b0, ldh, rb, g);
+                    }
+                }
                 // two groups of 4 bands (the second may lie beyond ldb)
                 float lo0[4], lo1[4], hi0[4], hi1[4];
 #pragma unroll
@@ -1383,7 +1519,13 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.pk = c->pkBits && use_packed() ? c->pk.p : nullptr;
     a.den = c->den.p;
     a.pkBits = c->pkBits;
+    a.sides = c->S;
     if (band_parallel(a.ldb)) {
+        // the band-parallel gathers address the gathered rows by 32-bit byte
+        // offsets below kNoRow
+        if (uint64_t(c->S) * a.ldb * 4u >= uint64_t(kNoRow))
+            fail(FH_RUNTIME_ERROR, "radiosity array of " + std::to_string(uint64_t(c->S) * a.ldb * 4u) +
+                                       " bytes exceeds the 4 GiB the band-parallel sweeps address");
         // lanes per team: one 16-byte vector of the row each, at most 64
         // bands per team pass
         const bool half = (mode == MODE_SWEEP && w16 && a.h0) || a.inc;
@@ -1416,8 +1558,15 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
             p.partOff = g;
         }
         const uint32_t parts = std::max(1u, g + gp);
-        c->partial.ensure(parts);
-        a.partial = p.partial = c->partial.p;
+        if (a.partialExt) {
+            if (parts > a.partialCap)
+                fail(FH_INVALID_ARGUMENT, "sweep needs " + std::to_string(parts) + " block partials; the fixed scratch holds " +
+                                              std::to_string(a.partialCap));
+            a.partial = p.partial = a.partialExt;
+        } else {
+            c->partial.ensure(parts);
+            a.partial = p.partial = c->partial.p;
+        }
         if (a.retire) {
             c->partAct.ensure(parts);
             a.partAct = p.partAct = c->partAct.p;
@@ -1469,8 +1618,15 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.ctrl = c->ctrl.p;
     const uint64_t threads = uint64_t(c->F) * uint64_t(team);
     const uint32_t grid = uint32_t((threads + 255u) / 256u);
-    c->partial.ensure(grid ? grid : 1u);
-    a.partial = c->partial.p;
+    if (a.partialExt) {
+        if (std::max(grid, 1u) > a.partialCap)
+            fail(FH_INVALID_ARGUMENT, "sweep needs " + std::to_string(grid) + " block partials; the fixed scratch holds " +
+                                          std::to_string(a.partialCap));
+        a.partial = a.partialExt;
+    } else {
+        c->partial.ensure(grid ? grid : 1u);
+        a.partial = c->partial.p;
+    }
     switch (nbc) {
         case 1: launch_team<1>(team, mode, w16, a, c->stream); break;
         case 4: launch_team<4>(team, mode, w16, a, c->stream); break;
@@ -1815,8 +1971,10 @@ void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, f
 
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
                          const float* Bin, float* Bout, bool w16, uint16_t* h16in, uint16_t* h16out, float* gacc,
-                         float* carry, bool pairs) {
+                         float* carry, bool pairs, float2* partials, uint32_t partialCap) {
     SweepArgs a{};
+    a.partialExt = partials;
+    a.partialCap = partialCap;
     a.jacobiOnly = pairs ? 0u : 1u;
     if (w16 && h16in && gacc && carry && band_parallel(ldb)) {
         a.inc = 2u;

@@ -76,6 +76,11 @@ def load():
         "or_solve_direct": (_I, [C.POINTER(OrGraph), _P, _F, _P]),
         "or_eb_node": (_U, [C.POINTER(OrGraph), _U, _U, _P, _P, _P, _P, _P, C.POINTER(OrEbParams), _P, _P, _P, _P,
                             _P, _P]),
+        "or_sat_vp": (_D, [_D]),
+        "or_sat_vp_slope": (_D, [_D]),
+        "or_latent": (_D, [_D, _D, _D, _D, _D, _D]),
+        "or_latent_slope": (_D, [_D, _D, _D, _D, _D, _D]),
+        "or_soil_storage": (_D, [C.POINTER(OrEbMaterial), _D]),
         "or_mlp_forward": (None, [_P, _U, _U, _P, _U, _P]),
         "or_mlp_loss_grad": (_D, [_P, _U, _U, _P, _P, _U, _P]),
     }
@@ -293,6 +298,29 @@ def eb_node(g: Graph, F: int, leaf_count: int, material, mats, absorbed, params:
                            _p(Blw),
                            C.cast(C.pointer(conv), _P), _p(mdt), _p(meq))
     return T, Blw, int(it), bool(conv.value), float(mdt[0]), float(meq[0])
+
+
+def sat_vp(T: float) -> float:
+    """Saturation vapour pressure (hPa) at T (K), core:854-858."""
+    return float(load().or_sat_vp(T))
+
+
+def sat_vp_slope(T: float) -> float:
+    return float(load().or_sat_vp_slope(T))
+
+
+def latent(T, rho, P, ea, ra, rs) -> float:
+    """Latent heat flux (W m-2), core:867-883."""
+    return float(load().or_latent(T, rho, P, ea, ra, rs))
+
+
+def latent_slope(T, rho, P, ea, ra, rs) -> float:
+    return float(load().or_latent_slope(T, rho, P, ea, ra, rs))
+
+
+def soil_storage(material: OrEbMaterial, dtime: float) -> float:
+    """Soil storage coefficient (W m-2 K-1), core:909-915."""
+    return float(load().or_soil_storage(C.byref(material), dtime))
 
 
 def mlp_forward(params, x, hidden: int) -> np.ndarray:
