@@ -209,6 +209,19 @@ int fh_sweep_device(fh_context *ctx, uint32_t nb, const float *d_c, const float 
                     const float *d_t, const float *d_Bin, float *d_Bout, uint32_t precision);
 uint32_t fh_band_stride(uint32_t nb);
 
+/* fh_sweep_device for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the same sweep and results, with the per-block residual partials in a
+ * fixed 2 MiB scratch array of the library instead of a context buffer that
+ * grows on demand; the call makes no device allocation.  No counterpart in
+ * the reference.  Limit: a sweep needing more than 262144 block partials
+ * (about 2 M facets at the usual team sizes) fails with FH_INVALID_ARGUMENT
+ * before any launch.  Every context on a device shares that device's
+ * scratch, which keeps the last call's contents: calls that overlap on one
+ * device (other streams or threads) race on it. */
+int fh_sweep_device_noalloc(fh_context *ctx, uint32_t nb, const float *d_c, const float *d_r,
+                            const float *d_t, const float *d_Bin, float *d_Bout, uint32_t precision);
+
 /* The context's HIP stream (hipStream_t), for event timing by the caller. */
 void *fh_stream(const fh_context *ctx);
 

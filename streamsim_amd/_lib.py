@@ -119,6 +119,7 @@ SIGNATURES = {
                                    C.POINTER(FhSolveStats)]),
     "fh_incident_bands": (_I, [_P, _U32, _P, _P, _P, _P]),
     "fh_sweep_device": (_I, [_P, _U32, _P, _P, _P, _P, _P, _U32]),
+    "fh_sweep_device_noalloc": (_I, [_P, _U32, _P, _P, _P, _P, _P, _U32]),
     "fh_band_stride": (_U32, [_U32]),
     "fh_stream": (_P, [_P]),
     "fh_eb_set_background_angular_strength": (_I, [_P, _F]),

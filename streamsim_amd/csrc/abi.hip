@@ -11,14 +11,41 @@
 #include <cmath>
 #include <cstdio>
 #include <memory>
+#include <mutex>
+#include <vector>
 
 #include "solver.hpp"
 
 using fh::fail;
 
+// Fixed scratch of fh_sweep_device_noalloc: the per-block residual
+// partials of one sweep, 2 MiB on each device the library runs on (held
+// until the process ends).  Host code never passes `scratch` itself (its
+// name is a host-side stand-in): fh_create looks up the device address once
+// per device and every context keeps it in fixedPartials.
+constexpr uint32_t kFixedPartials = 262144;  // float2 entries
+static __device__ __align__(256) float2 scratch[kFixedPartials];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+std::mutex g_scratch_mu;
+std::vector<float2*> g_scratch_addr;  // per device ordinal
+
+// Device address of this device's scratch array; the device is current.
+float2* fixed_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_addr.size() <= size_t(device)) g_scratch_addr.resize(size_t(device) + 1u, nullptr);
+    if (!g_scratch_addr[size_t(device)]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            fail(FH_RUNTIME_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[size_t(device)] = static_cast<float2*>(ptr);
+    }
+    return g_scratch_addr[size_t(device)];
+}
 
 template <class F>
 int guarded(fh_context* c, F&& body) {
@@ -208,6 +235,7 @@ int fh_create(const fh_config* config, fh_context** out) {
         c->cfg = *config;
         c->device = int(config->gpuIndex);
         FH_HIP(hipSetDevice(c->device));
+        c->fixedPartials = fixed_scratch(c->device);
         FH_HIP(hipStreamCreate(&c->stream));
         FH_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->hctrl), sizeof(fh::SolveCtrl), hipHostMallocDefault));
         c->ctrl.ensure(1);
@@ -713,6 +741,21 @@ int fh_sweep_device(fh_context* c, uint32_t nb, const float* dc, const float* dr
         if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
         if (!dc || !dr || !dt || !dBin || !dBout) fail(FH_INVALID_ARGUMENT, "arrays must not be null");
         fh::launch_single_sweep(c, fh::band_stride(nb), dc, dr, dt, dBin, dBout, precision == FH_PRECISION_BF16_DC);
+    });
+}
+
+// fh_sweep_device with the block partials in the fixed scratch array: no
+// device allocation in the call
+int fh_sweep_device_noalloc(fh_context* c, uint32_t nb, const float* dc, const float* dr, const float* dt,
+                            const float* dBin, float* dBout, uint32_t precision) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "sweep()");
+        if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
+        if (!dc || !dr || !dt || !dBin || !dBout) fail(FH_INVALID_ARGUMENT, "arrays must not be null");
+        if (!c->fixedPartials) fail(FH_RUNTIME_ERROR, "fixed scratch is not set up");
+        fh::launch_single_sweep(c, fh::band_stride(nb), dc, dr, dt, dBin, dBout, precision == FH_PRECISION_BF16_DC,
+                                nullptr, nullptr, nullptr, nullptr, false, c->fixedPartials, kFixedPartials);
     });
 }
 

@@ -245,6 +245,10 @@ struct fh_context {
 
     fh::EbState eb;
     fh::MlpState mlp;
+
+    // device address of this device's fixed scratch array (abi.hip), the
+    // per-block partials of fh_sweep_device_noalloc
+    float2* fixedPartials = nullptr;
 };
 
 namespace fh {
