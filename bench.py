@@ -41,6 +41,14 @@ WORKLOADS = {
 }
 
 
+WORKLOAD_SCENES = {
+    "forest": "25 ellipsoid crowns (2 x 2 x 3 m, 1600 leaves of 0.1 m) on 50 x 50 m ground of 20k triangles",
+    "terrain": "512 x 512 DEM at 1 m: sum of 4 sines of 20 m amplitude each (wavelengths 50-300 m) + N(0, 0.5 m) "
+               "noise; 300 crowns (2 x 2 x 3 m, 1600 leaves of 0.1 m)",
+    "urban": "40 box buildings (5-30 m) meshed at 0.5 m + 100 crowns",
+}
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -61,21 +69,34 @@ def parse():
     return ap.parse_args()
 
 
-def sweep_bytes(S: int, nnz: int, nb: int, entry: int, increments: bool = False) -> int:
-    """Algorithmic HBM bytes of one sweep: row offsets, `entry` bytes per CSR
-    entry (8: column + FP32 weight; 4: packed column | count), fsky, and per
-    side-band c, r, t read + B read + B write; the increment sweep of BF16
-    solves besides reads the BF16 increments it gathers once and writes the
-    next ones (2 + 2 bytes per side-band) and reads and writes the FP32
-    accumulated gathers and carries (16 bytes per side-band)."""
-    b = 4 * (S + 1) + entry * nnz + 4 * S + 20 * S * nb
-    return b + (20 * S * nb if increments else 0)
+def sweep_bytes(S: int, F: int, nnz: int, nb: int, tables: bool, entry: int = 8) -> int:
+    """Algorithmic HBM bytes of one SpMM sweep (compulsory traffic, gathers
+    served from cache not counted): row offsets 4 (S + 1), `entry` bytes per
+    CSR entry (8: column + FP32 weight; 4: packed column | count), fsky 4 S,
+    and per side-band c, r, t read + B read + B write (20 S nb).  In the
+    energy balance's form (`tables`) r and t come from per-material tables
+    (4 F: the material index per facet) and c, D from per-side direct
+    factors (4 S), so the per side-band part is B read + B write (8 S nb)."""
+    b = 4 * (S + 1) + entry * nnz + 4 * S
+    if tables:
+        return b + 4 * F + 4 * S + 8 * S * nb
+    return b + 20 * S * nb
 
 
-def time_sweep(eng, nb: int, precision: int, reps: int) -> float:
+def scheme_bytes(S: int, nb: int, increments: bool, chebyshev: bool) -> int:
+    """Bytes per sweep that the solver's own schemes add to the algorithmic
+    ones: the increment sweep of the BF16 solves reads and writes the FP32
+    accumulated gathers and carries (16 bytes per side-band), writes the next
+    BF16 increments and reads the gathered ones once (2 + 2); Chebyshev
+    reads the iterate before B (4)."""
+    return S * nb * ((20 if increments else 0) + (4 if chebyshev else 0))
+
+
+def time_sweep(eng, nb: int, precision: int, reps: int, tables: bool = True) -> float:
     """Average duration (s) of the SpMM sweep kernel: HIP events recorded on
-    the engine's stream around `reps` back-to-back launches, inputs in HBM."""
-    return eng.benchmarkSweep(nb, precision, reps)
+    the engine's stream around `reps` back-to-back launches, inputs in HBM;
+    `tables`: the form the energy balance's shortwave solves run."""
+    return eng.benchmarkSweep(nb, precision, reps, tables=tables)
 
 
 def cpu_baseline(eng, scene, windows, meteo, cfg, nb_total: int, budget_s: float = 25.0):
@@ -179,11 +200,12 @@ def precision_report(eng, scene, windows, meteo, cfg) -> dict:
 
 
 def cpu_baseline_sampled(eng, scene, windows, nb_total: int, st_plain, reps: int = 3):
-    """For graphs too large to converge a node on one core within the bench:
-    the oracle (its current thread count) times `reps` full-scene FP64 sweeps
-    of all shortwave bands and `reps` longwave sweeps; a node is then priced
-    with the plain GPU run's sweep counts per node (the oracle's Jacobi
-    sweeps would need more of them).  Returns facet-band-steps/s."""
+    """The metric's criterion on the CPU for graphs too large to converge a
+    node within the bench: the oracle (its current thread count) times `reps`
+    full-scene FP64 sweeps of all shortwave bands and `reps` longwave sweeps;
+    a node is then priced with the plain GPU run's sweep counts per node (the
+    oracle's Jacobi sweeps would need more of them).  Returns
+    facet-band-steps/s."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_ref as O
 
@@ -205,10 +227,58 @@ def cpu_baseline_sampled(eng, scene, windows, nb_total: int, st_plain, reps: int
     t_lw = (time.perf_counter() - t0) / reps
     nodes = st_plain.nodes
     per_node = (sum(st_plain.sw_sweeps) * t_sw + sum(st_plain.lw_sweeps) * t_lw) / nodes
-    sample = (f"{reps} FP64 sweeps of {nb} bands ({t_sw:.1f}s each) + {reps} LW sweeps ({t_lw:.1f}s each) over the full graph, "
-              f"priced at the plain run's {sum(st_plain.sw_sweeps) / nodes:.1f} SW + "
+    sample = (f"{reps} FP64 sweeps of {nb} bands ({t_sw:.2f}s each) + {reps} LW sweeps ({t_lw:.2f}s each) over the full "
+              f"graph, priced at the plain run's {sum(st_plain.sw_sweeps) / nodes:.1f} SW + "
               f"{sum(st_plain.lw_sweeps) / nodes:.1f} LW sweeps per node")
     return scene.facet_count * nb_total / per_node, reps * (t_sw + t_lw), sample
+
+
+# the reference's fixed schedule per node (core:3495-3565, 3745-3846)
+REF_SWEEPS = 64            # kIterations, core:81
+REF_COUPLING_BUDGET = 20   # couplingIterations default, core:3206-3208
+
+
+def cpu_baseline_reference(eng, scene, windows, nb_total: int, day: list, budget_s: float = 12.0):
+    """The reference's own CPU route (facetBackend=cpu) at its own schedule:
+    the oracle's restatement of solveCpu (core:1476-1529: one band, FP64
+    accumulation, single-threaded as in the reference) timed on full-graph
+    sweeps of the benchmark graph; a node is priced at the reference's fixed
+    schedule: per shortwave window with light a 64-sweep solve, an incident
+    pass and a 64-sweep diffuse-only solve (core:3528-3536); per coupling
+    iteration a 64-sweep longwave solve and an incident pass, for the
+    default budget of 20 iterations (core:3745-3832); the final 64-sweep
+    longwave solve and incident pass (core:3834-3846).  An incident pass is
+    counted as one sweep (both are one pass over the CSR).  Returns
+    facet-band-steps/s, the CPU seconds spent and the sample description."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_ref as O
+
+    g = eng.getGraph()
+    og = O.Graph(g.rowOffsets, g.neighbors, g.counts, g.denominators, g.skyCounts)
+    S = scene.surface_count
+    rng = np.random.default_rng(1)
+    optics = np.zeros((S, 4), np.float32)
+    optics[:, 0] = rng.uniform(0.0, 0.4, S)
+    optics[:, 1] = 0.5 * optics[:, 0]
+    optics[:, 3] = rng.uniform(0.0, 300.0, S)
+    t0 = time.perf_counter()
+    O.solve(og, optics, 20.0, 1)
+    t1 = time.perf_counter() - t0
+    reps = int(max(1, min(64, budget_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    O.solve(og, optics, 20.0, reps)
+    t_sweep = (time.perf_counter() - t0) / reps
+    lit = sum(1 for m in day if m.shortwave > 0.0)
+    per_window = 2 * REF_SWEEPS + 1
+    per_coupling = REF_SWEEPS + 1
+    sweeps = lit * windows.count * per_window + len(day) * (REF_COUPLING_BUDGET * per_coupling + per_coupling)
+    seconds = sweeps * t_sweep
+    sample = (f"{reps + 1} full-graph single-band FP64 sweeps of the solveCpu restatement ({t_sweep:.3f}s each, "
+              f"1 core); {len(day)} nodes ({lit} lit) priced at the reference's schedule: per lit window "
+              f"2 x {REF_SWEEPS} sweeps + incident ({windows.count} windows), per node "
+              f"{REF_COUPLING_BUDGET} coupling iterations (its default budget) x ({REF_SWEEPS} LW sweeps + incident) "
+              f"+ the final LW solve; {sweeps} sweeps, {seconds:.0f} s")
+    return scene.facet_count * nb_total * len(day) / seconds, (reps + 1) * t_sweep, sample
 
 
 def load_surrogate(eng, path: str) -> bool:
@@ -291,18 +361,23 @@ def main() -> None:
         print(json.dumps(rep), flush=True)
         return
     if a.mode == "sweep":
-        # counter runs: the SW sweep in FP32 and the increment sweep of BF16
-        # solves, the LW sweep, and a 1 GiB copy as the byte calibration of
-        # the traffic counters
+        # counter runs: the SW sweeps of the energy balance (material tables)
+        # in FP32 and as the increment sweep of BF16 solves, the same with
+        # streamed c, r, t arrays, the LW sweep, and a 1 GiB copy as the byte
+        # calibration of the traffic counters
         nb = windows.count
-        out = {"workload": a.workload, "sides": S, "nnz": diag.directedEdgeCount}
-        for name, nbk, prec in (("sw_fp32", nb, FH_PRECISION_FP32), ("sw_bf16", nb, FH_PRECISION_BF16_DC),
-                                ("lw_fp32", 1, FH_PRECISION_FP32)):
-            sec = time_sweep(eng, nbk, prec, a.roofline_reps)
+        F, nnz = scene.facet_count, diag.directedEdgeCount
+        out = {"workload": a.workload, "sides": S, "facets": F, "nnz": nnz}
+        for name, nbk, prec, tab in (("sw_fp32", nb, FH_PRECISION_FP32, True), ("sw_inc", nb, FH_PRECISION_BF16_DC, True),
+                                     ("sw_fp32_arrays", nb, FH_PRECISION_FP32, False),
+                                     ("sw_inc_arrays", nb, FH_PRECISION_BF16_DC, False),
+                                     ("lw_fp32", 1, FH_PRECISION_FP32, False)):
+            sec = time_sweep(eng, nbk, prec, a.roofline_reps, tables=tab)
             # single-band sweeps read the packed entries
-            alg = sweep_bytes(S, diag.directedEdgeCount, nbk, 8 if nbk >= 16 else 4,
-                              increments=prec != FH_PRECISION_FP32 and nbk >= 16)
-            out[name] = {"us": sec * 1e6, "alg_bytes": alg, "GBps": alg / sec / 1e9}
+            alg = sweep_bytes(S, F, nnz, nbk, tables=tab and nbk >= 16, entry=8 if nbk >= 16 else 4)
+            extra = scheme_bytes(S, nbk, increments=prec != FH_PRECISION_FP32 and nbk >= 16, chebyshev=tab and nbk >= 16)
+            out[name] = {"us": sec * 1e6, "alg_bytes": alg, "scheme_bytes": extra, "GBps": alg / sec / 1e9,
+                         "frac": alg / sec / 1e9 / HBM_PEAK_GBS}
         cs = eng.benchmarkCopy(1 << 30, 20)
         out["copy_1GiB"] = {"us": cs * 1e6, "GBps": 2.0 * (1 << 30) / cs / 1e9}
         print(json.dumps(out), flush=True)
@@ -340,44 +415,47 @@ def main() -> None:
     log(f"[bench] ablation {json.dumps({k: round(v['seconds'], 4) for k, v in ablation.items()})}")
 
     # roofline of the fast path's dominant kernel, the increment sweep of
-    # the BF16 shortwave solves (all bands active), and of the plain path's,
-    # the FP32 sweep
+    # the BF16 shortwave solves (all bands active, the energy balance's form:
+    # material tables, direct factors, Chebyshev), and of the plain path's,
+    # the FP32 sweep in the same form
     nb = windows.count
     nnz = diag.directedEdgeCount
+    F = scene.facet_count
     sec16 = time_sweep(eng, nb, FH_PRECISION_BF16_DC, a.roofline_reps)
     sec32 = time_sweep(eng, nb, FH_PRECISION_FP32, a.roofline_reps)
-    alg16 = sweep_bytes(S, nnz, nb, 8, increments=True)
-    alg32 = sweep_bytes(S, nnz, nb, 8)
-    achieved = alg16 / sec16 / 1e9
+    alg = sweep_bytes(S, F, nnz, nb, tables=True)
+    extra16 = scheme_bytes(S, nb, increments=True, chebyshev=True)
+    extra32 = scheme_bytes(S, nb, increments=False, chebyshev=True)
+    achieved = alg / sec16 / 1e9
     log(f"[bench] SW sweep nb={nb}: increments {sec16 * 1e6:.1f} us ({achieved:.0f} GB/s), "
-        f"fp32 {sec32 * 1e6:.1f} us ({alg32 / sec32 / 1e9:.0f} GB/s)")
+        f"fp32 {sec32 * 1e6:.1f} us ({alg / sec32 / 1e9:.0f} GB/s)")
     traffic32 = traffic16 = None
     tf = ROOT / "profiles" / f"sweep_traffic_{a.workload}.json"
     if tf.exists():
         tj = json.loads(tf.read_text())
-        traffic32 = tj.get("k_sweep_v", {}).get("bytes_per_launch")
-        traffic16 = tj.get("k_sweep_inc", {}).get("bytes_per_launch")
+        traffic32 = tj.get("sw_fp32", {}).get("bytes_per_launch")
+        traffic16 = tj.get("sw_inc", {}).get("bytes_per_launch")
 
-    cpu = cpu_all = None
+    cpu = cpu_metric = None
     if not a.no_cpu_baseline:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_ref as O
         ncores = O.max_threads()
-        for threads in (1, ncores):
-            O.set_threads(threads)
-            if nnz <= 50_000_000:
-                v, dt, sample = cpu_baseline(eng, scene, windows, meteo, cfg, nb_total)
-            else:
-                v, dt, sample = cpu_baseline_sampled(eng, scene, windows, nb_total, st_plain)
-            rec = {"value": v, "unit": "facet-band-steps/s", "cores": threads, "kind": "port",
-                   "sample": f"{sample}; {dt:.1f}s on {threads} core{'s' if threads > 1 else ''}"}
-            log(f"[bench] cpu baseline ({threads} cores) {v:.4g} facet-band-steps/s ({dt:.1f}s)")
-            if threads == 1:
-                cpu = rec
-            else:
-                cpu_all = rec
+        v, dt, sample = cpu_baseline_reference(eng, scene, windows, nb_total, day)
+        cpu = {"value": v, "unit": "facet-band-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{sample}; {dt:.1f}s timed"}
+        log(f"[bench] cpu baseline, reference schedule (1 core) {v:.4g} facet-band-steps/s ({dt:.1f}s)")
         O.set_threads(ncores)
+        if nnz <= 50_000_000:
+            v, dt, sample = cpu_baseline(eng, scene, windows, meteo, cfg, nb_total)
+        else:
+            v, dt, sample = cpu_baseline_sampled(eng, scene, windows, nb_total, st_plain)
+        cpu_metric = {"value": v, "unit": "facet-band-steps/s", "cores": ncores, "kind": "port",
+                      "sample": f"{sample}; {dt:.1f}s on {ncores} cores"}
+        log(f"[bench] cpu baseline, metric criterion ({ncores} cores) {v:.4g} facet-band-steps/s ({dt:.1f}s)")
 
+    # all-in: the form-factor build (host BVH + device rays) charged to the run
+    t_setup = t_graph + t_bvh
     out = {
         "metric": "converged facet-band-steps/s at 0.01 K: surrogate+BF16 vs plain FP32; HBM GB/s",
         "value": value,
@@ -405,25 +483,36 @@ def main() -> None:
                          "fixed point)" + (" + surrogate warm start" if have_sur else "") +
                          "; both paths: Chebyshev, per-band retirement, coupled facet pairs as 2-facet blocks",
             "parallelism": "single GPU",
+            "scene": WORKLOAD_SCENES[a.workload],
         },
         "plain_fp32": {"value": plain_value, "seconds": t_plain, **st_plain.summary()},
         "fast": {"value": value, "seconds": t_fast, **st_fast.summary()},
         "speedup_vs_plain": t_plain / t_fast,
+        "all_in": {"value": work / (t_fast + t_setup), "plain_value": work / (t_plain + t_setup),
+                   "speedup_vs_plain": (t_plain + t_setup) / (t_fast + t_setup), "setup_seconds": t_setup,
+                   "note": "form factors (host BVH + device rays) charged to the run"},
         "ablation": ablation,
         "form_factors": {"seconds": t_graph, "rays_per_s": rays_per_s, "directed_edges": nnz,
                          "bvh_host_seconds": t_bvh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic16,
                      "kernel": f"k_sweep_hv<2, false> (single facets) + k_sweep_hv<2, true> (coupled pairs, second "
-                               f"stream, side by side): increment sweep of the BF16 shortwave solves, {nb} bands; "
-                               f"bytes and time per sweep",
-                     "bytes_per_launch": alg16, "us_per_launch": sec16 * 1e6,
+                               f"stream, side by side): increment sweep of the BF16 shortwave solves, {nb} bands, "
+                               f"material tables; bytes and time per sweep",
+                     "bytes_per_launch": alg, "us_per_launch": sec16 * 1e6,
+                     "formula": "4(S+1) + 8 nnz + 4S + 4F + 4S + 8 S nb (row offsets, column + FP32 weight, fsky, "
+                                "material per facet, direct factor per side, B read + write)",
+                     "scheme_bytes_per_launch": extra16,
+                     "scheme_note": "increment scheme's accumulated gathers and carries (16 B), BF16 increments "
+                                    "written and gathered (2 + 2 B), Chebyshev's previous iterate (4 B) per side-band: "
+                                    "not algorithmic, not in achieved",
                      "fp32_sweep": {"kernel": f"k_sweep_v<0, false, false> + k_sweep_v<0, false, true>: FP32 "
-                                              f"shortwave sweep of the plain path, {nb} bands",
-                                    "achieved": alg32 / sec32 / 1e9, "bytes_per_launch": alg32,
+                                              f"shortwave sweep of the plain path, {nb} bands, material tables",
+                                    "achieved": alg / sec32 / 1e9, "frac": alg / sec32 / 1e9 / HBM_PEAK_GBS,
+                                    "bytes_per_launch": alg, "scheme_bytes_per_launch": extra32,
                                     "us_per_launch": sec32 * 1e6, "traffic": traffic32}},
         "cpu_baseline": cpu,
-        "cpu_baseline_all_cores": cpu_all,
+        "cpu_baseline_metric": cpu_metric,
     }
     print(json.dumps(out), flush=True)
     eng.destroy()

@@ -232,9 +232,13 @@ void *fh_stream(const fh_context *ctx);
  * coupled facet pairs are updated as blocks by a second kernel on a second
  * stream beside the first (joined before the next sweep), and BF16
  * precision times the increment sweep the BF16 solves run (BF16 increments
- * of B gathered with FP32 weights, FP32 accumulated gathers and carries). */
-int fh_benchmark_sweep(fh_context *ctx, uint32_t nb, uint32_t precision, uint32_t reps,
-                       float *microsecondsPerSweep);
+ * of B gathered with FP32 weights, FP32 accumulated gathers and carries).
+ * materialTables = 1 (band-parallel strides): the sweep in the form the
+ * energy balance's shortwave solves run it -- r and t from per-material
+ * tables, c and D from per-side direct factors, Chebyshev weights (the
+ * previous iterate read) -- instead of streamed [S][ldb] c, r, t arrays. */
+int fh_benchmark_sweep(fh_context *ctx, uint32_t nb, uint32_t precision, uint32_t materialTables,
+                       uint32_t reps, float *microsecondsPerSweep);
 /* Average duration (microseconds) of a plain device-to-device copy kernel of
  * `bytes` bytes (reads `bytes`, writes `bytes`) on the context's stream: the
  * calibration point for hardware traffic counters and achievable bandwidth. */

@@ -134,7 +134,7 @@ SIGNATURES = {
     "fh_mlp_forward": (_I, [_P, _P, _U32, _P]),
     "fh_mlp_loss_grad": (_I, [_P, _P, _P, _U32, _P, _P]),
     "fh_mlp_train": (_I, [_P, _P, _P, _U32, _U32, _U32, _F, _F, _U32, _P]),
-    "fh_benchmark_sweep": (_I, [_P, _U32, _U32, _U32, _P]),
+    "fh_benchmark_sweep": (_I, [_P, _U32, _U32, _U32, _U32, _P]),
     "fh_synchronize": (_I, [_P]),
     "fh_benchmark_copy": (_I, [_P, C.c_uint64, _U32, _P]),
     "fh_eb_unit_count": (_U32, [_P]),

@@ -759,13 +759,49 @@ int fh_sweep_device_noalloc(fh_context* c, uint32_t nb, const float* dc, const f
     });
 }
 
-int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t reps, float* us) {
+int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t materialTables, uint32_t reps,
+                       float* us) {
     if (!c) return FH_INVALID_ARGUMENT;
     return guarded(c, [&] {
         require_graph(c, "sweep()");
         if (nb == 0u || nb > 4096u || reps == 0u || !us) fail(FH_INVALID_ARGUMENT, "invalid benchmark arguments");
         const uint32_t ldb = fh::band_stride(nb);
         const size_t n = size_t(c->S) * ldb;
+        // the energy balance's form of the band-parallel sweep: r, t from two
+        // material tables (first half of the facets, second half), c and D
+        // from per-side direct factors, Chebyshev weights != 1 (the previous
+        // iterate is read)
+        fh::DBuf<uint32_t> tabMat;
+        fh::DBuf<float> tabF;
+        fh::SweepTables tables;
+        const bool useTables = materialTables && fh::uses_bf16_gather(ldb);
+        if (useTables) {
+            std::vector<uint32_t> mat(c->F);
+            for (uint32_t f = 0; f < c->F; ++f) mat[f] = f < c->F / 2u ? 0u : 1u;
+            std::vector<float> host(4u * ldb + c->S, 0.0f);
+            for (uint32_t b = 0; b < nb; ++b) {
+                host[b] = 0.45f;               // rtab row 0
+                host[ldb + b] = 0.15f;         // rtab row 1
+                host[2u * ldb + b] = 0.40f;    // ttab row 0 (row 1: 0)
+                host[3u * ldb + b] = 900.0f / float(nb);  // dni
+            }
+            for (uint32_t s = 0; s < c->S; ++s) host[4u * ldb + s] = (s & 1u) ? 0.0f : 0.6f;  // direct factors
+            tabMat.ensure(c->F);
+            tabF.ensure(host.size() + 2u * ldb + ldb);
+            upload(c, tabMat.p, mat.data(), mat.size());
+            upload(c, tabF.p, host.data(), host.size());
+            float* omega = tabF.p + host.size();
+            fh::launch_fill_f32(c, omega, ldb, 1.2f);
+            std::vector<float> sky(nb, 60.0f / float(nb));
+            fh::set_sky(c, sky.data(), nb, ldb);
+            tables.fmat = tabMat.p;
+            tables.rtab = tabF.p;
+            tables.ttab = tabF.p + 2u * ldb;
+            tables.dniv = tabF.p + 3u * ldb;
+            tables.dirf = tabF.p + 4u * ldb;
+            tables.omega = omega;
+        }
+        const fh::SweepTables* tab = useTables ? &tables : nullptr;
         fh::DBuf<float> buf;
         buf.ensure(5 * n);
         float* cc = buf.p;
@@ -799,14 +835,15 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
             fh::launch_fill_f32(c, G, n, 100.0f);
             fh::launch_fill_f32(c, K, n, 0.0f);
         }
-        for (int i = 0; i < 3; ++i) fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1, G, K, true);
+        for (int i = 0; i < 3; ++i)
+            fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1, G, K, true, nullptr, 0u, tab);
         hipEvent_t e0, e1;
         FH_HIP(hipEventCreate(&e0));
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c->stream));
         for (uint32_t i = 0; i < reps; ++i)
             fh::launch_single_sweep(c, ldb, cc, rr, tt, (i & 1u) ? B1 : B0, (i & 1u) ? B0 : B1, w16,
-                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1, G, K, true);
+                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1, G, K, true, nullptr, 0u, tab);
         FH_HIP(hipEventRecord(e1, c->stream));
         FH_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

@@ -116,6 +116,19 @@ const uint32_t* sweep_schedule(const fh_context* c);
 // lanes per facet team of the SpMM kernels for the current graph
 int sweep_team(const fh_context* c);
 
+// Energy-balance form of a band-parallel sweep for launch_single_sweep: r, t
+// from material tables and c, D from per-side direct factors (as the
+// shortwave solves of a node run them), and the Chebyshev weights per band
+// (null: plain Jacobi).
+struct SweepTables {
+    const uint32_t* fmat = nullptr;  // material per facet
+    const float* rtab = nullptr;     // [materials][ldb]
+    const float* ttab = nullptr;     // [materials][ldb]
+    const float* dirf = nullptr;     // [S]
+    const float* dniv = nullptr;     // [ldb]
+    const float* omega = nullptr;    // [ldb]
+};
+
 // One sweep, no host synchronisation (timing entry).  With w16 and BF16
 // copies given (band-parallel strides), the BF16-gather sweep h16in -> h16out;
 // with gacc and carry ([S][ldb]) also given, the increment sweep of BF16
@@ -127,6 +140,7 @@ int sweep_team(const fh_context* c);
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
                          const float* Bin, float* Bout, bool w16, uint16_t* h16in = nullptr,
                          uint16_t* h16out = nullptr, float* gacc = nullptr, float* carry = nullptr,
-                         bool pairs = false, float2* partials = nullptr, uint32_t partialCap = 0u);
+                         bool pairs = false, float2* partials = nullptr, uint32_t partialCap = 0u,
+                         const SweepTables* tables = nullptr);
 
 }  // namespace fh

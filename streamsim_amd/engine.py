@@ -362,10 +362,12 @@ class FacetEngine:
         return float(loss.value)
 
     # -- timing ------------------------------------------------------------------
-    def benchmarkSweep(self, nb: int, precision: int, reps: int = 50) -> float:
-        """Average SpMM sweep kernel duration in seconds (HIP events)."""
+    def benchmarkSweep(self, nb: int, precision: int, reps: int = 50, tables: bool = False) -> float:
+        """Average SpMM sweep kernel duration in seconds (HIP events); tables:
+        the energy balance's form (material tables, direct factors,
+        Chebyshev weights) at band-parallel strides."""
         us = C.c_float(0.0)
-        self._check(self._lib.fh_benchmark_sweep(self._h, nb, precision, reps, C.byref(us)))
+        self._check(self._lib.fh_benchmark_sweep(self._h, nb, precision, 1 if tables else 0, reps, C.byref(us)))
         return us.value * 1e-6
 
     def benchmarkCopy(self, nbytes: int, reps: int = 20) -> float:

@@ -245,15 +245,20 @@ def urban(seed: int = 3, buildings: int = 40, trees: int = 100, leaves: int = 16
     return _finish("urban", objects, ground, leaf_mat=mat)
 
 
-def terrain(seed: int = 5, n: int = 512, trees: int = 300, leaves: int = 1600) -> Scene:
-    """Large terrain forest: 512 x 512 DEM at 1 m + 300 crowns (~1.0 M facets)."""
+def terrain(seed: int = 5, n: int = 512, trees: int = 300, leaves: int = 1600, amplitude: float = 20.0) -> Scene:
+    """Large terrain forest: 512 x 512 DEM at 1 m + 300 crowns (~1.0 M facets).
+
+    Heights: the sum of 4 sines of `amplitude` metres each (random
+    wavevectors, wavelengths 50-300 m along each axis, random phases) plus
+    Gaussian noise of 0.5 m.  Crowns (semi-axes 2 x 3 x 2 m) are centred
+    4 m above the highest ground point under their footprint."""
     rng = np.random.default_rng(seed)
     kx = rng.uniform(2 * np.pi / 300.0, 2 * np.pi / 50.0, 4)
     kz = rng.uniform(2 * np.pi / 300.0, 2 * np.pi / 50.0, 4)
     ph = rng.uniform(0, 2 * np.pi, 4)
     xs = np.arange(n, dtype=np.float64)
     X, Z = np.meshgrid(xs, xs, indexing="ij")
-    H = sum(5.0 * np.sin(kx[k] * X + kz[k] * Z + ph[k]) for k in range(4)) + rng.normal(0.0, 0.5, X.shape)
+    H = sum(amplitude * np.sin(kx[k] * X + kz[k] * Z + ph[k]) for k in range(4)) + rng.normal(0.0, 0.5, X.shape)
     H = H - H.min()
 
     def height(x, z):
@@ -263,7 +268,8 @@ def terrain(seed: int = 5, n: int = 512, trees: int = 300, leaves: int = 1600) -
     tris = []
     for _ in range(trees):
         i, j = rng.integers(8, n - 8, 2)
-        tris.append(crown_leaves((float(i), H[i, j] + 4.0, float(j)), (2.0, 3.0, 2.0), leaves, 0.1, rng))
+        top = float(H[i - 2:i + 3, j - 2:j + 3].max())
+        tris.append(crown_leaves((float(i), top + 4.0, float(j)), (2.0, 3.0, 2.0), leaves, 0.1, rng))
     return _finish("terrain", np.concatenate(tris), ground)
 
 

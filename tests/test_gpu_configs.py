@@ -1,0 +1,270 @@
+"""GPU parity at the benchmark configurations' real sizes, through the C-ABI.
+
+* forest (60 000 facets, 1024 rays per side, 8 shortwave windows + longwave):
+  one daytime node against the FP64 oracle on the same graph, and the
+  24-node diurnal run's properties;
+* urban block (268 000 facets, 512 rays, 32 bands): the FP32 and BF16
+  shortwave solves per band against the FP64 oracle;
+* terrain forest (1.0 M facets, 512 rays, 47 windows + longwave): graph
+  invariants, one device sweep against an FP64 oracle sweep on the full
+  graph, and the 24-node diurnal run (plain FP32 and BF16 + surrogate).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from streamsim_amd import FacetEngine, faceteb, optics, scenes
+from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(oracle, eng):
+    g = eng.getGraph()
+    return g, oracle.Graph(g.rowOffsets, g.neighbors, g.counts, g.denominators, g.skyCounts)
+
+
+def _oracle_materials(oracle, kinds):
+    out = []
+    for m in kinds:
+        om = oracle.OrEbMaterial()
+        for k, v in m.items():
+            setattr(om, k, v)
+        out.append(om)
+    return out
+
+
+def _sw_reference(oracle, og, sc, s, win, met, dni, dif):
+    """FP64 oracle shortwave solve and absorbed shortwave on the node's own
+    sunlit fractions and the material tables' optics."""
+    r_tab, t_tab = faceteb.material_tables(["leaf", "soil", "building"], win)
+    mat = np.repeat(sc.material, 2)
+    r = r_tab[mat]
+    t = np.minimum(np.clip(t_tab[mat], 0, 1 - r), 1 - r)
+    sun = faceteb.sun_vector(met.zenith, met.azimuth)
+    cos = np.maximum(0.0, optics.side_normals(sc) @ sun).astype(np.float32)
+    D = (np.clip(s["sunlit"], 0, 1) * cos)[:, None] * dni[None, :]
+    sw, it, _ = oracle.solve_bands(og, r, t, np.zeros_like(r), D, dif, tol_rel=1e-9, max_iter=4000)
+    H = oracle.incident_bands(og, sw, D, dif)
+    absorbed = np.sum(np.clip(1 - r - t, 0, 1) * H, axis=1)
+    return sw, absorbed
+
+
+def _rel(a, b, mask=None):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if mask is not None:
+        a, b = a[mask], b[mask]
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
+
+
+def test_forest_node_and_diurnal_run(oracle):
+    # 60 000 facets, 1024 rays per side, 8 shortwave windows + longwave
+    print("forest: 60k facets, 1024 rays, 8 windows", flush=True)
+    sc = scenes.forest()
+    eng = FacetEngine(rays_per_side=1024, sun_rays_per_side=32, seed=1)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    g, og = _graph(oracle, eng)
+    S = sc.surface_count
+    assert np.all(g.denominators == 1024)
+    win = faceteb.ShortwaveWindows.for_bands(8)
+    faceteb.setup_scene(eng, sc, win)
+    meteo = faceteb.diurnal_meteo(24)
+    met = meteo[12]
+    dni, dif = win.irradiance(met.shortwave, math.cos(math.radians(met.zenith)))
+    cfg = faceteb.EbConfig(tolerance=1e-3, radiosity_tolerance=1e-7, max_coupling=300)
+    st = eng.ebRunNode(faceteb.node_params(met, cfg, first=True), dni, dif)
+    assert st.converged
+    s = eng.ebGetState()
+    sw, absorbed = _sw_reference(oracle, og, sc, s, win, met, dni, dif)
+    assert _rel(s["shortwaveRadiosity"], sw, sw > 1e-3) < 1e-5
+    assert _rel(s["absorbedShortwave"], absorbed, absorbed > 1.0) < 1e-5
+    op = oracle.OrEbParams()
+    op.airTemperatureK = met.air_temperature_k
+    op.vaporPressure = met.vapor_pressure
+    op.pressure = met.pressure
+    op.windSpeed = met.wind
+    op.skyLongwave = met.longwave
+    op.dTime = cfg.dtime
+    op.tolerance = 1e-3
+    op.relaxation = cfg.relaxation
+    op.maxCoupling = 300
+    op.lwMaxSweeps = 400
+    op.lwTolRel = 1e-9
+    T0 = np.full(S, met.air_temperature_k, np.float32)
+    mats = _oracle_materials(oracle, (faceteb.LEAF, faceteb.SOIL, faceteb.BUILDING))
+    T, Blw, _, conv, _, _ = oracle.eb_node(og, sc.facet_count, sc.leaf_count, sc.material, mats,
+                                           s["absorbedShortwave"], op, T0, T0)
+    assert conv
+    assert np.max(np.abs(s["temperature"] - T)) < 0.01
+    assert _rel(s["longwaveRadiosity"], Blw) < 1e-5
+    # the 24-node diurnal run at the benchmark's tolerances: every node
+    # converges, night nodes absorb no shortwave, temperatures stay physical,
+    # and the plain and fast settings reach the same states
+    run_cfg = faceteb.EbConfig()
+    plain = faceteb.run_diurnal(eng, meteo, win, run_cfg, precision=FH_PRECISION_FP32)
+    T_plain = eng.ebGetState()["temperature"]
+    fast = faceteb.run_diurnal(eng, meteo, win, run_cfg, precision=FH_PRECISION_BF16_DC)
+    T_fast = eng.ebGetState()["temperature"]
+    assert all(plain.converged) and all(fast.converged)
+    assert max(abs(c) for c in plain.closure) < 1e-4
+    assert np.max(np.abs(T_plain - T_fast)) < 0.02
+    assert 250.0 < float(T_plain.min()) and float(T_plain.max()) < 340.0
+    night = faceteb.node_params(meteo[0], run_cfg, first=False)
+    dni0, dif0 = win.irradiance(meteo[0].shortwave, math.cos(math.radians(meteo[0].zenith)))
+    st = eng.ebRunNode(night, dni0, dif0)
+    if meteo[0].shortwave <= 0.0:
+        assert np.all(eng.ebGetState()["absorbedShortwave"] == 0.0)
+    eng.destroy()
+
+
+def test_urban_bands_bf16_and_fp32_against_fp64(oracle):
+    # 268 000 facets, 512 rays, 32 shortwave bands: per band, the largest
+    # relative error over sides above 1e-3 of the band's maximum, FP32 and
+    # BF16 (increments) solves at 1e-6 against the oracle at 1e-9
+    print("urban: 268k facets, 512 rays, 32 bands", flush=True)
+    sc = scenes.urban()
+    eng = FacetEngine(rays_per_side=512, sun_rays_per_side=16, seed=1)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    _, og = _graph(oracle, eng)
+    win = faceteb.ShortwaveWindows.for_bands(32)
+    assert win.count == 32
+    met = faceteb.diurnal_meteo(24)[13]
+    dni, dif = win.irradiance(met.shortwave, math.cos(math.radians(met.zenith)))
+    sun = faceteb.sun_vector(met.zenith, met.azimuth)
+    sl = eng.computeSunlitFraction(sun)
+    r_tab, t_tab = faceteb.material_tables(["leaf", "soil", "building"], win)
+    mat = np.repeat(sc.material, 2)
+    r = r_tab[mat].astype(np.float32)
+    t = np.minimum(np.clip(t_tab[mat], 0, 1 - r), 1 - r).astype(np.float32)
+    cos = np.maximum(0.0, optics.side_normals(sc) @ sun).astype(np.float32)
+    D = ((np.clip(sl, 0, 1) * cos)[:, None] * dni[None, :]).astype(np.float32)
+    e = np.zeros_like(r)
+    ref, _, _ = oracle.solve_bands(og, r, t, e, D, dif, max_iter=4000, tol_rel=1e-9)
+    ref = ref.astype(np.float64)
+    mask = ref > 1e-3 * ref.max(axis=0, keepdims=True)
+    for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC):
+        res = eng.solveBands(r, t, e, D, dif, tolRel=1e-6, precision=prec)
+        got = res.radiosity.astype(np.float64)
+        err = [float(np.max(np.abs(got[mask[:, b], b] - ref[mask[:, b], b]) / ref[mask[:, b], b]))
+               for b in range(win.count)]
+        print(f"  precision {prec}: {res.iterations} sweeps, worst band error {max(err):.3g}", flush=True)
+        assert max(err) < 1e-5, (prec, err)
+    eng.destroy()
+
+
+@pytest.fixture(scope="module")
+def terrain():
+    print("terrain: 1.0M facets, 512 rays (graph build)", flush=True)
+    sc = scenes.terrain()
+    eng = FacetEngine(rays_per_side=512, sun_rays_per_side=16, seed=1)
+    eng.setGeometry(sc.vertices)
+    diag = eng.buildVisibilityGraph()
+    yield sc, eng, diag
+    eng.destroy()
+
+
+def test_terrain_graph_invariants(terrain):
+    # every side casts 512 rays: den = 512, and every ray hits one side or
+    # escapes (closure 0); counts positive, columns in range, rows sorted,
+    # no entry between the two sides of one facet
+    print("terrain: graph invariants", flush=True)
+    sc, eng, diag = terrain
+    g = eng.getGraph()
+    S = sc.surface_count
+    assert diag.maxClosureError == 0.0
+    assert np.all(g.denominators == 512)
+    row_sum = np.add.reduceat(g.counts.astype(np.int64), g.rowOffsets[:-1].astype(np.int64))
+    row_sum[np.diff(g.rowOffsets) == 0] = 0
+    np.testing.assert_array_equal(row_sum + g.skyCounts, g.denominators)
+    assert g.counts.min() >= 1 and g.neighbors.max() < S
+    rows = np.repeat(np.arange(S, dtype=np.uint32), np.diff(g.rowOffsets))
+    assert not np.any((g.neighbors >> 1) == (rows >> 1))
+    step = np.diff(g.neighbors.astype(np.int64))
+    same_row = rows[1:] == rows[:-1]
+    assert np.all(step[same_row] > 0)
+
+
+def test_terrain_sweep_against_fp64_oracle(oracle, terrain):
+    # one Jacobi sweep of 47 bands on device arrays over the full 468 M-entry
+    # graph against the oracle's FP64-accumulated sweep of the same inputs
+    print("terrain: one 47-band device sweep vs FP64 oracle", flush=True)
+    import ctypes as C
+    from streamsim_amd import _lib
+    sc, eng, _ = terrain
+    lib = _lib.load()
+    _, og = _graph(oracle, eng)
+    S, nb = sc.surface_count, 47
+    ldb = int(lib.fh_band_stride(nb))
+    rng = np.random.default_rng(5)
+    c = rng.uniform(0.0, 50.0, (S, ldb)).astype(np.float32)
+    r = rng.uniform(0.0, 0.5, (S, ldb)).astype(np.float32)
+    t = (rng.uniform(0.0, 0.4, (S, ldb)) * (1.0 - r)).astype(np.float32)
+    B = rng.uniform(0.0, 80.0, (S, ldb)).astype(np.float32)
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    held = []
+
+    def put(a):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), a.nbytes) == 0
+        assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0
+        held.append(p)
+        return p
+
+    try:
+        dc, dr, dt, dB, dO = put(c), put(r), put(t), put(B), put(np.zeros((S, ldb), np.float32))
+        assert lib.fh_sweep_device(eng.handle, nb, dc, dr, dt, dB, dO, FH_PRECISION_FP32) == 0
+        out = np.empty((S, ldb), np.float32)
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemcpy(out.ctypes.data, dO, out.nbytes, 2) == 0
+    finally:
+        for p in held:
+            hip.hipFree(p)
+    # the oracle's sweep: B' = e + r h_s + t h_o with h = in(B) (no sky, no
+    # direct term) and e = c is c + r G_s + t G_o
+    zero = np.zeros((S, nb), np.float32)
+    ref, _, _ = oracle.solve_bands(og, r[:, :nb], t[:, :nb], c[:, :nb], zero, np.zeros(nb, np.float32),
+                                   init=B[:, :nb], max_iter=1, tol_rel=-1.0)
+    # FP32 accumulation of a row of n entries is within (n - 1) u of the
+    # exact sum (u = 2^-24, recursive summation); the weights' rounding and
+    # the update's three operations add 4 u at most: the worst element is
+    # bounded by (n_max + 4) u, here about 2.5e-5 for the longest rows, and
+    # the typical element is far inside it
+    n_max = int(np.max(np.diff(og.rowOffsets)))
+    err = np.abs(out[:, :nb].astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-30)
+    bound = (n_max + 4) * 2.0 ** -24
+    print(f"  longest row {n_max}, worst error {err.max():.3g} (bound {bound:.3g}), median {np.median(err):.3g}",
+          flush=True)
+    assert err.max() <= bound
+    assert np.median(err) < 1e-6
+
+
+def test_terrain_diurnal_plain_and_fast(terrain):
+    # the benchmark's 24-node run: every node converges; the BF16 + surrogate
+    # settings reach the plain FP32 state within 0.02 K; the shortwave
+    # energy closure (input - absorbed - escaped, with ray-cast form factors
+    # not exactly reciprocal) stays small
+    print("terrain: 24-node diurnal run, plain vs fast", flush=True)
+    from pathlib import Path
+    sc, eng, _ = terrain
+    win = faceteb.ShortwaveWindows.for_bands(47)
+    faceteb.setup_scene(eng, sc, win)
+    cfg = faceteb.EbConfig()
+    meteo = faceteb.diurnal_meteo(24)
+    plain = faceteb.run_diurnal(eng, meteo, win, cfg, precision=FH_PRECISION_FP32)
+    T_plain = eng.ebGetState()["temperature"]
+    z = np.load(Path(__file__).resolve().parents[1] / "streamsim_amd" / "data" / "surrogate.npz", allow_pickle=False)
+    eng.mlpSetParams(int(z["hidden"]), z["params"], z["feature_mean"], z["feature_std"], float(z["target_mean"]),
+                     float(z["target_std"]))
+    fast = faceteb.run_diurnal(eng, meteo, win, cfg, precision=FH_PRECISION_BF16_DC, warm=1)
+    T_fast = eng.ebGetState()["temperature"]
+    assert all(plain.converged) and all(fast.converged)
+    assert np.max(np.abs(T_plain - T_fast)) < 0.02
+    assert max(abs(c) for c in plain.closure) < 1e-4
+    assert max(abs(c) for c in fast.closure) < 1e-4

@@ -792,3 +792,17 @@ def test_device_pointer_entry_points(oracle, nb):
     finally:
         hip.free()
 
+
+
+def test_benchmark_entry_points():
+    # the timing entry points run in both forms and both precisions
+    print("benchmark entry points", flush=True)
+    sc = scenes.row_crop(leaves_per_row=20, cells=4)
+    eng = _engine(R=32, seed=3)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    for nb in (1, 24, 47):
+        for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC):
+            for tables in (False, True):
+                assert eng.benchmarkSweep(nb, prec, 3, tables=tables) > 0.0
+    assert eng.benchmarkCopy(1 << 20, 3) > 0.0
