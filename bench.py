@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="terrain", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fast-precision", default="bf16", choices=["bf16", "fp8"],
+                    help="increments the fast path's shortwave solves gather (BF16, or FP8 in 12-band chunks)")
     ap.add_argument("--surrogate", default=str(ROOT / "streamsim_amd" / "data" / "surrogate.npz"))
     ap.add_argument("--roofline-reps", type=int, default=50)
     ap.add_argument("--mode", default="run", choices=["run", "sweep", "precision", "raster"],
@@ -83,13 +85,14 @@ def sweep_bytes(S: int, F: int, nnz: int, nb: int, tables: bool, entry: int = 8)
     return b + 20 * S * nb
 
 
-def scheme_bytes(S: int, nb: int, increments: bool, chebyshev: bool) -> int:
+def scheme_bytes(S: int, nb: int, increments: str | None, chebyshev: bool) -> int:
     """Bytes per sweep that the solver's own schemes add to the algorithmic
-    ones: the increment sweep of the BF16 solves reads and writes the FP32
-    accumulated gathers and carries (16 bytes per side-band), writes the next
-    BF16 increments and reads the gathered ones once (2 + 2); Chebyshev
-    reads the iterate before B (4)."""
-    return S * nb * ((20 if increments else 0) + (4 if chebyshev else 0))
+    ones: an increment sweep reads and writes the FP32 accumulated gathers
+    and carries (16 bytes per side-band) and writes the next increments and
+    reads the gathered ones once (BF16: 2 + 2; FP8: 1 + 1, plus the chunk
+    exponents, 1 byte per 12 bands); Chebyshev reads the iterate before B (4)."""
+    per = {None: 0.0, "bf16": 20.0, "fp8": 18.0 + 2.0 / 12.0}[increments]
+    return int(S * nb * (per + (4 if chebyshev else 0)))
 
 
 def time_sweep(eng, nb: int, precision: int, reps: int, tables: bool = True) -> float:
@@ -333,7 +336,7 @@ def main() -> None:
     wl = WORKLOADS[a.workload]
     steps = a.steps if a.steps is not None else wl["steps"]
     from streamsim_amd import FacetEngine, faceteb, scenes
-    from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32, FH_WARM_PREVIOUS
+    from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP8_INC, FH_PRECISION_FP32, FH_WARM_PREVIOUS
 
     log(f"[bench] workload={a.workload} steps={steps} warmup={a.warmup}")
     t = time.perf_counter()
@@ -369,13 +372,15 @@ def main() -> None:
         F, nnz = scene.facet_count, diag.directedEdgeCount
         out = {"workload": a.workload, "sides": S, "facets": F, "nnz": nnz}
         for name, nbk, prec, tab in (("sw_fp32", nb, FH_PRECISION_FP32, True), ("sw_inc", nb, FH_PRECISION_BF16_DC, True),
+                                     ("sw_q8", nb, FH_PRECISION_FP8_INC, True),
                                      ("sw_fp32_arrays", nb, FH_PRECISION_FP32, False),
                                      ("sw_inc_arrays", nb, FH_PRECISION_BF16_DC, False),
                                      ("lw_fp32", 1, FH_PRECISION_FP32, False)):
             sec = time_sweep(eng, nbk, prec, a.roofline_reps, tables=tab)
             # single-band sweeps read the packed entries
             alg = sweep_bytes(S, F, nnz, nbk, tables=tab and nbk >= 16, entry=8 if nbk >= 16 else 4)
-            extra = scheme_bytes(S, nbk, increments=prec != FH_PRECISION_FP32 and nbk >= 16, chebyshev=tab and nbk >= 16)
+            inc = None if prec == FH_PRECISION_FP32 or nbk < 16 else ("fp8" if prec == FH_PRECISION_FP8_INC else "bf16")
+            extra = scheme_bytes(S, nbk, increments=inc, chebyshev=tab and nbk >= 16)
             out[name] = {"us": sec * 1e6, "alg_bytes": alg, "scheme_bytes": extra, "GBps": alg / sec / 1e9,
                          "frac": alg / sec / 1e9 / HBM_PEAK_GBS}
         cs = eng.benchmarkCopy(1 << 30, 20)
@@ -398,17 +403,19 @@ def main() -> None:
         eng.synchronize()
         return time.perf_counter() - t0, st
 
+    fast_prec = FH_PRECISION_FP8_INC if a.fast_precision == "fp8" else FH_PRECISION_BF16_DC
     t_plain, st_plain = run(FH_PRECISION_FP32, FH_WARM_PREVIOUS)
     log(f"[bench] plain FP32: {t_plain:.3f}s {st_plain.summary()}")
-    t_fast, st_fast = run(FH_PRECISION_BF16_DC, fast_warm)
-    log(f"[bench] BF16{'+surrogate' if have_sur else ''}: {t_fast:.3f}s {st_fast.summary()}")
+    t_fast, st_fast = run(fast_prec, fast_warm)
+    log(f"[bench] {a.fast_precision.upper()}{'+surrogate' if have_sur else ''}: {t_fast:.3f}s {st_fast.summary()}")
     work = scene.facet_count * nb_total * steps
     value = work / t_fast
     plain_value = work / t_plain
-    # ablations: each of the two changes alone
+    # ablations: each change alone
     ablation = {}
-    t_b, st_b = run(FH_PRECISION_BF16_DC, FH_WARM_PREVIOUS)
-    ablation["bf16_only"] = {"value": work / t_b, "seconds": t_b, **st_b.summary()}
+    for name, prec in (("bf16_increments_only", FH_PRECISION_BF16_DC), ("fp8_increments_only", FH_PRECISION_FP8_INC)):
+        t_b, st_b = run(prec, FH_WARM_PREVIOUS)
+        ablation[name] = {"value": work / t_b, "seconds": t_b, **st_b.summary()}
     if have_sur:
         t_s, st_s = run(FH_PRECISION_FP32, 1)
         ablation["surrogate_only"] = {"value": work / t_s, "seconds": t_s, **st_s.summary()}
@@ -421,11 +428,11 @@ def main() -> None:
     nb = windows.count
     nnz = diag.directedEdgeCount
     F = scene.facet_count
-    sec16 = time_sweep(eng, nb, FH_PRECISION_BF16_DC, a.roofline_reps)
+    sec16 = time_sweep(eng, nb, fast_prec, a.roofline_reps)
     sec32 = time_sweep(eng, nb, FH_PRECISION_FP32, a.roofline_reps)
     alg = sweep_bytes(S, F, nnz, nb, tables=True)
-    extra16 = scheme_bytes(S, nb, increments=True, chebyshev=True)
-    extra32 = scheme_bytes(S, nb, increments=False, chebyshev=True)
+    extra16 = scheme_bytes(S, nb, increments=a.fast_precision, chebyshev=True)
+    extra32 = scheme_bytes(S, nb, increments=None, chebyshev=True)
     achieved = alg / sec16 / 1e9
     log(f"[bench] SW sweep nb={nb}: increments {sec16 * 1e6:.1f} us ({achieved:.0f} GB/s), "
         f"fp32 {sec32 * 1e6:.1f} us ({alg / sec32 / 1e9:.0f} GB/s)")
@@ -434,7 +441,7 @@ def main() -> None:
     if tf.exists():
         tj = json.loads(tf.read_text())
         traffic32 = tj.get("sw_fp32", {}).get("bytes_per_launch")
-        traffic16 = tj.get("sw_inc", {}).get("bytes_per_launch")
+        traffic16 = tj.get("sw_inc" if a.fast_precision == "bf16" else "sw_q8", {}).get("bytes_per_launch")
 
     cpu = cpu_metric = None
     if not a.no_cpu_baseline:
@@ -479,8 +486,8 @@ def main() -> None:
             "time_steps": steps,
             "tolerance_K": cfg.tolerance,
             "radiosity_rel_tol": cfg.radiosity_tolerance,
-            "fast_path": "shortwave: BF16 increments of B gathered, FP32 weights and accumulation (FP32 "
-                         "fixed point)" + (" + surrogate warm start" if have_sur else "") +
+            "fast_path": f"shortwave: {a.fast_precision.upper()} increments of B gathered, FP32 weights and "
+                         "accumulation (FP32 fixed point)" + (" + surrogate warm start" if have_sur else "") +
                          "; both paths: Chebyshev, per-band retirement, coupled facet pairs as 2-facet blocks",
             "parallelism": "single GPU",
             "scene": WORKLOAD_SCENES[a.workload],
@@ -496,16 +503,19 @@ def main() -> None:
                          "bvh_host_seconds": t_bvh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic16,
-                     "kernel": f"k_sweep_hv<2, false> (single facets) + k_sweep_hv<2, true> (coupled pairs, second "
-                               f"stream, side by side): increment sweep of the BF16 shortwave solves, {nb} bands, "
-                               f"material tables; bytes and time per sweep",
+                     "kernel": (f"k_sweep_hv<2, false> (single facets) + k_sweep_hv<2, true> (coupled pairs, second "
+                                f"stream, side by side): increment sweep of the BF16 shortwave solves"
+                                if a.fast_precision == "bf16" else
+                                "k_sweep_q8_single (single facets) + k_sweep_q8_pair (coupled pairs, second stream, side "
+                                "by side): increment sweep of the FP8 shortwave solves") +
+                               f", {nb} bands, material tables; bytes and time per sweep",
                      "bytes_per_launch": alg, "us_per_launch": sec16 * 1e6,
                      "formula": "4(S+1) + 8 nnz + 4S + 4F + 4S + 8 S nb (row offsets, column + FP32 weight, fsky, "
                                 "material per facet, direct factor per side, B read + write)",
                      "scheme_bytes_per_launch": extra16,
-                     "scheme_note": "increment scheme's accumulated gathers and carries (16 B), BF16 increments "
-                                    "written and gathered (2 + 2 B), Chebyshev's previous iterate (4 B) per side-band: "
-                                    "not algorithmic, not in achieved",
+                     "scheme_note": "increment scheme's accumulated gathers and carries (16 B), increments "
+                                    "written and gathered (BF16 2 + 2 B, FP8 1 + 1 B + exponents), Chebyshev's previous "
+                                    "iterate (4 B) per side-band: not algorithmic, not in achieved",
                      "fp32_sweep": {"kernel": f"k_sweep_v<0, false, false> + k_sweep_v<0, false, true>: FP32 "
                                               f"shortwave sweep of the plain path, {nb} bands, material tables",
                                     "achieved": alg / sec32 / 1e9, "frac": alg / sec32 / 1e9 / HBM_PEAK_GBS,

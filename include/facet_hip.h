@@ -92,8 +92,14 @@ typedef struct {
 } fh_solve_stats;
 
 typedef enum {
-    FH_PRECISION_FP32 = 0,   /* FP32 F, FP32 accumulation                     */
-    FH_PRECISION_BF16_DC = 1 /* BF16 F sweeps + FP32-F defect correction      */
+    FH_PRECISION_FP32 = 0,    /* FP32 F, FP32 accumulation                     */
+    FH_PRECISION_BF16_DC = 1, /* BF16 F sweeps + FP32-F defect correction; from
+                                 16 bands on: BF16 increments of B gathered   */
+    FH_PRECISION_FP8_INC = 2  /* from 16 bands on: FP8 increments of B
+                                 gathered (12-band chunks with a shared
+                                 power-of-two exponent), FP32 weights,
+                                 accumulated gathers and carries; below 16
+                                 bands FP32                                   */
 } fh_precision;
 
 const char *fh_status_string(int status);
@@ -177,8 +183,9 @@ uint32_t fh_surface_count(const fh_context *ctx);
  * sides in an FP32 sweep is <= tolRel (from 16 bands on, a band that meets
  * it stops iterating while the others continue), or maxIter sweeps.
  * FH_PRECISION_BF16_DC from 16 bands on gathers BF16 increments of the
- * radiosity with FP32 weights and accumulation, reaching the FP32 fixed
- * point; its stopping test uses max(tolRel, 3e-7).  From 16 bands on,
+ * radiosity with FP32 weights and accumulation, FH_PRECISION_FP8_INC FP8
+ * increments (half the gathered bytes); both reach the FP32 fixed point,
+ * and their stopping test uses max(tolRel, 3e-7).  From 16 bands on,
  * facets that are each other's strongest partner (at least 0.2 of a side's
  * row weight) are updated together with the entries between the two
  * solved exactly (block Jacobi sweep): same fixed point and stopping test
@@ -305,8 +312,8 @@ typedef struct {
     float radiosityTolerance;  /* relative change, SW and LW solves        */
     uint32_t maxSweeps;        /* per radiosity solve                      */
     uint32_t precision;        /* fh_precision of the SW solves from 33 windows on (BF16
-                                  increments; FP32 rows then span more than one 128-byte
-                                  line); up to 32 windows and for the LW: FP32 */
+                                  or FP8 increments; FP32 rows then span more than one
+                                  128-byte line); up to 32 windows and for the LW: FP32 */
     uint32_t warmStart;        /* fh_warm_start                            */
     uint32_t firstNode;        /* 1: T = airTemperatureK (core:3470-3479)  */
     uint32_t zeroStart;        /* 1: every solve starts from B = 0         */

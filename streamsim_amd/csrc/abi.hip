@@ -667,7 +667,7 @@ int fh_solve_bands(fh_context* c, uint32_t nb, const float* r, const float* t, c
         require_graph(c, "solveBands()");
         validate_bands(c, nb, r, t, e, D, sky);
         validate_relax(relaxation);
-        if (precision > FH_PRECISION_BF16_DC) fail(FH_INVALID_ARGUMENT, "unknown precision");
+        if (precision > FH_PRECISION_FP8_INC) fail(FH_INVALID_ARGUMENT, "unknown precision");
         if (!radiosity) fail(FH_INVALID_ARGUMENT, "radiosity must not be null");
         const size_t n = size_t(c->S) * nb;
         fh::DBuf<float> in;
@@ -699,7 +699,7 @@ int fh_solve_bands_device(fh_context* c, uint32_t nb, const float* dr, const flo
         if (!dr || !dt || !de || !dD || !sky || !dout) fail(FH_INVALID_ARGUMENT, "band arrays must not be null");
         for (uint32_t b = 0; b < nb; ++b) validate_sky(sky[b]);
         validate_relax(relaxation);
-        if (precision > FH_PRECISION_BF16_DC) fail(FH_INVALID_ARGUMENT, "unknown precision");
+        if (precision > FH_PRECISION_FP8_INC) fail(FH_INVALID_ARGUMENT, "unknown precision");
         solve_bands_impl(c, nb, dr, dt, de, dD, sky, dinit, maxIter ? maxIter : 1u, tolRel, relaxation, precision,
                          dout, stats);
     });
@@ -740,6 +740,7 @@ int fh_sweep_device(fh_context* c, uint32_t nb, const float* dc, const float* dr
         require_graph(c, "sweep()");
         if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
         if (!dc || !dr || !dt || !dBin || !dBout) fail(FH_INVALID_ARGUMENT, "arrays must not be null");
+        if (precision > FH_PRECISION_BF16_DC) fail(FH_INVALID_ARGUMENT, "a single sweep reads FP32 or BF16 weights");
         fh::launch_single_sweep(c, fh::band_stride(nb), dc, dr, dt, dBin, dBout, precision == FH_PRECISION_BF16_DC);
     });
 }
@@ -753,6 +754,7 @@ int fh_sweep_device_noalloc(fh_context* c, uint32_t nb, const float* dc, const f
         require_graph(c, "sweep()");
         if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
         if (!dc || !dr || !dt || !dBin || !dBout) fail(FH_INVALID_ARGUMENT, "arrays must not be null");
+        if (precision > FH_PRECISION_BF16_DC) fail(FH_INVALID_ARGUMENT, "a single sweep reads FP32 or BF16 weights");
         if (!c->fixedPartials) fail(FH_RUNTIME_ERROR, "fixed scratch is not set up");
         fh::launch_single_sweep(c, fh::band_stride(nb), dc, dr, dt, dBin, dBout, precision == FH_PRECISION_BF16_DC,
                                 nullptr, nullptr, nullptr, nullptr, false, c->fixedPartials, kFixedPartials);
@@ -814,16 +816,32 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
         fh::launch_fill_f32(c, tt, n, 0.1f);
         fh::launch_fill_f32(c, B0, n, 50.0f);
         fh::launch_fill_f32(c, B1, n, 0.0f);
-        const bool w16 = precision == FH_PRECISION_BF16_DC;
+        const bool q8 = precision == FH_PRECISION_FP8_INC && fh::uses_bf16_gather(ldb);
+        const bool w16 = precision == FH_PRECISION_BF16_DC || q8;
         // BF16 solves of band-parallel strides run increment sweeps: BF16
-        // increments gathered, accumulated gathers and carries in FP32
+        // increments gathered, accumulated gathers and carries in FP32; FP8
+        // solves the same with FP8 increments
         fh::DBuf<uint16_t> h16;
+        fh::DBuf<uint8_t> h8;
         fh::DBuf<float> acc;
         uint16_t* H0 = nullptr;
         uint16_t* H1 = nullptr;
+        uint8_t* Q0 = nullptr;
+        uint8_t* Q1 = nullptr;
         float* G = nullptr;
         float* K = nullptr;
-        if (w16 && fh::uses_bf16_gather(ldb)) {
+        if (q8) {
+            const size_t nq = size_t(c->S) * fh::q8_stride(ldb);
+            h8.ensure(2 * nq);
+            Q0 = h8.p;
+            Q1 = h8.p + nq;
+            fh::launch_fill_u32(c, reinterpret_cast<uint32_t*>(h8.p), nq / 2u, 0u);  // FP8 zeros, exponent 0
+            acc.ensure(2 * n);
+            G = acc.p;
+            K = acc.p + n;
+            fh::launch_fill_f32(c, G, n, 100.0f);
+            fh::launch_fill_f32(c, K, n, 0.0f);
+        } else if (w16 && fh::uses_bf16_gather(ldb)) {
             const size_t nh = size_t(c->S) * fh::bf16_stride(ldb);
             h16.ensure(2 * nh);
             H0 = h16.p;
@@ -836,14 +854,15 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
             fh::launch_fill_f32(c, K, n, 0.0f);
         }
         for (int i = 0; i < 3; ++i)
-            fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1, G, K, true, nullptr, 0u, tab);
+            fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1, G, K, true, nullptr, 0u, tab, Q0, Q1);
         hipEvent_t e0, e1;
         FH_HIP(hipEventCreate(&e0));
         FH_HIP(hipEventCreate(&e1));
         FH_HIP(hipEventRecord(e0, c->stream));
         for (uint32_t i = 0; i < reps; ++i)
             fh::launch_single_sweep(c, ldb, cc, rr, tt, (i & 1u) ? B1 : B0, (i & 1u) ? B0 : B1, w16,
-                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1, G, K, true, nullptr, 0u, tab);
+                                    (i & 1u) ? H1 : H0, (i & 1u) ? H0 : H1, G, K, true, nullptr, 0u, tab,
+                                    (i & 1u) ? Q1 : Q0, (i & 1u) ? Q0 : Q1);
         FH_HIP(hipEventRecord(e1, c->stream));
         FH_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;

@@ -236,6 +236,7 @@ struct fh_context {
     fh::DBuf<float2> partial;        // per-block residual partials
     fh::DBuf<uint32_t> partAct;      // per-block 1 + highest unconverged band
     fh::DBuf<uint16_t> X16a, X16b;   // BF16 copies of the radiosity (or its increments) gathered by BF16 sweeps
+    fh::DBuf<uint8_t> X8a, X8b;      // FP8 increments of the radiosity gathered by FP8 increment sweeps
     fh::DBuf<float> gacc, carry;     // increment sweeps: accumulated gathers, B minus the gathered state
     fh::DBuf<float> omega;           // Chebyshev weights per band
     fh::DBuf<uint32_t> bandMax;      // per-band maxima (float bits) of the rho estimate

@@ -97,6 +97,8 @@ bool uses_bf16_gather(uint32_t ldb);
 // increments at this band stride (FP32 rows longer than one 128-byte line)
 bool bf16_pays(uint32_t ldb);
 uint32_t bf16_stride(uint32_t ldb);
+// bytes per row of FP8 increments: 16-byte chunks of 12 bands
+uint32_t q8_stride(uint32_t ldb);
 
 // whether FP32-weight gathers read the packed CSR entries (when present)
 bool use_packed();
@@ -132,7 +134,8 @@ struct SweepTables {
 // One sweep, no host synchronisation (timing entry).  With w16 and BF16
 // copies given (band-parallel strides), the BF16-gather sweep h16in -> h16out;
 // with gacc and carry ([S][ldb]) also given, the increment sweep of BF16
-// solves (h16in / h16out hold BF16 increments).  pairs: the sweep of the
+// solves (h16in / h16out hold BF16 increments); with q8in / q8out (rows of
+// q8_stride bytes), gacc and carry instead, the FP8 increment sweep.  pairs: the sweep of the
 // solves (pair blocks where the graph has them); else one Jacobi sweep.
 // partials: device buffer of partialCap per-block residual partials the
 // sweep writes instead of the context's own (grown on demand) buffer; a
@@ -141,6 +144,6 @@ void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const flo
                          const float* Bin, float* Bout, bool w16, uint16_t* h16in = nullptr,
                          uint16_t* h16out = nullptr, float* gacc = nullptr, float* carry = nullptr,
                          bool pairs = false, float2* partials = nullptr, uint32_t partialCap = 0u,
-                         const SweepTables* tables = nullptr);
+                         const SweepTables* tables = nullptr, uint8_t* q8in = nullptr, uint8_t* q8out = nullptr);
 
 }  // namespace fh

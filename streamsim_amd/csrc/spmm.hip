@@ -119,6 +119,13 @@ struct SweepArgs {
     // the context's partial buffer, grown as needed
     float2* partialExt;
     uint32_t partialCap;
+    // FP8 increment sweeps (k_sweep_q8_single / _pair): the gathered increments as rows of
+    // ldq bytes (16-byte chunks of 12 bands and their exponent), ping-pong
+    // between q0 / q1 by ctrl->iter like buf0 / buf1
+    uint8_t* q0;
+    uint8_t* q1;
+    uint32_t ldq;
+    uint32_t q8;  // 1: increments in FP8 instead of BF16
     uint32_t partOff;     // band-parallel: first partial of this launch (a sweep may be two launches)
 };
 
@@ -320,6 +327,8 @@ struct BandCtx {
     float* Bout;
     const uint16_t* Bin16;
     uint16_t* Bout16;
+    const uint8_t* Bin8;
+    uint8_t* Bout8;
 };
 
 // control-block state of a band-parallel sweep; false: the sweep is skipped
@@ -332,6 +341,8 @@ __device__ __forceinline__ bool band_ctx(const SweepArgs& a, bool inner, BandCtx
     x.Bout = a.Bout;
     x.Bin16 = a.h0;  // standalone: h0 -> h1
     x.Bout16 = a.h1;
+    x.Bin8 = a.q0;
+    x.Bout8 = a.q1;
     if (MODE == MODE_INCIDENT || a.standalone) return true;
     if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return false;
     if (inner && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return false;
@@ -346,6 +357,8 @@ __device__ __forceinline__ bool band_ctx(const SweepArgs& a, bool inner, BandCtx
         x.Bout = (it & 1u) ? a.buf0 : a.buf1;
         x.Bin16 = a.h0 ? ((it & 1u) ? a.h1 : a.h0) : nullptr;
         x.Bout16 = a.h0 ? ((it & 1u) ? a.h0 : a.h1) : nullptr;
+        x.Bin8 = a.q0 ? ((it & 1u) ? a.q1 : a.q0) : nullptr;
+        x.Bout8 = a.q0 ? ((it & 1u) ? a.q0 : a.q1) : nullptr;
     }
     return true;
 }
@@ -506,7 +519,10 @@ __device__ __forceinline__ bool upd_begin(const SweepArgs& a, const BandCtx& x, 
 // (ping-pong), or y = B + relax (m - B); the residual maxima of the Jacobi
 // change |n - B| (relaxed: of relax (n - B)), the reference's criterion;
 // the highest band above tolRel; the stores (and the next increment).
-template <int INC>
+// KEEP_V (increment sweeps): the next increment before rounding, v = (B'
+// - B) + carry, is left in u.k0 / u.k1 for the caller to round and store
+// (FP8 increments round a chunk of bands spread over three lanes together).
+template <int INC, bool KEEP_V = false>
 __device__ __forceinline__ void upd_end(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0, Upd4& u,
                                         const float (&m0)[4], const float (&m1)[4], float& myAbs, float& myRel,
                                         float& myAct) {
@@ -556,7 +572,13 @@ __device__ __forceinline__ void upd_end(const SweepArgs& a, const BandCtx& x, ui
     }
     st4(x.Bout + i0, y0);
     st4(x.Bout + i1, y1);
-    if (INC) {
+    if (INC && KEEP_V) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            u.k0[j] = (y0[j] - u.o0[j]) + u.k0[j];
+            u.k1[j] = (y1[j] - u.o1[j]) + u.k1[j];
+        }
+    } else if (INC) {
         // next increment and the part of it that BF16 does not carry
         uint32_t q[4];
 #pragma unroll
@@ -697,10 +719,12 @@ __device__ __forceinline__ void pair_solve4(const Pair4& f, const Pair4& g, floa
 
 // stores of one facet of a pair: B and the carry read again, n = B + j
 // (exact where B and n are within a factor 2), m = n + c beyond 2^-21 |n|
-template <int INC>
+// (KEEP_V: the next increment before rounding into v, see upd_end)
+template <int INC, bool KEEP_V = false>
 __device__ __forceinline__ void pair_end(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0,
                                          const float (&j0)[4], const float (&j1)[4], const float (&c0)[4],
-                                         const float (&c1)[4], float& myAbs, float& myRel, float& myAct) {
+                                         const float (&c1)[4], float& myAbs, float& myRel, float& myAct,
+                                         float (&v)[8]) {
     Upd4 u;
     u.i0 = size_t(s0) * a.ldb + b0;
     u.i1 = u.i0 + a.ldb;
@@ -726,24 +750,42 @@ __device__ __forceinline__ void pair_end(const SweepArgs& a, const BandCtx& x, u
         m0[j] = fabsf(c0[j]) > kDrop * fabsf(u.n0[j]) ? u.n0[j] + c0[j] : u.n0[j];
         m1[j] = fabsf(c1[j]) > kDrop * fabsf(u.n1[j]) ? u.n1[j] + c1[j] : u.n1[j];
     }
-    upd_end<INC>(a, x, s0, b0, u, m0, m1, myAbs, myRel, myAct);
+    upd_end<INC, KEEP_V>(a, x, s0, b0, u, m0, m1, myAbs, myRel, myAct);
+    if (KEEP_V) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[j] = u.k0[j];
+            v[4 + j] = u.k1[j];
+        }
+    }
 }
 
 // Update of a unit's two facets (side pairs s0 and s1) at 4 bands; INC as
-// upd_begin.
+// upd_begin.  KEEP_V: vf / vg receive each facet's next increments before
+// rounding (sides 0 and 1, 4 bands each); returns whether the bands were
+// updated (false: beyond the active bands).
+template <int INC, bool KEEP_V = false>
+__device__ __forceinline__ bool pair_update4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t s1,
+                                             uint32_t b0, const float4 Wf, const float4 Wg, float (&gf0)[4],
+                                             float (&gf1)[4], float (&gg0)[4], float (&gg1)[4], float& myAbs,
+                                             float& myRel, float& myAct, float (&vf)[8], float (&vg)[8]) {
+    Pair4 pf, pg;
+    const bool okf = pair_begin<INC>(a, x, s0, b0, gf0, gf1, Wf, pf);
+    const bool okg = pair_begin<INC>(a, x, s1, b0, gg0, gg1, Wg, pg);  // same bands: same outcome as f
+    if (!okf || !okg) return false;
+    float cf0[4], cf1[4], cg0[4], cg1[4];
+    pair_solve4(pf, pg, cf0, cf1, cg0, cg1);
+    pair_end<INC, KEEP_V>(a, x, s0, b0, pf.j0, pf.j1, cf0, cf1, myAbs, myRel, myAct, vf);
+    pair_end<INC, KEEP_V>(a, x, s1, b0, pg.j0, pg.j1, cg0, cg1, myAbs, myRel, myAct, vg);
+    return true;
+}
 template <int INC>
 __device__ __forceinline__ void pair_update4(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t s1,
                                              uint32_t b0, const float4 Wf, const float4 Wg, float (&gf0)[4],
                                              float (&gf1)[4], float (&gg0)[4], float (&gg1)[4], float& myAbs,
                                              float& myRel, float& myAct) {
-    Pair4 pf, pg;
-    const bool okf = pair_begin<INC>(a, x, s0, b0, gf0, gf1, Wf, pf);
-    const bool okg = pair_begin<INC>(a, x, s1, b0, gg0, gg1, Wg, pg);  // same bands: same outcome as f
-    if (!okf || !okg) return;
-    float cf0[4], cf1[4], cg0[4], cg1[4];
-    pair_solve4(pf, pg, cf0, cf1, cg0, cg1);
-    pair_end<INC>(a, x, s0, b0, pf.j0, pf.j1, cf0, cf1, myAbs, myRel, myAct);
-    pair_end<INC>(a, x, s1, b0, pg.j0, pg.j1, cg0, cg1, myAbs, myRel, myAct);
+    float vf[8], vg[8];
+    (void)pair_update4<INC, false>(a, x, s0, s1, b0, Wf, Wg, gf0, gf1, gg0, gg1, myAbs, myRel, myAct, vf, vg);
 }
 
 __device__ __forceinline__ void band_partials(const SweepArgs& a, float myAbs, float myRel, float myAct) {
@@ -790,6 +832,38 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, ui
 }
 constexpr uint32_t kNoRow = 0xfffffff0u;  // byte offset past any gathered array: loads zero
 
+// FP8 increments (k_sweep_q8_single / _pair): a row of ldq bytes holds 16-byte chunks of 12
+// bands, bytes 0..11 the bands' values in FP8 (E4M3) and byte 12 the
+// chunk's power-of-two exponent e (int8): value = fp8 * 2^e.  Each chunk
+// is rounded with its own exponent, chosen so that its largest magnitude
+// becomes [64, 128) (inside E4M3's range whatever its bias): values more
+// than 2^15 below it round to zero, which the carry keeps for later.
+constexpr uint32_t kQ8Bands = 12;
+
+__device__ __forceinline__ float pow2f(int e) { return __uint_as_float(uint32_t(e + 127) << 23); }  // e in [-126, 127]
+
+// the 4 FP8 values of a 32-bit word
+__device__ __forceinline__ void fp8x4(int w, float (&o)[4]) {
+    const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8(w, false);
+    const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8(w, true);
+    o[0] = lo[0];
+    o[1] = lo[1];
+    o[2] = hi[0];
+    o[3] = hi[1];
+}
+
+// 4 values (already scaled) rounded to FP8, packed in a 32-bit word
+__device__ __forceinline__ int to_fp8x4(float a, float b, float c, float d) {
+    const int w = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    return __builtin_amdgcn_cvt_pk_fp8_f32(c, d, w, true);
+}
+
+__device__ __forceinline__ int chunk_exponent(float m) {
+    if (!(m > 0.0f)) return 0;
+    const int k = max(int((__float_as_uint(m) >> 23) & 0xffu) - 127, -120);  // floor(log2 m), tiny m floored
+    return k - 6;
+}
+
 // Gathers of both rows of a facet with the CSR loads software-pipelined.
 // The aligned groups of 8 entries of row 0 ([rb[0], rb[1])) and then of
 // row 1 ([rb[1], rb[2])) form one sequence; the columns and weights of the
@@ -799,18 +873,22 @@ constexpr uint32_t kNoRow = 0xfffffff0u;  // byte offset past any gathered array
 // handed to side 0 where row 1 begins.  Each gather is a 16-byte buffer
 // load at byte offset col * rowBytes + laneBytes of `rows`; entries outside
 // the row load at kNoRow, which returns zero without a memory access.  The
-// 16 bytes are 4 FP32 bands (BF16 false) or 8 BF16 bands.
-template <bool BF16>
+// 16 bytes are 4 FP32 bands (KIND 0), 8 BF16 bands (KIND 1) or an FP8 chunk
+// (KIND 2) of which the lane takes word `wsel` (4 bands) and the exponent.
+enum { GATHER_F32 = 0, GATHER_BF16 = 1, GATHER_FP8 = 2, GATHER_FP8_CHUNK = 3 };
+
+// (GATHER_FP8_CHUNK: the lane takes the whole FP8 chunk, 12 bands)
+template <int KIND>
 struct RowSums {
-    static constexpr int NB = BF16 ? 8 : 4;
+    static constexpr int NB = KIND == GATHER_BF16 ? 8 : (KIND == GATHER_FP8_CHUNK ? 12 : 4);
     float s0[NB], s1[NB];
 };
 
-template <bool BF16>
-__device__ __forceinline__ RowSums<BF16> gather_rows_pipe(const SweepArgs& a, const uint32_t (&rb)[3],
+template <int KIND>
+__device__ __forceinline__ RowSums<KIND> gather_rows_pipe(const SweepArgs& a, const uint32_t (&rb)[3],
                                                           __amdgpu_buffer_rsrc_t rows, uint32_t rowBytes,
-                                                          uint32_t laneBytes) {
-    constexpr int NB = RowSums<BF16>::NB;
+                                                          uint32_t laneBytes, uint32_t wsel = 0u) {
+    constexpr int NB = RowSums<KIND>::NB;
     const uint32_t lo0 = rb[0], lo1 = rb[1], hi1 = rb[2];
     const uint32_t base0 = lo0 & ~uint32_t(kU - 1), base1 = lo1 & ~uint32_t(kU - 1);
     const uint32_t n0 = lo1 > lo0 ? (lo1 - base0 + uint32_t(kU - 1)) / uint32_t(kU) : 0u;
@@ -856,19 +934,35 @@ __device__ __forceinline__ RowSums<BF16> gather_rows_pipe(const SweepArgs& a, co
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             const uint32_t vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-            if constexpr (BF16) {
+            if constexpr (KIND == GATHER_BF16) {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     acc[2 * j] += w[u] * __uint_as_float(vv[j] << 16);
                     acc[2 * j + 1] += w[u] * __uint_as_float(vv[j] & 0xffff0000u);
                 }
+            } else if constexpr (KIND == GATHER_FP8_CHUNK) {
+                const float ws = w[u] * pow2f(int(int8_t(vv[3] & 0xffu)));
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    float q[4];
+                    fp8x4(int(vv[t]), q);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[4 * t + j] += ws * q[j];
+                }
+            } else if constexpr (KIND == GATHER_FP8) {
+                // the chunk's exponent folded into the weight (exact: a power of two)
+                const float ws = w[u] * pow2f(int(int8_t(vv[3] & 0xffu)));
+                float q[4];
+                fp8x4(int(wsel == 0u ? vv[0] : (wsel == 1u ? vv[1] : vv[2])), q);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[j] += ws * q[j];
             } else {
 #pragma unroll
                 for (int j = 0; j < 4; ++j) acc[j] += w[u] * __uint_as_float(vv[j]);
             }
         }
     }
-    RowSums<BF16> out;
+    RowSums<KIND> out;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         out.s0[j] = n1 ? r0[j] : acc[j];
@@ -905,8 +999,8 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
                 float g[2][4] = {};
                 if (b0 < x.nAct) {
                     if constexpr (!W16) {
-                        const RowSums<false> rs =
-                            gather_rows_pipe<false>(a, rb, make_rsrc(x.Bin, a.sides * ldb * 4u), ldb * 4u, b0 * 4u);
+                        const RowSums<GATHER_F32> rs =
+                            gather_rows_pipe<GATHER_F32>(a, rb, make_rsrc(x.Bin, a.sides * ldb * 4u), ldb * 4u, b0 * 4u);
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             g[0][j] = rs.s0[j];
@@ -928,8 +1022,8 @@ __global__ __launch_bounds__(256) void k_sweep_v(SweepArgs a) {
                 if (b0 < x.nAct) {
                     if constexpr (!W16) {
                         const __amdgpu_buffer_rsrc_t rows = make_rsrc(x.Bin, a.sides * ldb * 4u);
-                        const RowSums<false> rf = gather_rows_pipe<false>(a, rb, rows, ldb * 4u, b0 * 4u);
-                        const RowSums<false> rg = gather_rows_pipe<false>(a, rc, rows, ldb * 4u, b0 * 4u);
+                        const RowSums<GATHER_F32> rf = gather_rows_pipe<GATHER_F32>(a, rb, rows, ldb * 4u, b0 * 4u);
+                        const RowSums<GATHER_F32> rg = gather_rows_pipe<GATHER_F32>(a, rc, rows, ldb * 4u, b0 * 4u);
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             gf[0][j] = rf.s0[j];
@@ -1050,8 +1144,8 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
                 float g[2][8] = {};
                 if (b0 < x.nAct) {
                     if constexpr (INC != 0) {
-                        const RowSums<true> rs =
-                            gather_rows_pipe<true>(a, rb, make_rsrc(x.Bin16, a.sides * ldh * 2u), ldh * 2u, b0 * 2u);
+                        const RowSums<GATHER_BF16> rs =
+                            gather_rows_pipe<GATHER_BF16>(a, rb, make_rsrc(x.Bin16, a.sides * ldh * 2u), ldh * 2u, b0 * 2u);
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
                             g[0][j] = rs.s0[j];
@@ -1095,6 +1189,255 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
         }
     }
     band_partials(a, myAbs, myRel, INC == 0 ? 0.0f : myAct);
+}
+
+// FP8 increment sweep of the coupled pairs.  A team of bt lanes (a
+// multiple of 3, teams never straddle a wave) owns the two facets of a
+// unit; a lane owns 4 bands, so the three lanes of one 12-band chunk round
+// it together:
+// each lane's largest |v| of its side is max-reduced over the three by
+// shuffles, the chunk exponent follows, and every lane rounds its 4 values,
+// keeps what rounding drops in the carry and stores its 4 bytes (the first
+// lane also the exponent).  The update itself is the increment sweep's pair
+// block: FP32 weights, accumulated gathers and carries, FP32 Jacobi /
+// Chebyshev update, per-band retirement.
+//
+// emit: this lane has new increments v (sides 0 and 1 of facet s0, 4 bands
+// from b0; every lane of the wave calls this, emit or not, for the shuffles)
+__device__ __forceinline__ void emit_q8(const SweepArgs& a, const BandCtx& x, uint32_t s0, uint32_t b0, uint32_t lane,
+                                        uint32_t bt, bool emit, const float (&v)[8]) {
+    float m0 = 0.0f, m1 = 0.0f;
+    if (emit) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            m0 = fmaxf(m0, fabsf(v[j]));
+            m1 = fmaxf(m1, fabsf(v[4 + j]));
+        }
+    }
+    // the chunk's three lanes: lane - q .. lane - q + 2 of this team
+    const uint32_t q = lane % 3u;
+    const int wl = int(threadIdx.x & 63u);
+    float c0 = m0, c1 = m1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int src = wl - int(q) + k;
+        const bool inTeam = lane - q + uint32_t(k) < bt;
+        const float o0 = __shfl(m0, src & 63, 64), o1 = __shfl(m1, src & 63, 64);
+        if (inTeam) {
+            c0 = fmaxf(c0, o0);
+            c1 = fmaxf(c1, o1);
+        }
+    }
+    if (!emit) return;
+    const int e0 = chunk_exponent(c0), e1 = chunk_exponent(c1);
+    const float d0 = pow2f(-e0), d1 = pow2f(-e1), u0 = pow2f(e0), u1 = pow2f(e1);
+    const int w0 = to_fp8x4(v[0] * d0, v[1] * d0, v[2] * d0, v[3] * d0);
+    const int w1 = to_fp8x4(v[4] * d1, v[5] * d1, v[6] * d1, v[7] * d1);
+    float q0[4], q1[4], k0[4], k1[4];
+    fp8x4(w0, q0);
+    fp8x4(w1, q1);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        k0[j] = v[j] - q0[j] * u0;
+        k1[j] = v[4 + j] - q1[j] * u1;
+    }
+    const size_t i0 = size_t(s0) * a.ldb + b0;
+    st4(a.carry + i0, k0);
+    st4(a.carry + i0 + a.ldb, k1);
+    const uint32_t chunk = (b0 / kQ8Bands) * 16u, byte = b0 % kQ8Bands;
+    uint8_t* r0 = x.Bout8 + size_t(s0) * a.ldq + chunk;
+    uint8_t* r1 = r0 + a.ldq;
+    *reinterpret_cast<uint32_t*>(r0 + byte) = uint32_t(w0);
+    *reinterpret_cast<uint32_t*>(r1 + byte) = uint32_t(w1);
+    if (q == 0u) {
+        *reinterpret_cast<uint32_t*>(r0 + 12) = uint32_t(e0) & 0xffu;
+        *reinterpret_cast<uint32_t*>(r1 + 12) = uint32_t(e1) & 0xffu;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sweep_q8_pair(SweepArgs a) {
+    BandCtx x;
+    if (!band_ctx<MODE_SWEEP>(a, false, x)) return;
+    const uint32_t bt = a.bt, tpw = 64u / bt;
+    const uint32_t wl = threadIdx.x & 63u, tiw = wl / bt;
+    const uint32_t lane = wl - tiw * bt;
+    const uint32_t fl = xcd_block(a.xcd) * (4u * tpw) + (threadIdx.x >> 6) * tpw + tiw;
+    const bool live = tiw < tpw && fl < a.nUnits;
+    float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
+    uint32_t s0 = 0u, s1 = 0u;
+    uint32_t rb[3] = {0u, 0u, 0u}, rc[3] = {0u, 0u, 0u};
+    float4 Wf = make_float4(0.0f, 0.0f, 0.0f, 0.0f), Wg = Wf;
+    if (live) {
+        const uint2 un = a.units[fl];
+        s0 = 2u * un.x;
+        s1 = 2u * un.y;
+        rc[0] = a.rowptr[s1];
+        rc[1] = a.rowptr[s1 + 1];
+        rc[2] = a.rowptr[s1 + 2];
+        Wf = a.unitW[2u * fl];
+        Wg = a.unitW[2u * fl + 1u];
+        rb[0] = a.rowptr[s0];
+        rb[1] = a.rowptr[s0 + 1];
+        rb[2] = a.rowptr[s0 + 2];
+    }
+    const __amdgpu_buffer_rsrc_t rows = make_rsrc(x.Bin8, a.sides * a.ldq);
+    // passes of 4 bt bands, a multiple of 12: chunks never straddle a pass
+    for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
+        const uint32_t b0 = cb + 4u * lane;
+        const uint32_t laneBytes = (b0 / kQ8Bands) * 16u, wsel = (b0 % kQ8Bands) / 4u;
+        float vf[8] = {}, vg[8] = {};
+        bool emit = false;
+        if (live && b0 < x.nPrev) {
+            float gf[2][4] = {}, gg[2][4] = {};
+            if (b0 < x.nAct) {
+                const RowSums<GATHER_FP8> rf = gather_rows_pipe<GATHER_FP8>(a, rb, rows, a.ldq, laneBytes, wsel);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    gf[0][j] = rf.s0[j];
+                    gf[1][j] = rf.s1[j];
+                }
+                const RowSums<GATHER_FP8> rg = gather_rows_pipe<GATHER_FP8>(a, rc, rows, a.ldq, laneBytes, wsel);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    gg[0][j] = rg.s0[j];
+                    gg[1][j] = rg.s1[j];
+                }
+            }
+            emit = pair_update4<2, true>(a, x, s0, s1, b0, Wf, Wg, gf[0], gf[1], gg[0], gg[1], myAbs, myRel, myAct,
+                                         vf, vg);
+        }
+        emit_q8(a, x, s0, b0, lane, bt, emit, vf);
+        emit_q8(a, x, s1, b0, lane, bt, emit, vg);
+    }
+    band_partials(a, myAbs, myRel, myAct);
+}
+
+// FP8 increment sweep of the single facets: a team of bt lanes per facet,
+// each lane owns a whole 12-band chunk (one 16-byte gather per CSR entry),
+// updates its bands 4 at a time and rounds the chunk with one exponent per
+// side.  Fully retired 4-band groups keep their carry and accumulated
+// gathers; their bytes are written as zeros, which nobody gathers (a
+// group's gathered bands are only used while it is active).
+__global__ __launch_bounds__(256) void k_sweep_q8_single(SweepArgs a) {
+    BandCtx x;
+    if (!band_ctx<MODE_SWEEP>(a, false, x)) return;
+    const uint32_t bt = a.bt, tpb = 256u / bt;
+    const uint32_t team = threadIdx.x / bt, lane = threadIdx.x - team * bt;
+    const uint32_t fl = xcd_block(a.xcd) * tpb + team;
+    float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
+    if (team < tpb && fl < a.F) {
+        const uint32_t s0 = 2u * (a.sched ? a.sched[fl] : fl);
+        const uint32_t rb[3] = {a.rowptr[s0], a.rowptr[s0 + 1], a.rowptr[s0 + 2]};
+        const __amdgpu_buffer_rsrc_t rows = make_rsrc(x.Bin8, a.sides * a.ldq);
+        for (uint32_t cb = 0; cb < x.nPrev; cb += kQ8Bands * bt) {
+            const uint32_t b0 = cb + kQ8Bands * lane;
+            if (b0 >= x.nPrev) continue;
+            float g0[12] = {}, g1[12] = {};
+            if (b0 < x.nAct) {
+                const RowSums<GATHER_FP8_CHUNK> rs =
+                    gather_rows_pipe<GATHER_FP8_CHUNK>(a, rb, rows, a.ldq, (b0 / kQ8Bands) * 16u);
+#pragma unroll
+                for (int j = 0; j < 12; ++j) {
+                    g0[j] = rs.s0[j];
+                    g1[j] = rs.s1[j];
+                }
+            }
+            float v0[12] = {}, v1[12] = {};
+            bool em[3] = {false, false, false};
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                const uint32_t b = b0 + 4u * uint32_t(t);
+                if (b >= a.ldb) continue;
+                float h0[4], h1[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    h0[j] = g0[4 * t + j];
+                    h1[j] = g1[4 * t + j];
+                }
+                Upd4 u;
+                if (upd_begin<2>(a, x, s0, b, h0, h1, u)) {
+                    upd_end<2, true>(a, x, s0, b, u, u.n0, u.n1, myAbs, myRel, myAct);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        v0[4 * t + j] = u.k0[j];
+                        v1[4 * t + j] = u.k1[j];
+                    }
+                    em[t] = true;
+                }
+            }
+            if (!(em[0] || em[1] || em[2])) continue;
+            float m0 = 0.0f, m1 = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 12; ++j) {
+                m0 = fmaxf(m0, fabsf(v0[j]));
+                m1 = fmaxf(m1, fabsf(v1[j]));
+            }
+            const int e0 = chunk_exponent(m0), e1 = chunk_exponent(m1);
+            const float d0 = pow2f(-e0), d1 = pow2f(-e1), u0 = pow2f(e0), u1 = pow2f(e1);
+            uint32_t w0[4] = {0u, 0u, 0u, uint32_t(e0) & 0xffu}, w1[4] = {0u, 0u, 0u, uint32_t(e1) & 0xffu};
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                if (!em[t]) continue;
+                const int q0 = to_fp8x4(v0[4 * t] * d0, v0[4 * t + 1] * d0, v0[4 * t + 2] * d0, v0[4 * t + 3] * d0);
+                const int q1 = to_fp8x4(v1[4 * t] * d1, v1[4 * t + 1] * d1, v1[4 * t + 2] * d1, v1[4 * t + 3] * d1);
+                w0[t] = uint32_t(q0);
+                w1[t] = uint32_t(q1);
+                float f0[4], f1[4], k0[4], k1[4];
+                fp8x4(q0, f0);
+                fp8x4(q1, f1);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    k0[j] = v0[4 * t + j] - f0[j] * u0;
+                    k1[j] = v1[4 * t + j] - f1[j] * u1;
+                }
+                const size_t i0 = size_t(s0) * a.ldb + b0 + 4u * uint32_t(t);
+                st4(a.carry + i0, k0);
+                st4(a.carry + i0 + a.ldb, k1);
+            }
+            uint8_t* r0 = x.Bout8 + size_t(s0) * a.ldq + (b0 / kQ8Bands) * 16u;
+            *reinterpret_cast<uint4*>(r0) = make_uint4(w0[0], w0[1], w0[2], w0[3]);
+            *reinterpret_cast<uint4*>(r0 + a.ldq) = make_uint4(w1[0], w1[1], w1[2], w1[3]);
+        }
+    }
+    band_partials(a, myAbs, myRel, myAct);
+}
+
+// Start of an FP8 increment solve: Q_0 = the FP8 rounding of B_0 (rows of
+// ldq bytes), carry = B_0 - Q_0, accumulated gathers 0; one thread per
+// 12-band chunk of a side.
+__global__ void k_to_q8(const float* __restrict__ B, uint32_t S, uint32_t ldb, uint32_t ldq, uint8_t* __restrict__ Q,
+                        float* __restrict__ carry, float* __restrict__ gacc) {
+    const uint32_t nq = ldq / 16u;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= size_t(S) * nq) return;
+    const uint32_t s = uint32_t(i / nq), j = uint32_t(i - size_t(s) * nq);
+    const uint32_t b0 = j * kQ8Bands;
+    float v[kQ8Bands];
+    float m = 0.0f;
+#pragma unroll
+    for (uint32_t k = 0; k < kQ8Bands; ++k) {
+        v[k] = b0 + k < ldb ? B[size_t(s) * ldb + b0 + k] : 0.0f;
+        m = fmaxf(m, fabsf(v[k]));
+    }
+    const int e = chunk_exponent(m);
+    const float d = pow2f(-e), u = pow2f(e);
+    uint32_t w[4] = {0u, 0u, 0u, uint32_t(e) & 0xffu};
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int word = to_fp8x4(v[4 * t] * d, v[4 * t + 1] * d, v[4 * t + 2] * d, v[4 * t + 3] * d);
+        w[t] = uint32_t(word);
+        float qv[4];
+        fp8x4(word, qv);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t b = b0 + uint32_t(4 * t + k);
+            if (b < ldb) {
+                carry[size_t(s) * ldb + b] = v[4 * t + k] - qv[k] * u;
+                gacc[size_t(s) * ldb + b] = 0.0f;
+            }
+        }
+    }
+    *reinterpret_cast<uint4*>(Q + size_t(s) * ldq + size_t(j) * 16u) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
 // H = max(0, fsky S + G + D) from the accumulated gathers G of an increment
@@ -1349,8 +1692,11 @@ static bool use_increments() {
 }
 
 bool increment_solve(uint32_t ldb, uint32_t precision) {
-    return precision == FH_PRECISION_BF16_DC && band_parallel(ldb) && use_increments();
+    return band_parallel(ldb) &&
+           ((precision == FH_PRECISION_BF16_DC && use_increments()) || precision == FH_PRECISION_FP8_INC);
 }
+
+uint32_t q8_stride(uint32_t ldb) { return (ldb + kQ8Bands - 1u) / kQ8Bands * 16u; }
 
 const uint32_t* sweep_schedule(const fh_context* c) { return use_sched() && c->sched.p ? c->sched.p : nullptr; }
 
@@ -1536,8 +1882,14 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         // lanes per team: one 16-byte vector of the row each, at most 64
         // bands per team pass
         const bool half = (mode == MODE_SWEEP && w16 && a.h0) || a.inc;
-        const uint32_t vw = half ? 8u : 4u;
+        const bool q8 = a.inc && a.q8;
+        const uint32_t vw = half && !q8 ? 8u : 4u;
         a.bt = std::min((a.ldb + vw - 1u) / vw, 64u / vw);
+        // FP8 increment sweeps: the single facets one 12-band chunk per lane;
+        // the pairs 4 bands per lane, a multiple of 3 lanes (the lanes of one
+        // chunk), teams inside a wave
+        if (q8) a.bt = std::min((a.ldb + kQ8Bands - 1u) / kQ8Bands, 16u);
+        const uint32_t perBlock = 256u / a.bt;
         a.F = c->F;
         a.rowptr = c->rowptr.p;
         a.col = c->col.p;
@@ -1552,16 +1904,17 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         const bool pair =
             c->unitCount && use_pairs() && !a.jacobiOnly && (a.inc || (mode == MODE_SWEEP && !w16 && !half));
         SweepArgs p = a;
-        uint32_t g = std::max(1u, (c->F + 256u / a.bt - 1u) / (256u / a.bt)), gp = 0u;
+        uint32_t g = std::max(1u, (c->F + perBlock - 1u) / perBlock), gp = 0u;
         if (pair) {
             a.sched = c->singles.p;
             a.F = c->singleCount;
-            g = (a.F + 256u / a.bt - 1u) / (256u / a.bt);
-            p.bt = std::min((p.ldb + 3u) / 4u, 16u);
+            g = (a.F + perBlock - 1u) / perBlock;
+            p.bt = q8 ? std::min((p.ldb + 3u) / 4u + 2u, 15u) / 3u * 3u : std::min((p.ldb + 3u) / 4u, 16u);
             p.units = c->units.p;
             p.unitW = c->unitW.p;
             p.nUnits = c->unitCount;
-            gp = (p.nUnits + 256u / p.bt - 1u) / (256u / p.bt);
+            const uint32_t unitsPerBlock = q8 ? 4u * (64u / p.bt) : 256u / p.bt;
+            gp = (p.nUnits + unitsPerBlock - 1u) / unitsPerBlock;
             p.partOff = g;
         }
         const uint32_t parts = std::max(1u, g + gp);
@@ -1585,13 +1938,15 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
             hipStream_t ps = c->pairStream;
             FH_HIP(hipEventRecord(c->pairFork, st));
             FH_HIP(hipStreamWaitEvent(ps, c->pairFork, 0));
-            if (p.inc == 1u) hipLaunchKernelGGL((k_sweep_hv<1, true>), dim3(gp), dim3(256), 0, ps, p);
+            if (q8) hipLaunchKernelGGL(k_sweep_q8_pair, dim3(gp), dim3(256), 0, ps, p);
+            else if (p.inc == 1u) hipLaunchKernelGGL((k_sweep_hv<1, true>), dim3(gp), dim3(256), 0, ps, p);
             else if (p.inc) hipLaunchKernelGGL((k_sweep_hv<2, true>), dim3(gp), dim3(256), 0, ps, p);
             else hipLaunchKernelGGL((k_sweep_v<MODE_SWEEP, false, true>), dim3(gp), dim3(256), 0, ps, p);
             FH_HIP(hipEventRecord(c->pairJoin, ps));
         }
         if (g) {
-            if (a.inc == 1u) hipLaunchKernelGGL((k_sweep_hv<1, false>), dim3(g), dim3(256), 0, st, a);
+            if (q8) hipLaunchKernelGGL(k_sweep_q8_single, dim3(g), dim3(256), 0, st, a);
+            else if (a.inc == 1u) hipLaunchKernelGGL((k_sweep_hv<1, false>), dim3(g), dim3(256), 0, st, a);
             else if (a.inc) hipLaunchKernelGGL((k_sweep_hv<2, false>), dim3(g), dim3(256), 0, st, a);
             else if (half) hipLaunchKernelGGL((k_sweep_hv<0, false>), dim3(g), dim3(256), 0, st, a);
             else if (mode == MODE_SWEEP && !w16)
@@ -1748,8 +2103,10 @@ static float env_float(const char* name, float dflt) {
 
 int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const float* rr, const float* tt,
                float* bufs[2], fh_solve_stats* stats, bool* incidentDone) {
-    const bool bf16 = o.precision == FH_PRECISION_BF16_DC;
     const bool bp = band_parallel(o.ldb);
+    // FP8 increments on band-parallel strides (below them the solve is FP32)
+    const bool q8 = o.precision == FH_PRECISION_FP8_INC && bp;
+    const bool bf16 = o.precision == FH_PRECISION_BF16_DC || q8;
     const bool retire = o.retire && bp && o.tolRel >= 0.0f;
     const bool half = bf16 && bp;
     // FH_DC_FIRST overrides the end of the BF16 phase (tuning knob)
@@ -1758,7 +2115,7 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     // their last-ulp changes do not settle: such solves resolve relative
     // changes down to kIncrementTolFloor (2.5 FP32 ulps)
     constexpr float kIncrementTolFloor = 3.0e-7f;
-    const bool incr = half && use_increments();
+    const bool incr = half && (q8 || use_increments());
     const float tolRel = incr && o.tolRel >= 0.0f ? std::max(o.tolRel, kIncrementTolFloor) : o.tolRel;
     launch_ctrl_reset(c, o.maxIter ? o.maxIter : 1u, tolRel, o.tolAbs, o.checkEvery, o.ldb,
                       half ? std::max(firstInner, o.tolRel) : o.tolRel);
@@ -1803,28 +2160,47 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     uint32_t batch = 4;
     if (bf16 && !half) c->c2.ensure(size_t(c->S) * o.ldb);
     if (incr) {
-        // every sweep gathers BF16: the base sweep Q_0 = bf16(B_0), then the
-        // BF16 increments of B with their rounding carried to the next one
-        // (band_update_inc4); FP32 weights and accumulation, FP32 fixed point
-        const uint32_t ldh = bf16_stride(o.ldb);
-        const size_t nh = size_t(c->S) * ldh, nf = size_t(c->S) * o.ldb;
-        c->X16a.ensure(nh);
-        c->X16b.ensure(nh);
+        // every sweep gathers low-precision increments: BF16 -- the base
+        // sweep Q_0 = bf16(B_0), then the BF16 increments of B with their
+        // rounding carried to the next one (band_update_inc4) -- or FP8 --
+        // Q_0 = the FP8 rounding of B_0 with its carry and zero accumulated
+        // gathers set up first, then FP8 increment sweeps (k_sweep_q8_*); FP32
+        // weights and accumulation, FP32 fixed point
+        const size_t nf = size_t(c->S) * o.ldb;
         c->gacc.ensure(nf);
         c->carry.ensure(nf);
-        hipLaunchKernelGGL(k_to_bf16, dim3(uint32_t((nh + 255) / 256)), dim3(256), 0, c->stream, bufs[0], c->S, o.ldb,
-                           ldh, c->X16a.p);
-        FH_HIP(hipGetLastError());
-        a.h0 = c->X16a.p;
-        a.h1 = c->X16b.p;
-        a.ldh = ldh;
+        if (q8) {
+            const uint32_t ldq = q8_stride(o.ldb);
+            const size_t nq = size_t(c->S) * ldq;
+            c->X8a.ensure(nq);
+            c->X8b.ensure(nq);
+            const size_t chunks = size_t(c->S) * (ldq / 16u);
+            hipLaunchKernelGGL(k_to_q8, dim3(uint32_t((chunks + 255) / 256)), dim3(256), 0, c->stream, bufs[0], c->S,
+                               o.ldb, ldq, c->X8a.p, c->carry.p, c->gacc.p);
+            FH_HIP(hipGetLastError());
+            a.q0 = c->X8a.p;
+            a.q1 = c->X8b.p;
+            a.ldq = ldq;
+            a.q8 = 1u;
+        } else {
+            const uint32_t ldh = bf16_stride(o.ldb);
+            const size_t nh = size_t(c->S) * ldh;
+            c->X16a.ensure(nh);
+            c->X16b.ensure(nh);
+            hipLaunchKernelGGL(k_to_bf16, dim3(uint32_t((nh + 255) / 256)), dim3(256), 0, c->stream, bufs[0], c->S,
+                               o.ldb, ldh, c->X16a.p);
+            FH_HIP(hipGetLastError());
+            a.h0 = c->X16a.p;
+            a.h1 = c->X16b.p;
+            a.ldh = ldh;
+        }
         a.gacc = c->gacc.p;
         a.carry = c->carry.p;
         a.c = cc;
         a.judge = 1u;
         while (true) {
             for (uint32_t k = 0; k < batch; ++k) {
-                a.inc = launched == 0 ? 1u : 2u;
+                a.inc = launched == 0 && !q8 ? 1u : 2u;
                 launch_sweep(c, MODE_SWEEP, false, a);
                 ++launched;
             }
@@ -1979,10 +2355,18 @@ void run_incident(fh_context* c, uint32_t ldb, const float* B, const float* D, f
 void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const float* rr, const float* tt,
                          const float* Bin, float* Bout, bool w16, uint16_t* h16in, uint16_t* h16out, float* gacc,
                          float* carry, bool pairs, float2* partials, uint32_t partialCap,
-                         const SweepTables* tables) {
+                         const SweepTables* tables, uint8_t* q8in, uint8_t* q8out) {
     SweepArgs a{};
     a.partialExt = partials;
     a.partialCap = partialCap;
+    if (q8in && q8out && gacc && carry && band_parallel(ldb)) {
+        a.q8 = 1u;
+        a.q0 = q8in;
+        a.q1 = q8out;
+        a.ldq = q8_stride(ldb);
+        w16 = true;  // an increment sweep below
+        h16in = h16out = nullptr;
+    }
     if (tables && band_parallel(ldb)) {
         a.fmat = tables->fmat;
         a.rtab = tables->rtab;
@@ -1993,7 +2377,7 @@ void launch_single_sweep(fh_context* c, uint32_t ldb, const float* cc, const flo
         a.omega = tables->omega;
     }
     a.jacobiOnly = pairs ? 0u : 1u;
-    if (w16 && h16in && gacc && carry && band_parallel(ldb)) {
+    if (w16 && (h16in || a.q8) && gacc && carry && band_parallel(ldb)) {
         a.inc = 2u;
         a.gacc = gacc;
         a.carry = carry;

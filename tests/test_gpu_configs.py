@@ -7,7 +7,8 @@
   shortwave solves per band against the FP64 oracle;
 * terrain forest (1.0 M facets, 512 rays, 47 windows + longwave): graph
   invariants, one device sweep against an FP64 oracle sweep on the full
-  graph, and the 24-node diurnal run (plain FP32 and BF16 + surrogate).
+  graph, and the 24-node diurnal run (plain FP32, FP8 increments +
+  surrogate, BF16 increments).
 """
 import math
 
@@ -15,7 +16,7 @@ import numpy as np
 import pytest
 
 from streamsim_amd import FacetEngine, faceteb, optics, scenes
-from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32
+from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP8_INC, FH_PRECISION_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -146,7 +147,7 @@ def test_urban_bands_bf16_and_fp32_against_fp64(oracle):
     ref, _, _ = oracle.solve_bands(og, r, t, e, D, dif, max_iter=4000, tol_rel=1e-9)
     ref = ref.astype(np.float64)
     mask = ref > 1e-3 * ref.max(axis=0, keepdims=True)
-    for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC):
+    for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC, FH_PRECISION_FP8_INC):
         res = eng.solveBands(r, t, e, D, dif, tolRel=1e-6, precision=prec)
         got = res.radiosity.astype(np.float64)
         err = [float(np.max(np.abs(got[mask[:, b], b] - ref[mask[:, b], b]) / ref[mask[:, b], b]))
@@ -262,9 +263,12 @@ def test_terrain_diurnal_plain_and_fast(terrain):
     z = np.load(Path(__file__).resolve().parents[1] / "streamsim_amd" / "data" / "surrogate.npz", allow_pickle=False)
     eng.mlpSetParams(int(z["hidden"]), z["params"], z["feature_mean"], z["feature_std"], float(z["target_mean"]),
                      float(z["target_std"]))
-    fast = faceteb.run_diurnal(eng, meteo, win, cfg, precision=FH_PRECISION_BF16_DC, warm=1)
+    fast = faceteb.run_diurnal(eng, meteo, win, cfg, precision=FH_PRECISION_FP8_INC, warm=1)
     T_fast = eng.ebGetState()["temperature"]
-    assert all(plain.converged) and all(fast.converged)
+    bf16 = faceteb.run_diurnal(eng, meteo, win, cfg, precision=FH_PRECISION_BF16_DC)
+    T_bf16 = eng.ebGetState()["temperature"]
+    assert all(plain.converged) and all(fast.converged) and all(bf16.converged)
     assert np.max(np.abs(T_plain - T_fast)) < 0.02
+    assert np.max(np.abs(T_plain - T_bf16)) < 0.02
     assert max(abs(c) for c in plain.closure) < 1e-4
     assert max(abs(c) for c in fast.closure) < 1e-4

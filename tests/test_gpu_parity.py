@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from streamsim_amd import faceteb, optics, scenes
-from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32
+from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP8_INC, FH_PRECISION_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -538,6 +538,11 @@ def test_eb_diurnal_plain_vs_bf16(oracle, nb):
     T2 = eng.ebGetState()["temperature"]
     assert all(s1.converged) and all(s2.converged)
     assert np.max(np.abs(T1 - T2)) < 0.02
+    # FP8 increments (from 33 windows on) reach the same state too
+    s3 = faceteb.run_diurnal(eng, met, win, cfg, precision=FH_PRECISION_FP8_INC)
+    T3 = eng.ebGetState()["temperature"]
+    assert all(s3.converged)
+    assert np.max(np.abs(T1 - T3)) < 0.02
 
 
 def test_mlp_gradient_matches_oracle(oracle):
@@ -646,7 +651,7 @@ def test_pair_blocks_match_oracle(oracle, nb):
     sky = rng.uniform(10.0, 80.0, nb).astype(np.float32)
     ref, _, _ = oracle.solve_bands(og, r, t, e, D, sky, tol_rel=1e-9)
     _, it_jacobi, _ = oracle.solve_bands(og, r, t, e, D, sky, tol_rel=1e-7)
-    for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC):
+    for prec in (FH_PRECISION_FP32, FH_PRECISION_BF16_DC, FH_PRECISION_FP8_INC):
         res = eng.solveBands(r, t, e, D, sky, tolRel=1e-7, precision=prec)
         print(f"  precision {prec}: {res.iterations} sweeps, Jacobi {it_jacobi}", flush=True)
         assert _rel(res.radiosity, ref) < RTOL, prec
@@ -704,6 +709,18 @@ def test_bf16_increments_follow_fp32(oracle, nb):
     assert f32.iterations - 2 <= b16.iterations <= f32.iterations, (b16.iterations, f32.iterations)
     assert _rel(b16.radiosity, f32.radiosity) < 2e-6
     assert _rel(b16.radiosity, ref) < RTOL
+    # FP8 increments (12-band chunks, shared exponent): the same fixed point,
+    # at most two more sweeps than FP32 (each increment loses up to 1/16 to
+    # rounding, sent in the next sweep)
+    q8 = eng.solveBands(r, t, e, D, sky, init=init, tolRel=1e-7, precision=FH_PRECISION_FP8_INC)
+    print(f"  sweeps: fp32 {f32.iterations}, bf16 {b16.iterations}, fp8 {q8.iterations}", flush=True)
+    assert f32.iterations - 2 <= q8.iterations <= f32.iterations + 2, (q8.iterations, f32.iterations)
+    assert _rel(q8.radiosity, f32.radiosity) < 2e-6
+    assert _rel(q8.radiosity, ref) < RTOL
+    # from a zero start too (the first sweep gathers the FP8 rounding of 0)
+    z32 = eng.solveBands(r, t, e, D, sky, tolRel=1e-7, precision=FH_PRECISION_FP32)
+    z8 = eng.solveBands(r, t, e, D, sky, tolRel=1e-7, precision=FH_PRECISION_FP8_INC)
+    assert z8.iterations <= z32.iterations + 2 and _rel(z8.radiosity, ref) < RTOL
 
 
 class _Hip:
