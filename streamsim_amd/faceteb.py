@@ -239,8 +239,8 @@ def run_diurnal(engine, meteo: list[Meteo], windows: ShortwaveWindows, cfg: EbCo
     for k, m in enumerate(meteo):
         cz = math.cos(math.radians(m.zenith))
         dni, dif = windows.irradiance(m.shortwave, cz)
-        p = node_params(m, cfg, first=(k == 0), precision=precision, warm=warm if k > 0 else 0,
-                        zero_start=zero_start, record=record is not None)
+        p = node_params(m, cfg, first=(k == 0), precision=precision, warm=warm, zero_start=zero_start,
+                        record=record is not None)
         st = engine.ebRunNode(p, dni, dif)
         stats.add(st)
         if record is not None:

@@ -3,7 +3,8 @@
 The plain coupled solver (device FacetEB node step) generates (features,
 converged temperature) pairs on the forest and urban scenes under random
 conditions (sun zenith U(0, 70 deg), azimuth U(0, 360), air temperature
-U(280, 310) K, wind U(0.5, 6) m/s, a fifth of the nodes at night); the MLP
+U(280, 310) K, wind U(0.5, 6) m/s, a fifth of the nodes at night) -- two
+forests, the urban block and a 160 m terrain patch; the MLP
 9 -> 128 -> 128 -> 1 (GELU) is trained by the engine's own HIP forward /
 backward / Adam kernels (fh_mlp_train) on device-resident data.  The warm
 start only seeds the fixed point: every node still iterates to 0.01 K, so
@@ -122,17 +123,18 @@ def evaluate(eng, win, meteo, cfg, model):
         for k, m in enumerate(meteo):
             cz = math.cos(math.radians(m.zenith))
             dni, dif = win.irradiance(m.shortwave, cz)
-            p = faceteb.node_params(m, cfg, first=(k == 0), precision=FH_PRECISION_FP32,
-                                    warm=warm if k > 0 else FH_WARM_PREVIOUS, record=(warm == FH_WARM_PREVIOUS))
+            p = faceteb.node_params(m, cfg, first=(k == 0), precision=FH_PRECISION_FP32, warm=warm,
+                                    record=(warm == FH_WARM_PREVIOUS))
             t0 = time.perf_counter()
             st = eng.ebRunNode(p, dni, dif)
             times.append(time.perf_counter() - t0)
             its.append(st.couplingIterations)
-            if warm == FH_WARM_PREVIOUS and k > 0:
+            if warm == FH_WARM_PREVIOUS:
                 f, T = eng.ebGetFeatures()
                 pred = predict(eng, f, model) + m.air_temperature_k
                 errs.append(float(np.sqrt(np.mean((pred - T) ** 2))))
-        res[name] = dict(coupling=int(np.sum(its)), seconds=float(np.sum(times)), per_node=its)
+        res[name] = dict(coupling=int(np.sum(its)), seconds=float(np.sum(times)), per_node=its,
+                         first_node=int(its[0]))
         if errs:
             res["surrogate_rmse_K"] = float(np.mean(errs))
     return res
@@ -154,7 +156,7 @@ def main() -> None:
         plan = [("row_crop", dict(seed=11), 128, 4)]
     else:
         plan = [("forest", dict(seed=12), 1024, 8), ("forest", dict(seed=13), 1024, 8),
-                ("urban", dict(seed=14), 512, 31)]
+                ("urban", dict(seed=14), 512, 31), ("terrain", dict(seed=31, n=160, trees=30), 512, 47)]
     eng = None
     for name, kw, rays, bands in plan:
         t0 = time.perf_counter()
@@ -171,19 +173,32 @@ def main() -> None:
     log(f"[surrogate] dataset {X.shape}, target mean {Y.mean():.2f} K sd {Y.std():.2f} K")
     model = train(eng, X, Y, a.epochs, a.batch, a.lr, 7)
     log(f"[surrogate] trained in {float(model['train_seconds']):.1f}s, loss {model['losses'][-1]:.4f}")
-    # held-out geometry and conditions
-    held = scenes.row_crop(seed=99, leaves_per_row=100, cells=16) if a.quick else scenes.forest(seed=77)
+    # held-out geometry and conditions: a forest seed and the 1 M-facet
+    # terrain forest of the benchmark (neither in the training data)
+    held = [("row_crop", scenes.row_crop(seed=99, leaves_per_row=100, cells=16), 128, 4)] if a.quick else \
+        [("forest", scenes.forest(seed=77), 1024, 8), ("terrain", scenes.terrain(), 512, 47)]
     eng.destroy()
-    eng, win = engine_for(held, 128 if a.quick else 1024, 16, 4 if a.quick else 8)
-    eng.mlpSetParams(int(model["hidden"]), model["params"], model["feature_mean"], model["feature_std"],
-                     float(model["target_mean"]), float(model["target_std"]))
-    ev = evaluate(eng, win, faceteb.diurnal_meteo(24, t_mean=295.0, t_amp=7.0, wind=3.0, day_of_year=200), cfg,
-                  model)
-    log(f"[surrogate] held-out: {ev}")
+    evals = {}
+    for name, sc, rays, bands in held:
+        eng, win = engine_for(sc, rays, 16, bands)
+        eng.mlpSetParams(int(model["hidden"]), model["params"], model["feature_mean"], model["feature_std"],
+                         float(model["target_mean"]), float(model["target_std"]))
+        evals[name] = evaluate(eng, win, faceteb.diurnal_meteo(24, t_mean=295.0, t_amp=7.0, wind=3.0,
+                                                               day_of_year=200), cfg, model)
+        log(f"[surrogate] held-out {name}: {evals[name]}")
+        eng.destroy()
+    ev = evals[held[0][0]]
     Path(a.out).parent.mkdir(parents=True, exist_ok=True)
+    extra = {}
+    for name, e in evals.items():
+        extra[f"eval_{name}_plain_coupling"] = e["plain"]["coupling"]
+        extra[f"eval_{name}_surrogate_coupling"] = e["surrogate"]["coupling"]
+        extra[f"eval_{name}_plain_first_node"] = e["plain"]["first_node"]
+        extra[f"eval_{name}_surrogate_first_node"] = e["surrogate"]["first_node"]
+        extra[f"eval_{name}_rmse_K"] = e.get("surrogate_rmse_K", float("nan"))
     np.savez(a.out, **model, eval_plain_coupling=ev["plain"]["coupling"],
              eval_surrogate_coupling=ev["surrogate"]["coupling"],
-             eval_rmse_K=ev.get("surrogate_rmse_K", float("nan")))
+             eval_rmse_K=ev.get("surrogate_rmse_K", float("nan")), **extra)
     print(f"saved {a.out}", flush=True)
 
 
