@@ -336,6 +336,9 @@ typedef struct {
     double swAbsorbed;   /* sum A alpha H                                  */
     double swEscaped;    /* sum A fsky B                                   */
     double netRadiation; /* sum A Rn over energy-balance sides             */
+    float swMaxDelta;       /* max |dB| of the shortwave solve's last sweep   */
+    uint32_t lwFinalSweeps; /* sweeps of the final LW solve (core:3834-3846)  */
+    float lwMaxDelta;       /* max |dB| of its last sweep                     */
 } fh_eb_node_stats;
 
 /* Runs one node fully on the device: sunlit rays, SW optics, multi-band SW
@@ -364,6 +367,18 @@ int fh_eb_get_photovoltaic(fh_context *ctx, float *incidentShortwave, float *abs
                            float *power);
 /* Replaces the temperature state (surfaceCount values). */
 int fh_eb_set_temperature(fh_context *ctx, const float *temperature);
+
+/* The last node's shortwave windows solved again with the direct
+ * irradiance removed: the reference's per-window diffuse-only solve,
+ * solveRadiation(withoutDirectLight(windowOptics), windowDiffuse, ...)
+ * (core:3533-3536, core:1561-1569), whose sum over the windows gives the
+ * result's light enhancement max(0, B_SW - B_diffuse) (core:4046-4047).
+ * The node's optics and sky radiance, FP32 sweeps from zero to its
+ * radiosity tolerance; the node's state is not changed.  radiosity
+ * [surfaceCount]: the sum over the windows; bandRadiosity
+ * [surfaceCount][swBands]: per window (either may be NULL).  Zero when the
+ * last node had no shortwave; FH_LOGIC_ERROR before the first node. */
+int fh_eb_diffuse_radiosity(fh_context *ctx, float *radiosity, float *bandRadiosity, fh_solve_stats *stats);
 
 
 /* ---- learned warm start (no counterpart in the reference) -------------- */

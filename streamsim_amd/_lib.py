@@ -88,6 +88,7 @@ class FhEbNodeStats(C.Structure):
         ("lwSweeps", C.c_uint32), ("maxTemperatureDelta", C.c_float), ("maxResidualEquivalent", C.c_float),
         ("swRefreshes", C.c_uint32), ("maxEnergyResidual", C.c_float), ("swInput", C.c_double),
         ("swAbsorbed", C.c_double), ("swEscaped", C.c_double), ("netRadiation", C.c_double),
+        ("swMaxDelta", C.c_float), ("lwFinalSweeps", C.c_uint32), ("lwMaxDelta", C.c_float),
     ]
 
 
@@ -129,6 +130,7 @@ SIGNATURES = {
     "fh_eb_get_state": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "fh_eb_get_photovoltaic": (_I, [_P, _P, _P, _P]),
     "fh_eb_set_temperature": (_I, [_P, _P]),
+    "fh_eb_diffuse_radiosity": (_I, [_P, _P, _P, C.POINTER(FhSolveStats)]),
     "fh_mlp_param_count": (_U32, [_U32]),
     "fh_mlp_set_params": (_I, [_P, _U32, _P, _P, _P, _F, _F]),
     "fh_mlp_get_params": (_I, [_P, _P]),

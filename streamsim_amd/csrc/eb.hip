@@ -647,6 +647,16 @@ __global__ __launch_bounds__(256) void k_reduce_sum4(const double* __restrict__ 
     }
 }
 
+// per side: sum of the nb bands of a [S][ldb] array, in FP64
+__global__ void k_band_sum(const float* __restrict__ B, uint32_t S, uint32_t nb, uint32_t ldb,
+                           double* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    double acc = 0.0;
+    for (uint32_t j = 0; j < nb; ++j) acc += double(B[size_t(s) * ldb + j]);
+    out[s] = acc;
+}
+
 __global__ void k_copy_f32(const float* __restrict__ a, float* __restrict__ b, size_t n) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) b[i] = a[i];
@@ -754,6 +764,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     e.swValid = e.lwValid = e.tValid = false;
     e.swRhoValid = false;
     e.guessPrevValid = false;
+    e.lastNodeValid = false;
     if (swNb && c->hasGraph) {
         // spectral radius per internal band of this scene's SW operator (the
         // Chebyshev parameter): r and t do not depend on the sun, so the
@@ -797,7 +808,7 @@ static void newton(fh_context* c, const EbConst& k, int update, float& maxDT, fl
 }
 
 // LW: source from T, warm or zero start, solve, incident (into e.Hlw)
-static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p, uint32_t precision) {
+static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p, uint32_t precision, float* maxDelta = nullptr) {
     EbState& e = c->eb;
     const uint32_t S = c->S;
     hipLaunchKernelGGL(k_lw_source, blocks_for(S), dim3(256), 0, c->stream, S, e.T.p, e.eps.p, e.rlw.p, e.tlw.p,
@@ -820,6 +831,7 @@ static uint32_t lw_solve(fh_context* c, const fh_eb_node_params& p, uint32_t pre
     if (fin == 1) e.Blw0.swap(e.Blw1);
     e.lwValid = true;
     if (!hDone) run_incident(c, 1u, e.Blw0.p, e.zero.p, e.Hlw.p);
+    if (maxDelta) *maxDelta = st.maxDelta;
     return st.iterations;
 }
 
@@ -919,6 +931,8 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
             diffInt[b] = diffHost[e.swOrder[b]];
         }
         FH_HIP(hipMemcpyAsync(e.dni.p, dpad.data(), sizeof(float) * ldb, hipMemcpyHostToDevice, c->stream));
+        e.lastDni = dpad;
+        e.lastDiffuse = diffInt;
         const size_t n = size_t(S) * ldb;
         static const bool noTables = [] {
             const char* v = std::getenv("FH_SW_ARRAYS");
@@ -993,6 +1007,7 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         if (fin == 1) e.Bsw0.swap(e.Bsw1);
         st.swSweeps = ss.iterations;
         st.swRefreshes = ss.refreshes;
+        st.swMaxDelta = ss.maxDelta;
         if (!fromG && !hDone) {
             if (tab) optics_arrays();  // the incident sweep reads the D array
             run_incident(c, ldb, e.Bsw0.p, e.Dsw.p, e.Hsw.p);
@@ -1069,7 +1084,8 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
     // final LW solve with the converged temperatures and the residual check
     // (core:3834-3846), always with FP32 form factors: in BF16 mode this is
     // the defect check, and a failed check continues with FP32 sweeps
-    st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
+    st.lwFinalSweeps = lw_solve(c, p, FH_PRECISION_FP32, &st.lwMaxDelta);
+    st.lwSweeps += st.lwFinalSweeps;
     float dtFinal = 0.0f, maxEq = 0.0f;
     double rn = 0.0;
     float maxRes = 0.0f;
@@ -1077,7 +1093,8 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
     if (bf16 && !(maxDT <= p.temperatureTolerance && maxEq <= p.temperatureTolerance)) {
         iters += coupled_loop(c, p, k, false, std::max(1u, p.maxCoupling), last);
         maxDT = last[2];
-        st.lwSweeps += lw_solve(c, p, FH_PRECISION_FP32);
+        st.lwFinalSweeps = lw_solve(c, p, FH_PRECISION_FP32, &st.lwMaxDelta);
+        st.lwSweeps += st.lwFinalSweeps;
         newton(c, k, 0, dtFinal, maxEq, &rn, &maxRes);
     }
     st.lwSweeps += iters;
@@ -1087,7 +1104,76 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
     st.maxEnergyResidual = maxRes;
     st.converged = (maxDT <= p.temperatureTolerance && maxEq <= p.temperatureTolerance) ? 1u : 0u;
     st.netRadiation = rn;
+    e.lastSun[0] = p.sun.x;
+    e.lastSun[1] = p.sun.y;
+    e.lastSun[2] = p.sun.z;
+    e.lastTolerance = p.radiosityTolerance;
+    e.lastMaxSweeps = p.maxSweeps;
+    e.lastNodeValid = true;
     if (out) *out = st;
+}
+
+// Diffuse-only shortwave radiosity of the last node: the reference solves
+// every window a second time with the direct irradiance removed
+// (withoutDirectLight, core:1561-1569, core:3533-3536) for the light
+// enhancement max(0, B - B_diffuse) of the result (core:1571-1583,
+// core:4046-4047).  The node's optics (a Hapke ground at its sun) and sky
+// radiance with the direct term zero, FP32 sweeps from zero to the node's
+// radiosity tolerance; the node's own state (its radiosity, temperatures
+// and fluxes) is left as it is.  sumHost [S]: the sum over the windows;
+// bandHost [S][nb]: per window in window order (either may be null).
+void eb_diffuse_solve(fh_context* c, float* sumHost, float* bandHost, fh_solve_stats* out) {
+    EbState& e = c->eb;
+    const uint32_t S = c->S;
+    fh_solve_stats ss{};
+    const uint32_t nb = e.swNb, ldb = e.swLdb;
+    if (!nb || !e.swValid) {  // no shortwave at the last node: no window was solved
+        if (sumHost) std::fill(sumHost, sumHost + S, 0.0f);
+        if (bandHost) std::fill(bandHost, bandHost + size_t(S) * nb, 0.0f);
+        if (out) *out = ss;
+        return;
+    }
+    const size_t n = size_t(S) * ldb;
+    e.Bdf0.ensure(n);
+    e.Bdf1.ensure(n);
+    e.dfSum.ensure(S);
+    launch_fill_f32(c, e.dni.p, ldb, 0.0f);
+    hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p, e.swRefl.p,
+                       e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, e.lastSun[0], e.lastSun[1],
+                       e.lastSun[2], e.dni.p, e.leafCount, c->hapkeStrength, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
+    FH_HIP(hipGetLastError());
+    set_sky(c, e.lastDiffuse.data(), nb, ldb);
+    launch_fill_f32(c, e.Hsw.p, n, 0.0f);
+    launch_fold(c, e.Hsw.p, e.rsw.p, e.tsw.p, e.Dsw.p, ldb, e.csw.p);
+    launch_fill_f32(c, e.Bdf0.p, n, 0.0f);
+    JacobiOptions o;
+    o.ldb = ldb;
+    o.precision = FH_PRECISION_FP32;
+    o.maxIter = e.lastMaxSweeps;
+    o.tolRel = e.lastTolerance;
+    if (e.swRhoValid) o.rho2 = e.swRho2.p;  // the operator (r, t) of the node
+    float* bufs[2] = {e.Bdf0.p, e.Bdf1.p};
+    const int fin = run_jacobi(c, o, e.csw.p, e.rsw.p, e.tsw.p, bufs, &ss, nullptr);
+    const float* B = fin == 1 ? e.Bdf1.p : e.Bdf0.p;
+    // the node's direct normal irradiance back in place
+    FH_HIP(hipMemcpyAsync(e.dni.p, e.lastDni.data(), sizeof(float) * ldb, hipMemcpyHostToDevice, c->stream));
+    if (sumHost) {
+        hipLaunchKernelGGL(k_band_sum, blocks_for(S), dim3(256), 0, c->stream, B, S, nb, ldb, e.dfSum.p);
+        FH_HIP(hipGetLastError());
+        std::vector<double> sum(S);
+        FH_HIP(hipMemcpyAsync(sum.data(), e.dfSum.p, sizeof(double) * S, hipMemcpyDeviceToHost, c->stream));
+        FH_HIP(hipStreamSynchronize(c->stream));
+        for (uint32_t s = 0; s < S; ++s) sumHost[s] = float(sum[s]);
+    }
+    if (bandHost) {
+        std::vector<float> all(n);
+        FH_HIP(hipMemcpyAsync(all.data(), B, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+        FH_HIP(hipStreamSynchronize(c->stream));
+        for (uint32_t s = 0; s < S; ++s)
+            for (uint32_t j = 0; j < nb; ++j) bandHost[size_t(s) * nb + e.swOrder[j]] = all[size_t(s) * ldb + j];
+    }
+    FH_HIP(hipStreamSynchronize(c->stream));
+    if (out) *out = ss;
 }
 
 }  // namespace fh
@@ -1244,6 +1330,15 @@ int fh_eb_set_temperature(fh_context* c, const float* T) {
         FH_HIP(hipStreamSynchronize(c->stream));
         c->eb.tValid = true;
         c->eb.guessPrevValid = false;  // T no longer follows the warm-started chain
+    });
+}
+
+int fh_eb_diffuse_radiosity(fh_context* c, float* radiosity, float* bandRadiosity, fh_solve_stats* stats) {
+    return guarded_eb(c, [&] {
+        if (!c->eb.ready || c->eb.facets != c->F || !c->hasGraph)
+            fail(FH_LOGIC_ERROR, "fh_eb_set_scene() must be called before fh_eb_diffuse_radiosity()");
+        if (!c->eb.lastNodeValid) fail(FH_LOGIC_ERROR, "fh_eb_run_node() must be called before fh_eb_diffuse_radiosity()");
+        fh::eb_diffuse_solve(c, radiosity, bandRadiosity, stats);
     });
 }
 

@@ -161,6 +161,16 @@ struct EbState {
     float guessPrevTa = 0.0f;      // air temperature of that node
     bool guessPrevValid = false;   // the previous node was warm-started by the network
     bool featValid = false;
+    // the last node's shortwave inputs, for its diffuse-only solve: sun,
+    // direct normal (internal band order, padded to swLdb) and diffuse per
+    // internal band, radiosity tolerance and sweep cap
+    float lastSun[3] = {0.0f, 0.0f, 0.0f};
+    std::vector<float> lastDni, lastDiffuse;
+    float lastTolerance = 1.0e-5f;
+    uint32_t lastMaxSweeps = 0;
+    bool lastNodeValid = false;    // a node ran on the current graph and scene
+    DBuf<float> Bdf0, Bdf1;        // [S][swLdb] diffuse-only solve
+    DBuf<double> dfSum;            // [S] its sum over the windows
 };
 
 // Surrogate MLP (9 -> H -> H -> 1, GELU) parameters and normalisation.
@@ -264,6 +274,7 @@ inline void eb_graph_changed(fh_context* c) {
     e.lwValid = false;
     e.featValid = false;
     e.guessPrevValid = false;
+    e.lastNodeValid = false;
     e.gview.release();
 }
 

@@ -304,6 +304,19 @@ class FacetEngine:
                                                      _ptr(out["absorbedShortwave"]), _ptr(out["power"])))
         return out
 
+    def ebDiffuseRadiosity(self) -> tuple[np.ndarray, np.ndarray, FhSolveStats]:
+        """The last node's windows solved without the direct irradiance
+        (withoutDirectLight, facetrtcore.cpp:1561-1569, 3533-3536): per side
+        the sum over the windows, per side and window, and the solve stats."""
+        S = self.surfaceCount()
+        nb = getattr(self, "_eb_bands", 0)
+        total = np.zeros(S, np.float32)
+        bands = np.zeros((S, max(nb, 1)), np.float32)
+        st = FhSolveStats()
+        self._check(self._lib.fh_eb_diffuse_radiosity(self._h, _ptr(total), _ptr(bands) if nb else None,
+                                                      C.byref(st)))
+        return total, bands[:, :nb], st
+
     def ebSetTemperature(self, T) -> None:
         T = _f32(T)
         self._check(self._lib.fh_eb_set_temperature(self._h, _ptr(T)))

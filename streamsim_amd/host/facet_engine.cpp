@@ -555,6 +555,107 @@ EbSettings ebSettings() {
     return s;
 }
 
+// Direct (Esun) and diffuse (Esky) shortwave spectra, 1-nm samples from
+// 400 nm, for the energy windows and the sensor bands (ShortwaveSpectrum and
+// loadShortwaveSpectrum, core:2458-2551).  A project run reads the two files
+// its input names, Esun_.dat and Esky_.dat in the executable's "defined"
+// folder (facetInputView, core:1930-1931, base/projectjson.cpp:108); this
+// engine looks there too, or in $STREAMSIM_DEFINED_DIR.  Each file is a list
+// of numbers (negative or non-finite ones read as 0, at most 2001 kept);
+// both are cut to the shorter and scaled together so that the trapezoid
+// integral of direct + diffuse is one per micrometre.  Without two readable
+// spectra of at least two samples, or with a zero integral, the flat
+// spectrum 0.5 directRatio / 0.5 (1 - directRatio) over 2001 samples is
+// used (its integral is one too), as for an OBJ input, which names no files.
+struct ShortwaveSpectrum {
+    static constexpr size_t kMaxSamples = 2001;
+    std::vector<float> direct, diffuse;
+    std::string source;  // the folder the spectra came from; empty: flat
+
+    size_t count() const { return std::min(direct.size(), diffuse.size()); }
+    // the sample nearest a wavelength in nm (micrometres when <= 2.5)
+    static float at(const std::vector<float>& v, float wavelength) {
+        if (v.empty()) return 0.0f;
+        const float nm = wavelength <= 2.5f ? wavelength * 1000.0f : wavelength;
+        const long k = std::clamp(std::lround(nm - 400.0f), 0L, long(v.size()) - 1L);
+        return v[size_t(k)];
+    }
+    float directAt(float wavelength) const { return at(direct, wavelength); }
+    float diffuseAt(float wavelength) const { return at(diffuse, wavelength); }
+    // trapezoid integral over samples [a, b], per micrometre
+    static float integrate(const std::vector<float>& v, size_t a, size_t b) {
+        if (v.size() < 2 || a >= v.size() - 1) return 0.0f;
+        b = std::min(b, v.size() - 1);
+        double sum = 0.0;
+        for (size_t k = a; k < b; ++k) sum += 0.5 * double(v[k] + v[k + 1]);
+        return float(sum * 0.001);
+    }
+
+    static std::vector<float> readColumn(const std::filesystem::path& file) {
+        std::ifstream in(file);
+        std::vector<float> v;
+        float x = 0.0f;
+        while (v.size() < kMaxSamples && in >> x) v.push_back(std::isfinite(x) ? std::max(0.0f, x) : 0.0f);
+        return v;
+    }
+    static ShortwaveSpectrum flat(float directRatio) {
+        ShortwaveSpectrum s;
+        s.direct.assign(kMaxSamples, 0.5f * directRatio);
+        s.diffuse.assign(kMaxSamples, 0.5f * (1.0f - directRatio));
+        return s;
+    }
+    static ShortwaveSpectrum load(const std::filesystem::path& folder, float directRatio) {
+        ShortwaveSpectrum s;
+        s.direct = readColumn(folder / "Esun_.dat");
+        s.diffuse = readColumn(folder / "Esky_.dat");
+        const size_t n = s.count();
+        if (n < 2) return flat(directRatio);
+        s.direct.resize(n);
+        s.diffuse.resize(n);
+        double total = 0.0;
+        for (size_t k = 0; k + 1 < n; ++k)
+            total += 0.5 * double(s.direct[k] + s.direct[k + 1] + s.diffuse[k] + s.diffuse[k + 1]);
+        total *= 0.001;
+        if (!(total > 0.0) || !std::isfinite(total)) return flat(directRatio);
+        const float inv = float(1.0 / total);
+        for (float& x : s.direct) x *= inv;
+        for (float& x : s.diffuse) x *= inv;
+        s.source = folder.string();
+        return s;
+    }
+};
+
+std::filesystem::path definedDirectory() {
+    if (const char* v = std::getenv("STREAMSIM_DEFINED_DIR"); v && *v) return v;
+    std::error_code ec;
+    const std::filesystem::path exe = std::filesystem::read_symlink("/proc/self/exe", ec);
+    return ec ? std::filesystem::path("defined") : exe.parent_path() / "defined";
+}
+
+// Energy windows of spectralAccelerationWidth samples over the spectrum
+// (core:3506-3524): samples [first, last] with last = min(first + width,
+// count - 1), the window's optics at its centre sample, its direct and
+// diffuse integrals; windows whose integrals sum to at most 1e-10 carry no
+// energy and are not solved.
+struct EnergyWindow {
+    uint32_t first, last;
+    float directFraction, diffuseFraction;
+    uint32_t centre() const { return (first + last) / 2u; }
+};
+
+std::vector<EnergyWindow> energyWindows(const ShortwaveSpectrum& sp, uint32_t width) {
+    std::vector<EnergyWindow> w;
+    const size_t n = sp.count();
+    for (size_t f = 0; f + 1 < n; f += width) {
+        const size_t l = std::min(f + width, n - 1);
+        const float d = ShortwaveSpectrum::integrate(sp.direct, f, l);
+        const float q = ShortwaveSpectrum::integrate(sp.diffuse, f, l);
+        if (d + q <= 1.0e-10f) continue;
+        w.push_back({uint32_t(f), uint32_t(l), d, q});
+    }
+    return w;
+}
+
 struct MeteoStep {  // core:3172-3181
     int node = 0;
     float julianTime = 0.0f;
@@ -847,17 +948,19 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         mats[i].soilTemperatureMethod = 1;
         mats[i].vegetation = 1;
     }
-    // shortwave windows of the flat per-micrometre spectrum the reference
-    // falls back to without Esun/Esky files (core:2522-2529, 3495-3565):
-    // N1 = 2001 samples of 1 nm, windows of spectralAccelerationWidth samples
-    constexpr uint32_t kSamples = 2001;
-    std::vector<std::pair<uint32_t, uint32_t>> windows;
-    for (uint32_t f = 0; f + 1 < kSamples; f += set.spectralWidth)
-        windows.push_back({f, std::min(f + set.spectralWidth, kSamples - 1)});
+    // shortwave spectrum and energy windows (core:3238-3239, 3495-3565): a
+    // project's Esun / Esky files, else the flat spectrum
+    const ShortwaveSpectrum spectrum = project ? ShortwaveSpectrum::load(definedDirectory(), set.directRatio)
+                                               : ShortwaveSpectrum::flat(set.directRatio);
+    if (project)
+        std::cout << "PROGRESS\t9\tShortwave spectrum: "
+                  << (spectrum.source.empty() ? std::string("flat (no Esun_.dat / Esky_.dat)") : spectrum.source)
+                  << std::endl;
+    const std::vector<EnergyWindow> windows = energyWindows(spectrum, set.spectralWidth);
     const uint32_t nw = uint32_t(windows.size());
-    const float dLevel = 0.5f * set.directRatio, qLevel = 0.5f * (1.0f - set.directRatio);
-    const auto integral = [](float level, uint32_t a, uint32_t b) { return float(double(level) * (b - a) * 0.001); };
-    const float directIntegral = integral(dLevel, 0, kSamples - 1);
+    const size_t nSamples = spectrum.count();
+    const float directIntegral =
+        nSamples >= 2 ? ShortwaveSpectrum::integrate(spectrum.direct, 0, nSamples - 1) : 0.0f;
     std::vector<fh_eb_material> matList(mats, mats + 2);
     std::vector<float> swR(2 * size_t(nw)), swT(2 * size_t(nw));
     for (uint32_t w = 0; w < nw; ++w) {
@@ -893,7 +996,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
             e.pvBifaciality = pm.pvBifaciality;
             const bool leaf = m + 1 < nm;  // the last material is the ground's
             for (uint32_t w = 0; w < nw; ++w) {
-                const size_t centre = (size_t(windows[w].first) + windows[w].second) / 2u;
+                const size_t centre = windows[w].centre();
                 const float r = pm.swR(centre, leaf);
                 swR[m * nw + w] = r;
                 swT[m * nw + w] = pm.swT(centre, leaf, r);
@@ -998,8 +1101,8 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
                                 : 0.0f;
         std::vector<float> dni(nw, 0.0f), dif(nw, 0.0f);
         for (uint32_t w = 0; w < nw && total > 0.0f; ++w) {
-            dni[w] = scale * integral(dLevel, windows[w].first, windows[w].second);
-            dif[w] = total * integral(qLevel, windows[w].first, windows[w].second);
+            dni[w] = scale * windows[w].directFraction;
+            dif[w] = total * windows[w].diffuseFraction;
         }
         fh_eb_node_params p{};
         p.sun = {sd.x, sd.y, sd.z};
@@ -1034,13 +1137,12 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         const bool converged = st.converged != 0;
         allConverged = allConverged && converged;
         // sensor products (core:3862-3908, makeStateSpectralOptics
-        // core:1135-1199 with no reflectance texture and no background
-        // angular strength): per band the optics at the node's temperatures,
-        // direct total * 0.5 directRatio * 0.001 and diffuse total * 0.5 (1 -
-        // directRatio) * 0.001 of the flat spectrum, sky radiance diffuse / 2
-        // (thermal: Planck at the sky temperature), 64 sweeps; optical bands
-        // as apparent reflectance against direct cos(zenith) + diffuse / 2,
-        // thermal bands as radiance
+        // core:1135-1199 with no reflectance texture): per band the optics at
+        // the node's temperatures, direct total * Esun(band) * 0.001 and
+        // diffuse total * Esky(band) * 0.001 of the spectrum, sky radiance
+        // diffuse / 2 (thermal: Planck at the sky temperature), 64 sweeps;
+        // optical bands as apparent reflectance against direct cos(zenith) +
+        // diffuse / 2, thermal bands as radiance
         std::vector<float> bandOut(S * nbs, 0.0f);
         if (nbs) {
             std::vector<float> br(S * nbs), bt(S * nbs), be(S * nbs, 0.0f), bD(S * nbs, 0.0f), bsky(nbs),
@@ -1048,8 +1150,8 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
             std::vector<uint8_t> opt(nbs);
             for (uint32_t b = 0; b < nbs; ++b) {  // isOpticalWavelength (core:1127-1133)
                 opt[b] = (sensorBands[b] <= 2.5f ? sensorBands[b] * 1000.0f : sensorBands[b]) <= 2500.0f;
-                dS[b] = opt[b] ? total * dLevel * 0.001f : 0.0f;
-                qS[b] = opt[b] ? total * qLevel * 0.001f : 0.0f;
+                dS[b] = opt[b] ? total * spectrum.directAt(sensorBands[b]) * 0.001f : 0.0f;
+                qS[b] = opt[b] ? total * spectrum.diffuseAt(sensorBands[b]) * 0.001f : 0.0f;
                 bsky[b] = opt[b] ? qS[b] * 0.5f : planckRadiance(sensorBands[b], set.skyTemperature);
             }
             for (uint32_t f = 0; f < scene.facetCount(); ++f) {
@@ -1199,6 +1301,21 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         lastToken = token;
     }
 
+    // light enhancement of the last node (core:4046-4047): its windows solved
+    // again without the direct irradiance, max(0, B_SW - B_diffuse) per side
+    std::vector<float> enhancement(S, 0.0f);
+    {
+        std::vector<float> Bdiffuse(S, 0.0f);
+        fh_solve_stats ds{};
+        facethip::check(fh_eb_diffuse_radiosity(ctx, Bdiffuse.data(), nullptr, &ds), ctx);
+        for (size_t s = 0; s < S; ++s) enhancement[s] = std::max(0.0f, Bsw[s] - Bdiffuse[s]);
+    }
+    // iterations and maxDelta of the last node's radiative solution: the
+    // larger of its shortwave solve and its final longwave solve
+    // (combineRadiativeSolutions, core:1038-1055)
+    const uint32_t radiativeIterations = std::max(last.swSweeps, last.lwFinalSweeps);
+    const float radiativeMaxDelta = std::max(last.swMaxDelta, last.lwMaxDelta);
+
     std::ofstream out(result, std::ios::binary | std::ios::trunc);
     if (!out) throw std::runtime_error("FacetEB cannot write result: " + result);
     writeHeader(out, "HIP GPU Facet RT-EB coupled", scene);
@@ -1208,7 +1325,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
     writeVertices(out, scene);
     out << "  \"graph\": {\"directedEdges\":" << graph.directedEdgeCount << ",\"closureError\":"
         << graph.maxClosureError << ",\"fragmentOverflow\":0,\"hashOverflow\":0},\n"
-        << "  \"iterations\": " << last.lwSweeps << ",\n  \"maxDelta\": 0,\n"
+        << "  \"iterations\": " << radiativeIterations << ",\n  \"maxDelta\": " << radiativeMaxDelta << ",\n"
         << "  \"couplingIterations\": " << last.couplingIterations << ",\n"
         << "  \"temperatureDelta\": " << last.maxTemperatureDelta << ",\n"
         << "  \"converged\": " << (allConverged ? "true" : "false") << ",\n"
@@ -1219,13 +1336,13 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         << "  \"iterationBudget\": " << set.couplingIterations << ",\n";
     writeArray(out, "sunlit", sunlit, true);
     writeArray(out, "radiosity", Btot, true);
-    writeArray(out, "lightEnhancement", std::vector<float>(S, 0.0f), true);
+    writeArray(out, "lightEnhancement", enhancement, true);
     writeArray(out, "temperature", T, true);
     writeArray(out, "netRadiation", Rn, true);
     writeArray(out, "sensibleHeat", H, true);
     writeArray(out, "latentHeat", LE, true);
     writeArray(out, "storageHeat", G, true);
-    writeArray(out, "photovoltaicPower", std::vector<float>(S, 0.0f), false);
+    writeArray(out, "photovoltaicPower", pvPow, false);
     out << "}\n";
     out.close();
     if (!out) throw std::runtime_error("FacetEB cannot finish result output");
@@ -1262,6 +1379,29 @@ int runSceneObj(int argc, char** argv) try {
     return 1;
 }
 
+// facet_engine --energy-windows width directRatio: the energy windows a
+// project run solves with the spectra of the defined folder (flat without
+// them): the folder (or "flat"), then per window "first last centre
+// directFraction diffuseFraction", then the direct integral and the
+// spectrum's values at 550 nm
+int runEnergyWindows(int argc, char** argv) {
+    if (argc != 4) {
+        std::cerr << "Usage: facet_engine --energy-windows width directRatio\n";
+        return 2;
+    }
+    const uint32_t width = uint32_t(std::clamp(std::atoi(argv[2]), 1, 1000));
+    const float ratio = std::clamp(float(std::atof(argv[3])), 0.0f, 1.0f);
+    const ShortwaveSpectrum sp = ShortwaveSpectrum::load(definedDirectory(), ratio);
+    std::cout << std::setprecision(9) << (sp.source.empty() ? std::string("flat") : sp.source) << '\n';
+    for (const EnergyWindow& w : energyWindows(sp, width))
+        std::cout << w.first << ' ' << w.last << ' ' << w.centre() << ' ' << w.directFraction << ' '
+                  << w.diffuseFraction << '\n';
+    const size_t n = sp.count();
+    std::cout << "direct " << (n >= 2 ? ShortwaveSpectrum::integrate(sp.direct, 0, n - 1) : 0.0f) << " at550 "
+              << sp.directAt(550.0f) << ' ' << sp.diffuseAt(0.55f) << std::endl;
+    return 0;
+}
+
 int runSolarPosition(int argc, char** argv) {
     if (argc != 13) {
         std::cerr << "Usage: facet_engine --solar-position Y M D h m s tz lon lat pressure temperature\n";
@@ -1281,6 +1421,7 @@ int main(int argc, char** argv) {
     const std::string mode = argc > 1 ? argv[1] : "";
     if (mode == "--solar-position") return runSolarPosition(argc, argv);
     if (mode == "--scene-obj") return runSceneObj(argc, argv);
+    if (mode == "--energy-windows") return runEnergyWindows(argc, argv);
     const std::string input = argc > 2 ? argv[2] : "";
     const std::string output = argc > 3 ? argv[3] : "";
     if (mode != "eFacetRT" && mode != "eFacetEB") {

@@ -35,6 +35,91 @@ def _agreement(B, R):
     assert float(B.max()) == pytest.approx(float(R.max()), rel=1e-5)
 
 
+DEFINED = Path(__file__).resolve().parents[1] / "streamsim_amd" / "defined"
+
+
+def _spectra(folder=DEFINED):
+    """Esun / Esky of a folder, cut to the shorter (at most 2001 samples of
+    1 nm from 400 nm, negative values as 0) and scaled together so that the
+    trapezoid integral of direct + diffuse is 1 per micrometre."""
+    d = np.clip(np.loadtxt(folder / "Esun_.dat", dtype=np.float64, ndmin=1)[:2001], 0, None)
+    q = np.clip(np.loadtxt(folder / "Esky_.dat", dtype=np.float64, ndmin=1)[:2001], 0, None)
+    n = min(d.size, q.size)
+    d, q = d[:n], q[:n]
+    total = np.trapezoid(d + q, dx=1e-3)
+    return d / total, q / total
+
+
+def _energy_windows(width, folder=DEFINED):
+    """(first, last, centre, direct, diffuse) of the energy windows of a
+    spectrum: samples [f, min(f + width, n - 1)], trapezoid integrals per
+    micrometre, windows without energy left out."""
+    d, q = _spectra(folder)
+    n = d.size
+    out = []
+    for f in range(0, n - 1, width):
+        last = min(f + width, n - 1)
+        a = np.trapezoid(d[f:last + 1], dx=1e-3)
+        b = np.trapezoid(q[f:last + 1], dx=1e-3)
+        if a + b > 1e-10:
+            out.append((f, last, (f + last) // 2, a, b))
+    return out
+
+
+def _cli_windows(exe, width, ratio, env=None):
+    r = subprocess.run([str(exe), "--energy-windows", str(width), str(ratio)], capture_output=True, text=True,
+                       timeout=60, env=env)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.split("\n")
+    rows = [tuple(float(x) for x in l.split()) for l in lines[1:] if l and not l.startswith("direct")]
+    tail = next(l for l in lines if l.startswith("direct")).split()
+    return lines[0], rows, float(tail[1]), float(tail[3]), float(tail[4])
+
+
+@pytest.mark.parametrize("width", [1, 100, 250, 1000])
+def test_energy_windows_of_the_spectra(width):
+    # the windows a project run solves (core:2511-2551, 3495-3524) against
+    # the spectra integrated by numpy's trapezoid rule; the windows cover
+    # every sample, so their direct + diffuse fractions sum to one (the
+    # normalisation), and the band samples are the nearest 1-nm ones
+    exe = Path(__file__).resolve().parents[1] / "streamsim_amd" / "facet_engine"
+    src, rows, direct, d550, q550 = _cli_windows(exe, width, 0.8)
+    assert src == str(DEFINED)
+    want = _energy_windows(width)
+    assert len(rows) == len(want)
+    if width >= 100:  # (1-nm windows: samples where both spectra are zero hold none)
+        assert len(rows) == -(-2000 // width)
+    got = np.array(rows)
+    np.testing.assert_array_equal(got[:, :3], np.array([w[:3] for w in want], np.float64))
+    np.testing.assert_allclose(got[:, 3:], np.array([w[3:] for w in want]), rtol=2e-5, atol=1e-9)
+    assert abs(got[:, 3:].sum() - 1.0) < 2e-4
+    d, q = _spectra()
+    assert direct == pytest.approx(np.trapezoid(d, dx=1e-3), rel=1e-5)
+    assert d550 == pytest.approx(d[150], rel=1e-6) and q550 == pytest.approx(q[150], rel=1e-6)
+
+
+def test_energy_windows_flat_and_dark_spectra(tmp_path):
+    # no spectra: the flat spectrum, directRatio / 2 and (1 - directRatio) / 2
+    # per micrometre; a spectrum dark below 700 nm: its first three windows
+    # carry no energy and are not solved, negative samples count as zero
+    import os
+    exe = Path(__file__).resolve().parents[1] / "streamsim_amd" / "facet_engine"
+    env = dict(os.environ, STREAMSIM_DEFINED_DIR=str(tmp_path))
+    src, rows, direct, d550, q550 = _cli_windows(exe, 100, 0.7, env)
+    assert src == "flat" and len(rows) == 20
+    np.testing.assert_allclose(np.array(rows)[:, 3:], np.tile([0.035, 0.015], (20, 1)), rtol=1e-6)
+    assert direct == pytest.approx(0.7, rel=1e-6) and d550 == pytest.approx(0.35) and q550 == pytest.approx(0.15)
+    sun = np.where(np.arange(1200) < 300, -1.0, 2.0)
+    (tmp_path / "Esun_.dat").write_text("\n".join(f"{x:.6e}" for x in sun) + "\n")
+    (tmp_path / "Esky_.dat").write_text("\n".join("0" for _ in range(1500)) + "\n")
+    src, rows, direct, _, _ = _cli_windows(exe, 100, 0.8, env)
+    assert src == str(tmp_path)
+    assert [int(r[0]) for r in rows] == list(range(200, 1199, 100))  # 200-300 holds the step at 300
+    want = _energy_windows(100, tmp_path)
+    np.testing.assert_allclose(np.array(rows)[:, 3:], np.array([w[3:] for w in want]), rtol=2e-5, atol=1e-9)
+    assert direct == pytest.approx(1.0, rel=1e-5)
+
+
 def test_obj_scene_matches_reference_counts():
     sc, z = _load()
     assert sc.facet_count == int(z["facetCount"]) == 4746
@@ -191,10 +276,14 @@ def test_cli_faceteb_nodes_step_outputs_and_backpressure(tmp_path):
 def test_cli_faceteb_project(oracle, tmp_path):
     # eFacetEB on a project.json: the reference's validation project (trees
     # of a Vegetation object, a Building object, soil ground) with a
-    # meteorology block added; materials, components and settings come from
-    # the project (control, meteo), not from the environment.  The step files
-    # carry the project's sensor bands (550, 850, 10500 nm) after the nine
-    # energy values, checked at noon against the oracle (core:3862-3908)
+    # meteorology block added (a night node, then noon); materials,
+    # components and settings come from the project (control, meteo), not
+    # from the environment, and the shortwave spectrum from Esun_.dat /
+    # Esky_.dat of the defined folder.  The step files carry the project's
+    # sensor bands (550, 850, 10500 nm) after the nine energy values, checked
+    # at noon against the oracle (core:3862-3908); the result's light
+    # enhancement of the last node (noon) against the oracle's diffuse-only
+    # window solves (core:3533-3536, 4046-4047)
     print("golden cli eb project", flush=True)
     import os
     import shutil
@@ -202,7 +291,7 @@ def test_cli_faceteb_project(oracle, tmp_path):
     src = HERE / "urban_tree_project"
     prj = tmp_path / "prj"
     shutil.copytree(src, prj)
-    (prj / "meteo.txt").write_text("julian Tair e P u SW LW\n172.5 27 16 1000 2 850 360\n172.875 21 15 1000 1 0 340\n")
+    (prj / "meteo.txt").write_text("julian Tair e P u SW LW\n172.875 21 15 1000 1 0 340\n172.5 27 16 1000 2 850 360\n")
     d = json.loads((src / "project.json").read_text())
     c = d["configuration"]
     c["meteo"] = {"path": "meteo.txt", "start": 0, "end": 2, "dTime": 3600, "latitude": 40, "longitude": 116}
@@ -217,10 +306,13 @@ def test_cli_faceteb_project(oracle, tmp_path):
     r = subprocess.run([str(exe), "eFacetEB", str(prj / "project.json"), str(out)], env=env, capture_output=True,
                        text=True, timeout=300)
     assert r.returncode == 0, r.stderr
+    assert f"Shortwave spectrum: {DEFINED}" in r.stdout
     conv = [int(l.split("\t")[2]) for l in r.stdout.splitlines() if l.startswith("FACET_CONVERGENCE")]
     assert conv == [1, 1]
     res = json.loads(out.read_text())
     assert res["facetCount"] == 4138 and res["leafFacetCount"] == 138
+    assert res["time"] == "DOY172_12-00" and res["iterations"] > 0 and 0 <= res["maxDelta"] < 1.0
+    assert not np.any(res["photovoltaicPower"])  # no photovoltaic material
     comp = np.fromfile(tmp_path / "eb" / ".facet_steps" / "components.bin", np.uint8)
     # 7 trees of 18 facets (Vegetation: 2), the building's 12 (Building: 3), ground 1
     assert np.all(comp[:126] == 2) and np.all(comp[126:138] == 3) and np.all(comp[138:] == 1)
@@ -234,10 +326,10 @@ def test_cli_faceteb_project(oracle, tmp_path):
     night = np.fromfile(tmp_path / "eb" / ".facet_steps" / "energy_T=DOY172_21-00.bin", np.float32).reshape(-1, 12)
     assert np.all(night[:, 9:11] == 0) and np.all(night[:, 11] > 0)  # no light; thermal radiance
     # the noon sensor bands against the oracle: makeStateSpectralOptics with
-    # the node's temperatures and sunlit fractions, the flat spectrum's
-    # direct / diffuse levels 0.5 * 0.8 and 0.5 * 0.2 per micrometre of the
-    # 850 W m-2 total, 64 FP64 Jacobi sweeps per band, optical bands as
-    # apparent reflectance against direct cos(zenith) + diffuse / 2
+    # the node's temperatures and sunlit fractions, the spectrum's direct /
+    # diffuse values at the band (per micrometre) times the 850 W m-2 total,
+    # 64 FP64 Jacobi sweeps per band, optical bands as apparent reflectance
+    # against direct cos(zenith) + diffuse / 2
     from streamsim_amd.optics import planck_radiance
     v, nrm, mat = _project_scene(exe, prj / "project.json", tmp_path)
     F, S = 4138, 2 * 4138
@@ -249,22 +341,38 @@ def test_cli_faceteb_project(oracle, tmp_path):
     cos = np.maximum(0.0, sn @ np.asarray(sun, np.float64))
     p = np.array([VALIDATION_MATERIALS[m] for m in np.repeat(mat, 2)])
     ground = np.repeat(np.arange(F) >= 138, 2)
-    dS, qS = 850.0 * 0.4 * 0.001, 850.0 * 0.1 * 0.001
+    sd, sq = _spectra()
+    dS = 850.0 * sd[[150, 450]] * 0.001
+    qS = 850.0 * sq[[150, 450]] * 0.001
+    assert dS[0] != pytest.approx(dS[1], rel=0.05)  # the spectrum is not flat
     R = np.zeros((S, 3)); Tt = np.zeros((S, 3)); E = np.zeros((S, 3)); D = np.zeros((S, 3))
     for b in range(2):
         R[:, b] = np.where(ground, np.minimum(p[:, 0], 0.999), p[:, 0])
         Tt[:, b] = np.minimum(p[:, 1], 1 - p[:, 0])
-        D[:, b] = np.clip(sl, 0, 1) * cos * dS
+        D[:, b] = np.clip(sl, 0, 1) * cos * dS[b]
     R[:, 2] = p[:, 2]
     Tt[:, 2] = p[:, 3]
     eps = np.maximum(0.0, 1 - R[:, 2] - Tt[:, 2])
     E[:, 2] = eps * np.array([planck_radiance(10500, x) for x in np.clip(T.astype(np.float64), 1, 2000)])
-    sky = [qS * 0.5, qS * 0.5, planck_radiance(10500, 250.0)]
+    sky = [qS[0] * 0.5, qS[1] * 0.5, planck_radiance(10500, 250.0)]
     B, _, _ = oracle.solve_bands(g, R, Tt, E, D, sky, max_iter=64, tol_rel=-1.0)
     ref = dS * max(0.0, float(sun[1])) + qS * 0.5
-    want = np.concatenate([np.clip(B[:, :2] / ref, 0, 1), B[:, 2:]], axis=1)
+    want = np.concatenate([np.clip(B[:, :2] / ref[None, :], 0, 1), B[:, 2:]], axis=1)
     got = noon[:, 9:12].astype(np.float64)
     assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-3)) < 1e-3
+    # light enhancement: the windows (250 samples) solved with the direct
+    # term removed, summed, subtracted from the noon shortwave radiosity
+    win = _energy_windows(250)
+    nw = len(win)
+    Rw = np.repeat(np.where(ground, np.minimum(p[:, 0], 0.999), p[:, 0])[:, None], nw, axis=1)
+    Tw = np.repeat(np.minimum(p[:, 1], 1 - p[:, 0])[:, None], nw, axis=1)
+    Bd, _, _ = oracle.solve_bands(g, Rw, Tw, np.zeros((S, nw)), np.zeros((S, nw)), [850.0 * w[4] for w in win],
+                                  tol_rel=1e-9)
+    bsw = noon[:, 2].astype(np.float64)
+    enh_want = np.maximum(0.0, bsw - Bd.sum(axis=1))
+    enh = np.asarray(res["lightEnhancement"], np.float64)
+    assert np.max(np.abs(enh - enh_want)) < 2e-4 * bsw.max()
+    assert enh[sl > 0.9].mean() > 50.0 and np.median(enh[sl == 0]) < 0.05 * enh[sl > 0.9].mean()
 
 
 @pytest.mark.gpu

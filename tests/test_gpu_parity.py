@@ -308,6 +308,8 @@ def test_eb_state_follows_the_scene():
     with pytest.raises(LogicError):
         eng.ebGetState()
     faceteb.setup_scene(eng, bigger, win)
+    with pytest.raises(LogicError):  # no node on this scene yet
+        eng.ebDiffuseRadiosity()
     p = faceteb.node_params(met, cfg, first=True)
     p.sun.x = float("nan")
     with pytest.raises(ValueError):
@@ -320,6 +322,13 @@ def test_eb_state_follows_the_scene():
     st = eng.ebRunNode(faceteb.node_params(met, cfg, first=True), dni, dif)
     assert st.converged
     assert np.all(np.isfinite(eng.ebGetState()["temperature"]))
+    # a night node solves no window: its diffuse-only radiosity is zero
+    night = faceteb.diurnal_meteo(24)[0]
+    dn, fn = win.irradiance(night.shortwave, math.cos(math.radians(night.zenith)))
+    assert night.shortwave == 0 and not np.any(dn) and not np.any(fn)
+    st = eng.ebRunNode(faceteb.node_params(night, cfg, first=False), dn, fn)
+    total, per, ds = eng.ebDiffuseRadiosity()
+    assert not np.any(total) and not np.any(per) and ds.iterations == 0 and st.swSweeps == 0
 
 
 def _eb_case(oracle, leaves=40, cells=6, R=128, K=32, nb=4):
@@ -330,6 +339,25 @@ def _eb_case(oracle, leaves=40, cells=6, R=128, K=32, nb=4):
     win = faceteb.ShortwaveWindows.for_bands(nb)
     faceteb.setup_scene(eng, sc, win)
     return sc, eng, win
+
+
+def _check_diffuse_solve(oracle, eng, og, r, t, dif, s):
+    # the node's windows solved again without the direct irradiance
+    # (withoutDirectLight, facetrtcore.cpp:1561-1569, 3533-3536): the same
+    # optics and sky radiance, against the oracle's solve with D = 0; the
+    # node's own state is left as it was
+    total, per, ds = eng.ebDiffuseRadiosity()
+    want, _, _ = oracle.solve_bands(og, r, t, np.zeros_like(r), np.zeros_like(r), dif, tol_rel=1e-9)
+    assert _rel(per[want > 1e-3], want[want > 1e-3]) < RTOL
+    assert _rel(total[want.sum(1) > 1e-3], want.sum(1)[want.sum(1) > 1e-3]) < RTOL
+    assert ds.iterations > 0
+    # light enhancement is the direct light's share: non-negative up to the
+    # solves' tolerance, and positive where the sun shines
+    enh = s["shortwaveRadiosity"].sum(1) - total
+    assert enh.min() > -1e-4 * total.max() and enh[s["sunlit"] > 0.9].mean() > 1.0
+    after = eng.ebGetState()
+    for k in ("shortwaveRadiosity", "absorbedShortwave", "temperature", "longwaveRadiosity"):
+        np.testing.assert_array_equal(after[k], s[k])
 
 
 # 4 windows: entry-parallel sweeps; 24 and 47: band-parallel sweeps with the
@@ -360,6 +388,8 @@ def test_eb_node_matches_oracle(oracle, nb):
     H = oracle.incident_bands(og, sw, D, dif)
     absorbed = np.sum(np.clip(1 - r - t, 0, 1) * H, axis=1)
     assert _rel(s["absorbedShortwave"][absorbed > 1], absorbed[absorbed > 1]) < RTOL
+    assert st.swSweeps > 0 and st.lwFinalSweeps > 0 and 0 <= st.swMaxDelta < 1 and 0 <= st.lwMaxDelta < 1
+    _check_diffuse_solve(oracle, eng, og, r, t, dif, s)
     # coupled energy balance in FP64 with the oracle from the same start
     mats = []
     for m in (faceteb.LEAF, faceteb.SOIL, faceteb.BUILDING):
@@ -434,6 +464,7 @@ def test_eb_node_hapke_ground_matches_oracle(oracle, nb):
     H = oracle.incident_bands(og, sw, D, dif)
     absorbed = np.sum(np.clip(1 - r - t, 0, 1) * H, axis=1)
     assert _rel(s["absorbedShortwave"][absorbed > 1], absorbed[absorbed > 1]) < RTOL
+    _check_diffuse_solve(oracle, eng, og, r, t, dif, s)
     mats = []
     for m in (faceteb.LEAF, faceteb.SOIL, faceteb.BUILDING):
         om = oracle.OrEbMaterial()
