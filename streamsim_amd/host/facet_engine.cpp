@@ -168,17 +168,18 @@ std::string outputPath(const std::string& input, const std::string& requested, c
 // depth, core:2890-2894)
 // (a project's periodicNeighborCount is the default of the variable)
 // FACET_VISIBILITY=raster: the reference's raster graph and sunlit fraction
-// (FACET_RASTER_SIZE pixels square, default 1024; FACET_RASTER_LAYERS
-// fragments per pixel, default 64: the reference's kRasterSize /
-// kRasterLayers, core:75-76, where its adaptive choice stays at 64,
-// core:168-187) over the 40 hemisphere directions
+// over the 40 hemisphere directions, at the scene's visibility raster
+// (1024 pixels square and 64 fragments per pixel, kRasterSize /
+// kRasterLayers, core:75-76; a large project's 2048 and adapted layers,
+// core:168-188, 2895-2901), or FACET_RASTER_SIZE pixels square and
+// FACET_RASTER_LAYERS fragments per pixel
 facethip::Config engineConfig(const Scene& s, uint32_t periodic = 0) {
     facethip::Config c;
     c.visibilityMethod = facethip::VisibilityMethod::RayCast;
     if (const char* e = std::getenv("FACET_VISIBILITY"); e && std::string(e) == "raster") {
         c.visibilityMethod = facethip::VisibilityMethod::Raster;
-        c.rasterWidth = c.rasterHeight = 1024u;
-        c.maxFragmentsPerPixel = 64u;
+        c.rasterWidth = c.rasterHeight = s.rasterSize;
+        c.maxFragmentsPerPixel = s.rasterLayers;
         if (const char* r = std::getenv("FACET_RASTER_SIZE")) c.rasterWidth = c.rasterHeight = uint32_t(std::atoi(r));
         if (const char* l = std::getenv("FACET_RASTER_LAYERS")) c.maxFragmentsPerPixel = uint32_t(std::atoi(l));
     }
@@ -1372,7 +1373,8 @@ int runSceneObj(int argc, char** argv) try {
     const facetcli::Project prj = facetcli::loadProject(argv[2]);
     facetcli::writeSceneObj(prj.scene, argv[3]);
     std::cout << "facets " << prj.scene.facetCount() << " objects " << prj.scene.leafFacetCount << " materials "
-              << prj.materials.size() << std::endl;
+              << prj.materials.size() << " raster " << prj.scene.rasterSize << " layers " << prj.scene.rasterLayers
+              << std::endl;
     return 0;
 } catch (const std::exception& e) {
     std::cerr << "facet_engine: " << e.what() << '\n';

@@ -4,26 +4,31 @@
 // and ProjectJson, models/histream/src/base/projectjson.cpp).
 //
 // Built: objects read from OBJ files (or a box of the object's dimensions),
-// placed from their position files (x, y, z, scale, rotation), clipped to
-// the scene box and appended object by object; a flat ground grid of
-// voxel-size cells; one material per object mesh from the project's
-// spectra (custom spectra over the sensor bands), thermal, surface-energy
-// and soil entries; the FacetEB settings the project's control, light and
-// meteo blocks give (photovoltaic surface-energy entries included).  Not
-// read: DEM terrain, reflectance textures; PROSPECT, BSM and file spectra are
-// replaced by their entry's own listed values taken as a custom spectrum
-// (stated on stderr).  Fire, Fog and medium objects are skipped as the
-// reference does.
+// placed from their position files (x, y, z, scale, rotation) on the
+// terrain, clipped to the scene box and appended object by object; a ground
+// grid of voxel-size cells, flat or on a DEM (ESRI ASCII grid or baseline
+// uncompressed GeoTIFF) stretched over the scene; the visibility raster's
+// size and layers (large-scene rule, adaptive layers); one material per
+// object mesh from the project's spectra (custom spectra over the sensor
+// bands), thermal, surface-energy and soil entries; the FacetEB settings the
+// project's control, light and meteo blocks give (photovoltaic
+// surface-energy entries included).  Not read: reflectance textures;
+// PROSPECT, BSM and file spectra are replaced by their entry's own listed
+// values taken as a custom spectrum (stated on stderr).  Fire, Fog and
+// medium objects are skipped as the reference does.
 #pragma once
 
 #include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <iterator>
 #include <limits>
 #include <set>
 #include <sstream>
@@ -320,6 +325,209 @@ inline ObjGeometry boxGeometry(const std::vector<float>& dims) {
     return g;
 }
 
+// A DEM height field (DemHeightField and readDemHeightField,
+// core:2617-2697): the first band of a grid of width x height cells, row 0
+// the northernmost, no-data and non-finite cells replaced by the minimum of
+// the valid ones.  Stretched over the scene: scene x (north) runs over the
+// rows from the last to the first, scene z (east) over the columns, sampled
+// bilinearly; heights are given above the minimum.
+struct DemGrid {
+    int width = 0, height = 0;
+    float minimum = 0.0f;
+    std::vector<float> values;
+
+    bool valid() const { return width >= 2 && height >= 2 && values.size() == size_t(width) * size_t(height); }
+    float heightAbove(float northX, float eastZ, float W, float D) const {
+        if (!valid()) return 0.0f;
+        const float u = std::clamp(eastZ / D, 0.0f, 1.0f) * float(width - 1);
+        const float v = (1.0f - std::clamp(northX / W, 0.0f, 1.0f)) * float(height - 1);
+        const int c0 = int(std::floor(u)), r0 = int(std::floor(v));
+        const int c1 = std::min(width - 1, c0 + 1), r1 = std::min(height - 1, r0 + 1);
+        const float fu = u - float(c0), fv = v - float(r0);
+        const auto at = [&](int c, int r) { return values[size_t(r) * size_t(width) + size_t(c)]; };
+        const float north = at(c0, r0) * (1.0f - fu) + at(c1, r0) * fu;
+        const float south = at(c0, r1) * (1.0f - fu) + at(c1, r1) * fu;
+        return std::max(0.0f, north * (1.0f - fv) + south * fv - minimum);
+    }
+};
+
+// ESRI ASCII grid (.asc): "key value" header lines (ncols, nrows, the
+// corner or centre coordinates, cellsize, an optional nodata_value), then
+// nrows rows of ncols values from north to south
+inline DemGrid readAsciiGrid(const std::filesystem::path& path, bool& hasNoData, float& noData) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("Cannot open DEM: " + path.string());
+    DemGrid g;
+    hasNoData = false;
+    std::string key;
+    while (in >> key) {
+        const char c0 = key.empty() ? ' ' : key[0];
+        if ((c0 >= '0' && c0 <= '9') || c0 == '-' || c0 == '+' || c0 == '.') break;  // the first value
+        double v = 0.0;
+        if (!(in >> v)) throw std::runtime_error("Malformed DEM header: " + path.string());
+        const std::string k = lower(key);
+        if (k == "ncols") g.width = int(v);
+        else if (k == "nrows") g.height = int(v);
+        else if (k == "nodata_value") {
+            hasNoData = true;
+            noData = float(v);
+        }
+        key.clear();
+    }
+    if (g.width < 1 || g.height < 1 || size_t(g.width) * size_t(g.height) > (size_t(1) << 28))
+        throw std::runtime_error("Cannot open DEM: " + path.string());
+    const size_t n = size_t(g.width) * size_t(g.height);
+    g.values.reserve(n);
+    if (!key.empty()) g.values.push_back(std::strtof(key.c_str(), nullptr));
+    float x = 0.0f;
+    while (g.values.size() < n && in >> x) g.values.push_back(x);
+    if (g.values.size() != n) throw std::runtime_error("Cannot read DEM elevations: " + path.string());
+    return g;
+}
+
+// Baseline GeoTIFF (.tif): the first image, uncompressed, in strips or
+// tiles, one or more samples per pixel interleaved (the first is read), 8 to
+// 64 bits of unsigned, signed or IEEE samples, little- or big-endian; no-data
+// from the GDAL_NODATA tag (42113)
+inline DemGrid readGeoTiff(const std::filesystem::path& path, bool& hasNoData, float& noData) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("Cannot open DEM: " + path.string());
+    const std::vector<uint8_t> b((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    const auto fail = [&](const char* why) -> void {
+        throw std::runtime_error(std::string("Cannot read DEM (") + why + "): " + path.string());
+    };
+    if (b.size() < 8) fail("not a TIFF file");
+    const bool le = b[0] == 'I' && b[1] == 'I';
+    if (!le && !(b[0] == 'M' && b[1] == 'M')) fail("not a TIFF file");
+    const auto u16 = [&](size_t o) -> uint32_t {
+        if (o + 2 > b.size()) fail("truncated");
+        return le ? uint32_t(b[o]) | uint32_t(b[o + 1]) << 8 : uint32_t(b[o]) << 8 | uint32_t(b[o + 1]);
+    };
+    const auto u32 = [&](size_t o) -> uint32_t {
+        if (o + 4 > b.size()) fail("truncated");
+        return le ? u16(o) | u16(o + 2) << 16 : u16(o) << 16 | u16(o + 2);
+    };
+    if (u16(2) != 42) fail("BigTIFF and other variants are not read");
+    const size_t ifd = u32(4);
+    const uint32_t count = u16(ifd);
+    uint32_t width = 0, height = 0, bits = 8, compression = 1, samples = 1, planar = 1, format = 1;
+    uint32_t rowsPerStrip = 0, tileW = 0, tileH = 0;
+    std::vector<uint32_t> offsets, byteCounts;
+    hasNoData = false;
+    for (uint32_t e = 0; e < count; ++e) {
+        const size_t o = ifd + 2 + 12 * size_t(e);
+        const uint32_t tag = u16(o), type = u16(o + 2), n = u32(o + 4);
+        const uint32_t unit = type == 3 ? 2 : type == 4 ? 4 : 1;
+        const size_t data = size_t(unit) * n <= 4 ? o + 8 : size_t(u32(o + 8));
+        const auto value = [&](uint32_t i) -> uint32_t {
+            return type == 3 ? u16(data + 2 * size_t(i)) : type == 4 ? u32(data + 4 * size_t(i)) : b.at(data + i);
+        };
+        switch (tag) {
+        case 256: width = value(0); break;
+        case 257: height = value(0); break;
+        case 258: bits = value(0); break;
+        case 259: compression = value(0); break;
+        case 277: samples = value(0); break;
+        case 278: rowsPerStrip = value(0); break;
+        case 284: planar = value(0); break;
+        case 322: tileW = value(0); break;
+        case 323: tileH = value(0); break;
+        case 339: format = value(0); break;
+        case 273:
+        case 324:
+            offsets.resize(n);
+            for (uint32_t i = 0; i < n; ++i) offsets[i] = value(i);
+            break;
+        case 279:
+        case 325:
+            byteCounts.resize(n);
+            for (uint32_t i = 0; i < n; ++i) byteCounts[i] = value(i);
+            break;
+        case 42113:
+            if (type == 2 && data + n <= b.size()) {
+                const std::string text(reinterpret_cast<const char*>(b.data() + data), n);
+                char* end = nullptr;
+                const float v = std::strtof(text.c_str(), &end);
+                if (end != text.c_str()) {
+                    hasNoData = true;
+                    noData = v;
+                }
+            }
+            break;
+        default: break;
+        }
+    }
+    if (compression != 1) fail("compressed images are not read");
+    if (width < 1 || height < 1 || size_t(width) * height > (size_t(1) << 28)) fail("bad size");
+    if (samples < 1 || (samples > 1 && planar != 1)) fail("planar multi-band images are not read");
+    if (!((bits == 8 || bits == 16 || bits == 32 || bits == 64) && format >= 1 && format <= 3) ||
+        (format == 3 && bits < 32))
+        fail("unsupported sample type");
+    if (offsets.empty()) fail("no image data");
+    const uint32_t bytes = bits / 8, pixelBytes = bytes * samples;
+    const auto sample = [&](size_t o) -> float {
+        if (o + bytes > b.size()) fail("truncated");
+        uint64_t raw = 0;
+        for (uint32_t k = 0; k < bytes; ++k) raw |= uint64_t(b[o + (le ? k : bytes - 1 - k)]) << (8 * k);
+        if (format == 3) {
+            if (bytes == 4) {
+                const uint32_t r32 = uint32_t(raw);
+                float f;
+                std::memcpy(&f, &r32, 4);
+                return f;
+            }
+            double d;
+            std::memcpy(&d, &raw, 8);
+            return float(d);
+        }
+        if (format == 2) {
+            const int64_t sgn = int64_t(raw << (64 - 8 * bytes)) >> (64 - 8 * bytes);
+            return float(sgn);
+        }
+        return float(raw);
+    };
+    DemGrid g;
+    g.width = int(width);
+    g.height = int(height);
+    g.values.assign(size_t(width) * height, 0.0f);
+    if (tileW && tileH) {
+        const uint32_t across = (width + tileW - 1) / tileW, down = (height + tileH - 1) / tileH;
+        if (offsets.size() < size_t(across) * down) fail("missing tiles");
+        for (uint32_t ty = 0; ty < down; ++ty)
+            for (uint32_t tx = 0; tx < across; ++tx) {
+                const size_t base = offsets[size_t(ty) * across + tx];
+                for (uint32_t y = 0; y < tileH && ty * tileH + y < height; ++y)
+                    for (uint32_t x = 0; x < tileW && tx * tileW + x < width; ++x)
+                        g.values[size_t(ty * tileH + y) * width + tx * tileW + x] =
+                            sample(base + (size_t(y) * tileW + x) * pixelBytes);
+            }
+    } else {
+        const uint32_t rows = rowsPerStrip ? std::min(rowsPerStrip, height) : height;
+        if (offsets.size() < size_t((height + rows - 1) / rows)) fail("missing strips");
+        for (uint32_t y = 0; y < height; ++y) {
+            const size_t base = offsets[y / rows] + size_t(y % rows) * width * pixelBytes;
+            for (uint32_t x = 0; x < width; ++x) g.values[size_t(y) * width + x] = sample(base + size_t(x) * pixelBytes);
+        }
+    }
+    return g;
+}
+
+// readDemHeightField: a grid of at least 2 x 2 cells with a valid elevation
+inline DemGrid readDem(const std::filesystem::path& path) {
+    bool hasNoData = false;
+    float noData = 0.0f;
+    const std::string ext = lower(path.extension().string());
+    DemGrid g = ext == ".asc" ? readAsciiGrid(path, hasNoData, noData) : readGeoTiff(path, hasNoData, noData);
+    if (g.width < 2 || g.height < 2) throw std::runtime_error("DEM must contain at least 2 x 2 cells: " + path.string());
+    g.minimum = std::numeric_limits<float>::infinity();
+    for (float v : g.values)
+        if (std::isfinite(v) && !(hasNoData && v == noData)) g.minimum = std::min(g.minimum, v);
+    if (!std::isfinite(g.minimum)) throw std::runtime_error("DEM contains no valid elevations: " + path.string());
+    for (float& v : g.values)
+        if (!std::isfinite(v) || (hasNoData && v == noData)) v = g.minimum;
+    return g;
+}
+
 struct Placement {
     float x = 0.0f, y = 0.0f, z = 0.0f, scale = 1.0f, rotation = 0.0f;
 };
@@ -471,8 +679,11 @@ inline Project loadProject(const std::string& file) {
     const float D = std::max(0.01f, json::num(sc, "y", 60.0f));
     const float H = std::max(0.01f, json::num(sc, "height", 15.0f));
     const float voxel = std::max(0.01f, json::num(sc, "voxel", 1.0f));
+    // DEM ground (facetInputView writes <isDEM> and <DEM> for scene.terrain
+    // with a demFile, core:1921, 1944-1945; readDemHeightField, core:2652-2697)
+    pd::DemGrid dem;
     if (json::boolean(sc, "terrain", false) && !json::str(sc, "demFile").empty())
-        throw std::runtime_error("DEM terrain is not read by this engine (scene.terrain with demFile)");
+        dem = pd::readDem(pd::asset(dir, root, json::str(sc, "demFile")));
     for (const json::Value& m : mats.items)
         if (json::str(m, "energyModel") == "photovoltaic" &&
             json::str(m, "name") == json::str(background, "materialName", "soilset"))
@@ -486,6 +697,11 @@ inline Project loadProject(const std::string& file) {
             : 0.0f;
     prj.scene.extentX = W;
     prj.scene.extentZ = D;
+    // large scenes rasterise at 2048 pixels and 16 layers (core:2895-2901)
+    if (std::max(W, D) >= 1000.0f) {
+        prj.scene.rasterSize = 2048u;
+        prj.scene.rasterLayers = 16u;
+    }
 
     // materials of the FacetRT parameters (facetRTParametersFromInput,
     // core:2385-2420): the object default from the first object's first
@@ -588,7 +804,8 @@ inline Project loadProject(const std::string& file) {
                                         : pd::readObj(pd::asset(dir, root, model));
         const std::string posFile = json::str(o, "positionFile");
         const Vec3 lo{-W * 0.5f, 0.0f, -D * 0.5f}, hi{W * 0.5f, H, D * 0.5f};
-        for (const pd::Placement& p : pd::placements(posFile.empty() ? fs::path{} : pd::asset(dir, root, posFile), W, D)) {
+        for (pd::Placement p : pd::placements(posFile.empty() ? fs::path{} : pd::asset(dir, root, posFile), W, D)) {
+            p.z += dem.heightAbove(p.x, p.y, W, D);  // objects stand on the terrain (core:2991-2992)
             // appendGeometry (core:2762-2799): a triangle takes the material of
             // the binding whose mesh name equals its group (case-insensitive),
             // else the first binding's
@@ -609,23 +826,40 @@ inline Project loadProject(const std::string& file) {
     }
     prj.scene.leafFacetCount = prj.scene.facetCount();
 
-    // appendProjectGround (core:2830-2878), flat: cells of about the voxel
-    // size, at most a quarter of the visibility raster per side
+    // appendProjectGround (core:2830-2878): cells of about the voxel size, at
+    // most a quarter of the visibility raster per side, corners 1 cm below
+    // the terrain (the DEM's height above its minimum, 0 without one)
     const uint32_t groundMat = uint32_t(prj.materials.size());
     prj.materials.push_back(ground);
-    const uint32_t maxGrid = (std::max(W, D) >= 1000.0f ? 2048u : 1024u) / 4u;
+    const uint32_t maxGrid = prj.scene.rasterSize / 4u;
     const uint32_t cols = std::clamp(std::max(1u, uint32_t(std::ceil(W / voxel))), 1u, maxGrid);
     const uint32_t rows = std::clamp(std::max(1u, uint32_t(std::ceil(D / voxel))), 1u, maxGrid);
-    const float cw = W / float(cols), cd = D / float(rows), x0 = -W * 0.5f, z0 = -D * 0.5f, h = 0.0f - 0.01f;
+    const float cw = W / float(cols), cd = D / float(rows), x0 = -W * 0.5f, z0 = -D * 0.5f;
+    const auto h = [&](float x, float z) { return dem.heightAbove(x + W * 0.5f, z + D * 0.5f, W, D) - 0.01f; };
     for (uint32_t r = 0; r < rows; ++r) {
         const float za = z0 + cd * float(r);
         const float zb = r + 1 == rows ? z0 + D : z0 + cd * float(r + 1);
         for (uint32_t c = 0; c < cols; ++c) {
             const float xa = x0 + cw * float(c);
             const float xb = c + 1 == cols ? x0 + W : x0 + cw * float(c + 1);
-            appendTriangle(prj.scene, {xa, h, za}, {xb, h, zb}, {xb, h, za}, groundMat);
-            appendTriangle(prj.scene, {xa, h, za}, {xa, h, zb}, {xb, h, zb}, groundMat);
+            const float haa = h(xa, za), hab = h(xa, zb), hba = h(xb, za), hbb = h(xb, zb);
+            appendTriangle(prj.scene, {xa, haa, za}, {xb, hbb, zb}, {xb, hba, za}, groundMat);
+            appendTriangle(prj.scene, {xa, haa, za}, {xa, hab, zb}, {xb, hbb, zb}, groundMat);
         }
+    }
+    // adaptiveVisibilityRasterLayers (core:168-188): 16 times the mean
+    // fragment depth over the raster (periodic copies included), the layers
+    // doubled up to 64 until they hold it
+    {
+        const double pixels = double(prj.scene.rasterSize) * double(prj.scene.rasterSize);
+        const double depth = double(prj.scene.facetCount()) * double(run.periodicNeighbors + 1u) / pixels;
+        const uint32_t want = uint32_t(std::ceil(std::max(1.0, depth * 16.0)));
+        const uint32_t initial = prj.scene.rasterLayers;
+        while (prj.scene.rasterLayers < want && prj.scene.rasterLayers < 64u)
+            prj.scene.rasterLayers = std::min(64u, prj.scene.rasterLayers * 2u);
+        if (prj.scene.rasterLayers != initial)
+            std::cout << "facetrt: automatically increased A-buffer layers from " << initial << " to "
+                      << prj.scene.rasterLayers << " for " << prj.scene.facetCount() << " facets" << std::endl;
     }
     prj.facetComponent.resize(prj.scene.facetCount());
     for (uint32_t f = 0; f < prj.scene.facetCount(); ++f)

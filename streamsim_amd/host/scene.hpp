@@ -20,6 +20,10 @@ struct Scene {
     uint32_t leafFacetCount = 0;
     float extentX = 1.0f, extentZ = 1.0f;  // ground extent (the periodic tile)
     float backgroundAngularStrength = 0.0f;  // Hapke ground (core:2904-2910); 0: Lambert
+    // visibility raster of the reference's graph (visibilityRasterSize /
+    // visibilityRasterLayers, core:159-160): 1024 pixels square and 64
+    // fragments per pixel, 2048 and 16 (then adapted) for large projects
+    uint32_t rasterSize = 1024u, rasterLayers = 64u;
     uint32_t facetCount() const { return uint32_t(vertices.size() / 3); }
 };
 

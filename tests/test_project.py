@@ -6,6 +6,7 @@ models and position files) against the reference's own assembled scene
 radiosity_scene.obj), plus the loader's rules on small synthetic projects."""
 import json
 import shutil
+import struct
 import subprocess
 from pathlib import Path
 
@@ -162,7 +163,10 @@ def test_photovoltaic_material_loads(tmp_path):
     ("pv_ground", "Photovoltaic panels must be scene objects"),
     ("pv_params", "Invalid photovoltaic material parameters: pv_solid"),
     ("pv_opaque", "Photovoltaic surface must be opaque: pv_solid"),
-    ("dem", "DEM terrain is not read"),
+    ("dem", "Cannot open DEM"),
+    ("dem_packed", "compressed images are not read"),
+    ("dem_small", "DEM must contain at least 2 x 2 cells"),
+    ("dem_nodata", "DEM contains no valid elevations"),
     ("missing", "Cannot open scene OBJ"),
     ("name", "accepts project.json only"),
 ])
@@ -185,6 +189,15 @@ def test_project_errors(tmp_path, case, message):
                                       {"name": "pv_solid", "type": "Building", "energyModel": "photovoltaic"}])
     elif case == "dem":
         p = _write_project(tmp_path, [], scene={"terrain": True, "demFile": "terrain/dem.tif"})
+    elif case == "dem_packed":
+        p = _write_project(tmp_path, [], scene={"terrain": True, "demFile": "dem.tif"})
+        (p.parent / "dem.tif").write_bytes(_tiff(np.ones((3, 3), np.float32), compression=5))
+    elif case == "dem_small":
+        p = _write_project(tmp_path, [], scene={"terrain": True, "demFile": "dem.asc"})
+        (p.parent / "dem.asc").write_text(_asc(np.ones((1, 4))))
+    elif case == "dem_nodata":
+        p = _write_project(tmp_path, [], scene={"terrain": True, "demFile": "dem.asc"})
+        (p.parent / "dem.asc").write_text(_asc(np.full((2, 2), -9999.0)))
     elif case == "missing":
         p = _write_project(tmp_path, [{"name": "tree", "fileName": "models/none.obj"}])
     else:
@@ -192,3 +205,181 @@ def test_project_errors(tmp_path, case, message):
         p = p.rename(p.with_name("scene.json"))
     r = subprocess.run([str(EXE), "--scene-obj", str(p), str(tmp_path / "o.obj")], capture_output=True, text=True)
     assert r.returncode == 1 and message in r.stderr, r.stderr
+
+
+def _asc(grid, nodata=-9999.0):
+    """An ESRI ASCII grid, rows from north to south."""
+    h, w = grid.shape
+    rows = "\n".join(" ".join(f"{x:.6g}" for x in row) for row in grid)
+    return f"ncols {w}\nNROWS {h}\nxllcorner 0\nyllcorner 0\ncellsize 1\nNODATA_value {nodata:g}\n{rows}\n"
+
+
+def _tiff(grid, big=False, compression=1, tile=None, nodata=None):
+    """A baseline TIFF of one band (grid's dtype: float32 or int16), in
+    strips of 2 rows or in tiles of tile x tile pixels."""
+    e = ">" if big else "<"
+    h, w = grid.shape
+    fmt = 3 if grid.dtype == np.float32 else 2
+    bits = grid.dtype.itemsize * 8
+    data, offsets, counts = b"", [], []
+    head = 8
+    if tile:
+        for ty in range(0, h, tile):
+            for tx in range(0, w, tile):
+                block = np.zeros((tile, tile), grid.dtype)
+                part = grid[ty:ty + tile, tx:tx + tile]
+                block[:part.shape[0], :part.shape[1]] = part
+                offsets.append(head + len(data))
+                raw = block.astype(grid.dtype.newbyteorder(e)).tobytes()
+                counts.append(len(raw))
+                data += raw
+    else:
+        for y in range(0, h, 2):
+            offsets.append(head + len(data))
+            raw = grid[y:y + 2].astype(grid.dtype.newbyteorder(e)).tobytes()
+            counts.append(len(raw))
+            data += raw
+    extra = b""
+    ifd_at = head + len(data)
+    n_entries = 8 + (4 if tile else 3) + (1 if nodata is not None else 0)
+    after_ifd = ifd_at + 2 + 12 * n_entries + 4
+
+    def arr(values):
+        nonlocal extra
+        if len(values) == 1:
+            return 4, 1, struct.pack(e + "I", values[0])
+        off = after_ifd + len(extra)
+        extra += struct.pack(e + f"{len(values)}I", *values)
+        return 4, len(values), struct.pack(e + "I", off)
+
+    def short(v):
+        return 3, 1, struct.pack(e + "HH", v, 0)
+
+    entries = [(256, 4, 1, struct.pack(e + "I", w)), (257, 4, 1, struct.pack(e + "I", h)),
+               (258,) + short(bits), (259,) + short(compression), (262,) + short(1), (277,) + short(1),
+               (284,) + short(1), (339,) + short(fmt)]
+    if tile:
+        entries += [(322, 4, 1, struct.pack(e + "I", tile)), (323, 4, 1, struct.pack(e + "I", tile)),
+                    (324,) + arr(offsets), (325,) + arr(counts)]
+    else:
+        entries += [(273,) + arr(offsets), (278, 4, 1, struct.pack(e + "I", 2)), (279,) + arr(counts)]
+    if nodata is not None:
+        text = f"{nodata:g}".encode() + b"\0"
+        off = after_ifd + len(extra)
+        extra += text
+        entries.append((42113, 2, len(text), struct.pack(e + "I", off) if len(text) > 4 else text.ljust(4, b"\0")))
+    entries.sort()
+    assert len(entries) == n_entries
+    ifd = struct.pack(e + "H", len(entries))
+    for tag, typ, cnt, val in entries:
+        ifd += struct.pack(e + "HHI", tag, typ, cnt) + val
+    ifd += struct.pack(e + "I", 0)
+    return (b"II" if not big else b"MM") + struct.pack(e + "HI", 42, ifd_at) + data + ifd + extra
+
+
+def _ground_heights(grid, nodata, W, D, x, z):
+    """Bilinear DEM heights above the valid minimum at scene-centred x
+    (north), z (east): rows run from north (x = W / 2) to south, columns
+    from west (z = -D / 2) to east."""
+    from scipy.interpolate import RegularGridInterpolator
+    g = np.array(grid, np.float64)
+    valid = np.isfinite(g) & (g != nodata)
+    lo = g[valid].min()
+    g[~valid] = lo
+    h, w = g.shape
+    f = RegularGridInterpolator((np.arange(h), np.arange(w)), g)
+    v = (1 - np.clip((x + W / 2) / W, 0, 1)) * (h - 1)
+    u = np.clip((z + D / 2) / D, 0, 1) * (w - 1)
+    return np.maximum(0.0, f(np.stack([v, u], -1)) - lo)
+
+
+_GRID = (np.arange(20, dtype=np.float64).reshape(5, 4) * 0.7 + np.array([0.0, 1.5, -0.5, 3.0])
+         + np.array([[0], [2], [5], [9], [14]]))  # rows 5 (north to south), columns 4 (west to east)
+
+
+@pytest.mark.parametrize("fmt", ["asc", "tif_le", "tif_be", "tif_tiles", "tif_int16"])
+def test_dem_ground_follows_the_grid(tmp_path, fmt):
+    # a DEM stretched over the scene (readDemHeightField / heightAboveMinimum
+    # / appendProjectGround, core:2617-2697, 2830-2878): every ground vertex
+    # 1 cm below the bilinear height above the valid minimum (a no-data cell
+    # takes the minimum), the north-west corner on the grid's first cell and
+    # the south-east one on its last; a box stands on the terrain at its
+    # placement (core:2991-2992); the same scene from every encoding
+    _need_exe()
+    grid = _GRID.copy()
+    grid[2, 1] = -9999.0
+    name = "dem.asc" if fmt == "asc" else "dem.tif"
+    d = tmp_path / "prj"
+    d.mkdir()
+    if fmt == "asc":
+        (d / name).write_text(_asc(grid))
+    elif fmt == "tif_int16":
+        grid = np.round(grid).astype(np.int16)
+        (d / name).write_bytes(_tiff(grid, nodata=-9999))
+    else:
+        (d / name).write_bytes(_tiff(grid.astype(np.float32), big=fmt == "tif_be",
+                                     tile=16 if fmt == "tif_tiles" else None, nodata=-9999))
+    (d / "pos.txt").write_text("3 6 0\n")
+    p = _write_project(tmp_path, [{"name": "box", "type": "Building", "dimensions": [1, 1, 1],
+                                   "positionFile": "pos.txt"}],
+                       scene={"terrain": True, "demFile": name, "height": 50})
+    tris, r = _assemble(p, tmp_path)
+    ground = np.array([t for t, g in tris if g == "STREAMSIM_TERRAIN"])
+    box = np.array([t for t, g in tris if g == "object"])
+    assert ground.shape == (40, 3, 3)  # 5 x 4 cells of 2 m
+    pts = ground.reshape(-1, 3)
+    want = _ground_heights(grid, -9999.0, 10.0, 8.0, pts[:, 0], pts[:, 2]) - 0.01
+    np.testing.assert_allclose(pts[:, 1], want, rtol=0, atol=2e-5)
+    g = np.where(grid == -9999, np.inf, grid).astype(np.float64)
+    lo = g.min()
+    nw = pts[(pts[:, 0] == 5.0) & (pts[:, 2] == -4.0)]
+    se = pts[(pts[:, 0] == -5.0) & (pts[:, 2] == 4.0)]
+    assert np.allclose(nw[:, 1], grid[0, 0] - lo - 0.01, atol=1e-5)
+    assert np.allclose(se[:, 1], grid[-1, -1] - lo - 0.01, atol=1e-5)
+    # the box's base at its placement (x 3, y 6 -> centred -2, 2)
+    base = _ground_heights(grid, -9999.0, 10.0, 8.0, np.array([3.0 - 5.0]), np.array([6.0 - 4.0]))[0]
+    assert abs(box[:, :, 1].min() - base) < 1e-5 and abs(box[:, :, 1].max() - base - 1.0) < 1e-5
+
+
+def test_dem_library_grids(tmp_path):
+    # the reference's DEM library: central_hill_65x65.asc (65 x 65 cells,
+    # 0..20 m) and single_slope_60x60m.tif (60 x 60 float32 strips, GDAL_NODATA
+    # -9999), their elevations read here with numpy
+    _need_exe()
+    hill = np.loadtxt(GOLDEN / "dem" / "central_hill_65x65.asc", skiprows=6)
+    raw = (GOLDEN / "dem" / "single_slope_60x60m.tif").read_bytes()
+    slope = np.frombuffer(raw, "<f4", count=3600, offset=8).reshape(60, 60).astype(np.float64)
+    for name, grid in (("central_hill_65x65.asc", hill), ("single_slope_60x60m.tif", slope)):
+        d = tmp_path / name
+        d.mkdir()
+        p = _write_project(d, [], scene={"terrain": True, "demFile": name, "x": 30, "y": 20, "voxel": 1})
+        shutil.copy(GOLDEN / "dem" / name, p.parent / name)
+        tris, r = _assemble(p, d)
+        pts = np.array([t for t, g in tris if g == "STREAMSIM_TERRAIN"]).reshape(-1, 3)
+        assert len(pts) == 30 * 20 * 6
+        want = _ground_heights(grid, -9999.0, 30.0, 20.0, pts[:, 0], pts[:, 2]) - 0.01
+        np.testing.assert_allclose(pts[:, 1], want, rtol=0, atol=1e-4)
+        assert np.ptp(pts[:, 1]) > 1.0
+
+
+def test_large_scene_raster_and_adaptive_layers(tmp_path):
+    # scenes of 1000 m or more rasterise at 2048 pixels and 16 layers, and
+    # their ground grid at up to 512 cells a side (core:2895-2901,
+    # 2843-2849); the layers then double while 16 times the mean fragment
+    # depth over the raster, periodic copies included, exceeds them
+    # (adaptiveVisibilityRasterLayers, core:168-188)
+    _need_exe()
+    p = _write_project(tmp_path, [], scene={"x": 1200, "y": 1000, "voxel": 4})
+    tris, r = _assemble(p, tmp_path)
+    assert "raster 2048 layers 16" in r.stdout
+    assert len(tris) == 300 * 250 * 2  # 300 x 250 cells: above the small raster's 256 cap
+    p = _write_project(tmp_path, [], scene={"x": 100, "y": 100, "voxel": 0.25})
+    tris, r = _assemble(p, tmp_path)
+    assert "raster 1024 layers 64" in r.stdout and len(tris) == 256 * 256 * 2
+    # 104 135 cells -> 208 270 facets, 21 copies: 16 x 1.04 mean depth -> 17 > 16 layers -> 32
+    p = _write_project(tmp_path, [], scene={"x": 1200, "y": 1000, "voxel": 3.4},
+                       extra={"control": {"periodicBoundary": True, "periodicTraversalCount": 20}})
+    tris, r = _assemble(p, tmp_path)
+    assert len(tris) == 353 * 295 * 2
+    assert "automatically increased A-buffer layers from 16 to 32 for 208270 facets" in r.stdout
+    assert "raster 2048 layers 32" in r.stdout
