@@ -368,6 +368,16 @@ int fh_eb_get_photovoltaic(fh_context *ctx, float *incidentShortwave, float *abs
 /* Replaces the temperature state (surfaceCount values). */
 int fh_eb_set_temperature(fh_context *ctx, const float *temperature);
 
+/* Per-facet factors on the shortwave reflectance of the energy balance
+ * (reflectanceTextureFactor, core:848-851, applied in
+ * makeEnergyShortwaveOptics, core:965-972): a side's window reflectance is
+ * clamp(r_material * factor, 0, 1), the transmittance then limited to
+ * 1 - r.  factors: facetCount values, finite and >= 0; NULL removes them.
+ * Call after fh_eb_set_scene, which clears them.  No counterpart in the
+ * reference's backend API: its host loop builds the optics with the
+ * factors (core:952-1003). */
+int fh_eb_set_reflectance_factors(fh_context *ctx, const float *factors);
+
 /* The last node's shortwave windows solved again with the direct
  * irradiance removed: the reference's per-window diffuse-only solve,
  * solveRadiation(withoutDirectLight(windowOptics), windowDiffuse, ...)

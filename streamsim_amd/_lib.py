@@ -131,6 +131,7 @@ SIGNATURES = {
     "fh_eb_get_photovoltaic": (_I, [_P, _P, _P, _P]),
     "fh_eb_set_temperature": (_I, [_P, _P]),
     "fh_eb_diffuse_radiosity": (_I, [_P, _P, _P, C.POINTER(FhSolveStats)]),
+    "fh_eb_set_reflectance_factors": (_I, [_P, _P]),
     "fh_mlp_param_count": (_U32, [_U32]),
     "fh_mlp_set_params": (_I, [_P, _U32, _P, _P, _P, _F, _F]),
     "fh_mlp_get_params": (_I, [_P, _P]),

@@ -71,7 +71,7 @@ def build(jobs: int | None = None, verbose: bool = True) -> Path:
     # command-line engine (host C++ over the C-ABI), rpath to this folder
     cxx = shutil.which("g++") or "g++"
     cmd = [cxx, "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(PKG / "host" / "facet_engine.cpp"),
-           f"-L{PKG}", "-lfacet_hip", "-Wl,-rpath,$ORIGIN", "-o", str(CLI)]
+           f"-L{PKG}", "-lfacet_hip", "-lz", "-Wl,-rpath,$ORIGIN", "-o", str(CLI)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"facet_engine build failed:\n{res.stdout}\n{res.stderr}")

@@ -123,17 +123,18 @@ __global__ void k_lw_source(uint32_t S, const float* __restrict__ T, const float
 }
 
 // SW window optics per side and band (makeEnergyShortwaveOptics,
-// core:952-1003, no reflectance texture); the ground sides' reflectance is
-// the Hapke value for this sun (core:988-996): the Lambertian value clamped
-// to 0.999 at strength 0.
+// core:952-1003): the material's reflectance times the facet's texture
+// factor (when given), clamped to [0, 1]; the ground sides' reflectance is
+// the Hapke value of that for this sun (core:988-996): the Lambertian value
+// clamped to 0.999 at strength 0.
 // Internal band b is window order[b]; dni is given in internal order.
 __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_t* __restrict__ mat,
                             const float* __restrict__ refl, const float* __restrict__ trans,
                             const uint32_t* __restrict__ order, const float4* __restrict__ nrm,
                             const float* __restrict__ sunlit, float sx, float sy, float sz,
                             const float* __restrict__ dni, uint32_t leafCount, float strength,
-                            float* __restrict__ r, float* __restrict__ t, float* __restrict__ D,
-                            float* __restrict__ dir) {
+                            const float* __restrict__ factor, float* __restrict__ r, float* __restrict__ t,
+                            float* __restrict__ D, float* __restrict__ dir) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= size_t(S) * ldb) return;
     const uint32_t s = uint32_t(i / ldb), b = uint32_t(i - size_t(s) * ldb);
@@ -150,7 +151,8 @@ __global__ void k_sw_optics(uint32_t S, uint32_t nb, uint32_t ldb, const uint32_
         return;
     }
     const uint32_t m = mat[s >> 1], w = order[b];
-    float rr = fminf(fmaxf(refl[size_t(m) * nb + w], 0.0f), 1.0f);
+    const float mean = refl[size_t(m) * nb + w];
+    float rr = fminf(fmaxf(factor ? mean * factor[s >> 1] : mean, 0.0f), 1.0f);
     const float tt = fminf(fmaxf(trans[size_t(m) * nb + w], 0.0f), 1.0f - rr);
     if ((s >> 1) >= leafCount) {
         // ground sides: backgroundHapkeReflectance for every side (core:989-992);
@@ -765,6 +767,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     e.swRhoValid = false;
     e.guessPrevValid = false;
     e.lastNodeValid = false;
+    e.textured = false;
     if (swNb && c->hasGraph) {
         // spectral radius per internal band of this scene's SW operator (the
         // Chebyshev parameter): r and t do not depend on the sun, so the
@@ -773,7 +776,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
         launch_fill_f32(c, e.dni.p, ldb, 0.0f);
         hipLaunchKernelGGL(k_sw_optics, blocks_for(nsw), dim3(256), 0, c->stream, S, swNb, ldb, e.material.p,
                            e.swRefl.p, e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, 0.0f, 1.0f, 0.0f,
-                           e.dni.p, e.leafCount, c->hapkeStrength, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
+                           e.dni.p, e.leafCount, c->hapkeStrength, nullptr, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
         FH_HIP(hipGetLastError());
         e.swRho2.ensure(ldb);
         estimate_rho2(c, ldb, swNb, e.rsw.p, e.tsw.p, e.Bsw1.p, e.Hsw.p, e.csw.p, e.swRho2.p);
@@ -946,7 +949,8 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         // the spectral radius estimated again for every node
         const bool hapkeOn = c->hapkeStrength > 0.0f;
         if (hapkeOn) e.swRhoValid = false;
-        const bool tab = !noTables && !hapkeOn && !e.groundClamp && uses_bf16_gather(ldb);
+        // per-facet reflectance factors (textures) also take the per-side arrays
+        const bool tab = !noTables && !hapkeOn && !e.groundClamp && !e.textured && uses_bf16_gather(ldb);
         const uint32_t precision = bf16_pays(ldb) ? p.precision : FH_PRECISION_FP32;
         // an increment solve keeps the gathers of every band's last sweep:
         // the absorbed pass forms the incident from them
@@ -954,7 +958,8 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         const auto optics_arrays = [&] {
             hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p,
                                e.swRefl.p, e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, p.sun.x, p.sun.y,
-                               p.sun.z, e.dni.p, e.leafCount, c->hapkeStrength, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
+                               p.sun.z, e.dni.p, e.leafCount, c->hapkeStrength, e.textured ? e.reflFactor.p : nullptr,
+                               e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
             FH_HIP(hipGetLastError());
         };
         if (!tab || !e.swRhoValid) {
@@ -1140,7 +1145,8 @@ void eb_diffuse_solve(fh_context* c, float* sumHost, float* bandHost, fh_solve_s
     launch_fill_f32(c, e.dni.p, ldb, 0.0f);
     hipLaunchKernelGGL(k_sw_optics, blocks_for(n), dim3(256), 0, c->stream, S, nb, ldb, e.material.p, e.swRefl.p,
                        e.swTrans.p, e.swOrderD.p, c->facetNormal.p, e.sunlit.p, e.lastSun[0], e.lastSun[1],
-                       e.lastSun[2], e.dni.p, e.leafCount, c->hapkeStrength, e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
+                       e.lastSun[2], e.dni.p, e.leafCount, c->hapkeStrength, e.textured ? e.reflFactor.p : nullptr,
+                       e.rsw.p, e.tsw.p, e.Dsw.p, e.swDir.p);
     FH_HIP(hipGetLastError());
     set_sky(c, e.lastDiffuse.data(), nb, ldb);
     launch_fill_f32(c, e.Hsw.p, n, 0.0f);
@@ -1330,6 +1336,25 @@ int fh_eb_set_temperature(fh_context* c, const float* T) {
         FH_HIP(hipStreamSynchronize(c->stream));
         c->eb.tValid = true;
         c->eb.guessPrevValid = false;  // T no longer follows the warm-started chain
+    });
+}
+
+int fh_eb_set_reflectance_factors(fh_context* c, const float* factors) {
+    return guarded_eb(c, [&] {
+        fh::EbState& e = c->eb;
+        if (!e.ready || e.facets != c->F)
+            fail(FH_LOGIC_ERROR, "fh_eb_set_scene() must be called before fh_eb_set_reflectance_factors()");
+        if (factors)
+            for (uint32_t f = 0; f < c->F; ++f)
+                if (!std::isfinite(factors[f]) || factors[f] < 0.0f)
+                    fail(FH_INVALID_ARGUMENT, "reflectance factors must be finite and non-negative");
+        e.textured = factors != nullptr;
+        if (factors) {
+            e.reflFactor.ensure(c->F);
+            FH_HIP(hipMemcpyAsync(e.reflFactor.p, factors, sizeof(float) * c->F, hipMemcpyHostToDevice, c->stream));
+            FH_HIP(hipStreamSynchronize(c->stream));
+        }
+        e.swRhoValid = false;  // the operator's reflectances changed
     });
 }
 

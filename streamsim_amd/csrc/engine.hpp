@@ -119,6 +119,8 @@ struct EbState {
     uint32_t nMat = 0;
     uint32_t swNb = 0, swLdb = 0;
     bool groundClamp = false;  // a ground material's SW reflectance exceeds 0.999 (per-side arrays)
+    bool textured = false;     // reflFactor scales the SW reflectance per facet (per-side arrays)
+    DBuf<float> reflFactor;    // F
     bool swValid = false;  // Bsw holds the previous node's SW solution
     bool lwValid = false;  // Blw holds a LW solution
     bool tValid = false;   // T holds a state

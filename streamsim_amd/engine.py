@@ -304,6 +304,14 @@ class FacetEngine:
                                                      _ptr(out["absorbedShortwave"]), _ptr(out["power"])))
         return out
 
+    def ebSetReflectanceFactors(self, factors=None) -> None:
+        """Per-facet factors on the shortwave reflectance (texture factors,
+        facetrtcore.cpp:848-851, 965-972); None removes them."""
+        f = None if factors is None else _f32(factors)
+        if f is not None and f.size != self.facetCount():
+            raise ValueError("one reflectance factor per facet is required")
+        self._check(self._lib.fh_eb_set_reflectance_factors(self._h, _ptr(f)))
+
     def ebDiffuseRadiosity(self) -> tuple[np.ndarray, np.ndarray, FhSolveStats]:
         """The last node's windows solved without the direct irradiance
         (withoutDirectLight, facetrtcore.cpp:1561-1569, 3533-3536): per side

@@ -324,7 +324,7 @@ int runFacetRTProject(const std::string& input, const std::string& outFile) try 
                 } else {
                     const float rb = m.bandR.empty() ? m.r : m.bandR[std::min<size_t>(b, m.bandR.size() - 1)];
                     const float tb = m.bandT.empty() ? m.t : m.bandT[std::min<size_t>(b, m.bandT.size() - 1)];
-                    refl = std::clamp(rb, 0.0f, 1.0f);
+                    refl = std::clamp(rb * scene.reflectanceFactorOf(f), 0.0f, 1.0f);  // texture (core:1085-1088)
                     trans = std::clamp(tb, 0.0f, 1.0f - refl);
                 }
                 float eps = std::max(0.0f, 1.0f - refl - trans);
@@ -1008,6 +1008,8 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
                                     matList.data(), nw, swR.data(), swT.data()),
                     ctx);
     facethip::check(fh_eb_set_background_angular_strength(ctx, scene.backgroundAngularStrength), ctx);
+    if (!scene.reflectanceFactor.empty())  // texture factors on the windows' reflectance (core:965-972)
+        facethip::check(fh_eb_set_reflectance_factors(ctx, scene.reflectanceFactor.data()), ctx);
 
     // sensor bands solved after every node (core:3862-3908): the project's
     // sensor bands, else the default 550 nm (core:141, 2389-2400); per
@@ -1168,7 +1170,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
                         const size_t i = side * nbs + b;
                         const bool optical = opt[b] != 0;
                         const float meanR = optical ? sm.r[std::min<size_t>(b, sm.r.size() - 1)] : sm.tirR;
-                        float refl = std::clamp(meanR, 0.0f, 1.0f);
+                        float refl = std::clamp(meanR * (optical ? scene.reflectanceFactorOf(f) : 1.0f), 0.0f, 1.0f);
                         const float trans = std::clamp(optical ? sm.t[std::min<size_t>(b, sm.t.size() - 1)] : sm.tirT,
                                                        0.0f, 1.0f - refl);
                         float eps = std::max(0.0f, 1.0f - refl - trans);
@@ -1363,18 +1365,45 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
 }
 
 // --solar-position Y M D h m s timezone longitude latitude pressure temperature
-// facet_engine --scene-obj project.json scene.obj: the assembled facet scene
-// of a project (object facets, then the ground) as OBJ
+// facet_engine --scene-obj project.json scene.obj [factors.bin]: the
+// assembled facet scene of a project (object facets, then the ground) as
+// OBJ, and optionally its texture factors (float32 per facet)
 int runSceneObj(int argc, char** argv) try {
     if (argc < 4) {
-        std::cerr << "Usage: facet_engine --scene-obj project.json scene.obj\n";
+        std::cerr << "Usage: facet_engine --scene-obj project.json scene.obj [factors.bin]\n";
         return 2;
     }
     const facetcli::Project prj = facetcli::loadProject(argv[2]);
     facetcli::writeSceneObj(prj.scene, argv[3]);
+    if (argc > 4) {
+        std::vector<float> k(prj.scene.facetCount());
+        for (uint32_t f = 0; f < prj.scene.facetCount(); ++f) k[f] = prj.scene.reflectanceFactorOf(f);
+        std::ofstream out(argv[4], std::ios::binary | std::ios::trunc);
+        out.write(reinterpret_cast<const char*>(k.data()), std::streamsize(k.size() * sizeof(float)));
+        if (!out) throw std::runtime_error(std::string("cannot write ") + argv[4]);
+    }
     std::cout << "facets " << prj.scene.facetCount() << " objects " << prj.scene.leafFacetCount << " materials "
               << prj.materials.size() << " raster " << prj.scene.rasterSize << " layers " << prj.scene.rasterLayers
               << std::endl;
+    return 0;
+} catch (const std::exception& e) {
+    std::cerr << "facet_engine: " << e.what() << '\n';
+    return 1;
+}
+
+// facet_engine --decode-image image out.raw: the image's samples as a
+// texture read takes them (interleaved bytes), "width height channels" on
+// stdout
+int runDecodeImage(int argc, char** argv) try {
+    if (argc != 4) {
+        std::cerr << "Usage: facet_engine --decode-image image out.raw\n";
+        return 2;
+    }
+    const facetimg::Image img = facetimg::readImage(argv[2]);
+    std::ofstream out(argv[3], std::ios::binary | std::ios::trunc);
+    out.write(reinterpret_cast<const char*>(img.data.data()), std::streamsize(img.data.size()));
+    if (!out) throw std::runtime_error("cannot write " + std::string(argv[3]));
+    std::cout << img.width << ' ' << img.height << ' ' << img.channels << std::endl;
     return 0;
 } catch (const std::exception& e) {
     std::cerr << "facet_engine: " << e.what() << '\n';
@@ -1424,6 +1453,7 @@ int main(int argc, char** argv) {
     if (mode == "--solar-position") return runSolarPosition(argc, argv);
     if (mode == "--scene-obj") return runSceneObj(argc, argv);
     if (mode == "--energy-windows") return runEnergyWindows(argc, argv);
+    if (mode == "--decode-image") return runDecodeImage(argc, argv);
     const std::string input = argc > 2 ? argv[2] : "";
     const std::string output = argc > 3 ? argv[3] : "";
     if (mode != "eFacetRT" && mode != "eFacetEB") {

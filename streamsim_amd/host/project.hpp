@@ -26,22 +26,94 @@
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+#include <functional>
 #include <iomanip>
 #include <iostream>
 #include <iterator>
 #include <limits>
+#include <map>
+#include <memory>
 #include <set>
 #include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "image.hpp"
 #include "json.hpp"
 #include "scene.hpp"
 
 namespace facetcli {
 
 constexpr uint32_t kEnergySamples = 2001;  // 1-nm samples from 400 nm (core N1)
+
+// Reflectance-variance texture (ReflectanceVarianceTexture and
+// loadReflectanceVarianceTexture, core:88-111, 623-702): the image's first
+// three bands (one band: grey) read as sRGB, converted to linear luminance
+// 0.2126 R + 0.7152 G + 0.0722 B, the image mean removed and the values
+// divided by the largest |deviation|; sampled bilinearly, wrapping in both
+// directions.
+struct VarianceTexture {
+    int width = 0, height = 0;
+    std::vector<float> values;
+
+    float sample(float u, float v) const {
+        if (width <= 0 || height <= 0 || values.empty()) return 0.0f;
+        const float x = (u - std::floor(u)) * float(width), y = (v - std::floor(v)) * float(height);
+        const int x0 = int(std::floor(x)) % width, y0 = int(std::floor(y)) % height;
+        const int x1 = (x0 + 1) % width, y1 = (y0 + 1) % height;
+        const float fx = x - std::floor(x), fy = y - std::floor(y);
+        const auto at = [&](int c, int r) { return values[size_t(r) * size_t(width) + size_t(c)]; };
+        const float top = at(x0, y0) * (1.0f - fx) + at(x1, y0) * fx;
+        const float bottom = at(x0, y1) * (1.0f - fx) + at(x1, y1) * fx;
+        return top * (1.0f - fy) + bottom * fy;
+    }
+};
+
+inline float srgbToLinear(float v) {
+    v = std::clamp(v, 0.0f, 1.0f);
+    return v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f);
+}
+
+inline std::shared_ptr<const VarianceTexture> loadVarianceTexture(const std::filesystem::path& path) {
+    static std::map<std::string, std::weak_ptr<const VarianceTexture>> cache;
+    const std::string key = std::filesystem::absolute(path).lexically_normal().string();
+    if (auto it = cache.find(key); it != cache.end())
+        if (auto t = it->second.lock()) return t;
+    facetimg::Image img;
+    try {
+        img = facetimg::readImage(key);
+    } catch (const std::exception& e) {
+        throw std::runtime_error("Cannot open reflectance variance texture: " + key + " (" + e.what() + ")");
+    }
+    if (img.width < 2 || img.height < 2)
+        throw std::runtime_error("Reflectance variance texture must contain at least 2 x 2 pixels: " + key);
+    auto t = std::make_shared<VarianceTexture>();
+    t->width = img.width;
+    t->height = img.height;
+    const size_t n = size_t(img.width) * size_t(img.height);
+    const int ch = std::min(3, img.channels);
+    t->values.resize(n);
+    double mean = 0.0;
+    for (size_t p = 0; p < n; ++p) {
+        const uint8_t* px = &img.data[p * size_t(img.channels)];
+        const float red = srgbToLinear(px[0] / 255.0f);
+        const float green = ch > 1 ? srgbToLinear(px[1] / 255.0f) : red;
+        const float blue = ch > 2 ? srgbToLinear(px[2] / 255.0f) : red;
+        t->values[p] = 0.2126f * red + 0.7152f * green + 0.0722f * blue;
+        mean += t->values[p];
+    }
+    mean /= double(n);
+    float largest = 0.0f;
+    for (float& v : t->values) {
+        v -= float(mean);
+        largest = std::max(largest, std::abs(v));
+    }
+    if (!(largest > 1.0e-6f)) throw std::runtime_error("Reflectance variance texture is effectively constant: " + key);
+    for (float& v : t->values) v /= largest;
+    cache[key] = t;
+    return t;
+}
 
 // One material of a project scene: the fields of the reference's
 // SurfaceParameters (core:113-143) this engine uses, with its defaults.
@@ -59,6 +131,9 @@ struct ProjectMaterial {
     bool pv = false;  // energyModel "photovoltaic"
     float pvEta25 = 0.22f, pvGamma = -0.0035f, pvBifaciality = 0.0f;
     uint8_t component = 0;  // 0 unknown, 1 soil, 2 vegetation, 3 building, 4 photovoltaic, 5 water
+    // reflectance-variance texture of the spectral entry (physicalTexture)
+    std::shared_ptr<const VarianceTexture> texture;
+    float textureStrength = 0.0f, textureRepeat = 1.0f;
 
     // shortwave reflectance / transmittance of an energy window whose centre
     // is the 1-nm sample k (makeEnergyShortwaveOptics, core:952-1003, with a
@@ -207,7 +282,8 @@ inline const json::Value* named(const json::Value& list, const std::string& name
 // thermal entry of the given names; a name without an entry leaves the
 // material as it is
 inline void surfaceParameters(const json::Value& cfg, const std::string& spectralName, const std::string& thermalName,
-                              const std::vector<float>& bands, ProjectMaterial& m) {
+                              const std::vector<float>& bands, ProjectMaterial& m,
+                              const std::function<std::filesystem::path(const std::string&)>& resolve = {}) {
     const json::Value* sp = named(json::member(cfg, "spectra", json::Value::Array), spectralName);
     if (sp) {
         const std::string model = json::str(*sp, "model", "custom");
@@ -250,6 +326,18 @@ inline void surfaceParameters(const json::Value& cfg, const std::string& spectra
         m.tirR = json::num(*sp, "refTir", 0.05f);
         m.tirT = json::num(*sp, "tauTir", 0.0f);
         m.label = spectralName;
+        // physicalTexture (facetInputView core:1977-1994, read at core:2320-2344)
+        m.texture.reset();
+        m.textureStrength = 0.0f;
+        m.textureRepeat = 1.0f;
+        const json::Value& tex = json::member(*sp, "physicalTexture", json::Value::Object);
+        if (json::boolean(tex, "enabled", false)) {
+            const std::string file = json::str(tex, "fileName");
+            if (file.empty()) throw std::runtime_error("Physical texture is enabled but has no file: " + spectralName);
+            m.textureStrength = std::clamp(json::num(tex, "strength", 0.2f), 0.0f, 1.0f);
+            m.textureRepeat = std::max(0.01f, json::num(tex, "repeatSize", 2.0f));
+            m.texture = loadVarianceTexture(resolve ? resolve(file) : std::filesystem::path(file));
+        }
     }
     const json::Value* th = named(json::member(cfg, "thermals", json::Value::Array), thermalName);
     if (th) {
@@ -528,6 +616,100 @@ inline DemGrid readDem(const std::filesystem::path& path) {
     return g;
 }
 
+// Texture factors per facet (finalizeReflectanceTextureFactors,
+// core:718-846): for the facets of a textured material of positive
+// strength, the texture sampled at the centroid by triplanar projection
+// (weights |n_x|, |n_y|, |n_z| on the (z, y), (x, z), (x, y) planes, in
+// units of the repeat size), less its area-weighted mean over the
+// material; the deviations scaled by the strength, limited so that no
+// factor falls to 0.001 or lifts any band of the material above r + t = 1
+// (r max(r) factor <= 1 - t, over the band and energy spectra), plus one.
+// The area-weighted mean factor of a material is then exactly one.
+inline void textureFactors(Scene& scene, const std::vector<ProjectMaterial>& mats) {
+    const uint32_t F = scene.facetCount();
+    scene.reflectanceFactor.assign(F, 1.0f);
+    if (!F || mats.empty()) return;
+    struct Stats {
+        double area = 0.0, weighted = 0.0, factorSum = 0.0;
+        float minDev = 0.0f, maxDev = 0.0f, scale = 0.0f;
+        float minFactor = std::numeric_limits<float>::infinity(), maxFactor = std::numeric_limits<float>::lowest();
+    };
+    std::vector<Stats> st(mats.size());
+    std::vector<float> sample(F, 0.0f), area(F, 0.0f);
+    for (uint32_t f = 0; f < F; ++f) {
+        if (f >= scene.material.size() || scene.material[f] >= mats.size()) continue;
+        const ProjectMaterial& m = mats[scene.material[f]];
+        if (!m.texture || !(m.textureStrength > 0.0f)) continue;
+        const facethip::Vertex& a = scene.vertices[3 * size_t(f)];
+        const facethip::Vertex& b = scene.vertices[3 * size_t(f) + 1];
+        const facethip::Vertex& c = scene.vertices[3 * size_t(f) + 2];
+        const float u[3] = {b.position[0] - a.position[0], b.position[1] - a.position[1], b.position[2] - a.position[2]};
+        const float w[3] = {c.position[0] - a.position[0], c.position[1] - a.position[1], c.position[2] - a.position[2]};
+        const float cx = u[1] * w[2] - u[2] * w[1], cy = u[2] * w[0] - u[0] * w[2], cz = u[0] * w[1] - u[1] * w[0];
+        area[f] = 0.5f * std::sqrt(cx * cx + cy * cy + cz * cz);
+        const float px = (a.position[0] + b.position[0] + c.position[0]) / 3.0f;
+        const float py = (a.position[1] + b.position[1] + c.position[1]) / 3.0f;
+        const float pz = (a.position[2] + b.position[2] + c.position[2]) / 3.0f;
+        const float nx = std::abs(a.normal[0]), ny = std::abs(a.normal[1]), nz = std::abs(a.normal[2]);
+        const float wsum = std::max(1.0e-6f, nx + ny + nz);
+        const float k = 1.0f / std::max(0.01f, m.textureRepeat);
+        const VarianceTexture& t = *m.texture;
+        sample[f] = (nx * t.sample(pz * k, py * k) + ny * t.sample(px * k, pz * k) + nz * t.sample(px * k, py * k)) /
+                    wsum;
+        Stats& s = st[scene.material[f]];
+        s.area += area[f];
+        s.weighted += double(area[f]) * sample[f];
+    }
+    for (uint32_t f = 0; f < F; ++f) {
+        if (!(area[f] > 0.0f)) continue;
+        Stats& s = st[scene.material[f]];
+        const float mean = s.area > 0.0 ? float(s.weighted / s.area) : 0.0f;
+        sample[f] -= mean;
+        s.minDev = std::min(s.minDev, sample[f]);
+        s.maxDev = std::max(s.maxDev, sample[f]);
+    }
+    for (size_t i = 0; i < mats.size(); ++i) {
+        Stats& s = st[i];
+        if (!(s.area > 0.0)) continue;
+        const ProjectMaterial& m = mats[i];
+        float scale = std::clamp(m.textureStrength, 0.0f, 1.0f);
+        if (s.minDev < 0.0f) scale = std::min(scale, 0.999f / -s.minDev);
+        // maximumMeanPreservingFactor (core:743-762)
+        float maxFactor = std::numeric_limits<float>::infinity();
+        const auto bands = [&](const std::vector<float>& r, const std::vector<float>& t) {
+            for (size_t b = 0; b < r.size(); ++b) {
+                const float base = std::clamp(r[b], 0.0f, 1.0f);
+                if (!(base > 1.0e-6f)) continue;
+                const float tau = t.empty() ? 0.0f : std::clamp(t[std::min(b, t.size() - 1)], 0.0f, 1.0f - base);
+                maxFactor = std::min(maxFactor, (1.0f - tau) / base);
+            }
+        };
+        bands(m.bandR.empty() ? std::vector<float>{m.r} : m.bandR, m.bandT.empty() ? std::vector<float>{m.t} : m.bandT);
+        bands(m.energyR, m.energyT);
+        if (s.maxDev > 0.0f && std::isfinite(maxFactor))
+            scale = std::min(scale, std::max(0.0f, maxFactor - 1.0f) / s.maxDev);
+        s.scale = scale;
+    }
+    for (uint32_t f = 0; f < F; ++f) {
+        if (!(area[f] > 0.0f)) continue;
+        Stats& s = st[scene.material[f]];
+        const float factor = 1.0f + s.scale * sample[f];
+        scene.reflectanceFactor[f] = factor;
+        s.factorSum += double(area[f]) * factor;
+        s.minFactor = std::min(s.minFactor, factor);
+        s.maxFactor = std::max(s.maxFactor, factor);
+    }
+    bool any = false;
+    for (size_t i = 0; i < st.size(); ++i)
+        if (st[i].area > 0.0) {
+            any = true;
+            std::cout << "facetrt: physical texture material " << i << ", area-weighted factor mean="
+                      << st[i].factorSum / st[i].area << ", range=[" << st[i].minFactor << ',' << st[i].maxFactor
+                      << "]" << std::endl;
+        }
+    if (!any) scene.reflectanceFactor.clear();  // no textured facet: 1 everywhere
+}
+
 struct Placement {
     float x = 0.0f, y = 0.0f, z = 0.0f, scale = 1.0f, rotation = 0.0f;
 };
@@ -629,6 +811,7 @@ inline Project loadProject(const std::string& file) {
     const json::Value& cfg = json::member(root, "configuration", json::Value::Object);
     if (cfg.members.empty()) throw std::runtime_error("project.json has no configuration object");
     const fs::path dir = path.parent_path();
+    const auto resolve = [&](const std::string& v) { return pd::asset(dir, root, v); };
     const json::Value& sc = json::member(cfg, "scene", json::Value::Object);
     const json::Value& light = json::member(cfg, "light", json::Value::Object);
     const json::Value& sensor = json::member(cfg, "sensor", json::Value::Object);
@@ -721,7 +904,7 @@ inline Project loadProject(const std::string& file) {
         const std::string sp = pd::names(pd::joined(o, "spectralName", json::str(o, "spectralName", "soil"))).front();
         const std::string th =
             pd::names(pd::joined(o, "thermalName", json::str(o, "thermalName", "soil_temperature"))).front();
-        pd::surfaceParameters(cfg, sp, th, run.bands, objectDefault);
+        pd::surfaceParameters(cfg, sp, th, run.bands, objectDefault, resolve);
     }
     ProjectMaterial ground;
     ground.r = 0.20f;
@@ -731,7 +914,7 @@ inline Project loadProject(const std::string& file) {
     ground.shadedT = 295.0f;
     ground.component = 1;
     pd::surfaceParameters(cfg, json::str(background, "spectralName", "soil"),
-                          json::str(background, "thermalName", "soil_temperature"), run.bands, ground);
+                          json::str(background, "thermalName", "soil_temperature"), run.bands, ground, resolve);
     {  // loadSoilParameters (core:2368-2383) from the background's soil set
         const std::string soilName = json::str(background, "materialName", "soilset");
         for (const json::Value& m : mats.items) {
@@ -771,7 +954,7 @@ inline Project loadProject(const std::string& file) {
             m.component = component;
             const std::string sp = spectral.empty() ? std::string{} : spectral[std::min(i, spectral.size() - 1)];
             const std::string th = thermal.empty() ? std::string{} : thermal[std::min(i, thermal.size() - 1)];
-            pd::surfaceParameters(cfg, sp, th, run.bands, m);
+            pd::surfaceParameters(cfg, sp, th, run.bands, m, resolve);
             const std::string en = energy.empty() ? std::string{} : energy[std::min(i, energy.size() - 1)];
             if (const json::Value* e = pd::named(mats, en)) {
                 // surfaceEnergy (core:2015-2026, 2953-2973)
@@ -861,6 +1044,7 @@ inline Project loadProject(const std::string& file) {
             std::cout << "facetrt: automatically increased A-buffer layers from " << initial << " to "
                       << prj.scene.rasterLayers << " for " << prj.scene.facetCount() << " facets" << std::endl;
     }
+    pd::textureFactors(prj.scene, prj.materials);
     prj.facetComponent.resize(prj.scene.facetCount());
     for (uint32_t f = 0; f < prj.scene.facetCount(); ++f)
         prj.facetComponent[f] = f < prj.scene.leafFacetCount ? prj.materials[prj.scene.material[f]].component : 1;

@@ -24,7 +24,12 @@ struct Scene {
     // visibilityRasterLayers, core:159-160): 1024 pixels square and 64
     // fragments per pixel, 2048 and 16 (then adapted) for large projects
     uint32_t rasterSize = 1024u, rasterLayers = 64u;
+    // per facet: factor on the material's shortwave reflectance from its
+    // reflectance-variance texture (reflectanceTextureFactors, core:152);
+    // empty: 1 everywhere
+    std::vector<float> reflectanceFactor;
     uint32_t facetCount() const { return uint32_t(vertices.size() / 3); }
+    float reflectanceFactorOf(uint32_t f) const { return f < reflectanceFactor.size() ? reflectanceFactor[f] : 1.0f; }
 };
 
 // appendTriangle (core:288-306): normal of cross(b - a, c - a) in float;

@@ -441,7 +441,8 @@ def _project_scene(exe, prj, tmp_path):
     with the facet normal as appendTriangle computes it (float32 cross(b - a,
     c - a) times 1 / sqrt of its squared length), normals, material per facet."""
     sobj = tmp_path / "scene.obj"
-    subprocess.run([str(exe), "--scene-obj", str(prj), str(sobj)], check=True, capture_output=True)
+    subprocess.run([str(exe), "--scene-obj", str(prj), str(sobj), str(tmp_path / "scene.factors")], check=True,
+                   capture_output=True)
     verts, mat, cur = [], [], None
     for line in open(sobj):
         t = line.split()
@@ -463,7 +464,7 @@ def _project_scene(exe, prj, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("solver", ["traditional", "accelerated", "hapke", "dem"])
+@pytest.mark.parametrize("solver", ["traditional", "accelerated", "hapke", "dem", "texture"])
 def test_cli_facetrt_project_bands(oracle, tmp_path, solver):
     # eFacetRT on the validation project (sensor bands 550, 850, 10500 nm),
     # against the oracle: the same assembled scene (--scene-obj), its ray
@@ -475,7 +476,10 @@ def test_cli_facetrt_project_bands(oracle, tmp_path, solver):
     # absolute change is 1e-4, warm-started from the previous band of the
     # same kind (core:1706-1724): the same fixed point in fewer sweeps.
     # "dem": the scene on the reference's central-hill DEM (objects placed on
-    # the terrain, the ground grid following it, core:2652-2697, 2830-2878)
+    # the terrain, the ground grid following it, core:2652-2697, 2830-2878).
+    # "texture": the concrete of the building and the soil with the library's
+    # concrete reflectance-variance texture (strength 0.5, 1.5 m repeat): the
+    # optical reflectance times each facet's factor (core:1085-1088)
     print(f"golden cli rt project {solver}", flush=True)
     import os
     import shutil
@@ -492,6 +496,12 @@ def test_cli_facetrt_project_bands(oracle, tmp_path, solver):
         elif solver == "dem":
             shutil.copy(HERE / "dem" / "central_hill_65x65.asc", tmp_path / "prj" / "hill.asc")
             d["configuration"]["scene"].update(terrain=True, demFile="hill.asc")
+        elif solver == "texture":
+            shutil.copy(HERE / "textures" / "concrete_variance.jpg", tmp_path / "prj" / "concrete.jpg")
+            for sp in d["configuration"]["spectra"]:
+                if sp["name"] in ("concrete", "soil"):
+                    sp["physicalTexture"] = {"enabled": True, "fileName": "concrete.jpg", "strength": 0.5,
+                                             "repeatSize": 1.5}
         else:  # the Hapke background (core:1942-1943, 2904-2910)
             d["configuration"]["scene"].setdefault("background", {}).update(
                 heterogeneityEnabled=True, angularEffectStrength=strength)
@@ -509,7 +519,12 @@ def test_cli_facetrt_project_bands(oracle, tmp_path, solver):
     got = np.array(res["bandRadiosity"], np.float64).reshape(3, S).T
     # the assembled scene and its material per facet
     v, nrm, mat = _project_scene(exe, prj, tmp_path)
-    assert len(mat) == F
+    factor = np.fromfile(tmp_path / "scene.factors", np.float32).astype(np.float64)
+    assert len(mat) == F == factor.size
+    if solver == "texture":  # the building's and the ground's facets vary, the trees' do not
+        assert np.all(factor[:126] == 1.0) and np.ptp(factor[126:138]) > 0.01 and np.ptp(factor[138:]) > 0.05
+    else:
+        assert np.all(factor == 1.0)
     if solver == "dem":  # the ground rises over the hill (the box top clips trees on it: more facets)
         L = res["leafFacetCount"]
         assert L > 138 and v[3 * L:, 1].max() > 10.0 and np.ptp(nrm[L:, 1]) > 0.01
@@ -525,8 +540,8 @@ def test_cli_facetrt_project_bands(oracle, tmp_path, solver):
     nb = 3
     R = np.zeros((S, nb)); T = np.zeros((S, nb)); E = np.zeros((S, nb)); D = np.zeros((S, nb))
     for b in range(2):
-        R[:, b] = p[:, 0]
-        T[:, b] = np.minimum(p[:, 1], 1 - p[:, 0])
+        R[:, b] = np.clip(p[:, 0] * np.repeat(factor, 2), 0, 1)
+        T[:, b] = np.minimum(p[:, 1], 1 - R[:, b])
         D[:, b] = sl * cos * 700.0
     R[:, 2] = p[:, 2]
     T[:, 2] = p[:, 3]

@@ -417,6 +417,54 @@ def test_eb_node_matches_oracle(oracle, nb):
     assert _rel(s["longwaveRadiosity"], Blw) < RTOL
 
 
+# Reflectance-variance texture factors (core:848-851, 965-972) on the
+# energy balance's windows: per facet r = clamp(r_material * factor, 0, 1),
+# t limited to 1 - r, on the per-side optics path; against the oracle's
+# solve with those optics; removing the factors restores the plain node
+@pytest.mark.parametrize("nb", [4, 24])
+def test_eb_node_texture_factors_match_oracle(oracle, nb):
+    print(f"eb texture factors nb={nb}", flush=True)
+    sc, eng, win = _eb_case(oracle, nb=nb)
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    rng = np.random.default_rng(11)
+    factor = rng.uniform(0.5, 1.6, sc.facet_count).astype(np.float32)
+    factor[:5] = 0.0
+    eng.ebSetReflectanceFactors(factor)
+    with pytest.raises(ValueError):
+        eng.ebSetReflectanceFactors(factor[:-1])
+    cfg = faceteb.EbConfig(tolerance=1e-3, radiosity_tolerance=1e-7, max_coupling=200)
+    met = faceteb.diurnal_meteo(24)[12]
+    dni, dif = win.irradiance(met.shortwave, math.cos(math.radians(met.zenith)))
+    st = eng.ebRunNode(faceteb.node_params(met, cfg, first=True), dni, dif)
+    assert st.converged
+    s = eng.ebGetState()
+    r_tab, t_tab = faceteb.material_tables(["leaf", "soil", "building"], win)
+    mat = np.repeat(sc.material, 2)
+    r = np.clip(r_tab[mat] * np.repeat(factor, 2)[:, None], 0, 1)
+    t = np.minimum(np.clip(t_tab[mat], 0, 1), 1 - r)
+    ground = np.repeat(np.arange(sc.facet_count) >= sc.leaf_count, 2)
+    r[ground] = np.minimum(r[ground], 0.999)
+    t = np.minimum(t, 1 - r)
+    sun = faceteb.sun_vector(met.zenith, met.azimuth)
+    cos = np.maximum(0.0, optics.side_normals(sc) @ sun).astype(np.float32)
+    D = (np.clip(s["sunlit"], 0, 1) * cos)[:, None] * dni[None, :]
+    sw, _, _ = oracle.solve_bands(og, r, t, np.zeros_like(r), D, dif, tol_rel=1e-9)
+    assert _rel(s["shortwaveRadiosity"][sw > 1e-3], sw[sw > 1e-3]) < RTOL
+    H = oracle.incident_bands(og, sw, D, dif)
+    absorbed = np.sum(np.clip(1 - r - t, 0, 1) * H, axis=1)
+    assert _rel(s["absorbedShortwave"][absorbed > 1], absorbed[absorbed > 1]) < RTOL
+    _check_diffuse_solve(oracle, eng, og, r, t, dif, s)
+    # without the factors: the material's own reflectance again
+    eng.ebSetReflectanceFactors(None)
+    eng.ebRunNode(faceteb.node_params(met, cfg, first=True), dni, dif)
+    plain = eng.ebGetState()["shortwaveRadiosity"]
+    r0 = r_tab[mat].copy()
+    r0[ground] = np.minimum(r0[ground], 0.999)
+    t0 = np.minimum(np.clip(t_tab[mat], 0, 1), 1 - r0)
+    sw0, _, _ = oracle.solve_bands(og, r0, t0, np.zeros_like(r0), D, dif, tol_rel=1e-9)
+    assert _rel(plain[sw0 > 1e-3], sw0[sw0 > 1e-3]) < RTOL
+
+
 # Hapke ground (core:546-605) at strength 0.7: the ground facets' normals
 # tilted so that |n_y| < 1 (the longwave emissivity then varies); shortwave
 # reflectance per ground side and band from the sun, longwave emissivity

@@ -383,3 +383,126 @@ def test_large_scene_raster_and_adaptive_layers(tmp_path):
     assert len(tris) == 353 * 295 * 2
     assert "automatically increased A-buffer layers from 16 to 32 for 208270 facets" in r.stdout
     assert "raster 2048 layers 32" in r.stdout
+
+
+def _pillow():
+    return pytest.importorskip("PIL.Image")
+
+
+def _decode(path, tmp_path):
+    out = tmp_path / "img.raw"
+    r = subprocess.run([str(EXE), "--decode-image", str(path), str(out)], capture_output=True, text=True)
+    if r.returncode:
+        return None, r.stderr
+    w, h, c = map(int, r.stdout.split())
+    return np.fromfile(out, np.uint8).reshape(h, w, c), ""
+
+
+def test_texture_images_decode_like_libjpeg(tmp_path):
+    # the texture reader's samples against Pillow's (libjpeg-turbo, libpng)
+    # decode of the same files: the library's concrete texture (baseline,
+    # 4:4:4), synthetic JPEGs at 4:4:4, 4:2:2 and 4:2:0 (fancy upsampling),
+    # greyscale and with restart markers, and PNGs of every colour type
+    # (palette: indices; 16-bit: clamped to 255, as a byte read gives them);
+    # progressive JPEGs are refused
+    _need_exe()
+    Image = _pillow()
+    a, _ = _decode(GOLDEN / "textures" / "concrete_variance.jpg", tmp_path)
+    np.testing.assert_array_equal(a, np.asarray(Image.open(GOLDEN / "textures" / "concrete_variance.jpg")))
+    rng = np.random.default_rng(3)
+    img = np.clip(np.cumsum(rng.integers(0, 40, (37, 53, 3)), axis=1) // 3, 0, 255).astype(np.uint8)
+    cases = [("s0.jpg", dict(quality=90, subsampling=0)), ("s1.jpg", dict(quality=60, subsampling=1)),
+             ("s2.jpg", dict(quality=75, subsampling=2)), ("rst.jpg", dict(quality=80, restart_marker_blocks=3))]
+    for name, kw in cases:
+        Image.fromarray(img).save(tmp_path / name, **kw)
+        a, err = _decode(tmp_path / name, tmp_path)
+        np.testing.assert_array_equal(a, np.asarray(Image.open(tmp_path / name).convert("RGB")), err_msg=name)
+    Image.fromarray(img[:, :, 1]).save(tmp_path / "g.jpg", quality=85)
+    a, _ = _decode(tmp_path / "g.jpg", tmp_path)
+    np.testing.assert_array_equal(a[:, :, 0], np.asarray(Image.open(tmp_path / "g.jpg")))
+    Image.fromarray(img).save(tmp_path / "p.jpg", progressive=True)
+    a, err = _decode(tmp_path / "p.jpg", tmp_path)
+    assert a is None and "progressive" in err
+    for mode in ("RGB", "RGBA", "L", "LA", "P"):
+        Image.fromarray(img).convert(mode).save(tmp_path / "x.png")
+        a, _ = _decode(tmp_path / "x.png", tmp_path)
+        want = np.asarray(Image.open(tmp_path / "x.png"))
+        want = want[:, :, None] if want.ndim == 2 else want[:, :, :3]
+        np.testing.assert_array_equal(a, want, err_msg=mode)
+    Image.fromarray(img[:, :, 0].astype(np.uint16) * 3).save(tmp_path / "w.png")
+    a, _ = _decode(tmp_path / "w.png", tmp_path)
+    np.testing.assert_array_equal(a[:, :, 0], np.minimum(img[:, :, 0].astype(np.uint16) * 3, 255))
+
+
+def _textured(tmp_path, image, strength=0.2, repeat=2.0, reflectance="0.25", objects=()):
+    spectra = [{"name": "soil", "model": "custom", "reflectance": reflectance, "transmittance": "0",
+                "physicalTexture": {"enabled": True, "fileName": image, "strength": strength, "repeatSize": repeat}},
+               {"name": "plain", "model": "custom", "reflectance": "0.3", "transmittance": "0"}]
+    p = _write_project(tmp_path, list(objects), spectra=spectra, scene={"voxel": 1, "height": 10})
+    return p
+
+
+def _factors(p, tmp_path):
+    k = tmp_path / "k.bin"
+    r = subprocess.run([str(EXE), "--scene-obj", str(p), str(tmp_path / "s.obj"), str(k)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    tris = _load_obj(tmp_path / "s.obj")
+    return np.fromfile(k, np.float32), tris, r
+
+
+def test_texture_factors(tmp_path):
+    # finalizeReflectanceTextureFactors (core:718-846) on the 10 x 8 ground
+    # (160 one-metre triangles of equal area): the area-weighted mean factor
+    # is one; the deviations scale with the strength; a ramp across the
+    # texture's columns varies the ground's factors along x only, and one
+    # down its rows along z only (the ground samples the (x, z) plane); at
+    # strength 1 and r = 0.9 the largest factor is capped at (1 - t) / r
+    _need_exe()
+    Image = _pillow()
+    d = tmp_path / "prj"
+    d.mkdir()
+    ramp = np.tile(np.arange(64, dtype=np.uint8)[None, :] * 2, (8, 1))
+    Image.fromarray(ramp).save(d / "cols.png")
+    Image.fromarray(np.ascontiguousarray(ramp.T)).save(d / "rows.png")
+    p = _textured(tmp_path, "cols.png", strength=0.1, repeat=40.0)
+    k1, tris, r = _factors(p, tmp_path)
+    assert "physical texture material 0, area-weighted factor mean=1" in r.stdout
+    ground = np.array([t for t, g in tris if g == "STREAMSIM_TERRAIN"])
+    assert k1.size == len(ground) == 160
+    assert abs(k1.astype(np.float64).mean() - 1.0) < 1e-6
+    cx, cz = ground[:, :, 0].mean(1), ground[:, :, 2].mean(1)
+    groups = {}
+    for f, x in zip(k1, np.round(cx, 4)):
+        groups.setdefault(x, []).append(f)
+    assert len(groups) > 10 and max(np.ptp(v) for v in groups.values()) < 1e-6  # constant along z
+    assert np.ptp(k1) > 0.01
+    p = _textured(tmp_path, "cols.png", strength=0.05, repeat=40.0)
+    k2, _, _ = _factors(p, tmp_path)
+    np.testing.assert_allclose(k1 - 1.0, 2.0 * (k2 - 1.0), rtol=1e-4, atol=1e-7)
+    p = _textured(tmp_path, "rows.png", strength=0.1, repeat=40.0)
+    k3, _, _ = _factors(p, tmp_path)
+    groups = {}
+    for f, z in zip(k3, np.round(cz, 4)):
+        groups.setdefault(z, []).append(f)
+    assert len(groups) > 8 and max(np.ptp(v) for v in groups.values()) < 1e-6  # constant along x
+    p = _textured(tmp_path, "cols.png", strength=1.0, repeat=40.0, reflectance="0.9")
+    k4, _, _ = _factors(p, tmp_path)
+    assert abs(float(k4.max()) - 1.0 / 0.9) < 1e-5 and k4.min() > 0.0
+    assert abs(k4.astype(np.float64).mean() - 1.0) < 1e-6
+    # no texture: every factor is one
+    p = _write_project(tmp_path, [])
+    k0, _, _ = _factors(p, tmp_path)
+    assert np.all(k0 == 1.0)
+
+
+def test_texture_errors(tmp_path):
+    _need_exe()
+    Image = _pillow()
+    p = _textured(tmp_path, "")
+    r = subprocess.run([str(EXE), "--scene-obj", str(p), str(tmp_path / "o.obj")], capture_output=True, text=True)
+    assert r.returncode == 1 and "Physical texture is enabled but has no file: soil" in r.stderr
+    Image.fromarray(np.full((4, 4), 90, np.uint8)).save(p.parent / "flat.png")
+    p = _textured(tmp_path, "flat.png")
+    r = subprocess.run([str(EXE), "--scene-obj", str(p), str(tmp_path / "o.obj")], capture_output=True, text=True)
+    assert r.returncode == 1 and "effectively constant" in r.stderr
