@@ -203,6 +203,50 @@ int fh_solve_bands_device(fh_context *ctx, uint32_t nb, const float *d_reflectan
                           uint32_t maxIter, float tolRel, float relaxation, uint32_t precision,
                           float *d_radiosity, fh_solve_stats *stats);
 
+/* ---- band optics on the device -------------------------------------- */
+/* The per-side, per-band optics the reference's host loops build before its
+ * band solves: FH_OPTICS_SCENE is makeSpectralOptics (core:1057-1126: bands
+ * above 2500 nm thermal, emission eps (f Planck(T_sunlit) + (1 - f)
+ * Planck(T_shaded)) with f = clamp(sunlit, 0, 1), direct sunlit cos
+ * directNormal), FH_OPTICS_STATE is makeStateSpectralOptics
+ * (core:1135-1199: isOpticalWavelength, emission eps Planck(clamp(T_side,
+ * 1, 2000)), direct clamp(sunlit, 0, 1) cos directNormal).  Both: r =
+ * clamp(r_material factor, 0, 1) on optical bands (factor 1 on thermal
+ * ones), t = clamp(t_material, 0, 1 - r); ground sides (facets from
+ * leafFacetCount) take the Hapke reflectance (optical) or emissivity
+ * (thermal) at backgroundAngularStrength. */
+enum { FH_OPTICS_SCENE = 0, FH_OPTICS_STATE = 1 };
+typedef struct {
+    uint32_t mode;                  /* FH_OPTICS_SCENE or FH_OPTICS_STATE                */
+    uint32_t bands;                 /* nb, 1..4096                                        */
+    uint32_t materialCount;
+    uint32_t leafFacetCount;
+    const uint32_t *facetMaterial;  /* [facetCount]                                       */
+    const float *reflectance;       /* [materialCount][bands]: thermal bands ref_TIR      */
+    const float *transmittance;     /* [materialCount][bands]: thermal bands tau_TIR      */
+    const float *wavelength;        /* [bands], nm (micrometres when <= 2.5)              */
+    const float *reflectanceFactor; /* [facetCount] texture factors, or NULL              */
+    float backgroundAngularStrength;
+    fh_direction sun;               /* unit vector towards the sun                        */
+    const float *sunlit;            /* [surfaceCount]                                     */
+    const float *directNormal;      /* [bands] direct normal irradiance (optical bands)   */
+    const float *temperature;       /* STATE: [surfaceCount] side temperatures, K         */
+    const float *sunlitTemperature; /* SCENE: [materialCount], K                          */
+    const float *shadedTemperature; /* SCENE: [materialCount], K                          */
+} fh_spectral_optics;
+
+/* The optics as host arrays [surfaceCount][bands] (any output may be NULL). */
+int fh_spectral_optics_build(fh_context *ctx, const fh_spectral_optics *optics, float *reflectance,
+                             float *transmittance, float *emission, float *direct);
+
+/* The optics built on the device and solved as fh_solve_bands does, with no
+ * per-band arrays crossing the bus: the reference's band loop with its host
+ * optics (core:1690-1763, 3875-3907) in one call.  radiosity is
+ * [surfaceCount][bands] on the host. */
+int fh_solve_spectral(fh_context *ctx, const fh_spectral_optics *optics, const float *sky,
+                      const float *init, uint32_t maxIter, float tolRel, float relaxation,
+                      uint32_t precision, float *radiosity, fh_solve_stats *stats);
+
 /* H = max(0, in(B) + D) per side and band (core:1531-1559, multi-band) */
 int fh_incident_bands(fh_context *ctx, uint32_t nb, const float *radiosity, const float *direct,
                       const float *sky, float *irradiance);

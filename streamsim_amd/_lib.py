@@ -41,6 +41,20 @@ class FhDirection(C.Structure):
     _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
 
 
+FH_OPTICS_SCENE = 0
+FH_OPTICS_STATE = 1
+
+
+class FhSpectralOptics(C.Structure):
+    _fields_ = [
+        ("mode", C.c_uint32), ("bands", C.c_uint32), ("materialCount", C.c_uint32), ("leafFacetCount", C.c_uint32),
+        ("facetMaterial", C.c_void_p), ("reflectance", C.c_void_p), ("transmittance", C.c_void_p),
+        ("wavelength", C.c_void_p), ("reflectanceFactor", C.c_void_p), ("backgroundAngularStrength", C.c_float),
+        ("sun", FhDirection), ("sunlit", C.c_void_p), ("directNormal", C.c_void_p), ("temperature", C.c_void_p),
+        ("sunlitTemperature", C.c_void_p), ("shadedTemperature", C.c_void_p),
+    ]
+
+
 class FhGraphDiagnostics(C.Structure):
     _fields_ = [
         ("facetCount", C.c_uint32),
@@ -117,6 +131,9 @@ SIGNATURES = {
     "fh_facet_count": (_U32, [_P]),
     "fh_surface_count": (_U32, [_P]),
     "fh_solve_bands": (_I, [_P, _U32, _P, _P, _P, _P, _P, _P, _U32, _F, _F, _U32, _P, C.POINTER(FhSolveStats)]),
+    "fh_spectral_optics_build": (_I, [_P, C.POINTER(FhSpectralOptics), _P, _P, _P, _P]),
+    "fh_solve_spectral": (_I, [_P, C.POINTER(FhSpectralOptics), _P, _P, _U32, _F, _F, _U32, _P,
+                               C.POINTER(FhSolveStats)]),
     "fh_solve_bands_device": (_I, [_P, _U32, _P, _P, _P, _P, _P, _P, _U32, _F, _F, _U32, _P,
                                    C.POINTER(FhSolveStats)]),
     "fh_incident_bands": (_I, [_P, _U32, _P, _P, _P, _P]),

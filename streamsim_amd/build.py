@@ -38,8 +38,9 @@ COMMON = [
 PER_FILE = {
     "raytrace.hip": ["-ffp-contract=off"],
     "raster.hip": ["-ffp-contract=off"],
+    "optics.hip": ["-ffp-contract=off"],
 }
-SOURCES = ["abi.hip", "raytrace.hip", "spmm.hip", "eb.hip", "mlp.hip", "raster.hip", "bvh.cpp"]
+SOURCES = ["abi.hip", "raytrace.hip", "spmm.hip", "eb.hip", "mlp.hip", "raster.hip", "optics.hip", "bvh.cpp"]
 
 
 def hipcc() -> str:

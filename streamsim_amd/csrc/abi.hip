@@ -705,6 +705,124 @@ int fh_solve_bands_device(fh_context* c, uint32_t nb, const float* dr, const flo
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// validates the optics inputs and builds them on the device into dr, dt,
+// de, dD ([S][nb] each); the host inputs travel in one staging buffer
+void build_spectral(fh_context* c, const fh_spectral_optics* o, fh::DBuf<float>& stage, float* dr, float* dt,
+                    float* de, float* dD) {
+    if (!o) fail(FH_INVALID_ARGUMENT, "optics must not be null");
+    const uint32_t nb = o->bands, nm = o->materialCount, F = c->F, S = c->S;
+    if (o->mode > FH_OPTICS_STATE) fail(FH_INVALID_ARGUMENT, "unknown optics mode");
+    if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
+    if (nm == 0u || !o->facetMaterial || !o->reflectance || !o->transmittance || !o->wavelength || !o->sunlit ||
+        !o->directNormal)
+        fail(FH_INVALID_ARGUMENT, "materials, bands, sunlit fractions and direct irradiance must be given");
+    if (o->leafFacetCount > F) fail(FH_INVALID_ARGUMENT, "leafFacetCount exceeds the facet count");
+    const bool state = o->mode == FH_OPTICS_STATE;
+    if (state ? !o->temperature : (!o->sunlitTemperature || !o->shadedTemperature))
+        fail(FH_INVALID_ARGUMENT, "temperatures must be given");
+    const float sunLen2 = o->sun.x * o->sun.x + o->sun.y * o->sun.y + o->sun.z * o->sun.z;
+    if (!std::isfinite(sunLen2) || (o->backgroundAngularStrength > 0.0f && !(sunLen2 > 0.0f)))
+        fail(FH_INVALID_ARGUMENT, "sun direction must be finite (and non-zero with a Hapke ground)");
+    if (!std::isfinite(o->backgroundAngularStrength)) fail(FH_INVALID_ARGUMENT, "angular strength must be finite");
+    for (uint32_t f = 0; f < F; ++f)
+        if (o->facetMaterial[f] >= nm) fail(FH_INVALID_ARGUMENT, "facet material index out of range");
+    const auto finite = [](const float* v, size_t n, bool nonneg) {
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(v[i]) || (nonneg && v[i] < 0.0f)) return false;
+        return true;
+    };
+    if (!finite(o->reflectance, size_t(nm) * nb, false) || !finite(o->transmittance, size_t(nm) * nb, false) ||
+        !finite(o->wavelength, nb, false) || !finite(o->directNormal, nb, true) || !finite(o->sunlit, S, false) ||
+        (o->reflectanceFactor && !finite(o->reflectanceFactor, F, true)) ||
+        (state && !finite(o->temperature, S, false)) ||
+        (!state && (!finite(o->sunlitTemperature, nm, false) || !finite(o->shadedTemperature, nm, false))))
+        fail(FH_INVALID_ARGUMENT, "optics inputs must be finite (direct irradiance and factors non-negative)");
+    // staging layout (floats): mat F | refl nm nb | trans nm nb | wl nb | dni nb | factor F | sunlit S | T S |
+    // Tsun nm | Tshade nm
+    const size_t total = size_t(F) * 2 + size_t(nm) * nb * 2 + size_t(nb) * 2 + size_t(S) * 2 + size_t(nm) * 2;
+    stage.ensure(total);
+    float* p = stage.p;
+    fh::SpectralOpticsArgs a{};
+    a.mode = o->mode;
+    a.nb = nb;
+    a.S = S;
+    a.leafCount = o->leafFacetCount;
+    const auto put = [&](const void* src, size_t count) {
+        float* at = p;
+        if (src) upload(c, reinterpret_cast<uint32_t*>(p), static_cast<const uint32_t*>(src), count);
+        p += count;
+        return at;
+    };
+    a.mat = reinterpret_cast<const uint32_t*>(put(o->facetMaterial, F));
+    a.reflectance = put(o->reflectance, size_t(nm) * nb);
+    a.transmittance = put(o->transmittance, size_t(nm) * nb);
+    a.wavelength = put(o->wavelength, nb);
+    a.dni = put(o->directNormal, nb);
+    a.factor = o->reflectanceFactor ? put(o->reflectanceFactor, F) : (p += F, nullptr);
+    a.sunlit = put(o->sunlit, S);
+    a.T = state ? put(o->temperature, S) : (p += S, nullptr);
+    a.Tsun = state ? (p += nm, nullptr) : put(o->sunlitTemperature, nm);
+    a.Tshade = state ? (p += nm, nullptr) : put(o->shadedTemperature, nm);
+    a.nrm = c->facetNormal.p;
+    a.strength = std::min(std::max(o->backgroundAngularStrength, 0.0f), 1.0f);
+    a.sx = o->sun.x;
+    a.sy = o->sun.y;
+    a.sz = o->sun.z;
+    a.r = dr;
+    a.t = dt;
+    a.e = de;
+    a.D = dD;
+    fh::launch_spectral_optics(c, a);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fh_spectral_optics_build(fh_context* c, const fh_spectral_optics* o, float* r, float* t, float* e, float* D) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_geometry(c, "spectral optics");
+        const size_t n = size_t(c->S) * (o ? o->bands : 0u);
+        fh::DBuf<float> out, stage;
+        out.ensure(4 * (n ? n : 1));
+        build_spectral(c, o, stage, out.p, out.p + n, out.p + 2 * n, out.p + 3 * n);
+        float* host[4] = {r, t, e, D};
+        for (int k = 0; k < 4; ++k)
+            if (host[k]) download(c, host[k], out.p + size_t(k) * n, n);
+        FH_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int fh_solve_spectral(fh_context* c, const fh_spectral_optics* o, const float* sky, const float* init,
+                      uint32_t maxIter, float tolRel, float relaxation, uint32_t precision, float* radiosity,
+                      fh_solve_stats* stats) {
+    if (!c) return FH_INVALID_ARGUMENT;
+    return guarded(c, [&] {
+        require_graph(c, "solveBands()");
+        if (!o || !sky || !radiosity) fail(FH_INVALID_ARGUMENT, "optics, sky and radiosity must not be null");
+        const uint32_t nb = o->bands;
+        if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
+        for (uint32_t b = 0; b < nb; ++b) validate_sky(sky[b]);
+        validate_relax(relaxation);
+        if (precision > FH_PRECISION_FP8_INC) fail(FH_INVALID_ARGUMENT, "unknown precision");
+        const size_t n = size_t(c->S) * nb;
+        fh::DBuf<float> in, stage;
+        in.ensure(n * (init ? 6u : 5u));
+        float* dout = in.p + 4 * n;
+        float* dinit = init ? dout + n : nullptr;
+        build_spectral(c, o, stage, in.p, in.p + n, in.p + 2 * n, in.p + 3 * n);
+        if (init) upload(c, dinit, init, n);
+        solve_bands_impl(c, nb, in.p, in.p + n, in.p + 2 * n, in.p + 3 * n, sky, dinit, maxIter ? maxIter : 1u,
+                         tolRel, relaxation, precision, dout, stats);
+        download(c, radiosity, dout, n);
+    });
+}
+
 int fh_incident_bands(fh_context* c, uint32_t nb, const float* radiosity, const float* direct, const float* sky,
                       float* irradiance) {
     if (!c) return FH_INVALID_ARGUMENT;

@@ -37,6 +37,21 @@ void launch_fold(fh_context* c, const float* e, const float* r, const float* t, 
 void launch_pad(fh_context* c, const float* src, uint32_t nb, uint32_t ldb, float* dst);
 void launch_unpad(fh_context* c, const float* src, uint32_t nb, uint32_t ldb, float* dst);
 void launch_split_optics(fh_context* c, const float4* o, float* r, float* t, float* e, float* D);
+
+// optics.hip: per-side, per-band optics [S][nb] from device inputs
+struct SpectralOpticsArgs {
+    uint32_t mode, nb, S, leafCount;
+    const uint32_t* mat;                          // [F]
+    const float *reflectance, *transmittance;     // [materials][nb]
+    const float *wavelength, *dni;                // [nb]
+    const float* factor;                          // [F] or null
+    const float *sunlit, *T;                      // [S] (T: state mode)
+    const float *Tsun, *Tshade;                   // [materials] (scene mode)
+    const float4* nrm;                            // [F]
+    float strength, sx, sy, sz;
+    float *r, *t, *e, *D;                         // [S][nb]
+};
+void launch_spectral_optics(fh_context* c, const SpectralOpticsArgs& a);
 void launch_ctrl_reset(fh_context* c, uint32_t maxIter, float tolRel, float tolAbs, uint32_t checkEvery,
                        uint32_t nAct = 1u, float innerTol = -1.0f);
 void launch_fill_u32(fh_context* c, uint32_t* p, size_t n, uint32_t v);

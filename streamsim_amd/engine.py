@@ -241,6 +241,59 @@ class FacetEngine:
                                              maxIter, tolRel, relaxation, precision, _ptr(out), C.byref(st)))
         return SolveResult(out, st.iterations, st.maxDelta, st.maxRelDelta, st.refreshes)
 
+    def _spectral(self, mode, material, reflectance, transmittance, wavelength, sun, sunlit, direct_normal,
+                  leaf_count, strength=0.0, factor=None, temperature=None, sunlit_temperature=None,
+                  shaded_temperature=None):
+        F, S = self.facetCount(), self.surfaceCount()
+        R = _f32(reflectance)
+        nm, nb = R.shape
+        keep = {
+            "mat": np.ascontiguousarray(material, np.uint32).reshape(-1),
+            "r": R, "t": _f32(transmittance, (nm, nb)), "wl": _f32(wavelength).reshape(-1),
+            "dni": _f32(direct_normal).reshape(-1), "sunlit": _f32(sunlit).reshape(-1),
+            "k": None if factor is None else _f32(factor).reshape(-1),
+            "T": None if temperature is None else _f32(temperature).reshape(-1),
+            "ts": None if sunlit_temperature is None else _f32(sunlit_temperature).reshape(-1),
+            "th": None if shaded_temperature is None else _f32(shaded_temperature).reshape(-1),
+        }
+        if keep["mat"].size != F or keep["wl"].size != nb or keep["dni"].size != nb or keep["sunlit"].size != S:
+            raise ValueError("optics inputs have the wrong sizes")
+        o = _lib.FhSpectralOptics()
+        ad = {k: (None if v is None else v.ctypes.data) for k, v in keep.items()}
+        o.mode, o.bands, o.materialCount, o.leafFacetCount = mode, nb, nm, leaf_count
+        o.facetMaterial, o.reflectance, o.transmittance = ad["mat"], ad["r"], ad["t"]
+        o.wavelength, o.reflectanceFactor = ad["wl"], ad["k"]
+        o.backgroundAngularStrength = strength
+        o.sun = FhDirection(*[float(x) for x in sun])
+        o.sunlit, o.directNormal, o.temperature = ad["sunlit"], ad["dni"], ad["T"]
+        o.sunlitTemperature, o.shadedTemperature = ad["ts"], ad["th"]
+        return o, keep, nb
+
+    def spectralOptics(self, mode, material, reflectance, transmittance, wavelength, sun, sunlit, direct_normal,
+                       leaf_count, **kw):
+        """Per-side, per-band optics built on the device (makeSpectralOptics /
+        makeStateSpectralOptics, facetrtcore.cpp:1057-1199): (r, t, e, D),
+        each [surfaceCount][bands]."""
+        o, keep, nb = self._spectral(mode, material, reflectance, transmittance, wavelength, sun, sunlit,
+                                     direct_normal, leaf_count, **kw)
+        out = [np.zeros((self.surfaceCount(), nb), np.float32) for _ in range(4)]
+        self._check(self._lib.fh_spectral_optics_build(self._h, C.byref(o), *[_ptr(a) for a in out]))
+        return tuple(out)
+
+    def solveSpectral(self, mode, material, reflectance, transmittance, wavelength, sun, sunlit, direct_normal,
+                      leaf_count, sky, maxIter: int = 64, tolRel: float = -1.0, relaxation: float = 1.0,
+                      precision: int = _lib.FH_PRECISION_FP32, **kw) -> SolveResult:
+        o, keep, nb = self._spectral(mode, material, reflectance, transmittance, wavelength, sun, sunlit,
+                                     direct_normal, leaf_count, **kw)
+        sk = _f32(sky).reshape(-1)
+        if sk.size != nb:
+            raise ValueError("sky must hold one value per band")
+        out = np.zeros((self.surfaceCount(), nb), np.float32)
+        st = FhSolveStats()
+        self._check(self._lib.fh_solve_spectral(self._h, C.byref(o), _ptr(sk), None, maxIter, tolRel, relaxation,
+                                                precision, _ptr(out), C.byref(st)))
+        return SolveResult(out, st.iterations, st.maxDelta, st.maxRelDelta, st.refreshes)
+
     def incidentBands(self, radiosity, direct, sky) -> np.ndarray:
         B = _f32(radiosity)
         S = self.surfaceCount()

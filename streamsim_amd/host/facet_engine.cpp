@@ -276,9 +276,9 @@ float planckRadiance(float wavelengthNm, float temperature) {
 }
 
 // eFacetRT on a project.json (runGpu's band loop, core:1672-1774): per
-// sensor band the optics of makeSpectralOptics (core:1057-1126; Lambertian
-// background, no texture): optical bands r, t of the band, D = sunlit
-// max(0, n.s) 700, sky 50; thermal bands (> 2500 nm) r, t = ref_TIR,
+// sensor band the optics of makeSpectralOptics (core:1057-1126, built on the
+// device): optical bands r, t of the band (r times the texture factor, the
+// Hapke ground), D = sunlit max(0, n.s) 700, sky 50; thermal bands (> 2500 nm) r, t = ref_TIR,
 // tau_TIR, emission e = eps (f Planck(T_sunlit) + (1 - f) Planck(T_shaded)),
 // sky Planck(T_sky).  All bands in one 64-sweep multi-band solve (the
 // reference's kIterations sweeps per band); optical results as apparent
@@ -305,51 +305,40 @@ int runFacetRTProject(const std::string& input, const std::string& outFile) try 
     std::vector<float> r(S * nb), t(S * nb), e(S * nb, 0.0f), D(S * nb, 0.0f), sky(nb);
     for (uint32_t b = 0; b < nb; ++b)
         sky[b] = bands[b] > 2500.0f ? planckRadiance(bands[b], prj.run.skyTemperature) : kSky * 0.5f;
-    for (uint32_t f = 0; f < scene.facetCount(); ++f) {
-        const facetcli::ProjectMaterial& m = prj.materials[scene.material[f]];
-        const bool object = f < scene.leafFacetCount;
-        const auto& vx = scene.vertices[3 * size_t(f)];
-        for (uint32_t ls = 0; ls < 2; ++ls) {
-            const size_t side = 2 * size_t(f) + ls;
-            const float sg = ls == 0 ? 1.0f : -1.0f;
-            const float cosv =
-                std::max(0.0f, sg * vx.normal[0] * sun.x + sg * vx.normal[1] * sun.y + sg * vx.normal[2] * sun.z);
-            for (uint32_t b = 0; b < nb; ++b) {
-                const size_t i = side * nb + b;
-                const bool thermal = bands[b] > 2500.0f;
-                float refl, trans;
-                if (thermal) {
-                    refl = std::clamp(m.tirR, 0.0f, 1.0f);
-                    trans = std::clamp(m.tirT, 0.0f, 1.0f - refl);
-                } else {
-                    const float rb = m.bandR.empty() ? m.r : m.bandR[std::min<size_t>(b, m.bandR.size() - 1)];
-                    const float tb = m.bandT.empty() ? m.t : m.bandT[std::min<size_t>(b, m.bandT.size() - 1)];
-                    refl = std::clamp(rb * scene.reflectanceFactorOf(f), 0.0f, 1.0f);  // texture (core:1085-1088)
-                    trans = std::clamp(tb, 0.0f, 1.0f - refl);
-                }
-                float eps = std::max(0.0f, 1.0f - refl - trans);
-                if (!object) {  // the background's Hapke optics (core:1095-1105)
-                    const float sn[3] = {sg * vx.normal[0], sg * vx.normal[1], sg * vx.normal[2]};
-                    if (thermal) {
-                        eps = fh::hapke::emissivity(eps, std::fabs(sn[1]), scene.backgroundAngularStrength);
-                        refl = std::clamp(1.0f - trans - eps, 0.0f, 1.0f);
-                    } else {
-                        const float sv[3] = {sun.x, sun.y, sun.z};
-                        refl = fh::hapke::reflectance(refl, sv, sn, sn, scene.backgroundAngularStrength);
-                    }
-                }
-                r[i] = refl;
-                t[i] = std::min(trans, 1.0f - refl);
-                if (thermal) {
-                    const float ill = std::clamp(sunlit[side], 0.0f, 1.0f);
-                    e[i] = eps * (ill * planckRadiance(bands[b], m.sunlitT) +
-                                  (1.0f - ill) * planckRadiance(bands[b], m.shadedT));
-                } else {
-                    D[i] = sunlit[side] * cosv * kBeam;
-                }
-            }
+    // makeSpectralOptics on the device: per material and band the band's r,
+    // t (ref_TIR, tau_TIR above 2500 nm), the materials' sunlit / shaded
+    // temperatures, the texture factors, 700 W m-2 normal direct
+    const size_t nm = prj.materials.size();
+    std::vector<float> matR(nm * nb), matT(nm * nb), tSun(nm), tShade(nm), dni(nb);
+    for (size_t m = 0; m < nm; ++m) {
+        const facetcli::ProjectMaterial& pm = prj.materials[m];
+        tSun[m] = pm.sunlitT;
+        tShade[m] = pm.shadedT;
+        for (uint32_t b = 0; b < nb; ++b) {
+            const bool thermal = bands[b] > 2500.0f;
+            matR[m * nb + b] = thermal ? pm.tirR : pm.bandR.empty() ? pm.r : pm.bandR[std::min<size_t>(b, pm.bandR.size() - 1)];
+            matT[m * nb + b] = thermal ? pm.tirT : pm.bandT.empty() ? pm.t : pm.bandT[std::min<size_t>(b, pm.bandT.size() - 1)];
         }
     }
+    for (uint32_t b = 0; b < nb; ++b) dni[b] = bands[b] > 2500.0f ? 0.0f : kBeam;
+    fh_spectral_optics so{};
+    so.mode = FH_OPTICS_SCENE;
+    so.bands = nb;
+    so.materialCount = uint32_t(nm);
+    so.leafFacetCount = scene.leafFacetCount;
+    so.facetMaterial = scene.material.data();
+    so.reflectance = matR.data();
+    so.transmittance = matT.data();
+    so.wavelength = bands.data();
+    so.reflectanceFactor = scene.reflectanceFactor.empty() ? nullptr : scene.reflectanceFactor.data();
+    so.backgroundAngularStrength = scene.backgroundAngularStrength;
+    so.sun = {sun.x, sun.y, sun.z};
+    so.sunlit = sunlit.data();
+    so.directNormal = dni.data();
+    so.sunlitTemperature = tSun.data();
+    so.shadedTemperature = tShade.data();
+    facethip::check(fh_spectral_optics_build(model.handle(), &so, r.data(), t.data(), e.data(), D.data()),
+                    model.handle());
     fh_context* ctx = model.handle();
     std::vector<float> out(S * nb);
     fh_solve_stats st{};
@@ -381,8 +370,8 @@ int runFacetRTProject(const std::string& input, const std::string& outFile) try 
         }
     } else {
         std::cout << "PROGRESS\t82\tHIP Jacobi radiosity: " << nb << " bands, 64 iterations" << std::endl;
-        facethip::check(fh_solve_bands(ctx, nb, r.data(), t.data(), e.data(), D.data(), sky.data(), nullptr, 64u,
-                                       -1.0f, 1.0f, FH_PRECISION_FP32, out.data(), &st),
+        facethip::check(fh_solve_spectral(ctx, &so, sky.data(), nullptr, 64u, -1.0f, 1.0f, FH_PRECISION_FP32,
+                                          out.data(), &st),
                         ctx);
     }
     const float ref = kBeam * std::max(0.0f, sun.y) + kSky * 0.5f;
@@ -1140,7 +1129,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         const bool converged = st.converged != 0;
         allConverged = allConverged && converged;
         // sensor products (core:3862-3908, makeStateSpectralOptics
-        // core:1135-1199 with no reflectance texture): per band the optics at
+        // core:1135-1199, built on the device): per band the optics at
         // the node's temperatures, direct total * Esun(band) * 0.001 and
         // diffuse total * Esky(band) * 0.001 of the spectrum, sky radiance
         // diffuse / 2 (thermal: Planck at the sky temperature), 64 sweeps;
@@ -1148,8 +1137,7 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
         // diffuse / 2, thermal bands as radiance
         std::vector<float> bandOut(S * nbs, 0.0f);
         if (nbs) {
-            std::vector<float> br(S * nbs), bt(S * nbs), be(S * nbs, 0.0f), bD(S * nbs, 0.0f), bsky(nbs),
-                dS(nbs, 0.0f), qS(nbs, 0.0f), bB(S * nbs);
+            std::vector<float> bsky(nbs), dS(nbs, 0.0f), qS(nbs, 0.0f), bB(S * nbs);
             std::vector<uint8_t> opt(nbs);
             for (uint32_t b = 0; b < nbs; ++b) {  // isOpticalWavelength (core:1127-1133)
                 opt[b] = (sensorBands[b] <= 2.5f ? sensorBands[b] * 1000.0f : sensorBands[b]) <= 2500.0f;
@@ -1157,43 +1145,38 @@ int runFacetEB(const std::string& input, const std::string& outFile) try {
                 qS[b] = opt[b] ? total * spectrum.diffuseAt(sensorBands[b]) * 0.001f : 0.0f;
                 bsky[b] = opt[b] ? qS[b] * 0.5f : planckRadiance(sensorBands[b], set.skyTemperature);
             }
-            for (uint32_t f = 0; f < scene.facetCount(); ++f) {
-                const SensorMaterial& sm = sensorMats[std::min<size_t>(scene.material[f], sensorMats.size() - 1)];
-                const bool object = f < scene.leafFacetCount;
-                const auto& vx = scene.vertices[3 * size_t(f)];
-                for (uint32_t ls = 0; ls < 2; ++ls) {
-                    const size_t side = 2 * size_t(f) + ls;
-                    const float sg = ls == 0 ? 1.0f : -1.0f;
-                    const float cosv = std::max(0.0f, sg * vx.normal[0] * sd.x + sg * vx.normal[1] * sd.y +
-                                                          sg * vx.normal[2] * sd.z);
-                    for (uint32_t b = 0; b < nbs; ++b) {
-                        const size_t i = side * nbs + b;
-                        const bool optical = opt[b] != 0;
-                        const float meanR = optical ? sm.r[std::min<size_t>(b, sm.r.size() - 1)] : sm.tirR;
-                        float refl = std::clamp(meanR * (optical ? scene.reflectanceFactorOf(f) : 1.0f), 0.0f, 1.0f);
-                        const float trans = std::clamp(optical ? sm.t[std::min<size_t>(b, sm.t.size() - 1)] : sm.tirT,
-                                                       0.0f, 1.0f - refl);
-                        float eps = std::max(0.0f, 1.0f - refl - trans);
-                        if (!object) {  // the background's Hapke optics (core:1170-1180)
-                            const float sn[3] = {sg * vx.normal[0], sg * vx.normal[1], sg * vx.normal[2]};
-                            if (optical) {
-                                const float sv[3] = {sd.x, sd.y, sd.z};
-                                refl = fh::hapke::reflectance(refl, sv, sn, sn, scene.backgroundAngularStrength);
-                            } else {
-                                eps = fh::hapke::emissivity(eps, std::fabs(sn[1]), scene.backgroundAngularStrength);
-                                refl = std::clamp(1.0f - trans - eps, 0.0f, 1.0f);
-                            }
-                        }
-                        br[i] = refl;
-                        bt[i] = std::min(trans, 1.0f - refl);
-                        if (optical) bD[i] = std::clamp(sunlit[side], 0.0f, 1.0f) * cosv * dS[b];
-                        else be[i] = eps * planckRadiance(sensorBands[b], std::clamp(T[side], 1.0f, 2000.0f));
-                    }
+            // makeStateSpectralOptics on the device: per material and band the
+            // band's r, t (ref_TIR, tau_TIR on thermal bands), the node's side
+            // temperatures and sunlit fractions, the texture factors
+            const size_t nsm = sensorMats.size();
+            std::vector<float> smR(nsm * nbs), smT(nsm * nbs);
+            for (size_t q = 0; q < nsm; ++q)
+                for (uint32_t b = 0; b < nbs; ++b) {
+                    const SensorMaterial& sm = sensorMats[q];
+                    smR[q * nbs + b] = opt[b] ? sm.r[std::min<size_t>(b, sm.r.size() - 1)] : sm.tirR;
+                    smT[q * nbs + b] = opt[b] ? sm.t[std::min<size_t>(b, sm.t.size() - 1)] : sm.tirT;
                 }
-            }
+            std::vector<uint32_t> smIndex(scene.facetCount());
+            for (uint32_t f = 0; f < scene.facetCount(); ++f)
+                smIndex[f] = uint32_t(std::min<size_t>(scene.material[f], nsm - 1));
+            fh_spectral_optics so{};
+            so.mode = FH_OPTICS_STATE;
+            so.bands = nbs;
+            so.materialCount = uint32_t(nsm);
+            so.leafFacetCount = scene.leafFacetCount;
+            so.facetMaterial = smIndex.data();
+            so.reflectance = smR.data();
+            so.transmittance = smT.data();
+            so.wavelength = sensorBands.data();
+            so.reflectanceFactor = scene.reflectanceFactor.empty() ? nullptr : scene.reflectanceFactor.data();
+            so.backgroundAngularStrength = scene.backgroundAngularStrength;
+            so.sun = {sd.x, sd.y, sd.z};
+            so.sunlit = sunlit.data();
+            so.directNormal = dS.data();
+            so.temperature = T.data();
             fh_solve_stats bs{};
-            facethip::check(fh_solve_bands(ctx, nbs, br.data(), bt.data(), be.data(), bD.data(), bsky.data(), nullptr,
-                                           64u, -1.0f, 1.0f, FH_PRECISION_FP32, bB.data(), &bs),
+            facethip::check(fh_solve_spectral(ctx, &so, bsky.data(), nullptr, 64u, -1.0f, 1.0f, FH_PRECISION_FP32,
+                                              bB.data(), &bs),
                             ctx);
             const float czs = std::max(0.0f, sd.y);
             for (uint32_t b = 0; b < nbs; ++b) {

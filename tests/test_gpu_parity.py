@@ -920,3 +920,86 @@ def test_benchmark_entry_points():
             for tables in (False, True):
                 assert eng.benchmarkSweep(nb, prec, 3, tables=tables) > 0.0
     assert eng.benchmarkCopy(1 << 20, 3) > 0.0
+
+
+# makeSpectralOptics / makeStateSpectralOptics (facetrtcore.cpp:1057-1199)
+# built on the device against the same formulas on the host with the
+# oracle's Hapke functions: optical and thermal bands (nm and micrometre
+# wavelengths), texture factors, a Hapke ground on tilted facets, both
+# modes; then fh_solve_spectral against fh_solve_bands on those arrays
+@pytest.mark.parametrize("mode", [0, 1])
+def test_spectral_optics_on_device(oracle, mode):
+    print(f"spectral optics mode={mode}", flush=True)
+    from streamsim_amd.optics import planck_radiance
+    sc = scenes.row_crop(leaves_per_row=20, cells=4)
+    v = np.array(sc.vertices, np.float32, copy=True)
+    ground = np.arange(sc.leaf_count, sc.facet_count)
+    n = np.stack([np.where(ground % 2 == 0, 0.3, -0.4), np.ones(ground.size), np.full(ground.size, 0.25)], 1)
+    n = (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+    for k in range(3):
+        v[3 * ground + k, 3:6] = n
+    sc.vertices = v
+    eng = _engine(R=64, K=16, seed=3)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    F, S, L = sc.facet_count, sc.surface_count, sc.leaf_count
+    rng = np.random.default_rng(5)
+    wl = np.array([450.0, 0.55, 865.0, 2400.0, 2.6, 10500.0], np.float32)
+    nb = wl.size
+    nm = 3
+    mat = np.asarray(sc.material, np.uint32)
+    R = rng.uniform(0.02, 0.6, (nm, nb)).astype(np.float32)
+    T = rng.uniform(0.0, 0.4, (nm, nb)).astype(np.float32)
+    R[1, 0] = 1.2  # clamped
+    factor = rng.uniform(0.4, 1.8, F).astype(np.float32)
+    sun = oracle.sun_direction(35.0, 140.0)
+    sunlit = rng.uniform(-0.2, 1.2, S).astype(np.float32)
+    dni = rng.uniform(0.1, 2.0, nb).astype(np.float32)
+    temp = rng.uniform(270.0, 330.0, S).astype(np.float32)
+    tsun, tshade = np.array([300.0, 310.0, 305.0], np.float32), np.array([296.0, 300.0, 295.0], np.float32)
+    strength = 0.6
+    kw = dict(strength=strength, factor=factor)
+    kw.update(temperature=temp) if mode == 1 else kw.update(sunlit_temperature=tsun, shaded_temperature=tshade)
+    r, t, e, D = eng.spectralOptics(mode, mat, R, T, wl, sun, sunlit, dni, L, **kw)
+    # host restatement
+    nrm = sc.facet_normals().astype(np.float32)
+    want = [np.zeros((S, nb), np.float32) for _ in range(4)]
+    for s in range(S):
+        f = s // 2
+        sg = np.float32(1.0 if s % 2 == 0 else -1.0)
+        sn = nrm[f] * sg
+        for b in range(nb):
+            w = float(wl[b])
+            nmw = w * 1000.0 if w <= 2.5 else w
+            thermal = (w > 2500.0) if mode == 0 else not (nmw <= 2500.0)
+            rr = float(np.clip(np.float32(R[mat[f], b]) * (np.float32(1.0) if thermal else factor[f]), 0, 1))
+            tt = float(np.clip(T[mat[f], b], 0, 1 - rr))
+            eps = max(0.0, 1 - rr - tt)
+            if f >= L and thermal:
+                eps = oracle.hapke_emissivity(eps, abs(float(sn[1])), strength)
+                rr = float(np.clip(1 - tt - eps, 0, 1))
+            elif f >= L:
+                rr = oracle.hapke_reflectance(rr, sun, sn, sn, strength)
+            want[0][s, b] = rr
+            want[1][s, b] = min(tt, 1 - rr)
+            if thermal:
+                if mode == 0:
+                    ill = min(max(float(sunlit[s]), 0.0), 1.0)
+                    want[2][s, b] = eps * (ill * planck_radiance(w, tsun[mat[f]]) +
+                                           (1 - ill) * planck_radiance(w, tshade[mat[f]]))
+                else:
+                    want[2][s, b] = eps * planck_radiance(w, min(max(float(temp[s]), 1.0), 2000.0))
+            else:
+                lit = float(sunlit[s]) if mode == 0 else min(max(float(sunlit[s]), 0.0), 1.0)
+                want[3][s, b] = lit * max(0.0, float(sn @ np.asarray(sun, np.float32))) * float(dni[b])
+    for got, exp, name in zip((r, t, e, D), want, "rteD"):
+        np.testing.assert_allclose(got, exp, rtol=2e-6, atol=1e-7, err_msg=name)
+    assert np.any(e > 0) and np.any(D > 0) and np.ptp(r[2 * L:, 1]) > 1e-3
+    # the fused solve equals fh_solve_bands on the built arrays (sunlit in
+    # [0, 1], as the sunlit pass gives it: the direct term is non-negative)
+    sky = np.full(nb, 3.0, np.float32)
+    lit = np.clip(sunlit, 0, 1)
+    r, t, e, D = eng.spectralOptics(mode, mat, R, T, wl, sun, lit, dni, L, **kw)
+    a = eng.solveSpectral(mode, mat, R, T, wl, sun, lit, dni, L, sky, maxIter=64, **kw)
+    b = eng.solveBands(r, t, e, D, sky, maxIter=64, tolRel=-1.0)
+    np.testing.assert_array_equal(a.radiosity, b.radiosity)
