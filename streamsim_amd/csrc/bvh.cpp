@@ -8,6 +8,7 @@
 #include <cfloat>
 #include <cmath>
 #include <numeric>
+#include <thread>
 
 #include "engine.hpp"
 
@@ -37,20 +38,37 @@ struct Box {
 };
 
 struct Ref {             // result of building a subtree
-    bool leaf;
-    int32_t child;       // leaf: first slot; internal: node index
-    int32_t count;       // leaf primitive count
+    bool leaf = true;
+    int32_t child = -1;  // leaf: first slot; internal: node index
+    int32_t count = 0;   // leaf primitive count
     Box box;
 };
 
-struct Builder {
-    const float* tri;
+// Large ranges near the root build their left subtree on another thread
+// into a node list of their own, appended afterwards with its internal
+// child indices shifted: the partition of idx is the sequential one (each
+// thread only reorders its own range), so the tree and the leaf order are
+// the same as a one-thread build, only numbered differently.
+constexpr uint32_t kParallelPrims = 1u << 15;
+constexpr int kParallelDepth = 5;  // up to 32 concurrent subtrees
+
+struct Shared {
     std::vector<Box> pbox;
     std::vector<float> cen;  // 3 per prim
     std::vector<uint32_t> idx;
+    float pad = 0.0f;
+};
+
+struct Builder {
+    Shared* sh;
+    std::vector<Box>& pbox;
+    std::vector<float>& cen;
+    std::vector<uint32_t>& idx;
     std::vector<BvhNode> nodes;
     float pad = 0.0f;
     int maxDepth = 0;
+
+    explicit Builder(Shared* s) : sh(s), pbox(s->pbox), cen(s->cen), idx(s->idx), pad(s->pad) {}
 
     Box range_box(uint32_t b, uint32_t e) const {
         Box x;
@@ -121,9 +139,25 @@ struct Builder {
         }
         const int32_t me = int32_t(nodes.size());
         nodes.emplace_back();
-        const Ref l = build(b, mid, depth + 1);
-        const Ref r = build(mid, e, depth + 1);
-        fill(nodes[me], l, r);
+        Ref l, r;
+        if (n >= kParallelPrims && depth < kParallelDepth) {
+            Builder left(sh);
+            std::thread worker([&] { l = left.build(b, mid, depth + 1); });
+            r = build(mid, e, depth + 1);
+            worker.join();
+            const int32_t off = int32_t(nodes.size());
+            for (BvhNode nd : left.nodes) {
+                if (nd.meta.z == 0 && nd.meta.x >= 0) nd.meta.x += off;
+                if (nd.meta.w == 0 && nd.meta.y >= 0) nd.meta.y += off;
+                nodes.push_back(nd);
+            }
+            if (!l.leaf) l.child += off;
+            maxDepth = std::max(maxDepth, left.maxDepth);
+        } else {
+            l = build(b, mid, depth + 1);
+            r = build(mid, e, depth + 1);
+        }
+        fill(nodes[size_t(me)], l, r);
         return Ref{false, me, 0, box};
     }
 
@@ -149,22 +183,22 @@ struct Builder {
 }  // namespace
 
 void build_bvh(const float* tri, uint32_t F, BvhHost& out) {
-    Builder bd;
-    bd.tri = tri;
-    bd.pbox.resize(F);
-    bd.cen.resize(3 * size_t(F));
-    bd.idx.resize(F);
+    Shared sh;
+    sh.pbox.resize(F);
+    sh.cen.resize(3 * size_t(F));
+    sh.idx.resize(F);
     float m = 1.0f;
     for (uint32_t f = 0; f < F; ++f) {
         const float* t = tri + 9 * size_t(f);
-        for (int c = 0; c < 3; ++c) bd.pbox[f].grow(t + 3 * c);
+        for (int c = 0; c < 3; ++c) sh.pbox[f].grow(t + 3 * c);
         for (int k = 0; k < 3; ++k) {
-            bd.cen[3 * size_t(f) + k] = (t[k] + t[3 + k] + t[6 + k]) * (1.0f / 3.0f);
+            sh.cen[3 * size_t(f) + k] = (t[k] + t[3 + k] + t[6 + k]) * (1.0f / 3.0f);
             for (int c = 0; c < 3; ++c) m = std::max(m, std::fabs(t[3 * c + k]));
         }
     }
-    bd.pad = 1.0e-5f * m;
-    std::iota(bd.idx.begin(), bd.idx.end(), 0u);
+    sh.pad = 1.0e-5f * m;
+    std::iota(sh.idx.begin(), sh.idx.end(), 0u);
+    Builder bd(&sh);
     bd.nodes.reserve(size_t(F) / 2 + 8);
     if (F <= 4) {
         Ref leaf{true, 0, int32_t(F), bd.range_box(0, F)};
