@@ -90,8 +90,10 @@ def scheme_bytes(S: int, nb: int, increments: str | None, chebyshev: bool) -> in
     ones: an increment sweep reads and writes the FP32 accumulated gathers
     and carries (16 bytes per side-band) and writes the next increments and
     reads the gathered ones once (BF16: 2 + 2; FP8: 1 + 1, plus the chunk
-    exponents, 1 byte per 12 bands); Chebyshev reads the iterate before B (4)."""
-    per = {None: 0.0, "bf16": 20.0, "fp8": 18.0 + 2.0 / 12.0}[increments]
+    exponents, 1 byte per 12 bands); the BF16 phase of the BF16-F scheme
+    writes and reads its BF16 copy of B (2 + 2); Chebyshev reads the iterate
+    before B (4)."""
+    per = {None: 0.0, "bf16": 20.0, "fp8": 18.0 + 2.0 / 12.0, "bf16f": 4.0}[increments]
     return int(S * nb * (per + (4 if chebyshev else 0)))
 
 
@@ -336,7 +338,8 @@ def main() -> None:
     wl = WORKLOADS[a.workload]
     steps = a.steps if a.steps is not None else wl["steps"]
     from streamsim_amd import FacetEngine, faceteb, scenes
-    from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP8_INC, FH_PRECISION_FP32, FH_WARM_PREVIOUS
+    from streamsim_amd._lib import (FH_PRECISION_BF16_DC, FH_PRECISION_BF16_F, FH_PRECISION_FP8_INC, FH_PRECISION_FP32,
+                                    FH_WARM_PREVIOUS)
 
     log(f"[bench] workload={a.workload} steps={steps} warmup={a.warmup}")
     t = time.perf_counter()
@@ -373,13 +376,15 @@ def main() -> None:
         out = {"workload": a.workload, "sides": S, "facets": F, "nnz": nnz}
         for name, nbk, prec, tab in (("sw_fp32", nb, FH_PRECISION_FP32, True), ("sw_inc", nb, FH_PRECISION_BF16_DC, True),
                                      ("sw_q8", nb, FH_PRECISION_FP8_INC, True),
+                                     ("sw_bf16f", nb, FH_PRECISION_BF16_F, True),
                                      ("sw_fp32_arrays", nb, FH_PRECISION_FP32, False),
                                      ("sw_inc_arrays", nb, FH_PRECISION_BF16_DC, False),
                                      ("lw_fp32", 1, FH_PRECISION_FP32, False)):
             sec = time_sweep(eng, nbk, prec, a.roofline_reps, tables=tab)
             # single-band sweeps read the packed entries
             alg = sweep_bytes(S, F, nnz, nbk, tables=tab and nbk >= 16, entry=8 if nbk >= 16 else 4)
-            inc = None if prec == FH_PRECISION_FP32 or nbk < 16 else ("fp8" if prec == FH_PRECISION_FP8_INC else "bf16")
+            inc = (None if prec == FH_PRECISION_FP32 or nbk < 16 else
+                   {FH_PRECISION_FP8_INC: "fp8", FH_PRECISION_BF16_F: "bf16f"}.get(prec, "bf16"))
             extra = scheme_bytes(S, nbk, increments=inc, chebyshev=tab and nbk >= 16)
             out[name] = {"us": sec * 1e6, "alg_bytes": alg, "scheme_bytes": extra, "GBps": alg / sec / 1e9,
                          "frac": alg / sec / 1e9 / HBM_PEAK_GBS}
@@ -413,7 +418,10 @@ def main() -> None:
     plain_value = work / t_plain
     # ablations: each change alone
     ablation = {}
-    for name, prec in (("bf16_increments_only", FH_PRECISION_BF16_DC), ("fp8_increments_only", FH_PRECISION_FP8_INC)):
+    # (the BF16-F scheme: BF16 F and a BF16 copy of B until the BF16
+    # operator's fixed point, then FP32 sweeps as the defect correction)
+    for name, prec in (("bf16_increments_only", FH_PRECISION_BF16_DC), ("fp8_increments_only", FH_PRECISION_FP8_INC),
+                       ("bf16_weights_defect_correction", FH_PRECISION_BF16_F)):
         t_b, st_b = run(prec, FH_WARM_PREVIOUS)
         ablation[name] = {"value": work / t_b, "seconds": t_b, **st_b.summary()}
     if have_sur:

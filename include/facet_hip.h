@@ -95,11 +95,15 @@ typedef enum {
     FH_PRECISION_FP32 = 0,    /* FP32 F, FP32 accumulation                     */
     FH_PRECISION_BF16_DC = 1, /* BF16 F sweeps + FP32-F defect correction; from
                                  16 bands on: BF16 increments of B gathered   */
-    FH_PRECISION_FP8_INC = 2  /* from 16 bands on: FP8 increments of B
+    FH_PRECISION_FP8_INC = 2, /* from 16 bands on: FP8 increments of B
                                  gathered (12-band chunks with a shared
                                  power-of-two exponent), FP32 weights,
                                  accumulated gathers and carries; below 16
                                  bands FP32                                   */
+    FH_PRECISION_BF16_F = 3   /* BF16 F and a BF16 copy of B gathered until
+                                 the BF16 operator's fixed point, then FP32
+                                 sweeps to the FP32 one (defect correction);
+                                 below 16 bands as FH_PRECISION_BF16_DC       */
 } fh_precision;
 
 const char *fh_status_string(int status);
@@ -283,7 +287,9 @@ void *fh_stream(const fh_context *ctx);
  * coupled facet pairs are updated as blocks by a second kernel on a second
  * stream beside the first (joined before the next sweep), and BF16
  * precision times the increment sweep the BF16 solves run (BF16 increments
- * of B gathered with FP32 weights, FP32 accumulated gathers and carries).
+ * of B gathered with FP32 weights, FP32 accumulated gathers and carries),
+ * FH_PRECISION_FP8_INC the FP8 increment sweep and FH_PRECISION_BF16_F the
+ * sweep of its BF16 phase (BF16 F, BF16 copy of B gathered).
  * materialTables = 1 (band-parallel strides): the sweep in the form the
  * energy balance's shortwave solves run it -- r and t from per-material
  * tables, c and D from per-side direct factors, Chebyshev weights (the

@@ -19,6 +19,7 @@ FH_OVERFLOW_ERROR = 4
 FH_PRECISION_FP32 = 0
 FH_PRECISION_BF16_DC = 1
 FH_PRECISION_FP8_INC = 2
+FH_PRECISION_BF16_F = 3
 
 FH_WARM_PREVIOUS = 0
 FH_WARM_SURROGATE = 1

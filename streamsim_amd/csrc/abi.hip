@@ -667,7 +667,7 @@ int fh_solve_bands(fh_context* c, uint32_t nb, const float* r, const float* t, c
         require_graph(c, "solveBands()");
         validate_bands(c, nb, r, t, e, D, sky);
         validate_relax(relaxation);
-        if (precision > FH_PRECISION_FP8_INC) fail(FH_INVALID_ARGUMENT, "unknown precision");
+        if (precision > FH_PRECISION_BF16_F) fail(FH_INVALID_ARGUMENT, "unknown precision");
         if (!radiosity) fail(FH_INVALID_ARGUMENT, "radiosity must not be null");
         const size_t n = size_t(c->S) * nb;
         fh::DBuf<float> in;
@@ -699,7 +699,7 @@ int fh_solve_bands_device(fh_context* c, uint32_t nb, const float* dr, const flo
         if (!dr || !dt || !de || !dD || !sky || !dout) fail(FH_INVALID_ARGUMENT, "band arrays must not be null");
         for (uint32_t b = 0; b < nb; ++b) validate_sky(sky[b]);
         validate_relax(relaxation);
-        if (precision > FH_PRECISION_FP8_INC) fail(FH_INVALID_ARGUMENT, "unknown precision");
+        if (precision > FH_PRECISION_BF16_F) fail(FH_INVALID_ARGUMENT, "unknown precision");
         solve_bands_impl(c, nb, dr, dt, de, dD, sky, dinit, maxIter ? maxIter : 1u, tolRel, relaxation, precision,
                          dout, stats);
     });
@@ -809,7 +809,7 @@ int fh_solve_spectral(fh_context* c, const fh_spectral_optics* o, const float* s
         if (nb == 0u || nb > 4096u) fail(FH_INVALID_ARGUMENT, "band count must be in [1, 4096]");
         for (uint32_t b = 0; b < nb; ++b) validate_sky(sky[b]);
         validate_relax(relaxation);
-        if (precision > FH_PRECISION_FP8_INC) fail(FH_INVALID_ARGUMENT, "unknown precision");
+        if (precision > FH_PRECISION_BF16_F) fail(FH_INVALID_ARGUMENT, "unknown precision");
         const size_t n = size_t(c->S) * nb;
         fh::DBuf<float> in, stage;
         in.ensure(n * (init ? 6u : 5u));
@@ -935,7 +935,9 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
         fh::launch_fill_f32(c, B0, n, 50.0f);
         fh::launch_fill_f32(c, B1, n, 0.0f);
         const bool q8 = precision == FH_PRECISION_FP8_INC && fh::uses_bf16_gather(ldb);
-        const bool w16 = precision == FH_PRECISION_BF16_DC || q8;
+        // FH_PRECISION_BF16_F: the BF16 phase's sweep (BF16 F, BF16 copy of B)
+        const bool wf = precision == FH_PRECISION_BF16_F;
+        const bool w16 = precision == FH_PRECISION_BF16_DC || wf || q8;
         // BF16 solves of band-parallel strides run increment sweeps: BF16
         // increments gathered, accumulated gathers and carries in FP32; FP8
         // solves the same with FP8 increments
@@ -965,11 +967,13 @@ int fh_benchmark_sweep(fh_context* c, uint32_t nb, uint32_t precision, uint32_t 
             H0 = h16.p;
             H1 = h16.p + nh;
             fh::launch_fill_u32(c, reinterpret_cast<uint32_t*>(h16.p), nh, 0x3c233c23u);  // 0.01 in BF16
-            acc.ensure(2 * n);
-            G = acc.p;
-            K = acc.p + n;
-            fh::launch_fill_f32(c, G, n, 100.0f);
-            fh::launch_fill_f32(c, K, n, 0.0f);
+            if (!wf) {
+                acc.ensure(2 * n);
+                G = acc.p;
+                K = acc.p + n;
+                fh::launch_fill_f32(c, G, n, 100.0f);
+                fh::launch_fill_f32(c, K, n, 0.0f);
+            }
         }
         for (int i = 0; i < 3; ++i)
             fh::launch_single_sweep(c, ldb, cc, rr, tt, B0, B1, w16, H0, H1, G, K, true, nullptr, 0u, tab, Q0, Q1);

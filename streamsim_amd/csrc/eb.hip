@@ -1272,7 +1272,7 @@ int fh_eb_run_node(fh_context* c, const fh_eb_node_params* p, const float* swDir
         if (!(p->temperatureRelaxation > 0.0f && p->temperatureRelaxation <= 1.0f))
             fail(FH_INVALID_ARGUMENT, "temperature relaxation must be in (0,1]");
         if (!(p->temperatureTolerance > 0.0f)) fail(FH_INVALID_ARGUMENT, "temperature tolerance must be positive");
-        if (p->precision > FH_PRECISION_FP8_INC || p->warmStart > FH_WARM_SURROGATE)
+        if (p->precision > FH_PRECISION_BF16_F || p->warmStart > FH_WARM_SURROGATE)
             fail(FH_INVALID_ARGUMENT, "unknown precision or warm-start mode");
         fh::eb_run_node(c, *p, swDirectNormal, swDiffuse, stats);
     });

@@ -2106,7 +2106,7 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     const bool bp = band_parallel(o.ldb);
     // FP8 increments on band-parallel strides (below them the solve is FP32)
     const bool q8 = o.precision == FH_PRECISION_FP8_INC && bp;
-    const bool bf16 = o.precision == FH_PRECISION_BF16_DC || q8;
+    const bool bf16 = o.precision == FH_PRECISION_BF16_DC || o.precision == FH_PRECISION_BF16_F || q8;
     const bool retire = o.retire && bp && o.tolRel >= 0.0f;
     const bool half = bf16 && bp;
     // FH_DC_FIRST overrides the end of the BF16 phase (tuning knob)
@@ -2115,7 +2115,7 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     // their last-ulp changes do not settle: such solves resolve relative
     // changes down to kIncrementTolFloor (2.5 FP32 ulps)
     constexpr float kIncrementTolFloor = 3.0e-7f;
-    const bool incr = half && (q8 || use_increments());
+    const bool incr = half && (q8 || (o.precision == FH_PRECISION_BF16_DC && use_increments()));
     const float tolRel = incr && o.tolRel >= 0.0f ? std::max(o.tolRel, kIncrementTolFloor) : o.tolRel;
     launch_ctrl_reset(c, o.maxIter ? o.maxIter : 1u, tolRel, o.tolAbs, o.checkEvery, o.ldb,
                       half ? std::max(firstInner, o.tolRel) : o.tolRel);

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 from streamsim_amd import faceteb, optics, scenes
-from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP8_INC, FH_PRECISION_FP32
+from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_BF16_F, FH_PRECISION_FP8_INC, FH_PRECISION_FP32
 
 pytestmark = pytest.mark.gpu
 
@@ -622,6 +622,11 @@ def test_eb_diurnal_plain_vs_bf16(oracle, nb):
     T3 = eng.ebGetState()["temperature"]
     assert all(s3.converged)
     assert np.max(np.abs(T1 - T3)) < 0.02
+    # BF16 F with FP32 defect correction (from 33 windows on) as well
+    s4 = faceteb.run_diurnal(eng, met, win, cfg, precision=FH_PRECISION_BF16_F)
+    T4 = eng.ebGetState()["temperature"]
+    assert all(s4.converged)
+    assert np.max(np.abs(T1 - T4)) < 0.02
 
 
 def test_mlp_gradient_matches_oracle(oracle):
@@ -818,6 +823,11 @@ def test_bf16_increments_follow_fp32(oracle, nb):
     z32 = eng.solveBands(r, t, e, D, sky, tolRel=1e-7, precision=FH_PRECISION_FP32)
     z8 = eng.solveBands(r, t, e, D, sky, tolRel=1e-7, precision=FH_PRECISION_FP8_INC)
     assert z8.iterations <= z32.iterations + 2 and _rel(z8.radiosity, ref) < RTOL
+    # BF16 F and a BF16 copy of B, then FP32 sweeps (defect correction): the
+    # FP32 fixed point, after the BF16 phase's sweeps
+    bf = eng.solveBands(r, t, e, D, sky, init=init, tolRel=1e-7, precision=FH_PRECISION_BF16_F)
+    print(f"  bf16 F + defect correction: {bf.iterations} sweeps", flush=True)
+    assert _rel(bf.radiosity, ref) < RTOL and _rel(bf.radiosity, f32.radiosity) < 2e-6
 
 
 class _Hip:
