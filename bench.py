@@ -156,16 +156,17 @@ def cpu_baseline(eng, scene, windows, meteo, cfg, nb_total: int, budget_s: float
 
 def precision_report(eng, scene, windows, meteo, cfg) -> dict:
     """Per-band error of the GPU shortwave multi-band solve on the brightest
-    node against the FP64 oracle on the same graph and inputs: FP32 sweeps
-    and the BF16 solve (BF16 increments gathered, FP32 weights and
-    accumulation), each to relative change 1e-7 and to the run's 1e-5; the
+    node against the FP64 oracle on the same graph and inputs: FP32 sweeps,
+    the BF16 solve (BF16 increments gathered, FP32 weights and
+    accumulation), the FP8 increments and BF16 F with FP32 defect
+    correction, each to relative change 1e-7 and to the run's 1e-5; the
     oracle to 1e-9 (FP64 accumulation, its threads).  Error per band: max
     |B - B_ref| / B_ref over the sides with B_ref above 1e-3 of the band's
     maximum."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_ref as O
     from streamsim_amd import faceteb, optics
-    from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_FP32
+    from streamsim_amd._lib import FH_PRECISION_BF16_DC, FH_PRECISION_BF16_F, FH_PRECISION_FP8_INC, FH_PRECISION_FP32
 
     g = eng.getGraph()
     og = O.Graph(g.rowOffsets, g.neighbors, g.counts, g.denominators, g.skyCounts)
@@ -189,7 +190,8 @@ def precision_report(eng, scene, windows, meteo, cfg) -> dict:
     mask = ref > 1e-3 * ref.max(axis=0, keepdims=True)
     out = {"node_hour": met.hour, "bands": int(windows.count), "sides": int(scene.surface_count),
            "oracle": {"iterations": it_ref, "seconds": t_ref, "threads": O.max_threads()}}
-    for name, prec in (("fp32", FH_PRECISION_FP32), ("bf16", FH_PRECISION_BF16_DC)):
+    for name, prec in (("fp32", FH_PRECISION_FP32), ("bf16", FH_PRECISION_BF16_DC), ("fp8", FH_PRECISION_FP8_INC),
+                       ("bf16f", FH_PRECISION_BF16_F)):
         for tol in (1e-7, cfg.radiosity_tolerance):
             eng.synchronize()
             t1 = time.perf_counter()
@@ -200,7 +202,7 @@ def precision_report(eng, scene, windows, meteo, cfg) -> dict:
             err = [float(np.max(np.abs(got[mask[:, b], b] - ref[mask[:, b], b]) / ref[mask[:, b], b]))
                    for b in range(windows.count)]
             out[f"{name}_tol{tol:g}"] = {"iterations": res.iterations, "seconds": dt, "max_rel_error": max(err),
-                                         "max_rel_error_per_band": err}
+                                         "last_rel_change": res.maxRelDelta, "max_rel_error_per_band": err}
     return out
 
 

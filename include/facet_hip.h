@@ -102,8 +102,9 @@ typedef enum {
                                  bands FP32                                   */
     FH_PRECISION_BF16_F = 3   /* BF16 F and a BF16 copy of B gathered until
                                  the BF16 operator's fixed point, then FP32
-                                 sweeps to the FP32 one (defect correction);
-                                 below 16 bands as FH_PRECISION_BF16_DC       */
+                                 sweeps to the FP32 one (defect correction),
+                                 stopping test max(tolRel, 3e-7); below 16
+                                 bands as FH_PRECISION_BF16_DC                */
 } fh_precision;
 
 const char *fh_status_string(int status);
