@@ -296,7 +296,7 @@ inline Image decodeJpeg(const std::vector<uint8_t>& b) {
             size_t p = seg;
             while (p < end) {
                 const int pq = b[p] >> 4, tq = b[p] & 15;
-                if (tq > 3) throw std::runtime_error("bad JPEG quantisation table");
+                if (tq > 3 || p + 1 + (pq ? 128u : 64u) > end) throw std::runtime_error("bad JPEG quantisation table");
                 ++p;
                 for (int k = 0; k < 64; ++k) {
                     const int v = pq ? u16(p + 2 * size_t(k)) : b.at(p + size_t(k));
@@ -317,8 +317,10 @@ inline Image decodeJpeg(const std::vector<uint8_t>& b) {
                 p += 17 + size_t(n);
             }
         } else if (m == 0xDD) {
+            if (end < seg + 2) throw std::runtime_error("bad JPEG restart interval");
             restartInterval = u16(seg);
         } else if (m == 0xC0 || m == 0xC1) {
+            if (end < seg + 6 || end < seg + 6 + 3 * size_t(b[seg + 5])) throw std::runtime_error("bad JPEG frame");
             if (b[seg] != 8) throw std::runtime_error("only 8-bit JPEG samples are read");
             height = u16(seg + 1);
             width = u16(seg + 3);
@@ -350,6 +352,7 @@ inline Image decodeJpeg(const std::vector<uint8_t>& b) {
         } else if (m == 0xDA) {
             if (!frame) throw std::runtime_error("JPEG scan before its frame");
             const int ns = b[seg];
+            if (ns < 1 || end < seg + 1 + 2 * size_t(ns)) throw std::runtime_error("bad JPEG scan header");
             std::vector<Component*> sc;
             for (int i = 0; i < ns; ++i) {
                 const int id = b[seg + 1 + 2 * size_t(i)], t = b[seg + 2 + 2 * size_t(i)];
@@ -489,6 +492,7 @@ inline Image decodePng(const std::vector<uint8_t>& b) {
         const size_t d = pos + 8;
         if (d + len + 4 > b.size()) throw std::runtime_error("truncated PNG chunk");
         if (kind == "IHDR") {
+            if (len < 13) throw std::runtime_error("bad PNG header");
             width = u32(d);
             height = u32(d + 4);
             depth = b[d + 8];
