@@ -82,10 +82,10 @@ public:
         --count_;
         return int((acc_ >> count_) & 1u);
     }
-    int bits(int n) {
-        int v = 0;
-        for (int i = 0; i < n; ++i) v = (v << 1) | bit();
-        return v;
+    int bits(int n) {  // n <= 16
+        uint32_t v = 0;
+        for (int i = 0; i < n; ++i) v = (v << 1) | uint32_t(bit());
+        return int(v);
     }
     // discards the partial byte and expects RSTn
     void restart() {
@@ -325,7 +325,8 @@ inline Image decodeJpeg(const std::vector<uint8_t>& b) {
             height = u16(seg + 1);
             width = u16(seg + 3);
             const int nc = b[seg + 5];
-            if (width < 1 || height < 1 || (nc != 1 && nc != 3)) throw std::runtime_error("unsupported JPEG frame");
+            if (width < 1 || height < 1 || (nc != 1 && nc != 3) || size_t(width) * size_t(height) > (size_t(1) << 28))
+                throw std::runtime_error("unsupported JPEG frame");
             comps.assign(size_t(nc), Component{});
             for (int i = 0; i < nc; ++i) {
                 Component& c = comps[size_t(i)];
@@ -369,6 +370,7 @@ inline Image decodeJpeg(const std::vector<uint8_t>& b) {
             const auto block = [&](Component& c, int bx, int by) {
                 int16_t* z = &c.coef[(size_t(by) * c.blocksX + bx) * 64];
                 const int t = decode(br, dc[c.td & 3]);
+                if (t > 16) throw std::runtime_error("corrupt JPEG DC code");
                 c.dcPred += t ? extend(br.bits(t), t) : 0;
                 z[0] = int16_t(c.dcPred);
                 for (int k = 1; k < 64;) {
