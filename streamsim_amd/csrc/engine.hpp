@@ -241,6 +241,10 @@ struct fh_context {
     // scratch for the ray graph build
     fh::DBuf<uint32_t> hits, rowlen, scanTmp;
     fh::DBuf<uint16_t> cnt16;
+    // raster graph build: fragments of the largest direction and directed
+    // exchanges of all directions measured by the last build (the first
+    // capacities of the next one)
+    size_t rasterFragHint = 0, rasterPairHint = 0;
 
     // solve workspace
     fh::DBuf<float> B0, B1, cc, rr, tt, DD, c2, HH, skyv;

@@ -16,10 +16,10 @@
 // depends on the atomic order (the reference's GPU path has the same
 // property); the overflow count is reported.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "solver.hpp"
 
@@ -48,16 +48,62 @@ __device__ __forceinline__ float edgef(float ax, float ay, float bx, float by, f
     return (px - ax) * (by - ay) - (py - ay) * (bx - ax);
 }
 
+// Per direction the A-buffer is built in two passes over the same pixel
+// coverage: the first counts each pixel's fragments, an exclusive scan of the
+// counts (each clamped to `layers`) gives every pixel a contiguous fragment
+// list, the second writes the fragments.  The write pass counts each pixel
+// down from its total, so the slot a fragment takes depends on the atomic
+// order; slots at or beyond `layers` are the overflow (the reference's
+// A-buffer keeps the first `layers` arrivals, an order-dependent subset too).
+enum class Pass { Count, Graph, Sun };
+
+// adds 1 to arr[idx] for every active lane, one atomic per distinct index of
+// the wave (neighbouring pixels mostly share their nearest facet)
+__device__ __forceinline__ void wave_count(uint32_t* __restrict__ arr, uint32_t idx, bool active) {
+    unsigned long long pending = __ballot(active);
+    const uint32_t lane = threadIdx.x & 63u;
+    while (pending) {
+        const uint32_t leader = uint32_t(__builtin_ctzll(pending));
+        const uint32_t lidx = __shfl(idx, int(leader), 64);
+        const unsigned long long same = __ballot(active && idx == lidx) & pending;
+        if (lane == leader) atomicAdd(&arr[lidx], uint32_t(__popcll(same)));
+        pending &= ~same;
+    }
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= uint32_t(d)) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
 // one wave per (tile, facet) in submission order, its lanes striding over
 // the pixels of the triangle's clamped bounding box (a building wall covers
 // 10^5 pixels, a leaf a few: per-thread triangles left one lane looping over
-// the wall); fragments are (depth bits, tile << 27 | camera side)
+// the wall); fragments are (depth bits, tile << 27 | camera side).
+// Pass::Count adds to the pixel counts; the write passes place the fragment
+// at off[pixel] + slot (below fragCap: a short capacity is measured by the
+// caller and the build runs again) and count the facet's kept fragments:
+// Pass::Graph for the denominators of both sides (every kept fragment adds to
+// both, core:1368-1372), Pass::Sun for the camera side's pixel total
+// (core:1456-1460).
+template <Pass P>
 __global__ __launch_bounds__(256) void k_raster(const float* __restrict__ fverts, const float4* __restrict__ fnorm,
                                                 uint32_t F, RasterView rv, uint32_t* __restrict__ counts,
-                                                uint2* __restrict__ frags, uint32_t* __restrict__ overflow) {
+                                                const uint32_t* __restrict__ off, uint2* __restrict__ frags,
+                                                size_t fragCap, uint32_t* __restrict__ kept,
+                                                uint32_t* __restrict__ overflow) {
     const uint64_t i = uint64_t(blockIdx.x) * (blockDim.x / 64u) + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63u;
-    if (i >= uint64_t(F) * rv.tiles) return;
+    if (i >= uint64_t(F) * rv.tiles) return;  // whole waves
     const uint32_t t = uint32_t(i / F), f = uint32_t(i - uint64_t(t) * F);
     const float tx = float(kRasterTiles[t][0]) * rv.px, tz = float(kRasterTiles[t][1]) * rv.pz;
     float x[3], y[3], d[3];
@@ -89,6 +135,7 @@ __global__ __launch_bounds__(256) void k_raster(const float* __restrict__ fverts
     const uint32_t npx = bw * uint32_t(y1 - y0 + 1);  // <= W H <= 2^28
     const uint32_t key = (t << kTileShift) | side;
     const float invArea = 1.0f / area;
+    uint32_t keptN = 0, over = 0;
     for (uint32_t q = lane; q < npx; q += 64u) {
         const uint32_t row = q / bw;
         const int py = y0 + int(row), pxl = x0 + int(q - row * bw);
@@ -100,13 +147,35 @@ __global__ __launch_bounds__(256) void k_raster(const float* __restrict__ fverts
                                         : (w0 <= 0.0f && w1 <= 0.0f && w2 <= 0.0f);
         if (!inside) continue;
         const size_t pixel = size_t(py) * rv.W + size_t(pxl);
-        const uint32_t layer = atomicAdd(&counts[pixel], 1u);
-        if (layer >= rv.L) {
-            atomicAdd(overflow, 1u);
-            continue;
+        if constexpr (P == Pass::Count) {
+            atomicAdd(&counts[pixel], 1u);
+        } else {
+            const uint32_t slot = atomicSub(&counts[pixel], 1u) - 1u;
+            if (slot >= rv.L) {
+                ++over;
+                continue;
+            }
+            const size_t at = size_t(off[pixel]) + slot;
+            if (at >= fragCap) continue;
+            const float depth = (w0 * d[0] + w1 * d[1] + w2 * d[2]) * invArea;
+            frags[at] = make_uint2(__float_as_uint(depth), key);
+            ++keptN;
         }
-        const float depth = (w0 * d[0] + w1 * d[1] + w2 * d[2]) * invArea;
-        frags[size_t(layer) * (size_t(rv.W) * rv.H) + pixel] = make_uint2(__float_as_uint(depth), key);
+    }
+    if constexpr (P != Pass::Count) {
+        keptN = wave_sum(keptN);
+        over = wave_sum(over);
+        if (lane == 0u) {
+            if (keptN) {
+                if constexpr (P == Pass::Graph) {
+                    atomicAdd(&kept[f * 2u], keptN);
+                    atomicAdd(&kept[f * 2u + 1u], keptN);
+                } else {
+                    atomicAdd(&kept[side], keptN);
+                }
+            }
+            if (over) atomicAdd(overflow, over);
+        }
     }
 }
 
@@ -115,92 +184,101 @@ __device__ __forceinline__ bool frag_less(uint2 a, uint2 b) {
     return da < db || (da == db && a.y < b.y);
 }
 
-// sorts a pixel's fragments in place (insertion sort; lists are short)
-// (fragment i of a pixel at frags[i * pixels + pixel]: layer-major, so the
-// lanes of a wave, neighbouring pixels, read and write neighbouring entries)
-__device__ __forceinline__ uint32_t pixel_sorted(const uint32_t* __restrict__ counts, uint2* __restrict__ frags,
-                                                 size_t pixel, size_t pixels, uint32_t L) {
-    const uint32_t cnt = min(counts[pixel], L);
-    uint2* fr = frags + pixel;
-    for (uint32_t i = 1; i < cnt; ++i) {
-        const uint2 v = fr[i * pixels];
-        uint32_t j = i;
-        while (j > 0u && frag_less(v, fr[(j - 1u) * pixels])) {
-            fr[j * pixels] = fr[(j - 1u) * pixels];
-            --j;
-        }
-        fr[j * pixels] = v;
-    }
-    return cnt;
-}
-
 __device__ __forceinline__ uint32_t frag_side(uint2 v) { return v.y & ((1u << kTileShift) - 1u); }
 
-// upper bound of this direction's directed exchange pairs: 2 (count - 1)
-// summed over the pixels
-__global__ __launch_bounds__(256) void k_pair_bound(const uint32_t* __restrict__ counts, size_t pixels, uint32_t L,
-                                                    unsigned long long* __restrict__ total) {
-    unsigned long long s = 0;
-    for (size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x; p < pixels; p += size_t(gridDim.x) * blockDim.x) {
-        const uint32_t c = min(counts[p], L);
-        s += c > 1u ? 2ull * (c - 1u) : 0ull;
-    }
-    for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
-    if ((threadIdx.x & 63u) == 0u) atomicAdd(total, s);
-}
+// the largest fragment total of a direction (the scan's last offset)
+__global__ void k_note_max(const uint32_t* __restrict__ v, uint32_t* __restrict__ mx) { atomicMax(mx, *v); }
 
-// graph resolve (buildCpuGraph's pixel loop, core:1361-1396): every
-// fragment adds to the denominators of both sides of its facet, the first
-// camera side and the last back side see the sky, and consecutive surfaces
-// of different facets exchange (back side of the nearer, camera side of the
-// farther), emitted as both directed pairs
-__global__ __launch_bounds__(256) void k_graph_resolve(const uint32_t* __restrict__ counts, uint2* __restrict__ frags,
-                                                       size_t pixels, uint32_t L, uint32_t* __restrict__ den,
-                                                       uint32_t* __restrict__ sky,
-                                                       unsigned long long* __restrict__ pairs,
+// graph resolve (buildCpuGraph's pixel loop, core:1361-1396), one thread per
+// pixel: its fragments sorted by (depth, submission order) in place
+// (insertion sort; lists are short), the first camera side and the last back
+// side see the sky, and consecutive surfaces of different facets exchange
+// (back side of the nearer, camera side of the farther), emitted as both
+// directed pairs a << sbits | b.  A wave reserves its pairs with one atomic
+// on the cursor (a prefix sum over the lanes); pairs beyond cap are counted,
+// not written, and the caller runs the build again with the measured total.
+// Pixels whose list lies beyond fragCap are skipped (that build is repeated).
+// The write pass counts every pixel back to zero when both passes covered the
+// same pixels; a pixel left nonzero is flagged in *mismatch and skipped, so
+// no unwritten slot is read.
+template <class K>
+__global__ __launch_bounds__(256) void k_graph_resolve(const uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ off, uint2* __restrict__ frags,
+                                                       size_t pixels, size_t fragCap, uint32_t sbits,
+                                                       uint32_t* __restrict__ sky, K* __restrict__ pairs,
                                                        unsigned long long* __restrict__ cursor,
-                                                       unsigned long long cap) {
+                                                       unsigned long long cap, uint32_t* __restrict__ mismatch) {
     const size_t pixel = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (pixel >= pixels) return;
-    const uint32_t cnt = pixel_sorted(counts, frags, pixel, pixels, L);
-    if (cnt == 0u) return;
-    const uint2* fr = frags + pixel;
-    for (uint32_t i = 0; i < cnt; ++i) {
-        const uint32_t s = frag_side(fr[i * pixels]);
-        atomicAdd(&den[s], 1u);
-        atomicAdd(&den[s ^ 1u], 1u);
+    uint32_t cnt = 0;
+    size_t b = 0;
+    if (pixel < pixels) {
+        b = off[pixel];
+        const size_t e = off[pixel + 1u];
+        if (counts[pixel] != 0u)
+            atomicOr(mismatch, 1u);
+        else if (e <= fragCap)
+            cnt = uint32_t(e - b);
     }
-    atomicAdd(&sky[frag_side(fr[0])], 1u);
-    atomicAdd(&sky[frag_side(fr[(cnt - 1u) * pixels]) ^ 1u], 1u);
-    for (uint32_t i = 0; i + 1u < cnt; ++i) {
-        const uint32_t a = frag_side(fr[i * pixels]) ^ 1u, b = frag_side(fr[(i + 1u) * pixels]);
-        if ((a >> 1) == (b >> 1)) continue;
-        const unsigned long long slot = atomicAdd(cursor, 2ull);
-        if (slot + 2ull <= cap) {  // cap is the exact bound: always true
-            pairs[slot] = (static_cast<unsigned long long>(a) << 32) | b;
-            pairs[slot + 1ull] = (static_cast<unsigned long long>(b) << 32) | a;
+    uint2* fr = frags + b;
+    for (uint32_t i = 1; i < cnt; ++i) {
+        const uint2 v = fr[i];
+        uint32_t j = i;
+        while (j > 0u && frag_less(v, fr[j - 1u])) {
+            fr[j] = fr[j - 1u];
+            --j;
         }
+        fr[j] = v;
+    }
+    uint32_t first = 0, last = 0, np = 0;
+    if (cnt) {
+        first = frag_side(fr[0]);
+        last = frag_side(fr[cnt - 1u]) ^ 1u;
+        for (uint32_t i = 0; i + 1u < cnt; ++i)
+            if ((frag_side(fr[i]) >> 1) != (frag_side(fr[i + 1u]) >> 1)) np += 2u;
+    }
+    wave_count(sky, first, cnt != 0u);
+    wave_count(sky, last, cnt != 0u);
+    const uint32_t incl = wave_inclusive_sum(np);
+    const uint32_t waveTotal = __shfl(incl, 63, 64);
+    unsigned long long base = 0;
+    if ((threadIdx.x & 63u) == 63u && waveTotal) base = atomicAdd(cursor, static_cast<unsigned long long>(waveTotal));
+    base = __shfl(base, 63, 64);
+    unsigned long long slot = base + (incl - np);
+    for (uint32_t i = 0; np && i + 1u < cnt; ++i) {
+        const uint32_t a = frag_side(fr[i]) ^ 1u, c = frag_side(fr[i + 1u]);
+        if ((a >> 1) == (c >> 1)) continue;
+        if (slot + 2ull <= cap) {
+            pairs[slot] = (K(a) << sbits) | K(c);
+            pairs[slot + 1ull] = (K(c) << sbits) | K(a);
+        }
+        slot += 2ull;
     }
 }
 
-// sunlit resolve (computeCpuSunlit, core:1438-1474): every fragment counts
-// for its side, the nearest one (first in submission order among equal
-// depths) is visible
+// sunlit resolve (computeCpuSunlit, core:1438-1474): the nearest fragment of
+// each pixel (first in submission order among equal depths) is visible; the
+// per-side totals come from the write pass
 __global__ __launch_bounds__(256) void k_sun_resolve(const uint32_t* __restrict__ counts,
-                                                     const uint2* __restrict__ frags, size_t pixels, uint32_t L,
-                                                     uint32_t* __restrict__ total, uint32_t* __restrict__ visible) {
+                                                     const uint32_t* __restrict__ off, const uint2* __restrict__ frags,
+                                                     size_t pixels, uint32_t* __restrict__ visible,
+                                                     uint32_t* __restrict__ mismatch) {
     const size_t pixel = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (pixel >= pixels) return;
-    const uint32_t cnt = min(counts[pixel], L);
-    if (cnt == 0u) return;
-    const uint2* fr = frags + pixel;
-    uint2 best = fr[0];
-    for (uint32_t i = 0; i < cnt; ++i) {
-        const uint2 v = fr[i * pixels];
-        atomicAdd(&total[frag_side(v)], 1u);
+    uint32_t cnt = 0;
+    size_t b = 0;
+    if (pixel < pixels) {
+        b = off[pixel];
+        if (counts[pixel] != 0u)
+            atomicOr(mismatch, 1u);
+        else
+            cnt = uint32_t(off[pixel + 1u] - b);
+    }
+    uint2 best = make_uint2(0u, 0u);
+    if (cnt) best = frags[b];
+    for (uint32_t i = 1; i < cnt; ++i) {
+        const uint2 v = frags[b + i];
         if (frag_less(v, best)) best = v;
     }
-    atomicAdd(&visible[frag_side(best)], 1u);
+    wave_count(visible, frag_side(best), cnt != 0u);
 }
 
 __global__ void k_sun_fraction_raster(const uint32_t* __restrict__ total, const uint32_t* __restrict__ visible,
@@ -209,15 +287,119 @@ __global__ void k_sun_fraction_raster(const uint32_t* __restrict__ total, const 
     if (s < S) frac[s] = total[s] ? float(visible[s]) / float(total[s]) : 0.0f;
 }
 
-__global__ void k_split_keys(const unsigned long long* __restrict__ keys, const uint32_t* __restrict__ vals,
-                             uint32_t n, uint32_t* __restrict__ rowlen, uint32_t* __restrict__ col,
-                             uint32_t* __restrict__ cnt) {
+// ---- pair sort: LSD radix sort, 8-bit digits, tiles of 2048 keys ----------
+// Per pass: each tile's digit histogram (digit-major over the tiles), an
+// exclusive scan of it gives every (digit, tile) its output start, and the
+// scatter places each key after the equal digits of earlier tiles, earlier
+// rounds of its tile, earlier waves of its round and earlier lanes of its
+// wave (ranks from eight ballots over the digit bits): stable, so the passes
+// from the low digit up sort the keys.
+constexpr uint32_t kSortRounds = 8u, kSortTile = 256u * kSortRounds;
+
+template <class K>
+__global__ __launch_bounds__(256) void k_digit_hist(const K* __restrict__ keys, uint32_t n, uint32_t shift,
+                                                    uint32_t nTiles, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const size_t base = size_t(blockIdx.x) * kSortTile;
+    for (uint32_t r = 0; r < kSortRounds; ++r) {
+        const size_t i = base + r * 256u + threadIdx.x;
+        if (i < n) atomicAdd(&h[uint32_t(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[size_t(threadIdx.x) * nTiles + blockIdx.x] = h[threadIdx.x];
+}
+
+template <class K>
+__global__ __launch_bounds__(256) void k_digit_scatter(const K* __restrict__ in, K* __restrict__ out, uint32_t n,
+                                                       uint32_t shift, uint32_t nTiles,
+                                                       const uint32_t* __restrict__ start) {
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t waveCnt[4][256];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    running[tid] = start[size_t(tid) * nTiles + blockIdx.x];
+    for (uint32_t v = 0; v < 4u; ++v) waveCnt[v][tid] = 0u;
+    __syncthreads();
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const size_t base = size_t(blockIdx.x) * kSortTile;
+    for (uint32_t r = 0; r < kSortRounds; ++r) {
+        const size_t i = base + r * 256u + tid;
+        const bool valid = i < n;
+        const K key = valid ? in[i] : K(0);
+        const uint32_t dg = uint32_t(key >> shift) & 255u;
+        unsigned long long same = __ballot(valid);
+        for (uint32_t bit = 0; bit < 8u; ++bit) {
+            const bool set = (dg >> bit) & 1u;
+            const unsigned long long bb = __ballot(set);
+            same &= set ? bb : ~bb;
+        }
+        const uint32_t rank = uint32_t(__popcll(same & below));
+        if (valid && rank == 0u) waveCnt[w][dg] = uint32_t(__popcll(same));
+        __syncthreads();
+        if (valid) {
+            uint32_t at = running[dg];
+            for (uint32_t v = 0; v < w; ++v) at += waveCnt[v][dg];
+            out[at + rank] = key;
+        }
+        __syncthreads();
+        running[tid] += waveCnt[0][tid] + waveCnt[1][tid] + waveCnt[2][tid] + waveCnt[3][tid];
+        for (uint32_t v = 0; v < 4u; ++v) waveCnt[v][tid] = 0u;
+        __syncthreads();
+    }
+}
+
+// run-length encoding of the sorted pairs: head flags, their scan numbers
+// the distinct pairs, each head records its key and position
+template <class K>
+__global__ void k_run_heads(const K* __restrict__ keys, uint32_t n, uint32_t* __restrict__ head) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) head[i] = (i == 0u || keys[i] != keys[i - 1u]) ? 1u : 0u;
+}
+
+template <class K>
+__global__ void k_run_starts(const K* __restrict__ keys, uint32_t n, const uint32_t* __restrict__ headIdx,
+                             K* __restrict__ uniq, uint32_t* __restrict__ pos) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const unsigned long long k = keys[i];
-    col[i] = uint32_t(k & 0xffffffffull);
-    cnt[i] = vals[i];
-    atomicAdd(&rowlen[uint32_t(k >> 32)], 1u);
+    if (i == 0u || keys[i] != keys[i - 1u]) {
+        uniq[headIdx[i]] = keys[i];
+        pos[headIdx[i]] = i;
+    }
+    if (i == n - 1u) pos[headIdx[n]] = n;
+}
+
+// CSR entries from the distinct pairs: column and multiplicity (the run
+// length)
+template <class K>
+__global__ void k_csr_entries(const K* __restrict__ keys, const uint32_t* __restrict__ pos, uint32_t n,
+                              uint32_t sbits, uint32_t* __restrict__ col, uint32_t* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    col[i] = uint32_t(keys[i] & ((K(1) << sbits) - K(1)));
+    cnt[i] = pos[i + 1u] - pos[i];
+}
+
+// row offsets: the first sorted pair of each row (binary search)
+template <class K>
+__global__ void k_csr_rows(const K* __restrict__ keys, uint32_t n, uint32_t sbits, uint32_t S,
+                           uint32_t* __restrict__ rowptr) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > S) return;
+    if (r == S) {
+        rowptr[S] = n;
+        return;
+    }
+    const K target = K(r) << sbits;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if (keys[mid] < target)
+            lo = mid + 1u;
+        else
+            hi = mid;
+    }
+    rowptr[r] = lo;
 }
 
 // host twins of the CPU rasterizer's vector helpers (same float operations)
@@ -290,23 +472,131 @@ void check_raster(const fh_context* c, uint32_t W, uint32_t H, uint32_t layers, 
     }
 }
 
-// rasterises one direction into counts / frags; returns the overflow count
-uint32_t raster_pass(fh_context* c, const RasterView& rv, DBuf<uint32_t>& counts, DBuf<uint2>& frags,
-                     DBuf<uint32_t>& ovf) {
-    const size_t pixels = size_t(rv.W) * rv.H;
-    counts.ensure(pixels);
-    frags.ensure(pixels * rv.L);
-    ovf.ensure(1);
-    launch_fill_u32(c, counts.p, pixels, 0u);
-    launch_fill_u32(c, ovf.p, 1, 0u);
+// bits of a side index: pair keys are a << bits | b
+uint32_t side_bits(uint32_t S) {
+    uint32_t b = 1;
+    while ((uint64_t(1) << b) < S) ++b;
+    return b;
+}
+
+template <Pass P>
+void launch_raster(fh_context* c, const RasterView& rv, uint32_t* counts, const uint32_t* off, uint2* frags,
+                   size_t fragCap, uint32_t* kept, uint32_t* overflow) {
     const uint64_t n = uint64_t(c->F) * rv.tiles;  // one wave each, 4 per block
-    hipLaunchKernelGGL(k_raster, dim3(uint32_t((n + 3u) / 4u)), dim3(256), 0, c->stream, c->facetVerts.p,
-                       c->facetNormal.p, c->F, rv, counts.p, frags.p, ovf.p);
+    hipLaunchKernelGGL(k_raster<P>, dim3(uint32_t((n + 3u) / 4u)), dim3(256), 0, c->stream, c->facetVerts.p,
+                       c->facetNormal.p, c->F, rv, counts, off, frags, fragCap, kept, overflow);
     FH_HIP(hipGetLastError());
-    uint32_t o = 0;
-    FH_HIP(hipMemcpyAsync(&o, ovf.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+}
+
+// The directions run back to back without a host round trip: the fragment
+// lists and the directed pairs of all directions go to buffers of a guessed
+// capacity (the last build's measurements, else 4 fragments per pixel and
+// one pair per 2 pixels and direction), the kernels count what did not fit,
+// and the build runs again at the measured sizes (without overflow the
+// counts are deterministic, so the second run fits unless the first was
+// short of fragment space and the pairs were undercounted: a third run).
+// The pairs are then radix sorted once (2 x side bits) and run-length
+// encoded into the CSR: the multiplicity of a pair is its ray count.
+template <class K>
+uint32_t raster_graph(fh_context* c, const float* dirs, uint32_t ndirs, uint32_t W, uint32_t H, uint32_t layers,
+                      uint32_t sbits) {
+    const uint32_t S = c->S;
+    const size_t pixels = size_t(W) * H;
+    std::vector<RasterView> views(ndirs);
+    for (uint32_t d = 0; d < ndirs; ++d) views[d] = make_view(c, W, H, layers, dirs + 3u * d);
+    DBuf<uint32_t> counts, off, st;  // st: largest fragment total, overflow, pass mismatch
+    DBuf<uint2> frags;
+    DBuf<K> pairs, alt;
+    DBuf<unsigned long long> cursor;
+    counts.ensure(pixels);
+    off.ensure(pixels + 1u);
+    st.ensure(3);
+    cursor.ensure(1);
+    c->den.ensure(S);
+    c->sky.ensure(S);
+    size_t fragCap = c->rasterFragHint ? c->rasterFragHint : std::min(4u * pixels, pixels * layers);
+    size_t pairCap = c->rasterPairHint ? c->rasterPairHint : std::max<size_t>(pixels * ndirs / 2u, 65536u);
+    unsigned long long total = 0;
+    uint32_t hs[3] = {0u, 0u, 0u};
+    for (int attempt = 0;; ++attempt) {
+        frags.ensure(std::max<size_t>(fragCap, 1u));
+        pairs.ensure(std::max<size_t>(pairCap, 1u));
+        launch_fill_u32(c, c->den.p, S, 0u);
+        launch_fill_u32(c, c->sky.p, S, 0u);
+        launch_fill_u32(c, st.p, 3, 0u);
+        launch_fill_u32(c, reinterpret_cast<uint32_t*>(cursor.p), 2, 0u);
+        for (uint32_t d = 0; d < ndirs; ++d) {
+            const RasterView& rv = views[d];
+            launch_fill_u32(c, counts.p, pixels, 0u);
+            launch_raster<Pass::Count>(c, rv, counts.p, nullptr, nullptr, 0, nullptr, nullptr);
+            launch_scan(c, counts.p, uint32_t(pixels), off.p, layers);
+            hipLaunchKernelGGL(k_note_max, dim3(1), dim3(1), 0, c->stream, off.p + pixels, st.p);
+            launch_raster<Pass::Graph>(c, rv, counts.p, off.p, frags.p, fragCap, c->den.p, st.p + 1);
+            hipLaunchKernelGGL(k_graph_resolve<K>, dim3(uint32_t((pixels + 255u) / 256u)), dim3(256), 0, c->stream,
+                               counts.p, off.p, frags.p, pixels, fragCap, sbits, c->sky.p, pairs.p, cursor.p,
+                               static_cast<unsigned long long>(pairCap), st.p + 2);
+            FH_HIP(hipGetLastError());
+        }
+        FH_HIP(hipMemcpyAsync(hs, st.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+        FH_HIP(hipMemcpyAsync(&total, cursor.p, sizeof(total), hipMemcpyDeviceToHost, c->stream));
+        FH_HIP(hipStreamSynchronize(c->stream));
+        if (hs[2]) fail(FH_RUNTIME_ERROR, "raster graph: the count and write passes covered different pixels");
+        if (hs[1]) return hs[1];  // overflow: no graph
+        const bool fragOk = hs[0] <= fragCap, pairOk = total <= pairCap;
+        if (fragOk && pairOk) break;
+        if (attempt == 2) fail(FH_RUNTIME_ERROR, "raster graph: buffer sizes did not settle");
+        if (!fragOk) fragCap = hs[0];
+        if (!pairOk) pairCap = size_t(total);
+    }
+    c->rasterFragHint = hs[0];
+    c->rasterPairHint = size_t(total);
+    counts.release();
+    frags.release();
+    uint32_t nnz = 0;
+    DBuf<K> uniq;
+    DBuf<uint32_t> hist, start, head, headIdx, pos;
+    const K* sortedKeys = nullptr;
+    if (total) {
+        if (total > 0x7fffffffull) fail(FH_OVERFLOW_ERROR, "raster graph: too many exchanges");
+        const uint32_t n = uint32_t(total);
+        const uint32_t nTiles = (n + kSortTile - 1u) / kSortTile;
+        alt.ensure(total);
+        hist.ensure(size_t(nTiles) * 256u);
+        start.ensure(size_t(nTiles) * 256u + 1u);
+        K* src = pairs.p;
+        K* dst = alt.p;
+        for (uint32_t shift = 0; shift < 2u * sbits; shift += 8u) {
+            hipLaunchKernelGGL(k_digit_hist<K>, dim3(nTiles), dim3(256), 0, c->stream, src, n, shift, nTiles, hist.p);
+            launch_scan(c, hist.p, nTiles * 256u, start.p);
+            hipLaunchKernelGGL(k_digit_scatter<K>, dim3(nTiles), dim3(256), 0, c->stream, src, dst, n, shift, nTiles,
+                               start.p);
+            std::swap(src, dst);
+        }
+        head.ensure(n);
+        headIdx.ensure(size_t(n) + 1u);
+        const uint32_t g = (n + 255u) / 256u;
+        hipLaunchKernelGGL(k_run_heads<K>, dim3(g), dim3(256), 0, c->stream, src, n, head.p);
+        launch_scan(c, head.p, n, headIdx.p);
+        FH_HIP(hipMemcpyAsync(&nnz, headIdx.p + n, sizeof(nnz), hipMemcpyDeviceToHost, c->stream));
+        FH_HIP(hipStreamSynchronize(c->stream));
+        uniq.ensure(nnz);
+        pos.ensure(size_t(nnz) + 1u);
+        hipLaunchKernelGGL(k_run_starts<K>, dim3(g), dim3(256), 0, c->stream, src, n, headIdx.p, uniq.p, pos.p);
+        FH_HIP(hipGetLastError());
+        sortedKeys = uniq.p;
+    }
+    // CSR: rows and the columns within a row come out sorted
+    c->nnz = nnz;
+    alloc_csr_device(c, nnz);
+    if (nnz)
+        hipLaunchKernelGGL(k_csr_entries<K>, dim3((nnz + 255u) / 256u), dim3(256), 0, c->stream, sortedKeys, pos.p,
+                           nnz, sbits, c->col.p, c->cnt.p);
+    c->rowptr.ensure(size_t(S) + 1u);
+    hipLaunchKernelGGL(k_csr_rows<K>, dim3((S + 1u + 255u) / 256u), dim3(256), 0, c->stream, sortedKeys, nnz, sbits,
+                       S, c->rowptr.p);
+    FH_HIP(hipGetLastError());
     FH_HIP(hipStreamSynchronize(c->stream));
-    return o;
+    return 0;
 }
 
 }  // namespace
@@ -315,143 +605,48 @@ uint32_t launch_raster_graph(fh_context* c, const float* dirs, uint32_t ndirs, u
                              uint32_t layers) {
     check_raster(c, W, H, layers, nullptr);
     for (uint32_t d = 0; d < ndirs; ++d) check_raster(c, W, H, layers, dirs + 3u * d);
-    const uint32_t S = c->S;
-    DBuf<uint32_t> counts, ovf;
-    DBuf<uint2> frags;
-    DBuf<unsigned long long> pairs, sorted, uniq, accKeys, accKeys2, total;
-    DBuf<uint32_t> runs, accVals, accVals2, nRuns;
-    DBuf<uint8_t> temp;
-    c->den.ensure(S);
-    c->sky.ensure(S);
-    launch_fill_u32(c, c->den.p, S, 0u);
-    launch_fill_u32(c, c->sky.p, S, 0u);
-    total.ensure(1);
-    nRuns.ensure(1);
-    size_t accN = 0;
-    uint32_t overflow = 0;
-    const auto ensure_temp = [&](size_t bytes) {
-        if (bytes > temp.cap) temp.ensure(bytes);
-    };
-    for (uint32_t d = 0; d < ndirs; ++d) {
-        const RasterView rv = make_view(c, W, H, layers, dirs + 3u * d);
-        overflow += raster_pass(c, rv, counts, frags, ovf);
-        const size_t pixels = size_t(rv.W) * rv.H;
-        // exact capacity of this direction's pairs
-        FH_HIP(hipMemsetAsync(total.p, 0, sizeof(unsigned long long), c->stream));
-        hipLaunchKernelGGL(k_pair_bound, dim3(1024), dim3(256), 0, c->stream, counts.p, pixels, rv.L, total.p);
-        unsigned long long cap = 0;
-        FH_HIP(hipMemcpyAsync(&cap, total.p, sizeof(cap), hipMemcpyDeviceToHost, c->stream));
-        FH_HIP(hipStreamSynchronize(c->stream));
-        pairs.ensure(cap ? cap : 1);
-        FH_HIP(hipMemsetAsync(total.p, 0, sizeof(unsigned long long), c->stream));
-        hipLaunchKernelGGL(k_graph_resolve, dim3(uint32_t((pixels + 255u) / 256u)), dim3(256), 0, c->stream,
-                           counts.p, frags.p, pixels, rv.L, c->den.p, c->sky.p, pairs.p, total.p, cap);
-        FH_HIP(hipGetLastError());
-        unsigned long long np = 0;
-        FH_HIP(hipMemcpyAsync(&np, total.p, sizeof(np), hipMemcpyDeviceToHost, c->stream));
-        FH_HIP(hipStreamSynchronize(c->stream));
-        if (np == 0) continue;
-        if (np > 0x7fffffffull) fail(FH_OVERFLOW_ERROR, "raster graph: too many exchanges in one direction");
-        const int n = int(np);
-        // sort this direction's pairs and count each distinct pair
-        sorted.ensure(np);
-        size_t tb = 0;
-        FH_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, pairs.p, sorted.p, n, 0, 64, c->stream));
-        ensure_temp(tb);
-        FH_HIP(hipcub::DeviceRadixSort::SortKeys(temp.p, tb, pairs.p, sorted.p, n, 0, 64, c->stream));
-        uniq.ensure(np);
-        runs.ensure(np);
-        tb = 0;
-        FH_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb, sorted.p, uniq.p, runs.p, nRuns.p, n, c->stream));
-        ensure_temp(tb);
-        FH_HIP(hipcub::DeviceRunLengthEncode::Encode(temp.p, tb, sorted.p, uniq.p, runs.p, nRuns.p, n, c->stream));
-        uint32_t nu = 0;
-        FH_HIP(hipMemcpyAsync(&nu, nRuns.p, sizeof(nu), hipMemcpyDeviceToHost, c->stream));
-        FH_HIP(hipStreamSynchronize(c->stream));
-        // append to the running (pair, count) list
-        if (accN + nu > accKeys.cap) {
-            const size_t cap2 = std::max(accN + nu, 2 * accKeys.cap);
-            accKeys2.ensure(cap2);
-            accVals2.ensure(cap2);
-            if (accN) {
-                FH_HIP(hipMemcpyAsync(accKeys2.p, accKeys.p, accN * sizeof(unsigned long long),
-                                      hipMemcpyDeviceToDevice, c->stream));
-                FH_HIP(hipMemcpyAsync(accVals2.p, accVals.p, accN * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                                      c->stream));
-            }
-            accKeys.swap(accKeys2);
-            accVals.swap(accVals2);
-        }
-        FH_HIP(hipMemcpyAsync(accKeys.p + accN, uniq.p, nu * sizeof(unsigned long long), hipMemcpyDeviceToDevice,
-                              c->stream));
-        FH_HIP(hipMemcpyAsync(accVals.p + accN, runs.p, nu * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-        accN += nu;
-    }
-    counts.release();
-    frags.release();
-    pairs.release();
-    // merge the directions: sort by pair, sum the counts
-    uint32_t nnz = 0;
-    DBuf<unsigned long long> keysOut, keysFinal;
-    DBuf<uint32_t> valsOut, valsFinal;
-    if (accN) {
-        if (accN > 0x7fffffffull) fail(FH_OVERFLOW_ERROR, "raster graph: too many exchanges");
-        const int n = int(accN);
-        keysOut.ensure(accN);
-        valsOut.ensure(accN);
-        size_t tb = 0;
-        FH_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, accKeys.p, keysOut.p, accVals.p, valsOut.p, n, 0, 64,
-                                                  c->stream));
-        ensure_temp(tb);
-        FH_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, tb, accKeys.p, keysOut.p, accVals.p, valsOut.p, n, 0, 64,
-                                                  c->stream));
-        keysFinal.ensure(accN);
-        valsFinal.ensure(accN);
-        tb = 0;
-        FH_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, tb, keysOut.p, keysFinal.p, valsOut.p, valsFinal.p, nRuns.p,
-                                                 hipcub::Sum(), n, c->stream));
-        ensure_temp(tb);
-        FH_HIP(hipcub::DeviceReduce::ReduceByKey(temp.p, tb, keysOut.p, keysFinal.p, valsOut.p, valsFinal.p, nRuns.p,
-                                                 hipcub::Sum(), n, c->stream));
-        FH_HIP(hipMemcpyAsync(&nnz, nRuns.p, sizeof(nnz), hipMemcpyDeviceToHost, c->stream));
-        FH_HIP(hipStreamSynchronize(c->stream));
-    }
-    // CSR: rows from the pair keys (rows and the columns within a row come
-    // out sorted)
-    c->nnz = nnz;
-    alloc_csr_device(c, nnz);
-    c->rowlen.ensure(S);
-    launch_fill_u32(c, c->rowlen.p, S, 0u);
-    if (nnz)
-        hipLaunchKernelGGL(k_split_keys, dim3((nnz + 255u) / 256u), dim3(256), 0, c->stream, keysFinal.p,
-                           valsFinal.p, nnz, c->rowlen.p, c->col.p, c->cnt.p);
-    FH_HIP(hipGetLastError());
-    c->rowptr.ensure(size_t(S) + 1u);
-    launch_scan(c, c->rowlen.p, S, c->rowptr.p);
-    FH_HIP(hipStreamSynchronize(c->stream));
-    return overflow;
+    c->nnz = 0;
+    const uint32_t sbits = side_bits(c->S);
+    if (2u * sbits <= 32u) return raster_graph<uint32_t>(c, dirs, ndirs, W, H, layers, sbits);
+    return raster_graph<unsigned long long>(c, dirs, ndirs, W, H, layers, sbits);
 }
 
+// one direction: the fragment total is read back before the write pass, so
+// the lists get exactly their size
 uint32_t launch_raster_sunlit(fh_context* c, const float sun[3], uint32_t W, uint32_t H, uint32_t layers,
                               float* frac) {
     check_raster(c, W, H, layers, sun);
     const uint32_t S = c->S;
-    DBuf<uint32_t> counts, ovf, total, visible;
-    DBuf<uint2> frags;
     const RasterView rv = make_view(c, W, H, layers, sun);
-    const uint32_t overflow = raster_pass(c, rv, counts, frags, ovf);
+    const size_t pixels = size_t(rv.W) * rv.H;
+    DBuf<uint32_t> counts, off, ovf, total, visible;
+    DBuf<uint2> frags;
+    counts.ensure(pixels);
+    off.ensure(pixels + 1u);
+    ovf.ensure(2);  // overflow, pass mismatch
     total.ensure(S);
     visible.ensure(S);
+    launch_fill_u32(c, counts.p, pixels, 0u);
+    launch_raster<Pass::Count>(c, rv, counts.p, nullptr, nullptr, 0, nullptr, nullptr);
+    launch_scan(c, counts.p, uint32_t(pixels), off.p, layers);
+    uint32_t nfrag = 0;
+    FH_HIP(hipMemcpyAsync(&nfrag, off.p + pixels, sizeof(nfrag), hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipStreamSynchronize(c->stream));
+    frags.ensure(std::max<uint32_t>(nfrag, 1u));
     launch_fill_u32(c, total.p, S, 0u);
     launch_fill_u32(c, visible.p, S, 0u);
-    const size_t pixels = size_t(rv.W) * rv.H;
+    launch_fill_u32(c, ovf.p, 2, 0u);
+    launch_raster<Pass::Sun>(c, rv, counts.p, off.p, frags.p, nfrag, total.p, ovf.p);
     hipLaunchKernelGGL(k_sun_resolve, dim3(uint32_t((pixels + 255u) / 256u)), dim3(256), 0, c->stream, counts.p,
-                       frags.p, pixels, rv.L, total.p, visible.p);
+                       off.p, frags.p, pixels, visible.p, ovf.p + 1);
     hipLaunchKernelGGL(k_sun_fraction_raster, dim3((S + 255u) / 256u), dim3(256), 0, c->stream, total.p, visible.p,
                        S, frac);
     FH_HIP(hipGetLastError());
+    uint32_t res[2] = {0u, 0u};
+    FH_HIP(hipMemcpyAsync(res, ovf.p, sizeof(res), hipMemcpyDeviceToHost, c->stream));
     FH_HIP(hipStreamSynchronize(c->stream));
-    return overflow;
+    if (res[1]) fail(FH_RUNTIME_ERROR, "raster sunlit: the count and write passes covered different pixels");
+    return res[0];
 }
 
 }  // namespace fh

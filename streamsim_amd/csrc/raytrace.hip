@@ -526,12 +526,12 @@ __global__ __launch_bounds__(256) void k_rows(uint32_t* __restrict__ hits, uint1
     }
 }
 
-// ---- exclusive scan of S row lengths into S+1 offsets ----------------------
-__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict__ in, uint32_t n,
+// ---- exclusive scan of n lengths (each clamped to clampMax) into n+1 offsets
+__global__ __launch_bounds__(1024) void k_scan_blocks(const uint32_t* __restrict__ in, uint32_t n, uint32_t clampMax,
                                                       uint32_t* __restrict__ out, uint32_t* __restrict__ blockSums) {
     __shared__ uint32_t s[1024];
     const uint32_t i = blockIdx.x * 1024u + threadIdx.x;
-    const uint32_t x = i < n ? in[i] : 0u;
+    const uint32_t x = i < n ? min(in[i], clampMax) : 0u;
     s[threadIdx.x] = x;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -654,10 +654,11 @@ void launch_rows(fh_context* c, uint32_t R) {
     FH_HIP(hipGetLastError());
 }
 
-void launch_scan(fh_context* c, const uint32_t* in, uint32_t n, uint32_t* out) {
+void launch_scan(fh_context* c, const uint32_t* in, uint32_t n, uint32_t* out, uint32_t clampMax) {
     const uint32_t nblocks = (n + 1023u) / 1024u;
     c->scanTmp.ensure(size_t(nblocks) + 1u);
-    hipLaunchKernelGGL(k_scan_blocks, dim3(nblocks ? nblocks : 1), dim3(1024), 0, c->stream, in, n, out, c->scanTmp.p);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nblocks ? nblocks : 1), dim3(1024), 0, c->stream, in, n, clampMax, out,
+                       c->scanTmp.p);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, c->stream, c->scanTmp.p, nblocks, c->scanTmp.p + nblocks);
     hipLaunchKernelGGL(k_scan_add, dim3((n + 1u + 255u) / 256u), dim3(256), 0, c->stream, out, n, c->scanTmp.p,
                        c->scanTmp.p + nblocks);

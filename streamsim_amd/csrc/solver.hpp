@@ -11,7 +11,8 @@ struct SweepArgs;
 // raytrace.hip
 void launch_trace_graph(fh_context* c, uint32_t R, uint32_t seed, float tmin);
 void launch_rows(fh_context* c, uint32_t R);
-void launch_scan(fh_context* c, const uint32_t* in, uint32_t n, uint32_t* out);
+// out[0..n] = exclusive scan of min(in[i], clampMax); out[n] = the total
+void launch_scan(fh_context* c, const uint32_t* in, uint32_t n, uint32_t* out, uint32_t clampMax = 0xffffffffu);
 void launch_compact(fh_context* c, uint32_t R);
 // weights cnt / den, fsky and the closure; packed entries when packable
 // (every count <= its row's den <= 65535) and column and count (<= maxCount)

@@ -140,6 +140,32 @@ def test_raster_graph_and_sunlit_bitexact(oracle, neighbors, size):
         assert np.array_equal(f, of), (zen, az)
 
 
+def test_raster_graph_wide_keys_and_regrown_buffers(oracle):
+    # 38 000 facets: 76 000 sides need 17 bits, so the directed pairs are
+    # sorted as 64-bit keys.  The first build (128 pixels across) leaves
+    # fragment and pair capacities sized for itself; the second (384 across)
+    # outgrows both, is measured and runs again at the measured sizes.  Both
+    # graphs equal the oracle's.
+    print("raster graph wide keys case", flush=True)
+    from streamsim_amd import FacetEngine
+    sc = scenes.row_crop(rows=4, leaves_per_row=4500, cells=100)
+    assert 2 * sc.facet_count > 65536
+    eng = FacetEngine(rays_per_side=32, sun_rays_per_side=16, seed=1)
+    eng.setGeometry(sc.vertices)
+    dirs = oracle.hemisphere_directions(6)
+    lo, hi = _vertex_bounds(sc)
+    for size in (128, 384):
+        d = eng.buildRasterGraph(dirs, size, size, 64)
+        g = eng.getGraph()
+        og = oracle.raster_graph(sc.vertices, lo, hi, raster=size, layers=64, dirs=dirs)
+        assert d.fragmentOverflow == 0 and d.directedEdgeCount == og.nnz > 0, size
+        for k in ("rowOffsets", "neighbors", "counts", "denominators", "skyCounts"):
+            assert np.array_equal(getattr(g, k), getattr(og, k)), (size, k)
+    sun = oracle.sun_direction(40.0, 200.0)
+    f = eng.computeRasterSunlitFraction(sun, 384, 384, 64)
+    assert np.array_equal(f, oracle.raster_sunlit(sc.vertices, lo, hi, sun, raster=384, layers=64))
+
+
 def test_raster_graph_overflow_reported(oracle):
     # two fragments allowed per pixel: the leaves over the ground overflow,
     # the call fails and leaves no graph

@@ -362,6 +362,41 @@ def test_dem_library_grids(tmp_path):
         assert np.ptp(pts[:, 1]) > 1.0
 
 
+def test_dem_of_the_validation_project(tmp_path):
+    # the reference's urban_dem_building terrain (dem_test_input.tif: 32 x 24
+    # big-endian float32 strips) against its stored FacetRT run.  That run
+    # predates the current ground rule: its ground is the DEM's own pixel grid
+    # (31 x 23 cells over the 50 x 40 m scene, columns along x) at heights
+    # above the DEM mean, where the current source resamples the DEM to the
+    # scene's voxel grid at heights above the minimum, rows along x from the
+    # north (core:2630-2650, 2844-2850).  On a 23 x 31 m scene at 1 m voxels
+    # the scene grid's vertices fall on the DEM's pixels, so the ground read
+    # here must carry the stored run's pixel values: ours + 1 cm equals the
+    # stored height minus the stored minimum, at the pixel the two
+    # orientations share
+    _need_exe()
+    stored = np.load(GOLDEN / "dem" / "urban_dem_building_ground.npy")  # [z 24][x 32]
+    d = tmp_path / "pixels"
+    d.mkdir()
+    p = _write_project(d, [], scene={"terrain": True, "demFile": "dem.tif", "x": 23, "y": 31, "voxel": 1})
+    shutil.copy(GOLDEN / "dem" / "dem_test_input.tif", p.parent / "dem.tif")
+    tris, _ = _assemble(p, d)
+    pts = np.array([t for t, g in tris if g == "STREAMSIM_TERRAIN"]).reshape(-1, 3)
+    assert len(pts) == 23 * 31 * 6
+    row = np.rint(11.5 - pts[:, 0]).astype(int)  # DEM row from the north edge (x = 11.5)
+    col = np.rint(pts[:, 2] + 15.5).astype(int)  # DEM column from the west edge (z = -15.5)
+    assert np.allclose(pts[:, 0], 11.5 - row, atol=1e-5) and np.allclose(pts[:, 2], col - 15.5, atol=1e-5)
+    np.testing.assert_allclose(pts[:, 1] + 0.01, stored[row, col] - stored.min(), rtol=0, atol=2e-5)
+    # the validation project's own 50 x 40 m scene: 50 x 40 ground cells by
+    # the current rule (the stored run has 31 x 23)
+    d = tmp_path / "scene"
+    d.mkdir()
+    p = _write_project(d, [], scene={"terrain": True, "demFile": "dem.tif", "x": 50, "y": 40, "voxel": 1})
+    shutil.copy(GOLDEN / "dem" / "dem_test_input.tif", p.parent / "dem.tif")
+    tris, _ = _assemble(p, d)
+    assert sum(g == "STREAMSIM_TERRAIN" for _, g in tris) == 50 * 40 * 2
+
+
 def test_large_scene_raster_and_adaptive_layers(tmp_path):
     # scenes of 1000 m or more rasterise at 2048 pixels and 16 layers, and
     # their ground grid at up to 512 cells a side (core:2895-2901,
