@@ -501,7 +501,7 @@ struct CoupledArgs {
 // fsky L + G per side in Iin.
 template <int TEAM, bool W16>
 __global__ __launch_bounds__(256) void k_coupled_gather(CoupledArgs a) {
-    if (*reinterpret_cast<volatile uint32_t*>(&a.ctrl->done)) return;
+    if (ctrl_word(&a.ctrl->done)) return;
     const int lane = threadIdx.x % TEAM;
     const uint32_t fl = (blockIdx.x * 256u + threadIdx.x) / TEAM;
     if (fl >= a.F) return;
@@ -529,7 +529,7 @@ __global__ __launch_bounds__(256) void k_coupled_gather(CoupledArgs a) {
 // B' = e(T') + r in_s + t in_o; per-block maxima of |dB|, |dB|/|B|, dT and
 // the residual-equivalent.
 __global__ __launch_bounds__(256) void k_coupled_update(CoupledArgs a) {
-    if (*reinterpret_cast<volatile uint32_t*>(&a.ctrl->done)) return;
+    if (ctrl_word(&a.ctrl->done)) return;
     const uint32_t f = blockIdx.x * 256u + threadIdx.x;
     float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};  // |dB|, |dB|/|B|, dT, residual-equivalent
     if (f < a.F) {

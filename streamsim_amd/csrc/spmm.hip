@@ -135,13 +135,13 @@ template <int NBC, int TEAM, int MODE, bool W16>
 __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     SolveCtrl* ctrl = a.ctrl;
     if (MODE != MODE_INCIDENT && !a.standalone) {
-        if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return;
-        if (MODE == MODE_SWEEP && W16 && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return;
+        if (ctrl_word(&ctrl->done)) return;
+        if (MODE == MODE_SWEEP && W16 && ctrl_word(&ctrl->innerDone)) return;
     }
     const float* __restrict__ Bin = a.Bin;
     float* __restrict__ Bout = a.Bout;
     if (MODE != MODE_INCIDENT && !a.standalone && a.buf0) {
-        const uint32_t it = *reinterpret_cast<volatile uint32_t*>(&ctrl->iter);
+        const uint32_t it = ctrl_word(&ctrl->iter);
         Bin = (it & 1u) ? a.buf1 : a.buf0;
         Bout = (it & 1u) ? a.buf0 : a.buf1;
     }
@@ -344,15 +344,15 @@ __device__ __forceinline__ bool band_ctx(const SweepArgs& a, bool inner, BandCtx
     x.Bin8 = a.q0;
     x.Bout8 = a.q1;
     if (MODE == MODE_INCIDENT || a.standalone) return true;
-    if (*reinterpret_cast<volatile uint32_t*>(&ctrl->done)) return false;
-    if (inner && *reinterpret_cast<volatile uint32_t*>(&ctrl->innerDone)) return false;
+    if (ctrl_word(&ctrl->done)) return false;
+    if (inner && ctrl_word(&ctrl->innerDone)) return false;
     if (a.retire) {
-        x.nAct = *reinterpret_cast<volatile uint32_t*>(&ctrl->nAct);
-        x.nPrev = *reinterpret_cast<volatile uint32_t*>(&ctrl->nActPrev);
+        x.nAct = ctrl_word(&ctrl->nAct);
+        x.nPrev = ctrl_word(&ctrl->nActPrev);
         x.tolRel = ctrl->tolRel;
     }
     if (a.buf0) {
-        const uint32_t it = *reinterpret_cast<volatile uint32_t*>(&ctrl->iter);
+        const uint32_t it = ctrl_word(&ctrl->iter);
         x.Bin = (it & 1u) ? a.buf1 : a.buf0;
         x.Bout = (it & 1u) ? a.buf0 : a.buf1;
         x.Bin16 = a.h0 ? ((it & 1u) ? a.h1 : a.h0) : nullptr;
@@ -1191,6 +1191,452 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
     band_partials(a, myAbs, myRel, INC == 0 ? 0.0f : myAct);
 }
 
+// ---- Staged gathers: the increment sweeps of the single facets ------------
+//
+// A workgroup owns kStageFacets consecutive facets of the sweep order (one
+// team of bt lanes, 8 BF16 bands per lane, per facet) and gathers the
+// distinct columns of their kStageRows rows once per sweep through the LDS.
+// The block's distinct columns, in column order, are staged kStageSlots at a
+// time (a chunk).  A row's entries in one chunk form one segment (slots grow
+// along a row).  Per chunk every team sums two whole segments: the rows are
+// ranked by segment length and team t takes ranks t and kStageRows - 1 - t
+// (long with short), and a block-level copy of the CSR holds, per chunk and
+// team, the two segments back to back, each padded to a multiple of 4
+// entries with zero weights, as the slot in the chunk (2 bytes) and the FP32
+// weight; a run descriptor per chunk and team gives the first entry, the
+// groups of 4 of each segment and the two local rows.  The groups need no
+// masks and a team adds each segment's sum to its row's FP32 sums in the
+// LDS once (a row belongs to one team per chunk: no atomics, and the sums
+// do not depend on scheduling).  On the 468 M-entry terrain graph a block of
+// 128 facets references about 15 k distinct columns for 60 k entries: the
+// lines the sweep requests from L2 drop from 6.2e8 to 2.2e8, and the
+// entries read the LDS instead of the vector L1.
+constexpr uint32_t kStageFacets = 128u;                       // facets (teams) per workgroup
+constexpr uint32_t kStageRows = 2u * kStageFacets;            // local rows
+constexpr uint32_t kStageSlots = 1024u;                       // staged columns per chunk
+constexpr uint32_t kStagePer = kStageSlots / kStageFacets;    // staged columns per team and chunk
+constexpr uint32_t kStageMaxChunks = 32u;                     // chunks per block (build limit)
+constexpr uint32_t kStageNone = 0xffffffffu;
+constexpr uint32_t kStageMaxBands = 48u;  // bt <= 6: the LDS of stage_lds_bytes fits the CU's 160 KB
+
+struct StageArgs {
+    const uint32_t* dcol;   // distinct columns of block b, sorted, from dOff[b]
+    const uint32_t* dOff;   // [blocks + 1]
+    const uint32_t* mOff;   // [blocks] run descriptor of (chunk j, team t) of block b at mOff[b] + 64 j + t
+    const uint2* meta;      // x: first entry; y: groups of row A | groups of row B << 8 | A << 16 | B << 24
+    const uint2* key;       // padded entries, 4 per vector: slot in chunk (16 bits each)
+    const float4* w;        // their weights, 4 per vector (0 on padding)
+};
+
+// dynamic LDS of k_sweep_staged: staged BF16 rows and the rows' FP32 sums
+__host__ __device__ constexpr uint32_t stage_lds_bytes(uint32_t bt) {
+    return kStageSlots * bt * 16u + kStageRows * bt * 32u;
+}
+
+// max-reduce n floats over the blockDim.x / 64 waves of a block (at most 16)
+template <int N>
+__device__ __forceinline__ void block_max_waves(float (&v)[N], float (&out)[N]) {
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) v[q] = fmaxf(v[q], __shfl_xor(v[q], m, 64));
+    __shared__ float sh[16][N];
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < N; ++q) sh[threadIdx.x >> 6][q] = v[q];
+    __syncthreads();
+    const uint32_t nw = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        float r = sh[0][q];
+        for (uint32_t w = 1; w < nw; ++w) r = fmaxf(r, sh[w][q]);
+        out[q] = r;
+    }
+}
+
+// s += w * (8 BF16 values of v)
+__device__ __forceinline__ void fma_bf16x8(float (&s)[8], float w, const uint4 v) {
+    const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        s[2 * j] += w * __uint_as_float(vv[j] << 16);
+        s[2 * j + 1] += w * __uint_as_float(vv[j] & 0xffff0000u);
+    }
+}
+
+template <int INC>
+__global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st) {
+    extern __shared__ uint4 stageLds[];
+    BandCtx x;
+    if (!band_ctx<MODE_SWEEP>(a, false, x)) return;
+    const uint32_t bt = a.bt, nb8 = 8u * bt;
+    const uint32_t tid = threadIdx.x, team = tid / bt, lane = tid - team * bt;
+    const uint32_t blk = xcd_block(a.xcd);
+    uint4* rows = stageLds;                                               // [kStageSlots][bt]
+    float* acc = reinterpret_cast<float*>(rows + kStageSlots * bt);       // [kStageRows][nb8]
+    const uint32_t b0 = 8u * lane;
+    const bool act = b0 < x.nAct;
+    for (uint32_t i = tid; i < kStageRows * nb8 / 4u; i += blockDim.x)
+        reinterpret_cast<float4*>(acc)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t d0 = st.dOff[blk], U = st.dOff[blk + 1u] - d0;
+    const uint32_t nCh = (U + kStageSlots - 1u) / kStageSlots;
+    const uint32_t* dc = st.dcol + d0;
+    const uint2* meta = st.meta + st.mOff[blk];
+    const uint32_t rowBytes = a.ldh * 2u;
+    const __amdgpu_buffer_rsrc_t src = make_rsrc(x.Bin16, a.sides * rowBytes);
+    // staging: this thread moves 16 bytes (its lane's 8 bands) of the slots
+    // team + 64 k of a chunk; the columns are read one chunk ahead
+    uint32_t dq[kStagePer];
+    uint4 rv[kStagePer];
+    const auto load_dcol = [&](uint32_t j) {
+#pragma unroll
+        for (uint32_t k = 0; k < kStagePer; ++k) {
+            const uint32_t s = j * kStageSlots + team + kStageFacets * k;
+            dq[k] = s < U ? dc[s] : kStageNone;
+        }
+    };
+    const auto load_rows = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < kStagePer; ++k) {
+            const uint32_t off = dq[k] != kStageNone ? dq[k] * rowBytes + lane * 16u : kNoRow;
+            rv[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(src, off, 0, 0));
+        }
+    };
+    const auto store_rows = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < kStagePer; ++k) rows[(team + kStageFacets * k) * bt + lane] = rv[k];
+    };
+    if (nCh) {
+        load_dcol(0);
+        load_rows();
+        if (nCh > 1u) load_dcol(1);
+        store_rows();
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < nCh; ++j) {
+        const uint2 m = meta[j * kStageFacets + team];
+        const uint32_t qa = m.y & 255u, qn = qa + ((m.y >> 8) & 255u);
+        const uint32_t rA = (m.y >> 16) & 255u, rB = m.y >> 24;
+        const uint32_t g0 = m.x >> 2;  // first group of 4 entries
+        float s[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[q] = 0.0f;
+        // the slots and weights of the next two groups are loaded ahead
+        uint2 k1 = make_uint2(0u, 0u), k2 = k1;
+        float4 w1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w2 = w1;
+        if (qn) {
+            k1 = st.key[g0];
+            w1 = st.w[g0];
+        }
+        if (qn > 1u) {
+            k2 = st.key[g0 + 1u];
+            w2 = st.w[g0 + 1u];
+        }
+        for (uint32_t q = 0; q < qn; ++q) {
+            const uint2 k = k1;
+            const float4 w = w1;
+            k1 = k2;
+            w1 = w2;
+            if (q + 2u < qn) {
+                k2 = st.key[g0 + q + 2u];
+                w2 = st.w[g0 + q + 2u];
+            }
+            if (act) {
+                fma_bf16x8(s, w.x, rows[__umul24(k.x & 0xffffu, bt) + lane]);
+                fma_bf16x8(s, w.y, rows[__umul24(k.x >> 16, bt) + lane]);
+                fma_bf16x8(s, w.z, rows[__umul24(k.y & 0xffffu, bt) + lane]);
+                fma_bf16x8(s, w.w, rows[__umul24(k.y >> 16, bt) + lane]);
+            }
+            if (q + 1u == qa || q + 1u == qn) {  // a segment ends: add it to its row
+                const uint32_t r = q + 1u == qa ? rA : rB;
+                if (act) {
+                    float4* p = reinterpret_cast<float4*>(acc + __umul24(r, nb8) + b0);
+                    const float4 u0 = p[0], u1 = p[1];
+                    p[0] = make_float4(u0.x + s[0], u0.y + s[1], u0.z + s[2], u0.w + s[3]);
+                    p[1] = make_float4(u1.x + s[4], u1.y + s[5], u1.z + s[6], u1.w + s[7]);
+                }
+#pragma unroll
+                for (int t = 0; t < 8; ++t) s[t] = 0.0f;
+            }
+        }
+        // the next chunk's rows: loaded after this chunk's entries, which
+        // would otherwise wait behind them
+        if (j + 1u < nCh) {
+            load_rows();
+            if (j + 2u < nCh) load_dcol(j + 2u);
+        }
+        __syncthreads();
+        if (j + 1u < nCh) store_rows();
+        __syncthreads();
+    }
+    float myAbs = 0.0f, myRel = 0.0f, myAct = 0.0f;
+    const uint32_t fl = blk * kStageFacets + team;
+    if (fl < a.F) {
+        const uint32_t f = a.sched ? a.sched[fl] : fl;
+        const uint32_t s0 = 2u * f;
+        const float4* p0 = reinterpret_cast<const float4*>(acc + (2u * team) * nb8 + b0);
+        const float4* p1 = reinterpret_cast<const float4*>(acc + (2u * team + 1u) * nb8 + b0);
+        const float4 a0 = p0[0], a1 = p0[1], c0 = p1[0], c1 = p1[1];
+        float lo0[4] = {a0.x, a0.y, a0.z, a0.w}, hi0[4] = {a1.x, a1.y, a1.z, a1.w};
+        float lo1[4] = {c0.x, c0.y, c0.z, c0.w}, hi1[4] = {c1.x, c1.y, c1.z, c1.w};
+        band_update_inc4<INC == 1>(a, x, s0, b0, lo0, lo1, myAbs, myRel, myAct);
+        if (b0 + 4u < a.ldb) band_update_inc4<INC == 1>(a, x, s0, b0 + 4u, hi0, hi1, myAbs, myRel, myAct);
+    }
+    float v[3] = {myAbs, myRel, myAct}, out[3];
+    block_max_waves<3>(v, out);
+    if (threadIdx.x == 0) {
+        a.partial[a.partOff + blockIdx.x] = make_float2(out[0], out[1]);
+        if (a.partAct) a.partAct[a.partOff + blockIdx.x] = uint32_t(out[2]);
+    }
+}
+
+// ---- building the staged layout ------------------------------------------
+// Blocks of kStageFacets facets of `order` (null: facet i); local row lr =
+// 2 t + side of the block's t-th facet.  Built per batch of blocks (one
+// workgroup each) with a bitmap over the sides per block: the bits of the
+// block's columns, the word ranks and the sorted distinct columns, then the
+// segments.  Two passes over the batches: the first sizes every block
+// (distinct columns U, padded entries P, chunks), the second writes.
+
+// side of local row lr of block b, or kStageNone past the facets
+__device__ __forceinline__ uint32_t stage_side(const uint32_t* order, uint32_t nF, uint32_t b, uint32_t lr) {
+    const uint32_t i = b * kStageFacets + (lr >> 1);
+    if (i >= nF) return kStageNone;
+    return 2u * (order ? order[i] : i) + (lr & 1u);
+}
+
+__global__ __launch_bounds__(1024) void k_stage_bits(const uint32_t* __restrict__ rowptr,
+                                                     const uint32_t* __restrict__ col,
+                                                     const uint32_t* __restrict__ order, uint32_t nF, uint32_t nBlk,
+                                                     uint32_t b0, uint32_t words, uint32_t* __restrict__ bitmap) {
+    const uint32_t b = b0 + blockIdx.x;
+    if (b >= nBlk) return;
+    uint32_t* bm = bitmap + size_t(blockIdx.x) * words;
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    for (uint32_t lr = wave; lr < kStageRows; lr += 16u) {
+        const uint32_t s = stage_side(order, nF, b, lr);
+        if (s == kStageNone) continue;
+        for (uint32_t e = rowptr[s] + lane; e < rowptr[s + 1u]; e += 64u) {
+            const uint32_t c = col[e];
+            atomicOr(bm + (c >> 5), 1u << (c & 31u));
+        }
+    }
+}
+
+// word ranks (set bits before the word), U[b], and (dOff set) the sorted
+// distinct columns at dOff[b]
+__global__ __launch_bounds__(1024) void k_stage_rank(const uint32_t* __restrict__ bitmap, uint32_t nBlk, uint32_t b0,
+                                                     uint32_t words, uint32_t* __restrict__ rank,
+                                                     uint32_t* __restrict__ U, const uint32_t* __restrict__ dOff,
+                                                     uint32_t* __restrict__ dcol) {
+    const uint32_t b = b0 + blockIdx.x;
+    if (b >= nBlk) return;
+    const uint32_t* bm = bitmap + size_t(blockIdx.x) * words;
+    uint32_t* rk = rank + size_t(blockIdx.x) * words;
+    const uint32_t tid = threadIdx.x, per = (words + 1023u) / 1024u;
+    const uint32_t w0 = min(words, tid * per), w1 = min(words, w0 + per);
+    uint32_t n = 0;
+    for (uint32_t w = w0; w < w1; ++w) n += __popc(bm[w]);
+    __shared__ uint32_t sc[1024];
+    sc[tid] = n;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024u; off <<= 1) {
+        const uint32_t v = tid >= off ? sc[tid - off] : 0u;
+        __syncthreads();
+        sc[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = sc[tid] - n;
+    if (tid == 1023u) U[b] = sc[1023];
+    uint32_t* dc = dOff ? dcol + dOff[b] : nullptr;
+    for (uint32_t w = w0; w < w1; ++w) {
+        rk[w] = run;
+        uint32_t m = bm[w];
+        if (dc) {
+            while (m) {
+                dc[run++] = 32u * w + uint32_t(__ffs(m) - 1);
+                m &= m - 1u;
+            }
+        } else {
+            run += __popc(m);
+        }
+    }
+}
+
+// slot of column c in its block (rank of c among the block's columns)
+__device__ __forceinline__ uint32_t stage_slot(const uint32_t* bm, const uint32_t* rk, uint32_t c) {
+    return rk[c >> 5] + uint32_t(__popc(bm[c >> 5] & ((1u << (c & 31u)) - 1u)));
+}
+
+// Per row (one wave): the segments along the row.  fn(e, j, slot,
+// firstOfSegment, lastOfSegment) for every entry e of the row.
+template <class Fn>
+__device__ __forceinline__ void stage_row_runs(const uint32_t* __restrict__ col, const uint32_t* bm,
+                                               const uint32_t* rk, uint32_t e0, uint32_t e1, Fn&& fn) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t runJ = kStageNone, runStart = e0;
+    for (uint32_t base = e0; base < e1; base += 64u) {
+        const uint32_t e = base + lane;
+        const bool valid = e < e1;
+        const uint32_t slot = valid ? stage_slot(bm, rk, col[e]) : 0u;
+        const uint32_t j = valid ? slot / kStageSlots : kStageNone;
+        uint32_t prevJ = __shfl_up(j, 1u, 64);
+        if (lane == 0u) prevJ = runJ;
+        uint32_t nextJ = __shfl_down(j, 1u, 64);
+        if (lane == 63u) nextJ = e + 1u < e1 ? stage_slot(bm, rk, col[e + 1u]) / kStageSlots : kStageNone;
+        const bool start = valid && j != prevJ;
+        const unsigned long long starts = __ballot(start);
+        const unsigned long long upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
+        const unsigned long long below = starts & upto;
+        const uint32_t first = below ? base + uint32_t(63 - __clzll(below)) : runStart;
+        if (valid) fn(e, j, slot, first, j != nextJ);
+        const uint32_t lastLane = min(63u, e1 - 1u - base);
+        runJ = __shfl(j, int(lastLane), 64);
+        runStart = __shfl(first, int(lastLane), 64);
+    }
+}
+
+__host__ __device__ constexpr uint32_t stage_fill_lds_bytes() {
+    return 2u * kStageRows * kStageMaxChunks * 2u + 16u * kStageRows + 2u * (kStageMaxChunks + 1u) * 4u;
+}
+
+// Pass 1 (write false): U, P (padded entries) and chunks of the block.
+// Pass 2 (write true): the run descriptors and the padded entries at
+// eOff[b].  Segment lengths per (local row, chunk) in the LDS.
+template <bool WRITE>
+__global__ __launch_bounds__(1024) void k_stage_fill(const uint32_t* __restrict__ rowptr,
+                                                     const uint32_t* __restrict__ col, const float* __restrict__ w32,
+                                                     const uint32_t* __restrict__ order, uint32_t nF, uint32_t nBlk,
+                                                     uint32_t b0, uint32_t words, const uint32_t* __restrict__ bitmap,
+                                                     const uint32_t* __restrict__ rank, const uint32_t* __restrict__ U,
+                                                     uint32_t* __restrict__ P, const uint32_t* __restrict__ eOff,
+                                                     const uint32_t* __restrict__ mOff, uint2* __restrict__ meta,
+                                                     uint16_t* __restrict__ key, float* __restrict__ wout,
+                                                     uint32_t* __restrict__ fault) {
+    extern __shared__ uint32_t fillLds[];
+    const uint32_t b = b0 + blockIdx.x;
+    if (b >= nBlk) return;
+    const uint32_t* bm = bitmap + size_t(blockIdx.x) * words;
+    const uint32_t* rk = rank + size_t(blockIdx.x) * words;
+    const uint32_t nCh = (U[b] + kStageSlots - 1u) / kStageSlots;
+    if (nCh > kStageMaxChunks) {
+        if (threadIdx.x == 0) atomicOr(fault, 1u);
+        return;
+    }
+    uint16_t* len = reinterpret_cast<uint16_t*>(fillLds);   // [kStageRows][kStageMaxChunks] segment lengths
+    uint16_t* pos = len + kStageRows * kStageMaxChunks;     // segment place in its chunk
+    uint8_t* sorted = reinterpret_cast<uint8_t*>(pos + kStageRows * kStageMaxChunks);  // [16 waves][kStageRows]
+    uint32_t* cT = reinterpret_cast<uint32_t*>(sorted + 16u * kStageRows);           // chunk totals, then starts
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63u;
+    for (uint32_t i = tid; i < kStageRows * kStageMaxChunks; i += 1024u) len[i] = 0u;
+    __syncthreads();
+    for (uint32_t lr = wave; lr < kStageRows; lr += 16u) {
+        const uint32_t s = stage_side(order, nF, b, lr);
+        if (s == kStageNone) continue;
+        stage_row_runs(col, bm, rk, rowptr[s], rowptr[s + 1u],
+                       [&](uint32_t e, uint32_t j, uint32_t, uint32_t first, bool last) {
+                           if (last) len[lr * kStageMaxChunks + j] = uint16_t(e + 1u - first);
+                       });
+    }
+    __syncthreads();
+    const auto padded = [](uint32_t n) { return (n + 3u) & ~3u; };
+    if (!WRITE) {
+        uint32_t t = 0;
+        for (uint32_t i = tid; i < kStageRows * nCh; i += 1024u) t += padded(len[(i / nCh) * kStageMaxChunks + i % nCh]);
+        for (int m = 32; m > 0; m >>= 1) t += __shfl_xor(t, m, 64);
+        __shared__ uint32_t red[16];
+        if (lane == 0u) red[wave] = t;
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t sum = 0;
+            for (uint32_t w = 0; w < 16u; ++w) sum += red[w];
+            P[b] = sum;
+        }
+        return;
+    }
+    // per chunk (a wave each): rows ranked by padded length (longest first,
+    // then by local row), team t takes ranks t and kStageRows - 1 - t;
+    // each lane ranks kStageRows / 64 rows and pairs kStageFacets / 64 teams
+    constexpr uint32_t kRowsPerLane = kStageRows / 64u, kTeamsPerLane = kStageFacets / 64u;
+    constexpr uint32_t kChunksPerWave = (kStageMaxChunks + 15u) / 16u;
+    uint2 mine[kChunksPerWave][kTeamsPerLane];
+    for (uint32_t k = 0, j = wave; k < kChunksPerWave; ++k, j += 16u) {
+        if (j >= nCh) break;
+        uint8_t* srt = sorted + wave * kStageRows;
+        for (uint32_t h = 0; h < kRowsPerLane; ++h) {
+            const uint32_t lr = lane + 64u * h;
+            const uint32_t kl = (padded(len[lr * kStageMaxChunks + j]) << 8) | (kStageRows - 1u - lr);
+            uint32_t r = 0;
+            for (uint32_t o = 0; o < kStageRows; ++o)
+                r += ((padded(len[o * kStageMaxChunks + j]) << 8) | (kStageRows - 1u - o)) > kl ? 1u : 0u;
+            srt[r] = uint8_t(lr);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t base = 0;  // entries of the teams of the earlier lane passes
+        for (uint32_t h = 0; h < kTeamsPerLane; ++h) {
+            const uint32_t t = lane + 64u * h;
+            const uint32_t A = srt[t], B = srt[kStageRows - 1u - t];
+            const uint32_t pA = padded(len[A * kStageMaxChunks + j]), pB = padded(len[B * kStageMaxChunks + j]);
+            if (pA > 1020u || pB > 1020u) atomicOr(fault, 16u);  // 8-bit group counts
+            uint32_t inc = pA + pB;
+            for (uint32_t o = 1; o < 64u; o <<= 1) {
+                const uint32_t v = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += v;
+            }
+            const uint32_t start = base + inc - pA - pB;
+            pos[A * kStageMaxChunks + j] = uint16_t(start);
+            pos[B * kStageMaxChunks + j] = uint16_t(start + pA);
+            mine[k][h] = make_uint2(start, (pA >> 2) | ((pB >> 2) << 8) | (A << 16) | (B << 24));
+            base += __shfl(inc, 63, 64);
+        }
+        if (lane == 63u) {
+            cT[j] = base;
+            if (base > 0xffffu) atomicOr(fault, 8u);  // segment places are 16-bit
+        }
+        __builtin_amdgcn_wave_barrier();  // the next chunk's ranks overwrite this wave's list
+    }
+    __syncthreads();
+    uint32_t* cst = cT + kStageMaxChunks + 1u;
+    if (tid == 0) {
+        uint32_t run = 0;
+        for (uint32_t j = 0; j < nCh; ++j) {
+            cst[j] = run;
+            run += cT[j];
+        }
+        cst[nCh] = run;
+        if (run != eOff[b + 1u] - eOff[b]) atomicOr(fault, 4u);
+    }
+    __syncthreads();
+    const uint32_t eb = eOff[b], eEnd = eOff[b + 1u];
+    for (uint32_t k = 0, j = wave; k < kChunksPerWave; ++k, j += 16u) {
+        if (j >= nCh) break;
+        for (uint32_t h = 0; h < kTeamsPerLane; ++h)
+            meta[mOff[b] + j * kStageFacets + lane + 64u * h] = make_uint2(eb + cst[j] + mine[k][h].x, mine[k][h].y);
+    }
+    for (uint32_t lr = wave; lr < kStageRows; lr += 16u) {
+        const uint32_t s = stage_side(order, nF, b, lr);
+        if (s == kStageNone) continue;
+        stage_row_runs(col, bm, rk, rowptr[s], rowptr[s + 1u],
+                       [&](uint32_t e, uint32_t j, uint32_t slot, uint32_t first, bool last) {
+                           const uint32_t seg = eb + cst[j] + pos[lr * kStageMaxChunks + j];
+                           const uint32_t at = seg + (e - first);
+                           const uint32_t end = last ? seg + padded(e + 1u - first) : at + 1u;
+                           if (end > eEnd) {  // cannot happen with consistent counts; never write past the block
+                               atomicOr(fault, 2u);
+                               return;
+                           }
+                           key[at] = uint16_t(slot - j * kStageSlots);
+                           wout[at] = w32[e];
+                           for (uint32_t p = at + 1u; p < end; ++p) {  // the segment's padding
+                               key[p] = 0u;
+                               wout[p] = 0.0f;
+                           }
+                       });
+    }
+}
+
 // FP8 increment sweep of the coupled pairs.  A team of bt lanes (a
 // multiple of 3, teams never straddle a wave) owns the two facets of a
 // unit; a lane owns 4 bands, so the three lanes of one 12-band chunk round
@@ -1800,7 +2246,7 @@ __global__ void k_unit_weights(const uint32_t* __restrict__ rowptr, const uint32
     unitW[2u * u + 1u] = p.y == kNoFacet ? z : block_weights(rowptr, col, w, p.y, p.x);
 }
 
-void build_pairs(fh_context* c) {
+static void build_pair_units(fh_context* c) {
     c->units.release();
     c->unitW.release();
     c->singles.release();
@@ -1866,6 +2312,135 @@ void build_pairs(fh_context* c) {
     c->singleCount = nS;
 }
 
+static bool use_staged() {  // read at every launch: tests compare both paths in one process
+    const char* v = std::getenv("FH_STAGED");
+    return !(v && std::atoi(v) == 0);
+}
+
+static void release_stage(fh_context* c) {
+    c->stBlocks = 0u;
+    c->stDcol.release();
+    c->stDoff.release();
+    c->stMoff.release();
+    c->stMeta.release();
+    c->stKey.release();
+    c->stW.release();
+}
+
+__global__ void k_stage_chunks(const uint32_t* __restrict__ U, uint32_t nBlk, uint32_t* __restrict__ runs) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nBlk) runs[b] = (U[b] + kStageSlots - 1u) / kStageSlots * kStageFacets;
+}
+
+// The staged layout for the order of the increment sweeps' singles launch:
+// the singles list when there are pair blocks, else the sweep schedule.  A
+// block with more than kStageMaxChunks chunks of distinct columns leaves
+// the graph without a staged layout (the increment sweeps then gather
+// through the vector L1).
+static void build_stage(fh_context* c) {
+    release_stage(c);
+    const bool onSingles = c->unitCount > 0u;
+    const uint32_t* order = onSingles ? c->singles.p : sweep_schedule(c);
+    const uint32_t nF = onSingles ? c->singleCount : c->F;
+    if (nF == 0u || c->nnz == 0u) return;
+    static const bool attr = [] {
+        FH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stage_fill<false>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(stage_fill_lds_bytes())));
+        FH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_stage_fill<true>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(stage_fill_lds_bytes())));
+        return true;
+    }();
+    (void)attr;
+    const uint32_t nBlk = (nF + kStageFacets - 1u) / kStageFacets;
+    const uint32_t words = (c->S + 31u) / 32u;
+    const uint32_t G = std::min(nBlk, 512u);
+    DBuf<uint32_t> bitmap, rank, U, P, runs, eOff, fault;
+    bitmap.ensure(size_t(G) * words);
+    rank.ensure(size_t(G) * words);
+    U.ensure(nBlk);
+    P.ensure(nBlk);
+    runs.ensure(nBlk);
+    eOff.ensure(size_t(nBlk) + 1u);
+    fault.ensure(1);
+    launch_fill_u32(c, fault.p, 1, 0u);
+    c->stDoff.ensure(size_t(nBlk) + 1u);
+    c->stMoff.ensure(size_t(nBlk) + 1u);
+    const size_t fillLds = stage_fill_lds_bytes();
+    const auto batch = [&](uint32_t b0, bool write) {
+        launch_fill_u32(c, bitmap.p, size_t(G) * words, 0u);
+        hipLaunchKernelGGL(k_stage_bits, dim3(G), dim3(1024), 0, c->stream, c->rowptr.p, c->col.p, order, nF, nBlk, b0,
+                           words, bitmap.p);
+        hipLaunchKernelGGL(k_stage_rank, dim3(G), dim3(1024), 0, c->stream, bitmap.p, nBlk, b0, words, rank.p, U.p,
+                           write ? static_cast<const uint32_t*>(c->stDoff.p) : nullptr, c->stDcol.p);
+        if (write)
+            hipLaunchKernelGGL(k_stage_fill<true>, dim3(G), dim3(1024), fillLds, c->stream, c->rowptr.p, c->col.p,
+                               c->val32.p, order, nF, nBlk, b0, words, bitmap.p, rank.p, U.p, P.p, eOff.p,
+                               c->stMoff.p, c->stMeta.p, reinterpret_cast<uint16_t*>(c->stKey.p), c->stW.p, fault.p);
+        else
+            hipLaunchKernelGGL(k_stage_fill<false>, dim3(G), dim3(1024), fillLds, c->stream, c->rowptr.p, c->col.p,
+                               c->val32.p, order, nF, nBlk, b0, words, bitmap.p, rank.p, U.p, P.p, eOff.p,
+                               c->stMoff.p, c->stMeta.p, reinterpret_cast<uint16_t*>(c->stKey.p), c->stW.p, fault.p);
+        FH_HIP(hipGetLastError());
+    };
+    // sizes: distinct columns, padded entries and run descriptors per block
+    for (uint32_t b0 = 0; b0 < nBlk; b0 += G) batch(b0, false);
+    hipLaunchKernelGGL(k_stage_chunks, dim3((nBlk + 255u) / 256u), dim3(256), 0, c->stream, U.p, nBlk, runs.p);
+    launch_scan(c, U.p, nBlk, c->stDoff.p);
+    launch_scan(c, P.p, nBlk, eOff.p);
+    launch_scan(c, runs.p, nBlk, c->stMoff.p);
+    uint32_t tot[4] = {0u, 0u, 0u, 0u};
+    FH_HIP(hipMemcpyAsync(&tot[0], c->stDoff.p + nBlk, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipMemcpyAsync(&tot[1], eOff.p + nBlk, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipMemcpyAsync(&tot[2], c->stMoff.p + nBlk, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipMemcpyAsync(&tot[3], fault.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipStreamSynchronize(c->stream));
+    if (tot[3]) {
+        release_stage(c);
+        return;
+    }
+    c->stDcol.ensure(std::max<uint32_t>(tot[0], 1u));
+    c->stKey.ensure(tot[1] / 2u + kCsrPad);
+    c->stW.ensure(size_t(tot[1]) + 4u * kCsrPad);
+    c->stMeta.ensure(std::max<uint32_t>(tot[2], 1u));
+    for (uint32_t b0 = 0; b0 < nBlk; b0 += G) batch(b0, true);
+    FH_HIP(hipMemcpyAsync(&tot[3], fault.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    FH_HIP(hipStreamSynchronize(c->stream));
+    if (tot[3]) {
+        release_stage(c);
+        return;
+    }
+    c->stBlocks = nBlk;
+    c->stOnSingles = onSingles;
+}
+
+void build_pairs(fh_context* c) {
+    build_pair_units(c);
+    build_stage(c);
+}
+
+// the staged singles launch of an increment sweep: 64 teams of bt lanes per
+// workgroup, the LDS of stage_lds_bytes(bt)
+static void launch_staged(fh_context* c, const SweepArgs& a, uint32_t grid, hipStream_t st) {
+    static const bool attr = [] {
+        FH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep_staged<1>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(stage_lds_bytes(kStageMaxBands / 8u))));
+        FH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep_staged<2>),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(stage_lds_bytes(kStageMaxBands / 8u))));
+        return true;
+    }();
+    (void)attr;
+    StageArgs sa;
+    sa.dcol = c->stDcol.p;
+    sa.dOff = c->stDoff.p;
+    sa.mOff = c->stMoff.p;
+    sa.meta = c->stMeta.p;
+    sa.key = reinterpret_cast<const uint2*>(c->stKey.p);
+    sa.w = reinterpret_cast<const float4*>(c->stW.p);
+    const dim3 block(kStageFacets * a.bt);
+    if (a.inc == 1u) hipLaunchKernelGGL((k_sweep_staged<1>), dim3(grid), block, stage_lds_bytes(a.bt), st, a, sa);
+    else hipLaunchKernelGGL((k_sweep_staged<2>), dim3(grid), block, stage_lds_bytes(a.bt), st, a, sa);
+}
+
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.xcd = xcd_remap(band_parallel(a.ldb));
     a.sched = sweep_schedule(c);
@@ -1903,12 +2478,17 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         // writing its own range of the per-block partials
         const bool pair =
             c->unitCount && use_pairs() && !a.jacobiOnly && (a.inc || (mode == MODE_SWEEP && !w16 && !half));
+        // staged gathers for the single facets of increment sweeps, where the
+        // staged layout was built for this launch's order
+        const bool staged = a.inc && !q8 && c->stBlocks && use_staged() && a.ldb <= kStageMaxBands &&
+                            c->stOnSingles == pair;
         SweepArgs p = a;
         uint32_t g = std::max(1u, (c->F + perBlock - 1u) / perBlock), gp = 0u;
+        if (staged) g = c->stBlocks;
         if (pair) {
             a.sched = c->singles.p;
             a.F = c->singleCount;
-            g = (a.F + perBlock - 1u) / perBlock;
+            g = staged ? c->stBlocks : (a.F + perBlock - 1u) / perBlock;
             p.bt = q8 ? std::min((p.ldb + 3u) / 4u + 2u, 15u) / 3u * 3u : std::min((p.ldb + 3u) / 4u, 16u);
             p.units = c->units.p;
             p.unitW = c->unitW.p;
@@ -1946,6 +2526,7 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
         }
         if (g) {
             if (q8) hipLaunchKernelGGL(k_sweep_q8_single, dim3(g), dim3(256), 0, st, a);
+            else if (staged) launch_staged(c, a, g, st);
             else if (a.inc == 1u) hipLaunchKernelGGL((k_sweep_hv<1, false>), dim3(g), dim3(256), 0, st, a);
             else if (a.inc) hipLaunchKernelGGL((k_sweep_hv<2, false>), dim3(g), dim3(256), 0, st, a);
             else if (half) hipLaunchKernelGGL((k_sweep_hv<0, false>), dim3(g), dim3(256), 0, st, a);

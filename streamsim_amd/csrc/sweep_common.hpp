@@ -7,6 +7,18 @@
 
 namespace fh {
 
+// A control-block word (iteration count, active bands, done flags) as a
+// wave-uniform value.  The word is written only by kernels launched before
+// the reading one, so every lane reads the same value; readfirstlane moves
+// it into a scalar register, and the pointers selected by it (and the
+// buffer descriptors built from them) stay scalar.  Without it a buffer
+// load through a descriptor in vector registers becomes a loop over the
+// distinct descriptors of the wave (readfirstlane, compare, exec mask) at
+// every gather.
+__device__ __forceinline__ uint32_t ctrl_word(const uint32_t* p) {
+    return __builtin_amdgcn_readfirstlane(*reinterpret_cast<const volatile uint32_t*>(p));
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  With remap set, hardware block b runs logical block xcd_block(), so
 // each XCD works through one contiguous range of rows and the columns that

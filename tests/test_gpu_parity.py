@@ -856,6 +856,46 @@ def test_bf16_increments_follow_fp32(oracle, nb):
     assert _rel(bf.radiosity, ref) < RTOL and _rel(bf.radiosity, f32.radiosity) < 2e-6
 
 
+@pytest.mark.parametrize("scene,R,nb", [("forest", 256, 47), ("row_crop", 64, 40)])
+def test_staged_gathers_match_l1_gathers(oracle, monkeypatch, scene, R, nb):
+    # The increment sweeps of the single facets gather each workgroup's
+    # distinct columns once through the LDS (chunk-major copy of the CSR);
+    # FH_STAGED=0 gathers every entry through the vector L1.  Only the order
+    # of the FP32 sums differs: three sweeps from the same start agree to a
+    # few FP32 ulps, and converged solves take the same sweeps to the same
+    # fixed point.
+    print(f"staged gathers {scene} R={R} nb={nb}", flush=True)
+    sc = scenes.forest() if scene == "forest" else scenes.row_crop(leaves_per_row=200, cells=20)
+    eng = _engine(R=R, seed=3)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    rng = np.random.default_rng(7 + nb)
+    S = sc.surface_count
+    r = rng.uniform(0.0, 0.5, (S, nb)).astype(np.float32)
+    t = (rng.uniform(0.0, 1.0, (S, nb)) * (1.0 - r) * 0.8).astype(np.float32)
+    e = np.zeros((S, nb), np.float32)
+    D = rng.uniform(0.0, 300.0, (S, nb)).astype(np.float32)
+    sky = rng.uniform(10.0, 80.0, nb).astype(np.float32)
+    init = rng.uniform(10.0, 200.0, (S, nb)).astype(np.float32)
+    out = {}
+    for staged in ("1", "0"):
+        monkeypatch.setenv("FH_STAGED", staged)
+        few = eng.solveBands(r, t, e, D, sky, init=init, maxIter=3, tolRel=0.0, precision=FH_PRECISION_BF16_DC)
+        full = eng.solveBands(r, t, e, D, sky, init=init, tolRel=1e-7, precision=FH_PRECISION_BF16_DC)
+        out[staged] = (few, full)
+    (f1, c1), (f0, c0) = out["1"], out["0"]
+    print(f"  sweeps staged {c1.iterations}, L1 {c0.iterations}", flush=True)
+    assert f1.iterations == f0.iterations == 3
+    assert _rel(f1.radiosity, f0.radiosity) < 1e-6
+    assert abs(c1.iterations - c0.iterations) <= 1
+    assert _rel(c1.radiosity, c0.radiosity) < 2e-6
+    if scene == "row_crop":
+        og = _as_oracle_graph(oracle, eng.getGraph())
+        ref, _, _ = oracle.solve_bands(og, r, t, e, D, sky, tol_rel=1e-9)
+        assert _rel(c1.radiosity, ref) < RTOL
+    eng.destroy()
+
+
 class _Hip:
     """hipMalloc / hipMemcpy / hipFree through ctypes, for the device-pointer entry points."""
 
