@@ -1306,24 +1306,15 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
 #pragma unroll
         for (uint32_t k = 0; k < kStagePer; ++k) rows[(team + kStageFacets * k) * bt + lane] = rv[k];
     };
-    if (nCh) {
-        load_dcol(0);
-        load_rows();
-        if (nCh > 1u) load_dcol(1);
-        store_rows();
-    }
-    __syncthreads();
-    for (uint32_t j = 0; j < nCh; ++j) {
-        const uint2 m = meta[j * kStageFacets + team];
-        const uint32_t qa = m.y & 255u, qn = qa + ((m.y >> 8) & 255u);
-        const uint32_t rA = (m.y >> 16) & 255u, rB = m.y >> 24;
-        const uint32_t g0 = m.x >> 2;  // first group of 4 entries
-        float s[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) s[q] = 0.0f;
-        // the slots and weights of the next two groups are loaded ahead
-        uint2 k1 = make_uint2(0u, 0u), k2 = k1;
-        float4 w1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w2 = w1;
+    // Everything a chunk needs from global memory (its staged rows, its run
+    // descriptor and the first two groups of the run) is loaded during the
+    // chunk before, so a chunk waits for one batch of loads, not three
+    // dependent ones; only the later groups of a run are loaded in the run.
+    uint2 mNext = make_uint2(0u, 0u), mNext2 = mNext;
+    uint2 k1 = make_uint2(0u, 0u), k2 = k1;
+    float4 w1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w2 = w1;
+    const auto first_groups = [&](uint2 m) {  // the first two groups of the run of m
+        const uint32_t qn = (m.y & 255u) + ((m.y >> 8) & 255u), g0 = m.x >> 2;
         if (qn) {
             k1 = st.key[g0];
             w1 = st.w[g0];
@@ -1332,6 +1323,29 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
             k2 = st.key[g0 + 1u];
             w2 = st.w[g0 + 1u];
         }
+    };
+    if (nCh) {
+        mNext = meta[team];
+        load_dcol(0);
+        load_rows();
+        if (nCh > 1u) {
+            load_dcol(1);
+            mNext2 = meta[kStageFacets + team];
+        }
+        store_rows();
+        first_groups(mNext);
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < nCh; ++j) {
+        const uint2 m = mNext;
+        mNext = mNext2;
+        const uint32_t qa = m.y & 255u, qn = qa + ((m.y >> 8) & 255u);
+        const uint32_t rA = (m.y >> 16) & 255u, rB = m.y >> 24;
+        const uint32_t g0 = m.x >> 2;  // first group of 4 entries
+        float s[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s[q] = 0.0f;
+        // the slots and weights of the next two groups are loaded ahead
         for (uint32_t q = 0; q < qn; ++q) {
             const uint2 k = k1;
             const float4 w = w1;
@@ -1359,11 +1373,15 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
                 for (int t = 0; t < 8; ++t) s[t] = 0.0f;
             }
         }
-        // the next chunk's rows: loaded after this chunk's entries, which
-        // would otherwise wait behind them
+        // the next chunk's loads, issued after this chunk's run loads (loads
+        // complete in order)
         if (j + 1u < nCh) {
             load_rows();
-            if (j + 2u < nCh) load_dcol(j + 2u);
+            if (j + 2u < nCh) {
+                load_dcol(j + 2u);
+                mNext2 = meta[(j + 2u) * kStageFacets + team];
+            }
+            first_groups(mNext);
         }
         __syncthreads();
         if (j + 1u < nCh) store_rows();
