@@ -377,11 +377,15 @@ def main() -> None:
         F, nnz = scene.facet_count, diag.directedEdgeCount
         out = {"workload": a.workload, "sides": S, "facets": F, "nnz": nnz}
         for name, nbk, prec, tab in (("sw_fp32", nb, FH_PRECISION_FP32, True), ("sw_inc", nb, FH_PRECISION_BF16_DC, True),
+                                     ("sw_inc_l1", nb, FH_PRECISION_BF16_DC, True),
                                      ("sw_q8", nb, FH_PRECISION_FP8_INC, True),
                                      ("sw_bf16f", nb, FH_PRECISION_BF16_F, True),
                                      ("sw_fp32_arrays", nb, FH_PRECISION_FP32, False),
                                      ("sw_inc_arrays", nb, FH_PRECISION_BF16_DC, False),
                                      ("lw_fp32", 1, FH_PRECISION_FP32, False)):
+            # sw_inc_l1: the increment sweep without the staged gathers (every
+            # entry gathered through the vector L1), for comparison
+            os.environ["FH_STAGED"] = "0" if name == "sw_inc_l1" else "1"
             sec = time_sweep(eng, nbk, prec, a.roofline_reps, tables=tab)
             # single-band sweeps read the packed entries
             alg = sweep_bytes(S, F, nnz, nbk, tables=tab and nbk >= 16, entry=8 if nbk >= 16 else 4)
@@ -390,6 +394,7 @@ def main() -> None:
             extra = scheme_bytes(S, nbk, increments=inc, chebyshev=tab and nbk >= 16)
             out[name] = {"us": sec * 1e6, "alg_bytes": alg, "scheme_bytes": extra, "GBps": alg / sec / 1e9,
                          "frac": alg / sec / 1e9 / HBM_PEAK_GBS}
+        os.environ.pop("FH_STAGED", None)
         cs = eng.benchmarkCopy(1 << 30, 20)
         out["copy_1GiB"] = {"us": cs * 1e6, "GBps": 2.0 * (1 << 30) / cs / 1e9}
         print(json.dumps(out), flush=True)
@@ -513,8 +518,9 @@ def main() -> None:
                          "bvh_host_seconds": t_bvh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic16,
-                     "kernel": (f"k_sweep_hv<2, false> (single facets) + k_sweep_hv<2, true> (coupled pairs, second "
-                                f"stream, side by side): increment sweep of the BF16 shortwave solves"
+                     "kernel": (f"k_sweep_staged<2> (single facets, gathers staged through the LDS) + "
+                                f"k_sweep_hv<2, true> (coupled pairs, second stream, side by side): increment sweep "
+                                f"of the BF16 shortwave solves"
                                 if a.fast_precision == "bf16" else
                                 "k_sweep_q8_single (single facets) + k_sweep_q8_pair (coupled pairs, second stream, side "
                                 "by side): increment sweep of the FP8 shortwave solves") +
