@@ -239,10 +239,9 @@ struct fh_context {
     // 64 facets of the singles launch's order its sorted distinct columns
     // (stDcol from stDoff[b]), per chunk and team a run descriptor (stMeta
     // from stMoff[b]) and the padded entries it points to (stKey: slot in
-    // the chunk, 2 per word; stW: weight); stBlocks = 0: not built
+    // the chunk | count << 10, 0 on padding); stBlocks = 0: not built
     fh::DBuf<uint32_t> stDcol, stDoff, stMoff, stKey;
     fh::DBuf<uint2> stMeta;
-    fh::DBuf<float> stW;
     uint32_t stBlocks = 0;
     bool stOnSingles = false;  // built on the singles list (pair blocks on), else on the schedule
     fh::DBuf<float> val32, fsky;

@@ -1202,9 +1202,10 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
 // ranked by segment length and team t takes ranks t and kStageRows - 1 - t
 // (long with short), and a block-level copy of the CSR holds, per chunk and
 // team, the two segments back to back, each padded to a multiple of 4
-// entries with zero weights, as the slot in the chunk (2 bytes) and the FP32
-// weight; a run descriptor per chunk and team gives the first entry, the
-// groups of 4 of each segment and the two local rows.  The groups need no
+// entries of count 0, as 4 bytes: the slot in the chunk and the entry's ray
+// count (weight count / den of the row, formed as count * (1 / den)); a run
+// descriptor per chunk and team gives the first entry, the groups of 4 of
+// each segment and the two local rows.  The groups need no
 // masks and a team adds each segment's sum to its row's FP32 sums in the
 // LDS once (a row belongs to one team per chunk: no atomics, and the sums
 // do not depend on scheduling).  On the 468 M-entry terrain graph a block of
@@ -1224,13 +1225,14 @@ struct StageArgs {
     const uint32_t* dOff;   // [blocks + 1]
     const uint32_t* mOff;   // [blocks] run descriptor of (chunk j, team t) of block b at mOff[b] + 64 j + t
     const uint2* meta;      // x: first entry; y: groups of row A | groups of row B << 8 | A << 16 | B << 24
-    const uint2* key;       // padded entries, 4 per vector: slot in chunk (16 bits each)
-    const float4* w;        // their weights, 4 per vector (0 on padding)
+    const uint4* key;       // padded entries, 4 per vector: slot in chunk | count << 10 (count 0: padding)
+    const uint32_t* order;  // facet of the sweep order's position i (null: facet i)
+    const uint32_t* den;    // [S] denominators: an entry's weight is count / den of its row
 };
 
 // dynamic LDS of k_sweep_staged: staged BF16 rows and the rows' FP32 sums
 __host__ __device__ constexpr uint32_t stage_lds_bytes(uint32_t bt) {
-    return kStageSlots * bt * 16u + kStageRows * bt * 32u;
+    return kStageSlots * bt * 16u + kStageRows * bt * 32u + kStageRows * 4u;
 }
 
 // max-reduce n floats over the blockDim.x / 64 waves of a block (at most 16)
@@ -1274,10 +1276,16 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
     const uint32_t blk = xcd_block(a.xcd);
     uint4* rows = stageLds;                                               // [kStageSlots][bt]
     float* acc = reinterpret_cast<float*>(rows + kStageSlots * bt);       // [kStageRows][nb8]
+    float* rinv = acc + kStageRows * nb8;                                 // [kStageRows] 1 / den of the local rows
     const uint32_t b0 = 8u * lane;
     const bool act = b0 < x.nAct;
     for (uint32_t i = tid; i < kStageRows * nb8 / 4u; i += blockDim.x)
         reinterpret_cast<float4*>(acc)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t lr = tid; lr < kStageRows; lr += blockDim.x) {
+        const uint32_t i = blk * kStageFacets + (lr >> 1);
+        const uint32_t d = i < a.F ? st.den[2u * (st.order ? st.order[i] : i) + (lr & 1u)] : 0u;
+        rinv[lr] = d ? 1.0f / float(d) : 0.0f;
+    }
     const uint32_t d0 = st.dOff[blk], U = st.dOff[blk + 1u] - d0;
     const uint32_t nCh = (U + kStageSlots - 1u) / kStageSlots;
     const uint32_t* dc = st.dcol + d0;
@@ -1311,18 +1319,11 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
     // chunk before, so a chunk waits for one batch of loads, not three
     // dependent ones; only the later groups of a run are loaded in the run.
     uint2 mNext = make_uint2(0u, 0u), mNext2 = mNext;
-    uint2 k1 = make_uint2(0u, 0u), k2 = k1;
-    float4 w1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w2 = w1;
+    uint4 k1 = make_uint4(0u, 0u, 0u, 0u), k2 = k1;
     const auto first_groups = [&](uint2 m) {  // the first two groups of the run of m
         const uint32_t qn = (m.y & 255u) + ((m.y >> 8) & 255u), g0 = m.x >> 2;
-        if (qn) {
-            k1 = st.key[g0];
-            w1 = st.w[g0];
-        }
-        if (qn > 1u) {
-            k2 = st.key[g0 + 1u];
-            w2 = st.w[g0 + 1u];
-        }
+        if (qn) k1 = st.key[g0];
+        if (qn > 1u) k2 = st.key[g0 + 1u];
     };
     if (nCh) {
         mNext = meta[team];
@@ -1342,24 +1343,23 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
         const uint32_t qa = m.y & 255u, qn = qa + ((m.y >> 8) & 255u);
         const uint32_t rA = (m.y >> 16) & 255u, rB = m.y >> 24;
         const uint32_t g0 = m.x >> 2;  // first group of 4 entries
+        const float yA = rinv[rA], yB = rinv[rB];
         float s[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) s[q] = 0.0f;
         // the slots and weights of the next two groups are loaded ahead
         for (uint32_t q = 0; q < qn; ++q) {
-            const uint2 k = k1;
-            const float4 w = w1;
+            const uint4 k = k1;
             k1 = k2;
-            w1 = w2;
-            if (q + 2u < qn) {
-                k2 = st.key[g0 + q + 2u];
-                w2 = st.w[g0 + q + 2u];
-            }
+            if (q + 2u < qn) k2 = st.key[g0 + q + 2u];
             if (act) {
-                fma_bf16x8(s, w.x, rows[__umul24(k.x & 0xffffu, bt) + lane]);
-                fma_bf16x8(s, w.y, rows[__umul24(k.x >> 16, bt) + lane]);
-                fma_bf16x8(s, w.z, rows[__umul24(k.y & 0xffffu, bt) + lane]);
-                fma_bf16x8(s, w.w, rows[__umul24(k.y >> 16, bt) + lane]);
+                // weight count / den as count * (1 / den): within one FP32
+                // ulp of the correctly rounded quotient (exact for count 1)
+                const float y = q < qa ? yA : yB;
+                fma_bf16x8(s, float(k.x >> 10) * y, rows[__umul24(k.x & 1023u, bt) + lane]);
+                fma_bf16x8(s, float(k.y >> 10) * y, rows[__umul24(k.y & 1023u, bt) + lane]);
+                fma_bf16x8(s, float(k.z >> 10) * y, rows[__umul24(k.z & 1023u, bt) + lane]);
+                fma_bf16x8(s, float(k.w >> 10) * y, rows[__umul24(k.w & 1023u, bt) + lane]);
             }
             if (q + 1u == qa || q + 1u == qn) {  // a segment ends: add it to its row
                 const uint32_t r = q + 1u == qa ? rA : rB;
@@ -1523,13 +1523,13 @@ __host__ __device__ constexpr uint32_t stage_fill_lds_bytes() {
 // eOff[b].  Segment lengths per (local row, chunk) in the LDS.
 template <bool WRITE>
 __global__ __launch_bounds__(1024) void k_stage_fill(const uint32_t* __restrict__ rowptr,
-                                                     const uint32_t* __restrict__ col, const float* __restrict__ w32,
+                                                     const uint32_t* __restrict__ col,
                                                      const uint32_t* __restrict__ order, uint32_t nF, uint32_t nBlk,
                                                      uint32_t b0, uint32_t words, const uint32_t* __restrict__ bitmap,
                                                      const uint32_t* __restrict__ rank, const uint32_t* __restrict__ U,
                                                      uint32_t* __restrict__ P, const uint32_t* __restrict__ eOff,
                                                      const uint32_t* __restrict__ mOff, uint2* __restrict__ meta,
-                                                     uint16_t* __restrict__ key, float* __restrict__ wout,
+                                                     const uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
                                                      uint32_t* __restrict__ fault) {
     extern __shared__ uint32_t fillLds[];
     const uint32_t b = b0 + blockIdx.x;
@@ -1645,12 +1645,9 @@ __global__ __launch_bounds__(1024) void k_stage_fill(const uint32_t* __restrict_
                                atomicOr(fault, 2u);
                                return;
                            }
-                           key[at] = uint16_t(slot - j * kStageSlots);
-                           wout[at] = w32[e];
-                           for (uint32_t p = at + 1u; p < end; ++p) {  // the segment's padding
-                               key[p] = 0u;
-                               wout[p] = 0.0f;
-                           }
+                           if (cnt[e] >= (1u << 22)) atomicOr(fault, 32u);  // the count field holds 22 bits
+                           key[at] = (slot - j * kStageSlots) | (cnt[e] << 10);
+                           for (uint32_t p = at + 1u; p < end; ++p) key[p] = 0u;  // the segment's padding
                        });
     }
 }
@@ -2342,7 +2339,6 @@ static void release_stage(fh_context* c) {
     c->stMoff.release();
     c->stMeta.release();
     c->stKey.release();
-    c->stW.release();
 }
 
 __global__ void k_stage_chunks(const uint32_t* __restrict__ U, uint32_t nBlk, uint32_t* __restrict__ runs) {
@@ -2392,12 +2388,12 @@ static void build_stage(fh_context* c) {
                            write ? static_cast<const uint32_t*>(c->stDoff.p) : nullptr, c->stDcol.p);
         if (write)
             hipLaunchKernelGGL(k_stage_fill<true>, dim3(G), dim3(1024), fillLds, c->stream, c->rowptr.p, c->col.p,
-                               c->val32.p, order, nF, nBlk, b0, words, bitmap.p, rank.p, U.p, P.p, eOff.p,
-                               c->stMoff.p, c->stMeta.p, reinterpret_cast<uint16_t*>(c->stKey.p), c->stW.p, fault.p);
+                               order, nF, nBlk, b0, words, bitmap.p, rank.p, U.p, P.p, eOff.p, c->stMoff.p,
+                               c->stMeta.p, c->cnt.p, c->stKey.p, fault.p);
         else
             hipLaunchKernelGGL(k_stage_fill<false>, dim3(G), dim3(1024), fillLds, c->stream, c->rowptr.p, c->col.p,
-                               c->val32.p, order, nF, nBlk, b0, words, bitmap.p, rank.p, U.p, P.p, eOff.p,
-                               c->stMoff.p, c->stMeta.p, reinterpret_cast<uint16_t*>(c->stKey.p), c->stW.p, fault.p);
+                               order, nF, nBlk, b0, words, bitmap.p, rank.p, U.p, P.p, eOff.p, c->stMoff.p,
+                               c->stMeta.p, c->cnt.p, c->stKey.p, fault.p);
         FH_HIP(hipGetLastError());
     };
     // sizes: distinct columns, padded entries and run descriptors per block
@@ -2417,8 +2413,7 @@ static void build_stage(fh_context* c) {
         return;
     }
     c->stDcol.ensure(std::max<uint32_t>(tot[0], 1u));
-    c->stKey.ensure(tot[1] / 2u + kCsrPad);
-    c->stW.ensure(size_t(tot[1]) + 4u * kCsrPad);
+    c->stKey.ensure(size_t(tot[1]) + 4u * kCsrPad);
     c->stMeta.ensure(std::max<uint32_t>(tot[2], 1u));
     for (uint32_t b0 = 0; b0 < nBlk; b0 += G) batch(b0, true);
     FH_HIP(hipMemcpyAsync(&tot[3], fault.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -2452,8 +2447,9 @@ static void launch_staged(fh_context* c, const SweepArgs& a, uint32_t grid, hipS
     sa.dOff = c->stDoff.p;
     sa.mOff = c->stMoff.p;
     sa.meta = c->stMeta.p;
-    sa.key = reinterpret_cast<const uint2*>(c->stKey.p);
-    sa.w = reinterpret_cast<const float4*>(c->stW.p);
+    sa.key = reinterpret_cast<const uint4*>(c->stKey.p);
+    sa.order = a.sched;
+    sa.den = c->den.p;
     const dim3 block(kStageFacets * a.bt);
     if (a.inc == 1u) hipLaunchKernelGGL((k_sweep_staged<1>), dim3(grid), block, stage_lds_bytes(a.bt), st, a, sa);
     else hipLaunchKernelGGL((k_sweep_staged<2>), dim3(grid), block, stage_lds_bytes(a.bt), st, a, sa);
