@@ -886,12 +886,26 @@ static uint32_t coupled_loop(fh_context* c, const fh_eb_node_params& p, const Eb
         for (uint32_t q = 0; q < batch; ++q) {
             a.Bin = (launched & 1u) ? e.Blw1.p : e.Blw0.p;
             a.Bout = (launched & 1u) ? e.Blw0.p : e.Blw1.p;
-#define FH_COUPLED(T, W) hipLaunchKernelGGL((k_coupled_gather<T, W>), dim3(grid), dim3(256), 0, c->stream, a)
-            if (team == 4) { if (w16) FH_COUPLED(4, true); else FH_COUPLED(4, false); }
-            else if (team == 8) { if (w16) FH_COUPLED(8, true); else FH_COUPLED(8, false); }
-            else if (team == 16) { if (w16) FH_COUPLED(16, true); else FH_COUPLED(16, false); }
-            else if (team == 32) { if (w16) FH_COUPLED(32, true); else FH_COUPLED(32, false); }
-            else { if (w16) FH_COUPLED(64, true); else FH_COUPLED(64, false); }
+            // FP32 gathers through the staged layout (every distinct column
+            // of a workgroup gathered once; the pair blocks' facets, which
+            // the layout leaves out, in the team kernel), else the team
+            // kernel over all facets
+            const bool staged = !w16 && launch_gather1_staged(c, a.Bin, a.L, c->ctrl.p, a.Iin);
+            CoupledArgs g = a;
+            uint32_t gg = grid;
+            if (staged) {
+                g.sched = c->pairFacets.p;
+                g.F = 2u * c->unitCount;
+                gg = uint32_t((uint64_t(g.F) * uint64_t(team) + 255u) / 256u);
+            }
+#define FH_COUPLED(T, W) hipLaunchKernelGGL((k_coupled_gather<T, W>), dim3(gg), dim3(256), 0, c->stream, g)
+            if (gg) {
+                if (team == 4) { if (w16) FH_COUPLED(4, true); else FH_COUPLED(4, false); }
+                else if (team == 8) { if (w16) FH_COUPLED(8, true); else FH_COUPLED(8, false); }
+                else if (team == 16) { if (w16) FH_COUPLED(16, true); else FH_COUPLED(16, false); }
+                else if (team == 32) { if (w16) FH_COUPLED(32, true); else FH_COUPLED(32, false); }
+                else { if (w16) FH_COUPLED(64, true); else FH_COUPLED(64, false); }
+            }
 #undef FH_COUPLED
             hipLaunchKernelGGL(k_coupled_update, dim3(gridU), dim3(256), 0, c->stream, a);
             hipLaunchKernelGGL(k_coupled_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, e.part4.p, gridU,

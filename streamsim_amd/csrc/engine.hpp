@@ -229,6 +229,7 @@ struct fh_context {
     fh::DBuf<uint2> units;
     fh::DBuf<float4> unitW;
     fh::DBuf<uint32_t> singles;
+    fh::DBuf<uint32_t> pairFacets;  // the facets of the units, two per unit (single-band gathers)
     uint32_t unitCount = 0;
     uint32_t singleCount = 0;
     // the pair launch of a sweep runs beside the singles launch: second

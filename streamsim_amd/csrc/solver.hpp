@@ -133,6 +133,10 @@ const uint32_t* sweep_schedule(const fh_context* c);
 
 // lanes per facet team of the SpMM kernels for the current graph
 int sweep_team(const fh_context* c);
+// The single-band gather of the coupled longwave sweep through the staged
+// layout (Iin = fsky L + F B for the facets the layout covers: all, or the
+// singles when there are pair blocks); false: no staged layout.
+bool launch_gather1_staged(fh_context* c, const float* B, float L, const SolveCtrl* ctrl, float* Iin);
 
 // Energy-balance form of a band-parallel sweep for launch_single_sweep: r, t
 // from material tables and c, D from per-side direct factors (as the

@@ -1408,6 +1408,72 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
     }
 }
 
+// Single-band staged gather (the coupled longwave sweep): the same layout,
+// one lane per team (a workgroup of kStageFacets lanes), the chunk's 1024
+// values of B staged in the LDS (4 KB) and every run's sums added to its
+// rows' sums there; then Iin = fsky L + G for the workgroup's facets.  The
+// random single-value gathers of every entry become one gather per distinct
+// column and workgroup.
+__global__ __launch_bounds__(kStageFacets) void k_gather1_staged(StageArgs st, const float* __restrict__ B,
+                                                                  const float* __restrict__ fsky, float L, uint32_t nF,
+                                                                  const SolveCtrl* ctrl, float* __restrict__ Iin) {
+    if (ctrl && ctrl_word(&ctrl->done)) return;
+    __shared__ float vals[kStageSlots];
+    __shared__ float acc[kStageRows];
+    __shared__ float rinv[kStageRows];
+    const uint32_t t = threadIdx.x, blk = blockIdx.x;
+    for (uint32_t lr = t; lr < kStageRows; lr += kStageFacets) {
+        const uint32_t i = blk * kStageFacets + (lr >> 1);
+        const uint32_t d = i < nF ? st.den[2u * (st.order ? st.order[i] : i) + (lr & 1u)] : 0u;
+        rinv[lr] = d ? 1.0f / float(d) : 0.0f;
+        acc[lr] = 0.0f;
+    }
+    const uint32_t d0 = st.dOff[blk], U = st.dOff[blk + 1u] - d0;
+    const uint32_t nCh = (U + kStageSlots - 1u) / kStageSlots;
+    const uint32_t* dc = st.dcol + d0;
+    const uint2* meta = st.meta + st.mOff[blk];
+    for (uint32_t j = 0; j < nCh; ++j) {
+        float v[kStagePer];
+#pragma unroll
+        for (uint32_t k = 0; k < kStagePer; ++k) {
+            const uint32_t sl = j * kStageSlots + t + kStageFacets * k;
+            v[k] = sl < U ? B[dc[sl]] : 0.0f;
+        }
+        __syncthreads();  // the last chunk's runs are done with vals
+#pragma unroll
+        for (uint32_t k = 0; k < kStagePer; ++k) vals[t + kStageFacets * k] = v[k];
+        __syncthreads();
+        const uint2 m = meta[j * kStageFacets + t];
+        const uint32_t qa = m.y & 255u, qn = qa + ((m.y >> 8) & 255u);
+        const uint32_t rA = (m.y >> 16) & 255u, rB = m.y >> 24;
+        const uint32_t g0 = m.x >> 2;
+        const float yA = rinv[rA], yB = rinv[rB];
+        float sum = 0.0f;
+        uint4 k1 = make_uint4(0u, 0u, 0u, 0u);
+        if (qn) k1 = st.key[g0];
+        for (uint32_t q = 0; q < qn; ++q) {
+            const uint4 k = k1;
+            if (q + 1u < qn) k1 = st.key[g0 + q + 1u];
+            const float y = q < qa ? yA : yB;
+            sum += float(k.x >> 10) * y * vals[k.x & 1023u];
+            sum += float(k.y >> 10) * y * vals[k.y & 1023u];
+            sum += float(k.z >> 10) * y * vals[k.z & 1023u];
+            sum += float(k.w >> 10) * y * vals[k.w & 1023u];
+            if (q + 1u == qa || q + 1u == qn) {
+                acc[q + 1u == qa ? rA : rB] += sum;
+                sum = 0.0f;
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t fl = blk * kStageFacets + t;
+    if (fl < nF) {
+        const uint32_t s0 = 2u * (st.order ? st.order[fl] : fl);
+        Iin[s0] = fsky[s0] * L + acc[2u * t];
+        Iin[s0 + 1u] = fsky[s0 + 1u] * L + acc[2u * t + 1u];
+    }
+}
+
 // ---- building the staged layout ------------------------------------------
 // Blocks of kStageFacets facets of `order` (null: facet i); local row lr =
 // 2 t + side of the block's t-th facet.  Built per batch of blocks (one
@@ -2265,6 +2331,7 @@ static void build_pair_units(fh_context* c) {
     c->units.release();
     c->unitW.release();
     c->singles.release();
+    c->pairFacets.release();
     c->unitCount = 0u;
     c->singleCount = 0u;
     const uint32_t F = c->F;
@@ -2310,6 +2377,8 @@ static void build_pair_units(fh_context* c) {
     if (U == 0u) return;
     c->units.ensure(U);
     c->unitW.ensure(2u * size_t(U));
+    c->pairFacets.ensure(2u * size_t(U));
+    FH_HIP(hipMemcpyAsync(c->pairFacets.p, units.data(), sizeof(uint2) * U, hipMemcpyHostToDevice, c->stream));
     c->singles.ensure(std::max(nS, 1u));
     FH_HIP(hipMemcpyAsync(c->units.p, units.data(), sizeof(uint2) * U, hipMemcpyHostToDevice, c->stream));
     if (nS)
@@ -2453,6 +2522,23 @@ static void launch_staged(fh_context* c, const SweepArgs& a, uint32_t grid, hipS
     const dim3 block(kStageFacets * a.bt);
     if (a.inc == 1u) hipLaunchKernelGGL((k_sweep_staged<1>), dim3(grid), block, stage_lds_bytes(a.bt), st, a, sa);
     else hipLaunchKernelGGL((k_sweep_staged<2>), dim3(grid), block, stage_lds_bytes(a.bt), st, a, sa);
+}
+
+bool launch_gather1_staged(fh_context* c, const float* B, float L, const SolveCtrl* ctrl, float* Iin) {
+    if (!c->stBlocks || !use_staged()) return false;
+    StageArgs sa;
+    sa.dcol = c->stDcol.p;
+    sa.dOff = c->stDoff.p;
+    sa.mOff = c->stMoff.p;
+    sa.meta = c->stMeta.p;
+    sa.key = reinterpret_cast<const uint4*>(c->stKey.p);
+    sa.order = c->stOnSingles ? c->singles.p : sweep_schedule(c);
+    sa.den = c->den.p;
+    const uint32_t nF = c->stOnSingles ? c->singleCount : c->F;
+    hipLaunchKernelGGL(k_gather1_staged, dim3(c->stBlocks), dim3(kStageFacets), 0, c->stream, sa, B, c->fsky.p, L, nF,
+                       ctrl, Iin);
+    FH_HIP(hipGetLastError());
+    return true;
 }
 
 void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {

@@ -896,6 +896,30 @@ def test_staged_gathers_match_l1_gathers(oracle, monkeypatch, scene, R, nb):
     eng.destroy()
 
 
+def test_staged_coupled_gather_matches_team_gather(oracle, monkeypatch):
+    # The coupled longwave sweep gathers each workgroup's distinct columns
+    # once through the LDS (staged layout; the pair blocks' facets by the
+    # team kernel); FH_STAGED=0 runs the team kernel over every facet.  Only
+    # summation order and the weight's last bit (count * (1 / den)) differ:
+    # both reach the converged state within the coupling tolerance.
+    print("staged coupled gather", flush=True)
+    sc, eng, win = _eb_case(oracle, leaves=120, cells=12, R=64, K=16, nb=47)
+    met = faceteb.diurnal_meteo(6)
+    cfg = faceteb.EbConfig(tolerance=0.01)
+    out = {}
+    for staged in ("1", "0"):
+        monkeypatch.setenv("FH_STAGED", staged)
+        faceteb.setup_scene(eng, sc, win)  # the same cold start for both runs
+        st = faceteb.run_diurnal(eng, met, win, cfg, precision=FH_PRECISION_FP32)
+        out[staged] = (st, eng.ebGetState()["temperature"])
+    (s1, T1), (s0, T0) = out["1"], out["0"]
+    print(f"  coupling iterations staged {s1.coupling}, team {s0.coupling}", flush=True)
+    assert all(s1.converged) and all(s0.converged)
+    assert all(abs(x - y) <= 1 for x, y in zip(s1.coupling, s0.coupling))
+    assert np.max(np.abs(T1 - T0)) < 0.02
+    eng.destroy()
+
+
 class _Hip:
     """hipMalloc / hipMemcpy / hipFree through ctypes, for the device-pointer entry points."""
 
