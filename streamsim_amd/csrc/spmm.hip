@@ -1303,10 +1303,10 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
             dq[k] = s < U ? dc[s] : kStageNone;
         }
     };
-    const auto load_rows = [&]() {
+    const auto load_rows = [&]() {  // (lanes of retired bands load nothing)
 #pragma unroll
         for (uint32_t k = 0; k < kStagePer; ++k) {
-            const uint32_t off = dq[k] != kStageNone ? dq[k] * rowBytes + lane * 16u : kNoRow;
+            const uint32_t off = dq[k] != kStageNone && act ? dq[k] * rowBytes + lane * 16u : kNoRow;
             rv[k] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(src, off, 0, 0));
         }
     };
