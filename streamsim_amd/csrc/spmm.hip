@@ -1202,13 +1202,18 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
 // ranked by segment length and team t takes ranks t and kStageRows - 1 - t
 // (long with short), and a block-level copy of the CSR holds, per chunk and
 // team, the two segments back to back, each padded to a multiple of 4
-// entries of count 0, as 4 bytes: the slot in the chunk and the entry's ray
-// count (weight count / den of the row, formed as count * (1 / den)); a run
-// descriptor per chunk and team gives the first entry, the groups of 4 of
-// each segment and the two local rows.  The groups need no
-// masks and a team adds each segment's sum to its row's FP32 sums in the
-// LDS once (a row belongs to one team per chunk: no atomics, and the sums
-// do not depend on scheduling).  On the 468 M-entry terrain graph a block of
+// entries of count 0, as 4 bytes: the slot's byte offset in its LDS plane
+// (low half) and the entry's ray count in BF16 (high half; counts above 256
+// split in two exact parts); a run descriptor per chunk and team gives the
+// first entry, the groups of 4 of each segment and the two local rows.  A
+// segment sums count x value (two entries at a time: each word of the two
+// staged rows regrouped by band with v_perm, then one BF16 dot product per
+// band against the pair's counts, FP32 accumulation), and the team adds the
+// sum times 1 / den of the row to the row's FP32 sums in the LDS once (a
+// row belongs to one team per chunk: no atomics, and the sums do not depend
+// on scheduling).  The staged rows sit in one LDS plane per lane of the
+// team (16 bytes per slot, planes 16 KB + 96 bytes apart, so the lanes of a
+// team read distinct banks).  On the 468 M-entry terrain graph a block of
 // 128 facets references about 15 k distinct columns for 60 k entries: the
 // lines the sweep requests from L2 drop from 6.2e8 to 2.2e8, and the
 // entries read the LDS instead of the vector L1.
@@ -1225,14 +1230,17 @@ struct StageArgs {
     const uint32_t* dOff;   // [blocks + 1]
     const uint32_t* mOff;   // [blocks] run descriptor of (chunk j, team t) of block b at mOff[b] + 64 j + t
     const uint2* meta;      // x: first entry; y: groups of row A | groups of row B << 8 | A << 16 | B << 24
-    const uint4* key;       // padded entries, 4 per vector: slot in chunk | count << 10 (count 0: padding)
+    const uint4* key;       // padded entries, 4 per vector: 16 x slot in chunk | BF16 count << 16 (0: padding)
     const uint32_t* order;  // facet of the sweep order's position i (null: facet i)
     const uint32_t* den;    // [S] denominators: an entry's weight is count / den of its row
 };
 
-// dynamic LDS of k_sweep_staged: staged BF16 rows and the rows' FP32 sums
+constexpr uint32_t kStagePlane = kStageSlots * 16u + 96u;  // bytes between the LDS planes of two lanes
+
+// dynamic LDS of k_sweep_staged: staged BF16 rows (a plane per lane) and the
+// rows' FP32 sums
 __host__ __device__ constexpr uint32_t stage_lds_bytes(uint32_t bt) {
-    return kStageSlots * bt * 16u + kStageRows * bt * 32u + kStageRows * 4u;
+    return bt * kStagePlane + kStageRows * bt * 32u + kStageRows * 4u;
 }
 
 // max-reduce n floats over the blockDim.x / 64 waves of a block (at most 16)
@@ -1256,13 +1264,23 @@ __device__ __forceinline__ void block_max_waves(float (&v)[N], float (&out)[N]) 
     }
 }
 
-// s += w * (8 BF16 values of v)
-__device__ __forceinline__ void fma_bf16x8(float (&s)[8], float w, const uint4 v) {
-    const uint32_t vv[4] = {v.x, v.y, v.z, v.w};
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+
+// s += ca * (8 BF16 values of the staged row of key ka) + cb * (those of
+// kb): per word (bands 2i, 2i + 1) the two rows' values of band 2i, and of
+// band 2i + 1, are paired by v_perm and dotted with the pair (ca, cb) of
+// BF16 counts (products exact in FP32)
+__device__ __forceinline__ void stage_pair(float (&s)[8], const uint8_t* plane, uint32_t ka, uint32_t kb) {
+    const uint4 va = *reinterpret_cast<const uint4*>(plane + (ka & 0xffffu));
+    const uint4 vb = *reinterpret_cast<const uint4*>(plane + (kb & 0xffffu));
+    const bf16x2_t c = __builtin_bit_cast(bf16x2_t, __builtin_amdgcn_perm(kb, ka, 0x07060302u));
+    const uint32_t wa[4] = {va.x, va.y, va.z, va.w}, wb[4] = {vb.x, vb.y, vb.z, vb.w};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        s[2 * j] += w * __uint_as_float(vv[j] << 16);
-        s[2 * j + 1] += w * __uint_as_float(vv[j] & 0xffff0000u);
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t lo = __builtin_amdgcn_perm(wb[i], wa[i], 0x05040100u);
+        const uint32_t hi = __builtin_amdgcn_perm(wb[i], wa[i], 0x07060302u);
+        s[2 * i] = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2_t, lo), c, s[2 * i], false);
+        s[2 * i + 1] = __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2_t, hi), c, s[2 * i + 1], false);
     }
 }
 
@@ -1274,9 +1292,9 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
     const uint32_t bt = a.bt, nb8 = 8u * bt;
     const uint32_t tid = threadIdx.x, team = tid / bt, lane = tid - team * bt;
     const uint32_t blk = xcd_block(a.xcd);
-    uint4* rows = stageLds;                                               // [kStageSlots][bt]
-    float* acc = reinterpret_cast<float*>(rows + kStageSlots * bt);       // [kStageRows][nb8]
-    float* rinv = acc + kStageRows * nb8;                                 // [kStageRows] 1 / den of the local rows
+    uint8_t* plane = reinterpret_cast<uint8_t*>(stageLds) + lane * kStagePlane;  // [kStageSlots] x 16 bytes
+    float* acc = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(stageLds) + bt * kStagePlane);  // [kStageRows][nb8]
+    float* rinv = acc + kStageRows * nb8;  // [kStageRows] 1 / den of the local rows
     const uint32_t b0 = 8u * lane;
     const bool act = b0 < x.nAct;
     for (uint32_t i = tid; i < kStageRows * nb8 / 4u; i += blockDim.x)
@@ -1293,7 +1311,7 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
     const uint32_t rowBytes = a.ldh * 2u;
     const __amdgpu_buffer_rsrc_t src = make_rsrc(x.Bin16, a.sides * rowBytes);
     // staging: this thread moves 16 bytes (its lane's 8 bands) of the slots
-    // team + 64 k of a chunk; the columns are read one chunk ahead
+    // team + kStageFacets k of a chunk; the columns are read one chunk ahead
     uint32_t dq[kStagePer];
     uint4 rv[kStagePer];
     const auto load_dcol = [&](uint32_t j) {
@@ -1312,7 +1330,8 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
     };
     const auto store_rows = [&]() {
 #pragma unroll
-        for (uint32_t k = 0; k < kStagePer; ++k) rows[(team + kStageFacets * k) * bt + lane] = rv[k];
+        for (uint32_t k = 0; k < kStagePer; ++k)
+            *reinterpret_cast<uint4*>(plane + (team + kStageFacets * k) * 16u) = rv[k];
     };
     // Everything a chunk needs from global memory (its staged rows, its run
     // descriptor and the first two groups of the run) is loaded during the
@@ -1321,9 +1340,8 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
     uint2 mNext = make_uint2(0u, 0u), mNext2 = mNext;
     uint4 k1 = make_uint4(0u, 0u, 0u, 0u), k2 = k1;
     const auto first_groups = [&](uint2 m) {  // the first two groups of the run of m
-        const uint32_t qn = (m.y & 255u) + ((m.y >> 8) & 255u), g0 = m.x >> 2;
-        if (qn) k1 = st.key[g0];
-        if (qn > 1u) k2 = st.key[g0 + 1u];
+        k1 = st.key[m.x >> 2];
+        k2 = st.key[(m.x >> 2) + 1u];
     };
     if (nCh) {
         mNext = meta[team];
@@ -1343,34 +1361,46 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
         const uint32_t qa = m.y & 255u, qn = qa + ((m.y >> 8) & 255u);
         const uint32_t rA = (m.y >> 16) & 255u, rB = m.y >> 24;
         const uint32_t g0 = m.x >> 2;  // first group of 4 entries
-        const float yA = rinv[rA], yB = rinv[rB];
-        float s[8];
+        // segment A (groups [0, qa)), then segment B ([qa, qn)); k1 and k2
+        // hold groups q and q + 1, each reloaded (group q + 2, q + 3) once
+        // it is consumed, so the loop moves no key registers.  The loads are
+        // unconditional (a vmcnt wait then covers only the older load): they
+        // read at most two groups past the run, inside the layout or the key
+        // array's slack of 4 kCsrPad entries
+        uint32_t q = 0;
+        for (int seg = 0; seg < 2; ++seg) {
+            const uint32_t qe = seg ? qn : qa;
+            float s[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) s[q] = 0.0f;
-        // the slots and weights of the next two groups are loaded ahead
-        for (uint32_t q = 0; q < qn; ++q) {
-            const uint4 k = k1;
-            k1 = k2;
-            if (q + 2u < qn) k2 = st.key[g0 + q + 2u];
-            if (act) {
-                // weight count / den as count * (1 / den): within one FP32
-                // ulp of the correctly rounded quotient (exact for count 1)
-                const float y = q < qa ? yA : yB;
-                fma_bf16x8(s, float(k.x >> 10) * y, rows[__umul24(k.x & 1023u, bt) + lane]);
-                fma_bf16x8(s, float(k.y >> 10) * y, rows[__umul24(k.y & 1023u, bt) + lane]);
-                fma_bf16x8(s, float(k.z >> 10) * y, rows[__umul24(k.z & 1023u, bt) + lane]);
-                fma_bf16x8(s, float(k.w >> 10) * y, rows[__umul24(k.w & 1023u, bt) + lane]);
-            }
-            if (q + 1u == qa || q + 1u == qn) {  // a segment ends: add it to its row
-                const uint32_t r = q + 1u == qa ? rA : rB;
+            for (int t = 0; t < 8; ++t) s[t] = 0.0f;
+            const bool any = q < qe;
+            for (; q + 2u <= qe; q += 2u) {
                 if (act) {
-                    float4* p = reinterpret_cast<float4*>(acc + __umul24(r, nb8) + b0);
-                    const float4 u0 = p[0], u1 = p[1];
-                    p[0] = make_float4(u0.x + s[0], u0.y + s[1], u0.z + s[2], u0.w + s[3]);
-                    p[1] = make_float4(u1.x + s[4], u1.y + s[5], u1.z + s[6], u1.w + s[7]);
+                    stage_pair(s, plane, k1.x, k1.y);
+                    stage_pair(s, plane, k1.z, k1.w);
                 }
-#pragma unroll
-                for (int t = 0; t < 8; ++t) s[t] = 0.0f;
+                k1 = st.key[g0 + q + 2u];
+                if (act) {
+                    stage_pair(s, plane, k2.x, k2.y);
+                    stage_pair(s, plane, k2.z, k2.w);
+                }
+                k2 = st.key[g0 + q + 3u];
+            }
+            if (q < qe) {  // one group left in the segment
+                if (act) {
+                    stage_pair(s, plane, k1.x, k1.y);
+                    stage_pair(s, plane, k1.z, k1.w);
+                }
+                k1 = k2;
+                k2 = st.key[g0 + q + 2u];
+                ++q;
+            }
+            if (act && any) {  // the segment's count-weighted sum times 1 / den into its row
+                const float y = rinv[seg ? rB : rA];
+                float4* p = reinterpret_cast<float4*>(acc + __umul24(seg ? rB : rA, nb8) + b0);
+                const float4 u0 = p[0], u1 = p[1];
+                p[0] = make_float4(u0.x + y * s[0], u0.y + y * s[1], u0.z + y * s[2], u0.w + y * s[3]);
+                p[1] = make_float4(u1.x + y * s[4], u1.y + y * s[5], u1.z + y * s[6], u1.w + y * s[7]);
             }
         }
         // the next chunk's loads, issued after this chunk's run loads (loads
@@ -1447,22 +1477,20 @@ __global__ __launch_bounds__(kStageFacets) void k_gather1_staged(StageArgs st, c
         const uint32_t qa = m.y & 255u, qn = qa + ((m.y >> 8) & 255u);
         const uint32_t rA = (m.y >> 16) & 255u, rB = m.y >> 24;
         const uint32_t g0 = m.x >> 2;
-        const float yA = rinv[rA], yB = rinv[rB];
-        float sum = 0.0f;
+        const auto term = [&](uint32_t k) { return __uint_as_float(k & 0xffff0000u) * vals[(k & 0xffffu) >> 4]; };
         uint4 k1 = make_uint4(0u, 0u, 0u, 0u);
         if (qn) k1 = st.key[g0];
-        for (uint32_t q = 0; q < qn; ++q) {
-            const uint4 k = k1;
-            if (q + 1u < qn) k1 = st.key[g0 + q + 1u];
-            const float y = q < qa ? yA : yB;
-            sum += float(k.x >> 10) * y * vals[k.x & 1023u];
-            sum += float(k.y >> 10) * y * vals[k.y & 1023u];
-            sum += float(k.z >> 10) * y * vals[k.z & 1023u];
-            sum += float(k.w >> 10) * y * vals[k.w & 1023u];
-            if (q + 1u == qa || q + 1u == qn) {
-                acc[q + 1u == qa ? rA : rB] += sum;
-                sum = 0.0f;
+        uint32_t q = 0;
+        for (int seg = 0; seg < 2; ++seg) {  // segment A, then B: count-weighted sums times 1 / den
+            const uint32_t qe = seg ? qn : qa;
+            const bool any = q < qe;
+            float sum = 0.0f;
+            for (; q < qe; ++q) {
+                const uint4 k = k1;
+                if (q + 1u < qn) k1 = st.key[g0 + q + 1u];
+                sum += term(k.x) + term(k.y) + term(k.z) + term(k.w);
             }
+            if (any) acc[seg ? rB : rA] += rinv[seg ? rB : rA] * sum;
         }
     }
     __syncthreads();
@@ -1552,18 +1580,35 @@ __device__ __forceinline__ uint32_t stage_slot(const uint32_t* bm, const uint32_
     return rk[c >> 5] + uint32_t(__popc(bm[c >> 5] & ((1u << (c & 31u)) - 1u)));
 }
 
-// Per row (one wave): the segments along the row.  fn(e, j, slot,
-// firstOfSegment, lastOfSegment) for every entry e of the row.
+// Entries a CSR entry of count c takes in the staged layout: BF16 holds the
+// integers up to 256 exactly, so a larger count is split into c & ~255 and
+// c & 255 (both exact for c < 65536), two entries of the same slot.
+__device__ __forceinline__ uint32_t stage_parts(uint32_t c) { return c > 256u && (c & 255u) ? 2u : 1u; }
+
+// BF16 bits of an integer with at most 8 significant bits
+__device__ __forceinline__ uint32_t stage_bf16_bits(uint32_t n) { return __float_as_uint(float(n)) >> 16; }
+
+// Per row (one wave): the segments along the row.  fn(e, j, slot, at, n,
+// lastOfSegment) for every entry e of the row: at is the place of its first
+// layout entry in the segment (entries expanded by stage_parts), n its parts.
 template <class Fn>
-__device__ __forceinline__ void stage_row_runs(const uint32_t* __restrict__ col, const uint32_t* bm,
-                                               const uint32_t* rk, uint32_t e0, uint32_t e1, Fn&& fn) {
+__device__ __forceinline__ void stage_row_runs(const uint32_t* __restrict__ col, const uint32_t* __restrict__ cnt,
+                                               const uint32_t* bm, const uint32_t* rk, uint32_t e0, uint32_t e1,
+                                               Fn&& fn) {
     const uint32_t lane = threadIdx.x & 63u;
-    uint32_t runJ = kStageNone, runStart = e0;
+    uint32_t runJ = kStageNone, runCum = 0u, runStartCum = 0u;
     for (uint32_t base = e0; base < e1; base += 64u) {
         const uint32_t e = base + lane;
         const bool valid = e < e1;
         const uint32_t slot = valid ? stage_slot(bm, rk, col[e]) : 0u;
         const uint32_t j = valid ? slot / kStageSlots : kStageNone;
+        const uint32_t n = valid ? stage_parts(cnt[e]) : 0u;
+        uint32_t incl = n;  // layout entries of this batch up to e
+        for (uint32_t o = 1; o < 64u; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        const uint32_t cum = runCum + incl - n;  // layout entries of the row before e
         uint32_t prevJ = __shfl_up(j, 1u, 64);
         if (lane == 0u) prevJ = runJ;
         uint32_t nextJ = __shfl_down(j, 1u, 64);
@@ -1572,11 +1617,13 @@ __device__ __forceinline__ void stage_row_runs(const uint32_t* __restrict__ col,
         const unsigned long long starts = __ballot(start);
         const unsigned long long upto = lane == 63u ? ~0ull : ((2ull << lane) - 1ull);
         const unsigned long long below = starts & upto;
-        const uint32_t first = below ? base + uint32_t(63 - __clzll(below)) : runStart;
-        if (valid) fn(e, j, slot, first, j != nextJ);
+        const uint32_t cumFirst = __shfl(cum, below ? 63 - __clzll(below) : 0, 64);
+        const uint32_t segCum = below ? cumFirst : runStartCum;  // layout entries before the segment
+        if (valid) fn(e, j, slot, cum - segCum, n, j != nextJ);
         const uint32_t lastLane = min(63u, e1 - 1u - base);
         runJ = __shfl(j, int(lastLane), 64);
-        runStart = __shfl(first, int(lastLane), 64);
+        runStartCum = __shfl(segCum, int(lastLane), 64);
+        runCum += __shfl(incl, 63, 64);
     }
 }
 
@@ -1617,9 +1664,9 @@ __global__ __launch_bounds__(1024) void k_stage_fill(const uint32_t* __restrict_
     for (uint32_t lr = wave; lr < kStageRows; lr += 16u) {
         const uint32_t s = stage_side(order, nF, b, lr);
         if (s == kStageNone) continue;
-        stage_row_runs(col, bm, rk, rowptr[s], rowptr[s + 1u],
-                       [&](uint32_t e, uint32_t j, uint32_t, uint32_t first, bool last) {
-                           if (last) len[lr * kStageMaxChunks + j] = uint16_t(e + 1u - first);
+        stage_row_runs(col, cnt, bm, rk, rowptr[s], rowptr[s + 1u],
+                       [&](uint32_t, uint32_t j, uint32_t, uint32_t at, uint32_t n, bool last) {
+                           if (last) len[lr * kStageMaxChunks + j] = uint16_t(at + n);
                        });
     }
     __syncthreads();
@@ -1702,18 +1749,25 @@ __global__ __launch_bounds__(1024) void k_stage_fill(const uint32_t* __restrict_
     for (uint32_t lr = wave; lr < kStageRows; lr += 16u) {
         const uint32_t s = stage_side(order, nF, b, lr);
         if (s == kStageNone) continue;
-        stage_row_runs(col, bm, rk, rowptr[s], rowptr[s + 1u],
-                       [&](uint32_t e, uint32_t j, uint32_t slot, uint32_t first, bool last) {
+        stage_row_runs(col, cnt, bm, rk, rowptr[s], rowptr[s + 1u],
+                       [&](uint32_t e, uint32_t j, uint32_t slot, uint32_t p, uint32_t n, bool last) {
                            const uint32_t seg = eb + cst[j] + pos[lr * kStageMaxChunks + j];
-                           const uint32_t at = seg + (e - first);
-                           const uint32_t end = last ? seg + padded(e + 1u - first) : at + 1u;
+                           const uint32_t at = seg + p;
+                           const uint32_t end = last ? seg + padded(p + n) : at + n;
                            if (end > eEnd) {  // cannot happen with consistent counts; never write past the block
                                atomicOr(fault, 2u);
                                return;
                            }
-                           if (cnt[e] >= (1u << 22)) atomicOr(fault, 32u);  // the count field holds 22 bits
-                           key[at] = (slot - j * kStageSlots) | (cnt[e] << 10);
-                           for (uint32_t p = at + 1u; p < end; ++p) key[p] = 0u;  // the segment's padding
+                           const uint32_t c = cnt[e];
+                           if (c > 0xffffu) atomicOr(fault, 32u);  // the split counts are exact below 65536
+                           const uint32_t sk = (slot - j * kStageSlots) << 4;
+                           if (n == 1u) {
+                               key[at] = sk | (stage_bf16_bits(c) << 16);
+                           } else {
+                               key[at] = sk | (stage_bf16_bits(c & ~255u) << 16);
+                               key[at + 1u] = sk | (stage_bf16_bits(c & 255u) << 16);
+                           }
+                           for (uint32_t q = at + n; q < end; ++q) key[q] = 0u;  // the segment's padding
                        });
     }
 }

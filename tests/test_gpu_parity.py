@@ -896,6 +896,60 @@ def test_staged_gathers_match_l1_gathers(oracle, monkeypatch, scene, R, nb):
     eng.destroy()
 
 
+@pytest.mark.parametrize("scale", [300, 70000])
+def test_staged_gathers_split_large_counts(oracle, monkeypatch, scale):
+    # The staged layout holds ray counts in BF16, exact up to 256; a larger
+    # count is split into two exact parts (c & ~255, c & 255).  An uploaded
+    # graph with the counts scaled up (most of them above 256, low bytes
+    # random) must sweep as the L1 gathers and the FP64 oracle do; counts of
+    # 65536 and more (scale 70000) leave the layout unbuilt, and the L1
+    # gathers run (bit for bit the same sweeps with FH_STAGED 1 and 0, where
+    # the staged sums, in another order, differ in the last bits).
+    print(f"staged gathers, scaled counts x{scale}", flush=True)
+    sc = scenes.row_crop(leaves_per_row=120, cells=12)
+    eng = _engine(R=64, seed=5)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    g = eng.getGraph()
+    rng = np.random.default_rng(scale)
+    cnt = g.counts.astype(np.int64) * scale + rng.integers(0, scale, g.counts.size)
+    sky = g.skyCounts.astype(np.int64) * scale
+    rows = np.repeat(np.arange(g.denominators.size), np.diff(g.rowOffsets.astype(np.int64)))
+    den = np.bincount(rows, weights=cnt, minlength=g.denominators.size).astype(np.int64) + sky
+    assert np.mean((cnt > 256) & (cnt % 256 != 0)) > 0.5
+    d = eng.setGraph(g.rowOffsets, g.neighbors, cnt.astype(np.uint32), den.astype(np.uint32), sky.astype(np.uint32))
+    assert d.maxClosureError == 0.0
+    nb = 40
+    S = sc.surface_count
+    r = rng.uniform(0.0, 0.5, (S, nb)).astype(np.float32)
+    t = (rng.uniform(0.0, 1.0, (S, nb)) * (1.0 - r) * 0.8).astype(np.float32)
+    e = np.zeros((S, nb), np.float32)
+    D = rng.uniform(0.0, 300.0, (S, nb)).astype(np.float32)
+    skyv = rng.uniform(10.0, 80.0, nb).astype(np.float32)
+    init = rng.uniform(10.0, 200.0, (S, nb)).astype(np.float32)
+    out = {}
+    for staged in ("1", "0"):
+        monkeypatch.setenv("FH_STAGED", staged)
+        few = eng.solveBands(r, t, e, D, skyv, init=init, maxIter=3, tolRel=0.0, precision=FH_PRECISION_BF16_DC)
+        full = eng.solveBands(r, t, e, D, skyv, init=init, tolRel=1e-7, precision=FH_PRECISION_BF16_DC)
+        out[staged] = (few, full)
+    (f1, c1), (f0, c0) = out["1"], out["0"]
+    print(f"  sweeps staged {c1.iterations}, L1 {c0.iterations}", flush=True)
+    # the sums round differently (count x value summed, then times 1 / den,
+    # against FP32 weights count / den): a few ulps per sweep, amplified at a
+    # handful of strongly coupled facets (5 values above 1e-6 of 61 k after
+    # three sweeps)
+    d = np.abs(f1.radiosity.astype(np.float64) - f0.radiosity) / np.abs(f0.radiosity.astype(np.float64))
+    assert np.quantile(d, 0.999) < 1e-6 and d.max() < 2e-5
+    assert np.array_equal(f1.radiosity, f0.radiosity) == (scale > 65535)
+    assert abs(c1.iterations - c0.iterations) <= 1
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    ref, _, _ = oracle.solve_bands(og, r, t, e, D, skyv, tol_rel=1e-9)
+    assert _rel(c1.radiosity, ref) < RTOL
+    assert _rel(c0.radiosity, ref) < RTOL
+    eng.destroy()
+
+
 def test_staged_coupled_gather_matches_team_gather(oracle, monkeypatch):
     # The coupled longwave sweep gathers each workgroup's distinct columns
     # once through the LDS (staged layout; the pair blocks' facets by the
