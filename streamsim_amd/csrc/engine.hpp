@@ -236,12 +236,15 @@ struct fh_context {
     // stream (default priority) and the fork / join events
     hipStream_t pairStream = nullptr;
     hipEvent_t pairFork = nullptr, pairJoin = nullptr;
-    // staged gathers of the increment sweeps (single facets): per block of
-    // 64 facets of the singles launch's order its sorted distinct columns
-    // (stDcol from stDoff[b]), per chunk and team a run descriptor (stMeta
-    // from stMoff[b]) and the padded entries it points to (stKey: slot in
-    // the chunk | count << 10, 0 on padding); stBlocks = 0: not built
+    // staged gathers of the increment sweeps and the coupled longwave
+    // gathers (single facets): per block of 128 facets of the singles
+    // launch's order its sorted distinct columns (stDcol from stDoff[b]),
+    // per chunk and team a run descriptor (stMeta from stMoff[b]) and the
+    // padded entries it points to, twice: stKey (16 x slot in the chunk |
+    // BF16 count << 16) and stKey16 (slot | count << 10), 0 on padding;
+    // stBlocks = 0: not built
     fh::DBuf<uint32_t> stDcol, stDoff, stMoff, stKey;
+    fh::DBuf<uint16_t> stKey16;
     fh::DBuf<uint2> stMeta;
     uint32_t stBlocks = 0;
     bool stOnSingles = false;  // built on the singles list (pair blocks on), else on the schedule

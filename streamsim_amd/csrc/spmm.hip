@@ -1203,8 +1203,10 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
 // (long with short), and a block-level copy of the CSR holds, per chunk and
 // team, the two segments back to back, each padded to a multiple of 4
 // entries of count 0, as 4 bytes: the slot's byte offset in its LDS plane
-// (low half) and the entry's ray count in BF16 (high half; counts above 256
-// split in two exact parts); a run descriptor per chunk and team gives the
+// (low half) and the entry's ray count in BF16 (high half; counts above 63
+// split in parts of at most 63, 2 parts in 10 000 on the terrain graph);
+// the single-band gathers read a copy of 2 bytes per entry (slot | count
+// << 10) at the same places; a run descriptor per chunk and team gives the
 // first entry, the groups of 4 of each segment and the two local rows.  A
 // segment sums count x value (two entries at a time: each word of the two
 // staged rows regrouped by band with v_perm, then one BF16 dot product per
@@ -1231,6 +1233,7 @@ struct StageArgs {
     const uint32_t* mOff;   // [blocks] run descriptor of (chunk j, team t) of block b at mOff[b] + 64 j + t
     const uint2* meta;      // x: first entry; y: groups of row A | groups of row B << 8 | A << 16 | B << 24
     const uint4* key;       // padded entries, 4 per vector: 16 x slot in chunk | BF16 count << 16 (0: padding)
+    const uint2* key16;     // the same entries, 2 bytes each: slot in chunk | count << 10
     const uint32_t* order;  // facet of the sweep order's position i (null: facet i)
     const uint32_t* den;    // [S] denominators: an entry's weight is count / den of its row
 };
@@ -1462,38 +1465,86 @@ __global__ __launch_bounds__(kStageFacets) void k_gather1_staged(StageArgs st, c
     const uint32_t nCh = (U + kStageSlots - 1u) / kStageSlots;
     const uint32_t* dc = st.dcol + d0;
     const uint2* meta = st.meta + st.mOff[blk];
-    for (uint32_t j = 0; j < nCh; ++j) {
-        float v[kStagePer];
+    // a chunk's distinct columns and run descriptor are read two chunks
+    // ahead, its staged values and the first two key groups of the run one
+    // chunk ahead; the key loads are unconditional (at most two groups past
+    // a run: the key array's slack)
+    uint32_t dq[kStagePer];
+    float v[kStagePer];
+    const auto load_dcol = [&](uint32_t j) {
 #pragma unroll
         for (uint32_t k = 0; k < kStagePer; ++k) {
             const uint32_t sl = j * kStageSlots + t + kStageFacets * k;
-            v[k] = sl < U ? B[dc[sl]] : 0.0f;
+            dq[k] = sl < U ? dc[sl] : kStageNone;
         }
-        __syncthreads();  // the last chunk's runs are done with vals
+    };
+    const auto load_vals = [&]() {
+#pragma unroll
+        for (uint32_t k = 0; k < kStagePer; ++k) v[k] = dq[k] != kStageNone ? B[dq[k]] : 0.0f;
+    };
+    const auto store_vals = [&]() {
 #pragma unroll
         for (uint32_t k = 0; k < kStagePer; ++k) vals[t + kStageFacets * k] = v[k];
-        __syncthreads();
-        const uint2 m = meta[j * kStageFacets + t];
+    };
+    uint2 mNext = make_uint2(0u, 0u), mNext2 = mNext, k1 = mNext, k2 = mNext;
+    const auto first_groups = [&](uint2 m) {
+        k1 = st.key16[m.x >> 2];
+        k2 = st.key16[(m.x >> 2) + 1u];
+    };
+    if (nCh) {
+        mNext = meta[t];
+        load_dcol(0);
+        load_vals();
+        if (nCh > 1u) {
+            load_dcol(1);
+            mNext2 = meta[kStageFacets + t];
+        }
+        store_vals();
+        first_groups(mNext);
+    }
+    __syncthreads();
+    // 2-byte keys, 4 per 8 bytes: slot | count << 10
+    const auto term = [&](uint32_t k) { return float(k >> 10) * vals[k & 1023u]; };
+    const auto terms = [&](uint2 k) {
+        return term(k.x & 0xffffu) + term(k.x >> 16) + term(k.y & 0xffffu) + term(k.y >> 16);
+    };
+    for (uint32_t j = 0; j < nCh; ++j) {
+        const uint2 m = mNext;
+        mNext = mNext2;
         const uint32_t qa = m.y & 255u, qn = qa + ((m.y >> 8) & 255u);
         const uint32_t rA = (m.y >> 16) & 255u, rB = m.y >> 24;
         const uint32_t g0 = m.x >> 2;
-        const auto term = [&](uint32_t k) { return __uint_as_float(k & 0xffff0000u) * vals[(k & 0xffffu) >> 4]; };
-        uint4 k1 = make_uint4(0u, 0u, 0u, 0u);
-        if (qn) k1 = st.key[g0];
         uint32_t q = 0;
         for (int seg = 0; seg < 2; ++seg) {  // segment A, then B: count-weighted sums times 1 / den
             const uint32_t qe = seg ? qn : qa;
             const bool any = q < qe;
             float sum = 0.0f;
-            for (; q < qe; ++q) {
-                const uint4 k = k1;
-                if (q + 1u < qn) k1 = st.key[g0 + q + 1u];
-                sum += term(k.x) + term(k.y) + term(k.z) + term(k.w);
+            for (; q + 2u <= qe; q += 2u) {
+                sum += terms(k1);
+                k1 = st.key16[g0 + q + 2u];
+                sum += terms(k2);
+                k2 = st.key16[g0 + q + 3u];
+            }
+            if (q < qe) {
+                sum += terms(k1);
+                k1 = k2;
+                k2 = st.key16[g0 + q + 2u];
+                ++q;
             }
             if (any) acc[seg ? rB : rA] += rinv[seg ? rB : rA] * sum;
         }
+        if (j + 1u < nCh) {
+            load_vals();
+            if (j + 2u < nCh) {
+                load_dcol(j + 2u);
+                mNext2 = meta[(j + 2u) * kStageFacets + t];
+            }
+            first_groups(mNext);
+        }
+        __syncthreads();  // the runs are done with vals
+        if (j + 1u < nCh) store_vals();
+        __syncthreads();
     }
-    __syncthreads();
     const uint32_t fl = blk * kStageFacets + t;
     if (fl < nF) {
         const uint32_t s0 = 2u * (st.order ? st.order[fl] : fl);
@@ -1580,10 +1631,11 @@ __device__ __forceinline__ uint32_t stage_slot(const uint32_t* bm, const uint32_
     return rk[c >> 5] + uint32_t(__popc(bm[c >> 5] & ((1u << (c & 31u)) - 1u)));
 }
 
-// Entries a CSR entry of count c takes in the staged layout: BF16 holds the
-// integers up to 256 exactly, so a larger count is split into c & ~255 and
-// c & 255 (both exact for c < 65536), two entries of the same slot.
-__device__ __forceinline__ uint32_t stage_parts(uint32_t c) { return c > 256u && (c & 255u) ? 2u : 1u; }
+// Entries a CSR entry of count c takes in the staged layout: parts of at
+// most kStagePart rays (the 6-bit count of the 2-byte keys; exact in BF16),
+// entries of the same slot.
+constexpr uint32_t kStagePart = 63u;
+__device__ __forceinline__ uint32_t stage_parts(uint32_t c) { return (c + kStagePart - 1u) / kStagePart; }
 
 // BF16 bits of an integer with at most 8 significant bits
 __device__ __forceinline__ uint32_t stage_bf16_bits(uint32_t n) { return __float_as_uint(float(n)) >> 16; }
@@ -1643,7 +1695,7 @@ __global__ __launch_bounds__(1024) void k_stage_fill(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ P, const uint32_t* __restrict__ eOff,
                                                      const uint32_t* __restrict__ mOff, uint2* __restrict__ meta,
                                                      const uint32_t* __restrict__ cnt, uint32_t* __restrict__ key,
-                                                     uint32_t* __restrict__ fault) {
+                                                     uint16_t* __restrict__ key16, uint32_t* __restrict__ fault) {
     extern __shared__ uint32_t fillLds[];
     const uint32_t b = b0 + blockIdx.x;
     if (b >= nBlk) return;
@@ -1758,16 +1810,16 @@ __global__ __launch_bounds__(1024) void k_stage_fill(const uint32_t* __restrict_
                                atomicOr(fault, 2u);
                                return;
                            }
-                           const uint32_t c = cnt[e];
-                           if (c > 0xffffu) atomicOr(fault, 32u);  // the split counts are exact below 65536
-                           const uint32_t sk = (slot - j * kStageSlots) << 4;
-                           if (n == 1u) {
-                               key[at] = sk | (stage_bf16_bits(c) << 16);
-                           } else {
-                               key[at] = sk | (stage_bf16_bits(c & ~255u) << 16);
-                               key[at + 1u] = sk | (stage_bf16_bits(c & 255u) << 16);
+                           const uint32_t sl = slot - j * kStageSlots;
+                           for (uint32_t q = 0, c = cnt[e]; q < n; ++q, c -= kStagePart) {
+                               const uint32_t part = min(c, kStagePart);
+                               key[at + q] = (sl << 4) | (stage_bf16_bits(part) << 16);
+                               key16[at + q] = uint16_t(sl | (part << 10));
                            }
-                           for (uint32_t q = at + n; q < end; ++q) key[q] = 0u;  // the segment's padding
+                           for (uint32_t q = at + n; q < end; ++q) {  // the segment's padding
+                               key[q] = 0u;
+                               key16[q] = 0u;
+                           }
                        });
     }
 }
@@ -2462,6 +2514,7 @@ static void release_stage(fh_context* c) {
     c->stMoff.release();
     c->stMeta.release();
     c->stKey.release();
+    c->stKey16.release();
 }
 
 __global__ void k_stage_chunks(const uint32_t* __restrict__ U, uint32_t nBlk, uint32_t* __restrict__ runs) {
@@ -2512,11 +2565,11 @@ static void build_stage(fh_context* c) {
         if (write)
             hipLaunchKernelGGL(k_stage_fill<true>, dim3(G), dim3(1024), fillLds, c->stream, c->rowptr.p, c->col.p,
                                order, nF, nBlk, b0, words, bitmap.p, rank.p, U.p, P.p, eOff.p, c->stMoff.p,
-                               c->stMeta.p, c->cnt.p, c->stKey.p, fault.p);
+                               c->stMeta.p, c->cnt.p, c->stKey.p, c->stKey16.p, fault.p);
         else
             hipLaunchKernelGGL(k_stage_fill<false>, dim3(G), dim3(1024), fillLds, c->stream, c->rowptr.p, c->col.p,
                                order, nF, nBlk, b0, words, bitmap.p, rank.p, U.p, P.p, eOff.p, c->stMoff.p,
-                               c->stMeta.p, c->cnt.p, c->stKey.p, fault.p);
+                               c->stMeta.p, c->cnt.p, c->stKey.p, c->stKey16.p, fault.p);
         FH_HIP(hipGetLastError());
     };
     // sizes: distinct columns, padded entries and run descriptors per block
@@ -2537,6 +2590,7 @@ static void build_stage(fh_context* c) {
     }
     c->stDcol.ensure(std::max<uint32_t>(tot[0], 1u));
     c->stKey.ensure(size_t(tot[1]) + 4u * kCsrPad);
+    c->stKey16.ensure(size_t(tot[1]) + 4u * kCsrPad);
     c->stMeta.ensure(std::max<uint32_t>(tot[2], 1u));
     for (uint32_t b0 = 0; b0 < nBlk; b0 += G) batch(b0, true);
     FH_HIP(hipMemcpyAsync(&tot[3], fault.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -2571,6 +2625,7 @@ static void launch_staged(fh_context* c, const SweepArgs& a, uint32_t grid, hipS
     sa.mOff = c->stMoff.p;
     sa.meta = c->stMeta.p;
     sa.key = reinterpret_cast<const uint4*>(c->stKey.p);
+    sa.key16 = reinterpret_cast<const uint2*>(c->stKey16.p);
     sa.order = a.sched;
     sa.den = c->den.p;
     const dim3 block(kStageFacets * a.bt);
@@ -2586,6 +2641,7 @@ bool launch_gather1_staged(fh_context* c, const float* B, float L, const SolveCt
     sa.mOff = c->stMoff.p;
     sa.meta = c->stMeta.p;
     sa.key = reinterpret_cast<const uint4*>(c->stKey.p);
+    sa.key16 = reinterpret_cast<const uint2*>(c->stKey16.p);
     sa.order = c->stOnSingles ? c->singles.p : sweep_schedule(c);
     sa.den = c->den.p;
     const uint32_t nF = c->stOnSingles ? c->singleCount : c->F;
