@@ -759,6 +759,7 @@ void eb_set_scene(fh_context* c, uint32_t leafCount, const uint32_t* facetMateri
     launch_fill_f32(c, e.incident.p, S, 0.0f);
     launch_fill_f32(c, e.pvPower.p, S, 0.0f);
     launch_fill_f32(c, e.Bsw0.p, nsw, 0.0f);
+    e.swIncSerial = 0;
     launch_fill_f32(c, e.Blw0.p, S, 0.0f);
     for (DBuf<float>* b : {&e.Rn, &e.Hs, &e.LE, &e.G, &e.sunlit}) launch_fill_f32(c, b->p, S, 0.0f);
     FH_HIP(hipGetLastError());
@@ -994,8 +995,15 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
             launch_fill_f32(c, e.Hsw.p, n, 0.0f);  // emission is zero in the SW
             launch_fold(c, e.Hsw.p, e.rsw.p, e.tsw.p, e.Dsw.p, ldb, e.csw.p);
         }
-        if (!e.swValid || p.zeroStart) launch_fill_f32(c, e.Bsw0.p, n, 0.0f);
+        if (!e.swValid || p.zeroStart) {
+            launch_fill_f32(c, e.Bsw0.p, n, 0.0f);
+            e.swIncSerial = 0;
+        }
         JacobiOptions o;
+        // from the previous node's final radiosity, an increment solve
+        // continues that node's increment state (no base sweep) while it is
+        // the context's latest
+        o.incSerial = &e.swIncSerial;
         o.ldb = ldb;
         // BF16 gathers only where they save lines: below 16 windows a side's
         // FP32 row is at most one 64-byte gather, and the entry-parallel
@@ -1060,6 +1068,7 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
         launch_fill_f32(c, e.incident.p, S, 0.0f);
         if (e.swNb) launch_fill_f32(c, e.Bsw0.p, size_t(S) * e.swLdb, 0.0f);
         e.swValid = false;
+        e.swIncSerial = 0;
     }
     // --- temperature state ----------------------------------------------------
     EbConst k{};

@@ -144,6 +144,10 @@ struct EbState {
     DBuf<uint32_t> swOrderD;
     DBuf<float> swRho2;            // [swLdb] spectral radius^2 per internal band
     bool swRhoValid = false;
+    // serial of the increment state (accumulated gathers, carries, pending
+    // increments) the last shortwave solve left, 0: none; the next node's
+    // solve continues from it while it is the context's latest
+    uint64_t swIncSerial = 0;
     DBuf<double> dpart;            // per-block FP64 partial sums
     DBuf<double> dsum;             // reduced FP64 sums
     DBuf<float2> fout;             // reduced (max dT, max residual-equivalent)
@@ -267,6 +271,11 @@ struct fh_context {
     fh::DBuf<uint16_t> X16a, X16b;   // BF16 copies of the radiosity (or its increments) gathered by BF16 sweeps
     fh::DBuf<uint8_t> X8a, X8b;      // FP8 increments of the radiosity gathered by FP8 increment sweeps
     fh::DBuf<float> gacc, carry;     // increment sweeps: accumulated gathers, B minus the gathered state
+    // the increment state in gacc, carry and X16a (the increments written
+    // last, not yet gathered) belongs to the increment solve of this serial
+    // (its final B, ldb incLdb); 0: no intact state.  solveSerial counts them.
+    uint64_t incSerial = 0, solveSerial = 0;
+    uint32_t incLdb = 0;
     fh::DBuf<float> omega;           // Chebyshev weights per band
     fh::DBuf<uint32_t> bandMax;      // per-band maxima (float bits) of the rho estimate
     fh::SolveCtrl* hctrl = nullptr;  // pinned mirror
@@ -289,6 +298,7 @@ namespace fh {
 inline void eb_graph_changed(fh_context* c) {
     EbState& e = c->eb;
     e.swRhoValid = false;
+    e.swIncSerial = 0;
     e.swValid = false;
     e.lwValid = false;
     e.featValid = false;

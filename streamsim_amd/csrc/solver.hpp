@@ -91,6 +91,12 @@ struct JacobiOptions {
     const float* ttab = nullptr;     // [materials][ldb]
     const float* dirf = nullptr;     // [S] clamp(sunlit, 0, 1) max(0, cos)
     const float* dniv = nullptr;     // [ldb]
+    // increment solves: *incSerial (when given and equal to the context's
+    // intact increment state, whose final B the caller passes as bufs[0]
+    // under the same operator) lets the solve continue from that state
+    // instead of a base sweep; on return it holds this solve's serial (0:
+    // no increment state left)
+    uint64_t* incSerial = nullptr;
 };
 
 // Per-band spectral radius squared of the SW operator (r, t [S][ldb]) by

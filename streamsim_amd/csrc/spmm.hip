@@ -127,6 +127,11 @@ struct SweepArgs {
     uint32_t ldq;
     uint32_t q8;  // 1: increments in FP8 instead of BF16
     uint32_t partOff;     // band-parallel: first partial of this launch (a sweep may be two launches)
+    // 1 (increment sweep, BF16): the first sweep of a solve continuing the
+    // increment state of the solve before: no gathers (the accumulated
+    // gathers hold M (B - carry) already), the increments that solve wrote
+    // last (Bin16, never gathered) added to the carries
+    uint32_t fold;
 };
 
 constexpr uint32_t kNoFacet = 0xffffffffu;  // no partner facet
@@ -452,6 +457,12 @@ __device__ __forceinline__ bool upd_begin(const SweepArgs& a, const BandCtx& x, 
     if (b0 >= x.nAct) {
         st4(x.Bout + i0, u.o0);
         st4(x.Bout + i1, u.o1);
+        if (INC && x.Bout16) {  // the band's last increment, not gathered, follows into both buffers
+            *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0) * a.ldh + b0) =
+                *reinterpret_cast<const uint2*>(x.Bin16 + size_t(s0) * a.ldh + b0);
+            *reinterpret_cast<uint2*>(x.Bout16 + size_t(s0 + 1) * a.ldh + b0) =
+                *reinterpret_cast<const uint2*>(x.Bin16 + size_t(s0 + 1) * a.ldh + b0);
+        }
         return false;
     }
     if (INC == 1) {
@@ -471,6 +482,18 @@ __device__ __forceinline__ bool upd_begin(const SweepArgs& a, const BandCtx& x, 
         }
         arr4(ld4(a.carry + i0), u.k0);
         arr4(ld4(a.carry + i1), u.k1);
+        if (a.fold) {
+            const uint2 p0 = *reinterpret_cast<const uint2*>(x.Bin16 + size_t(s0) * a.ldh + b0);
+            const uint2 p1 = *reinterpret_cast<const uint2*>(x.Bin16 + size_t(s0 + 1) * a.ldh + b0);
+            const uint32_t w0[2] = {p0.x, p0.y}, w1[2] = {p1.x, p1.y};
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                u.k0[2 * j] += __uint_as_float(w0[j] << 16);
+                u.k0[2 * j + 1] += __uint_as_float(w0[j] & 0xffff0000u);
+                u.k1[2 * j] += __uint_as_float(w1[j] << 16);
+                u.k1[2 * j + 1] += __uint_as_float(w1[j] & 0xffff0000u);
+            }
+        }
     } else {
 #pragma unroll
         for (int j = 0; j < 4; ++j) u.k0[j] = u.k1[j] = 0.0f;
@@ -1142,7 +1165,7 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
             for (uint32_t cb = 0; cb < x.nPrev; cb += 8u * bt) {
                 const uint32_t b0 = cb + 8u * lane;
                 float g[2][8] = {};
-                if (b0 < x.nAct) {
+                if (b0 < x.nAct && !a.fold) {
                     if constexpr (INC != 0) {
                         const RowSums<GATHER_BF16> rs =
                             gather_rows_pipe<GATHER_BF16>(a, rb, make_rsrc(x.Bin16, a.sides * ldh * 2u), ldh * 2u, b0 * 2u);
@@ -1180,7 +1203,7 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
             for (uint32_t cb = 0; cb < x.nPrev; cb += 4u * bt) {
                 const uint32_t b0 = cb + 4u * lane;
                 float gf[2][4] = {}, gg[2][4] = {};
-                if (b0 < x.nAct) {
+                if (b0 < x.nAct && !a.fold) {
                     gather_h4<kPairGroups>(a, x.Bin16 + b0, ldh, rb, gf);
                     gather_h4<kPairGroups>(a, x.Bin16 + b0, ldh, rc, gg);
                 }
@@ -1308,7 +1331,7 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
         rinv[lr] = d ? 1.0f / float(d) : 0.0f;
     }
     const uint32_t d0 = st.dOff[blk], U = st.dOff[blk + 1u] - d0;
-    const uint32_t nCh = (U + kStageSlots - 1u) / kStageSlots;
+    const uint32_t nCh = a.fold ? 0u : (U + kStageSlots - 1u) / kStageSlots;  // (a folding sweep gathers nothing)
     const uint32_t* dc = st.dcol + d0;
     const uint2* meta = st.meta + st.mOff[blk];
     const uint32_t rowBytes = a.ldh * 2u;
@@ -2604,6 +2627,7 @@ static void build_stage(fh_context* c) {
 }
 
 void build_pairs(fh_context* c) {
+    c->incSerial = 0;  // a new graph: no increment state continues
     build_pair_units(c);
     build_stage(c);
 }
@@ -2955,6 +2979,13 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
     uint64_t launched = 0;
     uint32_t batch = 4;
     if (bf16 && !half) c->c2.ensure(size_t(c->S) * o.ldb);
+    // continue the increment state of the solve before (see below)?
+    // FH_SW_CONTINUE=0: always a base sweep (read per solve)
+    const char* contEnv = std::getenv("FH_SW_CONTINUE");
+    const bool contOn = !(contEnv && std::atoi(contEnv) == 0);
+    const bool cont = contOn && incr && !q8 && o.incSerial && *o.incSerial && *o.incSerial == c->incSerial &&
+                      c->incLdb == o.ldb;
+    uint64_t serial = 0;
     if (incr) {
         // every sweep gathers low-precision increments: BF16 -- the base
         // sweep Q_0 = bf16(B_0), then the BF16 increments of B with their
@@ -2983,9 +3014,11 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
             const size_t nh = size_t(c->S) * ldh;
             c->X16a.ensure(nh);
             c->X16b.ensure(nh);
-            hipLaunchKernelGGL(k_to_bf16, dim3(uint32_t((nh + 255) / 256)), dim3(256), 0, c->stream, bufs[0], c->S,
-                               o.ldb, ldh, c->X16a.p);
-            FH_HIP(hipGetLastError());
+            if (!cont) {
+                hipLaunchKernelGGL(k_to_bf16, dim3(uint32_t((nh + 255) / 256)), dim3(256), 0, c->stream, bufs[0],
+                                   c->S, o.ldb, ldh, c->X16a.p);
+                FH_HIP(hipGetLastError());
+            }
             a.h0 = c->X16a.p;
             a.h1 = c->X16b.p;
             a.ldh = ldh;
@@ -2994,15 +3027,29 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
         a.carry = c->carry.p;
         a.c = cc;
         a.judge = 1u;
+        // the first sweep: the base sweep gathering Q_0 = bf16(B_0), or,
+        // continuing the increment state of the solve before (B_0 its final
+        // B, the same operator: gacc = M (B_0 - carry) with the increments it
+        // wrote last pending in X16a), a sweep without gathers that adds the
+        // pending increments to the carries
         while (true) {
             for (uint32_t k = 0; k < batch; ++k) {
-                a.inc = launched == 0 && !q8 ? 1u : 2u;
+                a.inc = launched == 0 && !q8 && !cont ? 1u : 2u;
+                a.fold = launched == 0 && cont ? 1u : 0u;
                 launch_sweep(c, MODE_SWEEP, false, a);
                 ++launched;
             }
             read_ctrl(c);
             if (c->hctrl->done) break;
             if (batch < 32u) batch *= 2u;
+        }
+        if (!q8) {  // the increments written last in X16a, the state this solve's serial
+            if (c->hctrl->iter & 1u) c->X16a.swap(c->X16b);
+            c->incSerial = ++c->solveSerial;
+            c->incLdb = o.ldb;
+            serial = c->incSerial;
+        } else {
+            c->incSerial = 0;
         }
         if (o.H && o.D) {
             const size_t n = size_t(c->S) * o.ldb;
@@ -3011,6 +3058,7 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
             FH_HIP(hipGetLastError());
         }
     } else if (half) {
+        c->incSerial = 0;  // X16a / X16b overwritten
         // Phase 1: BF16 sweeps gathering BF16 copies of B (X16a = bf16(bufs[0])
         // first), ending at relative change firstInner (about the distance of
         // the BF16 operator's fixed point) or after innerBf16 sweeps.
@@ -3080,6 +3128,7 @@ int run_jacobi(fh_context* c, const JacobiOptions& o, const float* cc, const flo
         stats->maxRelDelta = c->hctrl->lastRel;
         stats->refreshes = c->hctrl->refreshes;
     }
+    if (o.incSerial) *o.incSerial = serial;
     if (incidentDone)
         *incidentDone = (fuse && (!retire || c->hctrl->nAct == 0u)) || (incr && o.H && o.D);
     return int(c->hctrl->iter & 1u);

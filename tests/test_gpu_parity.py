@@ -951,6 +951,38 @@ def test_staged_gathers_split_large_counts(oracle, monkeypatch, scale):
     eng.destroy()
 
 
+def test_sw_increment_state_continues_across_nodes(oracle, monkeypatch):
+    # From the previous node's final radiosity, a node's BF16 increment
+    # solve continues that node's increment state (accumulated gathers,
+    # carries, the increments written last folded into the carries) with a
+    # first sweep that gathers nothing, instead of a base sweep gathering
+    # bf16(B_0).  Both reach the FP32 fixed point of the node's operator: the
+    # same sweeps (+-1) and radiosities within the solve tolerance; the first
+    # node (no state yet) is bit for bit the same, the continued ones not.
+    print("sw increment continuation", flush=True)
+    sc, eng, win = _eb_case(oracle, leaves=120, cells=12, R=64, K=16, nb=47)
+    met = faceteb.diurnal_meteo(12)[4:8]  # 09:00 .. 15:00
+    cfg = faceteb.EbConfig(tolerance=0.01)
+    out = {}
+    for cont in ("1", "0"):
+        monkeypatch.setenv("FH_SW_CONTINUE", cont)
+        faceteb.setup_scene(eng, sc, win)  # the same cold start for both runs
+        caps = []
+        st = faceteb.run_diurnal(eng, met, win, cfg, precision=FH_PRECISION_BF16_DC,
+                                 record=lambda e, k: caps.append(e.ebGetState()["shortwaveRadiosity"].copy()))
+        out[cont] = (st, caps)
+    (s1, b1), (s0, b0) = out["1"], out["0"]
+    print(f"  sweeps continued {s1.sw_sweeps}, base {s0.sw_sweeps}", flush=True)
+    assert all(s1.converged) and all(s0.converged)
+    assert all(abs(x - y) <= 1 for x, y in zip(s1.sw_sweeps, s0.sw_sweeps))
+    assert np.array_equal(b1[0], b0[0])
+    for x, y in zip(b1[1:], b0[1:]):
+        assert not np.array_equal(x, y)
+        big = np.abs(y) > 1e-3 * np.abs(y).max()
+        assert _rel(x[big], y[big]) < 5e-5
+    eng.destroy()
+
+
 def test_staged_coupled_gather_matches_team_gather(oracle, monkeypatch):
     # The coupled longwave sweep gathers each workgroup's distinct columns
     # once through the LDS (staged layout; the pair blocks' facets by the
