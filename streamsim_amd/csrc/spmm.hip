@@ -266,7 +266,7 @@ __global__ __launch_bounds__(256) void k_sweep(SweepArgs a) {
     if (threadIdx.x == 0) {
         const float bA = fmaxf(fmaxf(sAbs[0], sAbs[1]), fmaxf(sAbs[2], sAbs[3]));
         const float bR = fmaxf(fmaxf(sRel[0], sRel[1]), fmaxf(sRel[2], sRel[3]));
-        a.partial[blockIdx.x] = make_float2(bA, bR);
+        a.partial[a.partOff + blockIdx.x] = make_float2(bA, bR);
     }
 }
 
@@ -1464,20 +1464,15 @@ __global__ __launch_bounds__(1024) void k_sweep_staged(SweepArgs a, StageArgs st
     }
 }
 
-// Single-band staged gather (the coupled longwave sweep): the same layout,
-// one lane per team (a workgroup of kStageFacets lanes), the chunk's 1024
-// values of B staged in the LDS (4 KB) and every run's sums added to its
-// rows' sums there; then Iin = fsky L + G for the workgroup's facets.  The
-// random single-value gathers of every entry become one gather per distinct
-// column and workgroup.
-__global__ __launch_bounds__(kStageFacets) void k_gather1_staged(StageArgs st, const float* __restrict__ B,
-                                                                  const float* __restrict__ fsky, float L, uint32_t nF,
-                                                                  const SolveCtrl* ctrl, float* __restrict__ Iin) {
-    if (ctrl && ctrl_word(&ctrl->done)) return;
-    __shared__ float vals[kStageSlots];
-    __shared__ float acc[kStageRows];
-    __shared__ float rinv[kStageRows];
-    const uint32_t t = threadIdx.x, blk = blockIdx.x;
+// Single-band staged gathers (the coupled longwave sweep and the single-band
+// sweeps of the final longwave solves): the same layout, one lane per team
+// (a workgroup of kStageFacets lanes), the chunk's 1024 values of B staged in
+// the LDS (4 KB) and every run's count-weighted sum, times 1 / den of its
+// row, added to its row's sum there.  The random single-value gathers of
+// every entry become one gather per distinct column and workgroup.  On
+// return acc[2 t + side] holds the gather of the workgroup's t-th facet.
+__device__ __forceinline__ void stage_gather1(const StageArgs& st, const float* __restrict__ B, uint32_t nF,
+                                              uint32_t blk, uint32_t t, float* vals, float* acc, float* rinv) {
     for (uint32_t lr = t; lr < kStageRows; lr += kStageFacets) {
         const uint32_t i = blk * kStageFacets + (lr >> 1);
         const uint32_t d = i < nF ? st.den[2u * (st.order ? st.order[i] : i) + (lr & 1u)] : 0u;
@@ -1568,11 +1563,80 @@ __global__ __launch_bounds__(kStageFacets) void k_gather1_staged(StageArgs st, c
         if (j + 1u < nCh) store_vals();
         __syncthreads();
     }
+}
+
+// Iin = fsky L + G for the workgroup's facets (coupled longwave sweep)
+__global__ __launch_bounds__(kStageFacets) void k_gather1_staged(StageArgs st, const float* __restrict__ B,
+                                                                  const float* __restrict__ fsky, float L, uint32_t nF,
+                                                                  const SolveCtrl* ctrl, float* __restrict__ Iin) {
+    if (ctrl && ctrl_word(&ctrl->done)) return;
+    __shared__ float vals[kStageSlots];
+    __shared__ float acc[kStageRows];
+    __shared__ float rinv[kStageRows];
+    const uint32_t t = threadIdx.x, blk = blockIdx.x;
+    stage_gather1(st, B, nF, blk, t, vals, acc, rinv);
     const uint32_t fl = blk * kStageFacets + t;
     if (fl < nF) {
         const uint32_t s0 = 2u * (st.order ? st.order[fl] : fl);
         Iin[s0] = fsky[s0] * L + acc[2u * t];
         Iin[s0 + 1u] = fsky[s0 + 1u] * L + acc[2u * t + 1u];
+    }
+}
+
+// Single-band FP32 sweep of the staged facets (entry-parallel solves at one
+// band: the final longwave solves): the staged gather of Bin, then the
+// relaxed Jacobi update of k_sweep (fused incident in judge sweeps) and the
+// block's residual partials at partOff + block
+__global__ __launch_bounds__(kStageFacets) void k_sweep1_staged(SweepArgs a, StageArgs st) {
+    SolveCtrl* ctrl = a.ctrl;
+    if (ctrl_word(&ctrl->done)) return;
+    const uint32_t it = ctrl_word(&ctrl->iter);
+    const float* __restrict__ Bin = (it & 1u) ? a.buf1 : a.buf0;
+    float* __restrict__ Bout = (it & 1u) ? a.buf0 : a.buf1;
+    __shared__ float vals[kStageSlots];
+    __shared__ float acc[kStageRows];
+    __shared__ float rinv[kStageRows];
+    const uint32_t t = threadIdx.x, blk = blockIdx.x;
+    stage_gather1(st, Bin, a.F, blk, t, vals, acc, rinv);
+    float myAbs = 0.0f, myRel = 0.0f;
+    const uint32_t fl = blk * kStageFacets + t;
+    if (fl < a.F) {
+        const uint32_t s0 = 2u * (st.order ? st.order[fl] : fl), s1 = s0 + 1u;
+        const float g0 = acc[2u * t], g1 = acc[2u * t + 1u];
+        const float n0 = a.c[s0] + a.r[s0] * g0 + a.t[s0] * g1;
+        const float n1 = a.c[s1] + a.r[s1] * g1 + a.t[s1] * g0;
+        if (a.fuseH && a.judge) {  // incident of this sweep's gather
+            const float sk = a.skyv[0];
+            a.H[s0] = fmaxf(a.fsky[s0] * sk + g0 + a.D[s0], 0.0f);
+            a.H[s1] = fmaxf(a.fsky[s1] * sk + g1 + a.D[s1], 0.0f);
+        }
+        const float o0 = Bin[s0], o1 = Bin[s1];
+        const float x0 = o0 + a.relax * (n0 - o0);
+        const float x1 = o1 + a.relax * (n1 - o1);
+        Bout[s0] = x0;
+        Bout[s1] = x1;
+        const float d0 = fabsf(x0 - o0), d1 = fabsf(x1 - o1);
+        myAbs = fmaxf(d0, d1);
+        myRel = fmaxf(d0 / fmaxf(fabsf(x0), 1.0e-30f), d1 / fmaxf(fabsf(x1), 1.0e-30f));
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+        myAbs = fmaxf(myAbs, __shfl_xor(myAbs, m, 64));
+        myRel = fmaxf(myRel, __shfl_xor(myRel, m, 64));
+    }
+    __shared__ float sAbs[kStageFacets / 64u], sRel[kStageFacets / 64u];
+    if ((t & 63u) == 0u) {
+        sAbs[t >> 6] = myAbs;
+        sRel[t >> 6] = myRel;
+    }
+    __syncthreads();
+    if (t == 0u) {
+        float bA = 0.0f, bR = 0.0f;
+        for (uint32_t w = 0; w < kStageFacets / 64u; ++w) {
+            bA = fmaxf(bA, sAbs[w]);
+            bR = fmaxf(bR, sRel[w]);
+        }
+        a.partial[a.partOff + blk] = make_float2(bA, bR);
     }
 }
 
@@ -2793,6 +2857,42 @@ void launch_sweep(fh_context* c, int mode, bool w16, SweepArgs a) {
     a.w16 = c->val16.p;
     a.fsky = c->fsky.p;
     a.ctrl = c->ctrl.p;
+    // single-band FP32 sweeps of a solve over the staged layout: its facets
+    // by k_sweep1_staged, the coupled pairs' facets (outside the layout when
+    // it was built on the single facets) by the team kernel beside them,
+    // each writing its own range of the per-block partials
+    if (a.ldb == 1u && mode == MODE_SWEEP && !w16 && c->stBlocks && use_staged() && !a.standalone &&
+        !a.partialExt && a.buf0) {
+        StageArgs sa;
+        sa.dcol = c->stDcol.p;
+        sa.dOff = c->stDoff.p;
+        sa.mOff = c->stMoff.p;
+        sa.meta = c->stMeta.p;
+        sa.key = reinterpret_cast<const uint4*>(c->stKey.p);
+        sa.key16 = reinterpret_cast<const uint2*>(c->stKey16.p);
+        sa.order = c->stOnSingles ? c->singles.p : sweep_schedule(c);
+        sa.den = c->den.p;
+        const uint32_t g = c->stBlocks;
+        SweepArgs p = a;
+        uint32_t gp = 0u;
+        if (c->stOnSingles && c->unitCount) {
+            p.sched = c->pairFacets.p;
+            p.F = 2u * c->unitCount;
+            p.xcd = 0u;
+            gp = uint32_t((uint64_t(p.F) * uint64_t(team) + 255u) / 256u);
+            p.partOff = g;
+        }
+        a.F = c->stOnSingles ? c->singleCount : c->F;
+        c->partial.ensure(size_t(g) + gp);
+        a.partial = p.partial = c->partial.p;
+        hipLaunchKernelGGL(k_sweep1_staged, dim3(g), dim3(kStageFacets), 0, c->stream, a, sa);
+        if (gp) launch_team<1>(team, mode, w16, p, c->stream);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, c->stream, c->ctrl.p, c->partial.p,
+                           static_cast<const uint32_t*>(nullptr), g + gp, a.judge, 0u, 0u,
+                           static_cast<const float*>(nullptr), static_cast<float*>(nullptr), a.ldb);
+        FH_HIP(hipGetLastError());
+        return;
+    }
     const uint64_t threads = uint64_t(c->F) * uint64_t(team);
     const uint32_t grid = uint32_t((threads + 255u) / 256u);
     if (a.partialExt) {

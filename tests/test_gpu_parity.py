@@ -983,6 +983,33 @@ def test_sw_increment_state_continues_across_nodes(oracle, monkeypatch):
     eng.destroy()
 
 
+def test_staged_single_band_sweeps_match_team_kernel(oracle, monkeypatch):
+    # Single-band FP32 solves sweep the staged facets with k_sweep1_staged
+    # (the coupled pairs' facets by the team kernel beside it); FH_STAGED=0
+    # sweeps every facet with the team kernel.  Only the order and rounding
+    # of the sums differ (count x value summed, times 1 / den): 64 fixed
+    # sweeps agree to a few ulps, and both match the FP64 oracle.
+    print("staged single-band sweeps", flush=True)
+    sc = scenes.row_crop(leaves_per_row=120, cells=12)
+    eng = _engine(R=64, seed=5)
+    eng.setGeometry(sc.vertices)
+    eng.buildVisibilityGraph()
+    sun = oracle.sun_direction(35.0, 120.0)
+    o = optics.make_optics(sc, eng.computeSunlitFraction(sun), sun)
+    out = {}
+    for staged in ("1", "0"):
+        monkeypatch.setenv("FH_STAGED", staged)
+        out[staged] = eng.solve(o, 100.0, 64, 0.9)
+    r1, r0 = out["1"], out["0"]
+    assert r1.iterations == r0.iterations == 64
+    assert not np.array_equal(r1.radiosity, r0.radiosity)
+    assert _rel(r1.radiosity, r0.radiosity) < 1e-6
+    og = _as_oracle_graph(oracle, eng.getGraph())
+    ref, _ = oracle.solve(og, o, 100.0, 64, 0.9)
+    assert _rel(r1.radiosity, ref) < RTOL
+    eng.destroy()
+
+
 def test_staged_coupled_gather_matches_team_gather(oracle, monkeypatch):
     # The coupled longwave sweep gathers each workgroup's distinct columns
     # once through the LDS (staged layout; the pair blocks' facets by the
