@@ -1805,7 +1805,11 @@ __global__ __launch_bounds__(1024) void k_stage_fill(const uint32_t* __restrict_
         if (s == kStageNone) continue;
         stage_row_runs(col, cnt, bm, rk, rowptr[s], rowptr[s + 1u],
                        [&](uint32_t, uint32_t j, uint32_t, uint32_t at, uint32_t n, bool last) {
-                           if (last) len[lr * kStageMaxChunks + j] = uint16_t(at + n);
+                           if (!last) return;
+                           // a segment longer than 1020 entries does not fit the 8-bit group
+                           // counts (nor, expanded by large counts, the 16-bit lengths)
+                           if (at + n > 1020u) atomicOr(fault, 16u);
+                           len[lr * kStageMaxChunks + j] = uint16_t(min(at + n, 1024u));
                        });
     }
     __syncthreads();
