@@ -1214,7 +1214,7 @@ __global__ __launch_bounds__(256) void k_sweep_hv(SweepArgs a) {
     band_partials(a, myAbs, myRel, INC == 0 ? 0.0f : myAct);
 }
 
-// ---- Staged gathers: the increment sweeps of the single facets ------------
+// ---- Staged gathers: the sweeps of the single facets ----------------------
 //
 // A workgroup owns kStageFacets consecutive facets of the sweep order (one
 // team of bt lanes, 8 BF16 bands per lane, per facet) and gathers the
@@ -2613,11 +2613,11 @@ __global__ void k_stage_chunks(const uint32_t* __restrict__ U, uint32_t nBlk, ui
     if (b < nBlk) runs[b] = (U[b] + kStageSlots - 1u) / kStageSlots * kStageFacets;
 }
 
-// The staged layout for the order of the increment sweeps' singles launch:
-// the singles list when there are pair blocks, else the sweep schedule.  A
-// block with more than kStageMaxChunks chunks of distinct columns leaves
-// the graph without a staged layout (the increment sweeps then gather
-// through the vector L1).
+// The staged layout for the order of the sweeps' singles launch: the singles
+// list when there are pair blocks, else the sweep schedule.  A block with
+// more than kStageMaxChunks chunks of distinct columns, or a segment of more
+// than 1020 entries, leaves the graph without a staged layout (the sweeps
+// then gather through the vector L1).
 static void build_stage(fh_context* c) {
     release_stage(c);
     const bool onSingles = c->unitCount > 0u;
@@ -2700,8 +2700,8 @@ void build_pairs(fh_context* c) {
     build_stage(c);
 }
 
-// the staged singles launch of an increment sweep: 64 teams of bt lanes per
-// workgroup, the LDS of stage_lds_bytes(bt)
+// the staged singles launch of an increment sweep: kStageFacets teams of bt
+// lanes per workgroup, the LDS of stage_lds_bytes(bt)
 static void launch_staged(fh_context* c, const SweepArgs& a, uint32_t grid, hipStream_t st) {
     static const bool attr = [] {
         FH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sweep_staged<1>),
