@@ -631,22 +631,32 @@ __global__ __launch_bounds__(1024) void k_reduce_max2(const float2* __restrict__
     if (threadIdx.x == 0) *out = make_float2(sa[0], sb[0]);
 }
 
-// sums column k (k < 4) of n rows of 4 doubles, fixed order (deterministic)
-__global__ __launch_bounds__(256) void k_reduce_sum4(const double* __restrict__ part, uint32_t n, uint32_t k0,
-                                                     uint32_t k1, double* __restrict__ out) {
-    __shared__ double sh[256];
-    for (uint32_t k = k0; k < k1; ++k) {
-        double acc = 0.0;
-        for (uint32_t i = threadIdx.x; i < n; i += 256u) acc += part[size_t(i) * 4 + k];
-        sh[threadIdx.x] = acc;
-        __syncthreads();
-        for (int off = 128; off > 0; off >>= 1) {
-            if (int(threadIdx.x) < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = sh[0];
+// sums columns [k0, k1) (k1 <= 4) of n rows of 4 doubles in one pass (each
+// thread a fixed set of rows, 32 coalesced bytes per row, then a fixed tree):
+// deterministic
+__global__ __launch_bounds__(1024) void k_reduce_sum4(const double* __restrict__ part, uint32_t n, uint32_t k0,
+                                                      uint32_t k1, double* __restrict__ out) {
+    __shared__ double sh[4][1024];
+    const uint32_t t = threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t i = t; i < n; i += 1024u) {
+        const double2 lo = *reinterpret_cast<const double2*>(part + size_t(i) * 4);
+        const double2 hi = *reinterpret_cast<const double2*>(part + size_t(i) * 4 + 2);
+        acc[0] += lo.x;
+        acc[1] += lo.y;
+        acc[2] += hi.x;
+        acc[3] += hi.y;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sh[k][t] = acc[k];
+    __syncthreads();
+    for (uint32_t off = 512u; off > 0u; off >>= 1) {
+        if (t < off)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + off];
         __syncthreads();
     }
+    if (t >= k0 && t < k1) out[t] = sh[t][0];
 }
 
 // per side: sum of the nb bands of a [S][ldb] array, in FP64
@@ -798,7 +808,7 @@ static void newton(fh_context* c, const EbConst& k, int update, float& maxDT, fl
                        e.incident.p, e.Hlw.p, e.eps.p, e.T.p, e.Tprev.p, update, e.Rn.p, e.Hs.p, e.LE.p, e.G.p,
                        e.pvPower.p, e.area.p, e.part4.p, e.dpart.p);
     hipLaunchKernelGGL(k_reduce_max4, dim3(1), dim3(1024), 0, c->stream, e.part4.p, nbk, e.last4.p);
-    if (rnSum) hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(256), 0, c->stream, e.dpart.p, nbk, 3u, 4u, e.dsum.p);
+    if (rnSum) hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(1024), 0, c->stream, e.dpart.p, nbk, 3u, 4u, e.dsum.p);
     FH_HIP(hipGetLastError());
     float h[4];
     FH_HIP(hipMemcpyAsync(h, e.last4.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
@@ -1054,7 +1064,7 @@ void eb_run_node(fh_context* c, const fh_eb_node_params& p, const float* dniHost
             hipLaunchKernelGGL((k_absorb<false, false>), dim3(nbk), dim3(256), 0, c->stream, S, nb, ldb, e.rsw.p,
                                e.tsw.p, e.Hsw.p, e.Dsw.p, e.Bsw0.p, c->fsky.p, c->skyv.p, e.area.p, e.absorbed.p,
                                e.incident.p, e.dpart.p, tb);
-        hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(256), 0, c->stream, e.dpart.p, nbk, 0u, 3u, e.dsum.p);
+        hipLaunchKernelGGL(k_reduce_sum4, dim3(1), dim3(1024), 0, c->stream, e.dpart.p, nbk, 0u, 3u, e.dsum.p);
         FH_HIP(hipGetLastError());
         double d4[4];
         FH_HIP(hipMemcpyAsync(d4, e.dsum.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
