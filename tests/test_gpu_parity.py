@@ -676,7 +676,7 @@ def test_mlp_gradient_matches_oracle(oracle):
 def test_mlp_forward_width128_matrix_cores(oracle):
     # width 128 runs layer 2 on the matrix cores (v_mfma_f32_32x32x2f32):
     # against the FP64-accumulating oracle on a unit count that leaves a
-    # partial block of 96
+    # partial block
     print("mlp forward width 128 case", flush=True)
     eng = _engine()
     rng = np.random.default_rng(11)
